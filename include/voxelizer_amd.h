@@ -1,0 +1,173 @@
+/*
+ * voxelizer_amd.h -- C ABI of the MI355X voxelizer (libvoxelizer_amd.so).
+ *
+ * Plain C: opaque context, plain pointers and sizes, status codes.  No C++
+ * types and no exceptions cross this boundary.  The C++ class
+ * vox::Voxelizer<Node> (include/vox/voxelizer.h) is a header-only wrapper over
+ * these entry points, mirroring the reference API of
+ * include/voxelizer.h:704-868 (hakarlss/Voxelizer).
+ *
+ * Every entry point below replaces one piece of the reference's host <-> device
+ * boundary (include/host_device_interface.h:15-151 launchers, driven by
+ * src/host_side.cpp); the replaced interface is named in each comment.
+ *
+ * Device pointers ("_dev") are HIP device addresses on the context's device.
+ * All work is enqueued on the context's stream; vox_sync() waits for it.
+ * Calls on one context are not re-entrant (as in the reference, where one
+ * Voxelizer instance mutates its options per call, src/host_side.cpp:1742).
+ */
+#ifndef VOXELIZER_AMD_H
+#define VOXELIZER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vox_ctx vox_ctx;
+
+typedef enum vox_status {
+    VOX_OK = 0,
+    VOX_ERR_INVALID = 1,     /* bad argument or unsupported configuration     */
+    VOX_ERR_HIP = 2,         /* HIP runtime error (message in vox_last_error) */
+    VOX_ERR_OOM = 3,         /* device allocation failed                      */
+    VOX_ERR_UNSUPPORTED = 4, /* mode the reference rejects or that is not built */
+    VOX_ERR_NO_DEVICE = 5    /* no HIP device (reference: initDevices throws)  */
+} vox_status;
+
+/* Node layouts, bit-identical to include/node_types.h of the reference. */
+typedef enum vox_node_kind {
+    VOX_NODE_SHORT = 0,     /* 1 B: mat<<5 | bid(5 bits)        node_types.h:19-177   */
+    VOX_NODE_LONG = 1,      /* 2 B: {bid, mat}                  node_types.h:178-283  */
+    VOX_NODE_PARTIAL = 2,   /* 8 B: {bid, mat, float ratio}     node_types.h:284-399  (not built) */
+    VOX_NODE_SHORT_FCC = 3, /* 2 B: mat<<12 | bid(12 bits)      node_types.h:400-565  */
+    VOX_NODE_LONG_FCC = 4   /* 4 B: {u16 bid, u8 mat, u8 pad=0} node_types.h:566-706  */
+} vox_node_kind;
+
+typedef enum vox_plan_mode {
+    VOX_PLAN_MAX_DIMENSION = 0, /* param = voxels along the longest side (uint) */
+    VOX_PLAN_CUBE_LENGTH = 1    /* param = distance between voxel centres       */
+} vox_plan_mode;
+
+/*
+ * Grid plan: the reference's CommonHostData dimensioning (include/common.h:
+ * 268-285) after determineDimensions + adjustResolution
+ * (src/host_side.cpp:2497-2634).
+ *   res[0]   = voxels along x incl. the +2 x padding, rounded up to a multiple
+ *              of 32*x_splits;
+ *   res[1..2]= voxels along y, z (no padding);
+ *   alloc    = (res[0], res[1]+2, res[2]+2), the node/bit array dimensions
+ *              (src/host_side.cpp:953-957), i.e. NodePointer.dim for
+ *              Cartesian nodes.
+ */
+typedef struct vox_grid {
+    double min_vertex[3];
+    double max_vertex[3];
+    double voxel_length;
+    uint32_t res[3];
+    uint32_t alloc[3];
+    uint32_t x_splits;
+    uint32_t pad_;
+} vox_grid;
+
+/* ---- context --------------------------------------------------------- */
+
+/* Creates a context on HIP device `device` with its own stream and arena.
+ * Replaces Voxelizer::initDevices (src/host_side.cpp:163-202). */
+vox_status vox_create(int device, vox_ctx **out);
+void vox_destroy(vox_ctx *ctx);
+/* Last error message of this context ("" if none). */
+const char *vox_last_error(const vox_ctx *ctx);
+/* Number of HIP devices (reference: cudaGetDeviceCount in initVariables,
+ * src/host_side.cpp:104-110). Returns 0 when the runtime finds none. */
+int vox_device_count(void);
+/* Waits for all work enqueued on the context's stream. */
+vox_status vox_sync(vox_ctx *ctx);
+/* Library/kernel build tag, e.g. "gfx950". */
+const char *vox_build_info(void);
+
+/* ---- device memory helpers (so C callers need no HIP headers) --------- */
+vox_status vox_alloc(vox_ctx *ctx, size_t bytes, void **dev_out);
+vox_status vox_free(vox_ctx *ctx, void *dev);
+vox_status vox_memcpy_h2d(vox_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
+vox_status vox_memcpy_d2h(vox_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+
+/* ---- mesh ------------------------------------------------------------- */
+
+/* Copies a host mesh to the device (reference: Voxelizer constructor copies
+ * the arrays, src/host_side.cpp:44-47, and allocStaticMem uploads them,
+ * src/host_side.cpp:447-476).  vertices: 3*nv floats, indices: 3*nt uint32. */
+vox_status vox_set_mesh(vox_ctx *ctx, const float *vertices, uint32_t nv,
+                        const uint32_t *indices, uint32_t nt);
+/* Uses a mesh that is already resident on the device (no copy; the caller
+ * keeps the buffers alive until the next vox_set_mesh*). */
+vox_status vox_set_mesh_device(vox_ctx *ctx, const float *vertices_dev, uint32_t nv,
+                               const uint32_t *indices_dev, uint32_t nt);
+/* Per-triangle material ids; NULL disables materials.  Host pointer: copied.
+ * Reference: Voxelizer::setMaterials (src/host_side.cpp:218-243). */
+vox_status vox_set_materials(vox_ctx *ctx, const uint8_t *materials, uint32_t n_unique);
+vox_status vox_set_materials_device(vox_ctx *ctx, const uint8_t *materials_dev, uint32_t n_unique);
+
+/* ---- planning --------------------------------------------------------- */
+
+/* Bounding box + dimensions of the current mesh, on the device.  Replaces
+ * calculateBoundingBox + determineDimensions(d) / determineDimensions() +
+ * splitResolutionByMaxDim(voxSplitRes.x) + adjustResolution
+ * (src/host_side.cpp:2462-2634, 717-738). */
+vox_status vox_plan_grid(vox_ctx *ctx, vox_plan_mode mode, double param,
+                         uint32_t vox_split_x, vox_grid *out);
+
+/* ---- hot path --------------------------------------------------------- */
+
+/* Solid (plain) voxelization into a bit grid: alloc[0]/32 words per row,
+ * alloc[1]*alloc[2] rows, voxel x at bit 31-(x%32) (MSB first).  bits_dev
+ * must hold alloc[0]*alloc[1]*alloc[2]/32 uint32 and is fully written.
+ * shift (may be NULL) is added to min_vertex (the FCC sub-grid shifts,
+ * src/host_side.cpp:1436-1463).  split_yz (may be NULL = one subspace) is
+ * voxSplitRes.yz: the solid stage runs per y/z subspace as the reference does.
+ * Replaces calcTileOverlap + prepareForConstructWorkQueue + calcWorkQueue +
+ * sortWorkQueue + compactWorkQueue + calcVoxelization
+ * (include/host_device_interface.h:15-60, src/host_side.cpp:275-397). */
+vox_status vox_solid_bits(vox_ctx *ctx, const vox_grid *grid, const double shift[3],
+                          const uint32_t split_yz[2], uint32_t *bits_dev);
+
+/* Full node pipeline: solid voxelization, optional surface (materials)
+ * voxelization with deterministic material winners, and boundary-id
+ * classification (Cartesian 6-neighbour + erosion to the fixed point, or FCC
+ * 12-neighbour mask).  nodes_dev must hold node_count(grid, kind) nodes.
+ * vox_split: voxSplitRes (x,y,z), mat_split: matSplitRes (x,y,z); NULL means
+ * the reference defaults (1024,512,512) and (512,512,512).
+ * Replaces calcNodeList, launchConvertToFCCGrid, calcTriangleClassification,
+ * calcOptSurfaceVoxelization, procNodeList (+ the host erosion loop) and
+ * launchCalculateFCCBoundaries (include/host_device_interface.h:62-151,
+ * src/host_side.cpp:901-1244 voxelizeWorker, 1262-1698 fccWorker). */
+vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *grid, vox_node_kind kind,
+                              int use_materials, const uint32_t vox_split[3],
+                              const uint32_t mat_split[3], void *nodes_dev);
+
+/* Node array dimensions / count / node size for a plan and node kind
+ * (FCC: 2*alloc[0] x alloc[1] x 2*alloc[2], src/host_side.cpp:1494-1499). */
+void vox_node_dims(const vox_grid *grid, vox_node_kind kind, uint32_t dims_out[3]);
+size_t vox_node_count(const vox_grid *grid, vox_node_kind kind);
+size_t vox_node_size(vox_node_kind kind);
+
+/* ---- instrumentation (no counterpart in the reference) ----------------- */
+
+/* Per-stage device times (ms) of the last vox_voxelize_nodes/vox_solid_bits
+ * call, measured with HIP events on the context stream when enabled. */
+typedef struct vox_stage_times {
+    float plan_ms, solid_bin_ms, solid_fill_ms, surface_ms, erode_ms, nodes_ms, total_ms;
+    uint32_t erosion_passes;
+    uint32_t solid_pairs;    /* (triangle, tile) work-queue entries */
+    uint32_t surface_pairs;  /* (triangle, brick) surface work items */
+    uint32_t surface_voxels; /* voxels written by the surface stage */
+} vox_stage_times;
+vox_status vox_enable_timing(vox_ctx *ctx, int enable);
+vox_status vox_get_stage_times(const vox_ctx *ctx, vox_stage_times *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VOXELIZER_AMD_H */

@@ -1,0 +1,1110 @@
+// oracle.cpp -- CPU restatement of hakarlss/Voxelizer's hot path.
+//
+// TEST INFRASTRUCTURE ONLY (see oracle.h).  Written from the reference's
+// algorithm description and source (cited per function); no reference code
+// is copied.  Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp).
+//
+// Parity pins for each function live in tests/test_oracle_*.py; functions
+// without an independent pin are marked "parity unpinned" there and in
+// DESIGN.md.
+#include "oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+typedef uint32_t u32;
+
+// ---------------------------------------------------------------------------
+// Vector arithmetic in the operation order of the reference's vector helpers
+// (include/helper_math.h: dot = a.x*b.x + a.y*b.y + a.z*b.z evaluated left to
+// right, cross = (a.y*b.z - a.z*b.y, a.z*b.x - a.x*b.z, a.x*b.y - a.y*b.x)).
+// Written from those formulas; nothing of that file is reproduced.
+// ---------------------------------------------------------------------------
+struct D2 { double x, y; };
+struct D3 { double x, y, z; };
+struct F2 { float x, y; };
+struct F3 { float x, y, z; };
+
+inline D3 sub(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline F3 sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline D3 cross(D3 a, D3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline F3 cross(F3 a, F3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline float dot(F2 a, F2 b) { return a.x * b.x + a.y * b.y; }
+
+// Saturating float -> int conversions: CUDA's cvt.rzi.{s,u}32.f{32,64} and
+// gfx950's v_cvt_{i,u}32_f{32,64} both truncate toward zero, clamp to the
+// destination range and map NaN to 0.
+inline int32_t sat_i32(double v) {
+    if (v != v) return 0;
+    if (v >= 2147483648.0) return INT32_MAX;
+    if (v <= -2147483649.0) return INT32_MIN;
+    return (int32_t)v;
+}
+inline u32 sat_u32(double v) {
+    if (v != v) return 0;
+    if (v >= 4294967296.0) return UINT32_MAX;
+    if (v <= -1.0) return 0;
+    return (u32)v;
+}
+// 32-bit shifts with PTX semantics (shift amounts >= 32 give 0); the
+// reference relies on them at src/voxelizer.cu:1579-1583,1607-1616.
+inline u32 shr32(u32 v, u32 s) { return s >= 32u ? 0u : (v >> s); }
+inline u32 shl32(u32 v, u32 s) { return s >= 32u ? 0u : (v << s); }
+
+inline D3 fetch_d(const float *V, const u32 *I, u32 t, int k) {
+    u32 i = I[3 * t + k];
+    return {(double)V[3 * i], (double)V[3 * i + 1], (double)V[3 * i + 2]};
+}
+inline F3 fetch_f(const float *V, const u32 *I, u32 t, int k) {
+    u32 i = I[3 * t + k];
+    return {V[3 * i], V[3 * i + 1], V[3 * i + 2]};
+}
+
+// splitResolutionByNrOfParts for one axis (src/host_side.cpp:3003-3064): the
+// length is divided from the top down with floor division.
+struct Span { u32 min, max; };
+std::vector<Span> split_axis(u32 len, u32 parts) {
+    std::vector<Span> out(parts);
+    u32 prev = len;
+    for (u32 i = parts; i > 0; --i) {
+        Span s;
+        s.max = prev;
+        s.min = s.max - s.max / i;
+        prev = s.min;
+        out[i - 1] = s;
+    }
+    return out;
+}
+// splitResolutionByMaxDim (src/host_side.cpp:2943-3002).
+std::vector<Span> split_axis_max(u32 len, u32 maxdim) {
+    u32 parts = (len + maxdim - 1) / maxdim;
+    return split_axis(len, parts);
+}
+
+// ---------------------------------------------------------------------------
+// Boundary-id table (src/voxelizer.cu:2142-2236; ids documented at
+// include/voxelizer.h:47-83).  The table is an interface: the same permutation
+// must give the same id, so the 27 (permutation, id) pairs are kept.
+// Permutation bits: -x 1, +x 2, -y 4, +y 8, -z 16, +z 32.
+// ---------------------------------------------------------------------------
+const uint8_t kOrientPairs[27][2] = {
+    {21, 1},  {22, 2},  {25, 3},  {26, 4},  {37, 5},  {38, 6},  {41, 7},
+    {42, 8},  {23, 9},  {27, 10}, {29, 11}, {30, 12}, {39, 13}, {43, 14},
+    {45, 15}, {46, 16}, {53, 17}, {54, 18}, {57, 19}, {58, 20}, {31, 21},
+    {59, 22}, {55, 23}, {62, 24}, {61, 25}, {47, 26}, {63, 27}};
+
+struct OrientTable {
+    uint8_t lut[64];
+    OrientTable() {
+        std::memset(lut, 0, sizeof(lut));
+        for (auto &p : kOrientPairs) lut[p[0]] = p[1];
+    }
+};
+const OrientTable kOrient;
+
+// ---------------------------------------------------------------------------
+// Solid voxelization (src/voxelizer.cu:840-1617, src/host_side.cpp:275-397).
+// ---------------------------------------------------------------------------
+struct TileSetup {
+    D2 ne[3];
+    double de[3];
+    u32 cmin[2], cmax[2];  // commonSpace (y, z), [min, max)
+    u32 tmin[2], tmax[2];  // tile range (y, z), inclusive
+};
+
+// Shared prologue of calculateTileOverlap / constructWorkQueue
+// (src/voxelizer.cu:861-945, 1080-1138).  Returns false where
+// calculateTileOverlap writes 0 through one of its early outs (895-922).
+bool tile_setup(const D3 v[3], const double extMin[3], double d, const u32 smin[2],
+                const u32 smax[2], TileSetup &ts) {
+    // getTriangleBounds(double3) (2398-2421): yz bounding box.
+    D2 bmin = {v[0].y, v[0].z}, bmax = bmin;
+    for (int i = 1; i < 3; i++) {
+        bmin.x = v[i].y < bmin.x ? v[i].y : bmin.x;
+        bmin.y = v[i].z < bmin.y ? v[i].z : bmin.y;
+        bmax.x = v[i].y > bmax.x ? v[i].y : bmax.x;
+        bmax.y = v[i].z > bmax.y ? v[i].z : bmax.y;
+    }
+    // getVoxelBoundsDouble (2450-2464).
+    D2 vbmin = {(bmin.x - extMin[1]) / d, (bmin.y - extMin[2]) / d};
+    D2 vbmax = {(bmax.x - extMin[1]) / d, (bmax.y - extMin[2]) / d};
+    int32_t vmin[2] = {sat_i32(std::ceil(vbmin.x)), sat_i32(std::ceil(vbmin.y))};
+    int32_t vmax[2] = {sat_i32(std::floor(vbmax.x) + 1.0), sat_i32(std::floor(vbmax.y) + 1.0)};
+    // x component of the normal, in the reference's own expression (882-884).
+    double tN = (v[0].y * v[1].z + v[2].y * v[0].z + v[2].z * v[1].y) -
+                (v[2].y * v[1].z + v[0].z * v[1].y + v[2].z * v[0].y);
+    if (tN == 0.0) return false;
+    if (vmin[0] >= vmax[0] || vmin[1] >= vmax[1]) return false;
+    if (vmax[0] <= (int32_t)smin[0] || vmax[1] <= (int32_t)smin[1]) return false;
+    if (vmin[0] >= (int32_t)smax[0] || vmin[1] >= (int32_t)smax[1]) return false;
+    for (int a = 0; a < 2; a++) {
+        int32_t lo = std::max((int32_t)smin[a], vmin[a]);
+        ts.cmin[a] = (u32)lo;
+        ts.cmax[a] = std::min(smax[a], (u32)vmax[a]);
+    }
+    for (int i = 0; i < 3; i++) {
+        const D3 &a = v[i], &b = v[(i + 1) % 3];
+        if (tN > 0.0)
+            ts.ne[i] = {a.z - b.z, b.y - a.y};
+        else
+            ts.ne[i] = {b.z - a.z, a.y - b.y};
+    }
+    for (int i = 0; i < 3; i++) ts.de[i] = -(ts.ne[i].x * v[i].y + ts.ne[i].y * v[i].z);
+    for (int a = 0; a < 2; a++) {
+        ts.tmin[a] = ts.cmin[a] / 4;
+        ts.tmax[a] = (ts.cmax[a] - 1) / 4;
+    }
+    return true;
+}
+
+// traverseTilesForOverlaps / traverseTilesForWorkQueue (980-1033, 1169-1232):
+// a tile is queued if any voxel centre of it inside commonSpace passes the
+// (>= 0) edge tests.
+template <class F>
+void traverse_tiles(const TileSetup &ts, const double extMin[3], double d, F &&emit) {
+    for (u32 t = ts.tmin[0]; t <= ts.tmax[0]; t++) {
+        for (u32 u = ts.tmin[1]; u <= ts.tmax[1]; u++) {
+            bool hit = false;
+            for (u32 i = t * 4; i < t * 4 + 4 && i < ts.cmax[0] && !hit; i++) {
+                for (u32 j = u * 4; j < u * 4 + 4 && j < ts.cmax[1] && !hit; j++) {
+                    if (i >= ts.cmin[0] && j >= ts.cmin[1]) {
+                        D2 p = {double(i) * d + extMin[1], double(j) * d + extMin[2]};
+                        bool r = true;
+                        for (int k = 0; k < 3; k++)
+                            r &= (ts.ne[k].x * p.x + ts.ne[k].y * p.y + ts.de[k] >= 0.0);
+                        if (r) hit = true;
+                    }
+                }
+            }
+            if (hit) emit(t, u);
+        }
+    }
+}
+
+// Per-triangle data of generateVoxelization (1334-1372).
+struct FillSetup {
+    D3 v0, n;
+    D2 ne[3];
+    double de[3];
+};
+FillSetup fill_setup(const D3 v[3]) {
+    FillSetup fs;
+    fs.v0 = v[0];
+    fs.n = cross(sub(v[0], v[2]), sub(v[1], v[0]));
+    for (int i = 0; i < 3; i++) {
+        const D3 &a = v[i], &b = v[(i + 1) % 3];
+        if (fs.n.x > 0.0)
+            fs.ne[i] = {a.z - b.z, b.y - a.y};
+        else
+            fs.ne[i] = {b.z - a.z, a.y - b.y};
+        fs.de[i] = -(fs.ne[i].x * a.y + fs.ne[i].y * a.z);
+    }
+    return fs;
+}
+// Centre test + crossing (1418-1453): returns the x voxel (incl. +1 padding).
+bool fill_center(const FillSetup &fs, const double extMin[3], double d, u32 cy, u32 cz,
+                 u32 &voxelX) {
+    D2 p = {double(cy) * d + extMin[1], double(cz) * d + extMin[2]};
+    u32 ok = 0;
+    for (int i = 0; i < 3; i++) {
+        double r = fs.ne[i].x * p.x + fs.ne[i].y * p.y + fs.de[i];
+        if (fs.ne[i].x > 0.0 || (fs.ne[i].x == 0.0 && fs.ne[i].y < 0.0)) r += DBL_MIN;
+        if (r > 0.0) ok++;
+    }
+    if (ok != 3) return false;
+    D3 A = {fs.v0.x - extMin[0], fs.v0.y - p.x, fs.v0.z - p.y};
+    double B = A.x * fs.n.x + A.y * fs.n.y + A.z * fs.n.z;
+    double px = B / fs.n.x;
+    px = px < 0.0 ? 0.0 : px;
+    voxelX = sat_u32(std::ceil(px / d));
+    voxelX += 1u;
+    return true;
+}
+
+struct SolidCtx {
+    const float *V;
+    const u32 *I;
+    u32 nt;
+    double extMin[3];
+    double d;
+    u32 X, Yext, Zext;  // ext resolution (x already padded/rounded)
+    u32 xSplits;
+    u32 *bits;
+};
+
+// generateVoxelization for one tile and one x split, per voxel-centre lane
+// (1273-1617), with the reference's active segment / word bitmask bookkeeping
+// and its atomicXor flushes applied to the bit array.
+void fill_tile_lane(const SolidCtx &c, const std::vector<FillSetup> &tris, u32 tileNr,
+                    const u32 sub_min[3], const u32 sub_max[3], int lane) {
+    const u32 tilesY = (c.Yext + 3) / 4;
+    const u32 tileZ = tileNr / tilesY, tileY = tileNr % tilesY;
+    const u32 vZ0 = u32(lane / 4) + tileZ * 4;
+    const u32 vY0 = u32(lane % 4) + tileY * 4;
+    if (!(vY0 >= sub_min[1] && vY0 < sub_max[1] && vZ0 >= sub_min[2] && vZ0 < sub_max[2]))
+        return;
+    u32 active = 0, current = UINT32_MAX, bitmask = 0;
+    const u32 resY = c.Yext + 2, resXw = c.X >> 5;
+    const u32 vY = vY0 + 1, vZ = vZ0 + 1;  // single device: no neighbours
+    for (const FillSetup &fs : tris) {
+        u32 voxelX;
+        if (!fill_center(fs, c.extMin, c.d, vY0, vZ0, voxelX)) continue;
+        u32 ix = voxelX >> 5;
+        u32 min_ix = (sub_min[0] + 2) >> 5;
+        if (ix >= ((sub_max[0] + 2) >> 5)) {
+            // crossing beyond this x split: ignored
+        } else if (ix < min_ix) {
+            u32 inti = resY * resXw * vZ + resXw * vY + min_ix;
+            if (current != inti) {
+                if (active != 0) c.bits[current] ^= active;
+                active = 0;
+                current = inti;
+            }
+            bitmask ^= UINT32_MAX;
+        } else {
+            u32 inti = resY * resXw * vZ + resXw * vY + ix;
+            if (current != inti) {
+                if (active != 0) c.bits[current] ^= active;
+                active = 0;
+                current = inti;
+            }
+            active ^= UINT32_MAX >> (voxelX % 32u);
+            u32 lastIX = (sub_max[0] + 1) >> 5;
+            u32 ints = (sub_max[0] - sub_min[0]) >> 5;
+            u32 bm = ix < lastIX ? shr32(UINT32_MAX, ix % ints + 1u) : 0u;
+            bitmask ^= bm;
+        }
+    }
+    if (current == UINT32_MAX) return;
+    if (active != 0) c.bits[current] ^= active;
+    u32 intsPerDim = (sub_max[0] - sub_min[0]) >> 5;
+    u32 start = current - (current % intsPerDim);
+    for (u32 i = 0; i < intsPerDim; i++) {
+        u32 value = (bitmask & shl32(1u, 31u - i)) > 0 ? UINT32_MAX : 0u;
+        if (value != 0) c.bits[start + i] ^= value;
+    }
+}
+
+// performVoxelization for one y/z subspace (src/host_side.cpp:275-397).
+void perform_voxelization(const SolidCtx &c, Span ys, Span zs) {
+    const u32 smin[2] = {ys.min, zs.min}, smax[2] = {ys.max, zs.max};
+    // calculateTileOverlap + prepareForConstructWorkQueue + constructWorkQueue:
+    // the queue holds (tile, triangle) pairs; sortWorkQueue/compactWorkQueue
+    // group it by tile.  Grouping order does not matter (XOR commutes).
+    std::vector<std::pair<u32, u32>> queue;  // (tileId, triangle)
+    const u32 tilesY = (c.Yext + 3) / 4;
+    for (u32 t = 0; t < c.nt; t++) {
+        D3 v[3] = {fetch_d(c.V, c.I, t, 0), fetch_d(c.V, c.I, t, 1), fetch_d(c.V, c.I, t, 2)};
+        TileSetup ts;
+        if (!tile_setup(v, c.extMin, c.d, smin, smax, ts)) continue;
+        traverse_tiles(ts, c.extMin, c.d,
+                       [&](u32 ty, u32 tz) { queue.push_back({tz * tilesY + ty, t}); });
+    }
+    if (queue.empty()) return;
+    std::stable_sort(queue.begin(), queue.end(),
+                     [](const std::pair<u32, u32> &a, const std::pair<u32, u32> &b) {
+                         return a.first < b.first;
+                     });
+    std::vector<size_t> starts;
+    for (size_t i = 0; i < queue.size(); i++)
+        if (i == 0 || queue[i].first != queue[i - 1].first) starts.push_back(i);
+    starts.push_back(queue.size());
+    const u32 xRes = c.X / c.xSplits;
+    for (u32 xs = 0; xs < c.xSplits; xs++) {
+        const u32 sub_min[3] = {xs * xRes, ys.min, zs.min};
+        const u32 sub_max[3] = {(xs + 1) * xRes, ys.max, zs.max};
+        const long ntiles = (long)starts.size() - 1;
+#pragma omp parallel for schedule(dynamic, 16)
+        for (long g = 0; g < ntiles; g++) {
+            std::vector<FillSetup> tris;
+            for (size_t k = starts[g]; k < starts[g + 1]; k++) {
+                u32 t = queue[k].second;
+                D3 v[3] = {fetch_d(c.V, c.I, t, 0), fetch_d(c.V, c.I, t, 1),
+                           fetch_d(c.V, c.I, t, 2)};
+                tris.push_back(fill_setup(v));
+            }
+            // Each lane owns one row, so tiles never share bit words.
+            for (int lane = 0; lane < 16; lane++)
+                fill_tile_lane(c, tris, queue[starts[g]].first, sub_min, sub_max, lane);
+        }
+    }
+}
+
+void solid_bits(const float *V, const u32 *I, u32 nt, const vox_grid &g,
+                const double extMin[3], u32 splitY, u32 splitZ, u32 *bits) {
+    SolidCtx c;
+    c.V = V;
+    c.I = I;
+    c.nt = nt;
+    for (int a = 0; a < 3; a++) c.extMin[a] = extMin[a];
+    c.d = g.voxel_length;
+    c.X = g.res[0];
+    c.Yext = g.res[1];
+    c.Zext = g.res[2];
+    c.xSplits = g.x_splits;
+    c.bits = bits;
+    size_t words = (size_t)g.alloc[0] / 32 * g.alloc[1] * g.alloc[2];
+    std::memset(bits, 0, words * 4);
+    // extYzSplits over extPartition; subspaces ordered z outer, y inner
+    // (src/host_side.cpp:3100-3112).
+    std::vector<Span> ys = split_axis_max(c.Yext, splitY), zs = split_axis_max(c.Zext, splitZ);
+    for (const Span &z : zs)
+        for (const Span &y : ys) perform_voxelization(c, y, z);
+}
+
+// ---------------------------------------------------------------------------
+// Surface voxelization (src/voxelizer.cu:2364-2590, 2677-2757, 2892-2942,
+// 3040-3221, 3307-3993, 4004-4201).  FP32 throughout.
+// ---------------------------------------------------------------------------
+enum { BB_DEGEN = 0, BB_1D = 1, BB_2D = 2, BB_3D = 3 };
+enum { AX_X = 0, AX_Y = 1, AX_Z = 2 };
+
+struct I3 { int32_t x, y, z; };
+struct VoxBB { I3 min, max; };
+
+struct OverlapData { F2 ne[3]; float de[3]; };
+
+inline int32_t f2i(float v) { return sat_i32((double)v); }
+
+// getTriangleBounds(float3) (2364-2392): strict comparisons.
+void tri_bounds(const F3 t[3], F3 &mn, F3 &mx) {
+    mn = t[0];
+    mx = t[0];
+    for (int i = 1; i < 3; i++) {
+        if (t[i].x < mn.x) mn.x = t[i].x;
+        if (t[i].y < mn.y) mn.y = t[i].y;
+        if (t[i].z < mn.z) mn.z = t[i].z;
+        if (t[i].x > mx.x) mx.x = t[i].x;
+        if (t[i].y > mx.y) mx.y = t[i].y;
+        if (t[i].z > mx.z) mx.z = t[i].z;
+    }
+}
+// getVoxelBoundsHalf (2471-2492).
+VoxBB voxel_bounds_half(F3 mn, F3 mx, F3 m, float d) {
+    VoxBB b;
+    b.min = {f2i(std::ceil((mn.x - m.x) / d - 0.5f)), f2i(std::ceil((mn.y - m.y) / d - 0.5f)),
+             f2i(std::ceil((mn.z - m.z) / d - 0.5f))};
+    b.max = {f2i(std::floor((mx.x - m.x) / d + 0.5f)), f2i(std::floor((mx.y - m.y) / d + 0.5f)),
+             f2i(std::floor((mx.z - m.z) / d + 0.5f))};
+    return b;
+}
+inline int32_t vox_cbound_min(float v, float m, float d) {  // 2499-2510
+    return f2i(std::ceil((v - m) / d - 0.5f));
+}
+inline int32_t vox_cbound_max(float v, float m, float d) {  // 2513-2524
+    return f2i(std::floor((v - m) / d + 0.5f));
+}
+// determineDominantAxis (2531-2550).
+int dominant_axis(F3 n) {
+    float ax = std::fabs(n.x), ay = std::fabs(n.y), az = std::fabs(n.z);
+    if (ax > ay && ax > az) return AX_X;
+    if (ay > ax && ay > az) return AX_Y;
+    return AX_Z;
+}
+// getSingleVoxelBoundsComponent (2578-2590).
+inline float vox_corner(int32_t c, float m, float d) { return m + (float(c) - 0.5f) * d; }
+
+// analyzeBoundingBox (3117-3205) + classifyTriangles (3040-3105).
+struct TriClass { int bb, axis; u32 cols; };
+TriClass classify(const F3 t[3], F3 m, float d) {
+    F3 n = cross(sub(t[0], t[2]), sub(t[1], t[0]));
+    TriClass tc;
+    if (!(n.x != 0.0f || n.y != 0.0f || n.z != 0.0f)) {
+        tc.bb = BB_DEGEN;
+        tc.axis = AX_X;
+        tc.cols = 0;
+        return tc;
+    }
+    F3 mn, mx;
+    tri_bounds(t, mn, mx);
+    VoxBB vb = voxel_bounds_half(mn, mx, m, d);
+    I3 df = {vb.max.x - vb.min.x, vb.max.y - vb.min.y, vb.max.z - vb.min.z};
+    if (df.x == 0) {
+        if (df.y == 0) {
+            if (df.z == 0) tc = {BB_1D, AX_X, 1u};
+            else tc = {BB_1D, AX_Z, (u32)df.z};
+        } else if (df.z == 0) {
+            tc = {BB_1D, AX_Y, (u32)df.y};
+        } else {
+            tc = {BB_2D, AX_X, (u32)(df.y * df.z)};
+        }
+    } else if (df.y == 0) {
+        if (df.z == 0) tc = {BB_1D, AX_X, (u32)df.x};
+        else tc = {BB_2D, AX_Y, (u32)(df.x * df.z)};
+    } else if (df.z == 0) {
+        tc = {BB_2D, AX_Z, (u32)(df.x * df.y)};
+    } else {
+        tc.bb = BB_3D;
+        tc.axis = dominant_axis(n);
+        if (tc.axis == AX_X) tc.cols = (u32)(df.y * df.z);
+        else if (tc.axis == AX_Y) tc.cols = (u32)(df.x * df.z);
+        else tc.cols = (u32)(df.x * df.y);
+    }
+    return tc;
+}
+// encodeTriangleType (3214-3221).
+inline u32 encode_type(const TriClass &c) {
+    return c.cols | ((u32)c.axis << 28) | ((u32)c.bb << 30);
+}
+
+// setup{YZ,ZX,XY}OverlapTest (2677-2757).
+OverlapData setup_yz(const F3 t[3], F3 n, float d) {
+    OverlapData o;
+    for (int i = 0; i < 3; i++) {
+        const F3 &a = t[i], &b = t[(i + 1) % 3];
+        if (n.x >= 0.0f) o.ne[i] = {a.z - b.z, b.y - a.y};
+        else o.ne[i] = {b.z - a.z, a.y - b.y};
+        o.de[i] = -(o.ne[i].x * a.y + o.ne[i].y * a.z) + std::fmax(0.0f, d * o.ne[i].x) +
+                  std::fmax(0.0f, d * o.ne[i].y);
+    }
+    return o;
+}
+OverlapData setup_zx(const F3 t[3], F3 n, float d) {
+    OverlapData o;
+    for (int i = 0; i < 3; i++) {
+        const F3 &a = t[i], &b = t[(i + 1) % 3];
+        if (n.y >= 0.0f) o.ne[i] = {a.x - b.x, b.z - a.z};
+        else o.ne[i] = {b.x - a.x, a.z - b.z};
+        o.de[i] = -(o.ne[i].x * a.z + o.ne[i].y * a.x) + std::fmax(0.0f, d * o.ne[i].x) +
+                  std::fmax(0.0f, d * o.ne[i].y);
+    }
+    return o;
+}
+OverlapData setup_xy(const F3 t[3], F3 n, float d) {
+    OverlapData o;
+    for (int i = 0; i < 3; i++) {
+        const F3 &a = t[i], &b = t[(i + 1) % 3];
+        if (n.z >= 0.0f) o.ne[i] = {a.y - b.y, b.x - a.x};
+        else o.ne[i] = {b.y - a.y, a.x - b.x};
+        o.de[i] = -(o.ne[i].x * a.x + o.ne[i].y * a.y) + std::fmax(0.0f, d * o.ne[i].x) +
+                  std::fmax(0.0f, d * o.ne[i].y);
+    }
+    return o;
+}
+// overlapTest{YZ,ZX,XY} (2892-2942).
+inline bool test2(const OverlapData &o, F2 p2) {
+    bool r = true;
+    for (int i = 0; i < 3; i++) r &= dot(o.ne[i], p2) + o.de[i] >= 0.0f;
+    return r;
+}
+inline bool test_yz(const OverlapData &o, F3 p) { return test2(o, {p.y, p.z}); }
+inline bool test_zx(const OverlapData &o, F3 p) { return test2(o, {p.z, p.x}); }
+inline bool test_xy(const OverlapData &o, F3 p) { return test2(o, {p.x, p.y}); }
+
+// intersectWithPlane{X,Y,Z} (4147-4182) and determineDepthRange{X,Y,Z}
+// (4004-4119).
+inline float isect(F3 p, F3 v, F3 n, int axis) {
+    float num = dot(sub(v, p), n);
+    return num / (axis == AX_X ? n.x : axis == AX_Y ? n.y : n.z);
+}
+inline void bounds_check(float &a, float &b) {
+    if (a > b) {
+        float t = a;
+        a = b;
+        b = t;
+    }
+}
+void depth_range(const F3 t[3], F3 p, F3 n, F3 m, float d, int axis, int32_t &lo,
+                 int32_t &hi) {
+    F3 pmin, pmax;
+    float a, b;
+    if (axis == AX_X) {
+        bool ny = n.y > 0.0f, nz = n.z > 0.0f;
+        pmin = {0.0f, p.y + float(ny) * d, p.z + float(nz) * d};
+        pmax = {0.0f, p.y + float(!ny) * d, p.z + float(!nz) * d};
+    } else if (axis == AX_Y) {
+        bool nx = n.x > 0.0f, nz = n.z > 0.0f;
+        pmin = {p.x + float(nx) * d, 0.0f, p.z + float(nz) * d};
+        pmax = {p.x + float(!nx) * d, 0.0f, p.z + float(!nz) * d};
+    } else {
+        bool nx = n.x > 0.0f, ny = n.y > 0.0f;
+        pmin = {p.x + float(nx) * d, p.y + float(ny) * d, 0.0f};
+        pmax = {p.x + float(!nx) * d, p.y + float(!ny) * d, 0.0f};
+    }
+    a = isect(pmin, t[0], n, axis);
+    b = isect(pmax, t[0], n, axis);
+    bounds_check(a, b);
+    float mc = axis == AX_X ? m.x : axis == AX_Y ? m.y : m.z;
+    lo = vox_cbound_min(a, mc, d);
+    hi = vox_cbound_max(b, mc, d);
+}
+
+// adjustVoxelRange (3963-3993); returns false where it marks min = -1.
+bool adjust_range(VoxBB &vb, const u32 smin[3], const u32 smax[3]) {
+    I3 lo = {(int32_t)smin[0], (int32_t)smin[1], (int32_t)smin[2]};
+    I3 hi = {(int32_t)smax[0] - 1, (int32_t)smax[1] - 1, (int32_t)smax[2] - 1};
+    if (lo.x > vb.max.x || lo.y > vb.max.y || lo.z > vb.max.z) return false;
+    if (vb.min.x > hi.x || vb.min.y > hi.y || vb.min.z > hi.z) return false;
+    vb.min = {std::max(vb.min.x, lo.x), std::max(vb.min.y, lo.y), std::max(vb.min.z, lo.z)};
+    vb.max = {std::min(vb.max.x, hi.x), std::min(vb.max.y, hi.y), std::min(vb.max.z, hi.z)};
+    return true;
+}
+
+inline void atomic_max_u32(u32 *p, u32 v) {
+    u32 cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+    while (cur < v && !__atomic_compare_exchange_n(p, &cur, v, true, __ATOMIC_RELAXED,
+                                                    __ATOMIC_RELAXED)) {
+    }
+}
+
+// Where processVoxel (2946-2989) stores: node index of the (already +1
+// adjusted) voxel for a Cartesian grid or FCC sub-grid g.
+struct SurfTarget {
+    u32 resX, resY, resZ;  // allocated per-grid dims (ext dims + padding)
+    int gridType;          // 0: Cartesian, 1..4: FCC sub-grid
+    u32 *keys;
+    size_t nkeys;
+    size_t index(int32_t x, int32_t y, int32_t z) const {
+        if (gridType == 0) {
+            return (size_t)resX * resY * (u32)z + (size_t)resX * (u32)y + (u32)x;
+        }
+        size_t l = 2 * (size_t)resX, a = l * resY;
+        size_t ox = gridType == 2 || gridType == 4, oz = gridType == 3 || gridType == 4;
+        return a * (2 * (size_t)(u32)z + oz) + l * (u32)y + 2 * (size_t)(u32)x + ox;
+    }
+    void write(int32_t x, int32_t y, int32_t z, u32 key) const {
+        size_t i = index(x, y, z);
+        if (i < nkeys) atomic_max_u32(&keys[i], key);
+    }
+};
+
+// process{1D,2D,3D}Triangles for one triangle and one subspace
+// (3307-3411, 3419-3596, 3604-3955).  Coordinates are ext-relative; the
+// single-device padding adjustment is (1, 1, 1).
+void surface_triangle(const F3 t[3], u32 triIdx, u32 type, F3 m, float d, const u32 smin[3],
+                      const u32 smax[3], const SurfTarget &st) {
+    const int bb = (int)(type >> 30);
+    const int axis = (int)((type >> 28) & 3u);
+    const u32 key = ((u32)bb << 30) | triIdx;
+    F3 mn, mx;
+    tri_bounds(t, mn, mx);
+    VoxBB vb = voxel_bounds_half(mn, mx, m, d);
+    if (!adjust_range(vb, smin, smax)) return;
+    if (bb == BB_1D) {
+        I3 lo = {vb.min.x + 1, vb.min.y + 1, vb.min.z + 1};
+        I3 hi = {vb.max.x + 1, vb.max.y + 1, vb.max.z + 1};
+        if (axis == AX_X)
+            for (int32_t i = lo.x; i <= hi.x; i++) st.write(i, lo.y, lo.z, key);
+        else if (axis == AX_Y)
+            for (int32_t j = lo.y; j <= hi.y; j++) st.write(lo.x, j, lo.z, key);
+        else
+            for (int32_t k = lo.z; k <= hi.z; k++) st.write(lo.x, lo.y, k, key);
+        return;
+    }
+    F3 n = cross(sub(t[0], t[2]), sub(t[1], t[0]));
+    if (bb == BB_2D) {
+        if (axis == AX_X) {
+            OverlapData o = setup_yz(t, n, d);
+            for (int32_t j = vb.min.y; j <= vb.max.y; j++)
+                for (int32_t k = vb.min.z; k <= vb.max.z; k++) {
+                    F3 p = {0.0f, vox_corner(j, m.y, d), vox_corner(k, m.z, d)};
+                    if (test_yz(o, p)) st.write(vb.min.x + 1, j + 1, k + 1, key);
+                }
+        } else if (axis == AX_Y) {
+            OverlapData o = setup_zx(t, n, d);
+            for (int32_t i = vb.min.x; i <= vb.max.x; i++)
+                for (int32_t k = vb.min.z; k <= vb.max.z; k++) {
+                    F3 p = {vox_corner(i, m.x, d), 0.0f, vox_corner(k, m.z, d)};
+                    if (test_zx(o, p)) st.write(i + 1, vb.min.y + 1, k + 1, key);
+                }
+        } else {
+            OverlapData o = setup_xy(t, n, d);
+            for (int32_t i = vb.min.x; i <= vb.max.x; i++)
+                for (int32_t j = vb.min.y; j <= vb.max.y; j++) {
+                    F3 p = {vox_corner(i, m.x, d), vox_corner(j, m.y, d), 0.0f};
+                    if (test_xy(o, p)) st.write(i + 1, j + 1, vb.min.z + 1, key);
+                }
+        }
+        return;
+    }
+    if (bb != BB_3D) return;
+    OverlapData oyz = setup_yz(t, n, d), ozx = setup_zx(t, n, d), oxy = setup_xy(t, n, d);
+    if (axis == AX_X) {
+        for (int32_t j = vb.min.y; j <= vb.max.y; j++)
+            for (int32_t k = vb.min.z; k <= vb.max.z; k++) {
+                F3 p = {0.0f, vox_corner(j, m.y, d), vox_corner(k, m.z, d)};
+                if (!test_yz(oyz, p)) continue;
+                int32_t lo, hi;
+                depth_range(t, p, n, m, d, AX_X, lo, hi);
+                if (lo > vb.max.x || hi < vb.min.x) continue;
+                lo = lo < vb.min.x ? vb.min.x : lo;
+                hi = hi > vb.max.x ? vb.max.x : hi;
+                if (lo == hi) {
+                    st.write(lo + 1, j + 1, k + 1, key);
+                } else {
+                    for (int32_t i = lo; i <= hi; i++) {
+                        p.x = vox_corner(i, m.x, d);
+                        if (test_zx(ozx, p) && test_xy(oxy, p)) st.write(i + 1, j + 1, k + 1, key);
+                    }
+                }
+            }
+    } else if (axis == AX_Y) {
+        for (int32_t k = vb.min.z; k <= vb.max.z; k++)
+            for (int32_t i = vb.min.x; i <= vb.max.x; i++) {
+                F3 p = {vox_corner(i, m.x, d), 0.0f, vox_corner(k, m.z, d)};
+                if (!test_zx(ozx, p)) continue;
+                int32_t lo, hi;
+                depth_range(t, p, n, m, d, AX_Y, lo, hi);
+                if (lo > vb.max.y || hi < vb.min.y) continue;
+                lo = lo < vb.min.y ? vb.min.y : lo;
+                hi = hi > vb.max.y ? vb.max.y : hi;
+                if (lo == hi) {
+                    st.write(i + 1, lo + 1, k + 1, key);
+                } else {
+                    for (int32_t j = lo; j <= hi; j++) {
+                        p.y = vox_corner(j, m.y, d);
+                        if (test_xy(oxy, p) && test_yz(oyz, p)) st.write(i + 1, j + 1, k + 1, key);
+                    }
+                }
+            }
+    } else {
+        for (int32_t i = vb.min.x; i <= vb.max.x; i++)
+            for (int32_t j = vb.min.y; j <= vb.max.y; j++) {
+                F3 p = {vox_corner(i, m.x, d), vox_corner(j, m.y, d), 0.0f};
+                if (!test_xy(oxy, p)) continue;
+                int32_t lo, hi;
+                depth_range(t, p, n, m, d, AX_Z, lo, hi);
+                if (lo > vb.max.z || hi < vb.min.z) continue;
+                lo = lo < vb.min.z ? vb.min.z : lo;
+                hi = hi > vb.max.z ? vb.max.z : hi;
+                if (lo == hi) {
+                    st.write(i + 1, j + 1, lo + 1, key);
+                } else {
+                    for (int32_t k = lo; k <= hi; k++) {
+                        p.z = vox_corner(k, m.z, d);
+                        if (test_yz(oyz, p) && test_zx(ozx, p)) st.write(i + 1, j + 1, k + 1, key);
+                    }
+                }
+            }
+    }
+}
+
+// calcTriangleClassification + calcOptSurfaceVoxelization for one grid
+// (src/voxelizer.cu:505-724; splits as in src/host_side.cpp:1107-1134).
+void surface_grid(const float *V, const u32 *I, u32 nt, const vox_grid &g,
+                  const double extMin[3], const u32 matSplit[3], const SurfTarget &st) {
+    const F3 m = {(float)extMin[0], (float)extMin[1], (float)extMin[2]};
+    const float d = (float)g.voxel_length;
+    std::vector<u32> types(nt);
+    for (u32 t = 0; t < nt; t++) {
+        F3 tri[3] = {fetch_f(V, I, t, 0), fetch_f(V, I, t, 1), fetch_f(V, I, t, 2)};
+        types[t] = encode_type(classify(tri, m, d));
+    }
+    // The reference sorts by type and runs the 1D, 2D and 3D kernels on the
+    // class ranges; with the max-key winner rule only membership matters.
+    std::vector<Span> xs = split_axis_max(g.res[0], matSplit[0]);
+    std::vector<Span> ys = split_axis_max(g.res[1], matSplit[1]);
+    std::vector<Span> zs = split_axis_max(g.res[2], matSplit[2]);
+    for (const Span &z : zs)
+        for (const Span &y : ys)
+            for (const Span &x : xs) {
+                const u32 smin[3] = {x.min, y.min, z.min}, smax[3] = {x.max, y.max, z.max};
+#pragma omp parallel for schedule(dynamic, 256)
+                for (long t = 0; t < (long)nt; t++) {
+                    u32 bb = types[t] >> 30;
+                    if (bb == BB_DEGEN) continue;
+                    F3 tri[3] = {fetch_f(V, I, (u32)t, 0), fetch_f(V, I, (u32)t, 1),
+                                 fetch_f(V, I, (u32)t, 2)};
+                    surface_triangle(tri, (u32)t, types[t], m, d, smin, smax, st);
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Nodes.
+// ---------------------------------------------------------------------------
+struct NodeRec { uint16_t bid; uint8_t mat; };
+
+// fillNodeList2 repeated until the error flag stays false, per alloc subspace
+// (src/voxelizer.cu:1791-1913, src/host_side.cpp:1183-1206).  In place, in
+// index order: the literal single-thread restatement.
+void erode_inplace(std::vector<NodeRec> &nodes, u32 X, u32 Y, u32 Z, Span ys, Span zs) {
+    const size_t npx = X, npy = (size_t)X * Y;
+    bool error = true;
+    while (error) {
+        error = false;
+        for (u32 z = zs.min; z < zs.max; z++)
+            for (u32 y = ys.min; y < ys.max; y++)
+                for (u32 x = 0; x < X; x++) {
+                    size_t n = npy * z + npx * y + x;
+                    NodeRec &nd = nodes[n];
+                    if (nd.bid == 0) continue;
+                    uint8_t perm = 0;
+                    if (x > 0 && nodes[n - 1].bid > 0) perm += 1;
+                    if (x < X - 1 && nodes[n + 1].bid > 0) perm += 2;
+                    if (y > 0 && nodes[n - npx].bid > 0) perm += 4;
+                    if (y < Y - 1 && nodes[n + npx].bid > 0) perm += 8;
+                    if (z > 0 && nodes[n - npy].bid > 0) perm += 16;
+                    if (z < Z - 1 && nodes[n + npy].bid > 0) perm += 32;
+                    uint8_t nb = kOrient.lut[perm];
+                    nd.bid = nb;
+                    if (nb == 0) error = true;
+                }
+    }
+}
+
+// Same fixed point computed with snapshot (Jacobi) passes in parallel: the
+// removal rule is monotone, so both orders reach the greatest fixed point of
+// the subspace with the outside frozen (tests check the two agree).
+void erode_parallel(std::vector<NodeRec> &nodes, u32 X, u32 Y, u32 Z, Span ys, Span zs) {
+    const size_t npx = X, npy = (size_t)X * Y;
+    std::vector<uint8_t> solid(nodes.size());
+    for (size_t i = 0; i < nodes.size(); i++) solid[i] = nodes[i].bid > 0;
+    bool changed = true;
+    std::vector<uint8_t> next;
+    while (changed) {
+        next = solid;
+        long cnt = 0;
+#pragma omp parallel for reduction(+ : cnt) schedule(static)
+        for (long zz = zs.min; zz < (long)zs.max; zz++) {
+            u32 z = (u32)zz;
+            for (u32 y = ys.min; y < ys.max; y++)
+                for (u32 x = 0; x < X; x++) {
+                    size_t n = npy * z + npx * y + x;
+                    if (!solid[n]) continue;
+                    bool lx = x > 0 && solid[n - 1], rx = x < X - 1 && solid[n + 1];
+                    bool iy = y > 0 && solid[n - npx], oy = y < Y - 1 && solid[n + npx];
+                    bool dz = z > 0 && solid[n - npy], uz = z < Z - 1 && solid[n + npy];
+                    if (!((lx || rx) && (iy || oy) && (dz || uz))) {
+                        next[n] = 0;
+                        cnt++;
+                    }
+                }
+        }
+        changed = cnt > 0;
+        solid.swap(next);
+    }
+#pragma omp parallel for schedule(static)
+    for (long zz = zs.min; zz < (long)zs.max; zz++) {
+        u32 z = (u32)zz;
+        for (u32 y = ys.min; y < ys.max; y++)
+            for (u32 x = 0; x < X; x++) {
+                size_t n = npy * z + npx * y + x;
+                if (nodes[n].bid == 0) continue;
+                if (!solid[n]) {
+                    nodes[n].bid = 0;
+                    continue;
+                }
+                uint8_t perm = 0;
+                if (x > 0 && solid[n - 1]) perm += 1;
+                if (x < X - 1 && solid[n + 1]) perm += 2;
+                if (y > 0 && solid[n - npx]) perm += 4;
+                if (y < Y - 1 && solid[n + npx]) perm += 8;
+                if (z > 0 && solid[n - npy]) perm += 16;
+                if (z < Z - 1 && solid[n + npy]) perm += 32;
+                nodes[n].bid = kOrient.lut[perm];
+            }
+    }
+}
+
+// calculateFCCBoundaries (src/voxelizer.cu:1930-2048), in place in index
+// order, one pass per alloc subspace.
+void fcc_boundaries(std::vector<NodeRec> &nodes, u32 X, u32 Y, u32 Z, Span ys, Span zs) {
+    const size_t npx = X, npy = (size_t)X * Y;
+    for (u32 z = zs.min; z < zs.max; z++)
+        for (u32 y = ys.min; y < ys.max; y++)
+            for (u32 x = 0; x < X; x++) {
+                size_t n = npy * z + npx * y + x;
+                if (nodes[n].bid == 0) continue;
+                uint16_t nb = 0;
+                const bool left = x > 0, right = x < X - 1, in = y > 0, out = y < Y - 1;
+                const bool down = z > 0, up = z < Z - 1;
+                const bool e = (x + z) % 2 == 0;
+                const size_t ye = e ? npx : 0, yo = e ? 0 : npx;
+                auto s = [&](size_t i) { return nodes[i].bid > 0; };
+                if (down && ((e && in) || !e) && s(n - npy - ye)) nb += 1;
+                if (down && left && s(n - npy - 1)) nb += 2;
+                if (down && right && s(n - npy + 1)) nb += 4;
+                if (down && ((!e && out) || e) && s(n - npy + yo)) nb += 8;
+                if (left && ((e && in) || !e) && s(n - 1 - ye)) nb += 16;
+                if (right && ((e && in) || !e) && s(n + 1 - ye)) nb += 32;
+                if (left && ((!e && out) || e) && s(n - 1 + yo)) nb += 64;
+                if (right && ((!e && out) || e) && s(n + 1 + yo)) nb += 128;
+                if (up && ((e && in) || !e) && s(n + npy - ye)) nb += 256;
+                if (up && left && s(n + npy - 1)) nb += 512;
+                if (up && right && s(n + npy + 1)) nb += 1024;
+                if (up && ((!e && out) || e) && s(n + npy + yo)) nb += 2048;
+                nodes[n].bid = nb;
+            }
+}
+
+// Node packing (include/node_types.h): ShortNode bim = mat<<5 | bid&31 in a
+// uchar; LongNode {bid, mat}; ShortFCCNode bim = mat<<12 | bid&4095 in a
+// ushort; LongFCCNode {ushort bid, uchar mat, pad}.
+void pack_nodes(const std::vector<NodeRec> &nodes, int kind, void *out) {
+    size_t n = nodes.size();
+    if (kind == VOX_NODE_SHORT) {
+        uint8_t *o = (uint8_t *)out;
+        for (size_t i = 0; i < n; i++)
+            o[i] = (uint8_t)((nodes[i].mat << 5) | (nodes[i].bid & 31));
+    } else if (kind == VOX_NODE_LONG) {
+        uint8_t *o = (uint8_t *)out;
+        for (size_t i = 0; i < n; i++) {
+            o[2 * i] = (uint8_t)nodes[i].bid;
+            o[2 * i + 1] = nodes[i].mat;
+        }
+    } else if (kind == VOX_NODE_SHORT_FCC) {
+        uint16_t *o = (uint16_t *)out;
+        for (size_t i = 0; i < n; i++)
+            o[i] = (uint16_t)((nodes[i].mat << 12) | (nodes[i].bid & 4095));
+    } else {
+        uint8_t *o = (uint8_t *)out;
+        for (size_t i = 0; i < n; i++) {
+            uint16_t b = nodes[i].bid;
+            std::memcpy(o + 4 * i, &b, 2);
+            o[4 * i + 2] = nodes[i].mat;
+            o[4 * i + 3] = 0;
+        }
+    }
+}
+
+int voxelize_nodes(const float *V, u32 nv, const u32 *I, u32 nt, const uint8_t *mats,
+                   const vox_grid &g, int kind, const u32 voxSplit[3], const u32 matSplit[3],
+                   void *out, bool literal) {
+    (void)nv;
+    const bool fcc = kind == VOX_NODE_SHORT_FCC || kind == VOX_NODE_LONG_FCC;
+    if (kind != VOX_NODE_SHORT && kind != VOX_NODE_LONG && !fcc) return 1;
+    const u32 X = g.alloc[0], Y = g.alloc[1], Z = g.alloc[2];
+    const size_t words = (size_t)X / 32 * Y * Z;
+    std::vector<u32> bits(words);
+    const double base[3] = {g.min_vertex[0], g.min_vertex[1], g.min_vertex[2]};
+    if (!fcc) {
+        // voxelizeWorker (src/host_side.cpp:901-1244).
+        const size_t N = (size_t)X * Y * Z;
+        solid_bits(V, I, nt, g, base, voxSplit[1], voxSplit[2], bits.data());
+        std::vector<NodeRec> nodes(N);
+        // constructNodeList2 (src/voxelizer.cu:1627-1681).
+#pragma omp parallel for schedule(static)
+        for (long w = 0; w < (long)words; w++)
+            for (u32 b = 0; b < 32; b++)
+                nodes[(size_t)w * 32 + b] = {(uint16_t)((bits[w] >> (31 - b)) & 1u), 0};
+        if (mats) {
+            std::vector<u32> keys(N, 0u);
+            SurfTarget st{X, Y, Z, 0, keys.data(), N};
+            surface_grid(V, I, nt, g, base, matSplit, st);
+            // processVoxel stores Node(1, materials[tri]).
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < (long)N; i++)
+                if (keys[i]) nodes[i] = {1, mats[keys[i] & 0x3FFFFFFFu]};
+        }
+        // allocYzSplits over allocPartition (src/host_side.cpp:1062-1067).
+        std::vector<Span> ys = split_axis_max(Y, voxSplit[1]), zs = split_axis_max(Z, voxSplit[2]);
+        for (const Span &z : zs)
+            for (const Span &y : ys) {
+                if (literal) erode_inplace(nodes, X, Y, Z, y, z);
+                else erode_parallel(nodes, X, Y, Z, y, z);
+            }
+        pack_nodes(nodes, kind, out);
+        return 0;
+    }
+    // fccWorker (src/host_side.cpp:1262-1698).
+    const u32 FX = 2 * X, FY = Y, FZ = 2 * Z;
+    const size_t N = (size_t)FX * FY * FZ;
+    std::vector<NodeRec> nodes(N);
+    const double h = g.voxel_length / 2.0;
+    const double shifts[4][3] = {{0.0, 0.0, 0.0}, {h, h, 0.0}, {0.0, h, h}, {h, 0.0, h}};
+    for (int gt = 1; gt <= 4; gt++) {
+        double em[3] = {base[0] + shifts[gt - 1][0], base[1] + shifts[gt - 1][1],
+                        base[2] + shifts[gt - 1][2]};
+        solid_bits(V, I, nt, g, em, voxSplit[1], voxSplit[2], bits.data());
+        // convertToFCCGrid (src/voxelizer.cu:1711-1776).
+        const u32 ox = (gt == 2 || gt == 4), oz = (gt == 3 || gt == 4);
+#pragma omp parallel for schedule(static)
+        for (long zz = 0; zz < (long)Z; zz++) {
+            u32 z = (u32)zz;
+            for (u32 y = 0; y < Y; y++)
+                for (u32 x = 0; x < X; x++) {
+                    size_t w = ((size_t)z * Y + y) * (X / 32) + x / 32;
+                    uint16_t bit = (uint16_t)((bits[w] >> (31 - x % 32)) & 1u);
+                    size_t idx = (size_t)FX * FY * (2 * z + oz) + (size_t)FX * y + 2 * x + ox;
+                    nodes[idx] = {bit, 0};
+                }
+        }
+    }
+    if (mats) {
+        std::vector<u32> keys(N, 0u);
+        for (int gt = 1; gt <= 4; gt++) {
+            double em[3] = {base[0] + shifts[gt - 1][0], base[1] + shifts[gt - 1][1],
+                            base[2] + shifts[gt - 1][2]};
+            SurfTarget st{X, Y, Z, gt, keys.data(), N};
+            surface_grid(V, I, nt, g, em, matSplit, st);
+        }
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < (long)N; i++)
+            if (keys[i]) nodes[i] = {1, mats[keys[i] & 0x3FFFFFFFu]};
+    }
+    // allocYzSplits computed on the per-grid alloc partition, then z doubled
+    // (src/host_side.cpp:1415-1419, 1500-1511).
+    std::vector<Span> ys = split_axis_max(Y, voxSplit[1]), zs = split_axis_max(Z, voxSplit[2]);
+    for (const Span &z : zs)
+        for (const Span &y : ys) fcc_boundaries(nodes, FX, FY, FZ, y, {2 * z.min, 2 * z.max});
+    pack_nodes(nodes, kind, out);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint8_t orc_orientation(uint8_t permutation) { return kOrient.lut[permutation & 63]; }
+
+int orc_plan_grid(const float *v, uint32_t nv, int mode, double param, uint32_t vsx,
+                  vox_grid *g) {
+    if (nv == 0 || vsx == 0) return 1;
+    // calculateBoundingBox (src/host_side.cpp:2462-2488).
+    double mn[3] = {v[0], v[1], v[2]}, mx[3] = {v[0], v[1], v[2]};
+    for (uint32_t i = 1; i < nv; i++)
+        for (int c = 0; c < 3; c++) {
+            if (v[3 * i + c] > mx[c]) mx[c] = v[3 * i + c];
+            if (v[3 * i + c] < mn[c]) mn[c] = v[3 * i + c];
+        }
+    double dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+    u32 rx, ry, rz;
+    double d;
+    if (mode == 0) {
+        // determineDimensions() (src/host_side.cpp:2497-2580).
+        u32 R = (u32)param;
+        if (dx > dy) {
+            if (dx > dz) {
+                d = dx / double(R - 1);
+                ry = sat_u32(std::ceil(dy / d)) + 1;
+                rz = sat_u32(std::ceil(dz / d)) + 1;
+                rx = R + 2;
+            } else {
+                d = dz / double(R - 1);
+                rz = R;
+                rx = sat_u32(std::ceil(dx / d)) + 1;
+                ry = sat_u32(std::ceil(dy / d)) + 1;
+                rx += 2;
+            }
+        } else {
+            if (dy > dz) {
+                d = dy / double(R - 1);
+                ry = R;
+                rx = sat_u32(std::ceil(dx / d)) + 1;
+                rz = sat_u32(std::ceil(dz / d)) + 1;
+                rx += 2;
+            } else {
+                d = dz / double(R - 1);
+                rz = R;
+                rx = sat_u32(std::ceil(dx / d)) + 1;
+                ry = sat_u32(std::ceil(dy / d)) + 1;
+                rx += 2;
+            }
+        }
+    } else {
+        // determineDimensions(double d) (src/host_side.cpp:2592-2609).
+        d = param;
+        rx = sat_u32(std::ceil(dx / d)) + 1;
+        ry = sat_u32(std::ceil(dy / d)) + 1;
+        rz = sat_u32(std::ceil(dz / d)) + 1;
+        rx += 2;
+    }
+    // splitResolutionByMaxDim(voxSplitRes.x) + adjustResolution
+    // (src/host_side.cpp:730-738, 2620-2634).
+    u32 xs = (rx + vsx - 1) / vsx;
+    u32 xmod = 32u * xs;
+    if (rx % xmod != 0) rx += xmod - rx % xmod;
+    for (int c = 0; c < 3; c++) {
+        g->min_vertex[c] = mn[c];
+        g->max_vertex[c] = mx[c];
+    }
+    g->voxel_length = d;
+    g->res[0] = rx;
+    g->res[1] = ry;
+    g->res[2] = rz;
+    g->alloc[0] = rx;
+    g->alloc[1] = ry + 2;
+    g->alloc[2] = rz + 2;
+    g->x_splits = xs;
+    g->pad_ = 0;
+    return 0;
+}
+
+int orc_solid_bits(const float *vertices, const uint32_t *indices, uint32_t nt,
+                   const vox_grid *grid, const double ext_min[3], uint32_t split_y,
+                   uint32_t split_z, uint32_t *bits) {
+    if (!grid || grid->x_splits == 0 || split_y == 0 || split_z == 0) return 1;
+    solid_bits(vertices, indices, nt, *grid, ext_min, split_y, split_z, bits);
+    return 0;
+}
+
+int orc_voxelize_nodes(const float *vertices, uint32_t nv, const uint32_t *indices,
+                       uint32_t nt, const uint8_t *materials, const vox_grid *grid,
+                       int node_kind, const uint32_t vox_split[3], const uint32_t mat_split[3],
+                       void *nodes, int n_threads) {
+#ifdef _OPENMP
+    int prev = omp_get_max_threads();
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#endif
+    bool literal = n_threads == 1;
+    int r = voxelize_nodes(vertices, nv, indices, nt, materials, *grid, node_kind, vox_split,
+                           mat_split, nodes, literal);
+#ifdef _OPENMP
+    omp_set_num_threads(prev);
+#endif
+    return r;
+}
+
+int orc_surface_keys(const float *vertices, const uint32_t *indices, uint32_t nt,
+                     const vox_grid *grid, const double ext_min[3], const uint32_t mat_split[3],
+                     uint32_t *keys) {
+    const vox_grid &g = *grid;
+    size_t N = (size_t)g.alloc[0] * g.alloc[1] * g.alloc[2];
+    std::memset(keys, 0, N * 4);
+    SurfTarget st{g.alloc[0], g.alloc[1], g.alloc[2], 0, keys, N};
+    surface_grid(vertices, indices, nt, g, ext_min, mat_split, st);
+    return 0;
+}
+
+void orc_classify(const float *vertices, const uint32_t *indices, uint32_t nt,
+                  const float model_min[3], float d, uint32_t *out) {
+    F3 m = {model_min[0], model_min[1], model_min[2]};
+    for (u32 t = 0; t < nt; t++) {
+        F3 tri[3] = {fetch_f(vertices, indices, t, 0), fetch_f(vertices, indices, t, 1),
+                     fetch_f(vertices, indices, t, 2)};
+        out[t] = encode_type(classify(tri, m, d));
+    }
+}
+
+int orc_center_crossing(const float tri[9], const double ext_min[3], double d, uint32_t cy,
+                        uint32_t cz, uint32_t *voxel_x) {
+    D3 v[3];
+    for (int k = 0; k < 3; k++) v[k] = {tri[3 * k], tri[3 * k + 1], tri[3 * k + 2]};
+    FillSetup fs = fill_setup(v);
+    u32 vx = 0;
+    bool ok = fill_center(fs, ext_min, d, cy, cz, vx);
+    if (voxel_x) *voxel_x = vx;
+    return ok ? 1 : 0;
+}
+
+uint32_t orc_tile_overlaps(const float tri[9], const double ext_min[3], double d, uint32_t ext_y,
+                           uint32_t ext_z) {
+    D3 v[3];
+    for (int k = 0; k < 3; k++) v[k] = {tri[3 * k], tri[3 * k + 1], tri[3 * k + 2]};
+    TileSetup ts;
+    const u32 smin[2] = {0, 0}, smax[2] = {ext_y, ext_z};
+    if (!tile_setup(v, ext_min, d, smin, smax, ts)) return 0;
+    u32 n = 0;
+    traverse_tiles(ts, ext_min, d, [&](u32, u32) { n++; });
+    return n;
+}
+
+}  // extern "C"

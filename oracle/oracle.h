@@ -1,0 +1,80 @@
+/*
+ * oracle.h -- CPU restatement of hakarlss/Voxelizer's hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or the timed CPU baseline); the product library never links or calls it.
+ *
+ * Every function follows the reference file:line given in oracle.cpp.  FP64 in
+ * the solid stage, FP32 in the surface stage, no FMA contraction
+ * (-ffp-contract=off), IEEE rounding, saturating float->int conversions (what
+ * both CUDA's cvt.rzi and gfx950's v_cvt_* do).
+ *
+ * Material winners: the reference's processVoxel does a racy plain store
+ * (src/voxelizer.cu:2988); this restatement uses the deterministic rule
+ * "largest (bbType, triangle index) wins", which is one of the outcomes the
+ * reference can produce (kernels run 1D, 2D, 3D in order, so the highest class
+ * always wins there too).
+ */
+#ifndef VOX_ORACLE_H
+#define VOX_ORACLE_H
+
+#include <stdint.h>
+#include "../include/voxelizer_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Boundary id of a 6-bit neighbour permutation (src/voxelizer.cu:2142-2236). */
+uint8_t orc_orientation(uint8_t permutation);
+
+/* Bounding box + dims (src/host_side.cpp:2462-2634, 717-738).  mode 0:
+ * maxDimension (param = R), mode 1: cubeLength.  Returns 0 on success. */
+int orc_plan_grid(const float *vertices, uint32_t nv, int mode, double param,
+                  uint32_t vox_split_x, vox_grid *out);
+
+/* Solid bits exactly as performVoxelization + generateVoxelization compute
+ * them for one device with no neighbours: ext_min is extMinVertex (minVertex
+ * plus an optional FCC shift); split_y/split_z are voxSplitRes.y/z; the x
+ * splits come from grid->x_splits.  bits: alloc words, zeroed by the callee. */
+int orc_solid_bits(const float *vertices, const uint32_t *indices, uint32_t nt,
+                   const vox_grid *grid, const double ext_min[3],
+                   uint32_t split_y, uint32_t split_z, uint32_t *bits);
+
+/* Whole node pipeline (voxelizeWorker / fccWorker for one device).
+ * materials may be NULL (no surface stage).  nodes: node_count * node_size
+ * bytes.  vox_split/mat_split: voxSplitRes/matSplitRes.  Returns 0 on
+ * success, nonzero for an unsupported kind.  n_threads: OpenMP threads used
+ * for the embarrassingly parallel loops (0 = all), results do not depend on it. */
+int orc_voxelize_nodes(const float *vertices, uint32_t nv, const uint32_t *indices,
+                       uint32_t nt, const uint8_t *materials, const vox_grid *grid,
+                       int node_kind, const uint32_t vox_split[3],
+                       const uint32_t mat_split[3], void *nodes, int n_threads);
+
+/* Surface stage alone on a Cartesian grid: for each voxel of the alloc grid
+ * the winning key (bbType<<30 | triangle) or 0 (src/voxelizer.cu:3307-3955). */
+int orc_surface_keys(const float *vertices, const uint32_t *indices, uint32_t nt,
+                     const vox_grid *grid, const double ext_min[3],
+                     const uint32_t mat_split[3], uint32_t *keys);
+
+/* Per-triangle surface classification (src/voxelizer.cu:3040-3221):
+ * out[t] = bbType<<30 | domAxis<<28 | nrOfVoxCols. */
+void orc_classify(const float *vertices, const uint32_t *indices, uint32_t nt,
+                  const float model_min[3], float d, uint32_t *out);
+
+/* Fill-stage test of one voxel centre against one triangle (src/voxelizer.cu:
+ * 1375-1487): returns 1 and the x crossing voxelX (incl. the +1 padding) if the
+ * centre passes; used by the hand-built tie-break tests. */
+int orc_center_crossing(const float tri[9], const double ext_min[3], double d,
+                        uint32_t cy, uint32_t cz, uint32_t *voxel_x);
+
+/* Tile filter of the count pass (src/voxelizer.cu:840-1033): number of 4x4
+ * tiles the triangle is queued on for the given ext y/z extent. */
+uint32_t orc_tile_overlaps(const float tri[9], const double ext_min[3], double d,
+                           uint32_t ext_y, uint32_t ext_z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

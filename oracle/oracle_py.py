@@ -1,0 +1,162 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline,
+never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+NODE_SHORT, NODE_LONG, NODE_PARTIAL, NODE_SHORT_FCC, NODE_LONG_FCC = range(5)
+NODE_SIZE = {NODE_SHORT: 1, NODE_LONG: 2, NODE_SHORT_FCC: 2, NODE_LONG_FCC: 4}
+
+
+class Grid(C.Structure):
+    """Mirror of vox_grid (include/voxelizer_amd.h)."""
+    _fields_ = [("min_vertex", C.c_double * 3), ("max_vertex", C.c_double * 3),
+                ("voxel_length", C.c_double), ("res", C.c_uint32 * 3),
+                ("alloc", C.c_uint32 * 3), ("x_splits", C.c_uint32), ("pad_", C.c_uint32)]
+
+    def as_dict(self):
+        return {"min_vertex": list(self.min_vertex), "max_vertex": list(self.max_vertex),
+                "voxel_length": self.voxel_length, "res": list(self.res),
+                "alloc": list(self.alloc), "x_splits": self.x_splits}
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.orc_orientation.restype = C.c_uint8
+        L.orc_orientation.argtypes = [C.c_uint8]
+        L.orc_plan_grid.argtypes = [P, C.c_uint32, C.c_int, C.c_double, C.c_uint32, C.POINTER(Grid)]
+        L.orc_solid_bits.argtypes = [P, P, C.c_uint32, C.POINTER(Grid), P, C.c_uint32, C.c_uint32, P]
+        L.orc_voxelize_nodes.argtypes = [P, C.c_uint32, P, C.c_uint32, P, C.POINTER(Grid), C.c_int,
+                                         P, P, P, C.c_int]
+        L.orc_surface_keys.argtypes = [P, P, C.c_uint32, C.POINTER(Grid), P, P, P]
+        L.orc_classify.argtypes = [P, P, C.c_uint32, P, C.c_float, P]
+        L.orc_center_crossing.argtypes = [P, P, C.c_double, C.c_uint32, C.c_uint32, P]
+        L.orc_tile_overlaps.restype = C.c_uint32
+        L.orc_tile_overlaps.argtypes = [P, P, C.c_double, C.c_uint32, C.c_uint32]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def orientation(perm: int) -> int:
+    return int(lib().orc_orientation(perm))
+
+
+def plan_grid(vertices: np.ndarray, mode: int, param: float, vox_split_x: int = 1024) -> Grid:
+    v = np.ascontiguousarray(vertices, np.float32)
+    g = Grid()
+    r = lib().orc_plan_grid(_p(v), v.shape[0], mode, float(param), vox_split_x, C.byref(g))
+    if r != 0:
+        raise ValueError("orc_plan_grid failed")
+    return g
+
+
+def solid_bits(mesh, grid: Grid, shift=(0.0, 0.0, 0.0), split_yz=(512, 512)) -> np.ndarray:
+    v = np.ascontiguousarray(mesh.vertices, np.float32)
+    t = np.ascontiguousarray(mesh.indices, np.uint32)
+    X, Y, Z = grid.alloc
+    bits = np.zeros(X // 32 * Y * Z, np.uint32)
+    em = np.array([grid.min_vertex[a] + shift[a] for a in range(3)], np.float64)
+    r = lib().orc_solid_bits(_p(v), _p(t), t.shape[0], C.byref(grid), _p(em), split_yz[0],
+                             split_yz[1], _p(bits))
+    if r != 0:
+        raise ValueError("orc_solid_bits failed")
+    return bits
+
+
+def node_dims(grid: Grid, kind: int):
+    X, Y, Z = grid.alloc
+    if kind in (NODE_SHORT_FCC, NODE_LONG_FCC):
+        return (2 * X, Y, 2 * Z)
+    return (X, Y, Z)
+
+
+def voxelize_nodes(mesh, grid: Grid, kind: int, use_materials: bool = True,
+                   vox_split=(1024, 512, 512), mat_split=(512, 512, 512),
+                   n_threads: int = 0) -> np.ndarray:
+    v = np.ascontiguousarray(mesh.vertices, np.float32)
+    t = np.ascontiguousarray(mesh.indices, np.uint32)
+    dims = node_dims(grid, kind)
+    n = dims[0] * dims[1] * dims[2]
+    out = np.zeros(n * NODE_SIZE[kind], np.uint8)
+    mats = None
+    if use_materials and mesh.materials is not None:
+        mats = np.ascontiguousarray(mesh.materials, np.uint8)
+    vs = np.array(vox_split, np.uint32)
+    ms = np.array(mat_split, np.uint32)
+    r = lib().orc_voxelize_nodes(_p(v), v.shape[0], _p(t), t.shape[0],
+                                 _p(mats) if mats is not None else None, C.byref(grid), kind,
+                                 _p(vs), _p(ms), _p(out), n_threads)
+    if r != 0:
+        raise ValueError("orc_voxelize_nodes failed (kind %d)" % kind)
+    return out
+
+
+def surface_keys(mesh, grid: Grid, shift=(0.0, 0.0, 0.0), mat_split=(512, 512, 512)) -> np.ndarray:
+    v = np.ascontiguousarray(mesh.vertices, np.float32)
+    t = np.ascontiguousarray(mesh.indices, np.uint32)
+    X, Y, Z = grid.alloc
+    keys = np.zeros(X * Y * Z, np.uint32)
+    em = np.array([grid.min_vertex[a] + shift[a] for a in range(3)], np.float64)
+    ms = np.array(mat_split, np.uint32)
+    lib().orc_surface_keys(_p(v), _p(t), t.shape[0], C.byref(grid), _p(em), _p(ms), _p(keys))
+    return keys
+
+
+def classify(mesh, model_min, d) -> np.ndarray:
+    v = np.ascontiguousarray(mesh.vertices, np.float32)
+    t = np.ascontiguousarray(mesh.indices, np.uint32)
+    mm = np.array(model_min, np.float32)
+    out = np.zeros(t.shape[0], np.uint32)
+    lib().orc_classify(_p(v), _p(t), t.shape[0], _p(mm), float(d), _p(out))
+    return out
+
+
+def center_crossing(tri, ext_min, d, cy, cz):
+    tr = np.ascontiguousarray(tri, np.float32).ravel()
+    em = np.array(ext_min, np.float64)
+    vx = np.zeros(1, np.uint32)
+    ok = lib().orc_center_crossing(_p(tr), _p(em), float(d), cy, cz, _p(vx))
+    return bool(ok), int(vx[0])
+
+
+def tile_overlaps(tri, ext_min, d, ext_y, ext_z) -> int:
+    tr = np.ascontiguousarray(tri, np.float32).ravel()
+    em = np.array(ext_min, np.float64)
+    return int(lib().orc_tile_overlaps(_p(tr), _p(em), float(d), ext_y, ext_z))
+
+
+def bits_to_grid(bits: np.ndarray, alloc) -> np.ndarray:
+    """Unpack MSB-first bit words to a bool array indexed [z, y, x]."""
+    X, Y, Z = alloc
+    b = bits.view(np.uint32).reshape(Z, Y, X // 32)
+    # MSB first: bit 31 is x%32 == 0.
+    be = b.astype(">u4").view(np.uint8).reshape(Z, Y, X // 32, 4)
+    return np.unpackbits(be, axis=3, bitorder="big").reshape(Z, Y, X).astype(bool)

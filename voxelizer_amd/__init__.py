@@ -1,0 +1,265 @@
+"""voxelizer_amd: MI355X-native voxelizer (hand-written HIP for gfx950).
+
+Python binding of the C ABI in include/voxelizer_amd.h (libvoxelizer_amd.so,
+built in-tree by voxelizer_amd/build.py).  There is no CPU fallback: if the
+library or a HIP device is missing, the calls raise.
+
+    import voxelizer_amd as va
+    ctx = va.Context()
+    ctx.set_mesh(vertices, indices)
+    ctx.set_materials(materials, n_unique)
+    grid = ctx.plan_grid(va.PLAN_CUBE_LENGTH, 0.01)
+    nodes = ctx.voxelize_nodes(grid, va.NODE_LONG)        # numpy, copied back
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvoxelizer_amd.so")
+
+OK, ERR_INVALID, ERR_HIP, ERR_OOM, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(6)
+NODE_SHORT, NODE_LONG, NODE_PARTIAL, NODE_SHORT_FCC, NODE_LONG_FCC = range(5)
+PLAN_MAX_DIMENSION, PLAN_CUBE_LENGTH = 0, 1
+NODE_SIZE = {NODE_SHORT: 1, NODE_LONG: 2, NODE_PARTIAL: 8, NODE_SHORT_FCC: 2, NODE_LONG_FCC: 4}
+DEFAULT_VOX_SPLIT = (1024, 512, 512)
+DEFAULT_MAT_SPLIT = (512, 512, 512)
+
+
+class VoxError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"vox status {status}: {msg}")
+        self.status = status
+
+
+class Grid(C.Structure):
+    """vox_grid of include/voxelizer_amd.h."""
+    _fields_ = [("min_vertex", C.c_double * 3), ("max_vertex", C.c_double * 3),
+                ("voxel_length", C.c_double), ("res", C.c_uint32 * 3),
+                ("alloc", C.c_uint32 * 3), ("x_splits", C.c_uint32), ("pad_", C.c_uint32)]
+
+    def as_dict(self):
+        return {"min_vertex": list(self.min_vertex), "max_vertex": list(self.max_vertex),
+                "voxel_length": self.voxel_length, "res": list(self.res),
+                "alloc": list(self.alloc), "x_splits": self.x_splits}
+
+    def same(self, other) -> bool:
+        return self.as_dict() == other.as_dict()
+
+
+class StageTimes(C.Structure):
+    _fields_ = [("plan_ms", C.c_float), ("solid_bin_ms", C.c_float), ("solid_fill_ms", C.c_float),
+                ("surface_ms", C.c_float), ("erode_ms", C.c_float), ("nodes_ms", C.c_float),
+                ("total_ms", C.c_float), ("erosion_passes", C.c_uint32),
+                ("solid_pairs", C.c_uint32), ("surface_pairs", C.c_uint32),
+                ("surface_voxels", C.c_uint32)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+# Every symbol include/voxelizer_amd.h declares (checked by the CPU tests).
+EXPORTED = [
+    "vox_create", "vox_destroy", "vox_last_error", "vox_device_count", "vox_sync",
+    "vox_build_info", "vox_alloc", "vox_free", "vox_memcpy_h2d", "vox_memcpy_d2h",
+    "vox_set_mesh", "vox_set_mesh_device", "vox_set_materials", "vox_set_materials_device",
+    "vox_plan_grid", "vox_solid_bits", "vox_voxelize_nodes", "vox_node_dims",
+    "vox_node_count", "vox_node_size", "vox_enable_timing", "vox_get_stage_times",
+]
+
+_lib = None
+
+
+def lib():
+    """Loads libvoxelizer_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m voxelizer_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    P, U32, I32, D = C.c_void_p, C.c_uint32, C.c_int, C.c_double
+    S = C.c_int
+    L.vox_create.argtypes = [I32, C.POINTER(P)]
+    L.vox_create.restype = S
+    L.vox_destroy.argtypes = [P]
+    L.vox_destroy.restype = None
+    L.vox_last_error.argtypes = [P]
+    L.vox_last_error.restype = C.c_char_p
+    L.vox_device_count.restype = I32
+    L.vox_sync.argtypes = [P]
+    L.vox_build_info.restype = C.c_char_p
+    L.vox_alloc.argtypes = [P, C.c_size_t, C.POINTER(P)]
+    L.vox_free.argtypes = [P, P]
+    L.vox_memcpy_h2d.argtypes = [P, P, P, C.c_size_t]
+    L.vox_memcpy_d2h.argtypes = [P, P, P, C.c_size_t]
+    L.vox_set_mesh.argtypes = [P, P, U32, P, U32]
+    L.vox_set_mesh_device.argtypes = [P, P, U32, P, U32]
+    L.vox_set_materials.argtypes = [P, P, U32]
+    L.vox_set_materials_device.argtypes = [P, P, U32]
+    L.vox_plan_grid.argtypes = [P, I32, D, U32, C.POINTER(Grid)]
+    L.vox_solid_bits.argtypes = [P, C.POINTER(Grid), P, P, P]
+    L.vox_voxelize_nodes.argtypes = [P, C.POINTER(Grid), I32, I32, P, P, P]
+    L.vox_node_dims.argtypes = [C.POINTER(Grid), I32, P]
+    L.vox_node_dims.restype = None
+    L.vox_node_count.argtypes = [C.POINTER(Grid), I32]
+    L.vox_node_count.restype = C.c_size_t
+    L.vox_node_size.argtypes = [I32]
+    L.vox_node_size.restype = C.c_size_t
+    L.vox_enable_timing.argtypes = [P, I32]
+    L.vox_get_stage_times.argtypes = [P, C.POINTER(StageTimes)]
+    _lib = L
+    return L
+
+
+def device_count() -> int:
+    return int(lib().vox_device_count())
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class DeviceBuffer:
+    """Device allocation owned by a Context (vox_alloc / vox_free)."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        ctx._check(lib().vox_alloc(ctx.h, max(1, self.nbytes), C.byref(p)))
+        self.ptr = p
+
+    def upload(self, arr: np.ndarray):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        self.ctx._check(lib().vox_memcpy_h2d(self.ctx.h, self.ptr, _ptr(a), a.nbytes))
+
+    def download(self, dtype, count=None) -> np.ndarray:
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else int(count)
+        out = np.empty(n, dt)
+        self.ctx._check(lib().vox_memcpy_d2h(self.ctx.h, _ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ctx.h:
+            lib().vox_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One vox_ctx: a device, a stream and a device arena."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        st = lib().vox_create(device, C.byref(h))
+        if st != OK:
+            raise VoxError(st, "vox_create failed (no HIP device?)")
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vox_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int):
+        if st != OK:
+            msg = lib().vox_last_error(self.h)
+            raise VoxError(st, msg.decode() if msg else "")
+
+    # -- mesh ---------------------------------------------------------------
+    def set_mesh(self, vertices: np.ndarray, indices: np.ndarray):
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        self._check(lib().vox_set_mesh(self.h, _ptr(v), v.shape[0], _ptr(t), t.shape[0]))
+        self.nt = t.shape[0]
+
+    def set_mesh_device(self, vptr: int, nv: int, iptr: int, nt: int):
+        self._check(lib().vox_set_mesh_device(self.h, C.c_void_p(vptr), nv, C.c_void_p(iptr), nt))
+        self.nt = nt
+
+    def set_materials(self, materials, n_unique: int = 0):
+        if materials is None:
+            self._check(lib().vox_set_materials(self.h, None, 0))
+            return
+        m = np.ascontiguousarray(materials, np.uint8)
+        self._check(lib().vox_set_materials(self.h, _ptr(m), n_unique))
+
+    def set_materials_device(self, mptr: int, n_unique: int):
+        self._check(lib().vox_set_materials_device(self.h, C.c_void_p(mptr), n_unique))
+
+    # -- plan ---------------------------------------------------------------
+    def plan_grid(self, mode: int, param: float, vox_split_x: int = 1024) -> Grid:
+        g = Grid()
+        self._check(lib().vox_plan_grid(self.h, mode, float(param), vox_split_x, C.byref(g)))
+        return g
+
+    # -- stages -------------------------------------------------------------
+    def solid_bits_device(self, grid: Grid, out_ptr, shift=None, split_yz=(512, 512)):
+        sh = np.array(shift if shift is not None else (0.0, 0.0, 0.0), np.float64)
+        sp = np.array(split_yz, np.uint32)
+        self._check(lib().vox_solid_bits(self.h, C.byref(grid), _ptr(sh), _ptr(sp),
+                                         C.c_void_p(int(out_ptr) if not isinstance(out_ptr, C.c_void_p) else out_ptr.value)))
+
+    def solid_bits(self, grid: Grid, shift=None, split_yz=(512, 512)) -> np.ndarray:
+        words = grid.alloc[0] // 32 * grid.alloc[1] * grid.alloc[2]
+        buf = DeviceBuffer(self, words * 4)
+        self.solid_bits_device(grid, buf.ptr.value, shift, split_yz)
+        out = buf.download(np.uint32, words)
+        buf.free()
+        return out
+
+    def node_dims(self, grid: Grid, kind: int):
+        d = np.zeros(3, np.uint32)
+        lib().vox_node_dims(C.byref(grid), kind, _ptr(d))
+        return tuple(int(x) for x in d)
+
+    def voxelize_nodes_device(self, grid: Grid, kind: int, out_ptr: int, use_materials=True,
+                              vox_split=DEFAULT_VOX_SPLIT, mat_split=DEFAULT_MAT_SPLIT):
+        vs = np.array(vox_split, np.uint32)
+        ms = np.array(mat_split, np.uint32)
+        self._check(lib().vox_voxelize_nodes(self.h, C.byref(grid), kind, 1 if use_materials else 0,
+                                             _ptr(vs), _ptr(ms), C.c_void_p(int(out_ptr))))
+
+    def voxelize_nodes(self, grid: Grid, kind: int, use_materials=True,
+                       vox_split=DEFAULT_VOX_SPLIT, mat_split=DEFAULT_MAT_SPLIT) -> np.ndarray:
+        n = int(lib().vox_node_count(C.byref(grid), kind))
+        nbytes = n * NODE_SIZE[kind]
+        buf = DeviceBuffer(self, nbytes)
+        self.voxelize_nodes_device(grid, kind, buf.ptr.value, use_materials, vox_split, mat_split)
+        out = buf.download(np.uint8, nbytes)
+        buf.free()
+        return out
+
+    def sync(self):
+        self._check(lib().vox_sync(self.h))
+
+    def enable_timing(self, on: bool = True):
+        self._check(lib().vox_enable_timing(self.h, 1 if on else 0))
+
+    def stage_times(self) -> dict:
+        t = StageTimes()
+        self._check(lib().vox_get_stage_times(self.h, C.byref(t)))
+        return t.as_dict()
+
+
+__all__ = ["Context", "Grid", "DeviceBuffer", "VoxError", "lib", "device_count",
+           "NODE_SHORT", "NODE_LONG", "NODE_PARTIAL", "NODE_SHORT_FCC", "NODE_LONG_FCC",
+           "PLAN_MAX_DIMENSION", "PLAN_CUBE_LENGTH", "NODE_SIZE", "EXPORTED"]

@@ -1,0 +1,836 @@
+// vox_api.hip -- C ABI of libvoxelizer_amd.so: context, device arena,
+// planning and the stage pipelines (host side, HIP runtime).
+//
+// Replaces the reference's host orchestration for one device
+// (src/host_side.cpp:275-397 performVoxelization, 901-1244 voxelizeWorker,
+// 1262-1698 fccWorker) and its launcher layer (src/voxelizer.cu:39-806).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/voxelizer_amd.h"
+#include "vox_kernels.h"
+
+using namespace vk;
+
+namespace {
+
+struct Buf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct Span {
+    uint32_t min, max;
+};
+
+// splitResolutionByNrOfParts for one axis (src/host_side.cpp:3003-3064).
+std::vector<Span> split_axis(uint32_t len, uint32_t parts) {
+    std::vector<Span> out(parts);
+    uint32_t prev = len;
+    for (uint32_t i = parts; i > 0; --i) {
+        Span s;
+        s.max = prev;
+        s.min = s.max - s.max / i;
+        prev = s.min;
+        out[i - 1] = s;
+    }
+    return out;
+}
+std::vector<Span> split_axis_max(uint32_t len, uint32_t maxdim) {
+    return split_axis(len, (len + maxdim - 1) / maxdim);
+}
+
+const uint8_t kOrientPairs[27][2] = {
+    {21, 1},  {22, 2},  {25, 3},  {26, 4},  {37, 5},  {38, 6},  {41, 7},
+    {42, 8},  {23, 9},  {27, 10}, {29, 11}, {30, 12}, {39, 13}, {43, 14},
+    {45, 15}, {46, 16}, {53, 17}, {54, 18}, {57, 19}, {58, 20}, {31, 21},
+    {59, 22}, {55, 23}, {62, 24}, {61, 25}, {47, 26}, {63, 27}};
+
+}  // namespace
+
+struct vox_ctx {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<std::string, Buf> bufs;
+    // mesh
+    const float *V = nullptr;
+    const uint32_t *I = nullptr;
+    uint32_t nv = 0, nt = 0;
+    bool own_mesh = false;
+    const uint8_t *mats = nullptr;
+    bool own_mats = false;
+    uint32_t n_unique = 0;
+    // host staging for small read-backs
+    uint32_t *pinned = nullptr;
+    // timing
+    bool timing = false;
+    hipEvent_t ev[8] = {};
+    vox_stage_times times = {};
+    uint32_t mat_cap_hint = 1u << 20;
+};
+
+#define VOX_HIP(expr)                                                               \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            ctx->err = std::string(hipGetErrorString(e_)) + " @ " + #expr;          \
+            return e_ == hipErrorOutOfMemory ? VOX_ERR_OOM : VOX_ERR_HIP;           \
+        }                                                                           \
+    } while (0)
+
+#define VOX_TRY(expr)                 \
+    do {                              \
+        vox_status s_ = (expr);       \
+        if (s_ != VOX_OK) return s_;  \
+    } while (0)
+
+namespace {
+
+vox_status fail(vox_ctx *ctx, vox_status s, const std::string &msg) {
+    ctx->err = msg;
+    return s;
+}
+
+template <class T>
+vox_status buf(vox_ctx *ctx, const char *name, size_t count, T **out) {
+    Buf &b = ctx->bufs[name];
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (bytes > b.cap) {
+        if (b.p) {
+            VOX_HIP(hipStreamSynchronize(ctx->stream));
+            VOX_HIP(hipFree(b.p));
+            b.p = nullptr;
+            b.cap = 0;
+        }
+        size_t want = bytes + bytes / 4;
+        want = (want + 255) & ~(size_t)255;
+        VOX_HIP(hipMalloc(&b.p, want));
+        b.cap = want;
+    }
+    *out = (T *)b.p;
+    return VOX_OK;
+}
+
+inline uint32_t grid_for(size_t n, uint32_t threads, uint32_t cap = 16384) {
+    size_t g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    return (uint32_t)std::min<size_t>(g, cap);
+}
+
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Clears with a kernel (graph-capture safe) rather than hipMemsetAsync.
+vox_status fill(vox_ctx *ctx, uint32_t *p, size_t n, uint32_t v) {
+    if (n == 0) return VOX_OK;
+    k_fill_u32<<<grid_for(n, 256, 4096), 256, 0, ctx->stream>>>(p, n, v);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// Exclusive scan of in[0,n) into out[0,n]; out[n] is the total.
+vox_status scan(vox_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t n, int depth = 0) {
+    const uint32_t nblocks = n / kScanTile + 1;
+    char name[32];
+    std::snprintf(name, sizeof(name), "scan_part%d", depth);
+    uint32_t *part;
+    VOX_TRY(buf(ctx, name, nblocks + 1, &part));
+    k_scan_blocks<<<nblocks, kScanThreads, 0, ctx->stream>>>(in, out, n, part);
+    VOX_HIP(hipGetLastError());
+    if (nblocks > 1) {
+        std::snprintf(name, sizeof(name), "scan_psum%d", depth);
+        uint32_t *psum;
+        VOX_TRY(buf(ctx, name, nblocks + 1, &psum));
+        VOX_TRY(scan(ctx, part, psum, nblocks, depth + 1));
+        k_scan_add<<<nblocks, kScanThreads, 0, ctx->stream>>>(out, n, psum);
+        VOX_HIP(hipGetLastError());
+    }
+    return VOX_OK;
+}
+
+vox_status read_u32(vox_ctx *ctx, const uint32_t *dev, size_t n, uint32_t *host) {
+    VOX_HIP(hipMemcpyAsync(ctx->pinned, dev, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    VOX_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(host, ctx->pinned, n * 4);
+    return VOX_OK;
+}
+
+void record(vox_ctx *ctx, int i) {
+    if (ctx->timing) (void)hipEventRecord(ctx->ev[i], ctx->stream);
+}
+
+// ---------------------------------------------------------------------------
+// Solid stage for one grid (possibly FCC-shifted) over its y/z subspaces.
+// ---------------------------------------------------------------------------
+vox_status solid_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3], uint32_t splitY,
+                      uint32_t splitZ, uint32_t *bits, const char *tag) {
+    const uint32_t X = g.alloc[0], Yext = g.res[1], Zext = g.res[2];
+    const uint32_t W = X / 32;
+    if (W == 0 || (size_t)16 * W * 4 > 65536)
+        return fail(ctx, VOX_ERR_UNSUPPORTED, "x resolution beyond the fill kernel's LDS row buffer");
+    if (g.x_splits == 0 || X / g.x_splits > 1024)
+        return fail(ctx, VOX_ERR_UNSUPPORTED,
+                    "voxSplitRes.x gives x splits wider than 1024 voxels, where the reference's "
+                    "word bitmask is undefined");
+    const uint32_t rowsY = g.alloc[1], rowsZ = g.alloc[2];
+    const uint32_t tilesY = (Yext + 3) / 4, tilesZ = (Zext + 3) / 4;
+    const uint32_t nTiles = tilesY * tilesZ;
+    const uint32_t nt = ctx->nt;
+    (void)tag;
+
+    uint32_t *cand_count, *cand_off, *tile_hist, *tile_off, *tile_cursor, *list, *counters;
+    uint2 *pairs;
+    VOX_TRY(buf(ctx, "s_cand_count", nt, &cand_count));
+    VOX_TRY(buf(ctx, "s_cand_off", nt + 1, &cand_off));
+    VOX_TRY(buf(ctx, "s_tile_hist", nTiles, &tile_hist));
+    VOX_TRY(buf(ctx, "s_tile_off", nTiles + 1, &tile_off));
+    VOX_TRY(buf(ctx, "s_tile_cursor", nTiles, &tile_cursor));
+    VOX_TRY(buf(ctx, "s_counters", 4, &counters));
+
+    std::vector<Span> ys = split_axis_max(Yext, splitY), zs = split_axis_max(Zext, splitZ);
+    for (const Span &zsp : zs)
+        for (const Span &ysp : ys) {
+            SolidParams p;
+            p.V = ctx->V;
+            p.I = ctx->I;
+            p.nt = nt;
+            p.emx = extMin[0];
+            p.emy = extMin[1];
+            p.emz = extMin[2];
+            p.d = g.voxel_length;
+            p.smin[0] = ysp.min;
+            p.smin[1] = zsp.min;
+            p.smax[0] = ysp.max;
+            p.smax[1] = zsp.max;
+            p.tilesY = tilesY;
+            p.X = X;
+            p.rowsY = rowsY;
+            if (nt > 0) {
+                k_solid_setup<<<grid_for(nt, 256), 256, 0, ctx->stream>>>(p, cand_count);
+                VOX_HIP(hipGetLastError());
+            }
+            VOX_TRY(scan(ctx, cand_count, cand_off, nt));
+            uint32_t total = 0;
+            VOX_TRY(read_u32(ctx, cand_off + nt, 1, &total));
+            VOX_TRY(buf(ctx, "s_pairs", total, &pairs));
+            VOX_TRY(buf(ctx, "s_list", total, &list));
+            VOX_TRY(fill(ctx, counters, 4, 0u));
+            VOX_TRY(fill(ctx, tile_hist, nTiles, 0u));
+            VOX_TRY(fill(ctx, tile_cursor, nTiles, 0u));
+            if (total > 0) {
+                k_solid_candidates<<<grid_for(total, 256), 256, 0, ctx->stream>>>(
+                    p, cand_off, cand_off + nt, pairs, counters, total, tile_hist);
+                VOX_HIP(hipGetLastError());
+            }
+            VOX_TRY(scan(ctx, tile_hist, tile_off, nTiles));
+            if (total > 0) {
+                k_solid_scatter<<<grid_for(total, 256), 256, 0, ctx->stream>>>(
+                    pairs, counters, tile_off, tile_cursor, list);
+                VOX_HIP(hipGetLastError());
+            }
+            const uint32_t ty0 = ysp.min / 4, ty1 = (ysp.max - 1) / 4;
+            const uint32_t tz0 = zsp.min / 4, tz1 = (zsp.max - 1) / 4;
+            const uint32_t nty = ty1 - ty0 + 1, ntz = tz1 - tz0 + 1;
+            if (ysp.max > ysp.min && zsp.max > zsp.min) {
+                k_solid_fill<<<nty * ntz, kWave, 16 * W * 4, ctx->stream>>>(
+                    p, tile_off, tile_hist, list, ty0, nty, tz0, bits);
+                VOX_HIP(hipGetLastError());
+            }
+            if (ctx->timing) {
+                uint32_t q = 0;
+                VOX_TRY(read_u32(ctx, counters, 1, &q));
+                ctx->times.solid_pairs += q;
+            }
+        }
+    k_zero_pad_rows<<<grid_for((size_t)(2 * rowsZ + 2 * rowsY) * W, 256, 4096), 256, 0,
+                      ctx->stream>>>(bits, W, rowsY, rowsZ);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Surface stage for one grid.
+// ---------------------------------------------------------------------------
+struct SurfOut {
+    uint32_t *surf;
+    uint32_t *brick_mat_off;
+    uint8_t *mat_list;
+    uint32_t nb[3];
+};
+
+vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
+                        const uint32_t matSplit[3], const int bdim[3], uint32_t *bits,
+                        const char *tag, SurfOut &out) {
+    const uint32_t nt = ctx->nt;
+    SurfParams p;
+    p.V = ctx->V;
+    p.I = ctx->I;
+    p.nt = nt;
+    p.m = F3{(float)extMin[0], (float)extMin[1], (float)extMin[2]};
+    p.d = (float)g.voxel_length;
+    for (int a = 0; a < 3; a++) {
+        p.res[a] = (int32_t)g.res[a];
+        p.alloc[a] = g.alloc[a];
+        p.bdim[a] = bdim[a];
+        p.nb[a] = (g.alloc[a] + bdim[a] - 1) / bdim[a];
+        out.nb[a] = p.nb[a];
+    }
+    const uint32_t nbt = p.nb[0] * p.nb[1] * p.nb[2];
+    std::string t(tag);
+    uint32_t *tri_info, *cand, *cand_off, *brick_hist, *brick_off, *brick_cursor, *blist, *stats,
+        *mat_cursor;
+    SplitTable *dsplit;
+    VOX_TRY(buf(ctx, (t + "_tri_info").c_str(), nt, &tri_info));
+    VOX_TRY(buf(ctx, "f_cand", nt, &cand));
+    VOX_TRY(buf(ctx, "f_cand_off", nt + 1, &cand_off));
+    VOX_TRY(buf(ctx, "f_brick_hist", nbt, &brick_hist));
+    VOX_TRY(buf(ctx, "f_brick_off", nbt + 1, &brick_off));
+    VOX_TRY(buf(ctx, "f_brick_cursor", nbt, &brick_cursor));
+    VOX_TRY(buf(ctx, "f_stats", 4, &stats));
+    VOX_TRY(buf(ctx, "f_mat_cursor", 1, &mat_cursor));
+    VOX_TRY(buf(ctx, (t + "_split").c_str(), 1, &dsplit));
+    VOX_TRY(buf(ctx, (t + "_surf").c_str(), (size_t)g.alloc[0] / 32 * g.alloc[1] * g.alloc[2],
+                &out.surf));
+    VOX_TRY(buf(ctx, (t + "_bmo").c_str(), nbt, &out.brick_mat_off));
+
+    // matSplitRes subspaces over extPartition (src/host_side.cpp:1107-1111).
+    SplitTable st;
+    std::memset(&st, 0, sizeof(st));
+    for (int a = 0; a < 3; a++) {
+        std::vector<Span> sp = split_axis_max(g.res[a], matSplit[a]);
+        if (sp.empty() || sp.size() > (size_t)kMaxSplits)
+            return fail(ctx, VOX_ERR_UNSUPPORTED, "matSplitRes gives too many subspaces");
+        st.n[a] = (uint32_t)sp.size();
+        for (size_t k = 0; k < sp.size(); k++) {
+            st.lo[a][k] = sp[k].min;
+            st.hi[a][k] = sp[k].max;
+        }
+    }
+    VOX_HIP(hipMemcpyAsync(dsplit, &st, sizeof(st), hipMemcpyHostToDevice, ctx->stream));
+
+    if (nt > 0) {
+        k_surf_setup<<<grid_for(nt, 256), 256, 0, ctx->stream>>>(p, tri_info, cand);
+        VOX_HIP(hipGetLastError());
+    }
+    VOX_TRY(scan(ctx, cand, cand_off, nt));
+    uint32_t total = 0;
+    VOX_TRY(read_u32(ctx, cand_off + nt, 1, &total));
+    VOX_TRY(buf(ctx, "f_blist", total, &blist));
+    VOX_TRY(fill(ctx, brick_hist, nbt, 0u));
+    VOX_TRY(fill(ctx, brick_cursor, nbt, 0u));
+    if (total > 0) {
+        k_surf_bin<<<grid_for(total, 256), 256, 0, ctx->stream>>>(
+            p, cand_off, cand_off + nt, tri_info, brick_hist, brick_off, brick_cursor, blist, 0);
+        VOX_HIP(hipGetLastError());
+    }
+    VOX_TRY(scan(ctx, brick_hist, brick_off, nbt));
+    if (total > 0) {
+        k_surf_bin<<<grid_for(total, 256), 256, 0, ctx->stream>>>(
+            p, cand_off, cand_off + nt, tri_info, brick_hist, brick_off, brick_cursor, blist, 1);
+        VOX_HIP(hipGetLastError());
+    }
+    if (ctx->timing) ctx->times.surface_pairs += total;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        uint32_t cap = ctx->mat_cap_hint;
+        VOX_TRY(buf(ctx, (t + "_mats").c_str(), cap, &out.mat_list));
+        VOX_TRY(fill(ctx, stats, 4, 0u));
+        VOX_TRY(fill(ctx, mat_cursor, 1, 0u));
+        k_surf_brick<<<nbt, 256, 0, ctx->stream>>>(p, dsplit, tri_info, brick_off, brick_hist,
+                                                   blist, ctx->mats, bits, out.surf,
+                                                   out.brick_mat_off, out.mat_list, mat_cursor, cap,
+                                                   stats);
+        VOX_HIP(hipGetLastError());
+        uint32_t s[2];
+        VOX_TRY(read_u32(ctx, stats, 2, s));
+        if (ctx->timing) ctx->times.surface_voxels += s[1];
+        if (s[0] == 0u) return VOX_OK;
+        ctx->mat_cap_hint = s[1] + s[1] / 4 + 1024;
+    }
+    return fail(ctx, VOX_ERR_OOM, "surface material list overflow");
+}
+
+// Erosion to the fixed point, per alloc y/z subspace in the reference's order.
+vox_status erode(vox_ctx *ctx, const GridDims &gd, uint32_t splitY, uint32_t splitZ,
+                 uint32_t *&cur, uint32_t *other) {
+    const size_t words = (size_t)gd.W * gd.Y * gd.Z;
+    constexpr uint32_t kBatch = 4;
+    uint32_t *changes;
+    VOX_TRY(buf(ctx, "e_changes", kBatch, &changes));
+    std::vector<Span> ys = split_axis_max(gd.Y, splitY), zs = split_axis_max(gd.Z, splitZ);
+    uint32_t passes = 0;
+    for (const Span &zsp : zs)
+        for (const Span &ysp : ys) {
+            for (int guard = 0; guard < 1000000; guard++) {
+                VOX_TRY(fill(ctx, changes, kBatch, 0u));
+                uint32_t *bufs[2] = {cur, other};
+                for (uint32_t k = 0; k < kBatch; k++) {
+                    k_erode<<<grid_for(words, 256, 32768), 256, 0, ctx->stream>>>(
+                        gd, bufs[k & 1], bufs[(k + 1) & 1], ysp.min, ysp.max, zsp.min, zsp.max,
+                        changes, k);
+                    VOX_HIP(hipGetLastError());
+                }
+                uint32_t h[kBatch];
+                VOX_TRY(read_u32(ctx, changes, kBatch, h));
+                uint32_t k0 = kBatch;
+                for (uint32_t k = 0; k < kBatch; k++)
+                    if (h[k] == 0u) {
+                        k0 = k;
+                        break;
+                    }
+                if (k0 < kBatch) {
+                    passes += k0 + 1;
+                    // Pass k0 changed nothing: its input equals its output.
+                    cur = bufs[k0 & 1];
+                    other = bufs[(k0 + 1) & 1];
+                    break;
+                }
+                passes += kBatch;
+                cur = bufs[kBatch & 1];
+                other = bufs[(kBatch + 1) & 1];
+            }
+        }
+    ctx->times.erosion_passes += passes;
+    return VOX_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int vox_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char *vox_build_info(void) { return "voxelizer_amd gfx950 (wave64, HIP)"; }
+
+vox_status vox_create(int device, vox_ctx **out) {
+    if (!out) return VOX_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return VOX_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return VOX_ERR_NO_DEVICE;
+    vox_ctx *ctx = new vox_ctx();
+    ctx->dev = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
+        hipHostMalloc((void **)&ctx->pinned, 4096, hipHostMallocDefault) != hipSuccess) {
+        delete ctx;
+        return VOX_ERR_HIP;
+    }
+    for (auto &e : ctx->ev) (void)hipEventCreate(&e);
+    uint8_t lut[64] = {0};
+    for (auto &pr : kOrientPairs) lut[pr[0]] = pr[1];
+    if (set_orient_table(lut) != hipSuccess) {
+        vox_destroy(ctx);
+        return VOX_ERR_HIP;
+    }
+    *out = ctx;
+    return VOX_OK;
+}
+
+void vox_destroy(vox_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->dev);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->bufs)
+        if (kv.second.p) (void)hipFree(kv.second.p);
+    if (ctx->own_mesh) {
+        (void)hipFree((void *)ctx->V);
+        (void)hipFree((void *)ctx->I);
+    }
+    if (ctx->own_mats) (void)hipFree((void *)ctx->mats);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *vox_last_error(const vox_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+vox_status vox_sync(vox_ctx *ctx) {
+    if (!ctx) return VOX_ERR_INVALID;
+    VOX_HIP(hipStreamSynchronize(ctx->stream));
+    return VOX_OK;
+}
+
+vox_status vox_alloc(vox_ctx *ctx, size_t bytes, void **dev_out) {
+    if (!ctx || !dev_out) return VOX_ERR_INVALID;
+    VOX_HIP(hipSetDevice(ctx->dev));
+    VOX_HIP(hipMalloc(dev_out, std::max<size_t>(bytes, 1)));
+    return VOX_OK;
+}
+vox_status vox_free(vox_ctx *ctx, void *dev) {
+    if (!ctx) return VOX_ERR_INVALID;
+    VOX_HIP(hipStreamSynchronize(ctx->stream));
+    VOX_HIP(hipFree(dev));
+    return VOX_OK;
+}
+vox_status vox_memcpy_h2d(vox_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return VOX_ERR_INVALID;
+    VOX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    VOX_HIP(hipStreamSynchronize(ctx->stream));
+    return VOX_OK;
+}
+vox_status vox_memcpy_d2h(vox_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return VOX_ERR_INVALID;
+    VOX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    VOX_HIP(hipStreamSynchronize(ctx->stream));
+    return VOX_OK;
+}
+
+static void drop_mesh(vox_ctx *ctx) {
+    if (ctx->own_mesh) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree((void *)ctx->V);
+        (void)hipFree((void *)ctx->I);
+    }
+    ctx->V = nullptr;
+    ctx->I = nullptr;
+    ctx->own_mesh = false;
+}
+
+vox_status vox_set_mesh(vox_ctx *ctx, const float *vertices, uint32_t nv, const uint32_t *indices,
+                        uint32_t nt) {
+    if (!ctx || (!vertices && nv) || (!indices && nt)) return VOX_ERR_INVALID;
+    for (uint64_t i = 0; i < 3ull * nt; i++)
+        if (indices[i] >= nv) return fail(ctx, VOX_ERR_INVALID, "vertex index out of range");
+    drop_mesh(ctx);
+    float *v;
+    uint32_t *ix;
+    VOX_HIP(hipMalloc((void **)&v, std::max<size_t>(12ull * nv, 4)));
+    VOX_HIP(hipMalloc((void **)&ix, std::max<size_t>(12ull * nt, 4)));
+    if (nv) VOX_HIP(hipMemcpyAsync(v, vertices, 12ull * nv, hipMemcpyHostToDevice, ctx->stream));
+    if (nt) VOX_HIP(hipMemcpyAsync(ix, indices, 12ull * nt, hipMemcpyHostToDevice, ctx->stream));
+    VOX_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->V = v;
+    ctx->I = ix;
+    ctx->nv = nv;
+    ctx->nt = nt;
+    ctx->own_mesh = true;
+    return VOX_OK;
+}
+
+vox_status vox_set_mesh_device(vox_ctx *ctx, const float *vertices_dev, uint32_t nv,
+                               const uint32_t *indices_dev, uint32_t nt) {
+    if (!ctx || (!vertices_dev && nv) || (!indices_dev && nt)) return VOX_ERR_INVALID;
+    drop_mesh(ctx);
+    ctx->V = vertices_dev;
+    ctx->I = indices_dev;
+    ctx->nv = nv;
+    ctx->nt = nt;
+    return VOX_OK;
+}
+
+vox_status vox_set_materials(vox_ctx *ctx, const uint8_t *materials, uint32_t n_unique) {
+    if (!ctx) return VOX_ERR_INVALID;
+    if (ctx->own_mats) {
+        VOX_HIP(hipStreamSynchronize(ctx->stream));
+        VOX_HIP(hipFree((void *)ctx->mats));
+    }
+    ctx->mats = nullptr;
+    ctx->own_mats = false;
+    ctx->n_unique = n_unique;
+    if (!materials) return VOX_OK;
+    uint8_t *m;
+    VOX_HIP(hipMalloc((void **)&m, std::max<size_t>(ctx->nt, 1)));
+    if (ctx->nt)
+        VOX_HIP(hipMemcpyAsync(m, materials, ctx->nt, hipMemcpyHostToDevice, ctx->stream));
+    VOX_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->mats = m;
+    ctx->own_mats = true;
+    return VOX_OK;
+}
+
+vox_status vox_set_materials_device(vox_ctx *ctx, const uint8_t *materials_dev,
+                                    uint32_t n_unique) {
+    if (!ctx) return VOX_ERR_INVALID;
+    if (ctx->own_mats) {
+        VOX_HIP(hipStreamSynchronize(ctx->stream));
+        VOX_HIP(hipFree((void *)ctx->mats));
+    }
+    ctx->mats = materials_dev;
+    ctx->own_mats = false;
+    ctx->n_unique = n_unique;
+    return VOX_OK;
+}
+
+// Bounding box on the device; dimensions on the host in FP64 exactly as
+// determineDimensions / adjustResolution compute them.
+vox_status vox_plan_grid(vox_ctx *ctx, vox_plan_mode mode, double param, uint32_t vsx,
+                         vox_grid *g) {
+    if (!ctx || !g || vsx == 0) return VOX_ERR_INVALID;
+    if (ctx->nv == 0) return fail(ctx, VOX_ERR_INVALID, "no vertices");
+    if (mode == VOX_PLAN_MAX_DIMENSION && (param < 2.0 || param > 4294967295.0))
+        return fail(ctx, VOX_ERR_INVALID, "maxDimension must be >= 2");
+    if (mode == VOX_PLAN_CUBE_LENGTH && !(param > 0.0))
+        return fail(ctx, VOX_ERR_INVALID, "cubeLength must be > 0");
+    VOX_HIP(hipSetDevice(ctx->dev));
+    uint32_t *bb;
+    VOX_TRY(buf(ctx, "p_bbox", 6, &bb));
+    VOX_TRY(fill(ctx, bb, 3, UINT32_MAX));
+    VOX_TRY(fill(ctx, bb + 3, 3, 0u));
+    k_bbox<<<grid_for(ctx->nv, 256, 1024), 256, 0, ctx->stream>>>(ctx->V, ctx->nv, bb);
+    VOX_HIP(hipGetLastError());
+    uint32_t h[6];
+    VOX_TRY(read_u32(ctx, bb, 6, h));
+    double mn[3], mx[3];
+    for (int c = 0; c < 3; c++) {
+        uint32_t a = h[c], b = h[3 + c];
+        uint32_t ua = (a & 0x80000000u) ? (a & 0x7FFFFFFFu) : ~a;
+        uint32_t ub = (b & 0x80000000u) ? (b & 0x7FFFFFFFu) : ~b;
+        float fa, fb;
+        std::memcpy(&fa, &ua, 4);
+        std::memcpy(&fb, &ub, 4);
+        mn[c] = fa;
+        mx[c] = fb;
+    }
+    auto ceil_u = [](double v) -> uint32_t {
+        double c = std::ceil(v);
+        if (c != c || c <= -1.0) return 0u;
+        if (c >= 4294967296.0) return UINT32_MAX;
+        return (uint32_t)c;
+    };
+    double dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+    uint32_t rx, ry, rz;
+    double d;
+    if (mode == VOX_PLAN_MAX_DIMENSION) {
+        uint32_t R = (uint32_t)param;
+        if (dx > dy) {
+            if (dx > dz) {
+                d = dx / double(R - 1);
+                ry = ceil_u(dy / d) + 1;
+                rz = ceil_u(dz / d) + 1;
+                rx = R + 2;
+            } else {
+                d = dz / double(R - 1);
+                rz = R;
+                rx = ceil_u(dx / d) + 1 + 2;
+                ry = ceil_u(dy / d) + 1;
+            }
+        } else if (dy > dz) {
+            d = dy / double(R - 1);
+            ry = R;
+            rx = ceil_u(dx / d) + 1 + 2;
+            rz = ceil_u(dz / d) + 1;
+        } else {
+            d = dz / double(R - 1);
+            rz = R;
+            rx = ceil_u(dx / d) + 1 + 2;
+            ry = ceil_u(dy / d) + 1;
+        }
+    } else {
+        d = param;
+        rx = ceil_u(dx / d) + 1 + 2;
+        ry = ceil_u(dy / d) + 1;
+        rz = ceil_u(dz / d) + 1;
+    }
+    uint32_t xs = (rx + vsx - 1) / vsx;
+    uint32_t xmod = 32u * xs;
+    if (rx % xmod != 0) rx += xmod - rx % xmod;
+    for (int c = 0; c < 3; c++) {
+        g->min_vertex[c] = mn[c];
+        g->max_vertex[c] = mx[c];
+    }
+    g->voxel_length = d;
+    g->res[0] = rx;
+    g->res[1] = ry;
+    g->res[2] = rz;
+    g->alloc[0] = rx;
+    g->alloc[1] = ry + 2;
+    g->alloc[2] = rz + 2;
+    g->x_splits = xs;
+    g->pad_ = 0;
+    return VOX_OK;
+}
+
+void vox_node_dims(const vox_grid *g, vox_node_kind kind, uint32_t dims[3]) {
+    const bool fcc = kind == VOX_NODE_SHORT_FCC || kind == VOX_NODE_LONG_FCC;
+    dims[0] = g->alloc[0] * (fcc ? 2 : 1);
+    dims[1] = g->alloc[1];
+    dims[2] = g->alloc[2] * (fcc ? 2 : 1);
+}
+size_t vox_node_count(const vox_grid *g, vox_node_kind kind) {
+    uint32_t d[3];
+    vox_node_dims(g, kind, d);
+    return (size_t)d[0] * d[1] * d[2];
+}
+size_t vox_node_size(vox_node_kind kind) {
+    switch (kind) {
+        case VOX_NODE_SHORT: return 1;
+        case VOX_NODE_LONG: return 2;
+        case VOX_NODE_PARTIAL: return 8;
+        case VOX_NODE_SHORT_FCC: return 2;
+        case VOX_NODE_LONG_FCC: return 4;
+    }
+    return 0;
+}
+
+static bool grid_ok(const vox_grid *g) {
+    return g && g->alloc[0] % 32 == 0 && g->alloc[0] > 0 && g->alloc[1] >= 3 && g->alloc[2] >= 3 &&
+           g->alloc[1] == g->res[1] + 2 && g->alloc[2] == g->res[2] + 2 &&
+           g->alloc[0] == g->res[0] && g->voxel_length > 0.0 && g->x_splits > 0 &&
+           (double)g->alloc[0] * g->alloc[1] * g->alloc[2] < 4294967296.0;
+}
+
+vox_status vox_solid_bits(vox_ctx *ctx, const vox_grid *g, const double shift[3],
+                          const uint32_t split_yz[2], uint32_t *bits_dev) {
+    if (!ctx || !grid_ok(g) || !bits_dev) return VOX_ERR_INVALID;
+    if (!ctx->V && ctx->nv) return fail(ctx, VOX_ERR_INVALID, "no mesh");
+    VOX_HIP(hipSetDevice(ctx->dev));
+    ctx->times = vox_stage_times{};
+    double em[3];
+    for (int a = 0; a < 3; a++) em[a] = g->min_vertex[a] + (shift ? shift[a] : 0.0);
+    const uint32_t sy = split_yz ? split_yz[0] : 512u, sz = split_yz ? split_yz[1] : 512u;
+    if (sy == 0 || sz == 0) return VOX_ERR_INVALID;
+    record(ctx, 0);
+    VOX_TRY(solid_grid(ctx, *g, em, sy, sz, bits_dev, "g1"));
+    record(ctx, 1);
+    if (ctx->timing) {
+        VOX_HIP(hipEventSynchronize(ctx->ev[1]));
+        (void)hipEventElapsedTime(&ctx->times.solid_fill_ms, ctx->ev[0], ctx->ev[1]);
+        ctx->times.total_ms = ctx->times.solid_fill_ms;
+    }
+    return VOX_OK;
+}
+
+vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *g, vox_node_kind kind,
+                              int use_materials, const uint32_t vox_split[3],
+                              const uint32_t mat_split[3], void *nodes_dev) {
+    if (!ctx || !grid_ok(g) || !nodes_dev) return VOX_ERR_INVALID;
+    if (kind == VOX_NODE_PARTIAL)
+        return fail(ctx, VOX_ERR_UNSUPPORTED, "PartialNode (volume ratio) is not built");
+    const bool fcc = kind == VOX_NODE_SHORT_FCC || kind == VOX_NODE_LONG_FCC;
+    if (kind != VOX_NODE_SHORT && kind != VOX_NODE_LONG && !fcc) return VOX_ERR_INVALID;
+    if (use_materials && !ctx->mats)
+        return fail(ctx, VOX_ERR_INVALID, "materials requested but none set");
+    VOX_HIP(hipSetDevice(ctx->dev));
+    const uint32_t vs[3] = {vox_split ? vox_split[0] : 1024u, vox_split ? vox_split[1] : 512u,
+                            vox_split ? vox_split[2] : 512u};
+    const uint32_t ms[3] = {mat_split ? mat_split[0] : 512u, mat_split ? mat_split[1] : 512u,
+                            mat_split ? mat_split[2] : 512u};
+    if (!vs[1] || !vs[2] || !ms[0] || !ms[1] || !ms[2]) return VOX_ERR_INVALID;
+    ctx->times = vox_stage_times{};
+    const size_t words = (size_t)g->alloc[0] / 32 * g->alloc[1] * g->alloc[2];
+    const double base[3] = {g->min_vertex[0], g->min_vertex[1], g->min_vertex[2]};
+    GridDims gd{g->alloc[0], g->alloc[1], g->alloc[2], g->alloc[0] / 32};
+    record(ctx, 0);
+    if (!fcc) {
+        uint32_t *bits, *tmp;
+        VOX_TRY(buf(ctx, "n_bits", words, &bits));
+        VOX_TRY(buf(ctx, "n_bits_tmp", words, &tmp));
+        VOX_TRY(solid_grid(ctx, *g, base, vs[1], vs[2], bits, "g1"));
+        record(ctx, 1);
+        SurfOut so{};
+        if (use_materials) {
+            const int bdim[3] = {kCartBX, kCartBY, kCartBZ};
+            VOX_TRY(surface_grid(ctx, *g, base, ms, bdim, bits, "g1", so));
+        }
+        record(ctx, 2);
+        // Several erosion subspaces: keep the pre-erosion grid for the ids
+        // of nodes next to a later subspace (see k_nodes_cart).
+        std::vector<Span> eys = split_axis_max(g->alloc[1], vs[1]);
+        std::vector<Span> ezs = split_axis_max(g->alloc[2], vs[2]);
+        uint32_t *pre = nullptr;
+        uint8_t *ystart = nullptr, *zstart = nullptr;
+        if (eys.size() > 1 || ezs.size() > 1) {
+            VOX_TRY(buf(ctx, "n_bits_pre", words, &pre));
+            VOX_HIP(hipMemcpyAsync(pre, bits, words * 4, hipMemcpyDeviceToDevice, ctx->stream));
+            std::vector<uint8_t> fy(g->alloc[1], 0), fz(g->alloc[2], 0);
+            for (size_t k = 1; k < eys.size(); k++) fy[eys[k].min] = 1;
+            for (size_t k = 1; k < ezs.size(); k++) fz[ezs[k].min] = 1;
+            VOX_TRY(buf(ctx, "n_ystart", fy.size(), &ystart));
+            VOX_TRY(buf(ctx, "n_zstart", fz.size(), &zstart));
+            VOX_HIP(hipMemcpyAsync(ystart, fy.data(), fy.size(), hipMemcpyHostToDevice, ctx->stream));
+            VOX_HIP(hipMemcpyAsync(zstart, fz.data(), fz.size(), hipMemcpyHostToDevice, ctx->stream));
+            VOX_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        uint32_t *cur = bits;
+        VOX_TRY(erode(ctx, gd, vs[1], vs[2], cur, tmp));
+        record(ctx, 3);
+        const uint32_t nbx = (g->alloc[0] + kCartBX - 1) / kCartBX;
+        const uint32_t nby = (g->alloc[1] + kCartBY - 1) / kCartBY;
+        const uint32_t nbz = (g->alloc[2] + kCartBZ - 1) / kCartBZ;
+        MatSource msrc{so.surf, so.brick_mat_off, so.mat_list};
+        k_nodes_cart<<<nbx * nby * nbz, 128, 0, ctx->stream>>>(
+            gd, cur, pre, ystart, zstart, msrc, use_materials ? 1 : 0,
+            kind == VOX_NODE_SHORT ? 0 : 1, nodes_dev, nbx, nby);
+        VOX_HIP(hipGetLastError());
+        record(ctx, 4);
+    } else {
+        // fccWorker (src/host_side.cpp:1262-1698): four sub-grids shifted by
+        // (0,0,0), (h,h,0), (0,h,h), (h,0,h), h = d/2 (1436-1463).
+        const double h = g->voxel_length / 2.0;
+        const double shifts[4][3] = {{0.0, 0.0, 0.0}, {h, h, 0.0}, {0.0, h, h}, {h, 0.0, h}};
+        uint32_t *gb[4];
+        SurfOut so[4] = {};
+        const char *tags[4] = {"fg1", "fg2", "fg3", "fg4"};
+        char name[32];
+        for (int k = 0; k < 4; k++) {
+            std::snprintf(name, sizeof(name), "n_fcc_bits%d", k);
+            VOX_TRY(buf(ctx, name, words, &gb[k]));
+            const double em[3] = {base[0] + shifts[k][0], base[1] + shifts[k][1],
+                                  base[2] + shifts[k][2]};
+            VOX_TRY(solid_grid(ctx, *g, em, vs[1], vs[2], gb[k], tags[k]));
+        }
+        record(ctx, 1);
+        if (use_materials) {
+            const int bdim[3] = {kFccBX, kFccBY, kFccBZ};
+            for (int k = 0; k < 4; k++) {
+                const double em[3] = {base[0] + shifts[k][0], base[1] + shifts[k][1],
+                                      base[2] + shifts[k][2]};
+                VOX_TRY(surface_grid(ctx, *g, em, ms, bdim, gb[k], tags[k], so[k]));
+            }
+        }
+        record(ctx, 2);
+        record(ctx, 3);
+        const uint32_t nbx = (g->alloc[0] + kFccBX - 1) / kFccBX;
+        const uint32_t nby = (g->alloc[1] + kFccBY - 1) / kFccBY;
+        const uint32_t nbz = (g->alloc[2] + kFccBZ - 1) / kFccBZ;
+        MatSource m[4];
+        for (int k = 0; k < 4; k++) m[k] = MatSource{so[k].surf, so[k].brick_mat_off, so[k].mat_list};
+        k_nodes_fcc<<<nbx * nby * nbz, 128, 0, ctx->stream>>>(
+            gd, gb[0], gb[1], gb[2], gb[3], m[0], m[1], m[2], m[3], use_materials ? 1 : 0,
+            kind == VOX_NODE_SHORT_FCC ? 3 : 4, nodes_dev, nbx, nby);
+        VOX_HIP(hipGetLastError());
+        record(ctx, 4);
+    }
+    if (ctx->timing) {
+        VOX_HIP(hipEventSynchronize(ctx->ev[4]));
+        (void)hipEventElapsedTime(&ctx->times.solid_fill_ms, ctx->ev[0], ctx->ev[1]);
+        (void)hipEventElapsedTime(&ctx->times.surface_ms, ctx->ev[1], ctx->ev[2]);
+        (void)hipEventElapsedTime(&ctx->times.erode_ms, ctx->ev[2], ctx->ev[3]);
+        (void)hipEventElapsedTime(&ctx->times.nodes_ms, ctx->ev[3], ctx->ev[4]);
+        (void)hipEventElapsedTime(&ctx->times.total_ms, ctx->ev[0], ctx->ev[4]);
+    }
+    return VOX_OK;
+}
+
+vox_status vox_enable_timing(vox_ctx *ctx, int enable) {
+    if (!ctx) return VOX_ERR_INVALID;
+    ctx->timing = enable != 0;
+    return VOX_OK;
+}
+
+vox_status vox_get_stage_times(const vox_ctx *ctx, vox_stage_times *out) {
+    if (!ctx || !out) return VOX_ERR_INVALID;
+    *out = ctx->times;
+    return VOX_OK;
+}
+
+}  // extern "C"
