@@ -1095,6 +1095,32 @@ int orc_center_crossing(const float tri[9], const double ext_min[3], double d, u
     return ok ? 1 : 0;
 }
 
+int orc_erode_nodes(const uint8_t *solid, uint32_t X, uint32_t Y, uint32_t Z, uint32_t split_y,
+                    uint32_t split_z, int literal, uint8_t *bid_out) {
+    if (!split_y || !split_z) return 1;
+    const size_t N = (size_t)X * Y * Z;
+    std::vector<NodeRec> nodes(N);
+    for (size_t i = 0; i < N; i++) nodes[i] = {(uint16_t)(solid[i] ? 1 : 0), 0};
+    std::vector<Span> ys = split_axis_max(Y, split_y), zs = split_axis_max(Z, split_z);
+    for (const Span &z : zs)
+        for (const Span &y : ys) {
+            if (literal) erode_inplace(nodes, X, Y, Z, y, z);
+            else erode_parallel(nodes, X, Y, Z, y, z);
+        }
+    for (size_t i = 0; i < N; i++) bid_out[i] = (uint8_t)nodes[i].bid;
+    return 0;
+}
+
+int orc_fcc_boundaries(const uint8_t *solid, uint32_t FX, uint32_t FY, uint32_t FZ,
+                       uint16_t *bid_out) {
+    const size_t N = (size_t)FX * FY * FZ;
+    std::vector<NodeRec> nodes(N);
+    for (size_t i = 0; i < N; i++) nodes[i] = {(uint16_t)(solid[i] ? 1 : 0), 0};
+    fcc_boundaries(nodes, FX, FY, FZ, {0, FY}, {0, FZ});
+    for (size_t i = 0; i < N; i++) bid_out[i] = nodes[i].bid;
+    return 0;
+}
+
 uint32_t orc_tile_overlaps(const float tri[9], const double ext_min[3], double d, uint32_t ext_y,
                            uint32_t ext_z) {
     D3 v[3];

@@ -74,6 +74,16 @@ int orc_center_crossing(const float tri[9], const double ext_min[3], double d,
 uint32_t orc_tile_overlaps(const float tri[9], const double ext_min[3], double d,
                            uint32_t ext_y, uint32_t ext_z);
 
+/* Erosion + boundary ids on a raw Cartesian solid grid (0/1 per node),
+ * per y/z subspace like the reference's host loop; literal = in-place
+ * sequential passes, else snapshot passes (same fixed point). */
+int orc_erode_nodes(const uint8_t *solid, uint32_t X, uint32_t Y, uint32_t Z, uint32_t split_y,
+                    uint32_t split_z, int literal, uint8_t *bid_out);
+
+/* 12-neighbour FCC masks on a raw FCC solid grid (one pass, in place). */
+int orc_fcc_boundaries(const uint8_t *solid, uint32_t FX, uint32_t FY, uint32_t FZ,
+                       uint16_t *bid_out);
+
 #ifdef __cplusplus
 }
 #endif

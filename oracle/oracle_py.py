@@ -32,8 +32,11 @@ class Grid(C.Structure):
 
 
 def build(force: bool = False) -> str:
-    if force or not os.path.exists(LIB_PATH):
-        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    srcs = [os.path.join(HERE, f) for f in ("oracle.cpp", "oracle.h")]
+    stale = not os.path.exists(LIB_PATH) or any(
+        os.path.getmtime(s) > os.path.getmtime(LIB_PATH) for s in srcs if os.path.exists(s))
+    if force or stale:
+        subprocess.run(["make", "-s", "-B", "-C", HERE], check=True)
     return LIB_PATH
 
 
@@ -55,6 +58,9 @@ def lib():
         L.orc_surface_keys.argtypes = [P, P, C.c_uint32, C.POINTER(Grid), P, P, P]
         L.orc_classify.argtypes = [P, P, C.c_uint32, P, C.c_float, P]
         L.orc_center_crossing.argtypes = [P, P, C.c_double, C.c_uint32, C.c_uint32, P]
+        L.orc_erode_nodes.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.c_uint32, C.c_int, P]
+        L.orc_fcc_boundaries.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, P]
         L.orc_tile_overlaps.restype = C.c_uint32
         L.orc_tile_overlaps.argtypes = [P, P, C.c_double, C.c_uint32, C.c_uint32]
         _lib = L
@@ -151,6 +157,25 @@ def tile_overlaps(tri, ext_min, d, ext_y, ext_z) -> int:
     tr = np.ascontiguousarray(tri, np.float32).ravel()
     em = np.array(ext_min, np.float64)
     return int(lib().orc_tile_overlaps(_p(tr), _p(em), float(d), ext_y, ext_z))
+
+
+def erode_nodes(solid_zyx: np.ndarray, split_yz=(512, 512), literal=True) -> np.ndarray:
+    """Boundary ids after erosion of a solid grid indexed [z, y, x]."""
+    s = np.ascontiguousarray(solid_zyx, np.uint8)
+    Z, Y, X = s.shape
+    out = np.zeros_like(s)
+    r = lib().orc_erode_nodes(_p(s), X, Y, Z, split_yz[0], split_yz[1], 1 if literal else 0, _p(out))
+    if r:
+        raise ValueError("orc_erode_nodes failed")
+    return out
+
+
+def fcc_boundaries(solid_zyx: np.ndarray) -> np.ndarray:
+    s = np.ascontiguousarray(solid_zyx, np.uint8)
+    Z, Y, X = s.shape
+    out = np.zeros(s.shape, np.uint16)
+    lib().orc_fcc_boundaries(_p(s), X, Y, Z, _p(out))
+    return out
 
 
 def bits_to_grid(bits: np.ndarray, alloc) -> np.ndarray:

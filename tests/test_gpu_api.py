@@ -1,0 +1,81 @@
+"""Error behaviour of the C ABI on the GPU (status codes mirror the
+reference's exceptions / empty results, src/host_side.cpp:164-243)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import voxelizer_amd as va
+from voxelizer_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def test_partial_node_is_reported_unsupported(gpu_ctx):
+    m = W.icosphere()
+    gpu_ctx.set_mesh(m.vertices, m.indices)
+    gpu_ctx.set_materials(m.materials, 2)
+    g = gpu_ctx.plan_grid(va.PLAN_MAX_DIMENSION, 32)
+    with pytest.raises(va.VoxError) as e:
+        gpu_ctx.voxelize_nodes(g, va.NODE_PARTIAL)
+    assert e.value.status == va.ERR_UNSUPPORTED
+
+
+def test_materials_required_when_requested(gpu_ctx):
+    m = W.icosphere()
+    gpu_ctx.set_mesh(m.vertices, m.indices)
+    gpu_ctx.set_materials(None)
+    g = gpu_ctx.plan_grid(va.PLAN_MAX_DIMENSION, 32)
+    with pytest.raises(va.VoxError) as e:
+        gpu_ctx.voxelize_nodes(g, va.NODE_SHORT, use_materials=True)
+    assert e.value.status == va.ERR_INVALID
+    out = gpu_ctx.voxelize_nodes(g, va.NODE_SHORT, use_materials=False)
+    assert out.size == 34 * 34 * 34 or out.size > 0
+
+
+def test_bad_index_rejected(gpu_ctx):
+    v = np.zeros((3, 3), np.float32)
+    t = np.array([[0, 1, 5]], np.uint32)
+    with pytest.raises(va.VoxError) as e:
+        gpu_ctx.set_mesh(v, t)
+    assert e.value.status == va.ERR_INVALID
+
+
+def test_bad_plan_arguments(gpu_ctx):
+    m = W.icosphere()
+    gpu_ctx.set_mesh(m.vertices, m.indices)
+    for mode, p in [(va.PLAN_MAX_DIMENSION, 1), (va.PLAN_CUBE_LENGTH, 0.0),
+                    (va.PLAN_CUBE_LENGTH, -1.0)]:
+        with pytest.raises(va.VoxError) as e:
+            gpu_ctx.plan_grid(mode, p)
+        assert e.value.status == va.ERR_INVALID
+
+
+def test_wide_x_split_rejected(gpu_ctx):
+    # voxSplitRes.x above 1024 makes the reference's word bitmask undefined.
+    m = W.box((0, 0, 0), (30.0, 1.0, 1.0))
+    gpu_ctx.set_mesh(m.vertices, m.indices)
+    g = gpu_ctx.plan_grid(va.PLAN_CUBE_LENGTH, 0.01, 4096)
+    with pytest.raises(va.VoxError) as e:
+        gpu_ctx.solid_bits(g)
+    assert e.value.status == va.ERR_UNSUPPORTED
+
+
+def test_empty_triangle_list(gpu_ctx, oracle):
+    v = W.icosphere().vertices
+    gpu_ctx.set_mesh(v, np.zeros((0, 3), np.uint32))
+    g = gpu_ctx.plan_grid(va.PLAN_MAX_DIMENSION, 40)
+    out = gpu_ctx.voxelize_nodes(g, va.NODE_LONG, use_materials=False)
+    assert not out.any()
+
+
+def test_stage_times_reported(gpu_ctx):
+    m = W.shoebox_room(seed=5, n_boxes=4, subdiv=2)
+    gpu_ctx.set_mesh(m.vertices, m.indices)
+    gpu_ctx.set_materials(m.materials, 9)
+    g = gpu_ctx.plan_grid(va.PLAN_CUBE_LENGTH, 0.02)
+    gpu_ctx.enable_timing(True)
+    gpu_ctx.voxelize_nodes(g, va.NODE_LONG)
+    t = gpu_ctx.stage_times()
+    gpu_ctx.enable_timing(False)
+    assert t["total_ms"] > 0 and t["nodes_ms"] > 0 and t["erosion_passes"] >= 1

@@ -1,0 +1,193 @@
+"""Solid voxelization of the oracle (src/voxelizer.cu:840-1617,
+src/host_side.cpp:275-397).
+
+Pins, each independent of the oracle's code:
+  * brute-force ray parity: a voxel centre is solid iff an odd number of
+    triangles cross the +x ray ending at it -- evaluated here in float64 with
+    margins, skipping centres whose edge or depth test is within the margin;
+  * XOR additivity: voxelizing two meshes together equals XOR of each alone;
+  * the DBL_MIN "top-left" tie break: a centre on an edge shared by two
+    triangles, or on a vertex shared by a fan, is counted exactly once;
+  * the reference's per-x-split active-segment/bitmask bookkeeping equals one
+    pass over the whole row; y/z subspaces do not change generic results;
+  * closed forms for boxes placed between voxel centres.
+"""
+import numpy as np
+import pytest
+
+from voxelizer_amd import workloads as W
+
+
+def unpack(oracle, bits, g):
+    return oracle.bits_to_grid(bits, g.alloc)  # [z, y, x]
+
+
+def brute_force_parity(mesh, g, shift=(0.0, 0.0, 0.0), eps=1e-9):
+    """(solid[z,y,x], certain[z,y,x]) by direct ray parity in float64."""
+    X, Y, Z = g.alloc
+    d = g.voxel_length
+    em = [g.min_vertex[a] + shift[a] for a in range(3)]
+    ny, nz = g.res[1], g.res[2]
+    cy = np.arange(ny) * d + em[1]
+    cz = np.arange(nz) * d + em[2]
+    CY, CZ = np.meshgrid(cy, cz, indexing="xy")  # [nz, ny]
+    cx = np.arange(X) * d + em[0] - d  # padded x -> world x = min + (x-1) d
+    count = np.zeros((nz, ny, X), np.int32)
+    uncertain = np.zeros((nz, ny, X), bool)
+    V = mesh.vertices.astype(np.float64)
+    for tri in mesh.indices:
+        a, b, c = V[tri[0]], V[tri[1]], V[tri[2]]
+        n = np.cross(b - a, c - a)
+        if abs(n[0]) < 1e-300:
+            continue
+        def edge(p, q):
+            return (q[1] - p[1]) * (CZ - p[2]) - (q[2] - p[2]) * (CY - p[1])
+        e = np.stack([edge(a, b), edge(b, c), edge(c, a)])
+        scale = eps * (1.0 + np.abs(V).max()) ** 2
+        inside = np.all(e > scale, 0) | np.all(e < -scale, 0)
+        unsure_e = np.any(np.abs(e) <= scale, 0) & ~(np.any(e > scale, 0) & np.any(e < -scale, 0))
+        if not (inside.any() or unsure_e.any()):
+            continue
+        xi = a[0] - (n[1] * (CY - a[1]) + n[2] * (CZ - a[2])) / n[0]
+        left = xi[:, :, None] < cx[None, None, :] - eps
+        near = np.abs(xi[:, :, None] - cx[None, None, :]) <= eps
+        count += (inside[:, :, None] & left)
+        uncertain |= inside[:, :, None] & near
+        uncertain |= unsure_e[:, :, None]
+    solid = np.zeros((Z, Y, X), bool)
+    cert = np.ones((Z, Y, X), bool)
+    solid[1:nz + 1, 1:ny + 1, :] = (count % 2) == 1
+    cert[1:nz + 1, 1:ny + 1, :] = ~uncertain
+    return solid, cert
+
+
+@pytest.mark.parametrize("mesh_fn,mode,param", [
+    (lambda: W.tetrahedron(1.0, (0.013, 0.021, 0.034)), 1, 0.043),
+    (lambda: W.icosphere(seed=5, subdiv=2), 0, 40),
+    (lambda: W.box((0.1, 0.2, 0.3), (0.9, 0.7, 0.8), subdiv=3), 1, 0.031),
+])
+def test_solid_matches_ray_parity(oracle, mesh_fn, mode, param):
+    m = mesh_fn()
+    g = oracle.plan_grid(m.vertices, mode, param)
+    got = unpack(oracle, oracle.solid_bits(m, g), g)
+    ref, cert = brute_force_parity(m, g)
+    assert cert.mean() > 0.85  # the first centre row lies on the bbox silhouette
+    assert np.array_equal(got[cert], ref[cert])
+    assert got.sum() > 0
+
+
+def test_open_soup_matches_ray_parity(oracle):
+    # Open triangle soup: rows with an odd number of crossings stay solid to
+    # the end of the row, exactly as the suffix flips of the reference do.
+    m = W.random_soup(7, 30)
+    g = oracle.plan_grid(m.vertices, 1, 0.03)
+    got = unpack(oracle, oracle.solid_bits(m, g), g)
+    ref, cert = brute_force_parity(m, g)
+    assert np.array_equal(got[cert], ref[cert])
+
+
+def test_xor_additivity(oracle):
+    a = W.icosphere(seed=1, subdiv=2, radius=0.3)
+    b = W.box((0.3, 0.3, 0.3), (0.75, 0.7, 0.8), subdiv=2)
+    both = W._Builder()
+    both.add(a.vertices.astype(np.float64), a.indices.astype(np.int64))
+    both.add(b.vertices.astype(np.float64), b.indices.astype(np.int64))
+    m = both.mesh("ab")
+    g = oracle.plan_grid(m.vertices, 1, 0.02)
+    sa = oracle.solid_bits(a, g)
+    sb = oracle.solid_bits(b, g)
+    sab = oracle.solid_bits(m, g)
+    assert np.array_equal(sab, sa ^ sb)
+    assert np.count_nonzero(sa & sb) > 0  # the parts overlap: the XOR leaves a hole
+
+
+def test_shared_edge_counted_once(oracle):
+    # A square at x = 0.3 split along its diagonal; centres on the diagonal
+    # (y == z) lie exactly on the shared edge.
+    em = (0.0, 0.0, 0.0)
+    d = 0.125
+    quad = np.array([[0.3, 0, 0], [0.3, 1, 0], [0.3, 1, 1], [0.3, 0, 1]], np.float32)
+    for flip in (False, True):
+        tris = [quad[[0, 1, 2]], quad[[0, 2, 3]]]
+        if flip:
+            tris = [t[::-1].copy() for t in tris]
+        for k in range(1, 8):
+            hits = [oracle.center_crossing(t, em, d, k, k)[0] for t in tris]
+            assert sum(hits) == 1, (flip, k, hits)
+        # a centre strictly inside one triangle is counted by that one only
+        hits = [oracle.center_crossing(t, em, d, 6, 2)[0] for t in tris]
+        assert sum(hits) == 1
+
+
+def test_shared_vertex_fan_counted_once(oracle):
+    # Six triangles around the vertex (y, z) = (0.5, 0.5): the centre exactly
+    # on it is counted by exactly one of them.
+    em = (0.0, 0.0, 0.0)
+    d = 0.25
+    c = np.array([0.7, 0.5, 0.5])
+    ring = [np.array([0.7, 0.5 + dy, 0.5 + dz]) for dy, dz in
+            [(0.4, 0.0), (0.2, 0.35), (-0.2, 0.35), (-0.4, 0.0), (-0.2, -0.35), (0.2, -0.35)]]
+    tris = [np.array([c, ring[i], ring[(i + 1) % 6]], np.float32) for i in range(6)]
+    hits = [oracle.center_crossing(t, em, d, 2, 2)[0] for t in tris]
+    assert sum(hits) == 1
+
+
+def test_crossing_voxel_index(oracle):
+    # Plane x = 0.3 at d = 0.1 from min x = 0: ceil(0.3/0.1) + 1 in doubles.
+    tri = np.array([[0.3, 0, 0], [0.3, 1, 0], [0.3, 0, 1]], np.float32)
+    ok, vx = oracle.center_crossing(tri, (0.0, 0.0, 0.0), 0.1, 2, 3)
+    assert ok
+    assert vx == int(np.ceil(np.float64(np.float32(0.3)) / 0.1)) + 1
+
+
+@pytest.mark.parametrize("vsx", [32, 64, 1024])
+def test_x_splits_equal_single_pass(oracle, vsx):
+    m = W.icosphere(seed=2, subdiv=3)
+    g1 = oracle.plan_grid(m.vertices, 0, 126, vox_split_x=1024)
+    gs = oracle.plan_grid(m.vertices, 0, 126, vox_split_x=vsx)
+    assert gs.res[0] == g1.res[0]
+    b1 = oracle.solid_bits(m, g1)
+    bs = oracle.solid_bits(m, gs)
+    assert np.array_equal(b1, bs)
+
+
+def test_yz_splits_generic_mesh(oracle):
+    m = W.icosphere(seed=3, subdiv=3)
+    g = oracle.plan_grid(m.vertices, 0, 126)
+    ref = oracle.solid_bits(m, g, split_yz=(1024, 1024))
+    for sp in [(100, 60), (33, 17), (126, 1)]:
+        assert np.array_equal(oracle.solid_bits(m, g, split_yz=sp), ref), sp
+
+
+def test_box_between_centres_closed_form(oracle):
+    # Box faces half-way between voxel centres: exactly the centres inside.
+    d = 0.0625
+    lo = np.array([0.5, 0.25, 0.125]) + d * np.array([0.0, 0.0, 0.0])
+    m0 = W.box(lo, lo + d * np.array([10, 7, 5]), subdiv=2)
+    g = oracle.plan_grid(m0.vertices, 1, d)
+    # shift the mesh by half a voxel inside the grid built from its bbox
+    m = W.box(lo + d / 2 - d / 2, lo + d * np.array([10, 7, 5]), subdiv=2)
+    got = unpack(oracle, oracle.solid_bits(m, g, shift=(-d / 2, -d / 2, -d / 2)), g)
+    # centres at min - d/2 + i d lie strictly inside for i = 1..n
+    assert got.sum() == 10 * 7 * 5
+    zs, ys, xs = np.nonzero(got)
+    assert (zs.min(), zs.max()) == (2, 6) and (ys.min(), ys.max()) == (2, 8)
+    assert (xs.min(), xs.max()) == (2, 11)
+
+
+def test_padding_and_x0_never_solid(oracle):
+    m = W.shoebox_room(seed=4, n_boxes=8, subdiv=2)
+    g = oracle.plan_grid(m.vertices, 1, 0.05)
+    s = unpack(oracle, oracle.solid_bits(m, g), g)
+    assert not s[0].any() and not s[-1].any() and not s[:, 0].any() and not s[:, -1].any()
+    assert not s[:, :, 0].any()
+    assert s.sum() > 0
+
+
+def test_sphere_volume_within_shell(oracle):
+    m = W.icosphere(seed=0, subdiv=4, radius=0.4, jitter=0.0)
+    g = oracle.plan_grid(m.vertices, 1, 0.01)
+    n = int(unpack(oracle, oracle.solid_bits(m, g), g).sum())
+    r = 0.4 / 0.01
+    shell = 4 * np.pi * r * r * 1.8  # surface voxels: a generous shell
+    assert abs(n - 4 / 3 * np.pi * r ** 3) < shell

@@ -226,6 +226,9 @@ class Context:
         buf.free()
         return out
 
+    def node_count(self, grid: Grid, kind: int) -> int:
+        return int(lib().vox_node_count(C.byref(grid), kind))
+
     def node_dims(self, grid: Grid, kind: int):
         d = np.zeros(3, np.uint32)
         lib().vox_node_dims(C.byref(grid), kind, _ptr(d))

@@ -288,7 +288,8 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
     }
     const uint32_t nbt = p.nb[0] * p.nb[1] * p.nb[2];
     std::string t(tag);
-    uint32_t *tri_info, *cand, *cand_off, *brick_hist, *brick_off, *brick_cursor, *blist, *stats,
+    uint4 *tri_info;
+    uint32_t *cand, *cand_off, *brick_hist, *brick_off, *brick_cursor, *blist, *stats,
         *mat_cursor;
     SplitTable *dsplit;
     VOX_TRY(buf(ctx, (t + "_tri_info").c_str(), nt, &tri_info));
@@ -585,7 +586,7 @@ vox_status vox_plan_grid(vox_ctx *ctx, vox_plan_mode mode, double param, uint32_
     VOX_TRY(buf(ctx, "p_bbox", 6, &bb));
     VOX_TRY(fill(ctx, bb, 3, UINT32_MAX));
     VOX_TRY(fill(ctx, bb + 3, 3, 0u));
-    k_bbox<<<grid_for(ctx->nv, 256, 1024), 256, 0, ctx->stream>>>(ctx->V, ctx->nv, bb);
+    k_bbox<<<grid_for(ctx->nv, 256, 256), 256, 0, ctx->stream>>>(ctx->V, ctx->nv, bb);
     VOX_HIP(hipGetLastError());
     uint32_t h[6];
     VOX_TRY(read_u32(ctx, bb, 6, h));

@@ -124,16 +124,16 @@ __global__ void k_zero_pad_rows(uint32_t *__restrict__ bits, uint32_t words_per_
 // ---------------------------------------------------------------------------
 // Surface
 // ---------------------------------------------------------------------------
-__global__ void k_surf_setup(SurfParams p, uint32_t *__restrict__ tri_info,
+__global__ void k_surf_setup(SurfParams p, uint4 *__restrict__ tri_info,
                              uint32_t *__restrict__ cand_count);
 __global__ void k_surf_bin(SurfParams p, const uint32_t *__restrict__ cand_off,
                            const uint32_t *__restrict__ cand_total,
-                           const uint32_t *__restrict__ tri_info,
+                           const uint4 *__restrict__ tri_info,
                            uint32_t *__restrict__ brick_hist, const uint32_t *__restrict__ brick_off,
                            uint32_t *__restrict__ brick_cursor, uint32_t *__restrict__ blist,
                            int scatter);
 __global__ void k_surf_brick(SurfParams p, const SplitTable *__restrict__ splits,
-                             const uint32_t *__restrict__ tri_info,
+                             const uint4 *__restrict__ tri_info,
                              const uint32_t *__restrict__ brick_off,
                              const uint32_t *__restrict__ brick_hist,
                              const uint32_t *__restrict__ blist,

@@ -81,12 +81,26 @@ __global__ __launch_bounds__(256) void k_bbox(const float *__restrict__ V, uint3
             mx[c] = max(mx[c], (uint32_t)__shfl_xor((int)mx[c], off));
         }
     }
+    // Block reduction through LDS, then one set of atomics per block.
+    __shared__ uint32_t red[6][256 / kWave];
+    const uint32_t wv = threadIdx.x / kWave;
     if (lane_id() == 0) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            atomicMin(&out6[c], mn[c]);
-            atomicMax(&out6[3 + c], mx[c]);
+            red[c][wv] = mn[c];
+            red[3 + c][wv] = mx[c];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int c = threadIdx.x;
+        uint32_t r = red[c][0];
+        for (uint32_t k = 1; k < blockDim.x / kWave; k++)
+            r = c < 3 ? min(r, red[c][k]) : max(r, red[c][k]);
+        if (c < 3)
+            atomicMin(&out6[c], r);
+        else
+            atomicMax(&out6[c], r);
     }
 }
 
@@ -277,7 +291,10 @@ __device__ __forceinline__ void brick_range(const SurfParams &p, const VoxBB &vb
     hi[2] = (uint32_t)(vb.max.z + 1) / (uint32_t)p.bdim[2];
 }
 
-__global__ __launch_bounds__(256) void k_surf_setup(SurfParams p, uint32_t *__restrict__ tri_info,
+// Per triangle: class and dominant axis (classifyTriangles), and the box of
+// bricks its clipped voxel range touches, packed as
+// {info, lo.x | lo.y<<16, lo.z | w.x<<16, w.y | w.z<<16}.
+__global__ __launch_bounds__(256) void k_surf_setup(SurfParams p, uint4 *__restrict__ tri_info,
                                                     uint32_t *__restrict__ cand_count) {
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < p.nt; t += gridDim.x * blockDim.x) {
         F3 tri[3];
@@ -286,36 +303,54 @@ __global__ __launch_bounds__(256) void k_surf_setup(SurfParams p, uint32_t *__re
         classify(tri, p.m, p.d, bb, axis);
         uint32_t c = 0;
         VoxBB vb;
+        uint32_t lo[3] = {0, 0, 0}, w[3] = {0, 0, 0};
         if (bb != BB_DEGEN && surf_range(p, tri, vb)) {
-            uint32_t lo[3], hi[3];
+            uint32_t hi[3];
             brick_range(p, vb, lo, hi);
-            c = (hi[0] - lo[0] + 1u) * (hi[1] - lo[1] + 1u) * (hi[2] - lo[2] + 1u);
+            for (int a = 0; a < 3; a++) w[a] = hi[a] - lo[a] + 1u;
+            c = w[0] * w[1] * w[2];
         }
-        tri_info[t] = ((uint32_t)bb << 30) | ((uint32_t)axis << 28);
+        const uint32_t info = ((uint32_t)bb << 30) | ((uint32_t)axis << 28);
+        tri_info[t] = make_uint4(info, lo[0] | (lo[1] << 16), lo[2] | (w[0] << 16),
+                                 w[1] | (w[2] << 16));
         cand_count[t] = c;
     }
 }
 
+// One thread per (triangle, brick) candidate: counting pass (scatter = 0)
+// or placement pass (scatter = 1) of the counting sort by brick.  The
+// triangle of candidate c is found by binary search in the candidate offsets,
+// narrowed per block to the triangles its 256 candidates span.
 __global__ __launch_bounds__(256) void k_surf_bin(
     SurfParams p, const uint32_t *__restrict__ cand_off, const uint32_t *__restrict__ cand_total,
-    const uint32_t *__restrict__ tri_info, uint32_t *__restrict__ brick_hist,
+    const uint4 *__restrict__ tri_info, uint32_t *__restrict__ brick_hist,
     const uint32_t *__restrict__ brick_off, uint32_t *__restrict__ brick_cursor,
     uint32_t *__restrict__ blist, int scatter) {
+    __shared__ uint32_t trange[2];
     const uint32_t total = *cand_total;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < total;
-         c += gridDim.x * blockDim.x) {
-        const uint32_t t = upper_bound_u32(cand_off, p.nt, c) - 1u;
-        F3 tri[3];
-        load_tri_f(p, t, tri);
-        VoxBB vb;
-        if (!surf_range(p, tri, vb)) continue;
-        uint32_t lo[3], hi[3];
-        brick_range(p, vb, lo, hi);
-        const uint32_t w0 = hi[0] - lo[0] + 1u, w1 = hi[1] - lo[1] + 1u;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            const uint32_t key = threadIdx.x == 0 ? base : min(base + blockDim.x, total) - 1u;
+            trange[threadIdx.x] = upper_bound_u32(cand_off, p.nt, key) - 1u;
+        }
+        __syncthreads();
+        const uint32_t c = base + threadIdx.x;
+        if (c >= total) continue;
+        uint32_t lo = trange[0], hi = trange[1] + 1u;
+        while (lo + 1u < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (cand_off[mid] <= c) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t t = lo;
+        const uint4 ti = tri_info[t];
+        const uint32_t bx0 = ti.y & 0xFFFFu, by0 = ti.y >> 16, bz0 = ti.z & 0xFFFFu;
+        const uint32_t w0 = ti.z >> 16, w1 = ti.w & 0xFFFFu;
         const uint32_t local = c - cand_off[t];
-        const uint32_t bx = lo[0] + local % w0;
-        const uint32_t by = lo[1] + (local / w0) % w1;
-        const uint32_t bz = lo[2] + local / (w0 * w1);
+        const uint32_t bx = bx0 + local % w0;
+        const uint32_t by = by0 + (local / w0) % w1;
+        const uint32_t bz = bz0 + local / (w0 * w1);
         const uint32_t b = (bz * p.nb[1] + by) * p.nb[0] + bx;
         if (!scatter) {
             atomicAdd(&brick_hist[b], 1u);
@@ -324,7 +359,6 @@ __global__ __launch_bounds__(256) void k_surf_bin(
             blist[pos] = t;
         }
     }
-    (void)tri_info;
 }
 
 __device__ __forceinline__ int32_t split_hi_of(const SplitTable &s, int axis, int32_t i,
@@ -353,118 +387,124 @@ __device__ __forceinline__ void lds_write(uint32_t *keys, const BrickBox &bb, co
     atomicMax(&keys[(lz * bd[1] + ly) * bd[0] + lx], key);
 }
 
-// Surface voxels of one triangle inside one brick, per the 1D/2D/3D kernels
-// of the reference (src/voxelizer.cu:3307-3955) with their matSplitRes
-// subspace clipping: only the 3D single-voxel shortcut depends on the
-// subspace, through the dominant-axis interval the voxel falls in.
-__device__ void surf_triangle_in_brick(const SurfParams &p, const SplitTable &spl, uint32_t t,
-                                       uint32_t info, const BrickBox &bx, uint32_t *keys) {
+// Per-triangle data of one brick's chunk, set up once in LDS.
+struct SurfTri {
+    OverlapData o[3];          // projections YZ (0), ZX (1), XY (2)
+    F3 n, t0;                  // normal, first vertex
+    int32_t vmin[3], vmax[3];  // voxel range clipped to the ext space
+    int32_t rlo[3], rhi[3];    // part of it inside this brick
+    uint32_t key;              // bbType<<30 | triangle
+    int32_t bb, axis;
+};
+constexpr int kSurfChunk = 64;
+
+__device__ __forceinline__ bool test_plane(const OverlapData &o, int plane, const float pc[3]) {
+    return plane == 0 ? overlap2(o, pc[1], pc[2])
+         : plane == 1 ? overlap2(o, pc[2], pc[0])
+                      : overlap2(o, pc[0], pc[1]);
+}
+
+// Sets up triangle t for the brick; returns its number of work items in the
+// brick (voxels along the axis for 1D, dominant-plane columns for 2D/3D).
+__device__ uint32_t surf_tri_setup(const SurfParams &p, uint32_t t, uint32_t info,
+                                   const BrickBox &bx, SurfTri &st) {
     F3 tri[3];
     load_tri_f(p, t, tri);
     VoxBB vb;
-    if (!surf_range(p, tri, vb)) return;
-    const int bb = (int)(info >> 30);
-    const int axis = (int)((info >> 28) & 3u);
-    const uint32_t key = ((uint32_t)bb << 30) | t;
-    // Region of vb inside this brick.
-    int32_t rlo[3] = {max(vb.min.x, bx.lo[0]), max(vb.min.y, bx.lo[1]), max(vb.min.z, bx.lo[2])};
-    int32_t rhi[3] = {min(vb.max.x, bx.hi[0]), min(vb.max.y, bx.hi[1]), min(vb.max.z, bx.hi[2])};
-    if (rlo[0] > rhi[0] || rlo[1] > rhi[1] || rlo[2] > rhi[2]) return;
-    const int32_t vmin[3] = {vb.min.x, vb.min.y, vb.min.z};
-    const int32_t vmax[3] = {vb.max.x, vb.max.y, vb.max.z};
-    // Map (dominant, u, v) -> (x, y, z): u, v are the other axes in order.
-    const int ua = axis == AX_X ? AX_Y : AX_X;
-    const int va = axis == AX_Z ? AX_Y : AX_Z;
-    if (bb == BB_1D) {
-        int32_t c[3] = {vmin[0], vmin[1], vmin[2]};
-        for (int32_t i = rlo[axis]; i <= rhi[axis]; i++) {
-            c[axis] = i;
-            lds_write(keys, bx, p.bdim, c[0], c[1], c[2], key);
-        }
-        return;
+    if (!surf_range(p, tri, vb)) return 0u;
+    st.bb = (int32_t)(info >> 30);
+    st.axis = (int32_t)((info >> 28) & 3u);
+    st.key = ((uint32_t)st.bb << 30) | t;
+    st.vmin[0] = vb.min.x; st.vmin[1] = vb.min.y; st.vmin[2] = vb.min.z;
+    st.vmax[0] = vb.max.x; st.vmax[1] = vb.max.y; st.vmax[2] = vb.max.z;
+    for (int a = 0; a < 3; a++) {
+        st.rlo[a] = max(st.vmin[a], bx.lo[a]);
+        st.rhi[a] = min(st.vmax[a], bx.hi[a]);
+        if (st.rlo[a] > st.rhi[a]) return 0u;
     }
-    const F3 n = fcross(fsub(tri[0], tri[2]), fsub(tri[1], tri[0]));
-    const float mc[3] = {p.m.x, p.m.y, p.m.z};
-    // Column test plane for the dominant axis: x -> YZ(0), y -> ZX(1), z -> XY(2).
-    OverlapData od;
-    setup_overlap(tri, n, p.d, axis, od);
-    if (bb == BB_2D) {
-        for (int32_t u = rlo[ua]; u <= rhi[ua]; u++)
-            for (int32_t v = rlo[va]; v <= rhi[va]; v++) {
-                float pc[3];
-                pc[axis] = 0.0f;
-                pc[ua] = vox_corner(u, mc[ua], p.d);
-                pc[va] = vox_corner(v, mc[va], p.d);
-                bool ok = axis == AX_X ? overlap2(od, pc[1], pc[2])
-                        : axis == AX_Y ? overlap2(od, pc[2], pc[0])
-                                       : overlap2(od, pc[0], pc[1]);
-                if (!ok) continue;
-                int32_t c[3];
-                c[axis] = vmin[axis];
-                c[ua] = u;
-                c[va] = v;
-                lds_write(keys, bx, p.bdim, c[0], c[1], c[2], key);
-            }
-        return;
-    }
-    if (bb != BB_3D) return;
-    // The two remaining planes of the 3D test.
-    const int pa = axis == AX_X ? 1 : 0;  // first remaining plane
-    const int pb = axis == AX_Z ? 1 : 2;  // second remaining plane
-    OverlapData oa, ob;
-    setup_overlap(tri, n, p.d, pa, oa);
-    setup_overlap(tri, n, p.d, pb, ob);
-    for (int32_t u = rlo[ua]; u <= rhi[ua]; u++)
-        for (int32_t v = rlo[va]; v <= rhi[va]; v++) {
-            float pc[3];
-            pc[axis] = 0.0f;
-            pc[ua] = vox_corner(u, mc[ua], p.d);
-            pc[va] = vox_corner(v, mc[va], p.d);
-            bool ok = axis == AX_X ? overlap2(od, pc[1], pc[2])
-                    : axis == AX_Y ? overlap2(od, pc[2], pc[0])
-                                   : overlap2(od, pc[0], pc[1]);
-            if (!ok) continue;
-            int32_t lo, hi;
-            depth_range(tri[0], F3{pc[0], pc[1], pc[2]}, n, p.m, p.d, axis, lo, hi);
-            const int32_t ilo = max(lo, rlo[axis]), ihi = min(hi, rhi[axis]);
-            for (int32_t i = ilo; i <= ihi; i++) {
-                int32_t slo;
-                const int32_t shi = split_hi_of(spl, axis, i, slo);
-                const int32_t bl = max(vmin[axis], slo), bh = min(vmax[axis], shi - 1);
-                const int32_t rl = lo < bl ? bl : lo;
-                const int32_t rh = hi > bh ? bh : hi;
-                bool write = rl == rh;
-                if (!write) {
-                    pc[axis] = vox_corner(i, mc[axis], p.d);
-                    auto test = [&](const OverlapData &o, int plane) {
-                        return plane == 0 ? overlap2(o, pc[1], pc[2])
-                             : plane == 1 ? overlap2(o, pc[2], pc[0])
-                                          : overlap2(o, pc[0], pc[1]);
-                    };
-                    write = test(oa, pa) && test(ob, pb);
-                }
-                if (write) {
-                    int32_t c[3];
-                    c[axis] = i;
-                    c[ua] = u;
-                    c[va] = v;
-                    lds_write(keys, bx, p.bdim, c[0], c[1], c[2], key);
-                }
-            }
-        }
+    const int ax = st.axis;
+    if (st.bb == BB_1D) return (uint32_t)(st.rhi[ax] - st.rlo[ax] + 1);
+    st.n = fcross(fsub(tri[0], tri[2]), fsub(tri[1], tri[0]));
+    st.t0 = tri[0];
+    setup_overlap(tri, st.n, p.d, 0, st.o[0]);
+    setup_overlap(tri, st.n, p.d, 1, st.o[1]);
+    setup_overlap(tri, st.n, p.d, 2, st.o[2]);
+    const int ua = ax == AX_X ? AX_Y : AX_X;
+    const int va = ax == AX_Z ? AX_Y : AX_Z;
+    return (uint32_t)((st.rhi[ua] - st.rlo[ua] + 1) * (st.rhi[va] - st.rlo[va] + 1));
 }
 
-// One workgroup per brick: winners in LDS (ds_max of bbType<<30 | triangle),
-// then the brick's surface bit words, the solid|surface update of the bit
-// grid, and the materials of its surface voxels (compacted, voxel order).
+// One work item: per the 1D/2D/3D kernels of the reference
+// (src/voxelizer.cu:3307-3955) with their matSplitRes subspace clipping --
+// only the 3D single-voxel shortcut depends on the subspace, through the
+// dominant-axis interval the voxel falls in.
+__device__ void surf_item(const SurfParams &p, const SplitTable &spl, const SurfTri &st,
+                          uint32_t k, const BrickBox &bx, uint32_t *keys) {
+    const int ax = st.axis;
+    if (st.bb == BB_1D) {
+        int32_t c[3] = {st.vmin[0], st.vmin[1], st.vmin[2]};
+        c[ax] = st.rlo[ax] + (int32_t)k;
+        lds_write(keys, bx, p.bdim, c[0], c[1], c[2], st.key);
+        return;
+    }
+    const int ua = ax == AX_X ? AX_Y : AX_X;
+    const int va = ax == AX_Z ? AX_Y : AX_Z;
+    const int32_t wu = st.rhi[ua] - st.rlo[ua] + 1;
+    const int32_t u = st.rlo[ua] + (int32_t)k % wu;
+    const int32_t v = st.rlo[va] + (int32_t)k / wu;
+    const float mc[3] = {p.m.x, p.m.y, p.m.z};
+    float pc[3];
+    pc[ax] = 0.0f;
+    pc[ua] = vox_corner(u, mc[ua], p.d);
+    pc[va] = vox_corner(v, mc[va], p.d);
+    if (!test_plane(st.o[ax], ax, pc)) return;
+    int32_t c[3];
+    c[ua] = u;
+    c[va] = v;
+    if (st.bb == BB_2D) {
+        c[ax] = st.vmin[ax];
+        lds_write(keys, bx, p.bdim, c[0], c[1], c[2], st.key);
+        return;
+    }
+    int32_t lo, hi;
+    depth_range(st.t0, F3{pc[0], pc[1], pc[2]}, st.n, p.m, p.d, ax, lo, hi);
+    const int pa = ax == AX_X ? 1 : 0;
+    const int pb = ax == AX_Z ? 1 : 2;
+    const int32_t ilo = max(lo, st.rlo[ax]), ihi = min(hi, st.rhi[ax]);
+    for (int32_t i = ilo; i <= ihi; i++) {
+        int32_t slo;
+        const int32_t shi = split_hi_of(spl, ax, i, slo);
+        const int32_t bl = max(st.vmin[ax], slo), bh = min(st.vmax[ax], shi - 1);
+        const int32_t rl = lo < bl ? bl : lo;
+        const int32_t rh = hi > bh ? bh : hi;
+        bool write = rl == rh;
+        if (!write) {
+            pc[ax] = vox_corner(i, mc[ax], p.d);
+            write = test_plane(st.o[pa], pa, pc) && test_plane(st.o[pb], pb, pc);
+        }
+        if (write) {
+            c[ax] = i;
+            lds_write(keys, bx, p.bdim, c[0], c[1], c[2], st.key);
+        }
+    }
+}
+
+// One workgroup per brick.  The brick's triangles are taken 64 at a time:
+// each is set up once into LDS, then all (triangle, column) work items of the
+// chunk are spread over the 256 threads.  Winners accumulate in LDS
+// (ds_max of bbType<<30 | triangle); the brick then writes its surface bit
+// words, ORs them into the solid grid, and appends the materials of its
+// surface voxels (compacted, voxel order) to the material list.
 __global__ __launch_bounds__(256) void k_surf_brick(
-    SurfParams p, const SplitTable *__restrict__ splits, const uint32_t *__restrict__ tri_info,
+    SurfParams p, const SplitTable *__restrict__ splits, const uint4 *__restrict__ tri_info,
     const uint32_t *__restrict__ brick_off, const uint32_t *__restrict__ brick_hist,
     const uint32_t *__restrict__ blist, const uint8_t *__restrict__ materials,
     uint32_t *__restrict__ bits, uint32_t *__restrict__ surf, uint32_t *__restrict__ brick_mat_off,
     uint8_t *__restrict__ mat_list, uint32_t *__restrict__ mat_cursor, uint32_t mat_cap,
     uint32_t *__restrict__ stats) {
     __shared__ uint32_t keys[kMaxBrickVox];
+    __shared__ SurfTri tris[kSurfChunk];
+    __shared__ uint32_t colsum[kSurfChunk + 1];
     __shared__ uint32_t wsurf[kMaxBrickVox / 32];
     __shared__ uint32_t wexcl[kMaxBrickVox / 32];
     __shared__ uint32_t sbase;
@@ -484,36 +524,67 @@ __global__ __launch_bounds__(256) void k_surf_brick(
         uint32_t *dst = (uint32_t *)&spl;
         for (uint32_t i = tid; i < sizeof(SplitTable) / 4; i += blockDim.x) dst[i] = src[i];
     }
-    __syncthreads();
     BrickBox box;
     box.lo[0] = (int32_t)(bxi * p.bdim[0]) - 1;
     box.lo[1] = (int32_t)(byi * p.bdim[1]) - 1;
     box.lo[2] = (int32_t)(bzi * p.bdim[2]) - 1;
     for (int a = 0; a < 3; a++) box.hi[a] = box.lo[a] + p.bdim[a] - 1;
     const uint32_t off = brick_off[b];
-    for (uint32_t i = tid; i < cnt; i += blockDim.x) {
-        const uint32_t t = blist[off + i];
-        surf_triangle_in_brick(p, spl, t, tri_info[t], box, keys);
-    }
-    __syncthreads();
-    const int32_t wpr = p.bdim[0] / 32;  // words per brick row
-    const int32_t nwords = nvox / 32;
-    for (int32_t w = tid; w < nwords; w += blockDim.x) {
-        uint32_t s = 0u;
-        const int32_t v0 = w * 32;
-        for (int k = 0; k < 32; k++) s |= (keys[v0 + k] != 0u ? 1u : 0u) << (31 - k);
-        wsurf[w] = s;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t acc = 0u;
-        for (int32_t w = 0; w < nwords; w++) {
-            wexcl[w] = acc;
-            acc += __popc(wsurf[w]);
+    for (uint32_t c0 = 0; c0 < cnt; c0 += kSurfChunk) {
+        __syncthreads();
+        if (tid < (uint32_t)kSurfChunk) {
+            uint32_t items = 0u;
+            if (c0 + tid < cnt) {
+                const uint32_t t = blist[off + c0 + tid];
+                items = surf_tri_setup(p, t, tri_info[t].x, box, tris[tid]);
+            }
+            // wave-level inclusive scan of the item counts
+            uint32_t incl = items;
+            for (int d = 1; d < kSurfChunk; d <<= 1) {
+                uint32_t o = (uint32_t)__shfl_up((int)incl, d);
+                if ((int)tid >= d) incl += o;
+            }
+            colsum[tid + 1] = incl;
+            if (tid == 0) colsum[0] = 0u;
         }
+        __syncthreads();
+        const uint32_t total = colsum[kSurfChunk];
+        for (uint32_t c = tid; c < total; c += blockDim.x) {
+            uint32_t lo = 0, hi = kSurfChunk;  // find i with colsum[i] <= c < colsum[i+1]
+            while (lo + 1 < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (colsum[mid] <= c) lo = mid;
+                else hi = mid;
+            }
+            surf_item(p, spl, tris[lo], c - colsum[lo], box, keys);
+        }
+    }
+    __syncthreads();
+    // Surface bit words: one ballot per 64 voxels (MSB-first words).
+    const int32_t nwords = nvox / 32;
+    for (int32_t v0 = (int32_t)(tid & ~63u); v0 < nvox; v0 += (int32_t)blockDim.x) {
+        const unsigned long long m = __ballot(keys[v0 + (int32_t)lane_id()] != 0u);
+        if (lane_id() == 0) {
+            wsurf[v0 / 32] = __brev((uint32_t)m);
+            wsurf[v0 / 32 + 1] = __brev((uint32_t)(m >> 32));
+        }
+    }
+    __syncthreads();
+    // Exclusive popcount scan over the brick's words.
+    uint32_t pc = tid < (uint32_t)nwords ? (uint32_t)__popc(wsurf[tid]) : 0u;
+    if (tid < (uint32_t)nwords) wexcl[tid] = pc;
+    __syncthreads();
+    for (uint32_t d = 1; d < (uint32_t)nwords; d <<= 1) {
+        uint32_t o = (tid < (uint32_t)nwords && tid >= d) ? wexcl[tid - d] : 0u;
+        __syncthreads();
+        if (tid < (uint32_t)nwords) wexcl[tid] += o;
+        __syncthreads();
+    }
+    if (tid == (uint32_t)nwords - 1) {
+        const uint32_t acc = wexcl[tid];
         uint32_t base = atomicAdd(mat_cursor, acc);
         if (base + acc > mat_cap) {
-            atomicOr(&stats[0], 1u);  // overflow: caller retries with more room
+            atomicOr(&stats[0], 1u);  // overflow: the caller retries with more room
             base = UINT32_MAX;
         }
         sbase = base;
@@ -522,6 +593,7 @@ __global__ __launch_bounds__(256) void k_surf_brick(
     }
     __syncthreads();
     const uint32_t base = sbase;
+    const int32_t wpr = p.bdim[0] / 32;  // words per brick row
     for (int32_t w = tid; w < nwords; w += blockDim.x) {
         const int32_t wx = w % wpr, ly = (w / wpr) % p.bdim[1], lz = w / (wpr * p.bdim[1]);
         const uint32_t aw = bxi * (uint32_t)wpr + (uint32_t)wx;
@@ -531,9 +603,9 @@ __global__ __launch_bounds__(256) void k_surf_brick(
         const size_t gi = ((size_t)az * p.alloc[1] + ay) * (p.alloc[0] / 32u) + aw;
         const uint32_t s = wsurf[w];
         surf[gi] = s;
-        bits[gi] = bits[gi] | s;
+        if (s) bits[gi] = bits[gi] | s;
         if (base != UINT32_MAX && s) {
-            uint32_t r = base + wexcl[w];
+            uint32_t r = base + wexcl[w] - (uint32_t)__popc(s);
             uint32_t m = s;
             while (m) {
                 const int k = __clz(m);
