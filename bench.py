@@ -100,7 +100,21 @@ def main():
                     help="seconds of CPU work for the oracle baseline sample")
     ap.add_argument("--check", action="store_true",
                     help="compare the last output with the CPU oracle (slow for the hall)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (reference kernel of known "
+                         "size for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = 1 << 30
+        src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_each_way": n, "copies": 5}), flush=True)
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1",
                           "n_gpus": args.gpus}))

@@ -118,6 +118,10 @@ vox_status vox_set_materials_device(vox_ctx *ctx, const uint8_t *materials_dev, 
  * (src/host_side.cpp:2462-2634, 717-738). */
 vox_status vox_plan_grid(vox_ctx *ctx, vox_plan_mode mode, double param,
                          uint32_t vox_split_x, vox_grid *out);
+/* The same plan computed on the host from host vertices (no device needed);
+ * backs Voxelizer::getArrayDimensions (src/host_side.cpp:3329-3470). */
+vox_status vox_plan_grid_host(const float *vertices, uint32_t nv, vox_plan_mode mode,
+                              double param, uint32_t vox_split_x, vox_grid *out);
 
 /* ---- hot path --------------------------------------------------------- */
 

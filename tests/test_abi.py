@@ -5,6 +5,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import voxelizer_amd as va
@@ -52,3 +53,20 @@ def test_create_without_device_reports_no_device():
 
 def test_build_info():
     assert b"gfx950" in va.lib().vox_build_info()
+
+
+@pytest.mark.parametrize("mode,param,vsx", [(0, 126, 1024), (1, 0.01, 1024), (1, 0.013, 64),
+                                            (0, 1000, 128)])
+def test_host_plan_matches_oracle(oracle, mode, param, vsx):
+    from voxelizer_amd import workloads as W
+    for m in (W.icosphere(seed=0), W.shoebox_room(seed=1, n_boxes=4, subdiv=1)):
+        g = va.plan_grid_host(m.vertices, mode, param, vsx)
+        og = oracle.plan_grid(m.vertices, mode, param, vsx)
+        assert g.same(og), (g.as_dict(), og.as_dict())
+
+
+def test_host_plan_rejects_bad_arguments():
+    v = np.zeros((3, 3), np.float32)
+    for mode, param, vsx in [(0, 1, 1024), (1, 0.0, 1024), (1, 0.1, 0), (7, 1.0, 1024)]:
+        with pytest.raises(va.VoxError):
+            va.plan_grid_host(v, mode, param, vsx)

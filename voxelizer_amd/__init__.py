@@ -66,7 +66,7 @@ EXPORTED = [
     "vox_create", "vox_destroy", "vox_last_error", "vox_device_count", "vox_sync",
     "vox_build_info", "vox_alloc", "vox_free", "vox_memcpy_h2d", "vox_memcpy_d2h",
     "vox_set_mesh", "vox_set_mesh_device", "vox_set_materials", "vox_set_materials_device",
-    "vox_plan_grid", "vox_solid_bits", "vox_voxelize_nodes", "vox_node_dims",
+    "vox_plan_grid", "vox_plan_grid_host", "vox_solid_bits", "vox_voxelize_nodes", "vox_node_dims",
     "vox_node_count", "vox_node_size", "vox_enable_timing", "vox_get_stage_times",
 ]
 
@@ -101,6 +101,7 @@ def lib():
     L.vox_set_materials.argtypes = [P, P, U32]
     L.vox_set_materials_device.argtypes = [P, P, U32]
     L.vox_plan_grid.argtypes = [P, I32, D, U32, C.POINTER(Grid)]
+    L.vox_plan_grid_host.argtypes = [P, U32, I32, D, U32, C.POINTER(Grid)]
     L.vox_solid_bits.argtypes = [P, C.POINTER(Grid), P, P, P]
     L.vox_voxelize_nodes.argtypes = [P, C.POINTER(Grid), I32, I32, P, P, P]
     L.vox_node_dims.argtypes = [C.POINTER(Grid), I32, P]
@@ -117,6 +118,18 @@ def lib():
 
 def device_count() -> int:
     return int(lib().vox_device_count())
+
+
+def plan_grid_host(vertices, mode: int, param: float, vox_split_x: int = 1024) -> Grid:
+    """Grid plan computed on the host (no device); same result as
+    Context.plan_grid for the same vertices."""
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1)
+    g = Grid()
+    st = lib().vox_plan_grid_host(_ptr(v), v.size // 3, mode, float(param), vox_split_x,
+                                  C.byref(g))
+    if st != 0:
+        raise VoxError(st, "vox_plan_grid_host rejected its arguments")
+    return g
 
 
 def _ptr(a: np.ndarray):
@@ -264,5 +277,6 @@ class Context:
 
 
 __all__ = ["Context", "Grid", "DeviceBuffer", "VoxError", "lib", "device_count",
+           "plan_grid_host",
            "NODE_SHORT", "NODE_LONG", "NODE_PARTIAL", "NODE_SHORT_FCC", "NODE_LONG_FCC",
            "PLAN_MAX_DIMENSION", "PLAN_CUBE_LENGTH", "NODE_SIZE", "EXPORTED"]

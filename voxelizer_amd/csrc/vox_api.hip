@@ -74,7 +74,7 @@ struct vox_ctx {
     bool timing = false;
     hipEvent_t ev[8] = {};
     vox_stage_times times = {};
-    uint32_t mat_cap_hint = 1u << 20;
+    uint32_t mat_slot_cap = 1u << 16;  // bytes per material arena (multiple of 4)
 };
 
 #define VOX_HIP(expr)                                                               \
@@ -93,6 +93,9 @@ struct vox_ctx {
     } while (0)
 
 namespace {
+
+// Pinned staging buffer for small device-to-host reads (counters, totals).
+constexpr size_t kPinnedBytes = 64 * 1024;
 
 vox_status fail(vox_ctx *ctx, vox_status s, const std::string &msg) {
     ctx->err = msg;
@@ -117,6 +120,12 @@ vox_status buf(vox_ctx *ctx, const char *name, size_t count, T **out) {
     }
     *out = (T *)b.p;
     return VOX_OK;
+}
+
+// Row-major launches: block (64, 4) covers 4 rows of up to 64 words; a 1D
+// grid padded to a multiple of the XCD count (see xcd_logical_block).
+inline dim3 row_grid(uint32_t W, uint32_t Y, uint32_t Z) {
+    return dim3(xcd_grid(((W + kWave - 1) / kWave) * ((Y + 3) / 4) * Z));
 }
 
 inline uint32_t grid_for(size_t n, uint32_t threads, uint32_t cap = 16384) {
@@ -160,6 +169,7 @@ vox_status scan(vox_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t n, int
 }
 
 vox_status read_u32(vox_ctx *ctx, const uint32_t *dev, size_t n, uint32_t *host) {
+    if (n * 4 > kPinnedBytes) return fail(ctx, VOX_ERR_INVALID, "read_u32: read exceeds staging");
     VOX_HIP(hipMemcpyAsync(ctx->pinned, dev, n * 4, hipMemcpyDeviceToHost, ctx->stream));
     VOX_HIP(hipStreamSynchronize(ctx->stream));
     std::memcpy(host, ctx->pinned, n * 4);
@@ -197,6 +207,8 @@ vox_status solid_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3], u
     VOX_TRY(buf(ctx, "s_tile_off", nTiles + 1, &tile_off));
     VOX_TRY(buf(ctx, "s_tile_cursor", nTiles, &tile_cursor));
     VOX_TRY(buf(ctx, "s_counters", 4, &counters));
+    uint4 *cbox;
+    VOX_TRY(buf(ctx, "s_cbox", nt, &cbox));
 
     std::vector<Span> ys = split_axis_max(Yext, splitY), zs = split_axis_max(Zext, splitZ);
     for (const Span &zsp : zs)
@@ -217,7 +229,7 @@ vox_status solid_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3], u
             p.X = X;
             p.rowsY = rowsY;
             if (nt > 0) {
-                k_solid_setup<<<grid_for(nt, 256), 256, 0, ctx->stream>>>(p, cand_count);
+                k_solid_setup<<<grid_for(nt, 256), 256, 0, ctx->stream>>>(p, cand_count, cbox);
                 VOX_HIP(hipGetLastError());
             }
             VOX_TRY(scan(ctx, cand_count, cand_off, nt));
@@ -230,13 +242,13 @@ vox_status solid_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3], u
             VOX_TRY(fill(ctx, tile_cursor, nTiles, 0u));
             if (total > 0) {
                 k_solid_candidates<<<grid_for(total, 256), 256, 0, ctx->stream>>>(
-                    p, cand_off, cand_off + nt, pairs, counters, total, tile_hist);
+                    p, cand_off, cand_off + nt, cbox, pairs, tile_hist);
                 VOX_HIP(hipGetLastError());
             }
             VOX_TRY(scan(ctx, tile_hist, tile_off, nTiles));
             if (total > 0) {
                 k_solid_scatter<<<grid_for(total, 256), 256, 0, ctx->stream>>>(
-                    pairs, counters, tile_off, tile_cursor, list);
+                    pairs, cand_off + nt, tile_off, tile_cursor, list);
                 VOX_HIP(hipGetLastError());
             }
             const uint32_t ty0 = ysp.min / 4, ty1 = (ysp.max - 1) / 4;
@@ -247,11 +259,7 @@ vox_status solid_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3], u
                     p, tile_off, tile_hist, list, ty0, nty, tz0, bits);
                 VOX_HIP(hipGetLastError());
             }
-            if (ctx->timing) {
-                uint32_t q = 0;
-                VOX_TRY(read_u32(ctx, counters, 1, &q));
-                ctx->times.solid_pairs += q;
-            }
+            ctx->times.solid_pairs += total;  // candidate (triangle, tile) pairs
         }
     k_zero_pad_rows<<<grid_for((size_t)(2 * rowsZ + 2 * rowsY) * W, 256, 4096), 256, 0,
                       ctx->stream>>>(bits, W, rowsY, rowsZ);
@@ -265,6 +273,7 @@ vox_status solid_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3], u
 struct SurfOut {
     uint32_t *surf;
     uint32_t *brick_mat_off;
+    uint16_t *wrank;
     uint8_t *mat_list;
     uint32_t nb[3];
 };
@@ -289,8 +298,7 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
     const uint32_t nbt = p.nb[0] * p.nb[1] * p.nb[2];
     std::string t(tag);
     uint4 *tri_info;
-    uint32_t *cand, *cand_off, *brick_hist, *brick_off, *brick_cursor, *blist, *stats,
-        *mat_cursor;
+    uint32_t *cand, *cand_off, *brick_hist, *brick_off, *brick_cursor, *blist, *stats;
     SplitTable *dsplit;
     VOX_TRY(buf(ctx, (t + "_tri_info").c_str(), nt, &tri_info));
     VOX_TRY(buf(ctx, "f_cand", nt, &cand));
@@ -298,12 +306,13 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
     VOX_TRY(buf(ctx, "f_brick_hist", nbt, &brick_hist));
     VOX_TRY(buf(ctx, "f_brick_off", nbt + 1, &brick_off));
     VOX_TRY(buf(ctx, "f_brick_cursor", nbt, &brick_cursor));
-    VOX_TRY(buf(ctx, "f_stats", 4, &stats));
-    VOX_TRY(buf(ctx, "f_mat_cursor", 1, &mat_cursor));
+    VOX_TRY(buf(ctx, "f_stats", kMatSlots * kSlotStride + 1, &stats));
     VOX_TRY(buf(ctx, (t + "_split").c_str(), 1, &dsplit));
     VOX_TRY(buf(ctx, (t + "_surf").c_str(), (size_t)g.alloc[0] / 32 * g.alloc[1] * g.alloc[2],
                 &out.surf));
     VOX_TRY(buf(ctx, (t + "_bmo").c_str(), nbt, &out.brick_mat_off));
+    const uint32_t nwords = (uint32_t)(bdim[0] * bdim[1] * bdim[2] / 32);
+    VOX_TRY(buf(ctx, (t + "_wrank").c_str(), (size_t)nbt * nwords, &out.wrank));
 
     // matSplitRes subspaces over extPartition (src/host_side.cpp:1107-1111).
     SplitTable st;
@@ -343,44 +352,59 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
     }
     if (ctx->timing) ctx->times.surface_pairs += total;
     for (int attempt = 0; attempt < 3; attempt++) {
-        uint32_t cap = ctx->mat_cap_hint;
-        VOX_TRY(buf(ctx, (t + "_mats").c_str(), cap, &out.mat_list));
-        VOX_TRY(fill(ctx, stats, 4, 0u));
-        VOX_TRY(fill(ctx, mat_cursor, 1, 0u));
+        const uint32_t slot_cap = ctx->mat_slot_cap;
+        VOX_TRY(buf(ctx, (t + "_mats").c_str(), (size_t)slot_cap * kMatSlots, &out.mat_list));
+        VOX_TRY(fill(ctx, stats, kMatSlots * kSlotStride + 1, 0u));
+        uint32_t *overflow = stats + kMatSlots * kSlotStride;
         k_surf_brick<<<nbt, 256, 0, ctx->stream>>>(p, dsplit, tri_info, brick_off, brick_hist,
                                                    blist, ctx->mats, bits, out.surf,
-                                                   out.brick_mat_off, out.mat_list, mat_cursor, cap,
-                                                   stats);
+                                                   out.brick_mat_off, out.wrank, out.mat_list,
+                                                   stats, slot_cap, overflow);
         VOX_HIP(hipGetLastError());
-        uint32_t s[2];
-        VOX_TRY(read_u32(ctx, stats, 2, s));
-        if (ctx->timing) ctx->times.surface_voxels += s[1];
-        if (s[0] == 0u) return VOX_OK;
-        ctx->mat_cap_hint = s[1] + s[1] / 4 + 1024;
+        std::vector<uint32_t> h(kMatSlots * kSlotStride + 1);
+        VOX_TRY(read_u32(ctx, stats, h.size(), h.data()));
+        uint32_t used = 0;
+        uint64_t vox = 0;
+        for (uint32_t k = 0; k < kMatSlots; k++) {
+            used = std::max(used, h[k * kSlotStride]);
+            vox += h[k * kSlotStride + 1];
+        }
+        if (h.back() == 0u) {
+            if (ctx->timing) ctx->times.surface_voxels += (uint32_t)vox;
+            return VOX_OK;
+        }
+        ctx->mat_slot_cap = ((used + used / 4 + 1024u) + 3u) & ~3u;
     }
     return fail(ctx, VOX_ERR_OOM, "surface material list overflow");
 }
 
-// Erosion to the fixed point, per alloc y/z subspace in the reference's order.
+// Erosion to the fixed point, per alloc y/z subspace in the reference's order,
+// in place on `bits`.  Passes are launched kBatch at a time; a pass after one
+// that changed nothing exits at once, so one host read per batch suffices.
 vox_status erode(vox_ctx *ctx, const GridDims &gd, uint32_t splitY, uint32_t splitZ,
-                 uint32_t *&cur, uint32_t *other) {
-    const size_t words = (size_t)gd.W * gd.Y * gd.Z;
+                 uint32_t *bits) {
     constexpr uint32_t kBatch = 4;
-    uint32_t *changes;
+    const uint32_t nzc = (gd.Z + kErodeZ - 1) / kErodeZ;
+    const uint32_t nblocks = ((gd.W + kWave - 1) / kWave) * ((gd.Y + 3) / 4) * nzc;
+    uint32_t *changes, *stamp;
     VOX_TRY(buf(ctx, "e_changes", kBatch, &changes));
+    VOX_TRY(buf(ctx, "e_stamp", nblocks, &stamp));
+    VOX_TRY(fill(ctx, stamp, nblocks, 0u));
     std::vector<Span> ys = split_axis_max(gd.Y, splitY), zs = split_axis_max(gd.Z, splitZ);
-    uint32_t passes = 0;
+    uint32_t passes = 0, gpass = 0;
     for (const Span &zsp : zs)
         for (const Span &ysp : ys) {
+            bool first = true;
             for (int guard = 0; guard < 1000000; guard++) {
                 VOX_TRY(fill(ctx, changes, kBatch, 0u));
-                uint32_t *bufs[2] = {cur, other};
                 for (uint32_t k = 0; k < kBatch; k++) {
-                    k_erode<<<grid_for(words, 256, 32768), 256, 0, ctx->stream>>>(
-                        gd, bufs[k & 1], bufs[(k + 1) & 1], ysp.min, ysp.max, zsp.min, zsp.max,
-                        changes, k);
+                    k_erode<<<row_grid(gd.W, gd.Y, nzc), dim3(kWave, 4), 0, ctx->stream>>>(
+                        gd, bits, ysp.min, ysp.max, zsp.min, zsp.max, stamp, changes, k,
+                        gpass + k, (first && k == 0) ? 1 : 0);
                     VOX_HIP(hipGetLastError());
                 }
+                gpass += kBatch;
+                first = false;
                 uint32_t h[kBatch];
                 VOX_TRY(read_u32(ctx, changes, kBatch, h));
                 uint32_t k0 = kBatch;
@@ -391,14 +415,9 @@ vox_status erode(vox_ctx *ctx, const GridDims &gd, uint32_t splitY, uint32_t spl
                     }
                 if (k0 < kBatch) {
                     passes += k0 + 1;
-                    // Pass k0 changed nothing: its input equals its output.
-                    cur = bufs[k0 & 1];
-                    other = bufs[(k0 + 1) & 1];
                     break;
                 }
                 passes += kBatch;
-                cur = bufs[kBatch & 1];
-                other = bufs[(kBatch + 1) & 1];
             }
         }
     ctx->times.erosion_passes += passes;
@@ -429,7 +448,7 @@ vox_status vox_create(int device, vox_ctx **out) {
     vox_ctx *ctx = new vox_ctx();
     ctx->dev = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
-        hipHostMalloc((void **)&ctx->pinned, 4096, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&ctx->pinned, kPinnedBytes, hipHostMallocDefault) != hipSuccess) {
         delete ctx;
         return VOX_ERR_HIP;
     }
@@ -571,36 +590,14 @@ vox_status vox_set_materials_device(vox_ctx *ctx, const uint8_t *materials_dev,
     return VOX_OK;
 }
 
-// Bounding box on the device; dimensions on the host in FP64 exactly as
-// determineDimensions / adjustResolution compute them.
-vox_status vox_plan_grid(vox_ctx *ctx, vox_plan_mode mode, double param, uint32_t vsx,
-                         vox_grid *g) {
-    if (!ctx || !g || vsx == 0) return VOX_ERR_INVALID;
-    if (ctx->nv == 0) return fail(ctx, VOX_ERR_INVALID, "no vertices");
-    if (mode == VOX_PLAN_MAX_DIMENSION && (param < 2.0 || param > 4294967295.0))
-        return fail(ctx, VOX_ERR_INVALID, "maxDimension must be >= 2");
-    if (mode == VOX_PLAN_CUBE_LENGTH && !(param > 0.0))
-        return fail(ctx, VOX_ERR_INVALID, "cubeLength must be > 0");
-    VOX_HIP(hipSetDevice(ctx->dev));
-    uint32_t *bb;
-    VOX_TRY(buf(ctx, "p_bbox", 6, &bb));
-    VOX_TRY(fill(ctx, bb, 3, UINT32_MAX));
-    VOX_TRY(fill(ctx, bb + 3, 3, 0u));
-    k_bbox<<<grid_for(ctx->nv, 256, 256), 256, 0, ctx->stream>>>(ctx->V, ctx->nv, bb);
-    VOX_HIP(hipGetLastError());
-    uint32_t h[6];
-    VOX_TRY(read_u32(ctx, bb, 6, h));
-    double mn[3], mx[3];
-    for (int c = 0; c < 3; c++) {
-        uint32_t a = h[c], b = h[3 + c];
-        uint32_t ua = (a & 0x80000000u) ? (a & 0x7FFFFFFFu) : ~a;
-        uint32_t ub = (b & 0x80000000u) ? (b & 0x7FFFFFFFu) : ~b;
-        float fa, fb;
-        std::memcpy(&fa, &ua, 4);
-        std::memcpy(&fb, &ub, 4);
-        mn[c] = fa;
-        mx[c] = fb;
-    }
+namespace {
+
+// Dimensions in FP64 exactly as determineDimensions() /
+// determineDimensions(double) (src/host_side.cpp:2497-2609) and
+// splitResolutionByMaxDim + adjustResolution (src/host_side.cpp:730-738,
+// 2620-2634) compute them from the bounding box.
+void plan_from_bbox(const double mn[3], const double mx[3], vox_plan_mode mode, double param,
+                    uint32_t vsx, vox_grid *g) {
     auto ceil_u = [](double v) -> uint32_t {
         double c = std::ceil(v);
         if (c != c || c <= -1.0) return 0u;
@@ -657,6 +654,64 @@ vox_status vox_plan_grid(vox_ctx *ctx, vox_plan_mode mode, double param, uint32_
     g->alloc[2] = rz + 2;
     g->x_splits = xs;
     g->pad_ = 0;
+}
+
+bool plan_args_ok(vox_plan_mode mode, double param, uint32_t vsx) {
+    if (vsx == 0) return false;
+    if (mode == VOX_PLAN_MAX_DIMENSION) return param >= 2.0 && param <= 4294967295.0;
+    if (mode == VOX_PLAN_CUBE_LENGTH) return param > 0.0;
+    return false;
+}
+
+}  // namespace
+
+// Bounding box on the device (calculateBoundingBox, src/host_side.cpp:
+// 2462-2488, as an ordered-key min/max reduction); dimensions on the host.
+vox_status vox_plan_grid(vox_ctx *ctx, vox_plan_mode mode, double param, uint32_t vsx,
+                         vox_grid *g) {
+    if (!ctx || !g || vsx == 0) return VOX_ERR_INVALID;
+    if (ctx->nv == 0) return fail(ctx, VOX_ERR_INVALID, "no vertices");
+    if (!plan_args_ok(mode, param, vsx))
+        return fail(ctx, VOX_ERR_INVALID,
+                    mode == VOX_PLAN_MAX_DIMENSION ? "maxDimension must be >= 2"
+                                                   : "cubeLength must be > 0");
+    VOX_HIP(hipSetDevice(ctx->dev));
+    uint32_t *bb;
+    VOX_TRY(buf(ctx, "p_bbox", 6, &bb));
+    VOX_TRY(fill(ctx, bb, 3, UINT32_MAX));
+    VOX_TRY(fill(ctx, bb + 3, 3, 0u));
+    k_bbox<<<grid_for(ctx->nv, 256, 256), 256, 0, ctx->stream>>>(ctx->V, ctx->nv, bb);
+    VOX_HIP(hipGetLastError());
+    uint32_t h[6];
+    VOX_TRY(read_u32(ctx, bb, 6, h));
+    double mn[3], mx[3];
+    for (int c = 0; c < 3; c++) {
+        uint32_t a = h[c], b = h[3 + c];
+        uint32_t ua = (a & 0x80000000u) ? (a & 0x7FFFFFFFu) : ~a;
+        uint32_t ub = (b & 0x80000000u) ? (b & 0x7FFFFFFFu) : ~b;
+        float fa, fb;
+        std::memcpy(&fa, &ua, 4);
+        std::memcpy(&fb, &ub, 4);
+        mn[c] = fa;
+        mx[c] = fb;
+    }
+    plan_from_bbox(mn, mx, mode, param, vsx, g);
+    return VOX_OK;
+}
+
+// Host bounding box with the reference's strict comparisons
+// (src/host_side.cpp:2462-2488); no device is touched.
+vox_status vox_plan_grid_host(const float *v, uint32_t nv, vox_plan_mode mode, double param,
+                              uint32_t vsx, vox_grid *g) {
+    if (!v || !g || nv == 0 || !plan_args_ok(mode, param, vsx)) return VOX_ERR_INVALID;
+    double mn[3] = {v[0], v[1], v[2]}, mx[3] = {v[0], v[1], v[2]};
+    for (uint32_t i = 1; i < nv; i++)
+        for (int c = 0; c < 3; c++) {
+            const double x = v[3 * (size_t)i + c];
+            if (x > mx[c]) mx[c] = x;
+            if (x < mn[c]) mn[c] = x;
+        }
+    plan_from_bbox(mn, mx, mode, param, vsx, g);
     return VOX_OK;
 }
 
@@ -732,9 +787,8 @@ vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *g, vox_node_kind kin
     GridDims gd{g->alloc[0], g->alloc[1], g->alloc[2], g->alloc[0] / 32};
     record(ctx, 0);
     if (!fcc) {
-        uint32_t *bits, *tmp;
+        uint32_t *bits;
         VOX_TRY(buf(ctx, "n_bits", words, &bits));
-        VOX_TRY(buf(ctx, "n_bits_tmp", words, &tmp));
         VOX_TRY(solid_grid(ctx, *g, base, vs[1], vs[2], bits, "g1"));
         record(ctx, 1);
         SurfOut so{};
@@ -761,15 +815,16 @@ vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *g, vox_node_kind kin
             VOX_HIP(hipMemcpyAsync(zstart, fz.data(), fz.size(), hipMemcpyHostToDevice, ctx->stream));
             VOX_HIP(hipStreamSynchronize(ctx->stream));
         }
-        uint32_t *cur = bits;
-        VOX_TRY(erode(ctx, gd, vs[1], vs[2], cur, tmp));
+        VOX_TRY(erode(ctx, gd, vs[1], vs[2], bits));
         record(ctx, 3);
         const uint32_t nbx = (g->alloc[0] + kCartBX - 1) / kCartBX;
         const uint32_t nby = (g->alloc[1] + kCartBY - 1) / kCartBY;
         const uint32_t nbz = (g->alloc[2] + kCartBZ - 1) / kCartBZ;
-        MatSource msrc{so.surf, so.brick_mat_off, so.mat_list};
-        k_nodes_cart<<<nbx * nby * nbz, 128, 0, ctx->stream>>>(
-            gd, cur, pre, ystart, zstart, msrc, use_materials ? 1 : 0,
+        (void)nbz;
+        MatSource msrc{so.surf, so.brick_mat_off, so.wrank, so.mat_list};
+        k_nodes_cart<<<row_grid(gd.W, gd.Y, (gd.Z + kNodeZ - 1) / kNodeZ), dim3(kWave, 4), 0,
+                       ctx->stream>>>(
+            gd, bits, pre, ystart, zstart, msrc, use_materials ? 1 : 0,
             kind == VOX_NODE_SHORT ? 0 : 1, nodes_dev, nbx, nby);
         VOX_HIP(hipGetLastError());
         record(ctx, 4);
@@ -804,8 +859,10 @@ vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *g, vox_node_kind kin
         const uint32_t nby = (g->alloc[1] + kFccBY - 1) / kFccBY;
         const uint32_t nbz = (g->alloc[2] + kFccBZ - 1) / kFccBZ;
         MatSource m[4];
-        for (int k = 0; k < 4; k++) m[k] = MatSource{so[k].surf, so[k].brick_mat_off, so[k].mat_list};
-        k_nodes_fcc<<<nbx * nby * nbz, 128, 0, ctx->stream>>>(
+        for (int k = 0; k < 4; k++)
+            m[k] = MatSource{so[k].surf, so[k].brick_mat_off, so[k].wrank, so[k].mat_list};
+        (void)nbz;
+        k_nodes_fcc<<<row_grid(2 * gd.W, gd.Y, 2 * gd.Z), dim3(kWave, 4), 0, ctx->stream>>>(
             gd, gb[0], gb[1], gb[2], gb[3], m[0], m[1], m[2], m[3], use_materials ? 1 : 0,
             kind == VOX_NODE_SHORT_FCC ? 3 : 4, nodes_dev, nbx, nby);
         VOX_HIP(hipGetLastError());

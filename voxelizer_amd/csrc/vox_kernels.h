@@ -17,11 +17,47 @@
 namespace vk {
 
 constexpr int kWave = 64;
+constexpr uint32_t kXcds = 8;  // MI355X: 8 XCDs, each with its own L2
+
+// Workgroups are dispatched round-robin over the XCDs.  Row kernels launch
+// kXcds * ceil(n / kXcds) workgroups and map workgroup i to logical block
+// (i % kXcds) * per + i / kXcds, so each XCD walks a contiguous run of rows
+// and planes and its L2 serves the neighbouring-row reads.  Returns
+// UINT32_MAX for the padding workgroups.
+__device__ __forceinline__ uint32_t xcd_logical_block(uint32_t n) {
+    const uint32_t per = (n + kXcds - 1) / kXcds;
+    const uint32_t L = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    return L < n ? L : UINT32_MAX;
+}
+inline uint32_t xcd_grid(uint32_t n) { return kXcds * ((n + kXcds - 1) / kXcds); }
+
+// Logical row block -> (word chunk, 4-row group, plane).
+struct RowBlock {
+    uint32_t xc, y4, z;
+};
+__device__ __forceinline__ RowBlock row_block(uint32_t L, uint32_t nxc, uint32_t ny4) {
+    RowBlock r;
+    r.xc = L % nxc;
+    const uint32_t q = L / nxc;
+    r.y4 = q % ny4;
+    r.z = q / ny4;
+    return r;
+}
 
 // Cartesian bricks (node pass + surface) and FCC per-grid bricks.
 constexpr int kCartBX = 128, kCartBY = 4, kCartBZ = 8;
 constexpr int kFccBX = 64, kFccBY = 4, kFccBZ = 4;
 constexpr int kMaxBrickVox = 4096;
+// z planes per thread in the erosion pass (a block covers 64 words x 4 rows
+// x kErodeZ planes).
+constexpr uint32_t kErodeZ = 8;
+// z planes per thread in the Cartesian node pass; divides kCartBZ so a
+// thread's planes share one surface brick.
+constexpr uint32_t kNodeZ = 4;
+static_assert(kCartBZ % kNodeZ == 0, "node planes must not straddle a brick");
+// Material-list arenas of the surface stage: kMatSlots counters, each on its
+// own 128-byte line (kSlotStride words): {bytes used, surface voxels}.
+constexpr uint32_t kMatSlots = 64, kSlotStride = 32;
 constexpr int kMaxSplits = 64;
 
 struct SplitTable {
@@ -59,6 +95,34 @@ __device__ __forceinline__ uint32_t wave_bump(uint32_t *counter, bool pred) {
     base = __shfl(base, (int)leader);
     uint32_t below = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
     return base + below;
+}
+
+// Block-level grouping of keys in an LDS hash table (256-thread blocks):
+// candidates that land in the same bin share one global atomic per block.
+constexpr uint32_t kAggSlots = 1024;
+constexpr uint32_t kAggEmpty = 0xFFFFFFFFu;
+struct AggTable {
+    uint32_t key[kAggSlots];
+    uint32_t cnt[kAggSlots];
+    uint32_t base[kAggSlots];
+};
+__device__ __forceinline__ void agg_clear(AggTable &T) {
+    for (uint32_t i = threadIdx.x; i < kAggSlots; i += blockDim.x) {
+        T.key[i] = kAggEmpty;
+        T.cnt[i] = 0u;
+    }
+}
+// Inserts key; returns its slot and, in rank, its 0-based order among the
+// block's items with the same key.
+__device__ __forceinline__ uint32_t agg_insert(AggTable &T, uint32_t key, uint32_t &rank) {
+    uint32_t h = (key * 2654435761u) >> 22;  // 10 bits
+    while (true) {
+        const uint32_t prev = atomicCAS(&T.key[h], kAggEmpty, key);
+        if (prev == kAggEmpty || prev == key) break;
+        h = (h + 1u) & (kAggSlots - 1u);
+    }
+    rank = atomicAdd(&T.cnt[h], 1u);
+    return h;
 }
 
 __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *__restrict__ a, uint32_t n,
@@ -105,13 +169,14 @@ struct SolidParams {
     uint32_t rowsY;             // alloc y (= ext y + 2)
 };
 
-__global__ void k_solid_setup(SolidParams p, uint32_t *__restrict__ cand_count);
+__global__ void k_solid_setup(SolidParams p, uint32_t *__restrict__ cand_count,
+                              uint4 *__restrict__ cbox);
 __global__ void k_solid_candidates(SolidParams p, const uint32_t *__restrict__ cand_off,
                                    const uint32_t *__restrict__ cand_total,
-                                   uint2 *__restrict__ pairs, uint32_t *__restrict__ pair_count,
-                                   uint32_t pair_cap, uint32_t *__restrict__ tile_hist);
+                                   const uint4 *__restrict__ cbox, uint2 *__restrict__ pairs,
+                                   uint32_t *__restrict__ tile_hist);
 __global__ void k_solid_scatter(const uint2 *__restrict__ pairs,
-                                const uint32_t *__restrict__ pair_count,
+                                const uint32_t *__restrict__ cand_total,
                                 const uint32_t *__restrict__ tile_off,
                                 uint32_t *__restrict__ tile_cursor, uint32_t *__restrict__ list);
 __global__ void k_solid_fill(SolidParams p, const uint32_t *__restrict__ tile_off,
@@ -139,8 +204,9 @@ __global__ void k_surf_brick(SurfParams p, const SplitTable *__restrict__ splits
                              const uint32_t *__restrict__ blist,
                              const uint8_t *__restrict__ materials, uint32_t *__restrict__ bits,
                              uint32_t *__restrict__ surf, uint32_t *__restrict__ brick_mat_off,
-                             uint8_t *__restrict__ mat_list, uint32_t *__restrict__ mat_cursor,
-                             uint32_t mat_cap, uint32_t *__restrict__ stats);
+                             uint16_t *__restrict__ wrank, uint8_t *__restrict__ mat_list,
+                             uint32_t *__restrict__ slots, uint32_t slot_cap,
+                             uint32_t *__restrict__ overflow);
 
 // ---------------------------------------------------------------------------
 // Nodes
@@ -150,9 +216,9 @@ struct GridDims {
     uint32_t W;        // X / 32
 };
 
-__global__ void k_erode(GridDims g, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
-                        uint32_t y0, uint32_t y1, uint32_t z0, uint32_t z1,
-                        uint32_t *__restrict__ changes, uint32_t pass);
+__global__ void k_erode(GridDims g, uint32_t *__restrict__ bits, uint32_t y0, uint32_t y1,
+                        uint32_t z0, uint32_t z1, uint32_t *__restrict__ stamp,
+                        uint32_t *__restrict__ changes, uint32_t pass, uint32_t gpass, int full);
 
 // Boundary-id table (permutation -> id) in constant memory.
 hipError_t set_orient_table(const uint8_t lut[64]);
@@ -160,6 +226,7 @@ hipError_t set_orient_table(const uint8_t lut[64]);
 struct MatSource {
     const uint32_t *surf;          // surface bit words (same layout as bits)
     const uint32_t *brick_mat_off; // per brick offset into mat_list or UINT32_MAX
+    const uint16_t *wrank;         // per brick word: byte offset of its material segment in the brick
     const uint8_t *mat_list;
 };
 

@@ -113,65 +113,105 @@ __device__ __forceinline__ void load_tri_d(const SolidParams &p, uint32_t t, D3 
     v[2] = fetch_d(p.V, p.I, t, 2);
 }
 
-// Candidate tiles of each triangle: its commonSpace tile rectangle.
+// Candidate tiles of each triangle: its commonSpace tile rectangle; the
+// commonSpace box itself is kept so the candidate pass needs no divisions.
 __global__ __launch_bounds__(256) void k_solid_setup(SolidParams p,
-                                                     uint32_t *__restrict__ cand_count) {
+                                                     uint32_t *__restrict__ cand_count,
+                                                     uint4 *__restrict__ cbox) {
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < p.nt; t += gridDim.x * blockDim.x) {
         D3 v[3];
         load_tri_d(p, t, v);
         TileSetup ts;
         uint32_t c = 0;
-        if (tile_setup(v, p.emy, p.emz, p.d, p.smin[0], p.smin[1], p.smax[0], p.smax[1], ts))
+        if (tile_setup(v, p.emy, p.emz, p.d, p.smin[0], p.smin[1], p.smax[0], p.smax[1], ts)) {
             c = (ts.tmax[0] - ts.tmin[0] + 1u) * (ts.tmax[1] - ts.tmin[1] + 1u);
+            cbox[t] = make_uint4(ts.cmin[0], ts.cmin[1], ts.cmax[0], ts.cmax[1]);
+        }
         cand_count[t] = c;
     }
 }
 
 // One thread per candidate (triangle, tile): the reference's per-tile queue
 // test, done in parallel instead of one thread walking all tiles of a
-// triangle.  Queued pairs are appended with a wave-aggregated bump and
-// counted per tile for the counting sort.
+// triangle.  The triangle of a candidate is found by a binary search narrowed
+// to the triangles the block's candidates span.  Queued pairs are counted per
+// tile (LDS-grouped atomics) for the counting sort.
 __global__ __launch_bounds__(256) void k_solid_candidates(
     SolidParams p, const uint32_t *__restrict__ cand_off, const uint32_t *__restrict__ cand_total,
-    uint2 *__restrict__ pairs, uint32_t *__restrict__ pair_count, uint32_t pair_cap,
-    uint32_t *__restrict__ tile_hist) {
+    const uint4 *__restrict__ cbox, uint2 *__restrict__ pairs, uint32_t *__restrict__ tile_hist) {
+    __shared__ uint32_t trange[2];
+    __shared__ AggTable agg;
     const uint32_t total = *cand_total;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
+        __syncthreads();
+        agg_clear(agg);
+        if (threadIdx.x < 2) {
+            const uint32_t key = threadIdx.x == 0 ? base : min(base + blockDim.x, total) - 1u;
+            trange[threadIdx.x] = upper_bound_u32(cand_off, p.nt, key) - 1u;
+        }
+        __syncthreads();
         const uint32_t c = base + threadIdx.x;
         bool hit = false;
         uint32_t t = 0, tile = 0;
         if (c < total) {
-            t = upper_bound_u32(cand_off, p.nt, c) - 1u;
+            uint32_t lo = trange[0], hi = trange[1] + 1u;
+            while (lo + 1u < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (cand_off[mid] <= c) lo = mid;
+                else hi = mid;
+            }
+            t = lo;
             D3 v[3];
             load_tri_d(p, t, v);
             TileSetup ts;
-            if (tile_setup(v, p.emy, p.emz, p.d, p.smin[0], p.smin[1], p.smax[0], p.smax[1], ts)) {
-                const uint32_t w = ts.tmax[0] - ts.tmin[0] + 1u;
-                const uint32_t local = c - cand_off[t];
-                const uint32_t ty = ts.tmin[0] + local % w, tz = ts.tmin[1] + local / w;
-                hit = tile_hit(ts, ty, tz, p.emy, p.emz, p.d);
-                tile = tz * p.tilesY + ty;
-            }
+            const uint4 cb = cbox[t];
+            ts.cmin[0] = cb.x;
+            ts.cmin[1] = cb.y;
+            ts.cmax[0] = cb.z;
+            ts.cmax[1] = cb.w;
+            tile_edges(v, ts);
+            const uint32_t tmin0 = cb.x / 4u, tmin1 = cb.y / 4u, tmax0 = (cb.z - 1u) / 4u;
+            const uint32_t w = tmax0 - tmin0 + 1u;
+            const uint32_t local = c - cand_off[t];
+            const uint32_t ty = tmin0 + local % w, tz = tmin1 + local / w;
+            hit = tile_hit(ts, ty, tz, p.emy, p.emz, p.d);
+            tile = tz * p.tilesY + ty;
         }
-        const uint32_t pos = wave_bump(pair_count, hit);
-        if (hit) {
-            if (pos < pair_cap) pairs[pos] = make_uint2(t, tile);
-            atomicAdd(&tile_hist[tile], 1u);
-        }
+        // One slot per candidate (no global counter); misses are skipped
+        // by the scatter pass.
+        if (c < total) pairs[c] = make_uint2(t, hit ? tile : UINT32_MAX);
+        __syncthreads();
+        uint32_t rank;
+        if (hit) agg_insert(agg, tile, rank);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < kAggSlots; i += blockDim.x)
+            if (agg.key[i] != kAggEmpty) atomicAdd(&tile_hist[agg.key[i]], agg.cnt[i]);
     }
 }
 
 __global__ __launch_bounds__(256) void k_solid_scatter(const uint2 *__restrict__ pairs,
-                                                       const uint32_t *__restrict__ pair_count,
+                                                       const uint32_t *__restrict__ cand_total,
                                                        const uint32_t *__restrict__ tile_off,
                                                        uint32_t *__restrict__ tile_cursor,
                                                        uint32_t *__restrict__ list) {
-    const uint32_t n = *pair_count;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint2 pr = pairs[i];
-        uint32_t pos = tile_off[pr.y] + atomicAdd(&tile_cursor[pr.y], 1u);
-        list[pos] = pr.x;
+    __shared__ AggTable agg;
+    const uint32_t n = *cand_total;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        __syncthreads();
+        agg_clear(agg);
+        __syncthreads();
+        const uint32_t i = base + threadIdx.x;
+        uint2 pr = make_uint2(0u, UINT32_MAX);
+        uint32_t slot = 0, rank = 0;
+        if (i < n) pr = pairs[i];
+        const bool live = pr.y != UINT32_MAX;
+        if (live) slot = agg_insert(agg, pr.y, rank);
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < kAggSlots; k += blockDim.x)
+            if (agg.key[k] != kAggEmpty) agg.base[k] = atomicAdd(&tile_cursor[agg.key[k]], agg.cnt[k]);
+        __syncthreads();
+        if (live) list[tile_off[pr.y] + agg.base[slot] + rank] = pr.x;
     }
 }
 
@@ -211,28 +251,30 @@ __global__ __launch_bounds__(kWave) void k_solid_fill(SolidParams p,
         }
     }
     __syncthreads();
-    // Crossing parity -> solid bits (MSB-first bits: x%32 == 0 is bit 31).
-    if (tid < 16u) {
-        uint32_t carry = 0u;
-        uint32_t *row = cross + tid * W;
-        for (uint32_t w = 0; w < W; w++) {
-            uint32_t c = row[w];
-            uint32_t s = c;
-            s ^= s >> 1;
-            s ^= s >> 2;
-            s ^= s >> 4;
-            s ^= s >> 8;
-            s ^= s >> 16;
-            row[w] = s ^ carry;
-            carry ^= (__popc(c) & 1u) ? 0xFFFFFFFFu : 0u;
-        }
-    }
-    __syncthreads();
+    // Crossing parity -> solid bits (MSB-first bits: x%32 == 0 is bit 31):
+    // a prefix XOR inside each word, and across words the parity of the
+    // crossings before it (one ballot per 64 words), written straight out.
+    const uint32_t lane = tid;
     for (uint32_t r = 0; r < 16u; r++) {
         const uint32_t cy = ty * 4u + (r & 3u), cz = tz * 4u + (r >> 2);
         if (cy < p.smin[0] || cy >= p.smax[0] || cz < p.smin[1] || cz >= p.smax[1]) continue;
         const size_t rowIdx = (size_t)(cz + 1u) * p.rowsY + (cy + 1u);
-        for (uint32_t w = tid; w < W; w += kWave) bits[rowIdx * W + w] = cross[r * W + w];
+        uint32_t carry = 0u;
+        for (uint32_t w0 = 0; w0 < W; w0 += kWave) {
+            const uint32_t w = w0 + lane;
+            const uint32_t c = w < W ? cross[r * W + w] : 0u;
+            uint32_t sx = c;
+            sx ^= sx >> 1;
+            sx ^= sx >> 2;
+            sx ^= sx >> 4;
+            sx ^= sx >> 8;
+            sx ^= sx >> 16;
+            const unsigned long long odd = __ballot((__popc(c) & 1) != 0);
+            const unsigned long long below = lane ? (odd & (~0ull >> (64u - lane))) : 0ull;
+            const uint32_t before = carry ^ ((uint32_t)__popcll(below) & 1u);
+            if (w < W) bits[rowIdx * W + w] = sx ^ (before ? 0xFFFFFFFFu : 0u);
+            carry ^= (uint32_t)__popcll(odd) & 1u;
+        }
     }
 }
 
@@ -327,37 +369,48 @@ __global__ __launch_bounds__(256) void k_surf_bin(
     const uint32_t *__restrict__ brick_off, uint32_t *__restrict__ brick_cursor,
     uint32_t *__restrict__ blist, int scatter) {
     __shared__ uint32_t trange[2];
+    __shared__ AggTable agg;
     const uint32_t total = *cand_total;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
         __syncthreads();
+        agg_clear(agg);
         if (threadIdx.x < 2) {
             const uint32_t key = threadIdx.x == 0 ? base : min(base + blockDim.x, total) - 1u;
             trange[threadIdx.x] = upper_bound_u32(cand_off, p.nt, key) - 1u;
         }
         __syncthreads();
         const uint32_t c = base + threadIdx.x;
-        if (c >= total) continue;
+        const bool live = c < total;
         uint32_t lo = trange[0], hi = trange[1] + 1u;
+        if (!live) lo = hi = 0u;
         while (lo + 1u < hi) {
             const uint32_t mid = (lo + hi) >> 1;
             if (cand_off[mid] <= c) lo = mid;
             else hi = mid;
         }
         const uint32_t t = lo;
-        const uint4 ti = tri_info[t];
-        const uint32_t bx0 = ti.y & 0xFFFFu, by0 = ti.y >> 16, bz0 = ti.z & 0xFFFFu;
-        const uint32_t w0 = ti.z >> 16, w1 = ti.w & 0xFFFFu;
-        const uint32_t local = c - cand_off[t];
-        const uint32_t bx = bx0 + local % w0;
-        const uint32_t by = by0 + (local / w0) % w1;
-        const uint32_t bz = bz0 + local / (w0 * w1);
-        const uint32_t b = (bz * p.nb[1] + by) * p.nb[0] + bx;
-        if (!scatter) {
-            atomicAdd(&brick_hist[b], 1u);
-        } else {
-            uint32_t pos = brick_off[b] + atomicAdd(&brick_cursor[b], 1u);
-            blist[pos] = t;
+        uint32_t b = 0, slot = 0, rank = 0;
+        if (live) {
+            const uint4 ti = tri_info[t];
+            const uint32_t bx0 = ti.y & 0xFFFFu, by0 = ti.y >> 16, bz0 = ti.z & 0xFFFFu;
+            const uint32_t w0 = ti.z >> 16, w1 = ti.w & 0xFFFFu;
+            const uint32_t local = c - cand_off[t];
+            const uint32_t bx = bx0 + local % w0;
+            const uint32_t by = by0 + (local / w0) % w1;
+            const uint32_t bz = bz0 + local / (w0 * w1);
+            b = (bz * p.nb[1] + by) * p.nb[0] + bx;
+            slot = agg_insert(agg, b, rank);
         }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < kAggSlots; k += blockDim.x) {
+            if (agg.key[k] == kAggEmpty) continue;
+            if (!scatter)
+                atomicAdd(&brick_hist[agg.key[k]], agg.cnt[k]);
+            else
+                agg.base[k] = atomicAdd(&brick_cursor[agg.key[k]], agg.cnt[k]);
+        }
+        __syncthreads();
+        if (live && scatter) blist[brick_off[b] + agg.base[slot] + rank] = t;
     }
 }
 
@@ -500,14 +553,14 @@ __global__ __launch_bounds__(256) void k_surf_brick(
     const uint32_t *__restrict__ brick_off, const uint32_t *__restrict__ brick_hist,
     const uint32_t *__restrict__ blist, const uint8_t *__restrict__ materials,
     uint32_t *__restrict__ bits, uint32_t *__restrict__ surf, uint32_t *__restrict__ brick_mat_off,
-    uint8_t *__restrict__ mat_list, uint32_t *__restrict__ mat_cursor, uint32_t mat_cap,
-    uint32_t *__restrict__ stats) {
+    uint16_t *__restrict__ wrank, uint8_t *__restrict__ mat_list, uint32_t *__restrict__ slots,
+    uint32_t slot_cap, uint32_t *__restrict__ overflow) {
     __shared__ uint32_t keys[kMaxBrickVox];
     __shared__ SurfTri tris[kSurfChunk];
     __shared__ uint32_t colsum[kSurfChunk + 1];
     __shared__ uint32_t wsurf[kMaxBrickVox / 32];
     __shared__ uint32_t wexcl[kMaxBrickVox / 32];
-    __shared__ uint32_t sbase;
+    __shared__ uint32_t sbase, nsurf;
     __shared__ SplitTable spl;
     const uint32_t b = blockIdx.x;
     const uint32_t tid = threadIdx.x;
@@ -569,10 +622,15 @@ __global__ __launch_bounds__(256) void k_surf_brick(
             wsurf[v0 / 32 + 1] = __brev((uint32_t)(m >> 32));
         }
     }
+    if (tid == 0) nsurf = 0u;
     __syncthreads();
-    // Exclusive popcount scan over the brick's words.
-    uint32_t pc = tid < (uint32_t)nwords ? (uint32_t)__popc(wsurf[tid]) : 0u;
+    // Exclusive scan of the words' material segment sizes: each word's
+    // materials start 4-byte aligned (popcount rounded up to 4) so the node
+    // pass fetches them as whole dwords.
+    const uint32_t pop = tid < (uint32_t)nwords ? (uint32_t)__popc(wsurf[tid]) : 0u;
+    const uint32_t pc = (pop + 3u) & ~3u;
     if (tid < (uint32_t)nwords) wexcl[tid] = pc;
+    if (pop) atomicAdd(&nsurf, pop);
     __syncthreads();
     for (uint32_t d = 1; d < (uint32_t)nwords; d <<= 1) {
         uint32_t o = (tid < (uint32_t)nwords && tid >= d) ? wexcl[tid - d] : 0u;
@@ -581,15 +639,19 @@ __global__ __launch_bounds__(256) void k_surf_brick(
         __syncthreads();
     }
     if (tid == (uint32_t)nwords - 1) {
+        // Material space comes from one of kMatSlots arenas (brick b uses
+        // arena b % kMatSlots), so concurrent bricks bump different counters.
         const uint32_t acc = wexcl[tid];
-        uint32_t base = atomicAdd(mat_cursor, acc);
-        if (base + acc > mat_cap) {
-            atomicOr(&stats[0], 1u);  // overflow: the caller retries with more room
+        const uint32_t sl = b % kMatSlots;
+        const uint32_t local = atomicAdd(&slots[sl * kSlotStride], acc);
+        atomicAdd(&slots[sl * kSlotStride + 1], nsurf);
+        uint32_t base = sl * slot_cap + local;
+        if (local + acc > slot_cap) {
+            atomicOr(overflow, 1u);  // the caller retries with larger arenas
             base = UINT32_MAX;
         }
         sbase = base;
         brick_mat_off[b] = base;
-        atomicAdd(&stats[1], acc);
     }
     __syncthreads();
     const uint32_t base = sbase;
@@ -602,10 +664,12 @@ __global__ __launch_bounds__(256) void k_surf_brick(
         if (aw >= p.alloc[0] / 32u || ay >= p.alloc[1] || az >= p.alloc[2]) continue;
         const size_t gi = ((size_t)az * p.alloc[1] + ay) * (p.alloc[0] / 32u) + aw;
         const uint32_t s = wsurf[w];
+        const uint32_t seg = ((uint32_t)__popc(s) + 3u) & ~3u;
         surf[gi] = s;
+        wrank[(size_t)b * nwords + w] = (uint16_t)(wexcl[w] - seg);
         if (s) bits[gi] = bits[gi] | s;
         if (base != UINT32_MAX && s) {
-            uint32_t r = base + wexcl[w] - (uint32_t)__popc(s);
+            uint32_t r = base + wexcl[w] - seg;
             uint32_t m = s;
             while (m) {
                 const int k = __clz(m);
@@ -618,47 +682,98 @@ __global__ __launch_bounds__(256) void k_surf_brick(
 }
 
 // ---------------------------------------------------------------------------
-// Erosion: one snapshot pass of "solid and a solid neighbour on every axis"
-// over bit words (the invalid-permutation removal of src/voxelizer.cu:
-// 1791-1913, iterated to its fixed point by the host as in
-// src/host_side.cpp:1183-1206).  Words outside [y0,y1) x [z0,z1) are copied.
+// Erosion: removal of solid voxels that lack a solid neighbour on some axis
+// (the invalid-permutation removal of src/voxelizer.cu:1791-1913), repeated
+// until a pass removes nothing (src/host_side.cpp:1183-1206).  The rule is
+// monotone and only removes, so passes may run in place and in any order:
+// every such (chaotic) iteration reaches the same greatest fixed point as
+// the reference's.  The first pass of a subspace reads the whole grid; a
+// later pass only visits row blocks that changed, or whose face-adjacent
+// blocks changed, since the previous pass (stamp = 1 + last pass with a
+// change), and a word is written only when it changes.  Voxels outside
+// [y0,y1) x [z0,z1) are left alone.  Row-major launch: block (64, 4) = 4 rows
+// of up to 64 words.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t ldw(const uint32_t *__restrict__ a, const GridDims &g,
-                                        int64_t w, int64_t y, int64_t z) {
-    if (w < 0 || w >= (int64_t)g.W || y < 0 || y >= (int64_t)g.Y || z < 0 || z >= (int64_t)g.Z)
-        return 0u;
-    return a[((size_t)z * g.Y + (size_t)y) * g.W + (size_t)w];
-}
-
-__global__ __launch_bounds__(256) void k_erode(GridDims g, const uint32_t *__restrict__ src,
-                                               uint32_t *__restrict__ dst, uint32_t y0,
-                                               uint32_t y1, uint32_t z0, uint32_t z1,
-                                               uint32_t *__restrict__ changes, uint32_t pass) {
+__global__ __launch_bounds__(256) void k_erode(GridDims g, uint32_t *__restrict__ bits,
+                                               uint32_t y0, uint32_t y1, uint32_t z0,
+                                               uint32_t z1, uint32_t *__restrict__ stamp,
+                                               uint32_t *__restrict__ changes, uint32_t pass,
+                                               uint32_t gpass, int full) {
     if (pass > 0 && changes[pass - 1] == 0u) return;
-    const size_t total = (size_t)g.W * g.Y * g.Z;
-    for (size_t base = (size_t)blockIdx.x * blockDim.x; base < total;
-         base += (size_t)gridDim.x * blockDim.x) {
-        const size_t i = base + threadIdx.x;
-        bool changed = false;
-        if (i < total) {
-            const int64_t w = (int64_t)(i % g.W);
-            const int64_t y = (int64_t)((i / g.W) % g.Y);
-            const int64_t z = (int64_t)(i / ((size_t)g.W * g.Y));
-            const uint32_t G = src[i];
-            uint32_t out = G;
-            if (G && (uint64_t)y >= y0 && (uint64_t)y < y1 && (uint64_t)z >= z0 &&
-                (uint64_t)z < z1) {
-                const uint32_t L = (G >> 1) | (ldw(src, g, w - 1, y, z) << 31);
-                const uint32_t R = (G << 1) | (ldw(src, g, w + 1, y, z) >> 31);
-                const uint32_t Yn = ldw(src, g, w, y - 1, z) | ldw(src, g, w, y + 1, z);
-                const uint32_t Zn = ldw(src, g, w, y, z - 1) | ldw(src, g, w, y, z + 1);
-                out = G & (L | R) & Yn & Zn;
-                changed = out != G;
-            }
-            dst[i] = out;
+    const uint32_t nxc = (g.W + kWave - 1) / kWave, ny4 = (g.Y + 3) / 4;
+    const uint32_t nzc = (g.Z + kErodeZ - 1) / kErodeZ;
+    const uint32_t L = xcd_logical_block(nxc * ny4 * nzc);
+    if (L == UINT32_MAX) return;
+    const RowBlock rb = row_block(L, nxc, ny4);
+    const uint32_t zb = rb.z * kErodeZ;
+    if (zb >= z1 || zb + kErodeZ <= z0 || rb.y4 * 4 + 4 <= y0 || rb.y4 * 4 >= y1) return;
+    if (!full) {
+        const uint32_t plane = nxc * ny4;
+        bool visit = stamp[L] >= gpass;
+        if (rb.xc > 0) visit |= stamp[L - 1] >= gpass;
+        if (rb.xc + 1 < nxc) visit |= stamp[L + 1] >= gpass;
+        if (rb.y4 > 0) visit |= stamp[L - nxc] >= gpass;
+        if (rb.y4 + 1 < ny4) visit |= stamp[L + nxc] >= gpass;
+        if (rb.z > 0) visit |= stamp[L - plane] >= gpass;
+        if (rb.z + 1 < nzc) visit |= stamp[L + plane] >= gpass;
+        if (!visit) return;
+    }
+    const uint32_t tx = threadIdx.x;
+    const uint32_t w = rb.xc * kWave + tx;
+    const uint32_t y = rb.y4 * 4 + threadIdx.y;
+    const bool valid = w < g.W && y < g.Y;
+    const size_t rowW = g.W, planeW = (size_t)g.W * g.Y;
+    const size_t col0 = (size_t)y * rowW + w;  // index of (w, y) in plane 0
+    const bool hyn = valid && y > 0, hyp = valid && y + 1 < g.Y;
+    // Column z = zb-1 .. zb+kErodeZ of this word, the rows above and below it
+    // for the thread's planes, and the words across the chunk edges: all
+    // loads are independent and in flight together (clamped addresses).
+    uint32_t col[kErodeZ + 2], yn[kErodeZ], yp[kErodeZ], em[kErodeZ], ep[kErodeZ];
+#pragma unroll
+    for (uint32_t k = 0; k < kErodeZ + 2; k++) {
+        const uint32_t z = zb + k - 1u;  // wraps to UINT32_MAX for zb == 0, k == 0
+        const bool ok = valid && z < g.Z;
+        col[k] = bits[ok ? (size_t)z * planeW + col0 : 0];
+        if (!ok) col[k] = 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kErodeZ; k++) {
+        const uint32_t z = zb + k;
+        const bool ok = z < g.Z;
+        const size_t wi = (size_t)z * planeW + col0;
+        yn[k] = bits[ok && hyn ? wi - rowW : 0];
+        yp[k] = bits[ok && hyp ? wi + rowW : 0];
+        if (!(ok && hyn)) yn[k] = 0u;
+        if (!(ok && hyp)) yp[k] = 0u;
+        const bool lm = ok && valid && tx == 0 && w > 0;
+        const bool rp = ok && valid && tx == kWave - 1 && w + 1 < g.W;
+        em[k] = lm ? bits[wi - 1] : 0u;
+        ep[k] = rp ? bits[wi + 1] : 0u;
+    }
+    bool changed = false;
+#pragma unroll
+    for (uint32_t k = 0; k < kErodeZ; k++) {
+        const uint32_t z = zb + k;
+        const uint32_t G = col[k + 1];
+        // x neighbours: adjacent lanes hold the adjacent words of the row.
+        uint32_t gm = (uint32_t)__shfl_up((int)G, 1);
+        uint32_t gp = (uint32_t)__shfl_down((int)G, 1);
+        if (tx == 0) gm = em[k];
+        if (tx == kWave - 1) gp = ep[k];
+        if (!valid || !G || z >= g.Z || z < z0 || z >= z1 || y < y0 || y >= y1) continue;
+        const uint32_t Lm = (G >> 1) | (gm << 31);
+        const uint32_t Rm = (G << 1) | (gp >> 31);
+        const uint32_t out = G & (Lm | Rm) & (yn[k] | yp[k]) & (col[k] | col[k + 2]);
+        if (out != G) {
+            bits[(size_t)z * planeW + col0] = out;
+            col[k + 1] = out;  // later planes of this column see the update
+            changed = true;
         }
-        const unsigned long long m = __ballot(changed);
-        if (lane_id() == 0 && m) atomicAdd(&changes[pass], (uint32_t)__popcll(m));
+    }
+    const unsigned long long m = __ballot(changed);
+    if (lane_id() == 0 && m) {
+        atomicAdd(&changes[pass], (uint32_t)__popcll(m));
+        stamp[L] = gpass + 1u;
     }
 }
 
@@ -671,23 +786,6 @@ hipError_t set_orient_table(const uint8_t lut[64]) {
     return hipMemcpyToSymbol(HIP_SYMBOL(c_orient), lut, 64);
 }
 
-// Rank of each surface voxel inside its brick: exclusive popcount prefix of
-// the brick's surface words (same word order as k_surf_brick).
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, uint32_t n) {
-    const uint32_t tid = threadIdx.x;
-    sh[tid] = v;
-    __syncthreads();
-    for (uint32_t off = 1; off < n; off <<= 1) {
-        uint32_t t = (tid >= off && tid < n) ? sh[tid - off] : 0u;
-        __syncthreads();
-        if (tid < n) sh[tid] += t;
-        __syncthreads();
-    }
-    uint32_t r = (tid > 0 && tid < n) ? sh[tid - 1] : 0u;
-    __syncthreads();
-    return r;
-}
-
 // 4 bits (MSB = first node) -> 4 bytes of 0/1 in little-endian node order.
 __device__ __forceinline__ uint32_t spread4(uint32_t nib) {
     return ((nib >> 3) & 1u) | (((nib >> 2) & 1u) << 8) | (((nib >> 1) & 1u) << 16) |
@@ -698,117 +796,213 @@ __device__ __forceinline__ uint32_t spread2(uint32_t v) {
     return ((v >> 1) & 1u) | ((v & 1u) << 16);
 }
 
-// One 128-thread workgroup per 128x4x8 brick, one thread per 32-node bit
-// word: six neighbour masks from bit shifts of the word and its five
-// neighbour words, boundary ids (27 for fully surrounded nodes, the 64-entry
-// table for the rest), materials from the brick's surface list, and one
-// vector store of the 32 packed nodes.
+// Writes the 32 nodes of one bit word.  Fully surrounded nodes get boundary
+// id 27 as a bitwise pattern; the remaining boundary and surface voxels are
+// patched byte-wise in the thread's LDS slot (the 64-entry boundary-id table
+// is in LDS too), which is then stored with 16-byte writes.  md holds the
+// word's first material dword; seg the address of its material segment.
+__device__ __forceinline__ void write_word_nodes(int kind, void *__restrict__ nodes, size_t wi,
+                                                 uint32_t G, uint32_t L, uint32_t R, uint32_t Yn,
+                                                 uint32_t Yp, uint32_t Zn, uint32_t Zp,
+                                                 uint32_t S, uint32_t md0,
+                                                 const uint32_t *__restrict__ seg,
+                                                 const uint8_t *lut, uint4 *my) {
+    const uint32_t interior = G & L & R & Yn & Yp & Zn & Zp;
+    const uint32_t patch = (G & ~interior) | S;
+    const size_t node0 = wi * 32u;
+    const int nb = kind == 0 ? 1 : 2;  // bytes per node
+    uint4 *dst = (uint4 *)((uint8_t *)nodes + node0 * (size_t)nb);
+    if (kind == 0) {
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            uint32_t o[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                o[q] = spread4((interior >> (28 - 4 * (4 * j + q))) & 15u) * 27u;
+            const uint4 v = make_uint4(o[0], o[1], o[2], o[3]);
+            if (patch) my[j] = v;
+            else dst[j] = v;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            uint32_t o[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                o[q] = spread2((interior >> (30 - 2 * (4 * j + q))) & 3u) * 27u;
+            const uint4 v = make_uint4(o[0], o[1], o[2], o[3]);
+            if (patch) my[j] = v;
+            else dst[j] = v;
+        }
+    }
+    if (!patch) return;
+    uint32_t m = patch, cur = md0, next = 1;
+    int used = 0;
+    while (m) {
+        const int k = __clz(m);
+        const uint32_t bit = 0x80000000u >> k;
+        uint32_t bid = 0u;
+        if (interior & bit) {
+            bid = 27u;
+        } else if (G & bit) {
+            const uint32_t perm = ((L & bit) ? 1u : 0u) | ((R & bit) ? 2u : 0u) |
+                                  ((Yn & bit) ? 4u : 0u) | ((Yp & bit) ? 8u : 0u) |
+                                  ((Zn & bit) ? 16u : 0u) | ((Zp & bit) ? 32u : 0u);
+            bid = lut[perm];
+        }
+        uint32_t mat = 0u;
+        if (S & bit) {
+            mat = cur & 0xFFu;
+            cur >>= 8;
+            if (++used == 4) {  // the segment's next dword, if materials remain
+                used = 0;
+                cur = (S & (bit - 1u)) ? seg[next++] : 0u;
+            }
+        }
+        if (kind == 0)
+            ((uint8_t *)my)[k] = (uint8_t)(((mat << 5) & 0xE0u) | (bid & 31u));
+        else
+            ((uint16_t *)my)[k] = (uint16_t)(bid | (mat << 8));
+        m &= ~bit;
+    }
+    const int n16 = 2 * nb;
+    for (int j = 0; j < n16; j++) dst[j] = my[j];
+}
+
+// Cartesian node write: block (64, 4) = 64 words x 4 rows, each thread a
+// column of kNodeZ planes.  Every global load the column needs (the words
+// of planes zb-1..zb+kNodeZ, the rows above and below, the words across the
+// chunk edges, the surface words, the brick's material offset and word
+// ranks) is issued at once from clamped, always-valid addresses, so a wave
+// waits for memory once; the words' first material dwords come in a second
+// round trip.  x neighbours come from adjacent lanes.
 //
 // With several erosion subspaces (voxSplitRes y/z smaller than the grid) the
 // reference computes a subspace's ids before later subspaces erode
 // (src/host_side.cpp:1183-1206), so neighbours across a +y/+z seam are read
 // from the pre-erosion snapshot `pre` (seam rows flagged in ystart/zstart).
-__global__ __launch_bounds__(128) void k_nodes_cart(GridDims g, const uint32_t *__restrict__ solid,
+__global__ __launch_bounds__(256) void k_nodes_cart(GridDims g, const uint32_t *__restrict__ solid,
                                                     const uint32_t *__restrict__ pre,
                                                     const uint8_t *__restrict__ ystart,
                                                     const uint8_t *__restrict__ zstart,
                                                     MatSource ms, int use_mats, int kind,
                                                     void *__restrict__ nodes, uint32_t nbx,
                                                     uint32_t nby) {
-    __shared__ uint32_t scan_sh[128];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t b = blockIdx.x;
-    const uint32_t bxi = b % nbx, byi = (b / nbx) % nby, bzi = b / (nbx * nby);
-    const uint32_t wx = tid & 3u, ly = (tid >> 2) & 3u, lz = tid >> 4;
-    const uint32_t aw = bxi * 4u + wx, ay = byi * kCartBY + ly, az = bzi * kCartBZ + lz;
-    const bool valid = aw < g.W && ay < g.Y && az < g.Z;
-    uint32_t G = 0, L = 0, R = 0, Yn = 0, Yp = 0, Zn = 0, Zp = 0;
-    const size_t wi = ((size_t)az * g.Y + ay) * g.W + aw;
-    if (valid) {
-        G = solid[wi];
-        L = (G >> 1) | (ldw(solid, g, (int64_t)aw - 1, ay, az) << 31);
-        R = (G << 1) | (ldw(solid, g, (int64_t)aw + 1, ay, az) >> 31);
-        Yn = ldw(solid, g, aw, (int64_t)ay - 1, az);
-        const bool ys = pre && ay + 1 < g.Y && ystart[ay + 1];
-        const bool zs = pre && az + 1 < g.Z && zstart[az + 1];
-        Yp = ldw(ys ? pre : solid, g, aw, (int64_t)ay + 1, az);
-        Zn = ldw(solid, g, aw, ay, (int64_t)az - 1);
-        Zp = ldw(zs ? pre : solid, g, aw, ay, (int64_t)az + 1);
+    __shared__ uint8_t lut[64];
+    __shared__ uint4 slot[256][4];  // 64 output bytes per thread (LongNode; ShortNode uses 32)
+    const uint32_t nxc = (g.W + kWave - 1) / kWave, ny4 = (g.Y + 3) / 4;
+    const uint32_t nzc = (g.Z + kNodeZ - 1) / kNodeZ;
+    const uint32_t lb = xcd_logical_block(nxc * ny4 * nzc);
+    if (lb == UINT32_MAX) return;  // uniform over the block
+    const uint32_t tx = threadIdx.x;
+    const uint32_t tid = threadIdx.y * kWave + tx;
+    if (tid < 64u) lut[tid] = c_orient[tid];
+    const RowBlock rb = row_block(lb, nxc, ny4);
+    const uint32_t zb = rb.z * kNodeZ;
+    const uint32_t w = rb.xc * kWave + tx;
+    const uint32_t y = rb.y4 * 4 + threadIdx.y;
+    const bool valid = w < g.W && y < g.Y;
+    const size_t rowW = g.W, planeW = (size_t)g.W * g.Y;
+    const size_t col0 = valid ? (size_t)y * rowW + w : 0;
+    const bool hyn = valid && y > 0, hyp = valid && y + 1 < g.Y;
+    // Round trip 1.
+    uint32_t col[kNodeZ + 2], yn[kNodeZ], yp[kNodeZ], em[kNodeZ], ep[kNodeZ];
+#pragma unroll
+    for (uint32_t k = 0; k < kNodeZ + 2; k++) {
+        const uint32_t z = zb + k - 1u;  // wraps to UINT32_MAX for zb == 0, k == 0
+        const bool ok = valid && z < g.Z;
+        col[k] = solid[ok ? (size_t)z * planeW + col0 : 0];
+        if (!ok) col[k] = 0u;
     }
-    // Materials.
-    uint32_t S = 0u, mrank = 0u, mbase = UINT32_MAX;
-    if (use_mats) {
-        mbase = ms.brick_mat_off[b];
-        if (mbase != UINT32_MAX) {
-            S = valid ? ms.surf[wi] : 0u;
-            mrank = block_excl_scan((uint32_t)__popc(S), scan_sh, 128u);
+#pragma unroll
+    for (uint32_t k = 0; k < kNodeZ; k++) {
+        const uint32_t z = zb + k;
+        const bool ok = valid && z < g.Z;
+        const size_t wi = ok ? (size_t)z * planeW + col0 : 0;
+        yn[k] = solid[ok && hyn ? wi - rowW : 0];
+        yp[k] = solid[ok && hyp ? wi + rowW : 0];
+        if (!(ok && hyn)) yn[k] = 0u;
+        if (!(ok && hyp)) yp[k] = 0u;
+        const bool lm = ok && tx == 0 && w > 0;
+        const bool rp = ok && tx == kWave - 1 && w + 1 < g.W;
+        em[k] = lm ? solid[wi - 1] : 0u;
+        ep[k] = rp ? solid[wi + 1] : 0u;
+    }
+    uint32_t ypp[kNodeZ], zpp[kNodeZ];
+    bool ys = false, zs[kNodeZ];
+#pragma unroll
+    for (uint32_t k = 0; k < kNodeZ; k++) {
+        ypp[k] = 0u;
+        zpp[k] = 0u;
+        zs[k] = false;
+    }
+    if (pre) {  // uniform
+        ys = hyp && ystart[y + 1] != 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kNodeZ; k++) {
+            const uint32_t z = zb + k;
+            const bool ok = valid && z < g.Z;
+            const size_t wi = ok ? (size_t)z * planeW + col0 : 0;
+            const bool hz = ok && z + 1 < g.Z;
+            ypp[k] = pre[ok && hyp ? wi + rowW : 0];
+            zpp[k] = pre[hz ? wi + planeW : 0];
+            zs[k] = hz && zstart[z + 1] != 0;
         }
     }
-    if (!valid) return;
-    const uint32_t interior = G & L & R & Yn & Yp & Zn & Zp;
-    uint32_t boundary = G & ~interior;
-    const size_t node0 = wi * 32u;
-    if (kind == 0) {  // ShortNode: mat<<5 | bid
-        uint32_t out[8];
+    uint32_t mb = UINT32_MAX, S[kNodeZ], wr[kNodeZ];
 #pragma unroll
-        for (int j = 0; j < 8; j++) out[j] = spread4((interior >> (28 - 4 * j)) & 15u) * 27u;
-        while (boundary) {
-            const int k = __clz(boundary);
-            const uint32_t bit = 0x80000000u >> k;
-            const uint32_t perm = ((L & bit) ? 1u : 0u) | ((R & bit) ? 2u : 0u) |
-                                  ((Yn & bit) ? 4u : 0u) | ((Yp & bit) ? 8u : 0u) |
-                                  ((Zn & bit) ? 16u : 0u) | ((Zp & bit) ? 32u : 0u);
-            out[k >> 2] |= (uint32_t)c_orient[perm] << (8 * (k & 3));
-            boundary &= ~bit;
-        }
-        if (S) {
-            uint32_t r = mbase + mrank;
-            uint32_t m = S;
-            while (m) {
-                const int k = __clz(m);
-                const uint32_t mat = ms.mat_list[r++];
-                out[k >> 2] |= ((mat << 5) & 0xFFu) << (8 * (k & 3));
-                m &= ~(0x80000000u >> k);
-            }
-        }
-        uint4 *dst = (uint4 *)((uint8_t *)nodes + node0);
-        dst[0] = make_uint4(out[0], out[1], out[2], out[3]);
-        dst[1] = make_uint4(out[4], out[5], out[6], out[7]);
-    } else {  // LongNode: {bid, mat}
-        uint32_t out[16];
+    for (uint32_t k = 0; k < kNodeZ; k++) {
+        S[k] = 0u;
+        wr[k] = 0u;
+    }
+    if (use_mats) {  // uniform; the kNodeZ planes share one brick (kNodeZ | kCartBZ)
+        const uint32_t yy = valid ? y : 0u, ww = valid ? w : 0u;
+        const uint32_t b = ((zb / kCartBZ) * nby + yy / kCartBY) * nbx + ww / (kCartBX / 32);
+        mb = ms.brick_mat_off[b];
 #pragma unroll
-        for (int j = 0; j < 16; j++) out[j] = spread2((interior >> (30 - 2 * j)) & 3u) * 27u;
-        while (boundary) {
-            const int k = __clz(boundary);
-            const uint32_t bit = 0x80000000u >> k;
-            const uint32_t perm = ((L & bit) ? 1u : 0u) | ((R & bit) ? 2u : 0u) |
-                                  ((Yn & bit) ? 4u : 0u) | ((Yp & bit) ? 8u : 0u) |
-                                  ((Zn & bit) ? 16u : 0u) | ((Zp & bit) ? 32u : 0u);
-            out[k >> 1] |= (uint32_t)c_orient[perm] << (16 * (k & 1));
-            boundary &= ~bit;
+        for (uint32_t k = 0; k < kNodeZ; k++) {
+            const uint32_t z = zb + k;
+            const bool ok = valid && z < g.Z;
+            const uint32_t wb = (((z % kCartBZ) * kCartBY + yy % kCartBY) * (kCartBX / 32) +
+                                 ww % (kCartBX / 32));
+            S[k] = ms.surf[ok ? (size_t)z * planeW + col0 : 0];
+            wr[k] = ms.wrank[(size_t)b * (kCartBX * kCartBY * kCartBZ / 32) + (ok ? wb : 0)];
+            if (!ok) S[k] = 0u;
         }
-        if (S) {
-            uint32_t r = mbase + mrank;
-            uint32_t m = S;
-            while (m) {
-                const int k = __clz(m);
-                const uint32_t mat = ms.mat_list[r++];
-                out[k >> 1] |= mat << (16 * (k & 1) + 8);
-                m &= ~(0x80000000u >> k);
-            }
-        }
-        uint4 *dst = (uint4 *)((uint8_t *)nodes + node0 * 2u);
+    }
+    __syncthreads();  // lut ready
+    // Round trip 2: the first material dword of each surface word.
+    uint32_t md[kNodeZ];
+    const uint32_t *mats32 = (const uint32_t *)ms.mat_list;
 #pragma unroll
-        for (int j = 0; j < 4; j++)
-            dst[j] = make_uint4(out[4 * j], out[4 * j + 1], out[4 * j + 2], out[4 * j + 3]);
+    for (uint32_t k = 0; k < kNodeZ; k++) {
+        if (mb == UINT32_MAX) S[k] = 0u;
+        md[k] = S[k] ? mats32[(mb + wr[k]) >> 2] : 0u;
+    }
+    uint4 *my = slot[tid];
+#pragma unroll
+    for (uint32_t k = 0; k < kNodeZ; k++) {
+        const uint32_t z = zb + k;
+        const uint32_t G = col[k + 1];
+        uint32_t gm = (uint32_t)__shfl_up((int)G, 1);
+        uint32_t gp = (uint32_t)__shfl_down((int)G, 1);
+        if (tx == 0) gm = em[k];
+        if (tx == kWave - 1) gp = ep[k];
+        if (!valid || z >= g.Z) continue;
+        const uint32_t L = (G >> 1) | (gm << 31);
+        const uint32_t R = (G << 1) | (gp >> 31);
+        const uint32_t Yp = ys ? ypp[k] : yp[k];
+        const uint32_t Zp = zs[k] ? zpp[k] : col[k + 2];
+        const uint32_t *seg = S[k] ? mats32 + ((mb + wr[k]) >> 2) : mats32;
+        write_word_nodes(kind, nodes, (size_t)z * planeW + col0, G, L, R, yn[k], Yp, col[k], Zp,
+                         S[k], md[k], seg, lut, my);
     }
 }
-
 
 // ---------------------------------------------------------------------------
 // FCC nodes
 // ---------------------------------------------------------------------------
-// Spreads 16 bits to the odd... even positions: bit j -> bit 2j.
+// Spreads 16 bits to the even positions: bit j -> bit 2j.
 __device__ __forceinline__ uint32_t spread16(uint32_t x) {
     x &= 0xFFFFu;
     x = (x | (x << 8)) & 0x00FF00FFu;
@@ -823,7 +1017,7 @@ struct FccGrids {
 };
 
 // 32 FCC nodes of row (y, fz), x in [32*fw, 32*fw+32): even x from the
-// sub-grid with x shift 0, odd x from the one with x shift 1/2
+// sub-grid with x shift 0, odd x from the one with x shift d/2
 // (convertToFCCGrid, src/voxelizer.cu:1756-1772).
 __device__ __forceinline__ uint32_t fcc_word(const FccGrids &G, const GridDims &g, int64_t fw,
                                              int64_t y, int64_t fz) {
@@ -838,84 +1032,65 @@ __device__ __forceinline__ uint32_t fcc_word(const FccGrids &G, const GridDims &
     return (spread16(e) << 1) | spread16(o);
 }
 
-// One 128-thread workgroup per 128x4x8 FCC brick (= one 64x4x4 brick of each
-// sub-grid), one thread per 32-node word: the 12-bit neighbour mask of
+// Left/right shifted FCC row words: neighbours at x-1 / x+1 (the word of the
+// adjacent lane supplies the carried bit).
+__device__ __forceinline__ void lr_shift(const FccGrids &G, const GridDims &g, uint32_t word,
+                                         int64_t fw, int64_t y, int64_t fz, uint32_t &Lw,
+                                         uint32_t &Rw) {
+    const uint32_t tx = threadIdx.x;
+    uint32_t prev = (uint32_t)__shfl_up((int)word, 1);
+    uint32_t next = (uint32_t)__shfl_down((int)word, 1);
+    if (tx == 0) prev = fcc_word(G, g, fw - 1, y, fz);
+    if (tx == kWave - 1) next = fcc_word(G, g, fw + 1, y, fz);
+    Lw = (word >> 1) | (prev << 31);
+    Rw = (word << 1) | (next >> 31);
+}
+
+// FCC node write, row-major like k_nodes_cart: the 12-bit neighbour mask of
 // calculateFCCBoundaries (src/voxelizer.cu:1930-2048) as bitwise selects by
 // the (x+z) parity mask, materials from the four sub-grids' surface lists.
-__global__ __launch_bounds__(128) void k_nodes_fcc(GridDims g, const uint32_t *s1,
+__global__ __launch_bounds__(256) void k_nodes_fcc(GridDims g, const uint32_t *s1,
                                                    const uint32_t *s2, const uint32_t *s3,
                                                    const uint32_t *s4, MatSource m1, MatSource m2,
                                                    MatSource m3, MatSource m4, int use_mats,
                                                    int kind, void *__restrict__ nodes,
                                                    uint32_t nbx, uint32_t nby) {
-    __shared__ uint32_t rank_sh[128];
-    __shared__ uint32_t mbase_sh[4];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t b = blockIdx.x;
-    const uint32_t bxi = b % nbx, byi = (b / nbx) % nby, bzi = b / (nbx * nby);
-    const uint32_t fwl = tid & 3u, ly = (tid >> 2) & 3u, lz = tid >> 4;
-    const int64_t fw = (int64_t)bxi * 4 + fwl;
-    const int64_t y = (int64_t)byi * 4 + ly;
-    const int64_t fz = (int64_t)bzi * 8 + lz;
-    const uint32_t FW = 2 * g.W, FY = g.Y, FZ = 2 * g.Z;
-    const bool valid = fw < (int64_t)FW && y < (int64_t)FY && fz < (int64_t)FZ;
+    const uint32_t FW = 2 * g.W, FY = g.Y;
+    const uint32_t nxc = (FW + kWave - 1) / kWave, ny4 = (FY + 3) / 4;
+    const uint32_t L = xcd_logical_block(nxc * ny4 * 2 * g.Z);
+    if (L == UINT32_MAX) return;
+    const RowBlock rb = row_block(L, nxc, ny4);
+    const int64_t fw = (int64_t)rb.xc * kWave + threadIdx.x;
+    const int64_t y = (int64_t)rb.y4 * 4 + threadIdx.y;
+    const int64_t fz = rb.z;
+    const bool valid = fw < (int64_t)FW && y < (int64_t)FY;
     FccGrids G{{s1, s2, s3, s4}};
-    const MatSource ms[4] = {m1, m2, m3, m4};
-
-    // Surface ranks: thread t handles word (t & 31) of sub-grid (t >> 5)'s
-    // brick (64x4x4 voxels, word order (lz*4 + ly)*2 + wx).
-    uint32_t gsurf = 0u;
-    if (use_mats) {
-        const uint32_t gi = tid >> 5, w = tid & 31u;
-        const uint32_t wx = w & 1u, gly = (w >> 1) & 3u, glz = w >> 3;
-        const uint64_t aw = (uint64_t)bxi * 2 + wx, ay = (uint64_t)byi * 4 + gly,
-                       az = (uint64_t)bzi * 4 + glz;
-        const uint32_t mb = ms[gi].brick_mat_off[b];
-        if (tid < 4) mbase_sh[tid] = ms[tid].brick_mat_off[b];
-        if (mb != UINT32_MAX && aw < g.W && ay < g.Y && az < g.Z)
-            gsurf = ms[gi].surf[(az * g.Y + ay) * g.W + aw];
-        rank_sh[tid] = (uint32_t)__popc(gsurf);
-        __syncthreads();
-        // Segmented (per sub-grid, 32 entries) inclusive scan.
-        for (uint32_t off = 1; off < 32; off <<= 1) {
-            uint32_t t = (w >= off) ? rank_sh[tid - off] : 0u;
-            __syncthreads();
-            rank_sh[tid] += t;
-            __syncthreads();
-        }
-        rank_sh[tid] -= (uint32_t)__popc(gsurf);  // exclusive
-        __syncthreads();
-    }
+    // All lanes take part in the shuffles; invalid lanes hold zero words.
+    auto word = [&](int64_t yy, int64_t zz) { return valid ? fcc_word(G, g, fw, yy, zz) : 0u; };
+    const uint32_t M0 = word(y, fz), Mm = word(y - 1, fz), Mp = word(y + 1, fz);
+    const uint32_t D0 = word(y, fz - 1), Dm = word(y - 1, fz - 1), Dp = word(y + 1, fz - 1);
+    const uint32_t U0 = word(y, fz + 1), Um = word(y - 1, fz + 1), Up = word(y + 1, fz + 1);
+    uint32_t LD0, RD0, LMm, RMm, LM0, RM0, LMp, RMp, LU0, RU0;
+    lr_shift(G, g, D0, fw, y, fz - 1, LD0, RD0);
+    lr_shift(G, g, Mm, fw, y - 1, fz, LMm, RMm);
+    lr_shift(G, g, M0, fw, y, fz, LM0, RM0);
+    lr_shift(G, g, Mp, fw, y + 1, fz, LMp, RMp);
+    lr_shift(G, g, U0, fw, y, fz + 1, LU0, RU0);
     if (!valid) return;
-
-    const uint32_t M0 = fcc_word(G, g, fw, y, fz);
-    const uint32_t Mm = fcc_word(G, g, fw, y - 1, fz), Mp = fcc_word(G, g, fw, y + 1, fz);
-    const uint32_t D0 = fcc_word(G, g, fw, y, fz - 1);
-    const uint32_t Dm = fcc_word(G, g, fw, y - 1, fz - 1), Dp = fcc_word(G, g, fw, y + 1, fz - 1);
-    const uint32_t U0 = fcc_word(G, g, fw, y, fz + 1);
-    const uint32_t Um = fcc_word(G, g, fw, y - 1, fz + 1), Up = fcc_word(G, g, fw, y + 1, fz + 1);
-    auto Lsh = [&](uint32_t w, int64_t yy, int64_t zz) {
-        return (w >> 1) | (fcc_word(G, g, fw - 1, yy, zz) << 31);
-    };
-    auto Rsh = [&](uint32_t w, int64_t yy, int64_t zz) {
-        return (w << 1) | (fcc_word(G, g, fw + 1, yy, zz) >> 31);
-    };
     const uint32_t E = (fz & 1) ? 0x55555555u : 0xAAAAAAAAu;  // (x+z) even
     const uint32_t NE = ~E;
     uint32_t n[12];
     n[0] = (E & Dm) | (NE & D0);
-    n[1] = Lsh(D0, y, fz - 1);
-    n[2] = Rsh(D0, y, fz - 1);
+    n[1] = LD0;
+    n[2] = RD0;
     n[3] = (E & D0) | (NE & Dp);
-    const uint32_t LMm = Lsh(Mm, y - 1, fz), LM0 = Lsh(M0, y, fz), LMp = Lsh(Mp, y + 1, fz);
-    const uint32_t RMm = Rsh(Mm, y - 1, fz), RM0 = Rsh(M0, y, fz), RMp = Rsh(Mp, y + 1, fz);
     n[4] = (E & LMm) | (NE & LM0);
     n[5] = (E & RMm) | (NE & RM0);
     n[6] = (E & LM0) | (NE & LMp);
     n[7] = (E & RM0) | (NE & RMp);
     n[8] = (E & Um) | (NE & U0);
-    n[9] = Lsh(U0, y, fz + 1);
-    n[10] = Rsh(U0, y, fz + 1);
+    n[9] = LU0;
+    n[10] = RU0;
     n[11] = (E & U0) | (NE & Up);
     uint32_t interior = M0;
 #pragma unroll
@@ -940,23 +1115,26 @@ __global__ __launch_bounds__(128) void k_nodes_fcc(GridDims g, const uint32_t *s
     for (int k = 0; k < 32; k++) mat[k] = 0u;
     if (use_mats) {
         const int odd = (int)(fz & 1);
-        const int gsel[2] = {odd ? 2 : 0, odd ? 3 : 1};  // even x, odd x
-        const uint32_t glz = (uint32_t)(lz >> 1);
-        const uint32_t wx = (uint32_t)(fwl >> 1);
-        const uint32_t gw = (glz * 4 + ly) * 2 + wx;
-        const uint64_t aw = (uint64_t)bxi * 2 + wx, az = (uint64_t)(fz >> 1);
-        const int sh = (fwl & 1) ? 0 : 16;
+        const uint64_t gw = (uint64_t)(fw >> 1), gz = (uint64_t)(fz >> 1);
+        const uint32_t b = (uint32_t)(((gz / kFccBZ) * nby + (uint64_t)y / kFccBY) * nbx +
+                                      gw / (kFccBX / 32));
+        const uint32_t wb = (uint32_t)(((gz % kFccBZ) * kFccBY + (uint64_t)y % kFccBY) *
+                                           (kFccBX / 32) + gw % (kFccBX / 32));
+        const size_t gwi = (gz * g.Y + (uint64_t)y) * g.W + gw;
+        const int sh = (fw & 1) ? 0 : 16;
         for (int par = 0; par < 2; par++) {
-            const int gi = gsel[par];
-            const uint32_t mb = mbase_sh[gi];
+            const int gi = odd ? 2 + par : par;
+            const MatSource &src = gi == 0 ? m1 : gi == 1 ? m2 : gi == 2 ? m3 : m4;
+            const uint32_t mb = src.brick_mat_off[b];
             if (mb == UINT32_MAX) continue;
-            const uint32_t sw = ms[gi].surf[(az * g.Y + (uint64_t)y) * g.W + aw];
-            // voxels of this half, MSB first; rank counts set bits before it
-            uint32_t r = mb + rank_sh[gi * 32 + gw] + ((fwl & 1) ? (uint32_t)__popc(sw >> 16) : 0u);
+            const uint32_t sw = src.surf[gwi];
             uint32_t h = (sw >> sh) & 0xFFFFu;
+            if (!h) continue;
+            uint32_t r = mb + src.wrank[(size_t)b * (kFccBX * kFccBY * kFccBZ / 32) + wb] +
+                         ((fw & 1) ? (uint32_t)__popc(sw >> 16) : 0u);
             while (h) {
                 const int i = __clz(h) - 16;  // voxel i of the 16
-                const uint32_t mv = ms[gi].mat_list[r++];
+                const uint32_t mv = src.mat_list[r++];
                 const int k = 2 * i + par;
 #pragma unroll
                 for (int kk = 0; kk < 32; kk++)

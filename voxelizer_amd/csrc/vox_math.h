@@ -70,6 +70,28 @@ struct TileSetup {
     uint32_t tmin[2], tmax[2];
 };
 
+// x component of the normal in the reference's own expression
+// (src/voxelizer.cu:882-884) and the edge functions it orients.
+__device__ __forceinline__ double tri_tN(const D3 v[3]) {
+    return (v[0].y * v[1].z + v[2].y * v[0].z + v[2].z * v[1].y) -
+           (v[2].y * v[1].z + v[0].z * v[1].y + v[2].z * v[0].y);
+}
+__device__ __forceinline__ void edges_by_sign(const D3 v[3], bool pos, TileSetup &ts) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const D3 a = v[i], b = v[(i + 1) % 3];
+        if (pos)
+            ts.ne[i] = {a.z - b.z, b.y - a.y};
+        else
+            ts.ne[i] = {b.z - a.z, a.y - b.y};
+        ts.de[i] = -(ts.ne[i].x * a.y + ts.ne[i].y * a.z);
+    }
+}
+// Edge functions only (the commonSpace box comes from the setup pass).
+__device__ __forceinline__ void tile_edges(const D3 v[3], TileSetup &ts) {
+    edges_by_sign(v, tri_tN(v) > 0.0, ts);
+}
+
 __device__ __forceinline__ bool tile_setup(const D3 v[3], double emy, double emz, double d,
                                            uint32_t sminy, uint32_t sminz, uint32_t smaxy,
                                            uint32_t smaxz, TileSetup &ts) {
@@ -85,8 +107,7 @@ __device__ __forceinline__ bool tile_setup(const D3 v[3], double emy, double emz
     double vb2 = (bmax.x - emy) / d, vb3 = (bmax.y - emz) / d;
     int32_t vmin0 = sat_i32(ceil(vb0)), vmin1 = sat_i32(ceil(vb1));
     int32_t vmax0 = sat_i32(floor(vb2) + 1.0), vmax1 = sat_i32(floor(vb3) + 1.0);
-    double tN = (v[0].y * v[1].z + v[2].y * v[0].z + v[2].z * v[1].y) -
-                (v[2].y * v[1].z + v[0].z * v[1].y + v[2].z * v[0].y);
+    const double tN = tri_tN(v);
     if (tN == 0.0) return false;
     if (vmin0 >= vmax0 || vmin1 >= vmax1) return false;
     if (vmax0 <= (int32_t)sminy || vmax1 <= (int32_t)sminz) return false;
@@ -95,15 +116,7 @@ __device__ __forceinline__ bool tile_setup(const D3 v[3], double emy, double emz
     ts.cmin[1] = (uint32_t)max((int32_t)sminz, vmin1);
     ts.cmax[0] = min(smaxy, (uint32_t)vmax0);
     ts.cmax[1] = min(smaxz, (uint32_t)vmax1);
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        const D3 a = v[i], b = v[(i + 1) % 3];
-        if (tN > 0.0)
-            ts.ne[i] = {a.z - b.z, b.y - a.y};
-        else
-            ts.ne[i] = {b.z - a.z, a.y - b.y};
-        ts.de[i] = -(ts.ne[i].x * a.y + ts.ne[i].y * a.z);
-    }
+    edges_by_sign(v, tN > 0.0, ts);
     ts.tmin[0] = ts.cmin[0] / 4u;
     ts.tmin[1] = ts.cmin[1] / 4u;
     ts.tmax[0] = (ts.cmax[0] - 1u) / 4u;
