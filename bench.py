@@ -119,7 +119,7 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1",
                           "n_gpus": args.gpus}))
         print("bench.py: only --gpus 1 is supported (the voxelizer runs the whole grid "
-              "on one MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
+              "on one MI355X)", file=sys.stderr)
         return 2
 
     import voxelizer_amd as va
