@@ -164,12 +164,53 @@ size_t vox_node_size(vox_node_kind kind);
 typedef struct vox_stage_times {
     float plan_ms, solid_bin_ms, solid_fill_ms, surface_ms, erode_ms, nodes_ms, total_ms;
     uint32_t erosion_passes;
-    uint32_t solid_pairs;    /* (triangle, tile) work-queue entries */
-    uint32_t surface_pairs;  /* (triangle, brick) surface work items */
-    uint32_t surface_voxels; /* voxels written by the surface stage */
+    uint32_t solid_pairs;      /* (triangle, tile) work-queue entries */
+    uint32_t surface_pairs;    /* (triangle, brick) surface work items */
+    uint32_t surface_voxels;   /* voxels written by the surface stage */
+    uint32_t learned_schedule; /* 1 if the surface bricks ran in learned order */
 } vox_stage_times;
 vox_status vox_enable_timing(vox_ctx *ctx, int enable);
 vox_status vox_get_stage_times(const vox_ctx *ctx, vox_stage_times *out);
+
+/* ---- brick schedule and its learned cost model (no counterpart in the
+ * reference) --------------------------------------------------------------
+ * The surface pass processes its non-empty bricks from a list with
+ * persistent workgroups.  PLAIN lists them in index order; LEARNED sorts them
+ * by the cost an MLP (16 features -> 64 -> 64 -> 1, ReLU, FP32) predicts, most
+ * expensive first.  Outputs are identical under both.  LEARNED falls back to
+ * PLAIN while no model is set (vox_stage_times.learned_schedule says which
+ * ran). */
+typedef enum vox_schedule { VOX_SCHEDULE_PLAIN = 0, VOX_SCHEDULE_LEARNED = 1 } vox_schedule;
+#define VOX_COST_FEATURES 16
+#define VOX_COST_PARAMS 5313 /* W1[64][16], b1[64], W2[64][64], b2[64], W3[64], b3 */
+vox_status vox_set_schedule(vox_ctx *ctx, vox_schedule schedule);
+/* Model parameters (host arrays of VOX_COST_PARAMS floats). */
+vox_status vox_cost_set_params(vox_ctx *ctx, const float *params);
+vox_status vox_cost_get_params(vox_ctx *ctx, float *params);
+/* While enabled, every surface pass appends, per non-empty brick, its
+ * VOX_COST_FEATURES features and its measured duration (GPU clock cycles).
+ * Enabling clears earlier samples. */
+vox_status vox_cost_record(vox_ctx *ctx, int enable);
+/* Copies up to cap samples (features row-major, cycles) to host arrays (may
+ * be NULL to query the count) and reports the number recorded. */
+vox_status vox_cost_samples(vox_ctx *ctx, float *features, float *cycles, uint32_t cap,
+                            uint32_t *n_out);
+/* Trains the model on host samples with the library's GPU kernels: targets
+ * are standardized log(cycles); Adam, `epochs` passes of mini-batches of
+ * `batch` rows (multiple of 32), deterministic for a given seed.  Installs
+ * the result as the context's model and returns the final mean squared error
+ * (standardized units) over all samples. */
+vox_status vox_cost_train(vox_ctx *ctx, const float *features, const float *cycles, uint32_t n,
+                          uint32_t epochs, uint32_t batch, float lr, uint32_t seed,
+                          float *final_mse);
+/* The MLP kernels on device arrays (training and tests): y[i] = f(x[i]) for
+ * n rows of VOX_COST_FEATURES; and the gradient of
+ * (1/n) sum_i (f(x[i]) - t[i])^2 with respect to the parameters (grad: device
+ * array of VOX_COST_PARAMS) plus that loss (host). */
+vox_status vox_mlp_forward(vox_ctx *ctx, const float *params_dev, const float *x_dev,
+                           uint32_t n, float *y_dev);
+vox_status vox_mlp_grad(vox_ctx *ctx, const float *params_dev, const float *x_dev,
+                        const float *t_dev, uint32_t n, float *grad_dev, float *loss);
 
 #ifdef __cplusplus
 }

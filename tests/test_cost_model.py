@@ -1,0 +1,157 @@
+"""Learned brick schedule: the MLP kernels against a float64 numpy restatement
+(forward and backward), that restatement against finite differences, the
+training loop, and bit-identical voxelization under the learned and plain
+schedules (and against the oracle).
+
+Tolerances: forward outputs within 2e-5 + 2e-5*|y| of float64; gradients
+within 1e-4 * max|g| + 1e-6 of float64 (FP32 sums of at most 64 products in
+a fixed order)."""
+import numpy as np
+import pytest
+
+import voxelizer_amd as va
+from voxelizer_amd import workloads as W
+
+F, H = 16, 64
+
+
+def unpack(p):
+    p = np.asarray(p, np.float64)
+    W1 = p[0:1024].reshape(H, F)
+    b1 = p[1024:1088]
+    W2 = p[1088:5184].reshape(H, H)
+    b2 = p[5184:5248]
+    W3 = p[5248:5312]
+    b3 = p[5312]
+    return W1, b1, W2, b2, W3, b3
+
+
+def ref_forward(p, x):
+    W1, b1, W2, b2, W3, b3 = unpack(p)
+    x = np.asarray(x, np.float64)
+    z1 = x @ W1.T + b1
+    h1 = np.maximum(z1, 0)
+    z2 = h1 @ W2.T + b2
+    h2 = np.maximum(z2, 0)
+    return h2 @ W3 + b3, (x, z1, h1, z2, h2)
+
+
+def ref_grad(p, x, t):
+    """Gradient of mean((f(x) - t)^2) by backpropagation, float64."""
+    W1, b1, W2, b2, W3, b3 = unpack(p)
+    y, (x, z1, h1, z2, h2) = ref_forward(p, x)
+    n = x.shape[0]
+    dy = 2.0 * (y - np.asarray(t, np.float64)) / n
+    gW3 = h2.T @ dy
+    gb3 = dy.sum()
+    dz2 = np.outer(dy, W3) * (z2 > 0)
+    gW2 = dz2.T @ h1
+    gb2 = dz2.sum(0)
+    dz1 = (dz2 @ W2) * (z1 > 0)
+    gW1 = dz1.T @ x
+    gb1 = dz1.sum(0)
+    g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3, [gb3]])
+    return g, float(np.mean((y - t) ** 2))
+
+
+def random_model(seed, n):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(va.COST_PARAMS, np.float32)
+    p[0:1024] = rng.uniform(-0.6, 0.6, 1024)
+    p[1024:1088] = rng.uniform(-0.1, 0.1, 64)
+    p[1088:5184] = rng.uniform(-0.3, 0.3, 4096)
+    p[5184:5248] = rng.uniform(-0.1, 0.1, 64)
+    p[5248:5312] = rng.uniform(-0.3, 0.3, 64)
+    p[5312] = 0.05
+    x = rng.uniform(0, 3, (n, F)).astype(np.float32)
+    t = rng.normal(0, 1, n).astype(np.float32)
+    return p, x, t
+
+
+def test_reference_gradient_matches_finite_differences():
+    p, x, t = random_model(1, 40)
+    g, _ = ref_grad(p, x, t)
+    rng = np.random.default_rng(2)
+    p64 = p.astype(np.float64)
+    for k in list(rng.integers(0, va.COST_PARAMS, 40)) + [0, 1024, 1088, 5184, 5248, 5312]:
+        e = 1e-6
+        pp, pm = p64.copy(), p64.copy()
+        pp[k] += e
+        pm[k] -= e
+        fd = (np.mean((ref_forward(pp, x)[0] - t) ** 2) -
+              np.mean((ref_forward(pm, x)[0] - t) ** 2)) / (2 * e)
+        assert abs(fd - g[k]) <= 1e-6 + 1e-5 * abs(g[k]), (k, fd, g[k])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [1, 31, 32, 100, 1000])
+def test_mlp_forward_matches_reference(gpu_ctx, n):
+    p, x, _ = random_model(3 + n, n)
+    got = gpu_ctx.mlp_forward(p, x)
+    ref = ref_forward(p, x)[0]
+    assert np.all(np.abs(got - ref) <= 2e-5 + 2e-5 * np.abs(ref)), np.max(np.abs(got - ref))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [7, 64, 333])
+def test_mlp_grad_matches_reference(gpu_ctx, n):
+    p, x, t = random_model(10 + n, n)
+    g, loss = gpu_ctx.mlp_grad(p, x, t)
+    rg, rloss = ref_grad(p, x, t)
+    tol = 1e-4 * np.max(np.abs(rg)) + 1e-6
+    assert np.max(np.abs(g - rg)) <= tol, (np.max(np.abs(g - rg)), tol)
+    assert abs(loss - rloss) <= 1e-4 * rloss + 1e-6
+
+
+def record_room(ctx, seed):
+    m = W.shoebox_room(seed=seed, n_boxes=24, subdiv=4)
+    ctx.set_mesh(m.vertices, m.indices)
+    ctx.set_materials(m.materials, 9)
+    g = ctx.plan_grid(va.PLAN_CUBE_LENGTH, 0.012)
+    ctx.voxelize_nodes(g, va.NODE_LONG)
+    return m, g
+
+
+@pytest.mark.gpu
+def test_training_fits_recorded_costs(gpu_ctx):
+    gpu_ctx.cost_record(True)
+    for seed in (21, 22):
+        record_room(gpu_ctx, seed)
+    f, c = gpu_ctx.cost_samples()
+    gpu_ctx.cost_record(False)
+    assert f.shape[0] == c.shape[0] > 200
+    assert np.all(np.isfinite(f)) and np.all(c > 0)
+    assert np.all(f[:, 15] == 1.0)
+    mse = gpu_ctx.cost_train(f, c, epochs=40, batch=128, lr=3e-3, seed=1)
+    # standardized targets have variance 1; the model must explain most of it
+    assert mse < 0.5, mse
+    # deterministic for a given seed
+    p1 = gpu_ctx.cost_params()
+    mse2 = gpu_ctx.cost_train(f, c, epochs=40, batch=128, lr=3e-3, seed=1)
+    assert mse2 == mse and np.array_equal(p1, gpu_ctx.cost_params())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", [va.NODE_LONG, va.NODE_SHORT_FCC])
+def test_learned_schedule_output_identical(gpu_ctx, oracle, kind):
+    gpu_ctx.cost_record(True)
+    record_room(gpu_ctx, 23)
+    f, c = gpu_ctx.cost_samples()
+    gpu_ctx.cost_record(False)
+    gpu_ctx.cost_train(f, c, epochs=20, batch=128, lr=3e-3, seed=2)
+    m = W.shoebox_room(seed=24, n_boxes=16, subdiv=3)
+    gpu_ctx.set_mesh(m.vertices, m.indices)
+    gpu_ctx.set_materials(m.materials, 9)
+    g = gpu_ctx.plan_grid(va.PLAN_CUBE_LENGTH, 0.015)
+    gpu_ctx.set_schedule(va.SCHEDULE_PLAIN)
+    plain = gpu_ctx.voxelize_nodes(g, kind)
+    gpu_ctx.set_schedule(va.SCHEDULE_LEARNED)
+    gpu_ctx.enable_timing(True)
+    learned = gpu_ctx.voxelize_nodes(g, kind)
+    used = gpu_ctx.stage_times()["learned_schedule"]
+    gpu_ctx.enable_timing(False)
+    gpu_ctx.set_schedule(va.SCHEDULE_PLAIN)
+    assert used == 1
+    assert np.array_equal(plain, learned)
+    og = oracle.plan_grid(m.vertices, va.PLAN_CUBE_LENGTH, 0.015)
+    assert np.array_equal(learned, oracle.voxelize_nodes(m, og, kind, True))

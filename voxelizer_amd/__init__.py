@@ -27,6 +27,8 @@ PLAN_MAX_DIMENSION, PLAN_CUBE_LENGTH = 0, 1
 NODE_SIZE = {NODE_SHORT: 1, NODE_LONG: 2, NODE_PARTIAL: 8, NODE_SHORT_FCC: 2, NODE_LONG_FCC: 4}
 DEFAULT_VOX_SPLIT = (1024, 512, 512)
 DEFAULT_MAT_SPLIT = (512, 512, 512)
+SCHEDULE_PLAIN, SCHEDULE_LEARNED = 0, 1
+COST_FEATURES, COST_PARAMS = 16, 5313
 
 
 class VoxError(RuntimeError):
@@ -55,7 +57,7 @@ class StageTimes(C.Structure):
                 ("surface_ms", C.c_float), ("erode_ms", C.c_float), ("nodes_ms", C.c_float),
                 ("total_ms", C.c_float), ("erosion_passes", C.c_uint32),
                 ("solid_pairs", C.c_uint32), ("surface_pairs", C.c_uint32),
-                ("surface_voxels", C.c_uint32)]
+                ("surface_voxels", C.c_uint32), ("learned_schedule", C.c_uint32)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -68,6 +70,8 @@ EXPORTED = [
     "vox_set_mesh", "vox_set_mesh_device", "vox_set_materials", "vox_set_materials_device",
     "vox_plan_grid", "vox_plan_grid_host", "vox_solid_bits", "vox_voxelize_nodes", "vox_node_dims",
     "vox_node_count", "vox_node_size", "vox_enable_timing", "vox_get_stage_times",
+    "vox_set_schedule", "vox_cost_set_params", "vox_cost_get_params", "vox_cost_record",
+    "vox_cost_samples", "vox_cost_train", "vox_mlp_forward", "vox_mlp_grad",
 ]
 
 _lib = None
@@ -112,6 +116,15 @@ def lib():
     L.vox_node_size.restype = C.c_size_t
     L.vox_enable_timing.argtypes = [P, I32]
     L.vox_get_stage_times.argtypes = [P, C.POINTER(StageTimes)]
+    F = C.c_float
+    L.vox_set_schedule.argtypes = [P, I32]
+    L.vox_cost_set_params.argtypes = [P, P]
+    L.vox_cost_get_params.argtypes = [P, P]
+    L.vox_cost_record.argtypes = [P, I32]
+    L.vox_cost_samples.argtypes = [P, P, P, U32, C.POINTER(U32)]
+    L.vox_cost_train.argtypes = [P, P, P, U32, U32, U32, F, U32, C.POINTER(F)]
+    L.vox_mlp_forward.argtypes = [P, P, P, U32, P]
+    L.vox_mlp_grad.argtypes = [P, P, P, P, U32, P, C.POINTER(F)]
     _lib = L
     return L
 
@@ -275,8 +288,69 @@ class Context:
         self._check(lib().vox_get_stage_times(self.h, C.byref(t)))
         return t.as_dict()
 
+    # -- brick schedule and cost model --------------------------------------
+    def set_schedule(self, schedule: int):
+        self._check(lib().vox_set_schedule(self.h, schedule))
+
+    def cost_record(self, on: bool = True):
+        self._check(lib().vox_cost_record(self.h, 1 if on else 0))
+
+    def cost_samples(self):
+        """(features (n, 16) float32, cycles (n,) float32) recorded so far."""
+        n = C.c_uint32()
+        self._check(lib().vox_cost_samples(self.h, None, None, 0, C.byref(n)))
+        f = np.zeros((n.value, COST_FEATURES), np.float32)
+        c = np.zeros(n.value, np.float32)
+        self._check(lib().vox_cost_samples(self.h, _ptr(f), _ptr(c), n.value, C.byref(n)))
+        return f, c
+
+    def cost_train(self, features, cycles, epochs=30, batch=256, lr=2e-3, seed=0) -> float:
+        f = np.ascontiguousarray(features, np.float32)
+        c = np.ascontiguousarray(cycles, np.float32)
+        mse = C.c_float()
+        self._check(lib().vox_cost_train(self.h, _ptr(f), _ptr(c), c.size, epochs, batch,
+                                         float(lr), seed, C.byref(mse)))
+        return float(mse.value)
+
+    def cost_params(self) -> np.ndarray:
+        p = np.zeros(COST_PARAMS, np.float32)
+        self._check(lib().vox_cost_get_params(self.h, _ptr(p)))
+        return p
+
+    def set_cost_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == COST_PARAMS
+        self._check(lib().vox_cost_set_params(self.h, _ptr(p)))
+
+    def mlp_forward(self, params, x) -> np.ndarray:
+        """MLP outputs for rows of x (n, 16), computed by the HIP kernel."""
+        p = np.ascontiguousarray(params, np.float32)
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, COST_FEATURES)
+        n = x.shape[0]
+        bp, bx, by = DeviceBuffer(self, p.nbytes), DeviceBuffer(self, x.nbytes), DeviceBuffer(self, 4 * n)
+        bp.upload(p)
+        bx.upload(x)
+        self._check(lib().vox_mlp_forward(self.h, bp.ptr, bx.ptr, n, by.ptr))
+        return by.download(np.float32, n)
+
+    def mlp_grad(self, params, x, t):
+        """(gradient of mean squared error, loss) from the HIP kernels."""
+        p = np.ascontiguousarray(params, np.float32)
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, COST_FEATURES)
+        t = np.ascontiguousarray(t, np.float32)
+        n = x.shape[0]
+        bp, bx, bt = DeviceBuffer(self, p.nbytes), DeviceBuffer(self, x.nbytes), DeviceBuffer(self, t.nbytes)
+        bg = DeviceBuffer(self, p.nbytes)
+        bp.upload(p)
+        bx.upload(x)
+        bt.upload(t)
+        loss = C.c_float()
+        self._check(lib().vox_mlp_grad(self.h, bp.ptr, bx.ptr, bt.ptr, n, bg.ptr, C.byref(loss)))
+        return bg.download(np.float32, COST_PARAMS), float(loss.value)
+
 
 __all__ = ["Context", "Grid", "DeviceBuffer", "VoxError", "lib", "device_count",
            "plan_grid_host",
            "NODE_SHORT", "NODE_LONG", "NODE_PARTIAL", "NODE_SHORT_FCC", "NODE_LONG_FCC",
-           "PLAN_MAX_DIMENSION", "PLAN_CUBE_LENGTH", "NODE_SIZE", "EXPORTED"]
+           "PLAN_MAX_DIMENSION", "PLAN_CUBE_LENGTH", "NODE_SIZE", "EXPORTED",
+           "SCHEDULE_PLAIN", "SCHEDULE_LEARNED", "COST_FEATURES", "COST_PARAMS"]

@@ -75,6 +75,15 @@ struct vox_ctx {
     hipEvent_t ev[8] = {};
     vox_stage_times times = {};
     uint32_t mat_slot_cap = 1u << 16;  // bytes per material arena (multiple of 4)
+    // persistent surface launch: compute units and resident bricks per CU
+    int n_cu = 0;
+    int surf_blocks_per_cu = 0;
+    // brick schedule of the surface pass and its cost model
+    int schedule = VOX_SCHEDULE_PLAIN;
+    float *cost_params = nullptr;  // device, kCostParams floats
+    bool cost_ready = false;
+    bool record = false;
+    std::vector<float> rec_feats, rec_cycles;
 };
 
 #define VOX_HIP(expr)                                                               \
@@ -278,6 +287,26 @@ struct SurfOut {
     uint32_t nb[3];
 };
 
+// Appends (features, measured cycles) of the listed bricks to the
+// context's training samples.
+vox_status record_samples(vox_ctx *ctx, const uint32_t *list, const uint32_t *n_list,
+                          const float *feats, const uint64_t *cycles, uint32_t nbt) {
+    uint32_t n = 0;
+    VOX_TRY(read_u32(ctx, n_list, 1, &n));
+    if (n == 0) return VOX_OK;
+    std::vector<uint32_t> ids(n);
+    std::vector<float> f((size_t)n * kCostFeatures);
+    std::vector<uint64_t> cyc(nbt);
+    VOX_HIP(hipMemcpyAsync(ids.data(), list, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    VOX_HIP(hipMemcpyAsync(f.data(), feats, f.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+    VOX_HIP(hipMemcpyAsync(cyc.data(), cycles, (size_t)nbt * 8, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    VOX_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->rec_feats.insert(ctx->rec_feats.end(), f.begin(), f.end());
+    for (uint32_t q = 0; q < n; q++) ctx->rec_cycles.push_back((float)cyc[ids[q]]);
+    return VOX_OK;
+}
+
 vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
                         const uint32_t matSplit[3], const int bdim[3], uint32_t *bits,
                         const char *tag, SurfOut &out) {
@@ -297,10 +326,14 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
     }
     const uint32_t nbt = p.nb[0] * p.nb[1] * p.nb[2];
     std::string t(tag);
-    uint4 *tri_info;
+    uint4 *tri_info, *tri_ext = nullptr;
     uint32_t *cand, *cand_off, *brick_hist, *brick_off, *brick_cursor, *blist, *stats;
     SplitTable *dsplit;
+    const bool learned = ctx->schedule == VOX_SCHEDULE_LEARNED && ctx->cost_ready;
+    const bool features = learned || ctx->record;
+    if (ctx->timing) ctx->times.learned_schedule = learned ? 1u : 0u;
     VOX_TRY(buf(ctx, (t + "_tri_info").c_str(), nt, &tri_info));
+    if (features) VOX_TRY(buf(ctx, "f_tri_ext", nt, &tri_ext));
     VOX_TRY(buf(ctx, "f_cand", nt, &cand));
     VOX_TRY(buf(ctx, "f_cand_off", nt + 1, &cand_off));
     VOX_TRY(buf(ctx, "f_brick_hist", nbt, &brick_hist));
@@ -330,7 +363,7 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
     VOX_HIP(hipMemcpyAsync(dsplit, &st, sizeof(st), hipMemcpyHostToDevice, ctx->stream));
 
     if (nt > 0) {
-        k_surf_setup<<<grid_for(nt, 256), 256, 0, ctx->stream>>>(p, tri_info, cand);
+        k_surf_setup<<<grid_for(nt, 256), 256, 0, ctx->stream>>>(p, tri_info, cand, tri_ext);
         VOX_HIP(hipGetLastError());
     }
     VOX_TRY(scan(ctx, cand, cand_off, nt));
@@ -345,21 +378,73 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
         VOX_HIP(hipGetLastError());
     }
     VOX_TRY(scan(ctx, brick_hist, brick_off, nbt));
+    // Non-empty bricks in index order (the plain schedule of the brick pass).
+    uint32_t *flags, *blist_order, *blist_order_off, *next;
+    VOX_TRY(buf(ctx, "f_flags", nbt, &flags));
+    VOX_TRY(buf(ctx, "f_order", nbt, &blist_order));
+    VOX_TRY(buf(ctx, "f_order_off", nbt + 1, &blist_order_off));
+    VOX_TRY(buf(ctx, "f_next", 1, &next));
+    k_nonzero_flags<<<grid_for(nbt, 256), 256, 0, ctx->stream>>>(brick_hist, flags, nbt);
+    VOX_HIP(hipGetLastError());
+    VOX_TRY(scan(ctx, flags, blist_order_off, nbt));
+    k_nonzero_list<<<grid_for(nbt, 256), 256, 0, ctx->stream>>>(brick_hist, blist_order_off,
+                                                                blist_order, nbt);
+    VOX_HIP(hipGetLastError());
     if (total > 0) {
         k_surf_bin<<<grid_for(total, 256), 256, 0, ctx->stream>>>(
             p, cand_off, cand_off + nt, tri_info, brick_hist, brick_off, brick_cursor, blist, 1);
         VOX_HIP(hipGetLastError());
     }
     if (ctx->timing) ctx->times.surface_pairs += total;
+    // Learned schedule: the listed bricks sorted by predicted cost, most
+    // expensive first (features -> MLP -> 1024-bucket counting sort).
+    const uint32_t *n_list = blist_order_off + nbt;
+    const uint32_t *brick_list = blist_order;
+    float *feats = nullptr;
+    uint64_t *cycles = nullptr;
+    if (features) {
+        VOX_TRY(buf(ctx, "f_feats", (size_t)nbt * kCostFeatures, &feats));
+        k_brick_features<<<grid_for(nbt, 4, 4096), 256, 0, ctx->stream>>>(
+            blist_order, n_list, brick_hist, brick_off, blist, tri_info, tri_ext, bdim[0], bdim[1],
+            bdim[2], feats);
+        VOX_HIP(hipGetLastError());
+    }
+    if (learned) {
+        float *pred;
+        uint32_t *chist, *coff, *ccur, *sorted;
+        VOX_TRY(buf(ctx, "f_pred", nbt, &pred));
+        VOX_TRY(buf(ctx, "f_chist", kCostBuckets, &chist));
+        VOX_TRY(buf(ctx, "f_coff", kCostBuckets + 1, &coff));
+        VOX_TRY(buf(ctx, "f_ccur", kCostBuckets, &ccur));
+        VOX_TRY(buf(ctx, "f_sorted", nbt, &sorted));
+        k_mlp_forward_n<<<(nbt + kMlpBlockRows - 1) / kMlpBlockRows, 256, 0, ctx->stream>>>(
+            ctx->cost_params, feats, n_list, pred);
+        VOX_HIP(hipGetLastError());
+        VOX_TRY(fill(ctx, chist, kCostBuckets, 0u));
+        VOX_TRY(fill(ctx, ccur, kCostBuckets, 0u));
+        k_cost_hist<<<grid_for(nbt, 256, 1024), 256, 0, ctx->stream>>>(pred, n_list, chist);
+        VOX_HIP(hipGetLastError());
+        VOX_TRY(scan(ctx, chist, coff, kCostBuckets));
+        k_cost_scatter<<<grid_for(nbt, 256, 1024), 256, 0, ctx->stream>>>(pred, blist_order,
+                                                                        n_list, coff, ccur, sorted);
+        VOX_HIP(hipGetLastError());
+        brick_list = sorted;
+    }
+    if (ctx->record) VOX_TRY(buf(ctx, "f_cycles", nbt, &cycles));
     for (int attempt = 0; attempt < 3; attempt++) {
         const uint32_t slot_cap = ctx->mat_slot_cap;
         VOX_TRY(buf(ctx, (t + "_mats").c_str(), (size_t)slot_cap * kMatSlots, &out.mat_list));
         VOX_TRY(fill(ctx, stats, kMatSlots * kSlotStride + 1, 0u));
         uint32_t *overflow = stats + kMatSlots * kSlotStride;
-        k_surf_brick<<<nbt, 256, 0, ctx->stream>>>(p, dsplit, tri_info, brick_off, brick_hist,
-                                                   blist, ctx->mats, bits, out.surf,
-                                                   out.brick_mat_off, out.wrank, out.mat_list,
-                                                   stats, slot_cap, overflow);
+        // Empty bricks are not visited: mark them up front.
+        VOX_TRY(fill(ctx, out.brick_mat_off, nbt, UINT32_MAX));
+        VOX_TRY(fill(ctx, next, 1, 0u));
+        const uint32_t blocks =
+            std::min<uint32_t>(nbt, (uint32_t)(ctx->n_cu * ctx->surf_blocks_per_cu));
+        k_surf_brick<<<blocks, 256, 0, ctx->stream>>>(
+            p, dsplit, tri_info, brick_off, brick_hist, blist, ctx->mats, bits, out.surf,
+            out.brick_mat_off, out.wrank, out.mat_list, stats, slot_cap, overflow, brick_list,
+            n_list, next, cycles);
         VOX_HIP(hipGetLastError());
         std::vector<uint32_t> h(kMatSlots * kSlotStride + 1);
         VOX_TRY(read_u32(ctx, stats, h.size(), h.data()));
@@ -371,6 +456,7 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
         }
         if (h.back() == 0u) {
             if (ctx->timing) ctx->times.surface_voxels += (uint32_t)vox;
+            if (ctx->record) VOX_TRY(record_samples(ctx, blist_order, n_list, feats, cycles, nbt));
             return VOX_OK;
         }
         ctx->mat_slot_cap = ((used + used / 4 + 1024u) + 3u) & ~3u;
@@ -453,6 +539,15 @@ vox_status vox_create(int device, vox_ctx **out) {
         return VOX_ERR_HIP;
     }
     for (auto &e : ctx->ev) (void)hipEventCreate(&e);
+    if (hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->surf_blocks_per_cu, k_surf_brick, 256,
+                                                     0) != hipSuccess) {
+        vox_destroy(ctx);
+        return VOX_ERR_HIP;
+    }
+    ctx->n_cu = std::max(ctx->n_cu, 1);
+    ctx->surf_blocks_per_cu = std::max(ctx->surf_blocks_per_cu, 1);
     uint8_t lut[64] = {0};
     for (auto &pr : kOrientPairs) lut[pr[0]] = pr[1];
     if (set_orient_table(lut) != hipSuccess) {
@@ -477,6 +572,7 @@ void vox_destroy(vox_ctx *ctx) {
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->cost_params) (void)hipFree(ctx->cost_params);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -888,6 +984,187 @@ vox_status vox_enable_timing(vox_ctx *ctx, int enable) {
 vox_status vox_get_stage_times(const vox_ctx *ctx, vox_stage_times *out) {
     if (!ctx || !out) return VOX_ERR_INVALID;
     *out = ctx->times;
+    return VOX_OK;
+}
+
+// ---- brick schedule and cost model ----------------------------------------
+
+vox_status vox_set_schedule(vox_ctx *ctx, vox_schedule schedule) {
+    if (!ctx || (schedule != VOX_SCHEDULE_PLAIN && schedule != VOX_SCHEDULE_LEARNED))
+        return VOX_ERR_INVALID;
+    ctx->schedule = schedule;
+    return VOX_OK;
+}
+
+vox_status vox_cost_set_params(vox_ctx *ctx, const float *params) {
+    if (!ctx || !params) return VOX_ERR_INVALID;
+    VOX_HIP(hipSetDevice(ctx->dev));
+    if (!ctx->cost_params) VOX_HIP(hipMalloc(&ctx->cost_params, kCostParams * sizeof(float)));
+    VOX_HIP(hipMemcpyAsync(ctx->cost_params, params, kCostParams * sizeof(float),
+                           hipMemcpyHostToDevice, ctx->stream));
+    VOX_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->cost_ready = true;
+    return VOX_OK;
+}
+
+vox_status vox_cost_get_params(vox_ctx *ctx, float *params) {
+    if (!ctx || !params) return VOX_ERR_INVALID;
+    if (!ctx->cost_ready) return fail(ctx, VOX_ERR_INVALID, "no cost model set");
+    VOX_HIP(hipMemcpyAsync(params, ctx->cost_params, kCostParams * sizeof(float),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    VOX_HIP(hipStreamSynchronize(ctx->stream));
+    return VOX_OK;
+}
+
+vox_status vox_cost_record(vox_ctx *ctx, int enable) {
+    if (!ctx) return VOX_ERR_INVALID;
+    ctx->record = enable != 0;
+    if (ctx->record) {
+        ctx->rec_feats.clear();
+        ctx->rec_cycles.clear();
+    }
+    return VOX_OK;
+}
+
+vox_status vox_cost_samples(vox_ctx *ctx, float *features, float *cycles, uint32_t cap,
+                            uint32_t *n_out) {
+    if (!ctx) return VOX_ERR_INVALID;
+    const uint32_t n = (uint32_t)ctx->rec_cycles.size();
+    if (n_out) *n_out = n;
+    const uint32_t m = std::min(n, cap);
+    if (features)
+        std::memcpy(features, ctx->rec_feats.data(), (size_t)m * kCostFeatures * sizeof(float));
+    if (cycles) std::memcpy(cycles, ctx->rec_cycles.data(), (size_t)m * sizeof(float));
+    return VOX_OK;
+}
+
+vox_status vox_mlp_forward(vox_ctx *ctx, const float *params_dev, const float *x_dev, uint32_t n,
+                           float *y_dev) {
+    if (!ctx || !params_dev || !x_dev || !y_dev) return VOX_ERR_INVALID;
+    if (n == 0) return VOX_OK;
+    VOX_HIP(hipSetDevice(ctx->dev));
+    k_mlp_forward<<<(n + kMlpBlockRows - 1) / kMlpBlockRows, 256, 0, ctx->stream>>>(
+        params_dev, x_dev, n, y_dev);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+namespace {
+// Gradient of (1/n) sum (f(x) - t)^2 over n rows into grad (device); the sum
+// of squared errors into loss_dev (device, may be null).
+vox_status mlp_grad(vox_ctx *ctx, const float *P, const float *X, const float *T, uint32_t n,
+                    float *grad, float *loss_dev) {
+    const uint32_t nblocks = (n + kMlpBlockRows - 1) / kMlpBlockRows;
+    float *partial, *ploss;
+    VOX_TRY(buf(ctx, "m_partial", (size_t)nblocks * kCostParams, &partial));
+    VOX_TRY(buf(ctx, "m_ploss", nblocks, &ploss));
+    k_mlp_grad<<<nblocks, 256, 0, ctx->stream>>>(P, X, T, n, 1.0f / (float)n, partial, ploss);
+    VOX_HIP(hipGetLastError());
+    k_mlp_reduce<<<(kCostParams + 1 + 255) / 256, 256, 0, ctx->stream>>>(partial, ploss, nblocks,
+                                                                           grad, loss_dev);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// splitmix64: deterministic stream for initialisation and shuffling.
+uint64_t splitmix64(uint64_t &s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double uniform01(uint64_t &s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+}  // namespace
+
+vox_status vox_mlp_grad(vox_ctx *ctx, const float *params_dev, const float *x_dev,
+                        const float *t_dev, uint32_t n, float *grad_dev, float *loss) {
+    if (!ctx || !params_dev || !x_dev || !t_dev || !grad_dev || n == 0) return VOX_ERR_INVALID;
+    VOX_HIP(hipSetDevice(ctx->dev));
+    float *ldev;
+    VOX_TRY(buf(ctx, "m_loss", 1, &ldev));
+    VOX_TRY(mlp_grad(ctx, params_dev, x_dev, t_dev, n, grad_dev, ldev));
+    uint32_t bits_;
+    VOX_TRY(read_u32(ctx, (const uint32_t *)ldev, 1, &bits_));
+    float l;
+    std::memcpy(&l, &bits_, 4);
+    if (loss) *loss = l / (float)n;
+    return VOX_OK;
+}
+
+vox_status vox_cost_train(vox_ctx *ctx, const float *features, const float *cycles, uint32_t n,
+                          uint32_t epochs, uint32_t batch, float lr, uint32_t seed,
+                          float *final_mse) {
+    if (!ctx || !features || !cycles || n < kMlpBlockRows || epochs == 0 || batch == 0 ||
+        batch % kMlpBlockRows != 0 || !(lr > 0.0f))
+        return VOX_ERR_INVALID;
+    VOX_HIP(hipSetDevice(ctx->dev));
+    // Standardized log-cycle targets; rows shuffled once (deterministically).
+    double mean = 0.0, var = 0.0;
+    std::vector<double> y(n);
+    for (uint32_t i = 0; i < n; i++) {
+        y[i] = std::log(std::max<double>(cycles[i], 1.0));
+        mean += y[i];
+    }
+    mean /= n;
+    for (uint32_t i = 0; i < n; i++) var += (y[i] - mean) * (y[i] - mean);
+    const double sd = std::max(std::sqrt(var / n), 1e-6);
+    uint64_t rs = 0x5EED0000ull + seed;
+    std::vector<uint32_t> perm(n);
+    for (uint32_t i = 0; i < n; i++) perm[i] = i;
+    for (uint32_t i = n - 1; i > 0; i--) std::swap(perm[i], perm[splitmix64(rs) % (i + 1)]);
+    std::vector<float> X((size_t)n * kCostFeatures), T(n);
+    for (uint32_t i = 0; i < n; i++) {
+        std::memcpy(&X[(size_t)i * kCostFeatures], features + (size_t)perm[i] * kCostFeatures,
+                    kCostFeatures * sizeof(float));
+        T[i] = (float)((y[perm[i]] - mean) / sd);
+    }
+    // He-uniform weights, zero biases.
+    std::vector<float> P(kCostParams, 0.0f);
+    auto init = [&](int off, int count, int fan_in) {
+        const double a = std::sqrt(6.0 / fan_in);
+        for (int k = 0; k < count; k++) P[off + k] = (float)((2.0 * uniform01(rs) - 1.0) * a);
+    };
+    init(0, 64 * 16, 16);
+    init(1088, 64 * 64, 64);
+    init(5248, 64, 64);
+    float *dX, *dT, *dP, *dG, *dM, *dV, *dL;
+    VOX_TRY(buf(ctx, "m_x", X.size(), &dX));
+    VOX_TRY(buf(ctx, "m_t", n, &dT));
+    VOX_TRY(buf(ctx, "m_p", kCostParams, &dP));
+    VOX_TRY(buf(ctx, "m_g", kCostParams, &dG));
+    VOX_TRY(buf(ctx, "m_m", kCostParams, &dM));
+    VOX_TRY(buf(ctx, "m_v", kCostParams, &dV));
+    VOX_TRY(buf(ctx, "m_loss", 1, &dL));
+    VOX_HIP(hipMemcpyAsync(dX, X.data(), X.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    VOX_HIP(hipMemcpyAsync(dT, T.data(), (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+    VOX_HIP(hipMemcpyAsync(dP, P.data(), kCostParams * 4, hipMemcpyHostToDevice, ctx->stream));
+    VOX_TRY(fill(ctx, (uint32_t *)dM, kCostParams, 0u));
+    VOX_TRY(fill(ctx, (uint32_t *)dV, kCostParams, 0u));
+    const float b1 = 0.9f, b2 = 0.999f;
+    double p1 = 1.0, p2 = 1.0;
+    for (uint32_t e = 0; e < epochs; e++)
+        for (uint32_t r0 = 0; r0 < n; r0 += batch) {
+            const uint32_t rows = std::min(batch, n - r0);
+            VOX_TRY(mlp_grad(ctx, dP, dX + (size_t)r0 * kCostFeatures, dT + r0, rows, dG,
+                             nullptr));
+            p1 *= b1;
+            p2 *= b2;
+            k_adam<<<(kCostParams + 255) / 256, 256, 0, ctx->stream>>>(
+                dP, dG, dM, dV, kCostParams, lr, b1, b2, 1e-8f, (float)(1.0 - p1),
+                (float)(1.0 - p2));
+            VOX_HIP(hipGetLastError());
+        }
+    VOX_TRY(mlp_grad(ctx, dP, dX, dT, n, dG, dL));
+    uint32_t lb;
+    VOX_TRY(read_u32(ctx, (const uint32_t *)dL, 1, &lb));
+    float l;
+    std::memcpy(&l, &lb, 4);
+    if (final_mse) *final_mse = l / (float)n;
+    if (!ctx->cost_params) VOX_HIP(hipMalloc(&ctx->cost_params, kCostParams * sizeof(float)));
+    VOX_HIP(hipMemcpyAsync(ctx->cost_params, dP, kCostParams * 4, hipMemcpyDeviceToDevice,
+                           ctx->stream));
+    VOX_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->cost_ready = true;
     return VOX_OK;
 }
 

@@ -1,0 +1,349 @@
+// vox_cost.hip -- learned cost model of the surface brick schedule (gfx950).
+//
+// The surface pass (k_surf_brick) takes its bricks from a list.  The plain
+// schedule lists the non-empty bricks in index order; the learned schedule
+// sorts them by a cost predicted from 16 per-brick features by a small MLP
+// (16 -> 64 -> 64 -> 1, ReLU, FP32), so the expensive bricks start first and
+// the persistent workgroups finish together.  Training targets are the
+// per-brick durations the brick pass records (s_memtime) on seeded meshes.
+// Both schedules produce identical output: the order only decides which
+// workgroup processes a brick and when.
+//
+// Kernels: per-brick features, MLP forward, MLP gradient (per-block partial
+// sums reduced in a fixed order, so training is deterministic), Adam, and a
+// 1024-bucket counting sort of the bricks by predicted cost.
+#include "vox_kernels.h"
+
+namespace vk {
+
+// ---------------------------------------------------------------------------
+// Features of one brick, from the (triangle, brick) pairs of its list and
+// the per-triangle class, brick box and voxel extents of k_surf_setup:
+//  0 log1p(triangles)            1-3 log1p(1D / 2D / 3D triangles)
+//  4-6 log1p(estimated items of 1D / 2D / 3D triangles in the brick)
+//  7 log1p(estimated 3D voxel tests)   8 log1p(largest estimated items)
+//  9 log1p(64-triangle chunks)   10-11 share of 3D triangles with axis x / y
+// 12 log1p(bricks spanned, summed)  13 estimated items per brick voxel
+// 14 log1p(triangles with <= 4 items)  15 constant 1
+// A triangle's items in the brick are estimated as its items spread evenly
+// over the bricks its box spans across its dominant plane.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int d = kWave / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int d = kWave / 2; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d));
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_brick_features(
+    const uint32_t *__restrict__ order, const uint32_t *__restrict__ n_order,
+    const uint32_t *__restrict__ brick_hist, const uint32_t *__restrict__ brick_off,
+    const uint32_t *__restrict__ blist, const uint4 *__restrict__ tri_info,
+    const uint4 *__restrict__ tri_ext, int32_t bd0, int32_t bd1, int32_t bd2,
+    float *__restrict__ feats) {
+    const uint32_t n = *n_order;
+    const uint32_t lane = lane_id();
+    const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
+    const float bvox = (float)(bd0 * bd1 * bd2);
+    for (uint32_t q = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < n; q += nwaves) {
+        const uint32_t b = order[q];
+        const uint32_t cnt = brick_hist[b], off = brick_off[b];
+        float c1 = 0, c2 = 0, c3 = 0, i1 = 0, i2 = 0, i3 = 0, v3 = 0, mx = 0, span = 0,
+              small = 0, axx = 0, axy = 0;
+        for (uint32_t i = lane; i < cnt; i += kWave) {
+            const uint32_t t = blist[off + i];
+            const uint4 ti = tri_info[t];
+            const uint4 te = tri_ext[t];
+            const int bb = (int)(ti.x >> 30), a = (int)((ti.x >> 28) & 3u);
+            const uint32_t wbx = ti.z >> 16, wby = ti.w & 0xFFFFu, wbz = ti.w >> 16;
+            const uint32_t wa = a == AX_X ? wbx : a == AX_Y ? wby : wbz;
+            const uint32_t wuv = (wbx * wby * wbz) / max(wa, 1u);
+            const int32_t bda = a == AX_X ? bd0 : a == AX_Y ? bd1 : bd2;
+            float est;
+            if (bb == BB_1D) {
+                est = (float)te.w / (float)max(wa, 1u);
+                c1 += 1.0f;
+                i1 += est;
+            } else {
+                est = (float)te.w / (float)max(wuv, 1u);
+                if (bb == BB_2D) {
+                    c2 += 1.0f;
+                    i2 += est;
+                } else {
+                    c3 += 1.0f;
+                    i3 += est;
+                    v3 += est * (float)min((int32_t)te.z, bda);
+                    axx += a == AX_X ? 1.0f : 0.0f;
+                    axy += a == AX_Y ? 1.0f : 0.0f;
+                }
+            }
+            mx = fmaxf(mx, est);
+            span += (float)(wbx * wby * wbz);
+            small += est <= 4.0f ? 1.0f : 0.0f;
+        }
+        c1 = wave_sum(c1); c2 = wave_sum(c2); c3 = wave_sum(c3);
+        i1 = wave_sum(i1); i2 = wave_sum(i2); i3 = wave_sum(i3);
+        v3 = wave_sum(v3); mx = wave_max(mx); span = wave_sum(span);
+        small = wave_sum(small); axx = wave_sum(axx); axy = wave_sum(axy);
+        if (lane == 0) {
+            float *f = feats + (size_t)q * kCostFeatures;
+            f[0] = log1pf((float)cnt);
+            f[1] = log1pf(c1);
+            f[2] = log1pf(c2);
+            f[3] = log1pf(c3);
+            f[4] = log1pf(i1);
+            f[5] = log1pf(i2);
+            f[6] = log1pf(i3);
+            f[7] = log1pf(v3);
+            f[8] = log1pf(mx);
+            f[9] = log1pf((float)((cnt + 63u) / 64u));
+            f[10] = c3 > 0.0f ? axx / c3 : 0.0f;
+            f[11] = c3 > 0.0f ? axy / c3 : 0.0f;
+            f[12] = log1pf(span);
+            f[13] = (i1 + i2 + i3) / bvox;
+            f[14] = log1pf(small);
+            f[15] = 1.0f;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MLP.  Parameter layout (kCostParams = 5313 floats): W1[64][16] at 0,
+// b1[64] at 1024, W2[64][64] at 1088, b2[64] at 5184, W3[64] at 5248, b3 at
+// 5312.  A workgroup of 256 threads takes kMlpRows samples: lane j is hidden
+// neuron j, the four waves split the samples.  Weights sit in LDS padded to
+// odd row strides (17, 65 floats) so both row and column walks are free of
+// bank conflicts.  Sums run in a fixed order (inputs in index order, the
+// output layer as a wave butterfly).
+// ---------------------------------------------------------------------------
+constexpr int kIn = kCostFeatures, kHid = 64, kMlpRows = (int)kMlpBlockRows;
+constexpr int kOffB1 = 1024, kOffW2 = 1088, kOffB2 = 5184, kOffW3 = 5248, kOffB3 = 5312;
+
+struct MlpShared {
+    float w1[kHid][kIn + 1];
+    float w2[kHid][kHid + 1];
+    float b1[kHid], b2[kHid], w3[kHid];
+    float b3;
+    float x[kMlpRows][kIn];
+    float h1[kMlpRows][kHid];
+    float h2[kMlpRows][kHid];
+};
+
+__device__ void mlp_load(MlpShared &S, const float *__restrict__ P, const float *__restrict__ X,
+                         uint32_t row0, uint32_t n) {
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t k = tid; k < kHid * kIn; k += blockDim.x) S.w1[k / kIn][k % kIn] = P[k];
+    for (uint32_t k = tid; k < kHid * kHid; k += blockDim.x)
+        S.w2[k / kHid][k % kHid] = P[kOffW2 + k];
+    if (tid < (uint32_t)kHid) {
+        S.b1[tid] = P[kOffB1 + tid];
+        S.b2[tid] = P[kOffB2 + tid];
+        S.w3[tid] = P[kOffW3 + tid];
+    }
+    if (tid == 0) S.b3 = P[kOffB3];
+    for (uint32_t k = tid; k < kMlpRows * kIn; k += blockDim.x) {
+        const uint32_t r = row0 + k / kIn;
+        S.x[k / kIn][k % kIn] = r < n ? X[(size_t)r * kIn + k % kIn] : 0.0f;
+    }
+    __syncthreads();
+}
+
+// Hidden layers for the block's rows; returns the outputs of rows
+// wave, wave+4, ... in y[] (valid in every lane).
+__device__ void mlp_hidden(MlpShared &S, float y[kMlpRows / 4]) {
+    const uint32_t j = lane_id(), w = threadIdx.x / kWave;
+    for (int k = 0; k < kMlpRows / 4; k++) {
+        const int s = (int)w + 4 * k;
+        float z = S.b1[j];
+        for (int i = 0; i < kIn; i++) z += S.w1[j][i] * S.x[s][i];
+        S.h1[s][j] = fmaxf(z, 0.0f);
+    }
+    __syncthreads();
+    for (int k = 0; k < kMlpRows / 4; k++) {
+        const int s = (int)w + 4 * k;
+        float z = S.b2[j];
+        for (int i = 0; i < kHid; i++) z += S.w2[j][i] * S.h1[s][i];
+        const float h = fmaxf(z, 0.0f);
+        S.h2[s][j] = h;
+        y[k] = wave_sum(S.w3[j] * h) + S.b3;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ P,
+                                                     const float *__restrict__ X, uint32_t n,
+                                                     float *__restrict__ Y) {
+    __shared__ MlpShared S;
+    const uint32_t row0 = blockIdx.x * kMlpRows;
+    mlp_load(S, P, X, row0, n);
+    float y[kMlpRows / 4];
+    mlp_hidden(S, y);
+    const uint32_t w = threadIdx.x / kWave;
+    if (lane_id() == 0)
+        for (int k = 0; k < kMlpRows / 4; k++) {
+            const uint32_t r = row0 + w + 4 * k;
+            if (r < n) Y[r] = y[k];
+        }
+}
+
+// The same with the row count read from device memory (rows of a brick list
+// whose length only the device knows).
+__global__ __launch_bounds__(256) void k_mlp_forward_n(const float *__restrict__ P,
+                                                       const float *__restrict__ X,
+                                                       const uint32_t *__restrict__ n_dev,
+                                                       float *__restrict__ Y) {
+    __shared__ MlpShared S;
+    const uint32_t n = *n_dev;
+    const uint32_t row0 = blockIdx.x * kMlpRows;
+    if (row0 >= n) return;  // uniform over the block
+    mlp_load(S, P, X, row0, n);
+    float y[kMlpRows / 4];
+    mlp_hidden(S, y);
+    const uint32_t w = threadIdx.x / kWave;
+    if (lane_id() == 0)
+        for (int k = 0; k < kMlpRows / 4; k++) {
+            const uint32_t r = row0 + w + 4 * k;
+            if (r < n) Y[r] = y[k];
+        }
+}
+
+// Gradient of the mean squared error over rows [0, n) (scaled by 1/n_scale)
+// for the block's rows: the block writes its partial sums to
+// partial[block][kCostParams] and its squared-error sum to ploss[block].
+__global__ __launch_bounds__(256) void k_mlp_grad(const float *__restrict__ P,
+                                                  const float *__restrict__ X,
+                                                  const float *__restrict__ T, uint32_t n,
+                                                  float inv_scale, float *__restrict__ partial,
+                                                  float *__restrict__ ploss) {
+    __shared__ MlpShared S;
+    __shared__ float dz2[kMlpRows][kHid];
+    __shared__ float dz1[kMlpRows][kHid];
+    __shared__ float dy[kMlpRows];
+    __shared__ float lsum[4];
+    const uint32_t row0 = blockIdx.x * kMlpRows;
+    mlp_load(S, P, X, row0, n);
+    float y[kMlpRows / 4];
+    mlp_hidden(S, y);
+    const uint32_t j = lane_id(), w = threadIdx.x / kWave;
+    float l = 0.0f;
+    for (int k = 0; k < kMlpRows / 4; k++) {
+        const int s = (int)w + 4 * k;
+        const uint32_t r = row0 + (uint32_t)s;
+        const float e = r < n ? y[k] - T[r] : 0.0f;
+        l += e * e;
+        const float g = 2.0f * e * inv_scale;
+        if (j == 0) dy[s] = g;
+        dz2[s][j] = S.h2[s][j] > 0.0f ? g * S.w3[j] : 0.0f;
+    }
+    if (j == 0) lsum[w] = l;
+    __syncthreads();
+    for (int k = 0; k < kMlpRows / 4; k++) {
+        const int s = (int)w + 4 * k;
+        float d = 0.0f;  // dL/dh1[s][j] = sum_k W2[k][j] dz2[s][k]
+        for (int q = 0; q < kHid; q++) d += S.w2[q][j] * dz2[s][q];
+        dz1[s][j] = S.h1[s][j] > 0.0f ? d : 0.0f;
+    }
+    __syncthreads();
+    float *out = partial + (size_t)blockIdx.x * kCostParams;
+    // dW1[j][i], i = 4w .. 4w+3
+    for (int ii = 0; ii < kIn / 4; ii++) {
+        const int i = (int)w * (kIn / 4) + ii;
+        float g = 0.0f;
+        for (int s = 0; s < kMlpRows; s++) g += dz1[s][j] * S.x[s][i];
+        out[j * kIn + i] = g;
+    }
+    // dW2[j][i], i = 16w .. 16w+15
+    for (int ii = 0; ii < kHid / 4; ii++) {
+        const int i = (int)w * (kHid / 4) + ii;
+        float g = 0.0f;
+        for (int s = 0; s < kMlpRows; s++) g += dz2[s][j] * S.h1[s][i];
+        out[kOffW2 + j * kHid + i] = g;
+    }
+    float g = 0.0f;
+    if (w == 0) {
+        for (int s = 0; s < kMlpRows; s++) g += dz1[s][j];
+        out[kOffB1 + j] = g;
+    } else if (w == 1) {
+        for (int s = 0; s < kMlpRows; s++) g += dz2[s][j];
+        out[kOffB2 + j] = g;
+    } else if (w == 2) {
+        for (int s = 0; s < kMlpRows; s++) g += dy[s] * S.h2[s][j];
+        out[kOffW3 + j] = g;
+    } else if (j == 0) {
+        for (int s = 0; s < kMlpRows; s++) g += dy[s];
+        out[kOffB3] = g;
+        ploss[blockIdx.x] = (lsum[0] + lsum[1]) + (lsum[2] + lsum[3]);
+    }
+}
+
+// grad[k] = sum over blocks b (in order) of partial[b][k]; loss likewise.
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float *__restrict__ partial,
+                                                    const float *__restrict__ ploss,
+                                                    uint32_t nblocks, float *__restrict__ grad,
+                                                    float *__restrict__ loss) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < (uint32_t)kCostParams) {
+        float g = 0.0f;
+        for (uint32_t b = 0; b < nblocks; b++) g += partial[(size_t)b * kCostParams + k];
+        grad[k] = g;
+    } else if (k == (uint32_t)kCostParams && loss) {
+        float l = 0.0f;
+        for (uint32_t b = 0; b < nblocks; b++) l += ploss[b];
+        *loss = l;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_adam(float *__restrict__ P, const float *__restrict__ G,
+                                              float *__restrict__ M, float *__restrict__ V,
+                                              uint32_t n, float lr, float b1, float b2,
+                                              float eps, float c1, float c2) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float g = G[k];
+    const float m = b1 * M[k] + (1.0f - b1) * g;
+    const float v = b2 * V[k] + (1.0f - b2) * g * g;
+    M[k] = m;
+    V[k] = v;
+    P[k] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// Bricks by predicted cost, most expensive first: bucket = (4 - y) * 128
+// clamped to [0, 1023] (y is the standardized log cost), then a counting
+// sort (histogram, scan, scatter).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t cost_bucket(float y) {
+    const float f = (4.0f - y) * 128.0f;
+    if (!(f > 0.0f)) return 0u;  // also NaN
+    return f >= 1023.0f ? 1023u : (uint32_t)f;
+}
+
+__global__ __launch_bounds__(256) void k_cost_hist(const float *__restrict__ Y,
+                                                   const uint32_t *__restrict__ n_order,
+                                                   uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[kCostBuckets];
+    for (uint32_t i = threadIdx.x; i < kCostBuckets; i += blockDim.x) h[i] = 0u;
+    __syncthreads();
+    const uint32_t n = *n_order;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        atomicAdd(&h[cost_bucket(Y[i])], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kCostBuckets; i += blockDim.x)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+__global__ __launch_bounds__(256) void k_cost_scatter(const float *__restrict__ Y,
+                                                      const uint32_t *__restrict__ in,
+                                                      const uint32_t *__restrict__ n_order,
+                                                      const uint32_t *__restrict__ off,
+                                                      uint32_t *__restrict__ cursor,
+                                                      uint32_t *__restrict__ out) {
+    const uint32_t n = *n_order;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t k = cost_bucket(Y[i]);
+        out[off[k] + atomicAdd(&cursor[k], 1u)] = in[i];
+    }
+}
+
+}  // namespace vk

@@ -58,6 +58,11 @@ static_assert(kCartBZ % kNodeZ == 0, "node planes must not straddle a brick");
 // Material-list arenas of the surface stage: kMatSlots counters, each on its
 // own 128-byte line (kSlotStride words): {bytes used, surface voxels}.
 constexpr uint32_t kMatSlots = 64, kSlotStride = 32;
+// Cost model: features per brick, MLP parameters (16-64-64-1), sort buckets.
+constexpr int kCostFeatures = 16;
+constexpr int kCostParams = 64 * 16 + 64 + 64 * 64 + 64 + 64 + 1;  // 5313
+constexpr uint32_t kCostBuckets = 1024;
+constexpr uint32_t kMlpBlockRows = 32;  // samples per MLP workgroup
 constexpr int kMaxSplits = 64;
 
 struct SplitTable {
@@ -190,7 +195,7 @@ __global__ void k_zero_pad_rows(uint32_t *__restrict__ bits, uint32_t words_per_
 // Surface
 // ---------------------------------------------------------------------------
 __global__ void k_surf_setup(SurfParams p, uint4 *__restrict__ tri_info,
-                             uint32_t *__restrict__ cand_count);
+                             uint32_t *__restrict__ cand_count, uint4 *__restrict__ tri_ext);
 __global__ void k_surf_bin(SurfParams p, const uint32_t *__restrict__ cand_off,
                            const uint32_t *__restrict__ cand_total,
                            const uint4 *__restrict__ tri_info,
@@ -206,7 +211,13 @@ __global__ void k_surf_brick(SurfParams p, const SplitTable *__restrict__ splits
                              uint32_t *__restrict__ surf, uint32_t *__restrict__ brick_mat_off,
                              uint16_t *__restrict__ wrank, uint8_t *__restrict__ mat_list,
                              uint32_t *__restrict__ slots, uint32_t slot_cap,
-                             uint32_t *__restrict__ overflow);
+                             uint32_t *__restrict__ overflow, const uint32_t *__restrict__ order,
+                             const uint32_t *__restrict__ n_order, uint32_t *__restrict__ next,
+                             uint64_t *__restrict__ cycles);
+__global__ void k_nonzero_flags(const uint32_t *__restrict__ hist, uint32_t *__restrict__ flags,
+                                uint32_t n);
+__global__ void k_nonzero_list(const uint32_t *__restrict__ hist, const uint32_t *__restrict__ off,
+                               uint32_t *__restrict__ list, uint32_t n);
 
 // ---------------------------------------------------------------------------
 // Nodes
@@ -238,5 +249,33 @@ __global__ void k_nodes_fcc(GridDims g, const uint32_t *s1, const uint32_t *s2,
                             const uint32_t *s3, const uint32_t *s4, MatSource m1, MatSource m2,
                             MatSource m3, MatSource m4, int use_mats, int kind,
                             void *__restrict__ nodes, uint32_t nbx, uint32_t nby);
+
+// Cost model kernels (vox_cost.hip).
+__global__ void k_brick_features(const uint32_t *__restrict__ order,
+                                 const uint32_t *__restrict__ n_order,
+                                 const uint32_t *__restrict__ brick_hist,
+                                 const uint32_t *__restrict__ brick_off,
+                                 const uint32_t *__restrict__ blist,
+                                 const uint4 *__restrict__ tri_info,
+                                 const uint4 *__restrict__ tri_ext, int32_t bd0, int32_t bd1,
+                                 int32_t bd2, float *__restrict__ feats);
+__global__ void k_mlp_forward(const float *__restrict__ P, const float *__restrict__ X, uint32_t n,
+                              float *__restrict__ Y);
+__global__ void k_mlp_forward_n(const float *__restrict__ P, const float *__restrict__ X,
+                                const uint32_t *__restrict__ n_dev, float *__restrict__ Y);
+__global__ void k_mlp_grad(const float *__restrict__ P, const float *__restrict__ X,
+                           const float *__restrict__ T, uint32_t n, float inv_scale,
+                           float *__restrict__ partial, float *__restrict__ ploss);
+__global__ void k_mlp_reduce(const float *__restrict__ partial, const float *__restrict__ ploss,
+                             uint32_t nblocks, float *__restrict__ grad, float *__restrict__ loss);
+__global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float *__restrict__ M,
+                       float *__restrict__ V, uint32_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2);
+__global__ void k_cost_hist(const float *__restrict__ Y, const uint32_t *__restrict__ n_order,
+                            uint32_t *__restrict__ hist);
+__global__ void k_cost_scatter(const float *__restrict__ Y, const uint32_t *__restrict__ in,
+                               const uint32_t *__restrict__ n_order,
+                               const uint32_t *__restrict__ off, uint32_t *__restrict__ cursor,
+                               uint32_t *__restrict__ out);
 
 }  // namespace vk

@@ -322,6 +322,17 @@ __device__ __forceinline__ bool surf_range(const SurfParams &p, const F3 tri[3],
     return true;
 }
 
+// Axis helpers: for dominant axis a, u and v are the other two axes in
+// increasing order.
+__device__ __forceinline__ int axis_u(int a) { return a == AX_X ? AX_Y : AX_X; }
+__device__ __forceinline__ int axis_v(int a) { return a == AX_Z ? AX_Y : AX_Z; }
+__device__ __forceinline__ float pick3(const F3 &p, int a) {
+    return a == AX_X ? p.x : a == AX_Y ? p.y : p.z;
+}
+__device__ __forceinline__ int32_t pick3i(const I3 &p, int a) {
+    return a == AX_X ? p.x : a == AX_Y ? p.y : p.z;
+}
+
 // Bricks are laid over alloc coordinates (ext + 1).
 __device__ __forceinline__ void brick_range(const SurfParams &p, const VoxBB &vb, uint32_t lo[3],
                                             uint32_t hi[3]) {
@@ -335,9 +346,12 @@ __device__ __forceinline__ void brick_range(const SurfParams &p, const VoxBB &vb
 
 // Per triangle: class and dominant axis (classifyTriangles), and the box of
 // bricks its clipped voxel range touches, packed as
-// {info, lo.x | lo.y<<16, lo.z | w.x<<16, w.y | w.z<<16}.
+// {info, lo.x | lo.y<<16, lo.z | w.x<<16, w.y | w.z<<16}.  With tri_ext,
+// also its clipped voxel extents along (u, v, a) and its work items (voxels
+// for 1D, dominant-plane columns otherwise), the cost model's inputs.
 __global__ __launch_bounds__(256) void k_surf_setup(SurfParams p, uint4 *__restrict__ tri_info,
-                                                    uint32_t *__restrict__ cand_count) {
+                                                    uint32_t *__restrict__ cand_count,
+                                                    uint4 *__restrict__ tri_ext) {
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < p.nt; t += gridDim.x * blockDim.x) {
         F3 tri[3];
         load_tri_f(p, t, tri);
@@ -346,16 +360,22 @@ __global__ __launch_bounds__(256) void k_surf_setup(SurfParams p, uint4 *__restr
         uint32_t c = 0;
         VoxBB vb;
         uint32_t lo[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+        uint4 ext = make_uint4(0u, 0u, 0u, 0u);
         if (bb != BB_DEGEN && surf_range(p, tri, vb)) {
             uint32_t hi[3];
             brick_range(p, vb, lo, hi);
             for (int a = 0; a < 3; a++) w[a] = hi[a] - lo[a] + 1u;
             c = w[0] * w[1] * w[2];
+            const uint32_t eu = (uint32_t)(pick3i(vb.max, axis_u(axis)) - pick3i(vb.min, axis_u(axis)) + 1);
+            const uint32_t ev = (uint32_t)(pick3i(vb.max, axis_v(axis)) - pick3i(vb.min, axis_v(axis)) + 1);
+            const uint32_t ea = (uint32_t)(pick3i(vb.max, axis) - pick3i(vb.min, axis) + 1);
+            ext = make_uint4(eu, ev, ea, bb == BB_1D ? ea : eu * ev);
         }
         const uint32_t info = ((uint32_t)bb << 30) | ((uint32_t)axis << 28);
         tri_info[t] = make_uint4(info, lo[0] | (lo[1] << 16), lo[2] | (w[0] << 16),
                                  w[1] | (w[2] << 16));
         cand_count[t] = c;
+        if (tri_ext) tri_ext[t] = ext;
     }
 }
 
@@ -414,271 +434,389 @@ __global__ __launch_bounds__(256) void k_surf_bin(
     }
 }
 
-__device__ __forceinline__ int32_t split_hi_of(const SplitTable &s, int axis, int32_t i,
-                                               int32_t &lo_out) {
-    const uint32_t n = s.n[axis];
-    for (uint32_t k = 0; k < n; k++) {
-        if ((uint32_t)i < s.hi[axis][k]) {
-            lo_out = (int32_t)s.lo[axis][k];
-            return (int32_t)s.hi[axis][k];
-        }
-    }
-    lo_out = (int32_t)s.lo[axis][n - 1];
-    return (int32_t)s.hi[axis][n - 1];
-}
-
-struct BrickBox {
-    int32_t lo[3], hi[3];  // inclusive, ext coordinates
-};
-
-__device__ __forceinline__ void lds_write(uint32_t *keys, const BrickBox &bb, const int32_t bd[3],
-                                          int32_t x, int32_t y, int32_t z, uint32_t key) {
-    if (x < bb.lo[0] || x > bb.hi[0] || y < bb.lo[1] || y > bb.hi[1] || z < bb.lo[2] ||
-        z > bb.hi[2])
-        return;
-    const int32_t lx = x - bb.lo[0], ly = y - bb.lo[1], lz = z - bb.lo[2];
-    atomicMax(&keys[(lz * bd[1] + ly) * bd[0] + lx], key);
-}
-
-// Per-triangle data of one brick's chunk, set up once in LDS.
+// Per-triangle data of one brick's chunk, set up once in LDS.  Coordinates
+// are named (u, v, a): a is the triangle's dominant axis, u and v the other
+// two in increasing axis order (u = x unless a = x; v = z unless a = z).
+// Every 2-term edge function of the reference, (ne.x*c0 + ne.y*c1) + de, is
+// kept as (c_first*p_first + c_second*p_second) + de with the two products
+// in either order: IEEE addition is commutative, so the value is identical.
 struct SurfTri {
-    OverlapData o[3];          // projections YZ (0), ZX (1), XY (2)
-    F3 n, t0;                  // normal, first vertex
-    int32_t vmin[3], vmax[3];  // voxel range clipped to the ext space
-    int32_t rlo[3], rhi[3];    // part of it inside this brick
+    float dom[3][3];    // dominant-plane edges: (coef of u, coef of v, de)
+    float pl[2][3][3];  // the two other planes (3D): (coef of a, coef of other, de)
+    float n[3], t0[3];  // normal, first vertex (x, y, z)
+    int32_t lo[3], hi[3];      // brick-clipped voxel range, (u, v, a) order
+    int32_t vmin_a, vmax_a;    // unclipped range along a
+    int32_t s[3];              // LDS key strides of (u, v, a)
+    int32_t base;              // LDS key index of (lo_u, lo_v, lo_a)
+    int32_t wu;                // columns along u
+    float rwu;                 // 1 / wu
     uint32_t key;              // bbType<<30 | triangle
-    int32_t bb, axis;
+    uint32_t flags;            // bb | axis<<2 | (plane0 other is v)<<4 | (plane1 other is v)<<5
 };
 constexpr int kSurfChunk = 64;
 
-__device__ __forceinline__ bool test_plane(const OverlapData &o, int plane, const float pc[3]) {
-    return plane == 0 ? overlap2(o, pc[1], pc[2])
-         : plane == 1 ? overlap2(o, pc[2], pc[0])
-                      : overlap2(o, pc[0], pc[1]);
-}
 
-// Sets up triangle t for the brick; returns its number of work items in the
-// brick (voxels along the axis for 1D, dominant-plane columns for 2D/3D).
-__device__ uint32_t surf_tri_setup(const SurfParams &p, uint32_t t, uint32_t info,
-                                   const BrickBox &bx, SurfTri &st) {
+// Sets up triangle t for the brick whose alloc-coordinate box starts at
+// (ext coordinates) (blx, bly, blz); returns its work items in the brick: voxels along
+// the axis for 1D, dominant-plane columns for 2D/3D.
+__device__ __forceinline__ uint32_t surf_tri_setup(const SurfParams &p, uint32_t t, uint32_t info,
+                                                   int32_t blx, int32_t bly, int32_t blz,
+                                                   SurfTri &st) {
     F3 tri[3];
     load_tri_f(p, t, tri);
     VoxBB vb;
     if (!surf_range(p, tri, vb)) return 0u;
-    st.bb = (int32_t)(info >> 30);
-    st.axis = (int32_t)((info >> 28) & 3u);
-    st.key = ((uint32_t)st.bb << 30) | t;
-    st.vmin[0] = vb.min.x; st.vmin[1] = vb.min.y; st.vmin[2] = vb.min.z;
-    st.vmax[0] = vb.max.x; st.vmax[1] = vb.max.y; st.vmax[2] = vb.max.z;
-    for (int a = 0; a < 3; a++) {
-        st.rlo[a] = max(st.vmin[a], bx.lo[a]);
-        st.rhi[a] = min(st.vmax[a], bx.hi[a]);
-        if (st.rlo[a] > st.rhi[a]) return 0u;
+    const int bb = (int)(info >> 30), a = (int)((info >> 28) & 3u);
+    const int ua = axis_u(a), va = axis_v(a);
+    const int32_t bd[3] = {p.bdim[0], p.bdim[1], p.bdim[2]};
+    const int32_t stride[3] = {1, bd[0], bd[0] * bd[1]};
+    const int ax3[3] = {ua, va, a};
+    int32_t lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int ax = ax3[k];
+        const int32_t bl = ax == AX_X ? blx : ax == AX_Y ? bly : blz;
+        const int32_t bs = ax == AX_X ? bd[0] : ax == AX_Y ? bd[1] : bd[2];
+        lo[k] = max(pick3i(vb.min, ax), bl);
+        hi[k] = min(pick3i(vb.max, ax), bl + bs - 1);
+        if (lo[k] > hi[k]) return 0u;
+        st.lo[k] = lo[k];
+        st.hi[k] = hi[k];
+        st.s[k] = ax == AX_X ? stride[0] : ax == AX_Y ? stride[1] : stride[2];
     }
-    const int ax = st.axis;
-    if (st.bb == BB_1D) return (uint32_t)(st.rhi[ax] - st.rlo[ax] + 1);
-    st.n = fcross(fsub(tri[0], tri[2]), fsub(tri[1], tri[0]));
-    st.t0 = tri[0];
-    setup_overlap(tri, st.n, p.d, 0, st.o[0]);
-    setup_overlap(tri, st.n, p.d, 1, st.o[1]);
-    setup_overlap(tri, st.n, p.d, 2, st.o[2]);
-    const int ua = ax == AX_X ? AX_Y : AX_X;
-    const int va = ax == AX_Z ? AX_Y : AX_Z;
-    return (uint32_t)((st.rhi[ua] - st.rlo[ua] + 1) * (st.rhi[va] - st.rlo[va] + 1));
+    st.base = (lo[0] - (ua == AX_X ? blx : ua == AX_Y ? bly : blz)) * st.s[0] +
+              (lo[1] - (va == AX_X ? blx : va == AX_Y ? bly : blz)) * st.s[1] +
+              (lo[2] - (a == AX_X ? blx : a == AX_Y ? bly : blz)) * st.s[2];
+    st.vmin_a = pick3i(vb.min, a);
+    st.vmax_a = pick3i(vb.max, a);
+    st.key = ((uint32_t)bb << 30) | t;
+    st.wu = hi[0] - lo[0] + 1;
+    st.rwu = 1.0f / (float)st.wu;
+    uint32_t flags = (uint32_t)bb | ((uint32_t)a << 2);
+    if (bb == BB_1D) {
+        st.flags = flags;
+        return (uint32_t)(hi[2] - lo[2] + 1);
+    }
+    const F3 n = fcross(fsub(tri[0], tri[2]), fsub(tri[1], tri[0]));
+    st.n[0] = n.x; st.n[1] = n.y; st.n[2] = n.z;
+    st.t0[0] = tri[0].x; st.t0[1] = tri[0].y; st.t0[2] = tri[0].z;
+    // Plane P tests coordinates (c0, c1): plane 0 (y, z), 1 (z, x), 2 (x, y).
+    auto first_axis = [](int P) { return P == 0 ? AX_Y : P == 1 ? AX_Z : AX_X; };
+    auto second_axis = [](int P) { return P == 0 ? AX_Z : P == 1 ? AX_X : AX_Y; };
+    {
+        OverlapData o;
+        setup_overlap(tri, n, p.d, a, o);
+        const bool first_is_u = first_axis(a) == ua;
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            st.dom[e][0] = first_is_u ? o.ne[e].x : o.ne[e].y;
+            st.dom[e][1] = first_is_u ? o.ne[e].y : o.ne[e].x;
+            st.dom[e][2] = o.de[e];
+        }
+    }
+    if (bb == BB_3D) {
+        const int planes[2] = {a == AX_X ? 1 : 0, a == AX_Z ? 1 : 2};
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            OverlapData o;
+            setup_overlap(tri, n, p.d, planes[q], o);
+            const bool first_is_a = first_axis(planes[q]) == a;
+            const int other = first_is_a ? second_axis(planes[q]) : first_axis(planes[q]);
+            if (other == va) flags |= 1u << (4 + q);
+#pragma unroll
+            for (int e = 0; e < 3; e++) {
+                st.pl[q][e][0] = first_is_a ? o.ne[e].x : o.ne[e].y;
+                st.pl[q][e][1] = first_is_a ? o.ne[e].y : o.ne[e].x;
+                st.pl[q][e][2] = o.de[e];
+            }
+        }
+    }
+    st.flags = flags;
+    return (uint32_t)(st.wu * (hi[1] - lo[1] + 1));
 }
 
-// One work item: per the 1D/2D/3D kernels of the reference
+__device__ __forceinline__ void key_write(uint32_t *keys, uint8_t *touched, int32_t idx,
+                                          int32_t nvox, int32_t row_shift, uint32_t key) {
+    if ((uint32_t)idx >= (uint32_t)nvox) return;  // unreachable: items lie in the brick
+    atomicMax(&keys[idx], key);
+    touched[idx >> row_shift] = 1;
+}
+
+// One work item, following the 1D/2D/3D kernels of the reference
 // (src/voxelizer.cu:3307-3955) with their matSplitRes subspace clipping --
 // only the 3D single-voxel shortcut depends on the subspace, through the
 // dominant-axis interval the voxel falls in.
-__device__ void surf_item(const SurfParams &p, const SplitTable &spl, const SurfTri &st,
-                          uint32_t k, const BrickBox &bx, uint32_t *keys) {
-    const int ax = st.axis;
-    if (st.bb == BB_1D) {
-        int32_t c[3] = {st.vmin[0], st.vmin[1], st.vmin[2]};
-        c[ax] = st.rlo[ax] + (int32_t)k;
-        lds_write(keys, bx, p.bdim, c[0], c[1], c[2], st.key);
+__device__ __forceinline__ void surf_item(const SurfParams &p, const SplitTable &spl, const SurfTri &st,
+                          uint32_t k, uint32_t *keys, uint8_t *touched, int32_t nvox,
+                          int32_t row_shift) {
+    const uint32_t flags = st.flags;
+    const int bb = (int)(flags & 3u), a = (int)((flags >> 2) & 3u);
+    if (bb == BB_1D) {
+        key_write(keys, touched, st.base + (int32_t)k * st.s[2], nvox, row_shift, st.key);
         return;
     }
-    const int ua = ax == AX_X ? AX_Y : AX_X;
-    const int va = ax == AX_Z ? AX_Y : AX_Z;
-    const int32_t wu = st.rhi[ua] - st.rlo[ua] + 1;
-    const int32_t u = st.rlo[ua] + (int32_t)k % wu;
-    const int32_t v = st.rlo[va] + (int32_t)k / wu;
-    const float mc[3] = {p.m.x, p.m.y, p.m.z};
-    float pc[3];
-    pc[ax] = 0.0f;
-    pc[ua] = vox_corner(u, mc[ua], p.d);
-    pc[va] = vox_corner(v, mc[va], p.d);
-    if (!test_plane(st.o[ax], ax, pc)) return;
-    int32_t c[3];
-    c[ua] = u;
-    c[va] = v;
-    if (st.bb == BB_2D) {
-        c[ax] = st.vmin[ax];
-        lds_write(keys, bx, p.bdim, c[0], c[1], c[2], st.key);
+    // column (u, v) of the item: k = (v - lo_v) * wu + (u - lo_u)
+    int32_t q = (int32_t)((float)k * st.rwu);
+    int32_t r = (int32_t)k - q * st.wu;
+    if (r < 0) { q--; r += st.wu; }
+    if (r >= st.wu) { q++; r -= st.wu; }
+    const int32_t u = st.lo[0] + r, v = st.lo[1] + q;
+    const int ua = axis_u(a), va = axis_v(a);
+    const float mu = pick3(p.m, ua), mv = pick3(p.m, va), ma = pick3(p.m, a);
+    const float pu = vox_corner(u, mu, p.d), pv = vox_corner(v, mv, p.d);
+    bool in = true;
+#pragma unroll
+    for (int e = 0; e < 3; e++) in &= (st.dom[e][0] * pu + st.dom[e][1] * pv) + st.dom[e][2] >= 0.0f;
+    if (!in) return;
+    const int32_t col = st.base + r * st.s[0] + q * st.s[1];
+    if (bb == BB_2D) {
+        key_write(keys, touched, col, nvox, row_shift, st.key);
         return;
     }
+    const F3 pp = {a == AX_X ? 0.0f : pu, a == AX_Y ? 0.0f : (a == AX_X ? pu : pv),
+                   a == AX_Z ? 0.0f : pv};
     int32_t lo, hi;
-    depth_range(st.t0, F3{pc[0], pc[1], pc[2]}, st.n, p.m, p.d, ax, lo, hi);
-    const int pa = ax == AX_X ? 1 : 0;
-    const int pb = ax == AX_Z ? 1 : 2;
-    const int32_t ilo = max(lo, st.rlo[ax]), ihi = min(hi, st.rhi[ax]);
+    depth_range(F3{st.t0[0], st.t0[1], st.t0[2]}, pp, F3{st.n[0], st.n[1], st.n[2]}, p.m, p.d, a,
+                lo, hi);
+    const int32_t ilo = max(lo, st.lo[2]), ihi = min(hi, st.hi[2]);
+    if (ilo > ihi) return;
+    // Constant parts of the two side-plane edge functions along the column.
+    float kc[2][3];
+#pragma unroll
+    for (int q2 = 0; q2 < 2; q2++) {
+        const float po = (flags >> (4 + q2)) & 1u ? pv : pu;
+#pragma unroll
+        for (int e = 0; e < 3; e++) kc[q2][e] = st.pl[q2][e][1] * po;
+    }
+    // matSplitRes interval of the first voxel along a, advanced as i grows.
+    const uint32_t ns = spl.n[a];
+    uint32_t si = 0;
+    while (si + 1 < ns && (uint32_t)ilo >= spl.hi[a][si]) si++;
     for (int32_t i = ilo; i <= ihi; i++) {
-        int32_t slo;
-        const int32_t shi = split_hi_of(spl, ax, i, slo);
-        const int32_t bl = max(st.vmin[ax], slo), bh = min(st.vmax[ax], shi - 1);
+        while (si + 1 < ns && (uint32_t)i >= spl.hi[a][si]) si++;
+        const int32_t slo = (int32_t)spl.lo[a][si], shi = (int32_t)spl.hi[a][si];
+        const int32_t bl = max(st.vmin_a, slo), bh = min(st.vmax_a, shi - 1);
         const int32_t rl = lo < bl ? bl : lo;
         const int32_t rh = hi > bh ? bh : hi;
         bool write = rl == rh;
         if (!write) {
-            pc[ax] = vox_corner(i, mc[ax], p.d);
-            write = test_plane(st.o[pa], pa, pc) && test_plane(st.o[pb], pb, pc);
+            const float pa = vox_corner(i, ma, p.d);
+            bool ok = true;
+#pragma unroll
+            for (int q2 = 0; q2 < 2; q2++)
+#pragma unroll
+                for (int e = 0; e < 3; e++)
+                    ok &= (st.pl[q2][e][0] * pa + kc[q2][e]) + st.pl[q2][e][2] >= 0.0f;
+            write = ok;
         }
-        if (write) {
-            c[ax] = i;
-            lds_write(keys, bx, p.bdim, c[0], c[1], c[2], st.key);
-        }
+        if (write)
+            key_write(keys, touched, col + (i - st.lo[2]) * st.s[2], nvox, row_shift, st.key);
     }
 }
 
-// One workgroup per brick.  The brick's triangles are taken 64 at a time:
-// each is set up once into LDS, then all (triangle, column) work items of the
-// chunk are spread over the 256 threads.  Winners accumulate in LDS
-// (ds_max of bbType<<30 | triangle); the brick then writes its surface bit
-// words, ORs them into the solid grid, and appends the materials of its
-// surface voxels (compacted, voxel order) to the material list.
-__global__ __launch_bounds__(256) void k_surf_brick(
+// Persistent workgroups over a list of non-empty bricks (`order`: brick ids
+// in index order for the plain schedule, or sorted by predicted cost).  Each
+// workgroup takes the next list entry from a global counter (the grab for
+// the following brick is issued while the current one is processed), so
+// bricks of uneven cost balance over the GPU.  Per brick: its triangles are
+// taken 64 at a time, each set up once into LDS, and all (triangle, column)
+// items of the chunk are spread over the 256 threads; winners accumulate in
+// LDS (ds_max of bbType<<30 | triangle).  Only rows of the brick that
+// received a key are visited afterwards: one ballot per 64 voxels gives the
+// surface bit words, a wave scan their 4-byte aligned material segments, one
+// lane per surface voxel fetches its material, and the rows' keys are
+// cleared for the next brick.  Surface bits are ORed into the solid grid.
+// With `cycles`, each brick's duration (s_memtime) is recorded.
+__global__ __launch_bounds__(256, 5) void k_surf_brick(
     SurfParams p, const SplitTable *__restrict__ splits, const uint4 *__restrict__ tri_info,
     const uint32_t *__restrict__ brick_off, const uint32_t *__restrict__ brick_hist,
     const uint32_t *__restrict__ blist, const uint8_t *__restrict__ materials,
     uint32_t *__restrict__ bits, uint32_t *__restrict__ surf, uint32_t *__restrict__ brick_mat_off,
     uint16_t *__restrict__ wrank, uint8_t *__restrict__ mat_list, uint32_t *__restrict__ slots,
-    uint32_t slot_cap, uint32_t *__restrict__ overflow) {
+    uint32_t slot_cap, uint32_t *__restrict__ overflow, const uint32_t *__restrict__ order,
+    const uint32_t *__restrict__ n_order, uint32_t *__restrict__ next,
+    uint64_t *__restrict__ cycles) {
     __shared__ uint32_t keys[kMaxBrickVox];
     __shared__ SurfTri tris[kSurfChunk];
     __shared__ uint32_t colsum[kSurfChunk + 1];
     __shared__ uint32_t wsurf[kMaxBrickVox / 32];
-    __shared__ uint32_t wexcl[kMaxBrickVox / 32];
-    __shared__ uint32_t sbase, nsurf;
+    __shared__ uint32_t wseg[kMaxBrickVox / 32];  // exclusive scan of segment sizes
+    __shared__ uint32_t wtot[4], wpop[4];
+    __shared__ uint8_t touched[32];
+    __shared__ uint32_t sbase, s_q;
     __shared__ SplitTable spl;
-    const uint32_t b = blockIdx.x;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t cnt = brick_hist[b];
-    if (cnt == 0u) {
-        if (tid == 0) brick_mat_off[b] = UINT32_MAX;
-        return;
-    }
-    const uint32_t bxi = b % p.nb[0], byi = (b / p.nb[0]) % p.nb[1], bzi = b / (p.nb[0] * p.nb[1]);
-    const int32_t nvox = p.bdim[0] * p.bdim[1] * p.bdim[2];
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+    const int32_t bd0 = p.bdim[0], bd1 = p.bdim[1], bd2 = p.bdim[2];
+    const int32_t nvox = bd0 * bd1 * bd2;
+    const int32_t nrows = bd1 * bd2;  // <= 32
+    const int32_t wpr = bd0 / 32;     // words per row
+    const int32_t nwords = nvox / 32;
+    const int32_t row_shift = 31 - __clz(bd0);
     for (int32_t i = tid; i < nvox; i += blockDim.x) keys[i] = 0u;
+    if (tid < 32u) touched[tid] = 0;
     {
         const uint32_t *src = (const uint32_t *)splits;
         uint32_t *dst = (uint32_t *)&spl;
         for (uint32_t i = tid; i < sizeof(SplitTable) / 4; i += blockDim.x) dst[i] = src[i];
     }
-    BrickBox box;
-    box.lo[0] = (int32_t)(bxi * p.bdim[0]) - 1;
-    box.lo[1] = (int32_t)(byi * p.bdim[1]) - 1;
-    box.lo[2] = (int32_t)(bzi * p.bdim[2]) - 1;
-    for (int a = 0; a < 3; a++) box.hi[a] = box.lo[a] + p.bdim[a] - 1;
-    const uint32_t off = brick_off[b];
-    for (uint32_t c0 = 0; c0 < cnt; c0 += kSurfChunk) {
-        __syncthreads();
-        if (tid < (uint32_t)kSurfChunk) {
-            uint32_t items = 0u;
-            if (c0 + tid < cnt) {
-                const uint32_t t = blist[off + c0 + tid];
-                items = surf_tri_setup(p, t, tri_info[t].x, box, tris[tid]);
-            }
-            // wave-level inclusive scan of the item counts
-            uint32_t incl = items;
-            for (int d = 1; d < kSurfChunk; d <<= 1) {
-                uint32_t o = (uint32_t)__shfl_up((int)incl, d);
-                if ((int)tid >= d) incl += o;
-            }
-            colsum[tid + 1] = incl;
-            if (tid == 0) colsum[0] = 0u;
-        }
-        __syncthreads();
-        const uint32_t total = colsum[kSurfChunk];
-        for (uint32_t c = tid; c < total; c += blockDim.x) {
-            uint32_t lo = 0, hi = kSurfChunk;  // find i with colsum[i] <= c < colsum[i+1]
-            while (lo + 1 < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (colsum[mid] <= c) lo = mid;
-                else hi = mid;
-            }
-            surf_item(p, spl, tris[lo], c - colsum[lo], box, keys);
-        }
-    }
+    const uint32_t n_list = *n_order;
+    if (tid == 0) s_q = atomicAdd(next, 1u);
     __syncthreads();
-    // Surface bit words: one ballot per 64 voxels (MSB-first words).
-    const int32_t nwords = nvox / 32;
-    for (int32_t v0 = (int32_t)(tid & ~63u); v0 < nvox; v0 += (int32_t)blockDim.x) {
-        const unsigned long long m = __ballot(keys[v0 + (int32_t)lane_id()] != 0u);
-        if (lane_id() == 0) {
-            wsurf[v0 / 32] = __brev((uint32_t)m);
-            wsurf[v0 / 32 + 1] = __brev((uint32_t)(m >> 32));
-        }
-    }
-    if (tid == 0) nsurf = 0u;
-    __syncthreads();
-    // Exclusive scan of the words' material segment sizes: each word's
-    // materials start 4-byte aligned (popcount rounded up to 4) so the node
-    // pass fetches them as whole dwords.
-    const uint32_t pop = tid < (uint32_t)nwords ? (uint32_t)__popc(wsurf[tid]) : 0u;
-    const uint32_t pc = (pop + 3u) & ~3u;
-    if (tid < (uint32_t)nwords) wexcl[tid] = pc;
-    if (pop) atomicAdd(&nsurf, pop);
-    __syncthreads();
-    for (uint32_t d = 1; d < (uint32_t)nwords; d <<= 1) {
-        uint32_t o = (tid < (uint32_t)nwords && tid >= d) ? wexcl[tid - d] : 0u;
-        __syncthreads();
-        if (tid < (uint32_t)nwords) wexcl[tid] += o;
-        __syncthreads();
-    }
-    if (tid == (uint32_t)nwords - 1) {
-        // Material space comes from one of kMatSlots arenas (brick b uses
-        // arena b % kMatSlots), so concurrent bricks bump different counters.
-        const uint32_t acc = wexcl[tid];
-        const uint32_t sl = b % kMatSlots;
-        const uint32_t local = atomicAdd(&slots[sl * kSlotStride], acc);
-        atomicAdd(&slots[sl * kSlotStride + 1], nsurf);
-        uint32_t base = sl * slot_cap + local;
-        if (local + acc > slot_cap) {
-            atomicOr(overflow, 1u);  // the caller retries with larger arenas
-            base = UINT32_MAX;
-        }
-        sbase = base;
-        brick_mat_off[b] = base;
-    }
-    __syncthreads();
-    const uint32_t base = sbase;
-    const int32_t wpr = p.bdim[0] / 32;  // words per brick row
-    for (int32_t w = tid; w < nwords; w += blockDim.x) {
-        const int32_t wx = w % wpr, ly = (w / wpr) % p.bdim[1], lz = w / (wpr * p.bdim[1]);
-        const uint32_t aw = bxi * (uint32_t)wpr + (uint32_t)wx;
-        const uint32_t ay = byi * (uint32_t)p.bdim[1] + (uint32_t)ly;
-        const uint32_t az = bzi * (uint32_t)p.bdim[2] + (uint32_t)lz;
-        if (aw >= p.alloc[0] / 32u || ay >= p.alloc[1] || az >= p.alloc[2]) continue;
-        const size_t gi = ((size_t)az * p.alloc[1] + ay) * (p.alloc[0] / 32u) + aw;
-        const uint32_t s = wsurf[w];
-        const uint32_t seg = ((uint32_t)__popc(s) + 3u) & ~3u;
-        surf[gi] = s;
-        wrank[(size_t)b * nwords + w] = (uint16_t)(wexcl[w] - seg);
-        if (s) bits[gi] = bits[gi] | s;
-        if (base != UINT32_MAX && s) {
-            uint32_t r = base + wexcl[w] - seg;
-            uint32_t m = s;
-            while (m) {
-                const int k = __clz(m);
-                const uint32_t key = keys[w * 32 + k];
-                mat_list[r++] = materials ? materials[key & 0x3FFFFFFFu] : (uint8_t)0;
-                m &= ~(0x80000000u >> k);
+    uint32_t q = s_q;
+    while (q < n_list) {  // q is uniform over the block
+        uint32_t q_next = 0u;
+        if (tid == 0) q_next = atomicAdd(next, 1u);  // grab the next brick early
+        const uint32_t b = order[q];
+        const uint32_t cnt = brick_hist[b];
+        uint64_t t_start = 0;
+        if (cycles && tid == 0) t_start = __builtin_amdgcn_s_memtime();
+        const uint32_t bxi = b % p.nb[0], byi = (b / p.nb[0]) % p.nb[1],
+                       bzi = b / (p.nb[0] * p.nb[1]);
+        // Bricks are laid over alloc coordinates; ext = alloc - 1.
+        const int32_t blx = (int32_t)(bxi * bd0) - 1, bly = (int32_t)(byi * bd1) - 1,
+                      blz = (int32_t)(bzi * bd2) - 1;
+        // This thread's brick word (for the OR into the solid grid), early.
+        size_t gi = 0;
+        bool gword = false;
+        uint32_t bw = 0u;
+        if ((int32_t)tid < nwords) {
+            const int32_t wx = (int32_t)tid % wpr, ly = ((int32_t)tid / wpr) % bd1,
+                          lz = (int32_t)tid / (wpr * bd1);
+            const uint32_t aw = bxi * (uint32_t)wpr + (uint32_t)wx;
+            const uint32_t ay = byi * (uint32_t)bd1 + (uint32_t)ly;
+            const uint32_t az = bzi * (uint32_t)bd2 + (uint32_t)lz;
+            gword = aw < p.alloc[0] / 32u && ay < p.alloc[1] && az < p.alloc[2];
+            if (gword) {
+                gi = ((size_t)az * p.alloc[1] + ay) * (p.alloc[0] / 32u) + aw;
+                bw = bits[gi];
             }
         }
+        const uint32_t off = brick_off[b];
+        for (uint32_t c0 = 0; c0 < cnt; c0 += kSurfChunk) {
+            __syncthreads();
+            if (tid < (uint32_t)kSurfChunk) {
+                uint32_t items = 0u;
+                if (c0 + tid < cnt) {
+                    const uint32_t t = blist[off + c0 + tid];
+                    items = surf_tri_setup(p, t, tri_info[t].x, blx, bly, blz, tris[tid]);
+                }
+                uint32_t incl = items;  // wave-level inclusive scan
+                for (int d = 1; d < kSurfChunk; d <<= 1) {
+                    const uint32_t o = (uint32_t)__shfl_up((int)incl, d);
+                    if ((int)tid >= d) incl += o;
+                }
+                colsum[tid + 1] = incl;
+                if (tid == 0) colsum[0] = 0u;
+            }
+            __syncthreads();
+            const uint32_t total = colsum[kSurfChunk];
+            for (uint32_t c = tid; c < total; c += blockDim.x) {
+                uint32_t lo = 0, hi = kSurfChunk;  // colsum[lo] <= c < colsum[lo + 1]
+                while (lo + 1 < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (colsum[mid] <= c) lo = mid;
+                    else hi = mid;
+                }
+                surf_item(p, spl, tris[lo], c - colsum[lo], keys, touched, nvox, row_shift);
+            }
+        }
+        __syncthreads();
+        // Surface bit words of the touched rows (MSB-first), zero elsewhere.
+        for (int32_t r = (int32_t)wave; r < nrows; r += 4) {
+            if (!touched[r]) {
+                if ((int32_t)lane < wpr) wsurf[r * wpr + lane] = 0u;
+                continue;
+            }
+            for (int32_t h = 0; h < bd0 / 64; h++) {
+                const unsigned long long m = __ballot(keys[r * bd0 + h * 64 + lane] != 0u);
+                if (lane == 0) {
+                    wsurf[r * wpr + 2 * h] = __brev((uint32_t)m);
+                    wsurf[r * wpr + 2 * h + 1] = __brev((uint32_t)(m >> 32));
+                }
+            }
+        }
+        __syncthreads();
+        // Exclusive scan of the words' segment sizes (popcount rounded up to
+        // 4): wave scans, then the wave totals.
+        uint32_t pop = 0u, seg = 0u, incl = 0u;
+        if ((int32_t)tid < nwords) {
+            pop = (uint32_t)__popc(wsurf[tid]);
+            seg = (pop + 3u) & ~3u;
+        }
+        incl = seg;
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t o = (uint32_t)__shfl_up((int)incl, d);
+            if ((int)lane >= d) incl += o;
+        }
+        uint32_t npop = pop;
+        for (int d = kWave / 2; d >= 1; d >>= 1) npop += (uint32_t)__shfl_xor((int)npop, d);
+        if (lane == kWave - 1) wtot[wave] = incl;
+        if (lane == 0) wpop[wave] = npop;
+        __syncthreads();
+        uint32_t wbase = 0u;
+        for (uint32_t k = 0; k < wave; k++) wbase += wtot[k];
+        if ((int32_t)tid < nwords) wseg[tid] = wbase + incl - seg;
+        if (tid == 0) {
+            // Material space from one of kMatSlots arenas (brick b uses arena
+            // b % kMatSlots), so concurrent bricks bump different counters.
+            const uint32_t acc = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+            const uint32_t sl = b % kMatSlots;
+            const uint32_t local = atomicAdd(&slots[sl * kSlotStride], acc);
+            atomicAdd(&slots[sl * kSlotStride + 1], wpop[0] + wpop[1] + wpop[2] + wpop[3]);
+            uint32_t base = sl * slot_cap + local;
+            if (local + acc > slot_cap) {
+                atomicOr(overflow, 1u);  // the caller retries with larger arenas
+                base = UINT32_MAX;
+            }
+            sbase = base;
+            brick_mat_off[b] = base;
+        }
+        __syncthreads();
+        const uint32_t base = sbase;
+        if ((int32_t)tid < nwords && gword) {
+            const uint32_t sw = wsurf[tid];
+            surf[gi] = sw;
+            wrank[(size_t)b * nwords + tid] = (uint16_t)wseg[tid];
+            if (sw) bits[gi] = bw | sw;
+        }
+        // Materials: one lane per voxel of a touched row; keys cleared.
+        for (int32_t r = (int32_t)wave; r < nrows; r += 4) {
+            if (!touched[r]) continue;
+            for (int32_t h = 0; h < bd0 / 64; h++) {
+                const int32_t vi = r * bd0 + h * 64 + (int32_t)lane;
+                const uint32_t key = keys[vi];
+                if (key) {
+                    keys[vi] = 0u;
+                    if (base != UINT32_MAX) {
+                        const int32_t w = vi >> 5, qx = vi & 31;
+                        const uint32_t before = wsurf[w] & ~(0xFFFFFFFFu >> qx);
+                        mat_list[base + wseg[w] + (uint32_t)__popc(before)] =
+                            materials ? materials[key & 0x3FFFFFFFu] : (uint8_t)0;
+                    }
+                }
+            }
+        }
+        if (cycles && tid == 0) cycles[b] = __builtin_amdgcn_s_memtime() - t_start;
+        __syncthreads();
+        if (tid < 32u) touched[tid] = 0;
+        if (tid == 0) s_q = q_next;
+        __syncthreads();
+        q = s_q;
     }
+}
+
+// Non-empty bricks in index order: flags, then (after an exclusive scan of
+// the flags) the compacted list.
+__global__ __launch_bounds__(256) void k_nonzero_flags(const uint32_t *__restrict__ hist,
+                                                       uint32_t *__restrict__ flags, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        flags[i] = hist[i] != 0u ? 1u : 0u;
+}
+__global__ __launch_bounds__(256) void k_nonzero_list(const uint32_t *__restrict__ hist,
+                                                      const uint32_t *__restrict__ off,
+                                                      uint32_t *__restrict__ list, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (hist[i] != 0u) list[off[i]] = i;
 }
 
 // ---------------------------------------------------------------------------
