@@ -184,6 +184,10 @@ typedef enum vox_schedule { VOX_SCHEDULE_PLAIN = 0, VOX_SCHEDULE_LEARNED = 1 } v
 #define VOX_COST_FEATURES 16
 #define VOX_COST_PARAMS 5313 /* W1[64][16], b1[64], W2[64][64], b2[64], W3[64], b3 */
 vox_status vox_set_schedule(vox_ctx *ctx, vox_schedule schedule);
+/* LEARNED order: bricks whose predicted cost (standardized log cycles)
+ * exceeds `threshold` run first, most expensive first; the rest keep index
+ * order (neighbouring bricks share triangles in cache).  Default 1.5. */
+vox_status vox_set_heavy_threshold(vox_ctx *ctx, float threshold);
 /* Model parameters (host arrays of VOX_COST_PARAMS floats). */
 vox_status vox_cost_set_params(vox_ctx *ctx, const float *params);
 vox_status vox_cost_get_params(vox_ctx *ctx, float *params);
@@ -211,6 +215,33 @@ vox_status vox_mlp_forward(vox_ctx *ctx, const float *params_dev, const float *x
                            uint32_t n, float *y_dev);
 vox_status vox_mlp_grad(vox_ctx *ctx, const float *params_dev, const float *x_dev,
                         const float *t_dev, uint32_t n, float *grad_dev, float *loss);
+
+/* ---- batches of meshes (no counterpart in the reference) ---------------
+ * Many meshes voxelized in one call, each into its own node array.  Inputs
+ * are device arrays; the grid of each item is planned by the caller
+ * (vox_plan_grid / vox_plan_grid_host).  PLAIN runs the items one after
+ * another on the context's stream, as separate vox_voxelize_nodes calls
+ * would.  LEARNED runs them on `streams` worker streams at once (one host
+ * thread and one internal context each), taking the items in decreasing
+ * order of estimated work (node bytes + 64 B per triangle: longest first),
+ * and with the learned brick order inside each item when a cost model is
+ * set.  Outputs are identical under both. */
+typedef struct vox_batch_item {
+    const float *vertices_dev;
+    uint32_t nv;
+    uint32_t nt;
+    const uint32_t *indices_dev;
+    const uint8_t *materials_dev; /* NULL: no materials */
+    uint32_t n_unique;
+    vox_node_kind kind;
+    vox_grid grid;                /* planned grid of this mesh */
+    uint32_t vox_split[3];        /* voxSplitRes */
+    uint32_t mat_split[3];        /* matSplitRes */
+    void *nodes_dev;              /* vox_node_count(&grid, kind) nodes */
+    vox_status status;            /* out */
+} vox_batch_item;
+vox_status vox_voxelize_batch(vox_ctx *ctx, vox_batch_item *items, uint32_t n,
+                              vox_schedule schedule, uint32_t streams);
 
 #ifdef __cplusplus
 }

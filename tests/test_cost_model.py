@@ -155,3 +155,51 @@ def test_learned_schedule_output_identical(gpu_ctx, oracle, kind):
     assert np.array_equal(plain, learned)
     og = oracle.plan_grid(m.vertices, va.PLAN_CUBE_LENGTH, 0.015)
     assert np.array_equal(learned, oracle.voxelize_nodes(m, og, kind, True))
+
+
+def make_batch(ctx, meshes, kinds, keep):
+    """Device copies of the meshes, host-planned grids, output buffers."""
+    items = (va.BatchItem * len(meshes))()
+    outs = []
+    for i, (m, kind) in enumerate(zip(meshes, kinds)):
+        g = va.plan_grid_host(m.vertices, va.PLAN_CUBE_LENGTH, m.meta["cube_length"], 1024)
+        bufs = [va.DeviceBuffer(ctx, a.nbytes) for a in (m.vertices, m.indices, m.materials)]
+        for b, a in zip(bufs, (m.vertices, m.indices, m.materials)):
+            b.upload(np.ascontiguousarray(a))
+        nb = ctx.node_count(g, kind) * va.NODE_SIZE[kind]
+        out = va.DeviceBuffer(ctx, nb)
+        keep.extend(bufs + [out])
+        it = items[i]
+        it.vertices_dev, it.nv = bufs[0].ptr.value, m.nv
+        it.indices_dev, it.nt = bufs[1].ptr.value, m.nt
+        it.materials_dev, it.n_unique = bufs[2].ptr.value, int(m.materials.max()) + 1
+        it.kind = kind
+        it.grid = g
+        it.vox_split[:] = (1024, 1024, 1024)
+        it.mat_split[:] = (512, 512, 512)
+        it.nodes_dev = out.ptr.value
+        outs.append((out, nb, g))
+    return items, outs
+
+
+@pytest.mark.gpu
+def test_batch_schedules_identical_and_match_oracle(gpu_ctx, oracle):
+    meshes = W.batch_meshes(n=12, seed=7, tri_lo=200, tri_hi=20000)
+    for m in meshes:
+        m.meta["cube_length"] *= 2.0  # keep the oracle quick
+    kinds = [va.NODE_SHORT if int(m.materials.max()) <= 7 else va.NODE_LONG for m in meshes]
+    keep = []
+    items, outs = make_batch(gpu_ctx, meshes, kinds, keep)
+    gpu_ctx.voxelize_batch(items, va.SCHEDULE_PLAIN, 1)
+    plain = [o.download(np.uint8, nb) for o, nb, _ in outs]
+    for o, nb, _ in outs:
+        o.upload(np.zeros(nb, np.uint8))
+    gpu_ctx.voxelize_batch(items, va.SCHEDULE_LEARNED, 4)
+    for i, ((o, nb, g), m, kind) in enumerate(zip(outs, meshes, kinds)):
+        got = o.download(np.uint8, nb)
+        assert items[i].status == 0
+        assert np.array_equal(got, plain[i]), i
+        ref = oracle.voxelize_nodes(m, oracle.plan_grid(m.vertices, va.PLAN_CUBE_LENGTH,
+                                                        m.meta["cube_length"], 1024), kind, True,
+                                    vox_split=(1024, 1024, 1024))
+        assert np.array_equal(got, ref), i

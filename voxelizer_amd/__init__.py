@@ -52,6 +52,15 @@ class Grid(C.Structure):
         return self.as_dict() == other.as_dict()
 
 
+class BatchItem(C.Structure):
+    """vox_batch_item of include/voxelizer_amd.h."""
+    _fields_ = [("vertices_dev", C.c_void_p), ("nv", C.c_uint32), ("nt", C.c_uint32),
+                ("indices_dev", C.c_void_p), ("materials_dev", C.c_void_p),
+                ("n_unique", C.c_uint32), ("kind", C.c_int), ("grid", Grid),
+                ("vox_split", C.c_uint32 * 3), ("mat_split", C.c_uint32 * 3),
+                ("nodes_dev", C.c_void_p), ("status", C.c_int)]
+
+
 class StageTimes(C.Structure):
     _fields_ = [("plan_ms", C.c_float), ("solid_bin_ms", C.c_float), ("solid_fill_ms", C.c_float),
                 ("surface_ms", C.c_float), ("erode_ms", C.c_float), ("nodes_ms", C.c_float),
@@ -70,8 +79,9 @@ EXPORTED = [
     "vox_set_mesh", "vox_set_mesh_device", "vox_set_materials", "vox_set_materials_device",
     "vox_plan_grid", "vox_plan_grid_host", "vox_solid_bits", "vox_voxelize_nodes", "vox_node_dims",
     "vox_node_count", "vox_node_size", "vox_enable_timing", "vox_get_stage_times",
-    "vox_set_schedule", "vox_cost_set_params", "vox_cost_get_params", "vox_cost_record",
+    "vox_set_schedule", "vox_set_heavy_threshold", "vox_cost_set_params", "vox_cost_get_params", "vox_cost_record",
     "vox_cost_samples", "vox_cost_train", "vox_mlp_forward", "vox_mlp_grad",
+    "vox_voxelize_batch",
 ]
 
 _lib = None
@@ -118,6 +128,7 @@ def lib():
     L.vox_get_stage_times.argtypes = [P, C.POINTER(StageTimes)]
     F = C.c_float
     L.vox_set_schedule.argtypes = [P, I32]
+    L.vox_set_heavy_threshold.argtypes = [P, F]
     L.vox_cost_set_params.argtypes = [P, P]
     L.vox_cost_get_params.argtypes = [P, P]
     L.vox_cost_record.argtypes = [P, I32]
@@ -125,6 +136,7 @@ def lib():
     L.vox_cost_train.argtypes = [P, P, P, U32, U32, U32, F, U32, C.POINTER(F)]
     L.vox_mlp_forward.argtypes = [P, P, P, U32, P]
     L.vox_mlp_grad.argtypes = [P, P, P, P, U32, P, C.POINTER(F)]
+    L.vox_voxelize_batch.argtypes = [P, C.POINTER(BatchItem), U32, I32, U32]
     _lib = L
     return L
 
@@ -292,6 +304,13 @@ class Context:
     def set_schedule(self, schedule: int):
         self._check(lib().vox_set_schedule(self.h, schedule))
 
+    def voxelize_batch(self, items, schedule: int = 0, streams: int = 1):
+        """items: a ctypes array of BatchItem (device pointers filled in)."""
+        self._check(lib().vox_voxelize_batch(self.h, items, len(items), schedule, streams))
+
+    def set_heavy_threshold(self, threshold: float):
+        self._check(lib().vox_set_heavy_threshold(self.h, float(threshold)))
+
     def cost_record(self, on: bool = True):
         self._check(lib().vox_cost_record(self.h, 1 if on else 0))
 
@@ -349,7 +368,7 @@ class Context:
         return bg.download(np.float32, COST_PARAMS), float(loss.value)
 
 
-__all__ = ["Context", "Grid", "DeviceBuffer", "VoxError", "lib", "device_count",
+__all__ = ["Context", "Grid", "BatchItem", "DeviceBuffer", "VoxError", "lib", "device_count",
            "plan_grid_host",
            "NODE_SHORT", "NODE_LONG", "NODE_PARTIAL", "NODE_SHORT_FCC", "NODE_LONG_FCC",
            "PLAN_MAX_DIMENSION", "PLAN_CUBE_LENGTH", "NODE_SIZE", "EXPORTED",

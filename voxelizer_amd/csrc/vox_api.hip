@@ -7,11 +7,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/voxelizer_amd.h"
@@ -82,8 +84,11 @@ struct vox_ctx {
     int schedule = VOX_SCHEDULE_PLAIN;
     float *cost_params = nullptr;  // device, kCostParams floats
     bool cost_ready = false;
+    float heavy_cost = 1.5f;  // standardized log cost above which a brick goes first
     bool record = false;
     std::vector<float> rec_feats, rec_cycles;
+    // worker contexts of batched runs (same device, own streams and buffers)
+    std::vector<vox_ctx *> workers;
 };
 
 #define VOX_HIP(expr)                                                               \
@@ -420,13 +425,19 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
         k_mlp_forward_n<<<(nbt + kMlpBlockRows - 1) / kMlpBlockRows, 256, 0, ctx->stream>>>(
             ctx->cost_params, feats, n_list, pred);
         VOX_HIP(hipGetLastError());
+        uint32_t *light, *loff;
+        VOX_TRY(buf(ctx, "f_light", nbt, &light));
+        VOX_TRY(buf(ctx, "f_loff", nbt + 1, &loff));
         VOX_TRY(fill(ctx, chist, kCostBuckets, 0u));
         VOX_TRY(fill(ctx, ccur, kCostBuckets, 0u));
-        k_cost_hist<<<grid_for(nbt, 256, 1024), 256, 0, ctx->stream>>>(pred, n_list, chist);
+        VOX_TRY(fill(ctx, light, nbt, 0u));
+        k_cost_hist<<<grid_for(nbt, 256, 1024), 256, 0, ctx->stream>>>(pred, n_list,
+                                                                     ctx->heavy_cost, light, chist);
         VOX_HIP(hipGetLastError());
         VOX_TRY(scan(ctx, chist, coff, kCostBuckets));
-        k_cost_scatter<<<grid_for(nbt, 256, 1024), 256, 0, ctx->stream>>>(pred, blist_order,
-                                                                        n_list, coff, ccur, sorted);
+        VOX_TRY(scan(ctx, light, loff, nbt));
+        k_cost_scatter<<<grid_for(nbt, 256, 1024), 256, 0, ctx->stream>>>(
+            pred, blist_order, n_list, ctx->heavy_cost, coff, loff, ccur, sorted);
         VOX_HIP(hipGetLastError());
         brick_list = sorted;
     }
@@ -571,6 +582,8 @@ void vox_destroy(vox_ctx *ctx) {
     if (ctx->own_mats) (void)hipFree((void *)ctx->mats);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (vox_ctx *w : ctx->workers) vox_destroy(w);
+    ctx->workers.clear();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->cost_params) (void)hipFree(ctx->cost_params);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -987,12 +1000,94 @@ vox_status vox_get_stage_times(const vox_ctx *ctx, vox_stage_times *out) {
     return VOX_OK;
 }
 
+// ---- batches ---------------------------------------------------------------
+
+namespace {
+vox_status run_item(vox_ctx *c, vox_batch_item &it) {
+    VOX_TRY(vox_set_mesh_device(c, it.vertices_dev, it.nv, it.indices_dev, it.nt));
+    VOX_TRY(vox_set_materials_device(c, it.materials_dev, it.n_unique));
+    VOX_TRY(vox_voxelize_nodes(c, &it.grid, it.kind, it.materials_dev ? 1 : 0, it.vox_split,
+                               it.mat_split, it.nodes_dev));
+    return vox_sync(c);
+}
+}  // namespace
+
+vox_status vox_voxelize_batch(vox_ctx *ctx, vox_batch_item *items, uint32_t n,
+                              vox_schedule schedule, uint32_t streams) {
+    if (!ctx || (n && !items)) return VOX_ERR_INVALID;
+    if (schedule != VOX_SCHEDULE_PLAIN && schedule != VOX_SCHEDULE_LEARNED) return VOX_ERR_INVALID;
+    vox_status first = VOX_OK;
+    if (schedule == VOX_SCHEDULE_PLAIN || streams <= 1) {
+        const int saved = ctx->schedule;
+        ctx->schedule = schedule;
+        for (uint32_t i = 0; i < n; i++) {
+            items[i].status = run_item(ctx, items[i]);
+            if (items[i].status != VOX_OK && first == VOX_OK) first = items[i].status;
+        }
+        ctx->schedule = saved;
+        return first;
+    }
+    streams = std::min<uint32_t>(streams, 64);
+    while (ctx->workers.size() < streams) {
+        vox_ctx *w = nullptr;
+        const vox_status st = vox_create(ctx->dev, &w);
+        if (st != VOX_OK) return fail(ctx, st, "cannot create a worker context");
+        ctx->workers.push_back(w);
+    }
+    std::vector<float> params;
+    if (ctx->cost_ready) {
+        params.resize(kCostParams);
+        VOX_TRY(vox_cost_get_params(ctx, params.data()));
+    }
+    for (uint32_t k = 0; k < streams; k++) {
+        vox_ctx *w = ctx->workers[k];
+        w->schedule = VOX_SCHEDULE_LEARNED;
+        w->heavy_cost = ctx->heavy_cost;
+        w->timing = false;
+        if (!params.empty()) VOX_TRY(vox_cost_set_params(w, params.data()));
+    }
+    // Longest estimated work first.
+    std::vector<uint32_t> order(n);
+    std::vector<double> est(n);
+    for (uint32_t i = 0; i < n; i++) {
+        order[i] = i;
+        const size_t nodes = vox_node_count(&items[i].grid, items[i].kind);
+        est[i] = (double)nodes * (double)vox_node_size(items[i].kind) + 64.0 * items[i].nt;
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return est[a] > est[b]; });
+    std::atomic<uint32_t> next{0};
+    std::vector<std::thread> threads;
+    for (uint32_t k = 0; k < streams; k++)
+        threads.emplace_back([&, k]() {
+            vox_ctx *w = ctx->workers[k];
+            for (uint32_t q = next.fetch_add(1); q < n; q = next.fetch_add(1)) {
+                vox_batch_item &it = items[order[q]];
+                it.status = run_item(w, it);
+            }
+        });
+    for (auto &t : threads) t.join();
+    for (uint32_t i = 0; i < n; i++)
+        if (items[i].status != VOX_OK) {
+            first = items[i].status;
+            ctx->err = "batch item " + std::to_string(i) + " failed";
+            break;
+        }
+    return first;
+}
+
 // ---- brick schedule and cost model ----------------------------------------
 
 vox_status vox_set_schedule(vox_ctx *ctx, vox_schedule schedule) {
     if (!ctx || (schedule != VOX_SCHEDULE_PLAIN && schedule != VOX_SCHEDULE_LEARNED))
         return VOX_ERR_INVALID;
     ctx->schedule = schedule;
+    return VOX_OK;
+}
+
+vox_status vox_set_heavy_threshold(vox_ctx *ctx, float threshold) {
+    if (!ctx || threshold != threshold) return VOX_ERR_INVALID;
+    ctx->heavy_cost = threshold;
     return VOX_OK;
 }
 

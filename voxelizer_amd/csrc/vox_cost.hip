@@ -309,9 +309,11 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ P, const float
 }
 
 // ---------------------------------------------------------------------------
-// Bricks by predicted cost, most expensive first: bucket = (4 - y) * 128
-// clamped to [0, 1023] (y is the standardized log cost), then a counting
-// sort (histogram, scan, scatter).
+// Learned brick order.  Bricks predicted heavy (standardized log cost above
+// `heavy`) go first, most expensive first (1024-bucket counting sort on
+// bucket = (4 - y) * 128); the rest follow in index order, which keeps
+// neighbouring bricks -- which share triangles and solid rows in L2 --
+// together.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t cost_bucket(float y) {
     const float f = (4.0f - y) * 128.0f;
@@ -319,30 +321,46 @@ __device__ __forceinline__ uint32_t cost_bucket(float y) {
     return f >= 1023.0f ? 1023u : (uint32_t)f;
 }
 
+// light[q] = 1 for bricks that keep index order; heavy bricks counted per bucket.
 __global__ __launch_bounds__(256) void k_cost_hist(const float *__restrict__ Y,
                                                    const uint32_t *__restrict__ n_order,
+                                                   float heavy, uint32_t *__restrict__ light,
                                                    uint32_t *__restrict__ hist) {
     __shared__ uint32_t h[kCostBuckets];
     for (uint32_t i = threadIdx.x; i < kCostBuckets; i += blockDim.x) h[i] = 0u;
     __syncthreads();
     const uint32_t n = *n_order;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        atomicAdd(&h[cost_bucket(Y[i])], 1u);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float y = Y[i];
+        const bool hv = y > heavy;
+        light[i] = hv ? 0u : 1u;
+        if (hv) atomicAdd(&h[cost_bucket(y)], 1u);
+    }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < kCostBuckets; i += blockDim.x)
         if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
+// off: exclusive scan of hist (off[kCostBuckets] = heavy count); loff:
+// exclusive scan of light.
 __global__ __launch_bounds__(256) void k_cost_scatter(const float *__restrict__ Y,
                                                       const uint32_t *__restrict__ in,
                                                       const uint32_t *__restrict__ n_order,
+                                                      float heavy,
                                                       const uint32_t *__restrict__ off,
+                                                      const uint32_t *__restrict__ loff,
                                                       uint32_t *__restrict__ cursor,
                                                       uint32_t *__restrict__ out) {
     const uint32_t n = *n_order;
+    const uint32_t nheavy = off[kCostBuckets];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t k = cost_bucket(Y[i]);
-        out[off[k] + atomicAdd(&cursor[k], 1u)] = in[i];
+        const float y = Y[i];
+        if (y > heavy) {
+            const uint32_t k = cost_bucket(y);
+            out[off[k] + atomicAdd(&cursor[k], 1u)] = in[i];
+        } else {
+            out[nheavy + loff[i]] = in[i];
+        }
     }
 }
 

@@ -272,10 +272,10 @@ __global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float
                        float *__restrict__ V, uint32_t n, float lr, float b1, float b2, float eps,
                        float c1, float c2);
 __global__ void k_cost_hist(const float *__restrict__ Y, const uint32_t *__restrict__ n_order,
-                            uint32_t *__restrict__ hist);
+                            float heavy, uint32_t *__restrict__ light, uint32_t *__restrict__ hist);
 __global__ void k_cost_scatter(const float *__restrict__ Y, const uint32_t *__restrict__ in,
-                               const uint32_t *__restrict__ n_order,
-                               const uint32_t *__restrict__ off, uint32_t *__restrict__ cursor,
-                               uint32_t *__restrict__ out);
+                               const uint32_t *__restrict__ n_order, float heavy,
+                               const uint32_t *__restrict__ off, const uint32_t *__restrict__ loff,
+                               uint32_t *__restrict__ cursor, uint32_t *__restrict__ out);
 
 }  // namespace vk
