@@ -1,22 +1,38 @@
 #!/usr/bin/env python
 """bench.py -- throughput of the voxelizer node pipeline on one MI355X.
 
-One step = one pass of the hot path over one synthetic mesh already resident
-in HBM: grid planning (bounding box + dimensions), solid voxelization,
-surface voxelization with material winners, erosion to the fixed point and
-boundary-id node write, ending with the node grid resident in HBM
+One step = one pass of the hot path over synthetic meshes already resident in
+HBM: grid planning (bounding box + dimensions), solid voxelization, surface
+voxelization with material winners, erosion to the fixed point and boundary-id
+node write, ending with the node grids resident in HBM
 (Voxelizer<Node>::voxelizeToNodes with device-pointer output).
 
-Default workload: the 1.2 G-voxel large hall (1536x1024x768 ShortNodes,
-~1.4M triangles, 8 material ids), voxelizeToNodes(1/64 m,
-voxSplitRes=(1024,1024,768), matSplitRes=(512,512,512)).
+Workloads:
+  hall (default) the 1.2 G-voxel large hall: 1536x1024x768 ShortNodes,
+                 ~1.4M triangles, 8 material ids, voxelizeToNodes(1/64 m,
+                 voxSplitRes=(1024,1024,768), matSplitRes=(512,512,512));
+  batch          256 meshes (icospheres, rooms, boxes; 500..200k triangles,
+                 64..512 voxels along the longest side), one batch call per step;
+  room, fcc, icosphere  the shoebox room (LongNode), the FCC room (LongFCCNode)
+                 and the 128^3 icosphere.
 
-Prints ONE JSON line.  `value` is Gvoxels/s = allocated nodes / wall time per
-step, timed over --steps steps between two device synchronisations.
+Schedules: "plain" processes surface bricks in index order (and, for a batch,
+the meshes one after another on one stream); "learned" orders the bricks by
+the MLP cost model, trained first by the library's own GPU kernels on
+per-brick durations recorded on a differently seeded mesh (for a batch:
+meshes on 16 worker streams, longest first).  Both are timed; `value` is the
+schedule named by --schedule (default learned), the other is reported beside
+it.
+
+Prints ONE JSON line.  `value` = allocated nodes per step / wall time per step
+(Gvoxels/s), timed over --steps steps between two device synchronisations.
 `roofline` is for the node-write kernel (k_nodes_cart / k_nodes_fcc): its
-algorithmic bytes (one node write + one bit read per node) over its average
-HIP-event duration on the library's stream.  `cpu_baseline` times the CPU
-oracle (the restatement of the reference algorithm) on the host cores.
+algorithmic bytes (one node write + one solid-bit read per node) over its
+average HIP-event duration on the library's stream during the timed steps;
+`traffic` is the HBM bytes per launch that rocprofv3 counted for it
+(profiles/traffic_*.json: FETCH_SIZE and WRITE_SIZE scaled by a plain copy).
+`cpu_baseline` times the CPU oracle (the restatement of the reference
+algorithm) on the host cores on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -32,6 +48,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+METRIC = "Gvoxels/sec solid+classify, learned vs plain schedule, bit-exact; % HBM roofline"
 
 WORKLOADS = {
     # name: (mesh builder, plan mode, plan param, node kind, vox split, mat split, label)
@@ -43,37 +60,44 @@ WORKLOADS = {
             "shoebox room on the FCC lattice, 640x255x384 LongFCCNode, ids 1..8"),
     "icosphere": ("icosphere", 0, 126, 0, (1024, 512, 512), (512, 512, 512),
                   "icosphere 1280 triangles, maxDimension 126 -> 128^3 ShortNode, 1 material"),
+    "batch": ("batch", 1, None, None, (1024, 1024, 1024), (512, 512, 512),
+              "256 meshes (icospheres, rooms, boxes), 500..200k triangles, 64..512 voxels "
+              "on the longest side, ShortNode (ids <= 7) or LongNode"),
 }
+BATCH_STREAMS = 16
 
 
-def build_mesh(name):
+def build_mesh(name, seed=None):
     from voxelizer_amd import workloads as W
     if name == "hall":
-        return W.hall()
+        return W.hall() if seed is None else W.hall(seed=seed)
     if name == "room":
-        return W.shoebox_room()
-    return W.icosphere()
+        return W.shoebox_room() if seed is None else W.shoebox_room(seed=seed)
+    return W.icosphere() if seed is None else W.icosphere(seed=seed)
 
 
 def node_bytes(kind):
     return {0: 1, 1: 2, 3: 2, 4: 4}[kind]
 
 
-def run_cpu_baseline(mesh, wl, budget_s):
-    """Times the CPU oracle on the host cores on this workload; if the full
-    grid would exceed the budget, on a coarser grid of the same mesh (the
-    sample is named in the output)."""
-    from oracle import oracle_py as O
-    _, mode, param, kind, vs, ms, _ = wl
+def host_cores():
     cores = os.cpu_count() or 1
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         pass
-    cores = min(cores, 16)
-    scale = 1.0
+    return min(cores, 16)
+
+
+def run_cpu_baseline(mesh, wl, budget_s):
+    """The CPU oracle on the host cores; if the full grid would exceed the
+    budget, on a coarser grid of the same mesh (named in the output)."""
+    from oracle import oracle_py as O
+    _, mode, param, kind, vs, ms, _ = wl
+    cores = host_cores()
     g = O.plan_grid(mesh.vertices, mode, param)
-    n_full = g.alloc[0] * g.alloc[1] * g.alloc[2] * (4 if kind in (3, 4) else 1)
+    fcc = 4 if kind in (3, 4) else 1
+    n_full = g.alloc[0] * g.alloc[1] * g.alloc[2] * fcc
     # About 60 ns per node per core on this oracle; keep inside the budget.
     est = n_full * 60e-9 * 8 / cores
     sample = "full workload"
@@ -81,12 +105,226 @@ def run_cpu_baseline(mesh, wl, budget_s):
         scale = (budget_s / est) ** (1.0 / 3.0)
         g = O.plan_grid(mesh.vertices, mode, param / scale)
         sample = f"same mesh at cube length x{1 / scale:.2f} ({g.alloc[0]}x{g.alloc[1]}x{g.alloc[2]})"
-    n = g.alloc[0] * g.alloc[1] * g.alloc[2] * (4 if kind in (3, 4) else 1)
+    n = g.alloc[0] * g.alloc[1] * g.alloc[2] * fcc
     t0 = time.perf_counter()
     O.voxelize_nodes(mesh, g, kind, True, vox_split=vs, mat_split=ms, n_threads=cores)
     dt = time.perf_counter() - t0
     return {"value": round(n / dt / 1e9, 4), "unit": "Gvoxels/s", "cores": cores,
             "kind": "port", "sample": f"{sample}; {n} nodes in {dt:.2f} s"}
+
+
+def run_cpu_baseline_batch(meshes, kinds, budget_s):
+    """The CPU oracle over the batch's meshes in order until the budget is used."""
+    from oracle import oracle_py as O
+    cores = host_cores()
+    n = 0
+    k = 0
+    t0 = time.perf_counter()
+    for m, kind in zip(meshes, kinds):
+        g = O.plan_grid(m.vertices, 1, m.meta["cube_length"])
+        O.voxelize_nodes(m, g, kind, True, vox_split=(1024, 1024, 1024), n_threads=cores)
+        n += g.alloc[0] * g.alloc[1] * g.alloc[2]
+        k += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(n / dt / 1e9, 4), "unit": "Gvoxels/s", "cores": cores, "kind": "port",
+            "sample": f"first {k} of {len(meshes)} meshes; {n} nodes in {dt:.2f} s"}
+
+
+def load_traffic(kernel, workload):
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def train_cost_model(ctx, va, mesh_name, mode, param, kind, vs, ms):
+    """Records per-brick durations on a differently seeded mesh of the same
+    family and trains the cost model on them with the library's kernels."""
+    t0 = time.perf_counter()
+    train = build_mesh(mesh_name, seed=5)
+    ctx.set_mesh(train.vertices, train.indices)
+    ctx.set_materials(train.materials, int(train.materials.max()) + 1)
+    g = ctx.plan_grid(mode, param)
+    buf = va.DeviceBuffer(ctx, ctx.node_count(g, kind) * node_bytes(kind))
+    ctx.cost_record(True)
+    ctx.voxelize_nodes_device(g, kind, buf.ptr.value, True, vs, ms)
+    ctx.sync()
+    f, c = ctx.cost_samples()
+    ctx.cost_record(False)
+    buf.free()
+    mse = ctx.cost_train(f, c, epochs=20, batch=256, lr=2e-3, seed=0)
+    return {"training_mesh": f"{train.name} seed 5", "samples": int(c.size),
+            "mse_standardized": round(mse, 4), "seconds": round(time.perf_counter() - t0, 2)}
+
+
+def bench_single(args, va, wl):
+    mesh_name, mode, param, kind, vs, ms, label = wl
+    t0 = time.perf_counter()
+    mesh = build_mesh(mesh_name)
+    print(f"# mesh {mesh.name}: {mesh.nv} vertices, {mesh.nt} triangles "
+          f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    ctx = va.Context(0)
+    training = train_cost_model(ctx, va, mesh_name, mode, param, kind, vs, ms)
+    print(f"# cost model: {training}", file=sys.stderr, flush=True)
+    vbuf = va.DeviceBuffer(ctx, mesh.vertices.nbytes)
+    ibuf = va.DeviceBuffer(ctx, mesh.indices.nbytes)
+    mbuf = va.DeviceBuffer(ctx, mesh.materials.nbytes)
+    vbuf.upload(mesh.vertices)
+    ibuf.upload(mesh.indices)
+    mbuf.upload(mesh.materials)
+    ctx.set_mesh_device(vbuf.ptr.value, mesh.nv, ibuf.ptr.value, mesh.nt)
+    ctx.set_materials_device(mbuf.ptr.value, int(mesh.materials.max()) + 1)
+    grid = ctx.plan_grid(mode, param)
+    n_nodes = ctx.node_count(grid, kind)
+    nb = node_bytes(kind)
+    out = va.DeviceBuffer(ctx, n_nodes * nb)
+
+    def step():
+        g = ctx.plan_grid(mode, param)
+        ctx.voxelize_nodes_device(g, kind, out.ptr.value, True, vs, ms)
+
+    results = {}
+    for sched in (va.SCHEDULE_PLAIN, va.SCHEDULE_LEARNED):
+        ctx.set_schedule(sched)
+        for _ in range(args.warmup):
+            step()
+        ctx.sync()
+        # Timed region: K steps between two synchronisations.
+        ctx.enable_timing(True)
+        node_ms, stage = [], {}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+            st = ctx.stage_times()
+            node_ms.append(st["nodes_ms"])
+            for k, v in st.items():
+                stage[k] = stage.get(k, 0.0) + v
+        ctx.sync()
+        elapsed = time.perf_counter() - t0
+        ctx.enable_timing(False)
+        results[sched] = (elapsed, node_ms, {k: v / args.steps for k, v in stage.items()})
+    ctx.set_schedule(va.SCHEDULE_PLAIN)
+
+    chosen = va.SCHEDULE_LEARNED if args.schedule == "learned" else va.SCHEDULE_PLAIN
+    other = va.SCHEDULE_PLAIN if chosen == va.SCHEDULE_LEARNED else va.SCHEDULE_LEARNED
+    elapsed, node_ms, stage = results[chosen]
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = n_nodes / (elapsed / args.steps) / 1e9
+    plain_v = n_nodes / (results[va.SCHEDULE_PLAIN][0] / args.steps) / 1e9
+    learned_v = n_nodes / (results[va.SCHEDULE_LEARNED][0] / args.steps) / 1e9
+
+    kname = "k_nodes_fcc" if kind in (3, 4) else "k_nodes_cart"
+    kern_ms = float(np.mean(node_ms))
+    alg_bytes = n_nodes * (nb + 0.125)
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    pipe_bytes = n_nodes * nb + 12 * mesh.nv + 12 * mesh.nt + mesh.nt
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": load_traffic(kname, args.workload),
+            "kernel": kname, "kernel_ms": round(kern_ms, 4),
+            "algorithmic_bytes": int(alg_bytes),
+            "pipeline_frac": round(pipe_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    cpu = run_cpu_baseline(mesh, wl, args.cpu_budget) if args.cpu_baseline == "on" else None
+    check = None
+    if args.check:
+        from oracle import oracle_py as O
+        go = O.plan_grid(mesh.vertices, mode, param)
+        ref = O.voxelize_nodes(mesh, go, kind, True, vox_split=vs, mat_split=ms)
+        check = bool(np.array_equal(ref, out.download(np.uint8, n_nodes * nb)))
+    res = {
+        "metric": METRIC, "value": round(value, 4), "unit": "Gvoxels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded procedural mesh)",
+        "config": {"workload": label, "schedule": args.schedule, "nodes": n_nodes,
+                   "dims": list(ctx.node_dims(grid, kind)), "triangles": mesh.nt,
+                   "vox_split": list(vs), "mat_split": list(ms), "parallelism": "single GPU"},
+        "schedules": {"plain_gvox_s": round(plain_v, 4), "learned_gvox_s": round(learned_v, 4),
+                      "learned_over_plain": round(learned_v / plain_v, 4),
+                      "other_stages_ms": {k: round(v, 4) for k, v in results[other][2].items()},
+                      "cost_model": training},
+        "stages_ms": {k: round(v, 4) for k, v in stage.items()},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    if check is not None:
+        res["check_vs_oracle"] = check
+    return res
+
+
+def bench_batch(args, va, wl):
+    from voxelizer_amd import workloads as W
+    _, _, _, _, vs, ms, label = wl
+    t0 = time.perf_counter()
+    meshes = W.batch_meshes()
+    print(f"# batch: {len(meshes)} meshes, {sum(m.nt for m in meshes)} triangles "
+          f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    kinds = [va.NODE_SHORT if int(m.materials.max()) <= 7 else va.NODE_LONG for m in meshes]
+    ctx = va.Context(0)
+    training = train_cost_model(ctx, va, "room", 1, 0.01, va.NODE_LONG, (1024, 512, 512),
+                                (512, 512, 512))
+    items = (va.BatchItem * len(meshes))()
+    keep = []
+    n_nodes = 0
+    out_bytes = 0
+    for i, (m, kind) in enumerate(zip(meshes, kinds)):
+        g = va.plan_grid_host(m.vertices, va.PLAN_CUBE_LENGTH, m.meta["cube_length"], vs[0])
+        bufs = [va.DeviceBuffer(ctx, a.nbytes) for a in (m.vertices, m.indices, m.materials)]
+        for b, a in zip(bufs, (m.vertices, m.indices, m.materials)):
+            b.upload(np.ascontiguousarray(a))
+        nn = ctx.node_count(g, kind)
+        out = va.DeviceBuffer(ctx, nn * node_bytes(kind))
+        keep.extend(bufs + [out])
+        it = items[i]
+        it.vertices_dev, it.nv = bufs[0].ptr.value, m.nv
+        it.indices_dev, it.nt = bufs[1].ptr.value, m.nt
+        it.materials_dev, it.n_unique = bufs[2].ptr.value, int(m.materials.max()) + 1
+        it.kind = kind
+        it.grid = g
+        it.vox_split[:] = vs
+        it.mat_split[:] = ms
+        it.nodes_dev = out.ptr.value
+        n_nodes += nn
+        out_bytes += nn * node_bytes(kind)
+    results = {}
+    for sched, streams in ((va.SCHEDULE_PLAIN, 1), (va.SCHEDULE_LEARNED, BATCH_STREAMS)):
+        for _ in range(args.warmup):
+            ctx.voxelize_batch(items, sched, streams)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.voxelize_batch(items, sched, streams)
+        ctx.sync()
+        results[sched] = time.perf_counter() - t0
+    chosen = va.SCHEDULE_LEARNED if args.schedule == "learned" else va.SCHEDULE_PLAIN
+    elapsed = results[chosen]
+    plain_v = n_nodes / (results[va.SCHEDULE_PLAIN] / args.steps) / 1e9
+    learned_v = n_nodes / (results[va.SCHEDULE_LEARNED] / args.steps) / 1e9
+    value = n_nodes / (elapsed / args.steps) / 1e9
+    ms_per_step = elapsed * 1e3 / args.steps
+    pipe_bytes = out_bytes + sum(12 * m.nv + 13 * m.nt for m in meshes)
+    cpu = (run_cpu_baseline_batch(meshes, kinds, args.cpu_budget)
+           if args.cpu_baseline == "on" else None)
+    return {
+        "metric": METRIC, "value": round(value, 4), "unit": "Gvoxels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded procedural meshes)",
+        "config": {"workload": label, "schedule": args.schedule, "meshes": len(meshes),
+                   "nodes": n_nodes, "triangles": sum(m.nt for m in meshes),
+                   "streams": BATCH_STREAMS, "parallelism": "single GPU"},
+        "schedules": {"plain_gvox_s": round(plain_v, 4), "learned_gvox_s": round(learned_v, 4),
+                      "learned_over_plain": round(learned_v / plain_v, 4),
+                      "cost_model": training},
+        "roofline": {"bound": "hbm", "achieved": round(pipe_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(pipe_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "whole batch (node bytes + mesh reads per step)"},
+        "cpu_baseline": cpu,
+    }
 
 
 def main():
@@ -95,8 +333,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="hall", choices=sorted(WORKLOADS))
+    ap.add_argument("--schedule", default="learned", choices=["learned", "plain"],
+                    help="which schedule `value` reports (both are timed)")
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
-    ap.add_argument("--cpu-budget", type=float, default=25.0,
+    ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU work for the oracle baseline sample")
     ap.add_argument("--check", action="store_true",
                     help="compare the last output with the CPU oracle (slow for the hall)")
@@ -124,86 +364,7 @@ def main():
 
     import voxelizer_amd as va
     wl = WORKLOADS[args.workload]
-    mesh_name, mode, param, kind, vs, ms, label = wl
-    t0 = time.perf_counter()
-    mesh = build_mesh(mesh_name)
-    gen_s = time.perf_counter() - t0
-    print(f"# mesh {mesh.name}: {mesh.nv} vertices, {mesh.nt} triangles ({gen_s:.1f} s)",
-          file=sys.stderr, flush=True)
-
-    ctx = va.Context(0)
-    vbuf = va.DeviceBuffer(ctx, mesh.vertices.nbytes)
-    ibuf = va.DeviceBuffer(ctx, mesh.indices.nbytes)
-    mbuf = va.DeviceBuffer(ctx, mesh.materials.nbytes)
-    vbuf.upload(mesh.vertices)
-    ibuf.upload(mesh.indices)
-    mbuf.upload(mesh.materials)
-    ctx.set_mesh_device(vbuf.ptr.value, mesh.nv, ibuf.ptr.value, mesh.nt)
-    ctx.set_materials_device(mbuf.ptr.value, int(mesh.materials.max()) + 1)
-    grid = ctx.plan_grid(mode, param)
-    n_nodes = ctx.node_count(grid, kind)
-    nb = node_bytes(kind)
-    out = va.DeviceBuffer(ctx, n_nodes * nb)
-
-    def step():
-        g = ctx.plan_grid(mode, param)
-        ctx.voxelize_nodes_device(g, kind, out.ptr.value, True, vs, ms)
-
-    for _ in range(args.warmup):
-        step()
-    ctx.sync()
-    # Timed region: K steps between two synchronisations.
-    ctx.enable_timing(True)
-    node_ms = []
-    stage = {}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-        st = ctx.stage_times()
-        node_ms.append(st["nodes_ms"])
-        for k, v in st.items():
-            stage[k] = stage.get(k, 0.0) + v
-    ctx.sync()
-    elapsed = time.perf_counter() - t0
-    ctx.enable_timing(False)
-    ms_per_step = elapsed * 1e3 / args.steps
-    value = n_nodes / (elapsed / args.steps) / 1e9
-    for k in stage:
-        stage[k] /= args.steps
-
-    kern_ms = float(np.mean(node_ms))
-    alg_bytes = n_nodes * (nb + 0.125)
-    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
-    pipe_bytes = n_nodes * nb + 12 * mesh.nv + 12 * mesh.nt + mesh.nt
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_nodes_fcc" if kind in (3, 4) else "k_nodes_cart",
-            "kernel_ms": round(kern_ms, 4),
-            "pipeline_frac": round(pipe_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    cpu = None
-    if args.cpu_baseline == "on":
-        cpu = run_cpu_baseline(mesh, wl, args.cpu_budget)
-    check = None
-    if args.check:
-        from oracle import oracle_py as O
-        go = O.plan_grid(mesh.vertices, mode, param)
-        ref = O.voxelize_nodes(mesh, go, kind, True, vox_split=vs, mat_split=ms)
-        got = out.download(np.uint8, n_nodes * nb)
-        check = bool(np.array_equal(ref, got))
-    res = {
-        "metric": "Gvoxels/sec solid+classify (plain schedule), bit-exact; % HBM roofline",
-        "value": round(value, 4), "unit": "Gvoxels/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (seeded procedural mesh)",
-        "config": {"workload": label, "nodes": n_nodes, "dims": list(ctx.node_dims(grid, kind)),
-                   "triangles": mesh.nt, "vox_split": list(vs), "mat_split": list(ms),
-                   "parallelism": "single GPU"},
-        "stages_ms": {k: round(v, 4) for k, v in stage.items()},
-        "roofline": roof, "cpu_baseline": cpu,
-    }
-    if check is not None:
-        res["check_vs_oracle"] = check
+    res = bench_batch(args, va, wl) if args.workload == "batch" else bench_single(args, va, wl)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -184,9 +184,11 @@ typedef enum vox_schedule { VOX_SCHEDULE_PLAIN = 0, VOX_SCHEDULE_LEARNED = 1 } v
 #define VOX_COST_FEATURES 16
 #define VOX_COST_PARAMS 5313 /* W1[64][16], b1[64], W2[64][64], b2[64], W3[64], b3 */
 vox_status vox_set_schedule(vox_ctx *ctx, vox_schedule schedule);
-/* LEARNED order: bricks whose predicted cost (standardized log cycles)
- * exceeds `threshold` run first, most expensive first; the rest keep index
- * order (neighbouring bricks share triangles in cache).  Default 1.5. */
+/* LEARNED order: bricks with at least 1.5 times the mean number of
+ * (triangle, brick) pairs are scored by the model; those whose predicted cost
+ * (standardized log cycles) exceeds `threshold` run first, most expensive
+ * first; all others keep index order (neighbouring bricks share triangles in
+ * cache).  Default threshold 1.5. */
 vox_status vox_set_heavy_threshold(vox_ctx *ctx, float threshold);
 /* Model parameters (host arrays of VOX_COST_PARAMS floats). */
 vox_status vox_cost_set_params(vox_ctx *ctx, const float *params);

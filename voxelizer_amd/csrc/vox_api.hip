@@ -84,7 +84,8 @@ struct vox_ctx {
     int schedule = VOX_SCHEDULE_PLAIN;
     float *cost_params = nullptr;  // device, kCostParams floats
     bool cost_ready = false;
-    float heavy_cost = 1.5f;  // standardized log cost above which a brick goes first
+    float heavy_cost = 1.5f;    // standardized log cost above which a brick goes first
+    float screen_ratio = 1.5f;  // pair count / mean above which the model is consulted
     bool record = false;
     std::vector<float> rec_feats, rec_cycles;
     // worker contexts of batched runs (same device, own streams and buffers)
@@ -407,37 +408,54 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
     const uint32_t *brick_list = blist_order;
     float *feats = nullptr;
     uint64_t *cycles = nullptr;
-    if (features) {
-        VOX_TRY(buf(ctx, "f_feats", (size_t)nbt * kCostFeatures, &feats));
-        k_brick_features<<<grid_for(nbt, 4, 4096), 256, 0, ctx->stream>>>(
-            blist_order, n_list, brick_hist, brick_off, blist, tri_info, tri_ext, bdim[0], bdim[1],
-            bdim[2], feats);
+    if (features) VOX_TRY(buf(ctx, "f_feats", (size_t)nbt * kCostFeatures, &feats));
+    if (ctx->record) {  // training samples: features of every listed brick
+        k_brick_features<<<grid_for(nbt, 4, 65535), 256, 0, ctx->stream>>>(
+            blist_order, nullptr, n_list, brick_hist, brick_off, blist, tri_info, tri_ext, bdim[0],
+            bdim[1], bdim[2], feats);
         VOX_HIP(hipGetLastError());
-    }
-    if (learned) {
+    } else if (learned) {
+        // Screen by pair count, predict the candidates' cost, order.
         float *pred;
-        uint32_t *chist, *coff, *ccur, *sorted;
+        uint32_t *flags2, *soff, *sel, *light, *loff, *chist, *coff, *ccur, *sorted;
         VOX_TRY(buf(ctx, "f_pred", nbt, &pred));
+        VOX_TRY(buf(ctx, "f_sflags", nbt, &flags2));
+        VOX_TRY(buf(ctx, "f_soff", nbt + 1, &soff));
+        VOX_TRY(buf(ctx, "f_sel", nbt, &sel));
+        VOX_TRY(buf(ctx, "f_light", nbt, &light));
+        VOX_TRY(buf(ctx, "f_loff", nbt + 1, &loff));
         VOX_TRY(buf(ctx, "f_chist", kCostBuckets, &chist));
         VOX_TRY(buf(ctx, "f_coff", kCostBuckets + 1, &coff));
         VOX_TRY(buf(ctx, "f_ccur", kCostBuckets, &ccur));
         VOX_TRY(buf(ctx, "f_sorted", nbt, &sorted));
-        k_mlp_forward_n<<<(nbt + kMlpBlockRows - 1) / kMlpBlockRows, 256, 0, ctx->stream>>>(
-            ctx->cost_params, feats, n_list, pred);
-        VOX_HIP(hipGetLastError());
-        uint32_t *light, *loff;
-        VOX_TRY(buf(ctx, "f_light", nbt, &light));
-        VOX_TRY(buf(ctx, "f_loff", nbt + 1, &loff));
+        VOX_TRY(fill(ctx, flags2, nbt, 0u));
+        VOX_TRY(fill(ctx, light, nbt, 0u));
         VOX_TRY(fill(ctx, chist, kCostBuckets, 0u));
         VOX_TRY(fill(ctx, ccur, kCostBuckets, 0u));
-        VOX_TRY(fill(ctx, light, nbt, 0u));
-        k_cost_hist<<<grid_for(nbt, 256, 1024), 256, 0, ctx->stream>>>(pred, n_list,
-                                                                     ctx->heavy_cost, light, chist);
+        const uint32_t g256 = grid_for(nbt, 256, 1024);
+        k_cost_screen<<<g256, 256, 0, ctx->stream>>>(blist_order, n_list, brick_hist,
+                                                     cand_off + nt, ctx->screen_ratio, flags2,
+                                                     light);
+        VOX_HIP(hipGetLastError());
+        VOX_TRY(scan(ctx, flags2, soff, nbt));
+        const uint32_t *n_sel = soff + nbt;
+        k_cost_compact<<<g256, 256, 0, ctx->stream>>>(flags2, soff, n_list, sel);
+        VOX_HIP(hipGetLastError());
+        k_brick_features<<<grid_for(nbt, 4, 65535), 256, 0, ctx->stream>>>(
+            blist_order, sel, n_sel, brick_hist, brick_off, blist, tri_info, tri_ext, bdim[0],
+            bdim[1], bdim[2], feats);
+        VOX_HIP(hipGetLastError());
+        k_mlp_forward_n<<<(nbt + kMlpBlockRows - 1) / kMlpBlockRows, 256, 0, ctx->stream>>>(
+            ctx->cost_params, feats, n_sel, pred);
+        VOX_HIP(hipGetLastError());
+        k_cost_hist<<<g256, 256, 0, ctx->stream>>>(pred, sel, n_sel, ctx->heavy_cost, light,
+                                                   chist);
         VOX_HIP(hipGetLastError());
         VOX_TRY(scan(ctx, chist, coff, kCostBuckets));
         VOX_TRY(scan(ctx, light, loff, nbt));
-        k_cost_scatter<<<grid_for(nbt, 256, 1024), 256, 0, ctx->stream>>>(
-            pred, blist_order, n_list, ctx->heavy_cost, coff, loff, ccur, sorted);
+        k_cost_scatter<<<g256, 256, 0, ctx->stream>>>(pred, blist_order, n_list, sel, n_sel,
+                                                      ctx->heavy_cost, coff, light, loff, ccur,
+                                                      sorted);
         VOX_HIP(hipGetLastError());
         brick_list = sorted;
     }

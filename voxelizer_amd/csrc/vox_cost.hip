@@ -37,18 +37,19 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// Row r describes brick order[sel ? sel[r] : r], for r < *n_rows.
 __global__ __launch_bounds__(256) void k_brick_features(
-    const uint32_t *__restrict__ order, const uint32_t *__restrict__ n_order,
-    const uint32_t *__restrict__ brick_hist, const uint32_t *__restrict__ brick_off,
-    const uint32_t *__restrict__ blist, const uint4 *__restrict__ tri_info,
-    const uint4 *__restrict__ tri_ext, int32_t bd0, int32_t bd1, int32_t bd2,
-    float *__restrict__ feats) {
-    const uint32_t n = *n_order;
+    const uint32_t *__restrict__ order, const uint32_t *__restrict__ sel,
+    const uint32_t *__restrict__ n_rows, const uint32_t *__restrict__ brick_hist,
+    const uint32_t *__restrict__ brick_off, const uint32_t *__restrict__ blist,
+    const uint4 *__restrict__ tri_info, const uint4 *__restrict__ tri_ext, int32_t bd0,
+    int32_t bd1, int32_t bd2, float *__restrict__ feats) {
+    const uint32_t n = *n_rows;
     const uint32_t lane = lane_id();
     const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
     const float bvox = (float)(bd0 * bd1 * bd2);
     for (uint32_t q = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < n; q += nwaves) {
-        const uint32_t b = order[q];
+        const uint32_t b = order[sel ? sel[q] : q];
         const uint32_t cnt = brick_hist[b], off = brick_off[b];
         float c1 = 0, c2 = 0, c3 = 0, i1 = 0, i2 = 0, i3 = 0, v3 = 0, mx = 0, span = 0,
               small = 0, axx = 0, axy = 0;
@@ -126,9 +127,9 @@ struct MlpShared {
     float w2[kHid][kHid + 1];
     float b1[kHid], b2[kHid], w3[kHid];
     float b3;
-    float x[kMlpRows][kIn];
-    float h1[kMlpRows][kHid];
-    float h2[kMlpRows][kHid];
+    alignas(16) float x[kMlpRows][kIn];
+    alignas(16) float h1[kMlpRows][kHid];
+    alignas(16) float h2[kMlpRows][kHid];
 };
 
 __device__ void mlp_load(MlpShared &S, const float *__restrict__ P, const float *__restrict__ X,
@@ -152,19 +153,43 @@ __device__ void mlp_load(MlpShared &S, const float *__restrict__ P, const float 
 
 // Hidden layers for the block's rows; returns the outputs of rows
 // wave, wave+4, ... in y[] (valid in every lane).
-__device__ void mlp_hidden(MlpShared &S, float y[kMlpRows / 4]) {
+// Neuron j's weight rows live in registers; the row inputs are read as
+// 16-byte LDS broadcasts.
+__device__ __forceinline__ void mlp_hidden(MlpShared &S, float y[kMlpRows / 4]) {
     const uint32_t j = lane_id(), w = threadIdx.x / kWave;
+    float r1[kIn];
+#pragma unroll
+    for (int i = 0; i < kIn; i++) r1[i] = S.w1[j][i];
     for (int k = 0; k < kMlpRows / 4; k++) {
         const int s = (int)w + 4 * k;
         float z = S.b1[j];
-        for (int i = 0; i < kIn; i++) z += S.w1[j][i] * S.x[s][i];
+        const float4 *xv = (const float4 *)S.x[s];
+#pragma unroll
+        for (int i = 0; i < kIn / 4; i++) {
+            const float4 v = xv[i];
+            z += r1[4 * i] * v.x;
+            z += r1[4 * i + 1] * v.y;
+            z += r1[4 * i + 2] * v.z;
+            z += r1[4 * i + 3] * v.w;
+        }
         S.h1[s][j] = fmaxf(z, 0.0f);
     }
     __syncthreads();
+    float r2[kHid];
+#pragma unroll
+    for (int i = 0; i < kHid; i++) r2[i] = S.w2[j][i];
     for (int k = 0; k < kMlpRows / 4; k++) {
         const int s = (int)w + 4 * k;
         float z = S.b2[j];
-        for (int i = 0; i < kHid; i++) z += S.w2[j][i] * S.h1[s][i];
+        const float4 *hv = (const float4 *)S.h1[s];
+#pragma unroll
+        for (int i = 0; i < kHid / 4; i++) {
+            const float4 v = hv[i];
+            z += r2[4 * i] * v.x;
+            z += r2[4 * i + 1] * v.y;
+            z += r2[4 * i + 2] * v.z;
+            z += r2[4 * i + 3] * v.w;
+        }
         const float h = fmaxf(z, 0.0f);
         S.h2[s][j] = h;
         y[k] = wave_sum(S.w3[j] * h) + S.b3;
@@ -309,10 +334,12 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ P, const float
 }
 
 // ---------------------------------------------------------------------------
-// Learned brick order.  Bricks predicted heavy (standardized log cost above
+// Learned brick order.  Candidates are the bricks whose pair count is at
+// least `ratio` times the mean (the plain counts screen the list); the MLP
+// predicts their cost, and those predicted heavy (standardized log cost above
 // `heavy`) go first, most expensive first (1024-bucket counting sort on
-// bucket = (4 - y) * 128); the rest follow in index order, which keeps
-// neighbouring bricks -- which share triangles and solid rows in L2 --
+// bucket = (4 - y) * 128).  Every other brick follows in index order, which
+// keeps neighbouring bricks -- which share triangles and solid rows in L2 --
 // together.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t cost_bucket(float y) {
@@ -321,20 +348,46 @@ __device__ __forceinline__ uint32_t cost_bucket(float y) {
     return f >= 1023.0f ? 1023u : (uint32_t)f;
 }
 
-// light[q] = 1 for bricks that keep index order; heavy bricks counted per bucket.
+// flags[q] = 1 for candidate list positions q < n; light[q] = 1 for all q < n.
+__global__ __launch_bounds__(256) void k_cost_screen(const uint32_t *__restrict__ order,
+                                                     const uint32_t *__restrict__ n_order,
+                                                     const uint32_t *__restrict__ brick_hist,
+                                                     const uint32_t *__restrict__ total_pairs,
+                                                     float ratio, uint32_t *__restrict__ flags,
+                                                     uint32_t *__restrict__ light) {
+    const uint32_t n = *n_order;
+    const float thr = n ? ratio * (float)(*total_pairs) / (float)n : 0.0f;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        flags[q] = (float)brick_hist[order[q]] >= thr ? 1u : 0u;
+        light[q] = 1u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cost_compact(const uint32_t *__restrict__ flags,
+                                                      const uint32_t *__restrict__ off,
+                                                      const uint32_t *__restrict__ n_order,
+                                                      uint32_t *__restrict__ sel) {
+    const uint32_t n = *n_order;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+        if (flags[q]) sel[off[q]] = q;
+}
+
+// Heavy candidates: light[sel[c]] = 0 and one count in their cost bucket.
 __global__ __launch_bounds__(256) void k_cost_hist(const float *__restrict__ Y,
-                                                   const uint32_t *__restrict__ n_order,
+                                                   const uint32_t *__restrict__ sel,
+                                                   const uint32_t *__restrict__ n_sel,
                                                    float heavy, uint32_t *__restrict__ light,
                                                    uint32_t *__restrict__ hist) {
     __shared__ uint32_t h[kCostBuckets];
     for (uint32_t i = threadIdx.x; i < kCostBuckets; i += blockDim.x) h[i] = 0u;
     __syncthreads();
-    const uint32_t n = *n_order;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float y = Y[i];
-        const bool hv = y > heavy;
-        light[i] = hv ? 0u : 1u;
-        if (hv) atomicAdd(&h[cost_bucket(y)], 1u);
+    const uint32_t n = *n_sel;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const float y = Y[c];
+        if (y > heavy) {
+            light[sel[c]] = 0u;
+            atomicAdd(&h[cost_bucket(y)], 1u);
+        }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < kCostBuckets; i += blockDim.x)
@@ -344,24 +397,28 @@ __global__ __launch_bounds__(256) void k_cost_hist(const float *__restrict__ Y,
 // off: exclusive scan of hist (off[kCostBuckets] = heavy count); loff:
 // exclusive scan of light.
 __global__ __launch_bounds__(256) void k_cost_scatter(const float *__restrict__ Y,
-                                                      const uint32_t *__restrict__ in,
+                                                      const uint32_t *__restrict__ order,
                                                       const uint32_t *__restrict__ n_order,
+                                                      const uint32_t *__restrict__ sel,
+                                                      const uint32_t *__restrict__ n_sel,
                                                       float heavy,
                                                       const uint32_t *__restrict__ off,
+                                                      const uint32_t *__restrict__ light,
                                                       const uint32_t *__restrict__ loff,
                                                       uint32_t *__restrict__ cursor,
                                                       uint32_t *__restrict__ out) {
-    const uint32_t n = *n_order;
+    const uint32_t n = *n_order, nc = *n_sel;
     const uint32_t nheavy = off[kCostBuckets];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float y = Y[i];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += stride) {
+        const float y = Y[c];
         if (y > heavy) {
             const uint32_t k = cost_bucket(y);
-            out[off[k] + atomicAdd(&cursor[k], 1u)] = in[i];
-        } else {
-            out[nheavy + loff[i]] = in[i];
+            out[off[k] + atomicAdd(&cursor[k], 1u)] = order[sel[c]];
         }
     }
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+        if (light[q]) out[nheavy + loff[q]] = order[q];
 }
 
 }  // namespace vk

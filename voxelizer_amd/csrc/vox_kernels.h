@@ -252,7 +252,8 @@ __global__ void k_nodes_fcc(GridDims g, const uint32_t *s1, const uint32_t *s2,
 
 // Cost model kernels (vox_cost.hip).
 __global__ void k_brick_features(const uint32_t *__restrict__ order,
-                                 const uint32_t *__restrict__ n_order,
+                                 const uint32_t *__restrict__ sel,
+                                 const uint32_t *__restrict__ n_rows,
                                  const uint32_t *__restrict__ brick_hist,
                                  const uint32_t *__restrict__ brick_off,
                                  const uint32_t *__restrict__ blist,
@@ -271,11 +272,22 @@ __global__ void k_mlp_reduce(const float *__restrict__ partial, const float *__r
 __global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float *__restrict__ M,
                        float *__restrict__ V, uint32_t n, float lr, float b1, float b2, float eps,
                        float c1, float c2);
-__global__ void k_cost_hist(const float *__restrict__ Y, const uint32_t *__restrict__ n_order,
-                            float heavy, uint32_t *__restrict__ light, uint32_t *__restrict__ hist);
-__global__ void k_cost_scatter(const float *__restrict__ Y, const uint32_t *__restrict__ in,
-                               const uint32_t *__restrict__ n_order, float heavy,
-                               const uint32_t *__restrict__ off, const uint32_t *__restrict__ loff,
-                               uint32_t *__restrict__ cursor, uint32_t *__restrict__ out);
+__global__ void k_cost_screen(const uint32_t *__restrict__ order,
+                              const uint32_t *__restrict__ n_order,
+                              const uint32_t *__restrict__ brick_hist,
+                              const uint32_t *__restrict__ total_pairs, float ratio,
+                              uint32_t *__restrict__ flags, uint32_t *__restrict__ light);
+__global__ void k_cost_compact(const uint32_t *__restrict__ flags, const uint32_t *__restrict__ off,
+                               const uint32_t *__restrict__ n_order, uint32_t *__restrict__ sel);
+__global__ void k_cost_hist(const float *__restrict__ Y, const uint32_t *__restrict__ sel,
+                            const uint32_t *__restrict__ n_sel, float heavy,
+                            uint32_t *__restrict__ light, uint32_t *__restrict__ hist);
+__global__ void k_cost_scatter(const float *__restrict__ Y, const uint32_t *__restrict__ order,
+                               const uint32_t *__restrict__ n_order,
+                               const uint32_t *__restrict__ sel, const uint32_t *__restrict__ n_sel,
+                               float heavy, const uint32_t *__restrict__ off,
+                               const uint32_t *__restrict__ light,
+                               const uint32_t *__restrict__ loff, uint32_t *__restrict__ cursor,
+                               uint32_t *__restrict__ out);
 
 }  // namespace vk
