@@ -168,8 +168,13 @@ def bench_single(args, va, wl):
     print(f"# mesh {mesh.name}: {mesh.nv} vertices, {mesh.nt} triangles "
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     ctx = va.Context(0)
-    training = train_cost_model(ctx, va, mesh_name, mode, param, kind, vs, ms)
-    print(f"# cost model: {training}", file=sys.stderr, flush=True)
+    scheds = [va.SCHEDULE_PLAIN, va.SCHEDULE_LEARNED]
+    if args.only:
+        scheds = [va.SCHEDULE_LEARNED if args.schedule == "learned" else va.SCHEDULE_PLAIN]
+    training = None
+    if va.SCHEDULE_LEARNED in scheds:
+        training = train_cost_model(ctx, va, mesh_name, mode, param, kind, vs, ms)
+        print(f"# cost model: {training}", file=sys.stderr, flush=True)
     vbuf = va.DeviceBuffer(ctx, mesh.vertices.nbytes)
     ibuf = va.DeviceBuffer(ctx, mesh.indices.nbytes)
     mbuf = va.DeviceBuffer(ctx, mesh.materials.nbytes)
@@ -188,7 +193,7 @@ def bench_single(args, va, wl):
         ctx.voxelize_nodes_device(g, kind, out.ptr.value, True, vs, ms)
 
     results = {}
-    for sched in (va.SCHEDULE_PLAIN, va.SCHEDULE_LEARNED):
+    for sched in scheds:
         ctx.set_schedule(sched)
         for _ in range(args.warmup):
             step()
@@ -214,8 +219,7 @@ def bench_single(args, va, wl):
     elapsed, node_ms, stage = results[chosen]
     ms_per_step = elapsed * 1e3 / args.steps
     value = n_nodes / (elapsed / args.steps) / 1e9
-    plain_v = n_nodes / (results[va.SCHEDULE_PLAIN][0] / args.steps) / 1e9
-    learned_v = n_nodes / (results[va.SCHEDULE_LEARNED][0] / args.steps) / 1e9
+    gv = {s: n_nodes / (results[s][0] / args.steps) / 1e9 for s in results}
 
     kname = "k_nodes_fcc" if kind in (3, 4) else "k_nodes_cart"
     kern_ms = float(np.mean(node_ms))
@@ -243,16 +247,26 @@ def bench_single(args, va, wl):
         "config": {"workload": label, "schedule": args.schedule, "nodes": n_nodes,
                    "dims": list(ctx.node_dims(grid, kind)), "triangles": mesh.nt,
                    "vox_split": list(vs), "mat_split": list(ms), "parallelism": "single GPU"},
-        "schedules": {"plain_gvox_s": round(plain_v, 4), "learned_gvox_s": round(learned_v, 4),
-                      "learned_over_plain": round(learned_v / plain_v, 4),
-                      "other_stages_ms": {k: round(v, 4) for k, v in results[other][2].items()},
-                      "cost_model": training},
+        "schedules": schedules_block(va, gv, results, other, training),
         "stages_ms": {k: round(v, 4) for k, v in stage.items()},
         "roofline": roof, "cpu_baseline": cpu,
     }
     if check is not None:
         res["check_vs_oracle"] = check
     return res
+
+
+def schedules_block(va, gv, results, other, training):
+    out = {}
+    if va.SCHEDULE_PLAIN in gv:
+        out["plain_gvox_s"] = round(gv[va.SCHEDULE_PLAIN], 4)
+    if va.SCHEDULE_LEARNED in gv:
+        out["learned_gvox_s"] = round(gv[va.SCHEDULE_LEARNED], 4)
+    if len(gv) == 2:
+        out["learned_over_plain"] = round(gv[va.SCHEDULE_LEARNED] / gv[va.SCHEDULE_PLAIN], 4)
+        out["other_stages_ms"] = {k: round(v, 4) for k, v in results[other][2].items()}
+    out["cost_model"] = training
+    return out
 
 
 def bench_batch(args, va, wl):
@@ -335,6 +349,9 @@ def main():
     ap.add_argument("--workload", default="hall", choices=sorted(WORKLOADS))
     ap.add_argument("--schedule", default="learned", choices=["learned", "plain"],
                     help="which schedule `value` reports (both are timed)")
+    ap.add_argument("--only", action="store_true",
+                    help="time only the schedule named by --schedule (plain: no cost-model "
+                         "training; for profiling one schedule)")
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU work for the oracle baseline sample")

@@ -90,6 +90,8 @@ struct vox_ctx {
     std::vector<float> rec_feats, rec_cycles;
     // worker contexts of batched runs (same device, own streams and buffers)
     std::vector<vox_ctx *> workers;
+    // per-triangle vertex records of the current call (k_tri_records)
+    float4 *rec = nullptr;
 };
 
 #define VOX_HIP(expr)                                                               \
@@ -191,6 +193,18 @@ vox_status read_u32(vox_ctx *ctx, const uint32_t *dev, size_t n, uint32_t *host)
     return VOX_OK;
 }
 
+// Per-triangle vertex records of the current mesh, read by the solid and
+// surface stages instead of the indexed vertex arrays.
+vox_status make_records(vox_ctx *ctx) {
+    VOX_TRY(buf(ctx, "tri_rec", (size_t)std::max<uint32_t>(ctx->nt, 1) * 3, &ctx->rec));
+    if (ctx->nt) {
+        k_tri_records<<<grid_for(ctx->nt, 256), 256, 0, ctx->stream>>>(ctx->V, ctx->I, ctx->nt,
+                                                                        ctx->rec);
+        VOX_HIP(hipGetLastError());
+    }
+    return VOX_OK;
+}
+
 void record(vox_ctx *ctx, int i) {
     if (ctx->timing) (void)hipEventRecord(ctx->ev[i], ctx->stream);
 }
@@ -231,6 +245,7 @@ vox_status solid_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3], u
             SolidParams p;
             p.V = ctx->V;
             p.I = ctx->I;
+            p.rec = ctx->rec;
             p.nt = nt;
             p.emx = extMin[0];
             p.emy = extMin[1];
@@ -340,6 +355,7 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
     if (ctx->timing) ctx->times.learned_schedule = learned ? 1u : 0u;
     VOX_TRY(buf(ctx, (t + "_tri_info").c_str(), nt, &tri_info));
     if (features) VOX_TRY(buf(ctx, "f_tri_ext", nt, &tri_ext));
+    float4 *tri_rec = ctx->rec;
     VOX_TRY(buf(ctx, "f_cand", nt, &cand));
     VOX_TRY(buf(ctx, "f_cand_off", nt + 1, &cand_off));
     VOX_TRY(buf(ctx, "f_brick_hist", nbt, &brick_hist));
@@ -369,7 +385,8 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
     VOX_HIP(hipMemcpyAsync(dsplit, &st, sizeof(st), hipMemcpyHostToDevice, ctx->stream));
 
     if (nt > 0) {
-        k_surf_setup<<<grid_for(nt, 256), 256, 0, ctx->stream>>>(p, tri_info, cand, tri_ext);
+        k_surf_setup<<<grid_for(nt, 256), 256, 0, ctx->stream>>>(p, tri_info, cand, tri_ext,
+                                                                 tri_rec);
         VOX_HIP(hipGetLastError());
     }
     VOX_TRY(scan(ctx, cand, cand_off, nt));
@@ -471,7 +488,7 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
         const uint32_t blocks =
             std::min<uint32_t>(nbt, (uint32_t)(ctx->n_cu * ctx->surf_blocks_per_cu));
         k_surf_brick<<<blocks, 256, 0, ctx->stream>>>(
-            p, dsplit, tri_info, brick_off, brick_hist, blist, ctx->mats, bits, out.surf,
+            p, dsplit, tri_rec, brick_off, brick_hist, blist, ctx->mats, bits, out.surf,
             out.brick_mat_off, out.wrank, out.mat_list, stats, slot_cap, overflow, brick_list,
             n_list, next, cycles);
         VOX_HIP(hipGetLastError());
@@ -882,6 +899,7 @@ vox_status vox_solid_bits(vox_ctx *ctx, const vox_grid *g, const double shift[3]
     const uint32_t sy = split_yz ? split_yz[0] : 512u, sz = split_yz ? split_yz[1] : 512u;
     if (sy == 0 || sz == 0) return VOX_ERR_INVALID;
     record(ctx, 0);
+    VOX_TRY(make_records(ctx));
     VOX_TRY(solid_grid(ctx, *g, em, sy, sz, bits_dev, "g1"));
     record(ctx, 1);
     if (ctx->timing) {
@@ -913,6 +931,7 @@ vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *g, vox_node_kind kin
     const double base[3] = {g->min_vertex[0], g->min_vertex[1], g->min_vertex[2]};
     GridDims gd{g->alloc[0], g->alloc[1], g->alloc[2], g->alloc[0] / 32};
     record(ctx, 0);
+    VOX_TRY(make_records(ctx));
     if (!fcc) {
         uint32_t *bits;
         VOX_TRY(buf(ctx, "n_bits", words, &bits));

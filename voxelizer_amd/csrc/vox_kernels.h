@@ -166,6 +166,7 @@ __global__ void k_bbox(const float *__restrict__ V, uint32_t nv, uint32_t *__res
 struct SolidParams {
     const float *V;
     const uint32_t *I;
+    const float4 *rec;  // per-triangle vertex records (k_tri_records)
     uint32_t nt;
     double emx, emy, emz, d;
     uint32_t smin[2], smax[2];  // y/z subspace, ext coords
@@ -174,6 +175,11 @@ struct SolidParams {
     uint32_t rowsY;             // alloc y (= ext y + 2)
 };
 
+// Per-triangle records: vertices v0, v1, v2 in 48 contiguous bytes
+// (rec[3t] = v0.xyz v1.x, rec[3t+1] = v1.yz v2.xy, rec[3t+2] = v2.z, class
+// word (surface stage), 0, 0).
+__global__ void k_tri_records(const float *__restrict__ V, const uint32_t *__restrict__ I,
+                              uint32_t nt, float4 *__restrict__ rec);
 __global__ void k_solid_setup(SolidParams p, uint32_t *__restrict__ cand_count,
                               uint4 *__restrict__ cbox);
 __global__ void k_solid_candidates(SolidParams p, const uint32_t *__restrict__ cand_off,
@@ -195,7 +201,8 @@ __global__ void k_zero_pad_rows(uint32_t *__restrict__ bits, uint32_t words_per_
 // Surface
 // ---------------------------------------------------------------------------
 __global__ void k_surf_setup(SurfParams p, uint4 *__restrict__ tri_info,
-                             uint32_t *__restrict__ cand_count, uint4 *__restrict__ tri_ext);
+                             uint32_t *__restrict__ cand_count, uint4 *__restrict__ tri_ext,
+                             float4 *__restrict__ tri_rec);
 __global__ void k_surf_bin(SurfParams p, const uint32_t *__restrict__ cand_off,
                            const uint32_t *__restrict__ cand_total,
                            const uint4 *__restrict__ tri_info,
@@ -203,7 +210,7 @@ __global__ void k_surf_bin(SurfParams p, const uint32_t *__restrict__ cand_off,
                            uint32_t *__restrict__ brick_cursor, uint32_t *__restrict__ blist,
                            int scatter);
 __global__ void k_surf_brick(SurfParams p, const SplitTable *__restrict__ splits,
-                             const uint4 *__restrict__ tri_info,
+                             const float4 *__restrict__ tri_rec,
                              const uint32_t *__restrict__ brick_off,
                              const uint32_t *__restrict__ brick_hist,
                              const uint32_t *__restrict__ blist,

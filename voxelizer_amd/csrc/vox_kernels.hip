@@ -107,10 +107,26 @@ __global__ __launch_bounds__(256) void k_bbox(const float *__restrict__ V, uint3
 // ---------------------------------------------------------------------------
 // Solid stage
 // ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_tri_records(const float *__restrict__ V,
+                                                     const uint32_t *__restrict__ I, uint32_t nt,
+                                                     float4 *__restrict__ rec) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+        const uint32_t a = I[3u * t], b = I[3u * t + 1u], c = I[3u * t + 2u];
+        rec[3 * (size_t)t] = make_float4(V[3u * a], V[3u * a + 1u], V[3u * a + 2u], V[3u * b]);
+        rec[3 * (size_t)t + 1] =
+            make_float4(V[3u * b + 1u], V[3u * b + 2u], V[3u * c], V[3u * c + 1u]);
+        rec[3 * (size_t)t + 2] = make_float4(V[3u * c + 2u], 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// The triangle's vertices widened to FP64 (fetchTriangleVertices,
+// src/voxelizer.cu:2242-2253), from its 48-byte record.
 __device__ __forceinline__ void load_tri_d(const SolidParams &p, uint32_t t, D3 v[3]) {
-    v[0] = fetch_d(p.V, p.I, t, 0);
-    v[1] = fetch_d(p.V, p.I, t, 1);
-    v[2] = fetch_d(p.V, p.I, t, 2);
+    const float4 r0 = p.rec[3 * (size_t)t], r1 = p.rec[3 * (size_t)t + 1],
+                 r2 = p.rec[3 * (size_t)t + 2];
+    v[0] = {(double)r0.x, (double)r0.y, (double)r0.z};
+    v[1] = {(double)r0.w, (double)r1.x, (double)r1.y};
+    v[2] = {(double)r1.z, (double)r1.w, (double)r2.x};
 }
 
 // Candidate tiles of each triangle: its commonSpace tile rectangle; the
@@ -302,11 +318,6 @@ __global__ __launch_bounds__(256) void k_zero_pad_rows(uint32_t *__restrict__ bi
 // ---------------------------------------------------------------------------
 // Surface stage
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void load_tri_f(const SurfParams &p, uint32_t t, F3 tri[3]) {
-    tri[0] = fetch_f(p.V, p.I, t, 0);
-    tri[1] = fetch_f(p.V, p.I, t, 1);
-    tri[2] = fetch_f(p.V, p.I, t, 2);
-}
 
 // Voxel range of a triangle clipped to the whole ext space (the union of all
 // matSplitRes subspaces); false if it lies outside (adjustVoxelRange,
@@ -351,10 +362,12 @@ __device__ __forceinline__ void brick_range(const SurfParams &p, const VoxBB &vb
 // for 1D, dominant-plane columns otherwise), the cost model's inputs.
 __global__ __launch_bounds__(256) void k_surf_setup(SurfParams p, uint4 *__restrict__ tri_info,
                                                     uint32_t *__restrict__ cand_count,
-                                                    uint4 *__restrict__ tri_ext) {
+                                                    uint4 *__restrict__ tri_ext,
+                                                    float4 *__restrict__ tri_rec) {
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < p.nt; t += gridDim.x * blockDim.x) {
-        F3 tri[3];
-        load_tri_f(p, t, tri);
+        const float4 r0 = tri_rec[3 * (size_t)t], r1 = tri_rec[3 * (size_t)t + 1],
+                     r2 = tri_rec[3 * (size_t)t + 2];
+        const F3 tri[3] = {{r0.x, r0.y, r0.z}, {r0.w, r1.x, r1.y}, {r1.z, r1.w, r2.x}};
         int bb, axis;
         classify(tri, p.m, p.d, bb, axis);
         uint32_t c = 0;
@@ -374,6 +387,8 @@ __global__ __launch_bounds__(256) void k_surf_setup(SurfParams p, uint4 *__restr
         const uint32_t info = ((uint32_t)bb << 30) | ((uint32_t)axis << 28);
         tri_info[t] = make_uint4(info, lo[0] | (lo[1] << 16), lo[2] | (w[0] << 16),
                                  w[1] | (w[2] << 16));
+        // The class word goes into the triangle's record for the brick pass.
+        reinterpret_cast<float *>(tri_rec)[12 * (size_t)t + 9] = __uint_as_float(info);
         cand_count[t] = c;
         if (tri_ext) tri_ext[t] = ext;
     }
@@ -459,11 +474,14 @@ constexpr int kSurfChunk = 64;
 // Sets up triangle t for the brick whose alloc-coordinate box starts at
 // (ext coordinates) (blx, bly, blz); returns its work items in the brick: voxels along
 // the axis for 1D, dominant-plane columns for 2D/3D.
-__device__ __forceinline__ uint32_t surf_tri_setup(const SurfParams &p, uint32_t t, uint32_t info,
+__device__ __forceinline__ uint32_t surf_tri_setup(const SurfParams &p,
+                                                   const float4 *__restrict__ tri_rec, uint32_t t,
                                                    int32_t blx, int32_t bly, int32_t blz,
                                                    SurfTri &st) {
-    F3 tri[3];
-    load_tri_f(p, t, tri);
+    const float4 r0 = tri_rec[3 * (size_t)t], r1 = tri_rec[3 * (size_t)t + 1],
+                 r2 = tri_rec[3 * (size_t)t + 2];
+    const F3 tri[3] = {{r0.x, r0.y, r0.z}, {r0.w, r1.x, r1.y}, {r1.z, r1.w, r2.x}};
+    const uint32_t info = __float_as_uint(r2.y);
     VoxBB vb;
     if (!surf_range(p, tri, vb)) return 0u;
     const int bb = (int)(info >> 30), a = (int)((info >> 28) & 3u);
@@ -628,7 +646,7 @@ __device__ __forceinline__ void surf_item(const SurfParams &p, const SplitTable 
 // cleared for the next brick.  Surface bits are ORed into the solid grid.
 // With `cycles`, each brick's duration (s_memtime) is recorded.
 __global__ __launch_bounds__(256, 5) void k_surf_brick(
-    SurfParams p, const SplitTable *__restrict__ splits, const uint4 *__restrict__ tri_info,
+    SurfParams p, const SplitTable *__restrict__ splits, const float4 *__restrict__ tri_rec,
     const uint32_t *__restrict__ brick_off, const uint32_t *__restrict__ brick_hist,
     const uint32_t *__restrict__ blist, const uint8_t *__restrict__ materials,
     uint32_t *__restrict__ bits, uint32_t *__restrict__ surf, uint32_t *__restrict__ brick_mat_off,
@@ -698,7 +716,7 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
                 uint32_t items = 0u;
                 if (c0 + tid < cnt) {
                     const uint32_t t = blist[off + c0 + tid];
-                    items = surf_tri_setup(p, t, tri_info[t].x, blx, bly, blz, tris[tid]);
+                    items = surf_tri_setup(p, tri_rec, t, blx, bly, blz, tris[tid]);
                 }
                 uint32_t incl = items;  // wave-level inclusive scan
                 for (int d = 1; d < kSurfChunk; d <<= 1) {
@@ -950,6 +968,18 @@ __device__ __forceinline__ void write_word_nodes(int kind, void *__restrict__ no
     const size_t node0 = wi * 32u;
     const int nb = kind == 0 ? 1 : 2;  // bytes per node
     uint4 *dst = (uint4 *)((uint8_t *)nodes + node0 * (size_t)nb);
+    if (!patch && (interior == 0u || interior == 0xFFFFFFFFu)) {
+        // All 32 nodes empty, or all fully surrounded (id 27, no material).
+        const uint32_t v = interior ? (kind == 0 ? 0x1B1B1B1Bu : 0x001B001Bu) : 0u;
+        const uint4 q = make_uint4(v, v, v, v);
+        dst[0] = q;
+        dst[1] = q;
+        if (kind != 0) {
+            dst[2] = q;
+            dst[3] = q;
+        }
+        return;
+    }
     if (kind == 0) {
 #pragma unroll
         for (int j = 0; j < 2; j++) {
