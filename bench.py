@@ -192,6 +192,10 @@ def bench_single(args, va, wl):
         g = ctx.plan_grid(mode, param)
         ctx.voxelize_nodes_device(g, kind, out.ptr.value, True, vs, ms)
 
+    # Surface brick shape picked by the autotuner (before the timed steps).
+    bdims, bms = ctx.autotune_bricks(grid, kind, out.ptr.value, 2, vs, ms)
+    print(f"# autotuned brick {bdims} ({bms:.3f} ms)", file=sys.stderr, flush=True)
+
     results = {}
     for sched in scheds:
         ctx.set_schedule(sched)
@@ -246,7 +250,8 @@ def bench_single(args, va, wl):
         "data": "synthetic (seeded procedural mesh)",
         "config": {"workload": label, "schedule": args.schedule, "nodes": n_nodes,
                    "dims": list(ctx.node_dims(grid, kind)), "triangles": mesh.nt,
-                   "vox_split": list(vs), "mat_split": list(ms), "parallelism": "single GPU"},
+                   "vox_split": list(vs), "mat_split": list(ms), "parallelism": "single GPU",
+                   "surface_brick": list(bdims)},
         "schedules": schedules_block(va, gv, results, other, training),
         "stages_ms": {k: round(v, 4) for k, v in stage.items()},
         "roofline": roof, "cpu_baseline": cpu,

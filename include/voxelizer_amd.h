@@ -218,6 +218,23 @@ vox_status vox_mlp_forward(vox_ctx *ctx, const float *params_dev, const float *x
 vox_status vox_mlp_grad(vox_ctx *ctx, const float *params_dev, const float *x_dev,
                         const float *t_dev, uint32_t n, float *grad_dev, float *loss);
 
+/* ---- surface brick shape and its autotuner (no counterpart in the
+ * reference) -------------------------------------------------------------
+ * The surface pass works on bricks of voxels (default 128x4x8 on Cartesian
+ * grids, 64x4x4 on each FCC sub-grid).  Shapes are powers of two with x a
+ * multiple of 64, y*z <= 32 rows and at most 4096 voxels; Cartesian shapes
+ * need z a multiple of 4.  The shape changes speed only, never output. */
+vox_status vox_set_brick_dims(vox_ctx *ctx, int fcc, const uint32_t dims[3]);
+vox_status vox_get_brick_dims(const vox_ctx *ctx, int fcc, uint32_t dims[3]);
+/* Times vox_voxelize_nodes (with materials) for this grid and node kind
+ * with each candidate shape (one warm-up, then the best of `reps` runs),
+ * keeps the fastest for the context and returns it and its time.  nodes_dev
+ * ends up holding the nodes, as after a vox_voxelize_nodes call. */
+vox_status vox_autotune_bricks(vox_ctx *ctx, const vox_grid *grid, vox_node_kind kind,
+                               const uint32_t vox_split[3], const uint32_t mat_split[3],
+                               void *nodes_dev, uint32_t reps, uint32_t best_dims[3],
+                               float *best_ms);
+
 /* ---- batches of meshes (no counterpart in the reference) ---------------
  * Many meshes voxelized in one call, each into its own node array.  Inputs
  * are device arrays; the grid of each item is planned by the caller

@@ -79,3 +79,52 @@ def test_stage_times_reported(gpu_ctx):
     t = gpu_ctx.stage_times()
     gpu_ctx.enable_timing(False)
     assert t["total_ms"] > 0 and t["nodes_ms"] > 0 and t["erosion_passes"] >= 1
+
+
+@pytest.mark.parametrize("kind,shapes", [
+    (va.NODE_LONG, [(64, 4, 8), (128, 2, 8), (128, 4, 4), (256, 4, 4), (256, 2, 8), (128, 8, 4)]),
+    (va.NODE_LONG_FCC, [(64, 4, 8), (128, 4, 4), (64, 8, 4), (128, 2, 4), (64, 2, 8)]),
+])
+def test_brick_shapes_give_identical_nodes(gpu_ctx, oracle, kind, shapes):
+    fcc = kind == va.NODE_LONG_FCC
+    m = W.shoebox_room(seed=31, n_boxes=12, subdiv=3)
+    gpu_ctx.set_mesh(m.vertices, m.indices)
+    gpu_ctx.set_materials(m.materials, 9)
+    g = gpu_ctx.plan_grid(va.PLAN_CUBE_LENGTH, 0.017)
+    default = gpu_ctx.brick_dims(fcc)
+    ref = oracle.voxelize_nodes(m, oracle.plan_grid(m.vertices, va.PLAN_CUBE_LENGTH, 0.017), kind,
+                                True)
+    try:
+        for sh in shapes:
+            gpu_ctx.set_brick_dims(sh, fcc)
+            assert gpu_ctx.brick_dims(fcc) == sh
+            assert np.array_equal(gpu_ctx.voxelize_nodes(g, kind), ref), sh
+    finally:
+        gpu_ctx.set_brick_dims(default, fcc)
+
+
+def test_brick_shape_validation(gpu_ctx):
+    for bad in [(32, 4, 8), (128, 3, 8), (128, 8, 8), (256, 4, 8), (128, 4, 2)]:
+        with pytest.raises(va.VoxError):
+            gpu_ctx.set_brick_dims(bad, False)
+    with pytest.raises(va.VoxError):
+        gpu_ctx.set_brick_dims((128, 8, 8), True)
+
+
+def test_autotune_bricks(gpu_ctx, oracle):
+    m = W.shoebox_room(seed=32, n_boxes=8, subdiv=3)
+    gpu_ctx.set_mesh(m.vertices, m.indices)
+    gpu_ctx.set_materials(m.materials, 9)
+    g = gpu_ctx.plan_grid(va.PLAN_CUBE_LENGTH, 0.02)
+    n = gpu_ctx.node_count(g, va.NODE_LONG) * 2
+    buf = va.DeviceBuffer(gpu_ctx, n)
+    default = gpu_ctx.brick_dims(False)
+    try:
+        dims, ms = gpu_ctx.autotune_bricks(g, va.NODE_LONG, buf.ptr.value, reps=1)
+        assert gpu_ctx.brick_dims(False) == dims and ms > 0
+        ref = oracle.voxelize_nodes(m, oracle.plan_grid(m.vertices, va.PLAN_CUBE_LENGTH, 0.02),
+                                    va.NODE_LONG, True)
+        assert np.array_equal(buf.download(np.uint8, n), ref)
+    finally:
+        gpu_ctx.set_brick_dims(default, False)
+        buf.free()

@@ -81,7 +81,7 @@ EXPORTED = [
     "vox_node_count", "vox_node_size", "vox_enable_timing", "vox_get_stage_times",
     "vox_set_schedule", "vox_set_heavy_threshold", "vox_cost_set_params", "vox_cost_get_params", "vox_cost_record",
     "vox_cost_samples", "vox_cost_train", "vox_mlp_forward", "vox_mlp_grad",
-    "vox_voxelize_batch",
+    "vox_voxelize_batch", "vox_set_brick_dims", "vox_get_brick_dims", "vox_autotune_bricks",
 ]
 
 _lib = None
@@ -137,6 +137,9 @@ def lib():
     L.vox_mlp_forward.argtypes = [P, P, P, U32, P]
     L.vox_mlp_grad.argtypes = [P, P, P, P, U32, P, C.POINTER(F)]
     L.vox_voxelize_batch.argtypes = [P, C.POINTER(BatchItem), U32, I32, U32]
+    L.vox_set_brick_dims.argtypes = [P, I32, P]
+    L.vox_get_brick_dims.argtypes = [P, I32, P]
+    L.vox_autotune_bricks.argtypes = [P, C.POINTER(Grid), I32, P, P, P, U32, P, C.POINTER(F)]
     _lib = L
     return L
 
@@ -307,6 +310,28 @@ class Context:
     def voxelize_batch(self, items, schedule: int = 0, streams: int = 1):
         """items: a ctypes array of BatchItem (device pointers filled in)."""
         self._check(lib().vox_voxelize_batch(self.h, items, len(items), schedule, streams))
+
+    def set_brick_dims(self, dims, fcc: bool = False):
+        d = np.array(dims, np.uint32)
+        self._check(lib().vox_set_brick_dims(self.h, 1 if fcc else 0, _ptr(d)))
+
+    def brick_dims(self, fcc: bool = False):
+        d = np.zeros(3, np.uint32)
+        self._check(lib().vox_get_brick_dims(self.h, 1 if fcc else 0, _ptr(d)))
+        return tuple(int(x) for x in d)
+
+    def autotune_bricks(self, grid: Grid, kind: int, out_ptr: int, reps: int = 2,
+                        vox_split=DEFAULT_VOX_SPLIT, mat_split=DEFAULT_MAT_SPLIT):
+        """Picks the fastest surface brick shape for this grid and node kind;
+        returns (dims, ms)."""
+        vs = np.array(vox_split, np.uint32)
+        ms = np.array(mat_split, np.uint32)
+        best = np.zeros(3, np.uint32)
+        t = C.c_float()
+        self._check(lib().vox_autotune_bricks(self.h, C.byref(grid), kind, _ptr(vs), _ptr(ms),
+                                              C.c_void_p(int(out_ptr)), reps, _ptr(best),
+                                              C.byref(t)))
+        return tuple(int(x) for x in best), float(t.value)
 
     def set_heavy_threshold(self, threshold: float):
         self._check(lib().vox_set_heavy_threshold(self.h, float(threshold)))

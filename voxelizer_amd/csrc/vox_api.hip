@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -92,6 +93,9 @@ struct vox_ctx {
     std::vector<vox_ctx *> workers;
     // per-triangle vertex records of the current call (k_tri_records)
     float4 *rec = nullptr;
+    // surface brick shapes (voxels along x, y, z)
+    int cart_bdim[3] = {kCartBX, kCartBY, kCartBZ};
+    int fcc_bdim[3] = {kFccBX, kFccBY, kFccBZ};
 };
 
 #define VOX_HIP(expr)                                                               \
@@ -191,6 +195,29 @@ vox_status read_u32(vox_ctx *ctx, const uint32_t *dev, size_t n, uint32_t *host)
     VOX_HIP(hipStreamSynchronize(ctx->stream));
     std::memcpy(host, ctx->pinned, n * 4);
     return VOX_OK;
+}
+
+int ilog2(uint32_t v) { return 31 - __builtin_clz(v); }
+
+BrickDims brick_dims(const vox_grid &g, const int bdim[3]) {
+    BrickDims d;
+    d.nbx = (g.alloc[0] + bdim[0] - 1) / bdim[0];
+    d.nby = (g.alloc[1] + bdim[1] - 1) / bdim[1];
+    d.lwx = (uint32_t)ilog2((uint32_t)bdim[0] / 32u);
+    d.lby = (uint32_t)ilog2((uint32_t)bdim[1]);
+    d.lbz = (uint32_t)ilog2((uint32_t)bdim[2]);
+    return d;
+}
+
+// A brick shape the surface and node passes support: powers of two, x a
+// multiple of 64, at most 32 rows (y*z), at most kMaxBrickVox voxels, and (for
+// the Cartesian node pass, whose threads own kNodeZ planes) z a multiple of
+// kNodeZ.
+bool brick_ok(const uint32_t b[3], bool cart) {
+    for (int a = 0; a < 3; a++)
+        if (b[a] == 0 || (b[a] & (b[a] - 1)) != 0) return false;
+    return b[0] >= 64 && b[1] * b[2] <= 32 && b[0] * b[1] * b[2] <= (uint32_t)kMaxBrickVox &&
+           (!cart || b[2] % kNodeZ == 0);
 }
 
 // Per-triangle vertex records of the current mesh, read by the solid and
@@ -938,10 +965,8 @@ vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *g, vox_node_kind kin
         VOX_TRY(solid_grid(ctx, *g, base, vs[1], vs[2], bits, "g1"));
         record(ctx, 1);
         SurfOut so{};
-        if (use_materials) {
-            const int bdim[3] = {kCartBX, kCartBY, kCartBZ};
-            VOX_TRY(surface_grid(ctx, *g, base, ms, bdim, bits, "g1", so));
-        }
+        const int *bdim = ctx->cart_bdim;
+        if (use_materials) VOX_TRY(surface_grid(ctx, *g, base, ms, bdim, bits, "g1", so));
         record(ctx, 2);
         // Several erosion subspaces: keep the pre-erosion grid for the ids
         // of nodes next to a later subspace (see k_nodes_cart).
@@ -963,15 +988,11 @@ vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *g, vox_node_kind kin
         }
         VOX_TRY(erode(ctx, gd, vs[1], vs[2], bits));
         record(ctx, 3);
-        const uint32_t nbx = (g->alloc[0] + kCartBX - 1) / kCartBX;
-        const uint32_t nby = (g->alloc[1] + kCartBY - 1) / kCartBY;
-        const uint32_t nbz = (g->alloc[2] + kCartBZ - 1) / kCartBZ;
-        (void)nbz;
         MatSource msrc{so.surf, so.brick_mat_off, so.wrank, so.mat_list};
         k_nodes_cart<<<row_grid(gd.W, gd.Y, (gd.Z + kNodeZ - 1) / kNodeZ), dim3(kWave, 4), 0,
                        ctx->stream>>>(
             gd, bits, pre, ystart, zstart, msrc, use_materials ? 1 : 0,
-            kind == VOX_NODE_SHORT ? 0 : 1, nodes_dev, nbx, nby);
+            kind == VOX_NODE_SHORT ? 0 : 1, nodes_dev, brick_dims(*g, bdim));
         VOX_HIP(hipGetLastError());
         record(ctx, 4);
     } else {
@@ -991,8 +1012,8 @@ vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *g, vox_node_kind kin
             VOX_TRY(solid_grid(ctx, *g, em, vs[1], vs[2], gb[k], tags[k]));
         }
         record(ctx, 1);
+        const int *bdim = ctx->fcc_bdim;
         if (use_materials) {
-            const int bdim[3] = {kFccBX, kFccBY, kFccBZ};
             for (int k = 0; k < 4; k++) {
                 const double em[3] = {base[0] + shifts[k][0], base[1] + shifts[k][1],
                                       base[2] + shifts[k][2]};
@@ -1001,16 +1022,12 @@ vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *g, vox_node_kind kin
         }
         record(ctx, 2);
         record(ctx, 3);
-        const uint32_t nbx = (g->alloc[0] + kFccBX - 1) / kFccBX;
-        const uint32_t nby = (g->alloc[1] + kFccBY - 1) / kFccBY;
-        const uint32_t nbz = (g->alloc[2] + kFccBZ - 1) / kFccBZ;
         MatSource m[4];
         for (int k = 0; k < 4; k++)
             m[k] = MatSource{so[k].surf, so[k].brick_mat_off, so[k].wrank, so[k].mat_list};
-        (void)nbz;
         k_nodes_fcc<<<row_grid(2 * gd.W, gd.Y, 2 * gd.Z), dim3(kWave, 4), 0, ctx->stream>>>(
             gd, gb[0], gb[1], gb[2], gb[3], m[0], m[1], m[2], m[3], use_materials ? 1 : 0,
-            kind == VOX_NODE_SHORT_FCC ? 3 : 4, nodes_dev, nbx, nby);
+            kind == VOX_NODE_SHORT_FCC ? 3 : 4, nodes_dev, brick_dims(*g, bdim));
         VOX_HIP(hipGetLastError());
         record(ctx, 4);
     }
@@ -1034,6 +1051,62 @@ vox_status vox_enable_timing(vox_ctx *ctx, int enable) {
 vox_status vox_get_stage_times(const vox_ctx *ctx, vox_stage_times *out) {
     if (!ctx || !out) return VOX_ERR_INVALID;
     *out = ctx->times;
+    return VOX_OK;
+}
+
+// ---- surface brick shape ----------------------------------------------------
+
+vox_status vox_set_brick_dims(vox_ctx *ctx, int fcc, const uint32_t dims[3]) {
+    if (!ctx || !dims || !brick_ok(dims, fcc == 0))
+        return ctx ? fail(ctx, VOX_ERR_INVALID, "unsupported brick shape") : VOX_ERR_INVALID;
+    int *d = fcc ? ctx->fcc_bdim : ctx->cart_bdim;
+    for (int a = 0; a < 3; a++) d[a] = (int)dims[a];
+    return VOX_OK;
+}
+
+vox_status vox_get_brick_dims(const vox_ctx *ctx, int fcc, uint32_t dims[3]) {
+    if (!ctx || !dims) return VOX_ERR_INVALID;
+    const int *d = fcc ? ctx->fcc_bdim : ctx->cart_bdim;
+    for (int a = 0; a < 3; a++) dims[a] = (uint32_t)d[a];
+    return VOX_OK;
+}
+
+vox_status vox_autotune_bricks(vox_ctx *ctx, const vox_grid *grid, vox_node_kind kind,
+                               const uint32_t vox_split[3], const uint32_t mat_split[3],
+                               void *nodes_dev, uint32_t reps, uint32_t best_dims[3],
+                               float *best_ms) {
+    if (!ctx || !grid || !nodes_dev || reps == 0) return VOX_ERR_INVALID;
+    const bool fcc = kind == VOX_NODE_SHORT_FCC || kind == VOX_NODE_LONG_FCC;
+    static const uint32_t cart[][3] = {{128, 4, 8}, {64, 4, 8}, {128, 2, 8}, {128, 4, 4},
+                                       {256, 4, 4}, {256, 2, 8}, {64, 4, 4}, {128, 8, 4}};
+    static const uint32_t fccs[][3] = {{64, 4, 4}, {64, 4, 8}, {128, 4, 4}, {64, 8, 4},
+                                       {128, 2, 4}, {64, 2, 8}};
+    const uint32_t(*cand)[3] = fcc ? fccs : cart;
+    const size_t ncand = fcc ? sizeof(fccs) / sizeof(fccs[0]) : sizeof(cart) / sizeof(cart[0]);
+    double best = 1e300;
+    size_t bi = 0;
+    for (size_t c = 0; c < ncand; c++) {
+        VOX_TRY(vox_set_brick_dims(ctx, fcc ? 1 : 0, cand[c]));
+        double t_best = 1e300;
+        for (uint32_t r = 0; r <= reps; r++) {  // run 0 warms up
+            VOX_TRY(vox_sync(ctx));
+            const auto t0 = std::chrono::steady_clock::now();
+            VOX_TRY(vox_voxelize_nodes(ctx, grid, kind, 1, vox_split, mat_split, nodes_dev));
+            VOX_TRY(vox_sync(ctx));
+            const double dt =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (r > 0) t_best = std::min(t_best, dt);
+        }
+        if (t_best < best) {
+            best = t_best;
+            bi = c;
+        }
+    }
+    // Every shape writes the same nodes, so nodes_dev already holds them.
+    VOX_TRY(vox_set_brick_dims(ctx, fcc ? 1 : 0, cand[bi]));
+    if (best_dims)
+        for (int a = 0; a < 3; a++) best_dims[a] = cand[bi][a];
+    if (best_ms) *best_ms = (float)(best * 1e3);
     return VOX_OK;
 }
 

@@ -44,17 +44,27 @@ __device__ __forceinline__ RowBlock row_block(uint32_t L, uint32_t nxc, uint32_t
     return r;
 }
 
-// Cartesian bricks (node pass + surface) and FCC per-grid bricks.
+// Surface bricks: default shapes of the Cartesian grid and of each FCC
+// sub-grid; the shape is a run-time choice (the autotuner picks among
+// power-of-two shapes with x a multiple of 64, at most 32 rows and at most
+// kMaxBrickVox voxels).
 constexpr int kCartBX = 128, kCartBY = 4, kCartBZ = 8;
 constexpr int kFccBX = 64, kFccBY = 4, kFccBZ = 4;
 constexpr int kMaxBrickVox = 4096;
+
+// Brick geometry as the node passes need it: bricks per row/column and the
+// log2 of the brick's words per row (x), rows (y) and planes (z).
+struct BrickDims {
+    uint32_t nbx, nby;
+    uint32_t lwx, lby, lbz;
+};
 // z planes per thread in the erosion pass (a block covers 64 words x 4 rows
 // x kErodeZ planes).
 constexpr uint32_t kErodeZ = 8;
 // z planes per thread in the Cartesian node pass; divides kCartBZ so a
 // thread's planes share one surface brick.
 constexpr uint32_t kNodeZ = 4;
-static_assert(kCartBZ % kNodeZ == 0, "node planes must not straddle a brick");
+static_assert(kCartBZ % kNodeZ == 0, "node planes must not straddle a brick");  // z >= 4
 // Material-list arenas of the surface stage: kMatSlots counters, each on its
 // own 128-byte line (kSlotStride words): {bytes used, surface voxels}.
 constexpr uint32_t kMatSlots = 64, kSlotStride = 32;
@@ -251,11 +261,11 @@ struct MatSource {
 __global__ void k_nodes_cart(GridDims g, const uint32_t *__restrict__ solid,
                              const uint32_t *__restrict__ pre, const uint8_t *__restrict__ ystart,
                              const uint8_t *__restrict__ zstart, MatSource ms, int use_mats,
-                             int kind, void *__restrict__ nodes, uint32_t nbx, uint32_t nby);
+                             int kind, void *__restrict__ nodes, BrickDims bd);
 __global__ void k_nodes_fcc(GridDims g, const uint32_t *s1, const uint32_t *s2,
                             const uint32_t *s3, const uint32_t *s4, MatSource m1, MatSource m2,
                             MatSource m3, MatSource m4, int use_mats, int kind,
-                            void *__restrict__ nodes, uint32_t nbx, uint32_t nby);
+                            void *__restrict__ nodes, BrickDims bd);
 
 // Cost model kernels (vox_cost.hip).
 __global__ void k_brick_features(const uint32_t *__restrict__ order,

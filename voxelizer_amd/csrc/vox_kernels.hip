@@ -1054,8 +1054,7 @@ __global__ __launch_bounds__(256) void k_nodes_cart(GridDims g, const uint32_t *
                                                     const uint8_t *__restrict__ ystart,
                                                     const uint8_t *__restrict__ zstart,
                                                     MatSource ms, int use_mats, int kind,
-                                                    void *__restrict__ nodes, uint32_t nbx,
-                                                    uint32_t nby) {
+                                                    void *__restrict__ nodes, BrickDims bd) {
     __shared__ uint8_t lut[64];
     __shared__ uint4 slot[256][4];  // 64 output bytes per thread (LongNode; ShortNode uses 32)
     const uint32_t nxc = (g.W + kWave - 1) / kWave, ny4 = (g.Y + 3) / 4;
@@ -1123,18 +1122,21 @@ __global__ __launch_bounds__(256) void k_nodes_cart(GridDims g, const uint32_t *
         S[k] = 0u;
         wr[k] = 0u;
     }
-    if (use_mats) {  // uniform; the kNodeZ planes share one brick (kNodeZ | kCartBZ)
+    if (use_mats) {  // uniform; the kNodeZ planes share one brick (kNodeZ divides its depth)
         const uint32_t yy = valid ? y : 0u, ww = valid ? w : 0u;
-        const uint32_t b = ((zb / kCartBZ) * nby + yy / kCartBY) * nbx + ww / (kCartBX / 32);
+        const uint32_t b =
+            ((zb >> bd.lbz) * bd.nby + (yy >> bd.lby)) * bd.nbx + (ww >> bd.lwx);
         mb = ms.brick_mat_off[b];
+        const uint32_t bwords = 1u << (bd.lwx + bd.lby + bd.lbz);
 #pragma unroll
         for (uint32_t k = 0; k < kNodeZ; k++) {
             const uint32_t z = zb + k;
             const bool ok = valid && z < g.Z;
-            const uint32_t wb = (((z % kCartBZ) * kCartBY + yy % kCartBY) * (kCartBX / 32) +
-                                 ww % (kCartBX / 32));
+            const uint32_t wb =
+                ((((z & ((1u << bd.lbz) - 1u)) << bd.lby) + (yy & ((1u << bd.lby) - 1u)))
+                 << bd.lwx) + (ww & ((1u << bd.lwx) - 1u));
             S[k] = ms.surf[ok ? (size_t)z * planeW + col0 : 0];
-            wr[k] = ms.wrank[(size_t)b * (kCartBX * kCartBY * kCartBZ / 32) + (ok ? wb : 0)];
+            wr[k] = ms.wrank[(size_t)b * bwords + (ok ? wb : 0)];
             if (!ok) S[k] = 0u;
         }
     }
@@ -1222,7 +1224,7 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(GridDims g, const uint32_t *s
                                                    const uint32_t *s4, MatSource m1, MatSource m2,
                                                    MatSource m3, MatSource m4, int use_mats,
                                                    int kind, void *__restrict__ nodes,
-                                                   uint32_t nbx, uint32_t nby) {
+                                                   BrickDims bd) {
     const uint32_t FW = 2 * g.W, FY = g.Y;
     const uint32_t nxc = (FW + kWave - 1) / kWave, ny4 = (FY + 3) / 4;
     const uint32_t L = xcd_logical_block(nxc * ny4 * 2 * g.Z);
@@ -1284,10 +1286,12 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(GridDims g, const uint32_t *s
     if (use_mats) {
         const int odd = (int)(fz & 1);
         const uint64_t gw = (uint64_t)(fw >> 1), gz = (uint64_t)(fz >> 1);
-        const uint32_t b = (uint32_t)(((gz / kFccBZ) * nby + (uint64_t)y / kFccBY) * nbx +
-                                      gw / (kFccBX / 32));
-        const uint32_t wb = (uint32_t)(((gz % kFccBZ) * kFccBY + (uint64_t)y % kFccBY) *
-                                           (kFccBX / 32) + gw % (kFccBX / 32));
+        const uint32_t b = (uint32_t)((((gz >> bd.lbz) * bd.nby + ((uint64_t)y >> bd.lby)) *
+                                       bd.nbx) + (gw >> bd.lwx));
+        const uint32_t wb = (uint32_t)(((((gz & ((1u << bd.lbz) - 1u)) << bd.lby) +
+                                         ((uint64_t)y & ((1u << bd.lby) - 1u)))
+                                        << bd.lwx) + (gw & ((1u << bd.lwx) - 1u)));
+        const uint32_t bwords = 1u << (bd.lwx + bd.lby + bd.lbz);
         const size_t gwi = (gz * g.Y + (uint64_t)y) * g.W + gw;
         const int sh = (fw & 1) ? 0 : 16;
         for (int par = 0; par < 2; par++) {
@@ -1298,7 +1302,7 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(GridDims g, const uint32_t *s
             const uint32_t sw = src.surf[gwi];
             uint32_t h = (sw >> sh) & 0xFFFFu;
             if (!h) continue;
-            uint32_t r = mb + src.wrank[(size_t)b * (kFccBX * kFccBY * kFccBZ / 32) + wb] +
+            uint32_t r = mb + src.wrank[(size_t)b * bwords + wb] +
                          ((fw & 1) ? (uint32_t)__popc(sw >> 16) : 0u);
             while (h) {
                 const int i = __clz(h) - 16;  // voxel i of the 16
