@@ -464,11 +464,17 @@ struct SurfTri {
     int32_t s[3];              // LDS key strides of (u, v, a)
     int32_t base;              // LDS key index of (lo_u, lo_v, lo_a)
     int32_t wu;                // columns along u
-    float rwu;                 // 1 / wu
+    int32_t ngu;               // work items along u (kSurfCols columns each)
+    float rngu;                // 1 / ngu
     uint32_t key;              // bbType<<30 | triangle
     uint32_t flags;            // bb | axis<<2 | (plane0 other is v)<<4 | (plane1 other is v)<<5
 };
 constexpr int kSurfChunk = 64;
+// Columns (or 1D voxels) per work item.  Grouping neighbouring columns
+// shares the item's lookup and v terms but lowers the parallelism the LDS
+// latency needs: on the large hall 1 column is fastest (2 and 4 measured
+// 9% and 13% slower).
+constexpr int kSurfCols = 1;
 
 
 // Sets up triangle t for the brick whose alloc-coordinate box starts at
@@ -509,11 +515,12 @@ __device__ __forceinline__ uint32_t surf_tri_setup(const SurfParams &p,
     st.vmax_a = pick3i(vb.max, a);
     st.key = ((uint32_t)bb << 30) | t;
     st.wu = hi[0] - lo[0] + 1;
-    st.rwu = 1.0f / (float)st.wu;
+    st.ngu = (st.wu + kSurfCols - 1) / kSurfCols;
+    st.rngu = 1.0f / (float)st.ngu;
     uint32_t flags = (uint32_t)bb | ((uint32_t)a << 2);
     if (bb == BB_1D) {
         st.flags = flags;
-        return (uint32_t)(hi[2] - lo[2] + 1);
+        return (uint32_t)((hi[2] - lo[2] + kSurfCols) / kSurfCols);
     }
     const F3 n = fcross(fsub(tri[0], tri[2]), fsub(tri[1], tri[0]));
     st.n[0] = n.x; st.n[1] = n.y; st.n[2] = n.z;
@@ -550,7 +557,7 @@ __device__ __forceinline__ uint32_t surf_tri_setup(const SurfParams &p,
         }
     }
     st.flags = flags;
-    return (uint32_t)(st.wu * (hi[1] - lo[1] + 1));
+    return (uint32_t)(st.ngu * (hi[1] - lo[1] + 1));
 }
 
 __device__ __forceinline__ void key_write(uint32_t *keys, uint8_t *touched, int32_t idx,
@@ -570,65 +577,85 @@ __device__ __forceinline__ void surf_item(const SurfParams &p, const SplitTable 
     const uint32_t flags = st.flags;
     const int bb = (int)(flags & 3u), a = (int)((flags >> 2) & 3u);
     if (bb == BB_1D) {
-        key_write(keys, touched, st.base + (int32_t)k * st.s[2], nvox, row_shift, st.key);
-        return;
-    }
-    // column (u, v) of the item: k = (v - lo_v) * wu + (u - lo_u)
-    int32_t q = (int32_t)((float)k * st.rwu);
-    int32_t r = (int32_t)k - q * st.wu;
-    if (r < 0) { q--; r += st.wu; }
-    if (r >= st.wu) { q++; r -= st.wu; }
-    const int32_t u = st.lo[0] + r, v = st.lo[1] + q;
-    const int ua = axis_u(a), va = axis_v(a);
-    const float mu = pick3(p.m, ua), mv = pick3(p.m, va), ma = pick3(p.m, a);
-    const float pu = vox_corner(u, mu, p.d), pv = vox_corner(v, mv, p.d);
-    bool in = true;
-#pragma unroll
-    for (int e = 0; e < 3; e++) in &= (st.dom[e][0] * pu + st.dom[e][1] * pv) + st.dom[e][2] >= 0.0f;
-    if (!in) return;
-    const int32_t col = st.base + r * st.s[0] + q * st.s[1];
-    if (bb == BB_2D) {
-        key_write(keys, touched, col, nvox, row_shift, st.key);
-        return;
-    }
-    const F3 pp = {a == AX_X ? 0.0f : pu, a == AX_Y ? 0.0f : (a == AX_X ? pu : pv),
-                   a == AX_Z ? 0.0f : pv};
-    int32_t lo, hi;
-    depth_range(F3{st.t0[0], st.t0[1], st.t0[2]}, pp, F3{st.n[0], st.n[1], st.n[2]}, p.m, p.d, a,
-                lo, hi);
-    const int32_t ilo = max(lo, st.lo[2]), ihi = min(hi, st.hi[2]);
-    if (ilo > ihi) return;
-    // Constant parts of the two side-plane edge functions along the column.
-    float kc[2][3];
-#pragma unroll
-    for (int q2 = 0; q2 < 2; q2++) {
-        const float po = (flags >> (4 + q2)) & 1u ? pv : pu;
-#pragma unroll
-        for (int e = 0; e < 3; e++) kc[q2][e] = st.pl[q2][e][1] * po;
-    }
-    // matSplitRes interval of the first voxel along a, advanced as i grows.
-    const uint32_t ns = spl.n[a];
-    uint32_t si = 0;
-    while (si + 1 < ns && (uint32_t)ilo >= spl.hi[a][si]) si++;
-    for (int32_t i = ilo; i <= ihi; i++) {
-        while (si + 1 < ns && (uint32_t)i >= spl.hi[a][si]) si++;
-        const int32_t slo = (int32_t)spl.lo[a][si], shi = (int32_t)spl.hi[a][si];
-        const int32_t bl = max(st.vmin_a, slo), bh = min(st.vmax_a, shi - 1);
-        const int32_t rl = lo < bl ? bl : lo;
-        const int32_t rh = hi > bh ? bh : hi;
-        bool write = rl == rh;
-        if (!write) {
-            const float pa = vox_corner(i, ma, p.d);
-            bool ok = true;
-#pragma unroll
-            for (int q2 = 0; q2 < 2; q2++)
-#pragma unroll
-                for (int e = 0; e < 3; e++)
-                    ok &= (st.pl[q2][e][0] * pa + kc[q2][e]) + st.pl[q2][e][2] >= 0.0f;
-            write = ok;
+        const int32_t len = st.hi[2] - st.lo[2] + 1;
+        for (int j = 0; j < kSurfCols; j++) {
+            const int32_t kk = (int32_t)k * kSurfCols + j;
+            if (kk >= len) break;
+            key_write(keys, touched, st.base + kk * st.s[2], nvox, row_shift, st.key);
         }
-        if (write)
-            key_write(keys, touched, col + (i - st.lo[2]) * st.s[2], nvox, row_shift, st.key);
+        return;
+    }
+    // item k covers columns u0 .. u0+kSurfCols-1 of row v:
+    // k = (v - lo_v) * ngu + (u0 - lo_u) / kSurfCols
+    int32_t q = (int32_t)((float)k * st.rngu);
+    int32_t r = (int32_t)k - q * st.ngu;
+    if (r < 0) { q--; r += st.ngu; }
+    if (r >= st.ngu) { q++; r -= st.ngu; }
+    const int32_t v = st.lo[1] + q;
+    const int ua = axis_u(a), va = axis_v(a);
+    const float mx = p.m.x, my = p.m.y, mz = p.m.z;
+    const float mu = ua == AX_X ? mx : (ua == AX_Y ? my : mz);
+    const float mv = va == AX_X ? mx : (va == AX_Y ? my : mz);
+    const float ma = a == AX_X ? mx : (a == AX_Y ? my : mz);
+    const float pv = vox_corner(v, mv, p.d);
+    float kdom[3];  // v terms of the dominant-plane edge functions
+#pragma unroll
+    for (int e = 0; e < 3; e++) kdom[e] = st.dom[e][1] * pv;
+    const bool threeD = bb == BB_3D;
+    const F3 t0 = {st.t0[0], st.t0[1], st.t0[2]}, nn = {st.n[0], st.n[1], st.n[2]};
+    const uint32_t ns = spl.n[a];
+    for (int j = 0; j < kSurfCols; j++) {
+        const int32_t cu = r * kSurfCols + j;  // column offset along u
+        if (cu >= st.wu) break;
+        const int32_t u = st.lo[0] + cu;
+        const float pu = vox_corner(u, mu, p.d);
+        bool in = true;
+#pragma unroll
+        for (int e = 0; e < 3; e++) in &= (st.dom[e][0] * pu + kdom[e]) + st.dom[e][2] >= 0.0f;
+        if (!in) continue;
+        const int32_t col = st.base + cu * st.s[0] + q * st.s[1];
+        if (!threeD) {
+            key_write(keys, touched, col, nvox, row_shift, st.key);
+            continue;
+        }
+        const F3 pp = {a == AX_X ? 0.0f : pu, a == AX_Y ? 0.0f : (a == AX_X ? pu : pv),
+                       a == AX_Z ? 0.0f : pv};
+        int32_t lo, hi;
+        depth_range(t0, pp, nn, p.m, p.d, a, lo, hi);
+        const int32_t ilo = max(lo, st.lo[2]), ihi = min(hi, st.hi[2]);
+        if (ilo > ihi) continue;
+        // Constant parts of the two side-plane edge functions along the column.
+        float kc[2][3];
+#pragma unroll
+        for (int q2 = 0; q2 < 2; q2++) {
+            const float po = (flags >> (4 + q2)) & 1u ? pv : pu;
+#pragma unroll
+            for (int e = 0; e < 3; e++) kc[q2][e] = st.pl[q2][e][1] * po;
+        }
+        // matSplitRes interval of the first voxel along a, advanced as i grows.
+        uint32_t si = 0;
+        while (si + 1 < ns && (uint32_t)ilo >= spl.hi[a][si]) si++;
+        for (int32_t i = ilo; i <= ihi; i++) {
+            while (si + 1 < ns && (uint32_t)i >= spl.hi[a][si]) si++;
+            const int32_t slo = (int32_t)spl.lo[a][si], shi = (int32_t)spl.hi[a][si];
+            const int32_t bl = max(st.vmin_a, slo), bh = min(st.vmax_a, shi - 1);
+            const int32_t rl = lo < bl ? bl : lo;
+            const int32_t rh = hi > bh ? bh : hi;
+            bool write = rl == rh;
+            if (!write) {
+                const float pa = vox_corner(i, ma, p.d);
+                bool ok = true;
+#pragma unroll
+                for (int q2 = 0; q2 < 2; q2++)
+#pragma unroll
+                    for (int e = 0; e < 3; e++)
+                        ok &= (st.pl[q2][e][0] * pa + kc[q2][e]) + st.pl[q2][e][2] >= 0.0f;
+                write = ok;
+            }
+            if (write)
+                key_write(keys, touched, col + (i - st.lo[2]) * st.s[2], nvox, row_shift,
+                          st.key);
+        }
     }
 }
 
