@@ -19,10 +19,12 @@ Workloads:
 Schedules: "plain" processes surface bricks in index order (and, for a batch,
 the meshes one after another on one stream); "learned" orders the bricks by
 the MLP cost model, trained first by the library's own GPU kernels on
-per-brick durations recorded on a differently seeded mesh (for a batch:
-meshes on 16 worker streams, longest first).  Both are timed; `value` is the
-schedule named by --schedule (default learned), the other is reported beside
-it.
+per-brick durations recorded on a differently seeded mesh (for a batch: all
+meshes fused into each stage's launches, bricks of all meshes in cost
+order).  Both are timed; `value` is the schedule named by --schedule (default
+learned), the other is reported beside it.  A batch also reports the fused
+run with the plain brick order (the model's own share) and 16 worker
+streams.
 
 Prints ONE JSON line.  `value` = allocated nodes per step / wall time per step
 (Gvoxels/s), timed over --steps steps between two device synchronisations.
@@ -308,8 +310,13 @@ def bench_batch(args, va, wl):
         it.nodes_dev = out.ptr.value
         n_nodes += nn
         out_bytes += nn * node_bytes(kind)
+    # Modes: (brick order, streams); streams 0 = fused (each stage one launch
+    # over all meshes), 1 = one mesh after another, k = k worker streams.
+    modes = {"plain": (va.SCHEDULE_PLAIN, 1), "fused_plain": (va.SCHEDULE_PLAIN, 0),
+             "learned": (va.SCHEDULE_LEARNED, 0),
+             f"streams{BATCH_STREAMS}_learned": (va.SCHEDULE_LEARNED, BATCH_STREAMS)}
     results = {}
-    for sched, streams in ((va.SCHEDULE_PLAIN, 1), (va.SCHEDULE_LEARNED, BATCH_STREAMS)):
+    for name, (sched, streams) in modes.items():
         for _ in range(args.warmup):
             ctx.voxelize_batch(items, sched, streams)
         ctx.sync()
@@ -317,11 +324,16 @@ def bench_batch(args, va, wl):
         for _ in range(args.steps):
             ctx.voxelize_batch(items, sched, streams)
         ctx.sync()
-        results[sched] = time.perf_counter() - t0
-    chosen = va.SCHEDULE_LEARNED if args.schedule == "learned" else va.SCHEDULE_PLAIN
-    elapsed = results[chosen]
-    plain_v = n_nodes / (results[va.SCHEDULE_PLAIN] / args.steps) / 1e9
-    learned_v = n_nodes / (results[va.SCHEDULE_LEARNED] / args.steps) / 1e9
+        results[name] = time.perf_counter() - t0
+        print(f"# batch {name}: {results[name] * 1e3 / args.steps:.2f} ms/step", file=sys.stderr,
+              flush=True)
+    ctx.enable_timing(True)
+    ctx.voxelize_batch(items, *modes["learned"])
+    stages = ctx.stage_times()
+    ctx.enable_timing(False)
+    elapsed = results["learned" if args.schedule == "learned" else "plain"]
+    gv = {k: n_nodes / (v / args.steps) / 1e9 for k, v in results.items()}
+    plain_v, learned_v = gv["plain"], gv["learned"]
     value = n_nodes / (elapsed / args.steps) / 1e9
     ms_per_step = elapsed * 1e3 / args.steps
     pipe_bytes = out_bytes + sum(12 * m.nv + 13 * m.nt for m in meshes)
@@ -334,10 +346,14 @@ def bench_batch(args, va, wl):
         "data": "synthetic (seeded procedural meshes)",
         "config": {"workload": label, "schedule": args.schedule, "meshes": len(meshes),
                    "nodes": n_nodes, "triangles": sum(m.nt for m in meshes),
-                   "streams": BATCH_STREAMS, "parallelism": "single GPU"},
+                   "execution": "fused (each stage one launch over all meshes)"
+                   if args.schedule == "learned" else "one mesh after another",
+                   "parallelism": "single GPU"},
         "schedules": {"plain_gvox_s": round(plain_v, 4), "learned_gvox_s": round(learned_v, 4),
                       "learned_over_plain": round(learned_v / plain_v, 4),
-                      "cost_model": training},
+                      "modes_gvox_s": {k: round(v, 4) for k, v in gv.items()},
+                      "learned_over_fused_plain": round(learned_v / gv["fused_plain"], 4),
+                      "learned_stages_ms": stages, "cost_model": training},
         "roofline": {"bound": "hbm", "achieved": round(pipe_bytes / (ms_per_step * 1e-3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(pipe_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

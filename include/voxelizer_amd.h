@@ -238,13 +238,20 @@ vox_status vox_autotune_bricks(vox_ctx *ctx, const vox_grid *grid, vox_node_kind
 /* ---- batches of meshes (no counterpart in the reference) ---------------
  * Many meshes voxelized in one call, each into its own node array.  Inputs
  * are device arrays; the grid of each item is planned by the caller
- * (vox_plan_grid / vox_plan_grid_host).  PLAIN runs the items one after
- * another on the context's stream, as separate vox_voxelize_nodes calls
- * would.  LEARNED runs them on `streams` worker streams at once (one host
- * thread and one internal context each), taking the items in decreasing
- * order of estimated work (node bytes + 64 B per triangle: longest first),
- * and with the learned brick order inside each item when a cost model is
- * set.  Outputs are identical under both. */
+ * (vox_plan_grid / vox_plan_grid_host).  `streams` picks the execution:
+ *   0  fused: every stage is one launch over all items (a per-item job table
+ *      on the device; an FCC lattice counts as four grids), so a batch costs
+ *      about as many launches and host round trips as one mesh.  Items whose
+ *      Cartesian grid has several voxSplitRes y/z erosion subspaces run
+ *      afterwards one by one;
+ *   1  one item after another on the context's stream, as separate
+ *      vox_voxelize_nodes calls would;
+ *   k  k worker streams at once (one host thread and one internal context
+ *      each, at most 64), items in decreasing order of estimated work (node
+ *      bytes + 64 B per triangle: longest first).
+ * `schedule` is the brick order of the surface pass (LEARNED: cost-model
+ * order when a model is set).  Outputs are identical under every choice;
+ * on return each item's status is set. */
 typedef struct vox_batch_item {
     const float *vertices_dev;
     uint32_t nv;

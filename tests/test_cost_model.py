@@ -157,7 +157,7 @@ def test_learned_schedule_output_identical(gpu_ctx, oracle, kind):
     assert np.array_equal(learned, oracle.voxelize_nodes(m, og, kind, True))
 
 
-def make_batch(ctx, meshes, kinds, keep):
+def make_batch(ctx, meshes, kinds, keep, no_mats=(), vox_split=None):
     """Device copies of the meshes, host-planned grids, output buffers."""
     items = (va.BatchItem * len(meshes))()
     outs = []
@@ -172,10 +172,11 @@ def make_batch(ctx, meshes, kinds, keep):
         it = items[i]
         it.vertices_dev, it.nv = bufs[0].ptr.value, m.nv
         it.indices_dev, it.nt = bufs[1].ptr.value, m.nt
-        it.materials_dev, it.n_unique = bufs[2].ptr.value, int(m.materials.max()) + 1
+        if i not in no_mats:
+            it.materials_dev, it.n_unique = bufs[2].ptr.value, int(m.materials.max()) + 1
         it.kind = kind
         it.grid = g
-        it.vox_split[:] = (1024, 1024, 1024)
+        it.vox_split[:] = vox_split[i] if vox_split and vox_split[i] else (1024, 1024, 1024)
         it.mat_split[:] = (512, 512, 512)
         it.nodes_dev = out.ptr.value
         outs.append((out, nb, g))
@@ -183,23 +184,35 @@ def make_batch(ctx, meshes, kinds, keep):
 
 
 @pytest.mark.gpu
-def test_batch_schedules_identical_and_match_oracle(gpu_ctx, oracle):
+def test_batch_modes_identical_and_match_oracle(gpu_ctx, oracle):
+    """Sequential, fused (plain and learned brick order) and worker-stream
+    batches give the same nodes, equal to the oracle's, over a mix of node
+    kinds (Cartesian and FCC), an item without materials and an item whose
+    voxSplitRes splits its erosion into subspaces (run alone when fused)."""
     meshes = W.batch_meshes(n=12, seed=7, tri_lo=200, tri_hi=20000)
     for m in meshes:
         m.meta["cube_length"] *= 2.0  # keep the oracle quick
     kinds = [va.NODE_SHORT if int(m.materials.max()) <= 7 else va.NODE_LONG for m in meshes]
+    kinds[3] = va.NODE_SHORT_FCC
+    kinds[8] = va.NODE_LONG_FCC
+    no_mats = {5}
+    vsplit = [None] * len(meshes)
+    vsplit[6] = (1024, 24, 20)
     keep = []
-    items, outs = make_batch(gpu_ctx, meshes, kinds, keep)
+    items, outs = make_batch(gpu_ctx, meshes, kinds, keep, no_mats, vsplit)
     gpu_ctx.voxelize_batch(items, va.SCHEDULE_PLAIN, 1)
     plain = [o.download(np.uint8, nb) for o, nb, _ in outs]
-    for o, nb, _ in outs:
-        o.upload(np.zeros(nb, np.uint8))
-    gpu_ctx.voxelize_batch(items, va.SCHEDULE_LEARNED, 4)
-    for i, ((o, nb, g), m, kind) in enumerate(zip(outs, meshes, kinds)):
-        got = o.download(np.uint8, nb)
-        assert items[i].status == 0
-        assert np.array_equal(got, plain[i]), i
+    for i, (m, kind) in enumerate(zip(meshes, kinds)):
+        vs = vsplit[i] or (1024, 1024, 1024)
         ref = oracle.voxelize_nodes(m, oracle.plan_grid(m.vertices, va.PLAN_CUBE_LENGTH,
-                                                        m.meta["cube_length"], 1024), kind, True,
-                                    vox_split=(1024, 1024, 1024))
-        assert np.array_equal(got, ref), i
+                                                        m.meta["cube_length"], 1024), kind,
+                                    i not in no_mats, vox_split=vs)
+        assert np.array_equal(plain[i], ref), i
+    for schedule, streams in [(va.SCHEDULE_PLAIN, 0), (va.SCHEDULE_LEARNED, 0),
+                              (va.SCHEDULE_LEARNED, 4)]:
+        for o, nb, _ in outs:
+            o.upload(np.zeros(nb, np.uint8))
+        gpu_ctx.voxelize_batch(items, schedule, streams)
+        for i, (o, nb, _) in enumerate(outs):
+            assert items[i].status == 0
+            assert np.array_equal(o.download(np.uint8, nb), plain[i]), (schedule, streams, i)

@@ -308,7 +308,9 @@ class Context:
         self._check(lib().vox_set_schedule(self.h, schedule))
 
     def voxelize_batch(self, items, schedule: int = 0, streams: int = 1):
-        """items: a ctypes array of BatchItem (device pointers filled in)."""
+        """items: a ctypes array of BatchItem (device pointers filled in).
+        streams: 0 fused (each stage one launch over all items), 1 one item
+        after another, k > 1 k worker streams."""
         self._check(lib().vox_voxelize_batch(self.h, items, len(items), schedule, streams))
 
     def set_brick_dims(self, dims, fcc: bool = False):

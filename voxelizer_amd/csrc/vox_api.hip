@@ -91,8 +91,13 @@ struct vox_ctx {
     std::vector<float> rec_feats, rec_cycles;
     // worker contexts of batched runs (same device, own streams and buffers)
     std::vector<vox_ctx *> workers;
-    // per-triangle vertex records of the current call (k_tri_records)
+    // per-triangle-slot vertex records and job ids of the current call
     float4 *rec = nullptr;
+    uint32_t *tri_job = nullptr;
+    // pinned staging area of job-table uploads, and the event of the last one
+    void *up = nullptr;
+    size_t up_cap = 0;
+    hipEvent_t up_ev = nullptr;
     // surface brick shapes (voxels along x, y, z)
     int cart_bdim[3] = {kCartBX, kCartBY, kCartBZ};
     int fcc_bdim[3] = {kFccBX, kFccBY, kFccBZ};
@@ -141,12 +146,6 @@ vox_status buf(vox_ctx *ctx, const char *name, size_t count, T **out) {
     }
     *out = (T *)b.p;
     return VOX_OK;
-}
-
-// Row-major launches: block (64, 4) covers 4 rows of up to 64 words; a 1D
-// grid padded to a multiple of the XCD count (see xcd_logical_block).
-inline dim3 row_grid(uint32_t W, uint32_t Y, uint32_t Z) {
-    return dim3(xcd_grid(((W + kWave - 1) / kWave) * ((Y + 3) / 4) * Z));
 }
 
 inline uint32_t grid_for(size_t n, uint32_t threads, uint32_t cap = 16384) {
@@ -220,15 +219,209 @@ bool brick_ok(const uint32_t b[3], bool cart) {
            (!cart || b[2] % kNodeZ == 0);
 }
 
-// Per-triangle vertex records of the current mesh, read by the solid and
-// surface stages instead of the indexed vertex arrays.
-vox_status make_records(vox_ctx *ctx) {
-    VOX_TRY(buf(ctx, "tri_rec", (size_t)std::max<uint32_t>(ctx->nt, 1) * 3, &ctx->rec));
-    if (ctx->nt) {
-        k_tri_records<<<grid_for(ctx->nt, 256), 256, 0, ctx->stream>>>(ctx->V, ctx->I, ctx->nt,
-                                                                        ctx->rec);
-        VOX_HIP(hipGetLastError());
+// ---------------------------------------------------------------------------
+// Job tables.  Every call describes its grids as jobs (vox_kernels.h: Job):
+// a Cartesian mesh is one job, an FCC lattice four (the shifted sub-grids).
+// The per-pass item counts are summed into base arrays (njobs + 1 entries,
+// the last one the total), so each stage is one launch over all jobs.
+// ---------------------------------------------------------------------------
+struct Plan {
+    std::vector<Job> jobs;
+    std::vector<SplitTable> splits;  // one per job
+    std::vector<uint32_t> tri_b{0}, tile_b{0}, pad_b{0}, brick_b{0}, erode_b{0}, node_b{0};
+    std::vector<uint32_t> fcc_first, fcc_b{0};
+    std::vector<size_t> bits_off, surf_off, wrank_off;  // into the call's arenas
+    std::vector<uint32_t *> ext_bits;                   // caller's bit array or null
+    size_t bits_words = 0, surf_words = 0, wrank_n = 0;
+    uint32_t maxW = 0;
+    bool any_mats = false;
+};
+
+// Device copies of a plan's tables.
+struct PlanDev {
+    Job *jobs;
+    uint32_t *tri_b, *tile_b, *pad_b, *brick_b, *erode_b, *node_b, *fcc_first, *fcc_b;
+};
+
+// matSplitRes subspaces over extPartition (src/host_side.cpp:1107-1111).
+vox_status split_table(vox_ctx *ctx, const vox_grid &g, const uint32_t matSplit[3], SplitTable &st) {
+    std::memset(&st, 0, sizeof(st));
+    for (int a = 0; a < 3; a++) {
+        std::vector<Span> sp = split_axis_max(g.res[a], matSplit[a]);
+        if (sp.empty() || sp.size() > (size_t)kMaxSplits)
+            return fail(ctx, VOX_ERR_UNSUPPORTED, "matSplitRes gives too many subspaces");
+        st.n[a] = (uint32_t)sp.size();
+        for (size_t k = 0; k < sp.size(); k++) {
+            st.lo[a][k] = sp[k].min;
+            st.hi[a][k] = sp[k].max;
+        }
     }
+    return VOX_OK;
+}
+
+// Adds one grid; shift is the FCC sub-grid offset (zero for Cartesian).
+// cart: the job gets erosion and Cartesian node blocks.
+vox_status add_job(vox_ctx *ctx, Plan &P, const float *V, const uint32_t *I, uint32_t nt,
+                   const uint8_t *mats, const vox_grid &g, const double shift[3], bool use_mats,
+                   const uint32_t ms[3], const int bdim[3], bool cart, void *nodes, int kind,
+                   uint32_t *ext_bits) {
+    const uint32_t X = g.alloc[0], W = X / 32;
+    if (W == 0 || (size_t)16 * W * 4 > 65536)
+        return fail(ctx, VOX_ERR_UNSUPPORTED, "x resolution beyond the fill kernel's LDS row buffer");
+    if (g.x_splits == 0 || X / g.x_splits > 1024)
+        return fail(ctx, VOX_ERR_UNSUPPORTED,
+                    "voxSplitRes.x gives x splits wider than 1024 voxels, where the reference's "
+                    "word bitmask is undefined");
+    if ((uint64_t)P.tri_b.back() + nt + 256 >= (1ull << 30))
+        return fail(ctx, VOX_ERR_UNSUPPORTED, "more than 2^30 triangles in one call");
+    Job J;
+    std::memset(&J, 0, sizeof(J));
+    J.V = V;
+    J.I = I;
+    J.mats = mats;
+    J.nt = nt;
+    J.tri_base = P.tri_b.back();
+    P.tri_b.push_back(J.tri_base + (nt + 255u) / 256u * 256u);
+    J.gd = GridDims{g.alloc[0], g.alloc[1], g.alloc[2], W};
+    for (int a = 0; a < 3; a++) {
+        J.res[a] = (int32_t)g.res[a];
+        J.em[a] = g.min_vertex[a] + shift[a];
+    }
+    J.d = g.voxel_length;
+    J.tilesY = (g.res[1] + 3) / 4;
+    J.tilesZ = (g.res[2] + 3) / 4;
+    J.tile_base = P.tile_b.back();
+    P.tile_b.push_back(J.tile_base + J.tilesY * J.tilesZ);
+    J.pad_base = P.pad_b.back();
+    P.pad_b.push_back(J.pad_base + (pad_words(W, g.alloc[1], g.alloc[2]) + 255u) / 256u);
+    J.m = F3{(float)J.em[0], (float)J.em[1], (float)J.em[2]};
+    J.fd = (float)g.voxel_length;
+    uint32_t nbt = 1;
+    for (int a = 0; a < 3; a++) {
+        J.bdim[a] = bdim[a];
+        J.nb[a] = (g.alloc[a] + bdim[a] - 1) / bdim[a];
+        nbt *= J.nb[a];
+    }
+    J.use_mats = use_mats ? 1 : 0;
+    J.brick_base = P.brick_b.back();
+    P.brick_b.push_back(J.brick_base + (use_mats ? nbt : 0u));
+    J.bdn = brick_dims(g, bdim);
+    SplitTable st;
+    VOX_TRY(split_table(ctx, g, ms, st));
+    P.splits.push_back(st);
+    J.erode_base = P.erode_b.back();
+    J.node_base = P.node_b.back();
+    P.erode_b.push_back(J.erode_base + (cart ? erode_blocks(J.gd) : 0u));
+    P.node_b.push_back(J.node_base + (cart ? node_blocks(J.gd) : 0u));
+    J.nodes = nodes;
+    J.kind = kind;
+    const size_t words = (size_t)W * g.alloc[1] * g.alloc[2];
+    P.ext_bits.push_back(ext_bits);
+    P.bits_off.push_back(P.bits_words);
+    if (!ext_bits) P.bits_words += words;
+    P.surf_off.push_back(P.surf_words);
+    P.wrank_off.push_back(P.wrank_n);
+    if (use_mats) {
+        P.surf_words += words;
+        P.wrank_n += (size_t)nbt * (size_t)(bdim[0] * bdim[1] * bdim[2] / 32);
+        P.any_mats = true;
+    }
+    P.maxW = std::max(P.maxW, W);
+    P.jobs.push_back(J);
+    return VOX_OK;
+}
+
+// A Cartesian mesh (one job) or an FCC lattice (four jobs).
+vox_status add_mesh(vox_ctx *ctx, Plan &P, const float *V, const uint32_t *I, uint32_t nt,
+                    const uint8_t *mats, const vox_grid &g, vox_node_kind kind, bool use_mats,
+                    const uint32_t ms[3], void *nodes) {
+    const double zero[3] = {0.0, 0.0, 0.0};
+    if (kind == VOX_NODE_SHORT || kind == VOX_NODE_LONG)
+        return add_job(ctx, P, V, I, nt, mats, g, zero, use_mats, ms, ctx->cart_bdim, true, nodes,
+                       kind == VOX_NODE_SHORT ? 0 : 1, nullptr);
+    // fccWorker (src/host_side.cpp:1262-1698): four sub-grids shifted by
+    // (0,0,0), (h,h,0), (0,h,h), (h,0,h), h = d/2 (1436-1463).
+    const double h = g.voxel_length / 2.0;
+    const double shifts[4][3] = {{0.0, 0.0, 0.0}, {h, h, 0.0}, {0.0, h, h}, {h, 0.0, h}};
+    const uint32_t first = (uint32_t)P.jobs.size();
+    for (int k = 0; k < 4; k++)
+        VOX_TRY(add_job(ctx, P, V, I, nt, mats, g, shifts[k], use_mats, ms, ctx->fcc_bdim, false,
+                        k == 0 ? nodes : nullptr, kind == VOX_NODE_SHORT_FCC ? 3 : 4, nullptr));
+    P.fcc_first.push_back(first);
+    P.fcc_b.push_back(P.fcc_b.back() + fcc_node_blocks(P.jobs[first].gd));
+    return VOX_OK;
+}
+
+// Host-to-device upload through the context's pinned staging area; waits
+// for the previous upload to leave the area first.
+vox_status upload(vox_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return VOX_OK;
+    if (bytes > ctx->up_cap) {
+        if (ctx->up) {
+            VOX_HIP(hipEventSynchronize(ctx->up_ev));
+            VOX_HIP(hipHostFree(ctx->up));
+            ctx->up = nullptr;
+        }
+        const size_t want = (bytes + bytes / 2 + 4095) & ~(size_t)4095;
+        VOX_HIP(hipHostMalloc(&ctx->up, want, hipHostMallocDefault));
+        ctx->up_cap = want;
+    } else {
+        VOX_HIP(hipEventSynchronize(ctx->up_ev));
+    }
+    std::memcpy(ctx->up, src, bytes);
+    VOX_HIP(hipMemcpyAsync(dst, ctx->up, bytes, hipMemcpyHostToDevice, ctx->stream));
+    VOX_HIP(hipEventRecord(ctx->up_ev, ctx->stream));
+    return VOX_OK;
+}
+
+// Arenas of the call and the device job table (one upload: jobs, split
+// tables, base arrays).
+vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D) {
+    const uint32_t nj = (uint32_t)P.jobs.size(), nf = (uint32_t)P.fcc_first.size();
+    uint32_t *bits = nullptr, *surf = nullptr, *bmo = nullptr;
+    uint16_t *wrank = nullptr;
+    if (P.bits_words) VOX_TRY(buf(ctx, "a_bits", P.bits_words, &bits));
+    if (P.any_mats) {
+        VOX_TRY(buf(ctx, "a_surf", P.surf_words, &surf));
+        VOX_TRY(buf(ctx, "a_wrank", P.wrank_n, &wrank));
+        VOX_TRY(buf(ctx, "a_bmo", P.brick_b.back(), &bmo));
+    }
+    std::vector<uint32_t> bases;
+    for (auto *v : {&P.tri_b, &P.tile_b, &P.pad_b, &P.brick_b, &P.erode_b, &P.node_b})
+        bases.insert(bases.end(), v->begin(), v->end());
+    bases.insert(bases.end(), P.fcc_first.begin(), P.fcc_first.end());
+    bases.insert(bases.end(), P.fcc_b.begin(), P.fcc_b.end());
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t jb = al(sizeof(Job) * nj), sb = al(sizeof(SplitTable) * nj);
+    const size_t bb = bases.size() * 4;
+    uint8_t *dev;
+    VOX_TRY(buf(ctx, "j_table", jb + sb + bb, &dev));
+    const SplitTable *dsplit = (const SplitTable *)(dev + jb);
+    for (uint32_t k = 0; k < nj; k++) {
+        Job &J = P.jobs[k];
+        J.bits = P.ext_bits[k] ? P.ext_bits[k] : bits + P.bits_off[k];
+        J.splits = dsplit + k;
+        if (J.use_mats) {
+            J.surf = surf + P.surf_off[k];
+            J.wrank = wrank + P.wrank_off[k];
+            J.bmo = bmo + J.brick_base;
+        }
+    }
+    std::vector<uint8_t> host(jb + sb + bb);
+    std::memcpy(host.data(), P.jobs.data(), sizeof(Job) * nj);
+    std::memcpy(host.data() + jb, P.splits.data(), sizeof(SplitTable) * nj);
+    std::memcpy(host.data() + jb + sb, bases.data(), bb);
+    VOX_TRY(upload(ctx, dev, host.data(), host.size()));
+    D.jobs = (Job *)dev;
+    uint32_t *b = (uint32_t *)(dev + jb + sb);
+    D.tri_b = b;
+    D.tile_b = (b += nj + 1);
+    D.pad_b = (b += nj + 1);
+    D.brick_b = (b += nj + 1);
+    D.erode_b = (b += nj + 1);
+    D.node_b = (b += nj + 1);
+    D.fcc_first = (b += nj + 1);
+    D.fcc_b = b + nf;
     return VOX_OK;
 }
 
@@ -237,103 +430,69 @@ void record(vox_ctx *ctx, int i) {
 }
 
 // ---------------------------------------------------------------------------
-// Solid stage for one grid (possibly FCC-shifted) over its y/z subspaces.
+// Solid stage over all jobs: per-triangle records, candidate tiles, counting
+// sort by tile, fill.  The solid result does not depend on voxSplitRes y/z
+// (the subspaces partition the centres), so every grid is filled whole.
 // ---------------------------------------------------------------------------
-vox_status solid_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3], uint32_t splitY,
-                      uint32_t splitZ, uint32_t *bits, const char *tag) {
-    const uint32_t X = g.alloc[0], Yext = g.res[1], Zext = g.res[2];
-    const uint32_t W = X / 32;
-    if (W == 0 || (size_t)16 * W * 4 > 65536)
-        return fail(ctx, VOX_ERR_UNSUPPORTED, "x resolution beyond the fill kernel's LDS row buffer");
-    if (g.x_splits == 0 || X / g.x_splits > 1024)
-        return fail(ctx, VOX_ERR_UNSUPPORTED,
-                    "voxSplitRes.x gives x splits wider than 1024 voxels, where the reference's "
-                    "word bitmask is undefined");
-    const uint32_t rowsY = g.alloc[1], rowsZ = g.alloc[2];
-    const uint32_t tilesY = (Yext + 3) / 4, tilesZ = (Zext + 3) / 4;
-    const uint32_t nTiles = tilesY * tilesZ;
-    const uint32_t nt = ctx->nt;
-    (void)tag;
-
-    uint32_t *cand_count, *cand_off, *tile_hist, *tile_off, *tile_cursor, *list, *counters;
+vox_status solid_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
+    const uint32_t nj = (uint32_t)P.jobs.size();
+    const uint32_t nslots = P.tri_b.back(), nTiles = P.tile_b.back();
+    uint32_t *tri_job, *cand_count, *cand_off, *tile_hist, *tile_off, *tile_cursor, *list;
     uint2 *pairs;
-    VOX_TRY(buf(ctx, "s_cand_count", nt, &cand_count));
-    VOX_TRY(buf(ctx, "s_cand_off", nt + 1, &cand_off));
+    uint4 *cbox;
+    VOX_TRY(buf(ctx, "tri_rec", (size_t)std::max<uint32_t>(nslots, 1) * 3, &ctx->rec));
+    VOX_TRY(buf(ctx, "tri_job", nslots, &tri_job));
+    VOX_TRY(buf(ctx, "s_cand_count", nslots, &cand_count));
+    VOX_TRY(buf(ctx, "s_cand_off", nslots + 1, &cand_off));
+    VOX_TRY(buf(ctx, "s_cbox", nslots, &cbox));
     VOX_TRY(buf(ctx, "s_tile_hist", nTiles, &tile_hist));
     VOX_TRY(buf(ctx, "s_tile_off", nTiles + 1, &tile_off));
     VOX_TRY(buf(ctx, "s_tile_cursor", nTiles, &tile_cursor));
-    VOX_TRY(buf(ctx, "s_counters", 4, &counters));
-    uint4 *cbox;
-    VOX_TRY(buf(ctx, "s_cbox", nt, &cbox));
-
-    std::vector<Span> ys = split_axis_max(Yext, splitY), zs = split_axis_max(Zext, splitZ);
-    for (const Span &zsp : zs)
-        for (const Span &ysp : ys) {
-            SolidParams p;
-            p.V = ctx->V;
-            p.I = ctx->I;
-            p.rec = ctx->rec;
-            p.nt = nt;
-            p.emx = extMin[0];
-            p.emy = extMin[1];
-            p.emz = extMin[2];
-            p.d = g.voxel_length;
-            p.smin[0] = ysp.min;
-            p.smin[1] = zsp.min;
-            p.smax[0] = ysp.max;
-            p.smax[1] = zsp.max;
-            p.tilesY = tilesY;
-            p.X = X;
-            p.rowsY = rowsY;
-            if (nt > 0) {
-                k_solid_setup<<<grid_for(nt, 256), 256, 0, ctx->stream>>>(p, cand_count, cbox);
-                VOX_HIP(hipGetLastError());
-            }
-            VOX_TRY(scan(ctx, cand_count, cand_off, nt));
-            uint32_t total = 0;
-            VOX_TRY(read_u32(ctx, cand_off + nt, 1, &total));
-            VOX_TRY(buf(ctx, "s_pairs", total, &pairs));
-            VOX_TRY(buf(ctx, "s_list", total, &list));
-            VOX_TRY(fill(ctx, counters, 4, 0u));
-            VOX_TRY(fill(ctx, tile_hist, nTiles, 0u));
-            VOX_TRY(fill(ctx, tile_cursor, nTiles, 0u));
-            if (total > 0) {
-                k_solid_candidates<<<grid_for(total, 256), 256, 0, ctx->stream>>>(
-                    p, cand_off, cand_off + nt, cbox, pairs, tile_hist);
-                VOX_HIP(hipGetLastError());
-            }
-            VOX_TRY(scan(ctx, tile_hist, tile_off, nTiles));
-            if (total > 0) {
-                k_solid_scatter<<<grid_for(total, 256), 256, 0, ctx->stream>>>(
-                    pairs, cand_off + nt, tile_off, tile_cursor, list);
-                VOX_HIP(hipGetLastError());
-            }
-            const uint32_t ty0 = ysp.min / 4, ty1 = (ysp.max - 1) / 4;
-            const uint32_t tz0 = zsp.min / 4, tz1 = (zsp.max - 1) / 4;
-            const uint32_t nty = ty1 - ty0 + 1, ntz = tz1 - tz0 + 1;
-            if (ysp.max > ysp.min && zsp.max > zsp.min) {
-                k_solid_fill<<<nty * ntz, kWave, 16 * W * 4, ctx->stream>>>(
-                    p, tile_off, tile_hist, list, ty0, nty, tz0, bits);
-                VOX_HIP(hipGetLastError());
-            }
-            ctx->times.solid_pairs += total;  // candidate (triangle, tile) pairs
-        }
-    k_zero_pad_rows<<<grid_for((size_t)(2 * rowsZ + 2 * rowsY) * W, 256, 4096), 256, 0,
-                      ctx->stream>>>(bits, W, rowsY, rowsZ);
-    VOX_HIP(hipGetLastError());
+    ctx->tri_job = tri_job;
+    if (nslots > 0) {
+        k_tri_records<<<grid_for(nslots, 256), 256, 0, ctx->stream>>>(D.jobs, D.tri_b, nj, nslots,
+                                                                       ctx->rec, tri_job);
+        VOX_HIP(hipGetLastError());
+        k_solid_setup<<<grid_for(nslots, 256), 256, 0, ctx->stream>>>(D.jobs, D.tri_b, nj, nslots,
+                                                                       ctx->rec, cand_count, cbox);
+        VOX_HIP(hipGetLastError());
+    }
+    VOX_TRY(scan(ctx, cand_count, cand_off, nslots));
+    uint32_t total = 0;
+    VOX_TRY(read_u32(ctx, cand_off + nslots, 1, &total));
+    VOX_TRY(buf(ctx, "s_pairs", total, &pairs));
+    VOX_TRY(buf(ctx, "s_list", total, &list));
+    VOX_TRY(fill(ctx, tile_hist, nTiles, 0u));
+    VOX_TRY(fill(ctx, tile_cursor, nTiles, 0u));
+    if (total > 0) {
+        if (nj > 1)
+            k_solid_candidates<true><<<grid_for(total, 256), 256, 0, ctx->stream>>>(
+                D.jobs, tri_job, nslots, ctx->rec, cand_off, cand_off + nslots, cbox, pairs,
+                tile_hist);
+        else
+            k_solid_candidates<false><<<grid_for(total, 256), 256, 0, ctx->stream>>>(
+                D.jobs, tri_job, nslots, ctx->rec, cand_off, cand_off + nslots, cbox, pairs,
+                tile_hist);
+        VOX_HIP(hipGetLastError());
+    }
+    VOX_TRY(scan(ctx, tile_hist, tile_off, nTiles));
+    if (total > 0) {
+        k_solid_scatter<<<grid_for(total, 256), 256, 0, ctx->stream>>>(
+            pairs, cand_off + nslots, tile_off, tile_cursor, list);
+        VOX_HIP(hipGetLastError());
+    }
+    if (nTiles > 0) {
+        k_solid_fill<<<nTiles, kWave, 16 * P.maxW * 4, ctx->stream>>>(
+            D.jobs, D.tile_b, nj, ctx->rec, tile_off, tile_hist, list);
+        VOX_HIP(hipGetLastError());
+    }
+    if (P.pad_b.back() > 0) {
+        k_zero_pad_rows<<<P.pad_b.back(), 256, 0, ctx->stream>>>(D.jobs, D.pad_b, nj);
+        VOX_HIP(hipGetLastError());
+    }
+    ctx->times.solid_pairs += total;  // candidate (triangle, tile) pairs
     return VOX_OK;
 }
-
-// ---------------------------------------------------------------------------
-// Surface stage for one grid.
-// ---------------------------------------------------------------------------
-struct SurfOut {
-    uint32_t *surf;
-    uint32_t *brick_mat_off;
-    uint16_t *wrank;
-    uint8_t *mat_list;
-    uint32_t nb[3];
-};
 
 // Appends (features, measured cycles) of the listed bricks to the
 // context's training samples.
@@ -355,78 +514,55 @@ vox_status record_samples(vox_ctx *ctx, const uint32_t *list, const uint32_t *n_
     return VOX_OK;
 }
 
-vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
-                        const uint32_t matSplit[3], const int bdim[3], uint32_t *bits,
-                        const char *tag, SurfOut &out) {
-    const uint32_t nt = ctx->nt;
-    SurfParams p;
-    p.V = ctx->V;
-    p.I = ctx->I;
-    p.nt = nt;
-    p.m = F3{(float)extMin[0], (float)extMin[1], (float)extMin[2]};
-    p.d = (float)g.voxel_length;
-    for (int a = 0; a < 3; a++) {
-        p.res[a] = (int32_t)g.res[a];
-        p.alloc[a] = g.alloc[a];
-        p.bdim[a] = bdim[a];
-        p.nb[a] = (g.alloc[a] + bdim[a] - 1) / bdim[a];
-        out.nb[a] = p.nb[a];
-    }
-    const uint32_t nbt = p.nb[0] * p.nb[1] * p.nb[2];
-    std::string t(tag);
+// ---------------------------------------------------------------------------
+// Surface stage over the jobs that want materials: triangle classes and
+// brick boxes, counting sort of (triangle, brick) pairs by brick, the brick
+// order (plain or learned), the persistent brick pass.  Returns the material
+// byte list the node passes read.
+// ---------------------------------------------------------------------------
+vox_status surface_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, uint8_t **mat_list) {
+    const uint32_t nj = (uint32_t)P.jobs.size();
+    const uint32_t nslots = P.tri_b.back(), nbt = P.brick_b.back();
     uint4 *tri_info, *tri_ext = nullptr;
     uint32_t *cand, *cand_off, *brick_hist, *brick_off, *brick_cursor, *blist, *stats;
-    SplitTable *dsplit;
     const bool learned = ctx->schedule == VOX_SCHEDULE_LEARNED && ctx->cost_ready;
     const bool features = learned || ctx->record;
     if (ctx->timing) ctx->times.learned_schedule = learned ? 1u : 0u;
-    VOX_TRY(buf(ctx, (t + "_tri_info").c_str(), nt, &tri_info));
-    if (features) VOX_TRY(buf(ctx, "f_tri_ext", nt, &tri_ext));
+    VOX_TRY(buf(ctx, "f_tri_info", nslots, &tri_info));
+    if (features) VOX_TRY(buf(ctx, "f_tri_ext", nslots, &tri_ext));
     float4 *tri_rec = ctx->rec;
-    VOX_TRY(buf(ctx, "f_cand", nt, &cand));
-    VOX_TRY(buf(ctx, "f_cand_off", nt + 1, &cand_off));
+    VOX_TRY(buf(ctx, "f_cand", nslots, &cand));
+    VOX_TRY(buf(ctx, "f_cand_off", nslots + 1, &cand_off));
     VOX_TRY(buf(ctx, "f_brick_hist", nbt, &brick_hist));
     VOX_TRY(buf(ctx, "f_brick_off", nbt + 1, &brick_off));
     VOX_TRY(buf(ctx, "f_brick_cursor", nbt, &brick_cursor));
     VOX_TRY(buf(ctx, "f_stats", kMatSlots * kSlotStride + 1, &stats));
-    VOX_TRY(buf(ctx, (t + "_split").c_str(), 1, &dsplit));
-    VOX_TRY(buf(ctx, (t + "_surf").c_str(), (size_t)g.alloc[0] / 32 * g.alloc[1] * g.alloc[2],
-                &out.surf));
-    VOX_TRY(buf(ctx, (t + "_bmo").c_str(), nbt, &out.brick_mat_off));
-    const uint32_t nwords = (uint32_t)(bdim[0] * bdim[1] * bdim[2] / 32);
-    VOX_TRY(buf(ctx, (t + "_wrank").c_str(), (size_t)nbt * nwords, &out.wrank));
-
-    // matSplitRes subspaces over extPartition (src/host_side.cpp:1107-1111).
-    SplitTable st;
-    std::memset(&st, 0, sizeof(st));
-    for (int a = 0; a < 3; a++) {
-        std::vector<Span> sp = split_axis_max(g.res[a], matSplit[a]);
-        if (sp.empty() || sp.size() > (size_t)kMaxSplits)
-            return fail(ctx, VOX_ERR_UNSUPPORTED, "matSplitRes gives too many subspaces");
-        st.n[a] = (uint32_t)sp.size();
-        for (size_t k = 0; k < sp.size(); k++) {
-            st.lo[a][k] = sp[k].min;
-            st.hi[a][k] = sp[k].max;
-        }
-    }
-    VOX_HIP(hipMemcpyAsync(dsplit, &st, sizeof(st), hipMemcpyHostToDevice, ctx->stream));
-
-    if (nt > 0) {
-        k_surf_setup<<<grid_for(nt, 256), 256, 0, ctx->stream>>>(p, tri_info, cand, tri_ext,
-                                                                 tri_rec);
+    if (nslots > 0) {
+        k_surf_setup<<<grid_for(nslots, 256), 256, 0, ctx->stream>>>(D.jobs, D.tri_b, nj, nslots,
+                                                                      tri_info, cand, tri_ext,
+                                                                      tri_rec);
         VOX_HIP(hipGetLastError());
     }
-    VOX_TRY(scan(ctx, cand, cand_off, nt));
+    VOX_TRY(scan(ctx, cand, cand_off, nslots));
     uint32_t total = 0;
-    VOX_TRY(read_u32(ctx, cand_off + nt, 1, &total));
+    VOX_TRY(read_u32(ctx, cand_off + nslots, 1, &total));
     VOX_TRY(buf(ctx, "f_blist", total, &blist));
     VOX_TRY(fill(ctx, brick_hist, nbt, 0u));
     VOX_TRY(fill(ctx, brick_cursor, nbt, 0u));
-    if (total > 0) {
-        k_surf_bin<<<grid_for(total, 256), 256, 0, ctx->stream>>>(
-            p, cand_off, cand_off + nt, tri_info, brick_hist, brick_off, brick_cursor, blist, 0);
+    auto bin = [&](int scatter) -> vox_status {
+        if (total == 0) return VOX_OK;
+        if (nj > 1)
+            k_surf_bin<true><<<grid_for(total, 256), 256, 0, ctx->stream>>>(
+                D.jobs, ctx->tri_job, nslots, cand_off, cand_off + nslots, tri_info, brick_hist,
+                brick_off, brick_cursor, blist, scatter);
+        else
+            k_surf_bin<false><<<grid_for(total, 256), 256, 0, ctx->stream>>>(
+                D.jobs, ctx->tri_job, nslots, cand_off, cand_off + nslots, tri_info, brick_hist,
+                brick_off, brick_cursor, blist, scatter);
         VOX_HIP(hipGetLastError());
-    }
+        return VOX_OK;
+    };
+    VOX_TRY(bin(0));
     VOX_TRY(scan(ctx, brick_hist, brick_off, nbt));
     // Non-empty bricks in index order (the plain schedule of the brick pass).
     uint32_t *flags, *blist_order, *blist_order_off, *next;
@@ -440,11 +576,7 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
     k_nonzero_list<<<grid_for(nbt, 256), 256, 0, ctx->stream>>>(brick_hist, blist_order_off,
                                                                 blist_order, nbt);
     VOX_HIP(hipGetLastError());
-    if (total > 0) {
-        k_surf_bin<<<grid_for(total, 256), 256, 0, ctx->stream>>>(
-            p, cand_off, cand_off + nt, tri_info, brick_hist, brick_off, brick_cursor, blist, 1);
-        VOX_HIP(hipGetLastError());
-    }
+    VOX_TRY(bin(1));
     if (ctx->timing) ctx->times.surface_pairs += total;
     // Learned schedule: the listed bricks sorted by predicted cost, most
     // expensive first (features -> MLP -> 1024-bucket counting sort).
@@ -455,8 +587,8 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
     if (features) VOX_TRY(buf(ctx, "f_feats", (size_t)nbt * kCostFeatures, &feats));
     if (ctx->record) {  // training samples: features of every listed brick
         k_brick_features<<<grid_for(nbt, 4, 65535), 256, 0, ctx->stream>>>(
-            blist_order, nullptr, n_list, brick_hist, brick_off, blist, tri_info, tri_ext, bdim[0],
-            bdim[1], bdim[2], feats);
+            blist_order, nullptr, n_list, brick_hist, brick_off, blist, tri_info, tri_ext, D.jobs,
+            D.brick_b, nj, feats);
         VOX_HIP(hipGetLastError());
     } else if (learned) {
         // Screen by pair count, predict the candidates' cost, order.
@@ -478,7 +610,7 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
         VOX_TRY(fill(ctx, ccur, kCostBuckets, 0u));
         const uint32_t g256 = grid_for(nbt, 256, 1024);
         k_cost_screen<<<g256, 256, 0, ctx->stream>>>(blist_order, n_list, brick_hist,
-                                                     cand_off + nt, ctx->screen_ratio, flags2,
+                                                     cand_off + nslots, ctx->screen_ratio, flags2,
                                                      light);
         VOX_HIP(hipGetLastError());
         VOX_TRY(scan(ctx, flags2, soff, nbt));
@@ -486,8 +618,8 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
         k_cost_compact<<<g256, 256, 0, ctx->stream>>>(flags2, soff, n_list, sel);
         VOX_HIP(hipGetLastError());
         k_brick_features<<<grid_for(nbt, 4, 65535), 256, 0, ctx->stream>>>(
-            blist_order, sel, n_sel, brick_hist, brick_off, blist, tri_info, tri_ext, bdim[0],
-            bdim[1], bdim[2], feats);
+            blist_order, sel, n_sel, brick_hist, brick_off, blist, tri_info, tri_ext, D.jobs,
+            D.brick_b, nj, feats);
         VOX_HIP(hipGetLastError());
         k_mlp_forward_n<<<(nbt + kMlpBlockRows - 1) / kMlpBlockRows, 256, 0, ctx->stream>>>(
             ctx->cost_params, feats, n_sel, pred);
@@ -506,18 +638,19 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
     if (ctx->record) VOX_TRY(buf(ctx, "f_cycles", nbt, &cycles));
     for (int attempt = 0; attempt < 3; attempt++) {
         const uint32_t slot_cap = ctx->mat_slot_cap;
-        VOX_TRY(buf(ctx, (t + "_mats").c_str(), (size_t)slot_cap * kMatSlots, &out.mat_list));
+        VOX_TRY(buf(ctx, "f_mats", (size_t)slot_cap * kMatSlots, mat_list));
         VOX_TRY(fill(ctx, stats, kMatSlots * kSlotStride + 1, 0u));
         uint32_t *overflow = stats + kMatSlots * kSlotStride;
         // Empty bricks are not visited: mark them up front.
-        VOX_TRY(fill(ctx, out.brick_mat_off, nbt, UINT32_MAX));
+        uint32_t *bmo_all;
+        VOX_TRY(buf(ctx, "a_bmo", nbt, &bmo_all));
+        VOX_TRY(fill(ctx, bmo_all, nbt, UINT32_MAX));
         VOX_TRY(fill(ctx, next, 1, 0u));
         const uint32_t blocks =
             std::min<uint32_t>(nbt, (uint32_t)(ctx->n_cu * ctx->surf_blocks_per_cu));
         k_surf_brick<<<blocks, 256, 0, ctx->stream>>>(
-            p, dsplit, tri_rec, brick_off, brick_hist, blist, ctx->mats, bits, out.surf,
-            out.brick_mat_off, out.wrank, out.mat_list, stats, slot_cap, overflow, brick_list,
-            n_list, next, cycles);
+            D.jobs, D.brick_b, nj, tri_rec, brick_off, brick_hist, blist, *mat_list, stats,
+            slot_cap, overflow, brick_list, n_list, next, cycles);
         VOX_HIP(hipGetLastError());
         std::vector<uint32_t> h(kMatSlots * kSlotStride + 1);
         VOX_TRY(read_u32(ctx, stats, h.size(), h.data()));
@@ -537,35 +670,44 @@ vox_status surface_grid(vox_ctx *ctx, const vox_grid &g, const double extMin[3],
     return fail(ctx, VOX_ERR_OOM, "surface material list overflow");
 }
 
-// Erosion to the fixed point, per alloc y/z subspace in the reference's order,
-// in place on `bits`.  Passes are launched kBatch at a time; a pass after one
-// that changed nothing exits at once, so one host read per batch suffices.
-vox_status erode(vox_ctx *ctx, const GridDims &gd, uint32_t splitY, uint32_t splitZ,
-                 uint32_t *bits) {
+// Erosion to the fixed point over every Cartesian job, in place.  A single
+// job may have several alloc y/z subspaces, eroded one after another in the
+// reference's order; jobs of a batch have one each.  Passes are launched
+// kBatch at a time; a job's pass after one that changed nothing in it exits
+// at once, so one host read per batch of passes suffices.
+vox_status erode_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, uint32_t splitY,
+                       uint32_t splitZ) {
     constexpr uint32_t kBatch = 4;
-    const uint32_t nzc = (gd.Z + kErodeZ - 1) / kErodeZ;
-    const uint32_t nblocks = ((gd.W + kWave - 1) / kWave) * ((gd.Y + 3) / 4) * nzc;
-    uint32_t *changes, *stamp;
-    VOX_TRY(buf(ctx, "e_changes", kBatch, &changes));
+    const uint32_t nj = (uint32_t)P.jobs.size();
+    const uint32_t nblocks = P.erode_b.back();
+    if (nblocks == 0) return VOX_OK;
+    uint32_t *changes, *totals, *stamp;
+    VOX_TRY(buf(ctx, "e_changes", (size_t)kBatch * nj, &changes));
+    VOX_TRY(buf(ctx, "e_totals", kBatch, &totals));
     VOX_TRY(buf(ctx, "e_stamp", nblocks, &stamp));
     VOX_TRY(fill(ctx, stamp, nblocks, 0u));
-    std::vector<Span> ys = split_axis_max(gd.Y, splitY), zs = split_axis_max(gd.Z, splitZ);
+    std::vector<Span> ys{{0u, UINT32_MAX}}, zs{{0u, UINT32_MAX}};
+    if (nj == 1) {
+        ys = split_axis_max(P.jobs[0].gd.Y, splitY);
+        zs = split_axis_max(P.jobs[0].gd.Z, splitZ);
+    }
     uint32_t passes = 0, gpass = 0;
     for (const Span &zsp : zs)
         for (const Span &ysp : ys) {
             bool first = true;
             for (int guard = 0; guard < 1000000; guard++) {
-                VOX_TRY(fill(ctx, changes, kBatch, 0u));
+                VOX_TRY(fill(ctx, changes, (size_t)kBatch * nj, 0u));
+                VOX_TRY(fill(ctx, totals, kBatch, 0u));
                 for (uint32_t k = 0; k < kBatch; k++) {
-                    k_erode<<<row_grid(gd.W, gd.Y, nzc), dim3(kWave, 4), 0, ctx->stream>>>(
-                        gd, bits, ysp.min, ysp.max, zsp.min, zsp.max, stamp, changes, k,
-                        gpass + k, (first && k == 0) ? 1 : 0);
+                    k_erode<<<xcd_grid(nblocks), 256, 0, ctx->stream>>>(
+                        D.jobs, D.erode_b, nj, ysp.min, ysp.max, zsp.min, zsp.max, stamp,
+                        changes, totals, k, gpass + k, (first && k == 0) ? 1 : 0);
                     VOX_HIP(hipGetLastError());
                 }
                 gpass += kBatch;
                 first = false;
                 uint32_t h[kBatch];
-                VOX_TRY(read_u32(ctx, changes, kBatch, h));
+                VOX_TRY(read_u32(ctx, totals, kBatch, h));
                 uint32_t k0 = kBatch;
                 for (uint32_t k = 0; k < kBatch; k++)
                     if (h[k] == 0u) {
@@ -580,6 +722,74 @@ vox_status erode(vox_ctx *ctx, const GridDims &gd, uint32_t splitY, uint32_t spl
             }
         }
     ctx->times.erosion_passes += passes;
+    return VOX_OK;
+}
+
+// Node passes: Cartesian jobs and FCC lattices.  With several erosion
+// subspaces (single job only) the ids next to a later subspace read the
+// pre-erosion grid (see k_nodes_cart).
+vox_status node_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, const uint32_t *pre,
+                      const uint8_t *ystart, const uint8_t *zstart, const uint8_t *mat_list) {
+    const uint32_t nj = (uint32_t)P.jobs.size(), nf = (uint32_t)P.fcc_first.size();
+    if (P.node_b.back() > 0) {
+        k_nodes_cart<<<xcd_grid(P.node_b.back()), 256, 0, ctx->stream>>>(
+            D.jobs, D.node_b, nj, pre, ystart, zstart, mat_list);
+        VOX_HIP(hipGetLastError());
+    }
+    if (nf > 0) {
+        k_nodes_fcc<<<xcd_grid(P.fcc_b.back()), dim3(kWave, 4), 0, ctx->stream>>>(
+            D.jobs, D.fcc_first, D.fcc_b, nf, mat_list);
+        VOX_HIP(hipGetLastError());
+    }
+    return VOX_OK;
+}
+
+// The whole pipeline for a plan.  splitY/splitZ: voxSplitRes y/z (erosion
+// subspaces; single-job plans only).
+vox_status run_plan(vox_ctx *ctx, Plan &P, uint32_t splitY, uint32_t splitZ) {
+    PlanDev D;
+    record(ctx, 0);
+    VOX_TRY(plan_upload(ctx, P, D));
+    VOX_TRY(solid_stage(ctx, P, D));
+    record(ctx, 1);
+    uint8_t *mat_list = nullptr;
+    if (P.any_mats) VOX_TRY(surface_stage(ctx, P, D, &mat_list));
+    record(ctx, 2);
+    uint32_t *pre = nullptr;
+    uint8_t *ystart = nullptr, *zstart = nullptr;
+    if (P.jobs.size() == 1 && P.erode_b.back() > 0) {
+        // Several erosion subspaces: keep the pre-erosion grid for the ids
+        // of nodes next to a later subspace.
+        const Job &J = P.jobs[0];
+        std::vector<Span> eys = split_axis_max(J.gd.Y, splitY);
+        std::vector<Span> ezs = split_axis_max(J.gd.Z, splitZ);
+        if (eys.size() > 1 || ezs.size() > 1) {
+            const size_t words = (size_t)J.gd.W * J.gd.Y * J.gd.Z;
+            VOX_TRY(buf(ctx, "n_bits_pre", words, &pre));
+            VOX_HIP(hipMemcpyAsync(pre, J.bits, words * 4, hipMemcpyDeviceToDevice, ctx->stream));
+            std::vector<uint8_t> f(J.gd.Y + J.gd.Z, 0);
+            for (size_t k = 1; k < eys.size(); k++) f[eys[k].min] = 1;
+            for (size_t k = 1; k < ezs.size(); k++) f[J.gd.Y + ezs[k].min] = 1;
+            VOX_TRY(buf(ctx, "n_start", f.size(), &ystart));
+            zstart = ystart + J.gd.Y;
+            VOX_TRY(upload(ctx, ystart, f.data(), f.size()));
+        }
+    }
+    VOX_TRY(erode_stage(ctx, P, D, splitY, splitZ));
+    record(ctx, 3);
+    VOX_TRY(node_stage(ctx, P, D, pre, ystart, zstart, mat_list));
+    record(ctx, 4);
+    return VOX_OK;
+}
+
+vox_status finish_times(vox_ctx *ctx) {
+    if (!ctx->timing) return VOX_OK;
+    VOX_HIP(hipEventSynchronize(ctx->ev[4]));
+    (void)hipEventElapsedTime(&ctx->times.solid_fill_ms, ctx->ev[0], ctx->ev[1]);
+    (void)hipEventElapsedTime(&ctx->times.surface_ms, ctx->ev[1], ctx->ev[2]);
+    (void)hipEventElapsedTime(&ctx->times.erode_ms, ctx->ev[2], ctx->ev[3]);
+    (void)hipEventElapsedTime(&ctx->times.nodes_ms, ctx->ev[3], ctx->ev[4]);
+    (void)hipEventElapsedTime(&ctx->times.total_ms, ctx->ev[0], ctx->ev[4]);
     return VOX_OK;
 }
 
@@ -612,6 +822,10 @@ vox_status vox_create(int device, vox_ctx **out) {
         return VOX_ERR_HIP;
     }
     for (auto &e : ctx->ev) (void)hipEventCreate(&e);
+    if (hipEventCreateWithFlags(&ctx->up_ev, hipEventDisableTiming) != hipSuccess) {
+        vox_destroy(ctx);
+        return VOX_ERR_HIP;
+    }
     if (hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) !=
             hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->surf_blocks_per_cu, k_surf_brick, 256,
@@ -647,6 +861,8 @@ void vox_destroy(vox_ctx *ctx) {
     for (vox_ctx *w : ctx->workers) vox_destroy(w);
     ctx->workers.clear();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->up) (void)hipHostFree(ctx->up);
+    if (ctx->up_ev) (void)hipEventDestroy(ctx->up_ev);
     if (ctx->cost_params) (void)hipFree(ctx->cost_params);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -921,13 +1137,19 @@ vox_status vox_solid_bits(vox_ctx *ctx, const vox_grid *g, const double shift[3]
     if (!ctx->V && ctx->nv) return fail(ctx, VOX_ERR_INVALID, "no mesh");
     VOX_HIP(hipSetDevice(ctx->dev));
     ctx->times = vox_stage_times{};
-    double em[3];
-    for (int a = 0; a < 3; a++) em[a] = g->min_vertex[a] + (shift ? shift[a] : 0.0);
+    const double zero[3] = {0.0, 0.0, 0.0};
     const uint32_t sy = split_yz ? split_yz[0] : 512u, sz = split_yz ? split_yz[1] : 512u;
     if (sy == 0 || sz == 0) return VOX_ERR_INVALID;
+    // The solid grid does not depend on the y/z subspaces (they partition
+    // the voxel centres), so one pass covers the grid.
+    const uint32_t ms[3] = {512u, 512u, 512u};
+    Plan P;
+    VOX_TRY(add_job(ctx, P, ctx->V, ctx->I, ctx->nt, nullptr, *g, shift ? shift : zero, false, ms,
+                    ctx->cart_bdim, false, nullptr, 0, bits_dev));
+    PlanDev D;
     record(ctx, 0);
-    VOX_TRY(make_records(ctx));
-    VOX_TRY(solid_grid(ctx, *g, em, sy, sz, bits_dev, "g1"));
+    VOX_TRY(plan_upload(ctx, P, D));
+    VOX_TRY(solid_stage(ctx, P, D));
     record(ctx, 1);
     if (ctx->timing) {
         VOX_HIP(hipEventSynchronize(ctx->ev[1]));
@@ -937,14 +1159,18 @@ vox_status vox_solid_bits(vox_ctx *ctx, const vox_grid *g, const double shift[3]
     return VOX_OK;
 }
 
+static bool node_kind_ok(vox_node_kind kind) {
+    return kind == VOX_NODE_SHORT || kind == VOX_NODE_LONG || kind == VOX_NODE_SHORT_FCC ||
+           kind == VOX_NODE_LONG_FCC;
+}
+
 vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *g, vox_node_kind kind,
                               int use_materials, const uint32_t vox_split[3],
                               const uint32_t mat_split[3], void *nodes_dev) {
     if (!ctx || !grid_ok(g) || !nodes_dev) return VOX_ERR_INVALID;
     if (kind == VOX_NODE_PARTIAL)
         return fail(ctx, VOX_ERR_UNSUPPORTED, "PartialNode (volume ratio) is not built");
-    const bool fcc = kind == VOX_NODE_SHORT_FCC || kind == VOX_NODE_LONG_FCC;
-    if (kind != VOX_NODE_SHORT && kind != VOX_NODE_LONG && !fcc) return VOX_ERR_INVALID;
+    if (!node_kind_ok(kind)) return VOX_ERR_INVALID;
     if (use_materials && !ctx->mats)
         return fail(ctx, VOX_ERR_INVALID, "materials requested but none set");
     VOX_HIP(hipSetDevice(ctx->dev));
@@ -954,92 +1180,11 @@ vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *g, vox_node_kind kin
                             mat_split ? mat_split[2] : 512u};
     if (!vs[1] || !vs[2] || !ms[0] || !ms[1] || !ms[2]) return VOX_ERR_INVALID;
     ctx->times = vox_stage_times{};
-    const size_t words = (size_t)g->alloc[0] / 32 * g->alloc[1] * g->alloc[2];
-    const double base[3] = {g->min_vertex[0], g->min_vertex[1], g->min_vertex[2]};
-    GridDims gd{g->alloc[0], g->alloc[1], g->alloc[2], g->alloc[0] / 32};
-    record(ctx, 0);
-    VOX_TRY(make_records(ctx));
-    if (!fcc) {
-        uint32_t *bits;
-        VOX_TRY(buf(ctx, "n_bits", words, &bits));
-        VOX_TRY(solid_grid(ctx, *g, base, vs[1], vs[2], bits, "g1"));
-        record(ctx, 1);
-        SurfOut so{};
-        const int *bdim = ctx->cart_bdim;
-        if (use_materials) VOX_TRY(surface_grid(ctx, *g, base, ms, bdim, bits, "g1", so));
-        record(ctx, 2);
-        // Several erosion subspaces: keep the pre-erosion grid for the ids
-        // of nodes next to a later subspace (see k_nodes_cart).
-        std::vector<Span> eys = split_axis_max(g->alloc[1], vs[1]);
-        std::vector<Span> ezs = split_axis_max(g->alloc[2], vs[2]);
-        uint32_t *pre = nullptr;
-        uint8_t *ystart = nullptr, *zstart = nullptr;
-        if (eys.size() > 1 || ezs.size() > 1) {
-            VOX_TRY(buf(ctx, "n_bits_pre", words, &pre));
-            VOX_HIP(hipMemcpyAsync(pre, bits, words * 4, hipMemcpyDeviceToDevice, ctx->stream));
-            std::vector<uint8_t> fy(g->alloc[1], 0), fz(g->alloc[2], 0);
-            for (size_t k = 1; k < eys.size(); k++) fy[eys[k].min] = 1;
-            for (size_t k = 1; k < ezs.size(); k++) fz[ezs[k].min] = 1;
-            VOX_TRY(buf(ctx, "n_ystart", fy.size(), &ystart));
-            VOX_TRY(buf(ctx, "n_zstart", fz.size(), &zstart));
-            VOX_HIP(hipMemcpyAsync(ystart, fy.data(), fy.size(), hipMemcpyHostToDevice, ctx->stream));
-            VOX_HIP(hipMemcpyAsync(zstart, fz.data(), fz.size(), hipMemcpyHostToDevice, ctx->stream));
-            VOX_HIP(hipStreamSynchronize(ctx->stream));
-        }
-        VOX_TRY(erode(ctx, gd, vs[1], vs[2], bits));
-        record(ctx, 3);
-        MatSource msrc{so.surf, so.brick_mat_off, so.wrank, so.mat_list};
-        k_nodes_cart<<<row_grid(gd.W, gd.Y, (gd.Z + kNodeZ - 1) / kNodeZ), dim3(kWave, 4), 0,
-                       ctx->stream>>>(
-            gd, bits, pre, ystart, zstart, msrc, use_materials ? 1 : 0,
-            kind == VOX_NODE_SHORT ? 0 : 1, nodes_dev, brick_dims(*g, bdim));
-        VOX_HIP(hipGetLastError());
-        record(ctx, 4);
-    } else {
-        // fccWorker (src/host_side.cpp:1262-1698): four sub-grids shifted by
-        // (0,0,0), (h,h,0), (0,h,h), (h,0,h), h = d/2 (1436-1463).
-        const double h = g->voxel_length / 2.0;
-        const double shifts[4][3] = {{0.0, 0.0, 0.0}, {h, h, 0.0}, {0.0, h, h}, {h, 0.0, h}};
-        uint32_t *gb[4];
-        SurfOut so[4] = {};
-        const char *tags[4] = {"fg1", "fg2", "fg3", "fg4"};
-        char name[32];
-        for (int k = 0; k < 4; k++) {
-            std::snprintf(name, sizeof(name), "n_fcc_bits%d", k);
-            VOX_TRY(buf(ctx, name, words, &gb[k]));
-            const double em[3] = {base[0] + shifts[k][0], base[1] + shifts[k][1],
-                                  base[2] + shifts[k][2]};
-            VOX_TRY(solid_grid(ctx, *g, em, vs[1], vs[2], gb[k], tags[k]));
-        }
-        record(ctx, 1);
-        const int *bdim = ctx->fcc_bdim;
-        if (use_materials) {
-            for (int k = 0; k < 4; k++) {
-                const double em[3] = {base[0] + shifts[k][0], base[1] + shifts[k][1],
-                                      base[2] + shifts[k][2]};
-                VOX_TRY(surface_grid(ctx, *g, em, ms, bdim, gb[k], tags[k], so[k]));
-            }
-        }
-        record(ctx, 2);
-        record(ctx, 3);
-        MatSource m[4];
-        for (int k = 0; k < 4; k++)
-            m[k] = MatSource{so[k].surf, so[k].brick_mat_off, so[k].wrank, so[k].mat_list};
-        k_nodes_fcc<<<row_grid(2 * gd.W, gd.Y, 2 * gd.Z), dim3(kWave, 4), 0, ctx->stream>>>(
-            gd, gb[0], gb[1], gb[2], gb[3], m[0], m[1], m[2], m[3], use_materials ? 1 : 0,
-            kind == VOX_NODE_SHORT_FCC ? 3 : 4, nodes_dev, brick_dims(*g, bdim));
-        VOX_HIP(hipGetLastError());
-        record(ctx, 4);
-    }
-    if (ctx->timing) {
-        VOX_HIP(hipEventSynchronize(ctx->ev[4]));
-        (void)hipEventElapsedTime(&ctx->times.solid_fill_ms, ctx->ev[0], ctx->ev[1]);
-        (void)hipEventElapsedTime(&ctx->times.surface_ms, ctx->ev[1], ctx->ev[2]);
-        (void)hipEventElapsedTime(&ctx->times.erode_ms, ctx->ev[2], ctx->ev[3]);
-        (void)hipEventElapsedTime(&ctx->times.nodes_ms, ctx->ev[3], ctx->ev[4]);
-        (void)hipEventElapsedTime(&ctx->times.total_ms, ctx->ev[0], ctx->ev[4]);
-    }
-    return VOX_OK;
+    Plan P;
+    VOX_TRY(add_mesh(ctx, P, ctx->V, ctx->I, ctx->nt, ctx->mats, *g, kind, use_materials != 0, ms,
+                     nodes_dev));
+    VOX_TRY(run_plan(ctx, P, vs[1], vs[2]));
+    return finish_times(ctx);
 }
 
 vox_status vox_enable_timing(vox_ctx *ctx, int enable) {
@@ -1120,21 +1265,65 @@ vox_status run_item(vox_ctx *c, vox_batch_item &it) {
                                it.mat_split, it.nodes_dev));
     return vox_sync(c);
 }
+
+bool item_ok(const vox_batch_item &it) {
+    return grid_ok(&it.grid) && node_kind_ok(it.kind) && it.nodes_dev &&
+           (it.nt == 0 || (it.vertices_dev && it.indices_dev)) && it.vox_split[1] &&
+           it.vox_split[2] && it.mat_split[0] && it.mat_split[1] && it.mat_split[2];
+}
+
+// Meshes whose grid erodes as one y/z subspace (FCC lattices do not erode)
+// can share the launches of a fused batch.
+bool fusable(const vox_batch_item &it) {
+    const bool fcc = it.kind == VOX_NODE_SHORT_FCC || it.kind == VOX_NODE_LONG_FCC;
+    return fcc || (it.grid.alloc[1] <= it.vox_split[1] && it.grid.alloc[2] <= it.vox_split[2]);
+}
+
+// All listed meshes through one pipeline: every stage one launch.
+vox_status fused_batch(vox_ctx *ctx, vox_batch_item *items, const std::vector<uint32_t> &idx) {
+    if (idx.empty()) return VOX_OK;
+    Plan P;
+    for (uint32_t i : idx) {
+        const vox_batch_item &it = items[i];
+        VOX_TRY(add_mesh(ctx, P, it.vertices_dev, it.indices_dev, it.nt, it.materials_dev,
+                         it.grid, it.kind, it.materials_dev != nullptr, it.mat_split,
+                         it.nodes_dev));
+    }
+    ctx->times = vox_stage_times{};
+    VOX_TRY(run_plan(ctx, P, UINT32_MAX, UINT32_MAX));
+    VOX_TRY(finish_times(ctx));
+    return vox_sync(ctx);
+}
 }  // namespace
 
 vox_status vox_voxelize_batch(vox_ctx *ctx, vox_batch_item *items, uint32_t n,
                               vox_schedule schedule, uint32_t streams) {
     if (!ctx || (n && !items)) return VOX_ERR_INVALID;
     if (schedule != VOX_SCHEDULE_PLAIN && schedule != VOX_SCHEDULE_LEARNED) return VOX_ERR_INVALID;
+    for (uint32_t i = 0; i < n; i++) {
+        items[i].status = item_ok(items[i]) ? VOX_OK : VOX_ERR_INVALID;
+        if (items[i].status != VOX_OK) {
+            ctx->err = "batch item " + std::to_string(i) + " is invalid";
+            return VOX_ERR_INVALID;
+        }
+    }
+    VOX_HIP(hipSetDevice(ctx->dev));
     vox_status first = VOX_OK;
-    if (schedule == VOX_SCHEDULE_PLAIN || streams <= 1) {
-        const int saved = ctx->schedule;
+    const int saved = ctx->schedule;
+    if (streams <= 1) {
         ctx->schedule = schedule;
-        for (uint32_t i = 0; i < n; i++) {
+        std::vector<uint32_t> fused, single;
+        for (uint32_t i = 0; i < n; i++)
+            (streams == 0 && fusable(items[i]) ? fused : single).push_back(i);
+        const vox_status st = fused_batch(ctx, items, fused);
+        for (uint32_t i : fused) items[i].status = st;
+        if (st != VOX_OK) first = st;
+        for (uint32_t i : single) {
             items[i].status = run_item(ctx, items[i]);
             if (items[i].status != VOX_OK && first == VOX_OK) first = items[i].status;
         }
         ctx->schedule = saved;
+        if (first != VOX_OK && st == VOX_OK) ctx->err = "a batch item failed";
         return first;
     }
     streams = std::min<uint32_t>(streams, 64);
@@ -1151,9 +1340,13 @@ vox_status vox_voxelize_batch(vox_ctx *ctx, vox_batch_item *items, uint32_t n,
     }
     for (uint32_t k = 0; k < streams; k++) {
         vox_ctx *w = ctx->workers[k];
-        w->schedule = VOX_SCHEDULE_LEARNED;
+        w->schedule = schedule;
         w->heavy_cost = ctx->heavy_cost;
         w->timing = false;
+        for (int a = 0; a < 3; a++) {
+            w->cart_bdim[a] = ctx->cart_bdim[a];
+            w->fcc_bdim[a] = ctx->fcc_bdim[a];
+        }
         if (!params.empty()) VOX_TRY(vox_cost_set_params(w, params.data()));
     }
     // Longest estimated work first.

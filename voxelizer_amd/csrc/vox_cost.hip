@@ -42,14 +42,17 @@ __global__ __launch_bounds__(256) void k_brick_features(
     const uint32_t *__restrict__ order, const uint32_t *__restrict__ sel,
     const uint32_t *__restrict__ n_rows, const uint32_t *__restrict__ brick_hist,
     const uint32_t *__restrict__ brick_off, const uint32_t *__restrict__ blist,
-    const uint4 *__restrict__ tri_info, const uint4 *__restrict__ tri_ext, int32_t bd0,
-    int32_t bd1, int32_t bd2, float *__restrict__ feats) {
+    const uint4 *__restrict__ tri_info, const uint4 *__restrict__ tri_ext,
+    const Job *__restrict__ jobs, const uint32_t *__restrict__ brick_bases, uint32_t njobs,
+    float *__restrict__ feats) {
     const uint32_t n = *n_rows;
     const uint32_t lane = lane_id();
     const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
-    const float bvox = (float)(bd0 * bd1 * bd2);
     for (uint32_t q = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < n; q += nwaves) {
         const uint32_t b = order[sel ? sel[q] : q];
+        const Job &J = jobs[job_of(brick_bases, njobs, b)];
+        const int32_t bd0 = J.bdim[0], bd1 = J.bdim[1], bd2 = J.bdim[2];
+        const float bvox = (float)(bd0 * bd1 * bd2);
         const uint32_t cnt = brick_hist[b], off = brick_off[b];
         float c1 = 0, c2 = 0, c3 = 0, i1 = 0, i2 = 0, i3 = 0, v3 = 0, mx = 0, span = 0,
               small = 0, axx = 0, axy = 0;

@@ -44,6 +44,15 @@ __device__ __forceinline__ RowBlock row_block(uint32_t L, uint32_t nxc, uint32_t
     return r;
 }
 
+// Plane-linear blocks of the erosion and Cartesian node passes: the words of
+// a plane are numbered i = y * W + w and cut into chunks of kChunk
+// consecutive words; a 256-thread block takes one chunk over a group of
+// planes, so every lane has a word whatever the row width W.
+constexpr uint32_t kChunk = 256;
+__host__ __device__ __forceinline__ uint32_t plane_chunks(uint32_t W, uint32_t Y) {
+    return (W * Y + kChunk - 1u) / kChunk;
+}
+
 // Surface bricks: default shapes of the Cartesian grid and of each FCC
 // sub-grid; the shape is a run-time choice (the autotuner picks among
 // power-of-two shapes with x a multiple of 64, at most 32 rows and at most
@@ -58,8 +67,12 @@ struct BrickDims {
     uint32_t nbx, nby;
     uint32_t lwx, lby, lbz;
 };
-// z planes per thread in the erosion pass (a block covers 64 words x 4 rows
-// x kErodeZ planes).
+struct GridDims {
+    uint32_t X, Y, Z;  // alloc dims of the (per-grid) bit array
+    uint32_t W;        // X / 32
+};
+// z planes per thread in the erosion pass (a block covers kChunk words of a
+// plane x kErodeZ planes).
 constexpr uint32_t kErodeZ = 8;
 // z planes per thread in the Cartesian node pass; divides kCartBZ so a
 // thread's planes share one surface brick.
@@ -83,9 +96,6 @@ struct SplitTable {
 };
 
 struct SurfParams {
-    const float *V;
-    const uint32_t *I;
-    uint32_t nt;
     F3 m;       // float(extMinVertex)
     float d;    // float(voxelLength)
     int32_t res[3];  // ext resolution (x, y, z)
@@ -93,6 +103,76 @@ struct SurfParams {
     int32_t bdim[3];  // brick dims in voxels
     uint32_t nb[3];   // bricks per axis
 };
+
+// One grid to voxelize: a mesh on a Cartesian grid, or one of the four
+// shifted sub-grids of an FCC lattice.  Any number of jobs share each stage's
+// launches: the stage's work items (triangle slots, tiles, bricks, row
+// blocks) are numbered over all jobs, and a job owns the contiguous run that
+// starts at its *_base.  A job's triangle slots start at a multiple of 256,
+// so every 256-thread block of a per-triangle pass belongs to one job.
+struct Job {
+    const float *V;        // mesh vertices (x, y, z)
+    const uint32_t *I;     // mesh triangles (3 vertex indices)
+    const uint8_t *mats;   // material per mesh triangle, or null
+    uint32_t nt;           // mesh triangles
+    uint32_t tri_base;     // first triangle slot
+    GridDims gd;           // alloc dims of the bit grids
+    int32_t res[3];        // ext resolution
+    uint32_t *bits;        // solid bit words
+    // solid stage
+    double em[3];          // grid origin in FP64 (FCC sub-grids: shifted)
+    double d;              // voxel length
+    uint32_t tilesY, tilesZ;  // 4x4 tiles of voxel-centre rows
+    uint32_t tile_base;    // first tile (one fill workgroup per tile)
+    uint32_t pad_base;     // first block of the pad-row clear
+    // surface stage
+    F3 m;                  // float(origin)
+    float fd;              // float(voxel length)
+    int32_t bdim[3];       // brick dims (voxels)
+    uint32_t nb[3];        // bricks per axis
+    uint32_t brick_base;   // first brick
+    BrickDims bdn;         // brick geometry for the node passes
+    const SplitTable *splits;  // matSplitRes intervals
+    uint32_t *surf;        // surface bit words
+    uint32_t *bmo;         // per brick: offset of its materials, UINT32_MAX if none
+    uint16_t *wrank;       // per brick word: byte offset of its material segment
+    int32_t use_mats;      // surface stage and materials wanted
+    // erosion and nodes (Cartesian jobs; FCC sub-grids have no such blocks)
+    uint32_t erode_base, node_base;
+    void *nodes;
+    int32_t kind;          // 0 ShortNode, 1 LongNode, 3 ShortFCCNode, 4 LongFCCNode
+};
+
+// Job owning item x of a pass whose jobs start at bases[0..n) (ascending;
+// jobs with no items share the next job's base): the last job whose base is
+// <= x.  Uniform when x is.
+__device__ __forceinline__ uint32_t job_of(const uint32_t *__restrict__ bases, uint32_t n,
+                                           uint32_t x) {
+    if (n == 1) return 0u;
+    uint32_t lo = 0, hi = n;
+    while (lo + 1u < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (bases[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ SurfParams surf_params(const Job &J) {
+    SurfParams p;
+    p.m = J.m;
+    p.d = J.fd;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        p.res[a] = J.res[a];
+        p.bdim[a] = J.bdim[a];
+        p.nb[a] = J.nb[a];
+    }
+    p.alloc[0] = J.gd.X;
+    p.alloc[1] = J.gd.Y;
+    p.alloc[2] = J.gd.Z;
+    return p;
+}
 
 // ---------------------------------------------------------------------------
 // Wave helpers
@@ -174,25 +254,46 @@ __global__ void k_bbox(const float *__restrict__ V, uint32_t nv, uint32_t *__res
 // Solid
 // ---------------------------------------------------------------------------
 struct SolidParams {
-    const float *V;
-    const uint32_t *I;
     const float4 *rec;  // per-triangle vertex records (k_tri_records)
-    uint32_t nt;
     double emx, emy, emz, d;
-    uint32_t smin[2], smax[2];  // y/z subspace, ext coords
+    uint32_t smin[2], smax[2];  // y/z extent of the centres, ext coords
     uint32_t tilesY;            // (ext y + 3) / 4
     uint32_t X;                 // alloc x
     uint32_t rowsY;             // alloc y (= ext y + 2)
 };
+__device__ __forceinline__ SolidParams solid_params(const Job &J, const float4 *rec) {
+    SolidParams p;
+    p.rec = rec;
+    p.emx = J.em[0];
+    p.emy = J.em[1];
+    p.emz = J.em[2];
+    p.d = J.d;
+    p.smin[0] = 0u;
+    p.smin[1] = 0u;
+    p.smax[0] = (uint32_t)J.res[1];
+    p.smax[1] = (uint32_t)J.res[2];
+    p.tilesY = J.tilesY;
+    p.X = J.gd.X;
+    p.rowsY = J.gd.Y;
+    return p;
+}
 
 // Per-triangle records: vertices v0, v1, v2 in 48 contiguous bytes
 // (rec[3t] = v0.xyz v1.x, rec[3t+1] = v1.yz v2.xy, rec[3t+2] = v2.z, class
-// word (surface stage), 0, 0).
-__global__ void k_tri_records(const float *__restrict__ V, const uint32_t *__restrict__ I,
-                              uint32_t nt, float4 *__restrict__ rec);
-__global__ void k_solid_setup(SolidParams p, uint32_t *__restrict__ cand_count,
-                              uint4 *__restrict__ cbox);
-__global__ void k_solid_candidates(SolidParams p, const uint32_t *__restrict__ cand_off,
+// word (surface stage), 0, 0), one per triangle slot of every job; tri_job
+// holds each slot's job.  nslots is a multiple of 256 and the per-triangle
+// kernels run 256-thread blocks.
+__global__ void k_tri_records(const Job *__restrict__ jobs, const uint32_t *__restrict__ tri_bases,
+                              uint32_t njobs, uint32_t nslots, float4 *__restrict__ rec,
+                              uint32_t *__restrict__ tri_job);
+__global__ void k_solid_setup(const Job *__restrict__ jobs, const uint32_t *__restrict__ tri_bases,
+                              uint32_t njobs, uint32_t nslots, const float4 *__restrict__ rec,
+                              uint32_t *__restrict__ cand_count, uint4 *__restrict__ cbox);
+template <bool kMulti>
+__global__ void k_solid_candidates(const Job *__restrict__ jobs,
+                                   const uint32_t *__restrict__ tri_job, uint32_t nslots,
+                                   const float4 *__restrict__ rec,
+                                   const uint32_t *__restrict__ cand_off,
                                    const uint32_t *__restrict__ cand_total,
                                    const uint4 *__restrict__ cbox, uint2 *__restrict__ pairs,
                                    uint32_t *__restrict__ tile_hist);
@@ -200,33 +301,40 @@ __global__ void k_solid_scatter(const uint2 *__restrict__ pairs,
                                 const uint32_t *__restrict__ cand_total,
                                 const uint32_t *__restrict__ tile_off,
                                 uint32_t *__restrict__ tile_cursor, uint32_t *__restrict__ list);
-__global__ void k_solid_fill(SolidParams p, const uint32_t *__restrict__ tile_off,
+// One workgroup per tile of every job (grid = all tiles).
+__global__ void k_solid_fill(const Job *__restrict__ jobs, const uint32_t *__restrict__ tile_bases,
+                             uint32_t njobs, const float4 *__restrict__ rec,
+                             const uint32_t *__restrict__ tile_off,
                              const uint32_t *__restrict__ tile_hist,
-                             const uint32_t *__restrict__ list, uint32_t ty0, uint32_t nty,
-                             uint32_t tz0, uint32_t *__restrict__ bits);
-__global__ void k_zero_pad_rows(uint32_t *__restrict__ bits, uint32_t words_per_row,
-                                uint32_t rowsY, uint32_t rowsZ);
+                             const uint32_t *__restrict__ list);
+// Zeroes the pad rows (y or z at the alloc border); grid = all jobs' blocks.
+__global__ void k_zero_pad_rows(const Job *__restrict__ jobs,
+                                const uint32_t *__restrict__ pad_bases, uint32_t njobs);
+__host__ __device__ __forceinline__ uint32_t pad_words(uint32_t W, uint32_t rowsY,
+                                                    uint32_t rowsZ) {
+    return (2u * rowsZ + 2u * rowsY) * W;
+}
 
 // ---------------------------------------------------------------------------
 // Surface
 // ---------------------------------------------------------------------------
-__global__ void k_surf_setup(SurfParams p, uint4 *__restrict__ tri_info,
+__global__ void k_surf_setup(const Job *__restrict__ jobs, const uint32_t *__restrict__ tri_bases,
+                             uint32_t njobs, uint32_t nslots, uint4 *__restrict__ tri_info,
                              uint32_t *__restrict__ cand_count, uint4 *__restrict__ tri_ext,
                              float4 *__restrict__ tri_rec);
-__global__ void k_surf_bin(SurfParams p, const uint32_t *__restrict__ cand_off,
+template <bool kMulti>
+__global__ void k_surf_bin(const Job *__restrict__ jobs, const uint32_t *__restrict__ tri_job,
+                           uint32_t nslots, const uint32_t *__restrict__ cand_off,
                            const uint32_t *__restrict__ cand_total,
                            const uint4 *__restrict__ tri_info,
                            uint32_t *__restrict__ brick_hist, const uint32_t *__restrict__ brick_off,
                            uint32_t *__restrict__ brick_cursor, uint32_t *__restrict__ blist,
                            int scatter);
-__global__ void k_surf_brick(SurfParams p, const SplitTable *__restrict__ splits,
-                             const float4 *__restrict__ tri_rec,
+__global__ void k_surf_brick(const Job *__restrict__ jobs, const uint32_t *__restrict__ brick_bases,
+                             uint32_t njobs, const float4 *__restrict__ tri_rec,
                              const uint32_t *__restrict__ brick_off,
                              const uint32_t *__restrict__ brick_hist,
-                             const uint32_t *__restrict__ blist,
-                             const uint8_t *__restrict__ materials, uint32_t *__restrict__ bits,
-                             uint32_t *__restrict__ surf, uint32_t *__restrict__ brick_mat_off,
-                             uint16_t *__restrict__ wrank, uint8_t *__restrict__ mat_list,
+                             const uint32_t *__restrict__ blist, uint8_t *__restrict__ mat_list,
                              uint32_t *__restrict__ slots, uint32_t slot_cap,
                              uint32_t *__restrict__ overflow, const uint32_t *__restrict__ order,
                              const uint32_t *__restrict__ n_order, uint32_t *__restrict__ next,
@@ -239,14 +347,16 @@ __global__ void k_nonzero_list(const uint32_t *__restrict__ hist, const uint32_t
 // ---------------------------------------------------------------------------
 // Nodes
 // ---------------------------------------------------------------------------
-struct GridDims {
-    uint32_t X, Y, Z;  // alloc dims of the (per-grid) bit array
-    uint32_t W;        // X / 32
-};
-
-__global__ void k_erode(GridDims g, uint32_t *__restrict__ bits, uint32_t y0, uint32_t y1,
-                        uint32_t z0, uint32_t z1, uint32_t *__restrict__ stamp,
-                        uint32_t *__restrict__ changes, uint32_t pass, uint32_t gpass, int full);
+// Erosion pass over every Cartesian job's row blocks (grid = all blocks);
+// voxels outside [y0,y1) x [z0,z1) are left alone.  changes[pass * njobs + j]
+// counts job j's changed waves, totals[pass] all jobs'.
+__global__ void k_erode(const Job *__restrict__ jobs, const uint32_t *__restrict__ erode_bases,
+                        uint32_t njobs, uint32_t y0, uint32_t y1, uint32_t z0, uint32_t z1,
+                        uint32_t *__restrict__ stamp, uint32_t *__restrict__ changes,
+                        uint32_t *__restrict__ totals, uint32_t pass, uint32_t gpass, int full);
+inline uint32_t erode_blocks(const GridDims &g) {
+    return plane_chunks(g.W, g.Y) * ((g.Z + kErodeZ - 1u) / kErodeZ);
+}
 
 // Boundary-id table (permutation -> id) in constant memory.
 hipError_t set_orient_table(const uint8_t lut[64]);
@@ -258,14 +368,24 @@ struct MatSource {
     const uint8_t *mat_list;
 };
 
-__global__ void k_nodes_cart(GridDims g, const uint32_t *__restrict__ solid,
-                             const uint32_t *__restrict__ pre, const uint8_t *__restrict__ ystart,
-                             const uint8_t *__restrict__ zstart, MatSource ms, int use_mats,
-                             int kind, void *__restrict__ nodes, BrickDims bd);
-__global__ void k_nodes_fcc(GridDims g, const uint32_t *s1, const uint32_t *s2,
-                            const uint32_t *s3, const uint32_t *s4, MatSource m1, MatSource m2,
-                            MatSource m3, MatSource m4, int use_mats, int kind,
-                            void *__restrict__ nodes, BrickDims bd);
+// Node passes.  Cartesian: every Cartesian job's row blocks (grid = all);
+// pre/ystart/zstart only with a single job.  FCC: per FCC lattice f, its
+// four sub-grids are jobs fcc_first[f] .. +3 (the first holds the node array
+// and kind, 3 ShortFCCNode or 4 LongFCCNode) and its blocks start at
+// fcc_bases[f].
+__global__ void k_nodes_cart(const Job *__restrict__ jobs, const uint32_t *__restrict__ node_bases,
+                             uint32_t njobs, const uint32_t *__restrict__ pre,
+                             const uint8_t *__restrict__ ystart, const uint8_t *__restrict__ zstart,
+                             const uint8_t *__restrict__ mat_list);
+inline uint32_t node_blocks(const GridDims &g) {
+    return plane_chunks(g.W, g.Y) * ((g.Z + kNodeZ - 1u) / kNodeZ);
+}
+__global__ void k_nodes_fcc(const Job *__restrict__ jobs, const uint32_t *__restrict__ fcc_first,
+                            const uint32_t *__restrict__ fcc_bases, uint32_t nfcc,
+                            const uint8_t *__restrict__ mat_list);
+inline uint32_t fcc_node_blocks(const GridDims &g) {
+    return ((2u * g.W + kWave - 1u) / kWave) * ((g.Y + 3u) / 4u) * 2u * g.Z;
+}
 
 // Cost model kernels (vox_cost.hip).
 __global__ void k_brick_features(const uint32_t *__restrict__ order,
@@ -275,8 +395,9 @@ __global__ void k_brick_features(const uint32_t *__restrict__ order,
                                  const uint32_t *__restrict__ brick_off,
                                  const uint32_t *__restrict__ blist,
                                  const uint4 *__restrict__ tri_info,
-                                 const uint4 *__restrict__ tri_ext, int32_t bd0, int32_t bd1,
-                                 int32_t bd2, float *__restrict__ feats);
+                                 const uint4 *__restrict__ tri_ext, const Job *__restrict__ jobs,
+                                 const uint32_t *__restrict__ brick_bases, uint32_t njobs,
+                                 float *__restrict__ feats);
 __global__ void k_mlp_forward(const float *__restrict__ P, const float *__restrict__ X, uint32_t n,
                               float *__restrict__ Y);
 __global__ void k_mlp_forward_n(const float *__restrict__ P, const float *__restrict__ X,

@@ -107,11 +107,20 @@ __global__ __launch_bounds__(256) void k_bbox(const float *__restrict__ V, uint3
 // ---------------------------------------------------------------------------
 // Solid stage
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_tri_records(const float *__restrict__ V,
-                                                     const uint32_t *__restrict__ I, uint32_t nt,
-                                                     float4 *__restrict__ rec) {
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
-        const uint32_t a = I[3u * t], b = I[3u * t + 1u], c = I[3u * t + 2u];
+__global__ __launch_bounds__(256) void k_tri_records(const Job *__restrict__ jobs,
+                                                     const uint32_t *__restrict__ tri_bases,
+                                                     uint32_t njobs, uint32_t nslots,
+                                                     float4 *__restrict__ rec,
+                                                     uint32_t *__restrict__ tri_job) {
+    for (uint32_t t0 = blockIdx.x * 256u; t0 < nslots; t0 += gridDim.x * 256u) {
+        const uint32_t j = job_of(tri_bases, njobs, t0);
+        const Job &J = jobs[j];
+        const uint32_t t = t0 + threadIdx.x, lt = t - J.tri_base;
+        tri_job[t] = j;
+        if (lt >= J.nt) continue;  // padding slot
+        const float *V = J.V;
+        const uint32_t *I = J.I;
+        const uint32_t a = I[3u * lt], b = I[3u * lt + 1u], c = I[3u * lt + 2u];
         rec[3 * (size_t)t] = make_float4(V[3u * a], V[3u * a + 1u], V[3u * a + 2u], V[3u * b]);
         rec[3 * (size_t)t + 1] =
             make_float4(V[3u * b + 1u], V[3u * b + 2u], V[3u * c], V[3u * c + 1u]);
@@ -131,17 +140,26 @@ __device__ __forceinline__ void load_tri_d(const SolidParams &p, uint32_t t, D3 
 
 // Candidate tiles of each triangle: its commonSpace tile rectangle; the
 // commonSpace box itself is kept so the candidate pass needs no divisions.
-__global__ __launch_bounds__(256) void k_solid_setup(SolidParams p,
+__global__ __launch_bounds__(256) void k_solid_setup(const Job *__restrict__ jobs,
+                                                     const uint32_t *__restrict__ tri_bases,
+                                                     uint32_t njobs, uint32_t nslots,
+                                                     const float4 *__restrict__ rec,
                                                      uint32_t *__restrict__ cand_count,
                                                      uint4 *__restrict__ cbox) {
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < p.nt; t += gridDim.x * blockDim.x) {
-        D3 v[3];
-        load_tri_d(p, t, v);
-        TileSetup ts;
+    for (uint32_t t0 = blockIdx.x * 256u; t0 < nslots; t0 += gridDim.x * 256u) {
+        const Job &J = jobs[job_of(tri_bases, njobs, t0)];
+        const SolidParams p = solid_params(J, rec);
+        const uint32_t t = t0 + threadIdx.x;
         uint32_t c = 0;
-        if (tile_setup(v, p.emy, p.emz, p.d, p.smin[0], p.smin[1], p.smax[0], p.smax[1], ts)) {
-            c = (ts.tmax[0] - ts.tmin[0] + 1u) * (ts.tmax[1] - ts.tmin[1] + 1u);
-            cbox[t] = make_uint4(ts.cmin[0], ts.cmin[1], ts.cmax[0], ts.cmax[1]);
+        if (t - J.tri_base < J.nt) {
+            D3 v[3];
+            load_tri_d(p, t, v);
+            TileSetup ts;
+            if (tile_setup(v, p.emy, p.emz, p.d, p.smin[0], p.smin[1], p.smax[0], p.smax[1],
+                           ts)) {
+                c = (ts.tmax[0] - ts.tmin[0] + 1u) * (ts.tmax[1] - ts.tmin[1] + 1u);
+                cbox[t] = make_uint4(ts.cmin[0], ts.cmin[1], ts.cmax[0], ts.cmax[1]);
+            }
         }
         cand_count[t] = c;
     }
@@ -152,9 +170,12 @@ __global__ __launch_bounds__(256) void k_solid_setup(SolidParams p,
 // triangle.  The triangle of a candidate is found by a binary search narrowed
 // to the triangles the block's candidates span.  Queued pairs are counted per
 // tile (LDS-grouped atomics) for the counting sort.
+template <bool kMulti>
 __global__ __launch_bounds__(256) void k_solid_candidates(
-    SolidParams p, const uint32_t *__restrict__ cand_off, const uint32_t *__restrict__ cand_total,
-    const uint4 *__restrict__ cbox, uint2 *__restrict__ pairs, uint32_t *__restrict__ tile_hist) {
+    const Job *__restrict__ jobs, const uint32_t *__restrict__ tri_job, uint32_t nslots,
+    const float4 *__restrict__ rec, const uint32_t *__restrict__ cand_off,
+    const uint32_t *__restrict__ cand_total, const uint4 *__restrict__ cbox,
+    uint2 *__restrict__ pairs, uint32_t *__restrict__ tile_hist) {
     __shared__ uint32_t trange[2];
     __shared__ AggTable agg;
     const uint32_t total = *cand_total;
@@ -164,7 +185,7 @@ __global__ __launch_bounds__(256) void k_solid_candidates(
         agg_clear(agg);
         if (threadIdx.x < 2) {
             const uint32_t key = threadIdx.x == 0 ? base : min(base + blockDim.x, total) - 1u;
-            trange[threadIdx.x] = upper_bound_u32(cand_off, p.nt, key) - 1u;
+            trange[threadIdx.x] = upper_bound_u32(cand_off, nslots, key) - 1u;
         }
         __syncthreads();
         const uint32_t c = base + threadIdx.x;
@@ -178,6 +199,8 @@ __global__ __launch_bounds__(256) void k_solid_candidates(
                 else hi = mid;
             }
             t = lo;
+            const Job &J = jobs[kMulti ? tri_job[t] : 0u];
+            const SolidParams p = solid_params(J, rec);
             D3 v[3];
             load_tri_d(p, t, v);
             TileSetup ts;
@@ -192,7 +215,7 @@ __global__ __launch_bounds__(256) void k_solid_candidates(
             const uint32_t local = c - cand_off[t];
             const uint32_t ty = tmin0 + local % w, tz = tmin1 + local / w;
             hit = tile_hit(ts, ty, tz, p.emy, p.emz, p.d);
-            tile = tz * p.tilesY + ty;
+            tile = J.tile_base + tz * p.tilesY + ty;
         }
         // One slot per candidate (no global counter); misses are skipped
         // by the scatter pass.
@@ -205,6 +228,14 @@ __global__ __launch_bounds__(256) void k_solid_candidates(
             if (agg.key[i] != kAggEmpty) atomicAdd(&tile_hist[agg.key[i]], agg.cnt[i]);
     }
 }
+template __global__ void k_solid_candidates<false>(const Job *, const uint32_t *, uint32_t,
+                                                   const float4 *, const uint32_t *,
+                                                   const uint32_t *, const uint4 *, uint2 *,
+                                                   uint32_t *);
+template __global__ void k_solid_candidates<true>(const Job *, const uint32_t *, uint32_t,
+                                                  const float4 *, const uint32_t *,
+                                                  const uint32_t *, const uint4 *, uint2 *,
+                                                  uint32_t *);
 
 __global__ __launch_bounds__(256) void k_solid_scatter(const uint2 *__restrict__ pairs,
                                                        const uint32_t *__restrict__ cand_total,
@@ -236,20 +267,25 @@ __global__ __launch_bounds__(256) void k_solid_scatter(const uint2 *__restrict__
 // (ds_xor); a per-row prefix XOR then turns crossings into solid runs
 // (the suffix flips of src/voxelizer.cu:1487-1616) and each bit word is
 // written to HBM once, coalesced, with no global atomics.
-__global__ __launch_bounds__(kWave) void k_solid_fill(SolidParams p,
+__global__ __launch_bounds__(kWave) void k_solid_fill(const Job *__restrict__ jobs,
+                                                      const uint32_t *__restrict__ tile_bases,
+                                                      uint32_t njobs,
+                                                      const float4 *__restrict__ rec,
                                                       const uint32_t *__restrict__ tile_off,
                                                       const uint32_t *__restrict__ tile_hist,
-                                                      const uint32_t *__restrict__ list,
-                                                      uint32_t ty0, uint32_t nty, uint32_t tz0,
-                                                      uint32_t *__restrict__ bits) {
+                                                      const uint32_t *__restrict__ list) {
     extern __shared__ uint32_t cross[];  // [16][W]
+    const Job &J = jobs[job_of(tile_bases, njobs, blockIdx.x)];
+    const SolidParams p = solid_params(J, rec);
+    uint32_t *__restrict__ bits = J.bits;
     const uint32_t W = p.X >> 5;
     const uint32_t tid = threadIdx.x;
-    const uint32_t ty = ty0 + blockIdx.x % nty;
-    const uint32_t tz = tz0 + blockIdx.x / nty;
+    const uint32_t local = blockIdx.x - J.tile_base;
+    const uint32_t ty = local % p.tilesY;
+    const uint32_t tz = local / p.tilesY;
     for (uint32_t i = tid; i < 16u * W; i += kWave) cross[i] = 0u;
     __syncthreads();
-    const uint32_t tile = tz * p.tilesY + ty;
+    const uint32_t tile = blockIdx.x;
     const uint32_t cnt = tile_hist[tile];
     const uint32_t off = tile_off[tile];
     for (uint32_t i = tid; i < cnt; i += kWave) {
@@ -294,25 +330,24 @@ __global__ __launch_bounds__(kWave) void k_solid_fill(SolidParams p,
     }
 }
 
-__global__ __launch_bounds__(256) void k_zero_pad_rows(uint32_t *__restrict__ bits,
-                                                       uint32_t W, uint32_t rowsY,
-                                                       uint32_t rowsZ) {
-    const uint64_t nrows = 2ull * rowsZ + 2ull * rowsY;
-    const uint64_t total = nrows * W;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t r = i / W, w = i % W;
-        uint32_t y, z;
-        if (r < 2ull * rowsZ) {
-            z = (uint32_t)(r >> 1);
-            y = (r & 1ull) ? rowsY - 1u : 0u;
-        } else {
-            uint64_t q = r - 2ull * rowsZ;
-            y = (uint32_t)(q >> 1);
-            z = (q & 1ull) ? rowsZ - 1u : 0u;
-        }
-        bits[((uint64_t)z * rowsY + y) * W + w] = 0u;
+__global__ __launch_bounds__(256) void k_zero_pad_rows(const Job *__restrict__ jobs,
+                                                       const uint32_t *__restrict__ pad_bases,
+                                                       uint32_t njobs) {
+    const Job &J = jobs[job_of(pad_bases, njobs, blockIdx.x)];
+    const uint32_t W = J.gd.W, rowsY = J.gd.Y, rowsZ = J.gd.Z;
+    const uint32_t i = (blockIdx.x - J.pad_base) * 256u + threadIdx.x;
+    if (i >= pad_words(W, rowsY, rowsZ)) return;
+    const uint32_t r = i / W, w = i % W;
+    uint32_t y, z;
+    if (r < 2u * rowsZ) {
+        z = r >> 1;
+        y = (r & 1u) ? rowsY - 1u : 0u;
+    } else {
+        const uint32_t q = r - 2u * rowsZ;
+        y = q >> 1;
+        z = (q & 1u) ? rowsZ - 1u : 0u;
     }
+    J.bits[((size_t)z * rowsY + y) * W + w] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -360,11 +395,21 @@ __device__ __forceinline__ void brick_range(const SurfParams &p, const VoxBB &vb
 // {info, lo.x | lo.y<<16, lo.z | w.x<<16, w.y | w.z<<16}.  With tri_ext,
 // also its clipped voxel extents along (u, v, a) and its work items (voxels
 // for 1D, dominant-plane columns otherwise), the cost model's inputs.
-__global__ __launch_bounds__(256) void k_surf_setup(SurfParams p, uint4 *__restrict__ tri_info,
+__global__ __launch_bounds__(256) void k_surf_setup(const Job *__restrict__ jobs,
+                                                    const uint32_t *__restrict__ tri_bases,
+                                                    uint32_t njobs, uint32_t nslots,
+                                                    uint4 *__restrict__ tri_info,
                                                     uint32_t *__restrict__ cand_count,
                                                     uint4 *__restrict__ tri_ext,
                                                     float4 *__restrict__ tri_rec) {
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < p.nt; t += gridDim.x * blockDim.x) {
+    for (uint32_t t0 = blockIdx.x * 256u; t0 < nslots; t0 += gridDim.x * 256u) {
+        const Job &J = jobs[job_of(tri_bases, njobs, t0)];
+        const uint32_t t = t0 + threadIdx.x;
+        if (t - J.tri_base >= J.nt || !J.use_mats) {  // padding, or no surface wanted
+            cand_count[t] = 0u;
+            continue;
+        }
+        const SurfParams p = surf_params(J);
         const float4 r0 = tri_rec[3 * (size_t)t], r1 = tri_rec[3 * (size_t)t + 1],
                      r2 = tri_rec[3 * (size_t)t + 2];
         const F3 tri[3] = {{r0.x, r0.y, r0.z}, {r0.w, r1.x, r1.y}, {r1.z, r1.w, r2.x}};
@@ -398,8 +443,10 @@ __global__ __launch_bounds__(256) void k_surf_setup(SurfParams p, uint4 *__restr
 // or placement pass (scatter = 1) of the counting sort by brick.  The
 // triangle of candidate c is found by binary search in the candidate offsets,
 // narrowed per block to the triangles its 256 candidates span.
+template <bool kMulti>
 __global__ __launch_bounds__(256) void k_surf_bin(
-    SurfParams p, const uint32_t *__restrict__ cand_off, const uint32_t *__restrict__ cand_total,
+    const Job *__restrict__ jobs, const uint32_t *__restrict__ tri_job, uint32_t nslots,
+    const uint32_t *__restrict__ cand_off, const uint32_t *__restrict__ cand_total,
     const uint4 *__restrict__ tri_info, uint32_t *__restrict__ brick_hist,
     const uint32_t *__restrict__ brick_off, uint32_t *__restrict__ brick_cursor,
     uint32_t *__restrict__ blist, int scatter) {
@@ -411,7 +458,7 @@ __global__ __launch_bounds__(256) void k_surf_bin(
         agg_clear(agg);
         if (threadIdx.x < 2) {
             const uint32_t key = threadIdx.x == 0 ? base : min(base + blockDim.x, total) - 1u;
-            trange[threadIdx.x] = upper_bound_u32(cand_off, p.nt, key) - 1u;
+            trange[threadIdx.x] = upper_bound_u32(cand_off, nslots, key) - 1u;
         }
         __syncthreads();
         const uint32_t c = base + threadIdx.x;
@@ -426,6 +473,7 @@ __global__ __launch_bounds__(256) void k_surf_bin(
         const uint32_t t = lo;
         uint32_t b = 0, slot = 0, rank = 0;
         if (live) {
+            const Job &J = jobs[kMulti ? tri_job[t] : 0u];
             const uint4 ti = tri_info[t];
             const uint32_t bx0 = ti.y & 0xFFFFu, by0 = ti.y >> 16, bz0 = ti.z & 0xFFFFu;
             const uint32_t w0 = ti.z >> 16, w1 = ti.w & 0xFFFFu;
@@ -433,7 +481,7 @@ __global__ __launch_bounds__(256) void k_surf_bin(
             const uint32_t bx = bx0 + local % w0;
             const uint32_t by = by0 + (local / w0) % w1;
             const uint32_t bz = bz0 + local / (w0 * w1);
-            b = (bz * p.nb[1] + by) * p.nb[0] + bx;
+            b = J.brick_base + (bz * J.nb[1] + by) * J.nb[0] + bx;
             slot = agg_insert(agg, b, rank);
         }
         __syncthreads();
@@ -448,6 +496,14 @@ __global__ __launch_bounds__(256) void k_surf_bin(
         if (live && scatter) blist[brick_off[b] + agg.base[slot] + rank] = t;
     }
 }
+template __global__ void k_surf_bin<false>(const Job *, const uint32_t *, uint32_t,
+                                           const uint32_t *, const uint32_t *, const uint4 *,
+                                           uint32_t *, const uint32_t *, uint32_t *, uint32_t *,
+                                           int);
+template __global__ void k_surf_bin<true>(const Job *, const uint32_t *, uint32_t,
+                                          const uint32_t *, const uint32_t *, const uint4 *,
+                                          uint32_t *, const uint32_t *, uint32_t *, uint32_t *,
+                                          int);
 
 // Per-triangle data of one brick's chunk, set up once in LDS.  Coordinates
 // are named (u, v, a): a is the triangle's dominant axis, u and v the other
@@ -673,12 +729,11 @@ __device__ __forceinline__ void surf_item(const SurfParams &p, const SplitTable 
 // cleared for the next brick.  Surface bits are ORed into the solid grid.
 // With `cycles`, each brick's duration (s_memtime) is recorded.
 __global__ __launch_bounds__(256, 5) void k_surf_brick(
-    SurfParams p, const SplitTable *__restrict__ splits, const float4 *__restrict__ tri_rec,
-    const uint32_t *__restrict__ brick_off, const uint32_t *__restrict__ brick_hist,
-    const uint32_t *__restrict__ blist, const uint8_t *__restrict__ materials,
-    uint32_t *__restrict__ bits, uint32_t *__restrict__ surf, uint32_t *__restrict__ brick_mat_off,
-    uint16_t *__restrict__ wrank, uint8_t *__restrict__ mat_list, uint32_t *__restrict__ slots,
-    uint32_t slot_cap, uint32_t *__restrict__ overflow, const uint32_t *__restrict__ order,
+    const Job *__restrict__ jobs, const uint32_t *__restrict__ brick_bases, uint32_t njobs,
+    const float4 *__restrict__ tri_rec, const uint32_t *__restrict__ brick_off,
+    const uint32_t *__restrict__ brick_hist, const uint32_t *__restrict__ blist,
+    uint8_t *__restrict__ mat_list, uint32_t *__restrict__ slots, uint32_t slot_cap,
+    uint32_t *__restrict__ overflow, const uint32_t *__restrict__ order,
     const uint32_t *__restrict__ n_order, uint32_t *__restrict__ next,
     uint64_t *__restrict__ cycles) {
     __shared__ uint32_t keys[kMaxBrickVox];
@@ -691,30 +746,39 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
     __shared__ uint32_t sbase, s_q;
     __shared__ SplitTable spl;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
-    const int32_t bd0 = p.bdim[0], bd1 = p.bdim[1], bd2 = p.bdim[2];
-    const int32_t nvox = bd0 * bd1 * bd2;
-    const int32_t nrows = bd1 * bd2;  // <= 32
-    const int32_t wpr = bd0 / 32;     // words per row
-    const int32_t nwords = nvox / 32;
-    const int32_t row_shift = 31 - __clz(bd0);
-    for (int32_t i = tid; i < nvox; i += blockDim.x) keys[i] = 0u;
+    for (int32_t i = tid; i < kMaxBrickVox; i += blockDim.x) keys[i] = 0u;
     if (tid < 32u) touched[tid] = 0;
-    {
-        const uint32_t *src = (const uint32_t *)splits;
-        uint32_t *dst = (uint32_t *)&spl;
-        for (uint32_t i = tid; i < sizeof(SplitTable) / 4; i += blockDim.x) dst[i] = src[i];
-    }
     const uint32_t n_list = *n_order;
     if (tid == 0) s_q = atomicAdd(next, 1u);
     __syncthreads();
     uint32_t q = s_q;
+    uint32_t cur_job = UINT32_MAX;  // job whose split table is in LDS
     while (q < n_list) {  // q is uniform over the block
         uint32_t q_next = 0u;
         if (tid == 0) q_next = atomicAdd(next, 1u);  // grab the next brick early
-        const uint32_t b = order[q];
-        const uint32_t cnt = brick_hist[b];
+        const uint32_t gb = order[q];
+        const uint32_t cnt = brick_hist[gb];
         uint64_t t_start = 0;
         if (cycles && tid == 0) t_start = __builtin_amdgcn_s_memtime();
+        const uint32_t j = job_of(brick_bases, njobs, gb);
+        const Job &J = jobs[j];
+        if (j != cur_job) {  // uniform: the job's matSplitRes table into LDS
+            __syncthreads();
+            const uint32_t *src = (const uint32_t *)J.splits;
+            uint32_t *dst = (uint32_t *)&spl;
+            for (uint32_t i = tid; i < sizeof(SplitTable) / 4; i += blockDim.x) dst[i] = src[i];
+            cur_job = j;
+        }
+        const SurfParams p = surf_params(J);
+        const uint8_t *__restrict__ materials = J.mats;
+        uint32_t *__restrict__ bits = J.bits;
+        const uint32_t b = gb - J.brick_base;  // brick within the job
+        const int32_t bd0 = p.bdim[0], bd1 = p.bdim[1], bd2 = p.bdim[2];
+        const int32_t nvox = bd0 * bd1 * bd2;
+        const int32_t nrows = bd1 * bd2;  // <= 32
+        const int32_t wpr = bd0 / 32;     // words per row
+        const int32_t nwords = nvox / 32;
+        const int32_t row_shift = 31 - __clz(bd0);
         const uint32_t bxi = b % p.nb[0], byi = (b / p.nb[0]) % p.nb[1],
                        bzi = b / (p.nb[0] * p.nb[1]);
         // Bricks are laid over alloc coordinates; ext = alloc - 1.
@@ -736,7 +800,7 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
                 bw = bits[gi];
             }
         }
-        const uint32_t off = brick_off[b];
+        const uint32_t off = brick_off[gb];
         for (uint32_t c0 = 0; c0 < cnt; c0 += kSurfChunk) {
             __syncthreads();
             if (tid < (uint32_t)kSurfChunk) {
@@ -805,7 +869,7 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
             // Material space from one of kMatSlots arenas (brick b uses arena
             // b % kMatSlots), so concurrent bricks bump different counters.
             const uint32_t acc = wtot[0] + wtot[1] + wtot[2] + wtot[3];
-            const uint32_t sl = b % kMatSlots;
+            const uint32_t sl = gb % kMatSlots;
             const uint32_t local = atomicAdd(&slots[sl * kSlotStride], acc);
             atomicAdd(&slots[sl * kSlotStride + 1], wpop[0] + wpop[1] + wpop[2] + wpop[3]);
             uint32_t base = sl * slot_cap + local;
@@ -814,14 +878,14 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
                 base = UINT32_MAX;
             }
             sbase = base;
-            brick_mat_off[b] = base;
+            J.bmo[b] = base;
         }
         __syncthreads();
         const uint32_t base = sbase;
         if ((int32_t)tid < nwords && gword) {
             const uint32_t sw = wsurf[tid];
-            surf[gi] = sw;
-            wrank[(size_t)b * nwords + tid] = (uint16_t)wseg[tid];
+            J.surf[gi] = sw;
+            J.wrank[(size_t)b * nwords + tid] = (uint16_t)wseg[tid];
             if (sw) bits[gi] = bw | sw;
         }
         // Materials: one lane per voxel of a touched row; keys cleared.
@@ -836,12 +900,13 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
                         const int32_t w = vi >> 5, qx = vi & 31;
                         const uint32_t before = wsurf[w] & ~(0xFFFFFFFFu >> qx);
                         mat_list[base + wseg[w] + (uint32_t)__popc(before)] =
-                            materials ? materials[key & 0x3FFFFFFFu] : (uint8_t)0;
+                            materials ? materials[(key & 0x3FFFFFFFu) - J.tri_base]
+                                      : (uint8_t)0;
                     }
                 }
             }
         }
-        if (cycles && tid == 0) cycles[b] = __builtin_amdgcn_s_memtime() - t_start;
+        if (cycles && tid == 0) cycles[gb] = __builtin_amdgcn_s_memtime() - t_start;
         __syncthreads();
         if (tid < 32u) touched[tid] = 0;
         if (tid == 0) s_q = q_next;
@@ -871,45 +936,60 @@ __global__ __launch_bounds__(256) void k_nonzero_list(const uint32_t *__restrict
 // monotone and only removes, so passes may run in place and in any order:
 // every such (chaotic) iteration reaches the same greatest fixed point as
 // the reference's.  The first pass of a subspace reads the whole grid; a
-// later pass only visits row blocks that changed, or whose face-adjacent
-// blocks changed, since the previous pass (stamp = 1 + last pass with a
+// later pass only visits blocks that changed, or that hold neighbours of
+// words in blocks that changed, in the previous pass (stamp = 1 + last pass with a
 // change), and a word is written only when it changes.  Voxels outside
-// [y0,y1) x [z0,z1) are left alone.  Row-major launch: block (64, 4) = 4 rows
-// of up to 64 words.
+// [y0,y1) x [z0,z1) are left alone.  Plane-linear launch: a block takes
+// kChunk consecutive words of a plane over kErodeZ planes.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_erode(GridDims g, uint32_t *__restrict__ bits,
-                                               uint32_t y0, uint32_t y1, uint32_t z0,
-                                               uint32_t z1, uint32_t *__restrict__ stamp,
-                                               uint32_t *__restrict__ changes, uint32_t pass,
+__global__ __launch_bounds__(256) void k_erode(const Job *__restrict__ jobs,
+                                               const uint32_t *__restrict__ erode_bases,
+                                               uint32_t njobs, uint32_t y0, uint32_t y1,
+                                               uint32_t z0, uint32_t z1,
+                                               uint32_t *__restrict__ stamp,
+                                               uint32_t *__restrict__ changes,
+                                               uint32_t *__restrict__ totals, uint32_t pass,
                                                uint32_t gpass, int full) {
-    if (pass > 0 && changes[pass - 1] == 0u) return;
-    const uint32_t nxc = (g.W + kWave - 1) / kWave, ny4 = (g.Y + 3) / 4;
+    const uint32_t GL = xcd_logical_block(erode_bases[njobs]);  // bases[njobs] = all blocks
+    if (GL == UINT32_MAX) return;
+    const uint32_t j = job_of(erode_bases, njobs, GL);
+    if (pass > 0 && changes[(pass - 1u) * njobs + j] == 0u) return;
+    const Job &J = jobs[j];
+    const GridDims g = J.gd;
+    uint32_t *__restrict__ bits = J.bits;
+    const uint32_t P = g.W * g.Y;  // words per plane
+    const uint32_t nch = (P + kChunk - 1u) / kChunk;
     const uint32_t nzc = (g.Z + kErodeZ - 1) / kErodeZ;
-    const uint32_t L = xcd_logical_block(nxc * ny4 * nzc);
-    if (L == UINT32_MAX) return;
-    const RowBlock rb = row_block(L, nxc, ny4);
-    const uint32_t zb = rb.z * kErodeZ;
-    if (zb >= z1 || zb + kErodeZ <= z0 || rb.y4 * 4 + 4 <= y0 || rb.y4 * 4 >= y1) return;
+    const uint32_t L = GL - J.erode_base;  // block within the job
+    stamp += J.erode_base;
+    const uint32_t c = L % nch, zg = L / nch;
+    const uint32_t zb = zg * kErodeZ;
+    const uint32_t i0 = c * kChunk;
+    const uint32_t ylo = i0 / g.W, yhi = min(i0 + kChunk - 1u, P - 1u) / g.W;
+    if (zb >= z1 || zb + kErodeZ <= z0 || yhi < y0 || ylo >= y1) return;
     if (!full) {
-        const uint32_t plane = nxc * ny4;
-        bool visit = stamp[L] >= gpass;
-        if (rb.xc > 0) visit |= stamp[L - 1] >= gpass;
-        if (rb.xc + 1 < nxc) visit |= stamp[L + 1] >= gpass;
-        if (rb.y4 > 0) visit |= stamp[L - nxc] >= gpass;
-        if (rb.y4 + 1 < ny4) visit |= stamp[L + nxc] >= gpass;
-        if (rb.z > 0) visit |= stamp[L - plane] >= gpass;
-        if (rb.z + 1 < nzc) visit |= stamp[L + plane] >= gpass;
+        // A word depends on words at most W positions away in its plane and
+        // on the same word in the planes next to it: visit when any chunk
+        // within W words, or this chunk in the plane groups next to it,
+        // changed in the previous pass.
+        const uint32_t r = g.W / kChunk + 1u;
+        const uint32_t clo = c >= r ? c - r : 0u, chi = min(c + r, nch - 1u);
+        bool visit = false;
+        for (uint32_t cc = clo; cc <= chi; cc++) visit |= stamp[zg * nch + cc] >= gpass;
+        if (zg > 0) visit |= stamp[L - nch] >= gpass;
+        if (zg + 1 < nzc) visit |= stamp[L + nch] >= gpass;
         if (!visit) return;
     }
-    const uint32_t tx = threadIdx.x;
-    const uint32_t w = rb.xc * kWave + tx;
-    const uint32_t y = rb.y4 * 4 + threadIdx.y;
-    const bool valid = w < g.W && y < g.Y;
-    const size_t rowW = g.W, planeW = (size_t)g.W * g.Y;
-    const size_t col0 = (size_t)y * rowW + w;  // index of (w, y) in plane 0
+    const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1u);
+    const uint32_t i = i0 + tid;
+    const bool valid = i < P;
+    const uint32_t y = valid ? i / g.W : 0u;
+    const uint32_t w = valid ? i - y * g.W : 0u;
+    const size_t rowW = g.W, planeW = P;
+    const size_t col0 = valid ? i : 0;  // index of (w, y) in plane 0
     const bool hyn = valid && y > 0, hyp = valid && y + 1 < g.Y;
     // Column z = zb-1 .. zb+kErodeZ of this word, the rows above and below it
-    // for the thread's planes, and the words across the chunk edges: all
+    // for the thread's planes, and the words across the wave's edges: all
     // loads are independent and in flight together (clamped addresses).
     uint32_t col[kErodeZ + 2], yn[kErodeZ], yp[kErodeZ], em[kErodeZ], ep[kErodeZ];
 #pragma unroll
@@ -922,14 +1002,14 @@ __global__ __launch_bounds__(256) void k_erode(GridDims g, uint32_t *__restrict_
 #pragma unroll
     for (uint32_t k = 0; k < kErodeZ; k++) {
         const uint32_t z = zb + k;
-        const bool ok = z < g.Z;
-        const size_t wi = (size_t)z * planeW + col0;
+        const bool ok = valid && z < g.Z;
+        const size_t wi = ok ? (size_t)z * planeW + col0 : 0;
         yn[k] = bits[ok && hyn ? wi - rowW : 0];
         yp[k] = bits[ok && hyp ? wi + rowW : 0];
         if (!(ok && hyn)) yn[k] = 0u;
         if (!(ok && hyp)) yp[k] = 0u;
-        const bool lm = ok && valid && tx == 0 && w > 0;
-        const bool rp = ok && valid && tx == kWave - 1 && w + 1 < g.W;
+        const bool lm = ok && lane == 0 && w > 0;
+        const bool rp = ok && lane == kWave - 1 && w + 1 < g.W;
         em[k] = lm ? bits[wi - 1] : 0u;
         ep[k] = rp ? bits[wi + 1] : 0u;
     }
@@ -938,11 +1018,14 @@ __global__ __launch_bounds__(256) void k_erode(GridDims g, uint32_t *__restrict_
     for (uint32_t k = 0; k < kErodeZ; k++) {
         const uint32_t z = zb + k;
         const uint32_t G = col[k + 1];
-        // x neighbours: adjacent lanes hold the adjacent words of the row.
+        // x neighbours: adjacent lanes hold the adjacent words of the plane;
+        // across a row's ends there is nothing.
         uint32_t gm = (uint32_t)__shfl_up((int)G, 1);
         uint32_t gp = (uint32_t)__shfl_down((int)G, 1);
-        if (tx == 0) gm = em[k];
-        if (tx == kWave - 1) gp = ep[k];
+        if (lane == 0) gm = em[k];
+        if (lane == kWave - 1) gp = ep[k];
+        if (w == 0) gm = 0u;
+        if (w + 1 >= g.W) gp = 0u;
         if (!valid || !G || z >= g.Z || z < z0 || z >= z1 || y < y0 || y >= y1) continue;
         const uint32_t Lm = (G >> 1) | (gm << 31);
         const uint32_t Rm = (G << 1) | (gp >> 31);
@@ -954,8 +1037,9 @@ __global__ __launch_bounds__(256) void k_erode(GridDims g, uint32_t *__restrict_
         }
     }
     const unsigned long long m = __ballot(changed);
-    if (lane_id() == 0 && m) {
-        atomicAdd(&changes[pass], (uint32_t)__popcll(m));
+    if (lane == 0 && m) {
+        atomicAdd(&changes[pass * njobs + j], (uint32_t)__popcll(m));
+        atomicAdd(&totals[pass], (uint32_t)__popcll(m));
         stamp[L] = gpass + 1u;
     }
 }
@@ -1064,10 +1148,11 @@ __device__ __forceinline__ void write_word_nodes(int kind, void *__restrict__ no
     for (int j = 0; j < n16; j++) dst[j] = my[j];
 }
 
-// Cartesian node write: block (64, 4) = 64 words x 4 rows, each thread a
-// column of kNodeZ planes.  Every global load the column needs (the words
-// of planes zb-1..zb+kNodeZ, the rows above and below, the words across the
-// chunk edges, the surface words, the brick's material offset and word
+// Cartesian node write: a block takes kChunk consecutive words of a plane
+// (plane-linear, so no lane idles on narrow rows), each thread a column of
+// kNodeZ planes.  Every global load the column needs (the words of planes
+// zb-1..zb+kNodeZ, the rows above and below, the words across the wave's
+// edges, the surface words, the brick's material offset and word
 // ranks) is issued at once from clamped, always-valid addresses, so a wave
 // waits for memory once; the words' first material dwords come in a second
 // round trip.  x neighbours come from adjacent lanes.
@@ -1076,28 +1161,36 @@ __device__ __forceinline__ void write_word_nodes(int kind, void *__restrict__ no
 // reference computes a subspace's ids before later subspaces erode
 // (src/host_side.cpp:1183-1206), so neighbours across a +y/+z seam are read
 // from the pre-erosion snapshot `pre` (seam rows flagged in ystart/zstart).
-__global__ __launch_bounds__(256) void k_nodes_cart(GridDims g, const uint32_t *__restrict__ solid,
+__global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs,
+                                                    const uint32_t *__restrict__ node_bases,
+                                                    uint32_t njobs,
                                                     const uint32_t *__restrict__ pre,
                                                     const uint8_t *__restrict__ ystart,
                                                     const uint8_t *__restrict__ zstart,
-                                                    MatSource ms, int use_mats, int kind,
-                                                    void *__restrict__ nodes, BrickDims bd) {
+                                                    const uint8_t *__restrict__ mat_list) {
     __shared__ uint8_t lut[64];
     __shared__ uint4 slot[256][4];  // 64 output bytes per thread (LongNode; ShortNode uses 32)
-    const uint32_t nxc = (g.W + kWave - 1) / kWave, ny4 = (g.Y + 3) / 4;
-    const uint32_t nzc = (g.Z + kNodeZ - 1) / kNodeZ;
-    const uint32_t lb = xcd_logical_block(nxc * ny4 * nzc);
-    if (lb == UINT32_MAX) return;  // uniform over the block
-    const uint32_t tx = threadIdx.x;
-    const uint32_t tid = threadIdx.y * kWave + tx;
+    const uint32_t GL = xcd_logical_block(node_bases[njobs]);  // bases[njobs] = all blocks
+    if (GL == UINT32_MAX) return;  // uniform over the block
+    const Job &J = jobs[job_of(node_bases, njobs, GL)];
+    const GridDims g = J.gd;
+    const uint32_t *__restrict__ solid = J.bits;
+    const MatSource ms{J.surf, J.bmo, J.wrank, mat_list};
+    const int use_mats = J.use_mats, kind = J.kind;
+    void *__restrict__ nodes = J.nodes;
+    const BrickDims bd = J.bdn;
+    const uint32_t P = g.W * g.Y;  // words per plane
+    const uint32_t nch = (P + kChunk - 1u) / kChunk;
+    const uint32_t lb = GL - J.node_base;  // block within the job
+    const uint32_t tid = threadIdx.x, tx = tid & (kWave - 1u);
     if (tid < 64u) lut[tid] = c_orient[tid];
-    const RowBlock rb = row_block(lb, nxc, ny4);
-    const uint32_t zb = rb.z * kNodeZ;
-    const uint32_t w = rb.xc * kWave + tx;
-    const uint32_t y = rb.y4 * 4 + threadIdx.y;
-    const bool valid = w < g.W && y < g.Y;
-    const size_t rowW = g.W, planeW = (size_t)g.W * g.Y;
-    const size_t col0 = valid ? (size_t)y * rowW + w : 0;
+    const uint32_t zb = (lb / nch) * kNodeZ;
+    const uint32_t i = (lb % nch) * kChunk + tid;
+    const bool valid = i < P;
+    const uint32_t y = valid ? i / g.W : 0u;
+    const uint32_t w = valid ? i - y * g.W : 0u;
+    const size_t rowW = g.W, planeW = P;
+    const size_t col0 = valid ? i : 0;
     const bool hyn = valid && y > 0, hyp = valid && y + 1 < g.Y;
     // Round trip 1.
     uint32_t col[kNodeZ + 2], yn[kNodeZ], yp[kNodeZ], em[kNodeZ], ep[kNodeZ];
@@ -1185,6 +1278,8 @@ __global__ __launch_bounds__(256) void k_nodes_cart(GridDims g, const uint32_t *
         uint32_t gp = (uint32_t)__shfl_down((int)G, 1);
         if (tx == 0) gm = em[k];
         if (tx == kWave - 1) gp = ep[k];
+        if (w == 0) gm = 0u;
+        if (w + 1 >= g.W) gp = 0u;
         if (!valid || z >= g.Z) continue;
         const uint32_t L = (G >> 1) | (gm << 31);
         const uint32_t R = (G << 1) | (gp >> 31);
@@ -1246,16 +1341,27 @@ __device__ __forceinline__ void lr_shift(const FccGrids &G, const GridDims &g, u
 // FCC node write, row-major like k_nodes_cart: the 12-bit neighbour mask of
 // calculateFCCBoundaries (src/voxelizer.cu:1930-2048) as bitwise selects by
 // the (x+z) parity mask, materials from the four sub-grids' surface lists.
-__global__ __launch_bounds__(256) void k_nodes_fcc(GridDims g, const uint32_t *s1,
-                                                   const uint32_t *s2, const uint32_t *s3,
-                                                   const uint32_t *s4, MatSource m1, MatSource m2,
-                                                   MatSource m3, MatSource m4, int use_mats,
-                                                   int kind, void *__restrict__ nodes,
-                                                   BrickDims bd) {
+__global__ __launch_bounds__(256) void k_nodes_fcc(const Job *__restrict__ jobs,
+                                                   const uint32_t *__restrict__ fcc_first,
+                                                   const uint32_t *__restrict__ fcc_bases,
+                                                   uint32_t nfcc,
+                                                   const uint8_t *__restrict__ mat_list) {
+    const uint32_t GL = xcd_logical_block(fcc_bases[nfcc]);  // bases[nfcc] = all blocks
+    if (GL == UINT32_MAX) return;
+    const uint32_t f = job_of(fcc_bases, nfcc, GL);
+    const Job *__restrict__ Js = jobs + fcc_first[f];
+    const GridDims g = Js[0].gd;
+    const int use_mats = Js[0].use_mats, kind = Js[0].kind;
+    void *__restrict__ nodes = Js[0].nodes;
+    const BrickDims bd = Js[0].bdn;
+    const uint32_t *s1 = Js[0].bits, *s2 = Js[1].bits, *s3 = Js[2].bits, *s4 = Js[3].bits;
+    const MatSource m1{Js[0].surf, Js[0].bmo, Js[0].wrank, mat_list};
+    const MatSource m2{Js[1].surf, Js[1].bmo, Js[1].wrank, mat_list};
+    const MatSource m3{Js[2].surf, Js[2].bmo, Js[2].wrank, mat_list};
+    const MatSource m4{Js[3].surf, Js[3].bmo, Js[3].wrank, mat_list};
     const uint32_t FW = 2 * g.W, FY = g.Y;
     const uint32_t nxc = (FW + kWave - 1) / kWave, ny4 = (FY + 3) / 4;
-    const uint32_t L = xcd_logical_block(nxc * ny4 * 2 * g.Z);
-    if (L == UINT32_MAX) return;
+    const uint32_t L = GL - fcc_bases[f];  // block within the lattice
     const RowBlock rb = row_block(L, nxc, ny4);
     const int64_t fw = (int64_t)rb.xc * kWave + threadIdx.x;
     const int64_t y = (int64_t)rb.y4 * 4 + threadIdx.y;
