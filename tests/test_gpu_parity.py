@@ -126,6 +126,20 @@ def test_solid_bits_shift_and_splits(gpu_ctx, oracle, shift):
         assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("kind", [va.NODE_LONG, va.NODE_LONG_FCC])
+def test_material_lookup_path(oracle, monkeypatch, kind):
+    """Meshes above the key-embedding limit look winner materials up in
+    memory instead of carrying them in the winner key; the limit is lowered
+    to 0 here so the room takes that path."""
+    monkeypatch.setenv("VOX_EMBED_TRIS", "0")
+    ctx = va.Context(0)
+    try:
+        m = W.shoebox_room(seed=3, n_boxes=24, subdiv=3)
+        check_nodes(ctx, oracle, m, va.PLAN_CUBE_LENGTH, 0.012, kind)
+    finally:
+        ctx.close()
+
+
 def test_repeatable(gpu_ctx):
     m = W.shoebox_room(seed=3, n_boxes=12, subdiv=3)
     load(gpu_ctx, m)

@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -77,7 +78,6 @@ struct vox_ctx {
     bool timing = false;
     hipEvent_t ev[8] = {};
     vox_stage_times times = {};
-    uint32_t mat_slot_cap = 1u << 16;  // bytes per material arena (multiple of 4)
     // persistent surface launch: compute units and resident bricks per CU
     int n_cu = 0;
     int surf_blocks_per_cu = 0;
@@ -100,6 +100,10 @@ struct vox_ctx {
     hipEvent_t up_ev = nullptr;
     // surface brick shapes (voxels along x, y, z)
     int cart_bdim[3] = {kCartBX, kCartBY, kCartBZ};
+    // meshes with fewer triangles carry materials in the surface pass's
+    // winner keys (at most 2^22; VOX_EMBED_TRIS lowers it, for tests of the
+    // other path)
+    uint32_t embed_limit = 1u << 22;
     int fcc_bdim[3] = {kFccBX, kFccBY, kFccBZ};
 };
 
@@ -230,9 +234,9 @@ struct Plan {
     std::vector<SplitTable> splits;  // one per job
     std::vector<uint32_t> tri_b{0}, tile_b{0}, pad_b{0}, brick_b{0}, erode_b{0}, node_b{0};
     std::vector<uint32_t> fcc_first, fcc_b{0};
-    std::vector<size_t> bits_off, surf_off, wrank_off;  // into the call's arenas
-    std::vector<uint32_t *> ext_bits;                   // caller's bit array or null
-    size_t bits_words = 0, surf_words = 0, wrank_n = 0;
+    std::vector<size_t> bits_off, surf_off, mat_off;  // into the call's arenas
+    std::vector<uint32_t *> ext_bits;                 // caller's bit array or null
+    size_t bits_words = 0, surf_words = 0, mat_bytes = 0;
     uint32_t maxW = 0;
     bool any_mats = false;
 };
@@ -303,6 +307,7 @@ vox_status add_job(vox_ctx *ctx, Plan &P, const float *V, const uint32_t *I, uin
         nbt *= J.nb[a];
     }
     J.use_mats = use_mats ? 1 : 0;
+    J.embed = nt < ctx->embed_limit ? 1 : 0;
     J.brick_base = P.brick_b.back();
     P.brick_b.push_back(J.brick_base + (use_mats ? nbt : 0u));
     J.bdn = brick_dims(g, bdim);
@@ -320,10 +325,10 @@ vox_status add_job(vox_ctx *ctx, Plan &P, const float *V, const uint32_t *I, uin
     P.bits_off.push_back(P.bits_words);
     if (!ext_bits) P.bits_words += words;
     P.surf_off.push_back(P.surf_words);
-    P.wrank_off.push_back(P.wrank_n);
+    P.mat_off.push_back(P.mat_bytes);
     if (use_mats) {
         P.surf_words += words;
-        P.wrank_n += (size_t)nbt * (size_t)(bdim[0] * bdim[1] * bdim[2] / 32);
+        P.mat_bytes += (size_t)nbt * (size_t)(bdim[0] * bdim[1] * bdim[2]);  // 1 B per voxel
         P.any_mats = true;
     }
     P.maxW = std::max(P.maxW, W);
@@ -378,13 +383,13 @@ vox_status upload(vox_ctx *ctx, void *dst, const void *src, size_t bytes) {
 // tables, base arrays).
 vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D) {
     const uint32_t nj = (uint32_t)P.jobs.size(), nf = (uint32_t)P.fcc_first.size();
-    uint32_t *bits = nullptr, *surf = nullptr, *bmo = nullptr;
-    uint16_t *wrank = nullptr;
+    uint32_t *bits = nullptr, *surf = nullptr, *bhist = nullptr;
+    uint8_t *matgrid = nullptr;
     if (P.bits_words) VOX_TRY(buf(ctx, "a_bits", P.bits_words, &bits));
     if (P.any_mats) {
         VOX_TRY(buf(ctx, "a_surf", P.surf_words, &surf));
-        VOX_TRY(buf(ctx, "a_wrank", P.wrank_n, &wrank));
-        VOX_TRY(buf(ctx, "a_bmo", P.brick_b.back(), &bmo));
+        VOX_TRY(buf(ctx, "a_matgrid", P.mat_bytes, &matgrid));
+        VOX_TRY(buf(ctx, "f_brick_hist", P.brick_b.back(), &bhist));
     }
     std::vector<uint32_t> bases;
     for (auto *v : {&P.tri_b, &P.tile_b, &P.pad_b, &P.brick_b, &P.erode_b, &P.node_b})
@@ -403,8 +408,8 @@ vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D) {
         J.splits = dsplit + k;
         if (J.use_mats) {
             J.surf = surf + P.surf_off[k];
-            J.wrank = wrank + P.wrank_off[k];
-            J.bmo = bmo + J.brick_base;
+            J.bhist = bhist + J.brick_base;
+            J.matgrid = matgrid + P.mat_off[k];
         }
     }
     std::vector<uint8_t> host(jb + sb + bb);
@@ -517,10 +522,10 @@ vox_status record_samples(vox_ctx *ctx, const uint32_t *list, const uint32_t *n_
 // ---------------------------------------------------------------------------
 // Surface stage over the jobs that want materials: triangle classes and
 // brick boxes, counting sort of (triangle, brick) pairs by brick, the brick
-// order (plain or learned), the persistent brick pass.  Returns the material
-// byte list the node passes read.
+// order (plain or learned), the persistent brick pass, which leaves the
+// surface words and the material grid the node passes read.
 // ---------------------------------------------------------------------------
-vox_status surface_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, uint8_t **mat_list) {
+vox_status surface_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
     const uint32_t nj = (uint32_t)P.jobs.size();
     const uint32_t nslots = P.tri_b.back(), nbt = P.brick_b.back();
     uint4 *tri_info, *tri_ext = nullptr;
@@ -536,7 +541,7 @@ vox_status surface_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, uint8_t 
     VOX_TRY(buf(ctx, "f_brick_hist", nbt, &brick_hist));
     VOX_TRY(buf(ctx, "f_brick_off", nbt + 1, &brick_off));
     VOX_TRY(buf(ctx, "f_brick_cursor", nbt, &brick_cursor));
-    VOX_TRY(buf(ctx, "f_stats", kMatSlots * kSlotStride + 1, &stats));
+    VOX_TRY(buf(ctx, "f_stats", 1, &stats));
     if (nslots > 0) {
         k_surf_setup<<<grid_for(nslots, 256), 256, 0, ctx->stream>>>(D.jobs, D.tri_b, nj, nslots,
                                                                       tri_info, cand, tri_ext,
@@ -636,38 +641,20 @@ vox_status surface_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, uint8_t 
         brick_list = sorted;
     }
     if (ctx->record) VOX_TRY(buf(ctx, "f_cycles", nbt, &cycles));
-    for (int attempt = 0; attempt < 3; attempt++) {
-        const uint32_t slot_cap = ctx->mat_slot_cap;
-        VOX_TRY(buf(ctx, "f_mats", (size_t)slot_cap * kMatSlots, mat_list));
-        VOX_TRY(fill(ctx, stats, kMatSlots * kSlotStride + 1, 0u));
-        uint32_t *overflow = stats + kMatSlots * kSlotStride;
-        // Empty bricks are not visited: mark them up front.
-        uint32_t *bmo_all;
-        VOX_TRY(buf(ctx, "a_bmo", nbt, &bmo_all));
-        VOX_TRY(fill(ctx, bmo_all, nbt, UINT32_MAX));
-        VOX_TRY(fill(ctx, next, 1, 0u));
-        const uint32_t blocks =
-            std::min<uint32_t>(nbt, (uint32_t)(ctx->n_cu * ctx->surf_blocks_per_cu));
-        k_surf_brick<<<blocks, 256, 0, ctx->stream>>>(
-            D.jobs, D.brick_b, nj, tri_rec, brick_off, brick_hist, blist, *mat_list, stats,
-            slot_cap, overflow, brick_list, n_list, next, cycles);
-        VOX_HIP(hipGetLastError());
-        std::vector<uint32_t> h(kMatSlots * kSlotStride + 1);
-        VOX_TRY(read_u32(ctx, stats, h.size(), h.data()));
-        uint32_t used = 0;
-        uint64_t vox = 0;
-        for (uint32_t k = 0; k < kMatSlots; k++) {
-            used = std::max(used, h[k * kSlotStride]);
-            vox += h[k * kSlotStride + 1];
-        }
-        if (h.back() == 0u) {
-            if (ctx->timing) ctx->times.surface_voxels += (uint32_t)vox;
-            if (ctx->record) VOX_TRY(record_samples(ctx, blist_order, n_list, feats, cycles, nbt));
-            return VOX_OK;
-        }
-        ctx->mat_slot_cap = ((used + used / 4 + 1024u) + 3u) & ~3u;
+    VOX_TRY(fill(ctx, stats, 1, 0u));
+    VOX_TRY(fill(ctx, next, 1, 0u));
+    const uint32_t blocks = std::min<uint32_t>(nbt, (uint32_t)(ctx->n_cu * ctx->surf_blocks_per_cu));
+    k_surf_brick<<<blocks, 256, 0, ctx->stream>>>(D.jobs, D.brick_b, nj, tri_rec, brick_off,
+                                                  brick_hist, blist, ctx->timing ? stats : nullptr,
+                                                  brick_list, n_list, next, cycles);
+    VOX_HIP(hipGetLastError());
+    if (ctx->timing) {
+        uint32_t vox = 0;
+        VOX_TRY(read_u32(ctx, stats, 1, &vox));
+        ctx->times.surface_voxels += vox;
     }
-    return fail(ctx, VOX_ERR_OOM, "surface material list overflow");
+    if (ctx->record) VOX_TRY(record_samples(ctx, blist_order, n_list, feats, cycles, nbt));
+    return VOX_OK;
 }
 
 // Erosion to the fixed point over every Cartesian job, in place.  A single
@@ -729,16 +716,16 @@ vox_status erode_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, uint32_t s
 // subspaces (single job only) the ids next to a later subspace read the
 // pre-erosion grid (see k_nodes_cart).
 vox_status node_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, const uint32_t *pre,
-                      const uint8_t *ystart, const uint8_t *zstart, const uint8_t *mat_list) {
+                      const uint8_t *ystart, const uint8_t *zstart) {
     const uint32_t nj = (uint32_t)P.jobs.size(), nf = (uint32_t)P.fcc_first.size();
     if (P.node_b.back() > 0) {
         k_nodes_cart<<<xcd_grid(P.node_b.back()), 256, 0, ctx->stream>>>(
-            D.jobs, D.node_b, nj, pre, ystart, zstart, mat_list);
+            D.jobs, D.node_b, nj, pre, ystart, zstart);
         VOX_HIP(hipGetLastError());
     }
     if (nf > 0) {
         k_nodes_fcc<<<xcd_grid(P.fcc_b.back()), dim3(kWave, 4), 0, ctx->stream>>>(
-            D.jobs, D.fcc_first, D.fcc_b, nf, mat_list);
+            D.jobs, D.fcc_first, D.fcc_b, nf);
         VOX_HIP(hipGetLastError());
     }
     return VOX_OK;
@@ -752,8 +739,7 @@ vox_status run_plan(vox_ctx *ctx, Plan &P, uint32_t splitY, uint32_t splitZ) {
     VOX_TRY(plan_upload(ctx, P, D));
     VOX_TRY(solid_stage(ctx, P, D));
     record(ctx, 1);
-    uint8_t *mat_list = nullptr;
-    if (P.any_mats) VOX_TRY(surface_stage(ctx, P, D, &mat_list));
+    if (P.any_mats) VOX_TRY(surface_stage(ctx, P, D));
     record(ctx, 2);
     uint32_t *pre = nullptr;
     uint8_t *ystart = nullptr, *zstart = nullptr;
@@ -777,7 +763,7 @@ vox_status run_plan(vox_ctx *ctx, Plan &P, uint32_t splitY, uint32_t splitZ) {
     }
     VOX_TRY(erode_stage(ctx, P, D, splitY, splitZ));
     record(ctx, 3);
-    VOX_TRY(node_stage(ctx, P, D, pre, ystart, zstart, mat_list));
+    VOX_TRY(node_stage(ctx, P, D, pre, ystart, zstart));
     record(ctx, 4);
     return VOX_OK;
 }
@@ -834,6 +820,8 @@ vox_status vox_create(int device, vox_ctx **out) {
         return VOX_ERR_HIP;
     }
     ctx->n_cu = std::max(ctx->n_cu, 1);
+    if (const char *e = std::getenv("VOX_EMBED_TRIS"))
+        ctx->embed_limit = (uint32_t)std::min<unsigned long>(std::strtoul(e, nullptr, 10), 1ul << 22);
     ctx->surf_blocks_per_cu = std::max(ctx->surf_blocks_per_cu, 1);
     uint8_t lut[64] = {0};
     for (auto &pr : kOrientPairs) lut[pr[0]] = pr[1];

@@ -19,6 +19,18 @@ namespace vk {
 constexpr int kWave = 64;
 constexpr uint32_t kXcds = 8;  // MI355X: 8 XCDs, each with its own L2
 
+// Global-memory view of a pointer read from a job table.  Pointers the
+// compiler cannot place in an address space are accessed with flat_*
+// instructions, which also count against the LDS wait counter, so every
+// later LDS read waits for them; global_* accesses do not.
+#define VK_G __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ VK_G T *gptr(T *p) {
+    return (VK_G T *)p;
+}
+// Native 4 x u32 vector for 16-byte global accesses through VK_G pointers.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 // Workgroups are dispatched round-robin over the XCDs.  Row kernels launch
 // kXcds * ceil(n / kXcds) workgroups and map workgroup i to logical block
 // (i % kXcds) * per + i / kXcds, so each XCD walks a contiguous run of rows
@@ -78,9 +90,6 @@ constexpr uint32_t kErodeZ = 8;
 // thread's planes share one surface brick.
 constexpr uint32_t kNodeZ = 4;
 static_assert(kCartBZ % kNodeZ == 0, "node planes must not straddle a brick");  // z >= 4
-// Material-list arenas of the surface stage: kMatSlots counters, each on its
-// own 128-byte line (kSlotStride words): {bytes used, surface voxels}.
-constexpr uint32_t kMatSlots = 64, kSlotStride = 32;
 // Cost model: features per brick, MLP parameters (16-64-64-1), sort buckets.
 constexpr int kCostFeatures = 16;
 constexpr int kCostParams = 64 * 16 + 64 + 64 * 64 + 64 + 64 + 1;  // 5313
@@ -134,9 +143,10 @@ struct Job {
     BrickDims bdn;         // brick geometry for the node passes
     const SplitTable *splits;  // matSplitRes intervals
     uint32_t *surf;        // surface bit words
-    uint32_t *bmo;         // per brick: offset of its materials, UINT32_MAX if none
-    uint16_t *wrank;       // per brick word: byte offset of its material segment
+    const uint32_t *bhist; // per brick: (triangle, brick) pairs; 0 = not visited
+    uint8_t *matgrid;      // per voxel, brick-major: winner material (surface voxels)
     int32_t use_mats;      // surface stage and materials wanted
+    int32_t embed;         // winner keys carry the material (mesh-local index < 2^22)
     // erosion and nodes (Cartesian jobs; FCC sub-grids have no such blocks)
     uint32_t erode_base, node_base;
     void *nodes;
@@ -334,9 +344,8 @@ __global__ void k_surf_brick(const Job *__restrict__ jobs, const uint32_t *__res
                              uint32_t njobs, const float4 *__restrict__ tri_rec,
                              const uint32_t *__restrict__ brick_off,
                              const uint32_t *__restrict__ brick_hist,
-                             const uint32_t *__restrict__ blist, uint8_t *__restrict__ mat_list,
-                             uint32_t *__restrict__ slots, uint32_t slot_cap,
-                             uint32_t *__restrict__ overflow, const uint32_t *__restrict__ order,
+                             const uint32_t *__restrict__ blist, uint32_t *__restrict__ stats,
+                             const uint32_t *__restrict__ order,
                              const uint32_t *__restrict__ n_order, uint32_t *__restrict__ next,
                              uint64_t *__restrict__ cycles);
 __global__ void k_nonzero_flags(const uint32_t *__restrict__ hist, uint32_t *__restrict__ flags,
@@ -349,7 +358,7 @@ __global__ void k_nonzero_list(const uint32_t *__restrict__ hist, const uint32_t
 // ---------------------------------------------------------------------------
 // Erosion pass over every Cartesian job's row blocks (grid = all blocks);
 // voxels outside [y0,y1) x [z0,z1) are left alone.  changes[pass * njobs + j]
-// counts job j's changed waves, totals[pass] all jobs'.
+// is set to 1 if pass `pass` changed job j, totals[pass] if it changed any.
 __global__ void k_erode(const Job *__restrict__ jobs, const uint32_t *__restrict__ erode_bases,
                         uint32_t njobs, uint32_t y0, uint32_t y1, uint32_t z0, uint32_t z1,
                         uint32_t *__restrict__ stamp, uint32_t *__restrict__ changes,
@@ -361,12 +370,19 @@ inline uint32_t erode_blocks(const GridDims &g) {
 // Boundary-id table (permutation -> id) in constant memory.
 hipError_t set_orient_table(const uint8_t lut[64]);
 
+// Surface data of one grid as the node passes read it.  Brick b's voxels
+// occupy bytes [b * nvox, (b + 1) * nvox) of matgrid in the brick's word
+// order (z, y, x-word), 32 bytes per word; bytes of non-surface voxels are
+// never written.  Bricks the surface pass did not visit (bhist == 0) hold
+// stale surface words, so the readers check bhist first.
 struct MatSource {
-    const uint32_t *surf;          // surface bit words (same layout as bits)
-    const uint32_t *brick_mat_off; // per brick offset into mat_list or UINT32_MAX
-    const uint16_t *wrank;         // per brick word: byte offset of its material segment in the brick
-    const uint8_t *mat_list;
+    const VK_G uint32_t *surf;   // surface bit words (same layout as bits)
+    const VK_G uint32_t *bhist;  // per brick: nonzero if visited
+    const VK_G uint8_t *matgrid;
 };
+__device__ __forceinline__ MatSource mat_source(const Job &J) {
+    return MatSource{gptr(J.surf), gptr(J.bhist), gptr(J.matgrid)};
+}
 
 // Node passes.  Cartesian: every Cartesian job's row blocks (grid = all);
 // pre/ystart/zstart only with a single job.  FCC: per FCC lattice f, its
@@ -375,14 +391,12 @@ struct MatSource {
 // fcc_bases[f].
 __global__ void k_nodes_cart(const Job *__restrict__ jobs, const uint32_t *__restrict__ node_bases,
                              uint32_t njobs, const uint32_t *__restrict__ pre,
-                             const uint8_t *__restrict__ ystart, const uint8_t *__restrict__ zstart,
-                             const uint8_t *__restrict__ mat_list);
+                             const uint8_t *__restrict__ ystart, const uint8_t *__restrict__ zstart);
 inline uint32_t node_blocks(const GridDims &g) {
     return plane_chunks(g.W, g.Y) * ((g.Z + kNodeZ - 1u) / kNodeZ);
 }
 __global__ void k_nodes_fcc(const Job *__restrict__ jobs, const uint32_t *__restrict__ fcc_first,
-                            const uint32_t *__restrict__ fcc_bases, uint32_t nfcc,
-                            const uint8_t *__restrict__ mat_list);
+                            const uint32_t *__restrict__ fcc_bases, uint32_t nfcc);
 inline uint32_t fcc_node_blocks(const GridDims &g) {
     return ((2u * g.W + kWave - 1u) / kWave) * ((g.Y + 3u) / 4u) * 2u * g.Z;
 }

@@ -4,6 +4,8 @@
 // this library's own.
 #include "vox_kernels.h"
 
+#include <type_traits>
+
 namespace vk {
 
 // ---------------------------------------------------------------------------
@@ -118,13 +120,14 @@ __global__ __launch_bounds__(256) void k_tri_records(const Job *__restrict__ job
         const uint32_t t = t0 + threadIdx.x, lt = t - J.tri_base;
         tri_job[t] = j;
         if (lt >= J.nt) continue;  // padding slot
-        const float *V = J.V;
-        const uint32_t *I = J.I;
+        const VK_G float *V = gptr(J.V);
+        const VK_G uint32_t *I = gptr(J.I);
         const uint32_t a = I[3u * lt], b = I[3u * lt + 1u], c = I[3u * lt + 2u];
         rec[3 * (size_t)t] = make_float4(V[3u * a], V[3u * a + 1u], V[3u * a + 2u], V[3u * b]);
         rec[3 * (size_t)t + 1] =
             make_float4(V[3u * b + 1u], V[3u * b + 2u], V[3u * c], V[3u * c + 1u]);
-        rec[3 * (size_t)t + 2] = make_float4(V[3u * c + 2u], 0.0f, 0.0f, 0.0f);
+        const uint32_t mat = J.mats ? (uint32_t)gptr(J.mats)[lt] : 0u;
+        rec[3 * (size_t)t + 2] = make_float4(V[3u * c + 2u], 0.0f, __uint_as_float(mat), 0.0f);
     }
 }
 
@@ -277,7 +280,7 @@ __global__ __launch_bounds__(kWave) void k_solid_fill(const Job *__restrict__ jo
     extern __shared__ uint32_t cross[];  // [16][W]
     const Job &J = jobs[job_of(tile_bases, njobs, blockIdx.x)];
     const SolidParams p = solid_params(J, rec);
-    uint32_t *__restrict__ bits = J.bits;
+    VK_G uint32_t *__restrict__ bits = gptr(J.bits);
     const uint32_t W = p.X >> 5;
     const uint32_t tid = threadIdx.x;
     const uint32_t local = blockIdx.x - J.tile_base;
@@ -347,7 +350,7 @@ __global__ __launch_bounds__(256) void k_zero_pad_rows(const Job *__restrict__ j
         y = q >> 1;
         z = (q & 1u) ? rowsZ - 1u : 0u;
     }
-    J.bits[((size_t)z * rowsY + y) * W + w] = 0u;
+    gptr(J.bits)[((size_t)z * rowsY + y) * W + w] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -505,6 +508,17 @@ template __global__ void k_surf_bin<true>(const Job *, const uint32_t *, uint32_
                                           uint32_t *, const uint32_t *, uint32_t *, uint32_t *,
                                           int);
 
+// Winner key of a (triangle, voxel) hit; the largest key of a voxel wins
+// (bbType first, then the mesh-local triangle index, as the oracle's rule).
+// With `embed` (meshes of fewer than 2^22 triangles) the key carries the
+// triangle's material in its low byte (the index is unique, so the byte
+// never decides), and the brick pass reads no materials from memory; larger
+// meshes keep the plain index and look the material up.
+__device__ __forceinline__ uint32_t winner_key(uint32_t bb, uint32_t lt, uint32_t mat,
+                                               bool embed) {
+    return (bb << 30) | (embed ? (lt << 8) | (mat & 0xFFu) : lt);
+}
+
 // Per-triangle data of one brick's chunk, set up once in LDS.  Coordinates
 // are named (u, v, a): a is the triangle's dominant axis, u and v the other
 // two in increasing axis order (u = x unless a = x; v = z unless a = z).
@@ -522,7 +536,7 @@ struct SurfTri {
     int32_t wu;                // columns along u
     int32_t ngu;               // work items along u (kSurfCols columns each)
     float rngu;                // 1 / ngu
-    uint32_t key;              // bbType<<30 | triangle
+    uint32_t key;              // LDS winner key (winner_key)
     uint32_t flags;            // bb | axis<<2 | (plane0 other is v)<<4 | (plane1 other is v)<<5
 };
 constexpr int kSurfChunk = 64;
@@ -538,6 +552,7 @@ constexpr int kSurfCols = 1;
 // the axis for 1D, dominant-plane columns for 2D/3D.
 __device__ __forceinline__ uint32_t surf_tri_setup(const SurfParams &p,
                                                    const float4 *__restrict__ tri_rec, uint32_t t,
+                                                   uint32_t tri_base, bool embed,
                                                    int32_t blx, int32_t bly, int32_t blz,
                                                    SurfTri &st) {
     const float4 r0 = tri_rec[3 * (size_t)t], r1 = tri_rec[3 * (size_t)t + 1],
@@ -569,7 +584,7 @@ __device__ __forceinline__ uint32_t surf_tri_setup(const SurfParams &p,
               (lo[2] - (a == AX_X ? blx : a == AX_Y ? bly : blz)) * st.s[2];
     st.vmin_a = pick3i(vb.min, a);
     st.vmax_a = pick3i(vb.max, a);
-    st.key = ((uint32_t)bb << 30) | t;
+    st.key = winner_key((uint32_t)bb, t - tri_base, __float_as_uint(r2.z), embed);
     st.wu = hi[0] - lo[0] + 1;
     st.ngu = (st.wu + kSurfCols - 1) / kSurfCols;
     st.rngu = 1.0f / (float)st.ngu;
@@ -724,26 +739,24 @@ __device__ __forceinline__ void surf_item(const SurfParams &p, const SplitTable 
 // items of the chunk are spread over the 256 threads; winners accumulate in
 // LDS (ds_max of bbType<<30 | triangle).  Only rows of the brick that
 // received a key are visited afterwards: one ballot per 64 voxels gives the
-// surface bit words, a wave scan their 4-byte aligned material segments, one
-// lane per surface voxel fetches its material, and the rows' keys are
-// cleared for the next brick.  Surface bits are ORed into the solid grid.
-// With `cycles`, each brick's duration (s_memtime) is recorded.
+// surface bit words, one lane per surface voxel fetches its winner's material
+// into the brick's block of the material grid (one byte per voxel, brick
+// order), and the rows' keys are cleared for the next brick.  Surface bits
+// are ORed into the solid grid.  With `stats`, surface voxels are counted;
+// with `cycles`, each brick's duration (s_memtime) is recorded.
 __global__ __launch_bounds__(256, 5) void k_surf_brick(
     const Job *__restrict__ jobs, const uint32_t *__restrict__ brick_bases, uint32_t njobs,
     const float4 *__restrict__ tri_rec, const uint32_t *__restrict__ brick_off,
     const uint32_t *__restrict__ brick_hist, const uint32_t *__restrict__ blist,
-    uint8_t *__restrict__ mat_list, uint32_t *__restrict__ slots, uint32_t slot_cap,
-    uint32_t *__restrict__ overflow, const uint32_t *__restrict__ order,
+    uint32_t *__restrict__ stats, const uint32_t *__restrict__ order,
     const uint32_t *__restrict__ n_order, uint32_t *__restrict__ next,
     uint64_t *__restrict__ cycles) {
     __shared__ uint32_t keys[kMaxBrickVox];
     __shared__ SurfTri tris[kSurfChunk];
     __shared__ uint32_t colsum[kSurfChunk + 1];
     __shared__ uint32_t wsurf[kMaxBrickVox / 32];
-    __shared__ uint32_t wseg[kMaxBrickVox / 32];  // exclusive scan of segment sizes
-    __shared__ uint32_t wtot[4], wpop[4];
     __shared__ uint8_t touched[32];
-    __shared__ uint32_t sbase, s_q;
+    __shared__ uint32_t s_q;
     __shared__ SplitTable spl;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
     for (int32_t i = tid; i < kMaxBrickVox; i += blockDim.x) keys[i] = 0u;
@@ -753,6 +766,7 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
     __syncthreads();
     uint32_t q = s_q;
     uint32_t cur_job = UINT32_MAX;  // job whose split table is in LDS
+    uint32_t nsurf = 0u;            // surface voxels found by this wave
     while (q < n_list) {  // q is uniform over the block
         uint32_t q_next = 0u;
         if (tid == 0) q_next = atomicAdd(next, 1u);  // grab the next brick early
@@ -764,18 +778,18 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
         const Job &J = jobs[j];
         if (j != cur_job) {  // uniform: the job's matSplitRes table into LDS
             __syncthreads();
-            const uint32_t *src = (const uint32_t *)J.splits;
+            const VK_G uint32_t *src = (const VK_G uint32_t *)gptr(J.splits);
             uint32_t *dst = (uint32_t *)&spl;
             for (uint32_t i = tid; i < sizeof(SplitTable) / 4; i += blockDim.x) dst[i] = src[i];
             cur_job = j;
         }
         const SurfParams p = surf_params(J);
-        const uint8_t *__restrict__ materials = J.mats;
-        uint32_t *__restrict__ bits = J.bits;
+        const VK_G uint8_t *__restrict__ materials = gptr(J.mats);
+        const bool embed = J.embed != 0;
+        VK_G uint32_t *__restrict__ bits = gptr(J.bits);
         const uint32_t b = gb - J.brick_base;  // brick within the job
         const int32_t bd0 = p.bdim[0], bd1 = p.bdim[1], bd2 = p.bdim[2];
         const int32_t nvox = bd0 * bd1 * bd2;
-        const int32_t nrows = bd1 * bd2;  // <= 32
         const int32_t wpr = bd0 / 32;     // words per row
         const int32_t nwords = nvox / 32;
         const int32_t row_shift = 31 - __clz(bd0);
@@ -807,7 +821,8 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
                 uint32_t items = 0u;
                 if (c0 + tid < cnt) {
                     const uint32_t t = blist[off + c0 + tid];
-                    items = surf_tri_setup(p, tri_rec, t, blx, bly, blz, tris[tid]);
+                    items = surf_tri_setup(p, tri_rec, t, J.tri_base, embed, blx, bly, blz,
+                                           tris[tid]);
                 }
                 uint32_t incl = items;  // wave-level inclusive scan
                 for (int d = 1; d < kSurfChunk; d <<= 1) {
@@ -830,81 +845,47 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
             }
         }
         __syncthreads();
-        // Surface bit words of the touched rows (MSB-first), zero elsewhere.
-        for (int32_t r = (int32_t)wave; r < nrows; r += 4) {
-            if (!touched[r]) {
-                if ((int32_t)lane < wpr) wsurf[r * wpr + lane] = 0u;
-                continue;
-            }
-            for (int32_t h = 0; h < bd0 / 64; h++) {
-                const unsigned long long m = __ballot(keys[r * bd0 + h * 64 + lane] != 0u);
-                if (lane == 0) {
-                    wsurf[r * wpr + 2 * h] = __brev((uint32_t)m);
-                    wsurf[r * wpr + 2 * h + 1] = __brev((uint32_t)(m >> 32));
+        // Surface bit words and winner materials: one lane per voxel of the
+        // touched rows, in 64-voxel groups taken by the waves in turn (a
+        // group lies in one row).  Every lane of a touched group writes its
+        // voxel's byte of the brick's block in the material grid (the
+        // winner's material, or 0), so the grid sees whole 64-byte
+        // segments.  Keys are cleared for the next brick.  The two loops
+        // differ in where the material comes from: with materials in the
+        // keys the loop issues no loads, so its stores never wait (on this
+        // architecture a wait for a load also waits for earlier stores).
+        VK_G uint8_t *__restrict__ mg = gptr(J.matgrid) + (size_t)b * (uint32_t)nvox;
+        auto rows_out = [&](auto embed_c) {
+            constexpr bool kEmbedded = decltype(embed_c)::value;
+            for (int32_t v0 = (int32_t)wave * 64; v0 < nvox; v0 += 256) {
+                if (!touched[v0 >> row_shift]) {
+                    if (lane < 2u) wsurf[(v0 >> 5) + (int32_t)lane] = 0u;
+                    continue;
                 }
+                const uint32_t key = keys[v0 + (int32_t)lane];
+                const unsigned long long m = __ballot(key != 0u);
+                if (lane == 0) {
+                    wsurf[v0 >> 5] = __brev((uint32_t)m);
+                    wsurf[(v0 >> 5) + 1] = __brev((uint32_t)(m >> 32));
+                }
+                nsurf += (uint32_t)__popcll(m);
+                uint32_t mat = kEmbedded ? key & 0xFFu : 0u;
+                if (key) {
+                    keys[v0 + (int32_t)lane] = 0u;
+                    if (!kEmbedded && materials) mat = materials[key & 0x3FFFFFFFu];
+                }
+                mg[v0 + (int32_t)lane] = (uint8_t)mat;
             }
-        }
+        };
+        if (embed)
+            rows_out(std::integral_constant<bool, true>{});
+        else
+            rows_out(std::integral_constant<bool, false>{});
         __syncthreads();
-        // Exclusive scan of the words' segment sizes (popcount rounded up to
-        // 4): wave scans, then the wave totals.
-        uint32_t pop = 0u, seg = 0u, incl = 0u;
-        if ((int32_t)tid < nwords) {
-            pop = (uint32_t)__popc(wsurf[tid]);
-            seg = (pop + 3u) & ~3u;
-        }
-        incl = seg;
-        for (int d = 1; d < kWave; d <<= 1) {
-            const uint32_t o = (uint32_t)__shfl_up((int)incl, d);
-            if ((int)lane >= d) incl += o;
-        }
-        uint32_t npop = pop;
-        for (int d = kWave / 2; d >= 1; d >>= 1) npop += (uint32_t)__shfl_xor((int)npop, d);
-        if (lane == kWave - 1) wtot[wave] = incl;
-        if (lane == 0) wpop[wave] = npop;
-        __syncthreads();
-        uint32_t wbase = 0u;
-        for (uint32_t k = 0; k < wave; k++) wbase += wtot[k];
-        if ((int32_t)tid < nwords) wseg[tid] = wbase + incl - seg;
-        if (tid == 0) {
-            // Material space from one of kMatSlots arenas (brick b uses arena
-            // b % kMatSlots), so concurrent bricks bump different counters.
-            const uint32_t acc = wtot[0] + wtot[1] + wtot[2] + wtot[3];
-            const uint32_t sl = gb % kMatSlots;
-            const uint32_t local = atomicAdd(&slots[sl * kSlotStride], acc);
-            atomicAdd(&slots[sl * kSlotStride + 1], wpop[0] + wpop[1] + wpop[2] + wpop[3]);
-            uint32_t base = sl * slot_cap + local;
-            if (local + acc > slot_cap) {
-                atomicOr(overflow, 1u);  // the caller retries with larger arenas
-                base = UINT32_MAX;
-            }
-            sbase = base;
-            J.bmo[b] = base;
-        }
-        __syncthreads();
-        const uint32_t base = sbase;
         if ((int32_t)tid < nwords && gword) {
             const uint32_t sw = wsurf[tid];
-            J.surf[gi] = sw;
-            J.wrank[(size_t)b * nwords + tid] = (uint16_t)wseg[tid];
+            gptr(J.surf)[gi] = sw;
             if (sw) bits[gi] = bw | sw;
-        }
-        // Materials: one lane per voxel of a touched row; keys cleared.
-        for (int32_t r = (int32_t)wave; r < nrows; r += 4) {
-            if (!touched[r]) continue;
-            for (int32_t h = 0; h < bd0 / 64; h++) {
-                const int32_t vi = r * bd0 + h * 64 + (int32_t)lane;
-                const uint32_t key = keys[vi];
-                if (key) {
-                    keys[vi] = 0u;
-                    if (base != UINT32_MAX) {
-                        const int32_t w = vi >> 5, qx = vi & 31;
-                        const uint32_t before = wsurf[w] & ~(0xFFFFFFFFu >> qx);
-                        mat_list[base + wseg[w] + (uint32_t)__popc(before)] =
-                            materials ? materials[(key & 0x3FFFFFFFu) - J.tri_base]
-                                      : (uint8_t)0;
-                    }
-                }
-            }
         }
         if (cycles && tid == 0) cycles[gb] = __builtin_amdgcn_s_memtime() - t_start;
         __syncthreads();
@@ -913,6 +894,9 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
         __syncthreads();
         q = s_q;
     }
+    // One atomic per wave at exit: a shared counter bumped per brick would
+    // serialise every workgroup on one address.
+    if (stats && lane == 0 && nsurf) atomicAdd(stats, nsurf);
 }
 
 // Non-empty bricks in index order: flags, then (after an exclusive scan of
@@ -956,7 +940,7 @@ __global__ __launch_bounds__(256) void k_erode(const Job *__restrict__ jobs,
     if (pass > 0 && changes[(pass - 1u) * njobs + j] == 0u) return;
     const Job &J = jobs[j];
     const GridDims g = J.gd;
-    uint32_t *__restrict__ bits = J.bits;
+    VK_G uint32_t *__restrict__ bits = gptr(J.bits);
     const uint32_t P = g.W * g.Y;  // words per plane
     const uint32_t nch = (P + kChunk - 1u) / kChunk;
     const uint32_t nzc = (g.Z + kErodeZ - 1) / kErodeZ;
@@ -1036,10 +1020,12 @@ __global__ __launch_bounds__(256) void k_erode(const Job *__restrict__ jobs,
             changed = true;
         }
     }
-    const unsigned long long m = __ballot(changed);
+    // Flags, not counts: plain stores of 1 (atomics from every changing
+    // wave to one address would serialise).
+    const unsigned long long m = __ballot(changed);  // every lane takes part
     if (lane == 0 && m) {
-        atomicAdd(&changes[pass * njobs + j], (uint32_t)__popcll(m));
-        atomicAdd(&totals[pass], (uint32_t)__popcll(m));
+        changes[pass * njobs + j] = 1u;
+        totals[pass] = 1u;
         stamp[L] = gpass + 1u;
     }
 }
@@ -1068,21 +1054,21 @@ __device__ __forceinline__ uint32_t spread2(uint32_t v) {
 // patched byte-wise in the thread's LDS slot (the 64-entry boundary-id table
 // is in LDS too), which is then stored with 16-byte writes.  md holds the
 // word's first material dword; seg the address of its material segment.
-__device__ __forceinline__ void write_word_nodes(int kind, void *__restrict__ nodes, size_t wi,
+__device__ __forceinline__ void write_word_nodes(int kind, VK_G void *__restrict__ nodes, size_t wi,
                                                  uint32_t G, uint32_t L, uint32_t R, uint32_t Yn,
                                                  uint32_t Yp, uint32_t Zn, uint32_t Zp,
-                                                 uint32_t S, uint32_t md0,
-                                                 const uint32_t *__restrict__ seg,
+                                                 uint32_t S, const uint4 &ma, const uint4 &mb,
                                                  const uint8_t *lut, uint4 *my) {
     const uint32_t interior = G & L & R & Yn & Yp & Zn & Zp;
-    const uint32_t patch = (G & ~interior) | S;
+    const uint32_t bnd = G & ~interior;  // solid, some neighbour missing
+    const bool patch = (bnd | S) != 0u;
     const size_t node0 = wi * 32u;
     const int nb = kind == 0 ? 1 : 2;  // bytes per node
-    uint4 *dst = (uint4 *)((uint8_t *)nodes + node0 * (size_t)nb);
+    VK_G u32x4 *dst = (VK_G u32x4 *)((VK_G uint8_t *)nodes + node0 * (size_t)nb);
     if (!patch && (interior == 0u || interior == 0xFFFFFFFFu)) {
         // All 32 nodes empty, or all fully surrounded (id 27, no material).
         const uint32_t v = interior ? (kind == 0 ? 0x1B1B1B1Bu : 0x001B001Bu) : 0u;
-        const uint4 q = make_uint4(v, v, v, v);
+        const u32x4 q = {v, v, v, v};
         dst[0] = q;
         dst[1] = q;
         if (kind != 0) {
@@ -1091,6 +1077,7 @@ __device__ __forceinline__ void write_word_nodes(int kind, void *__restrict__ no
         }
         return;
     }
+    // Interior voxels (27) and empty ones (0).
     if (kind == 0) {
 #pragma unroll
         for (int j = 0; j < 2; j++) {
@@ -1098,9 +1085,8 @@ __device__ __forceinline__ void write_word_nodes(int kind, void *__restrict__ no
 #pragma unroll
             for (int q = 0; q < 4; q++)
                 o[q] = spread4((interior >> (28 - 4 * (4 * j + q))) & 15u) * 27u;
-            const uint4 v = make_uint4(o[0], o[1], o[2], o[3]);
-            if (patch) my[j] = v;
-            else dst[j] = v;
+            if (patch) my[j] = make_uint4(o[0], o[1], o[2], o[3]);
+            else dst[j] = u32x4{o[0], o[1], o[2], o[3]};
         }
     } else {
 #pragma unroll
@@ -1109,43 +1095,47 @@ __device__ __forceinline__ void write_word_nodes(int kind, void *__restrict__ no
 #pragma unroll
             for (int q = 0; q < 4; q++)
                 o[q] = spread2((interior >> (30 - 2 * (4 * j + q))) & 3u) * 27u;
-            const uint4 v = make_uint4(o[0], o[1], o[2], o[3]);
-            if (patch) my[j] = v;
-            else dst[j] = v;
+            if (patch) my[j] = make_uint4(o[0], o[1], o[2], o[3]);
+            else dst[j] = u32x4{o[0], o[1], o[2], o[3]};
         }
     }
     if (!patch) return;
-    uint32_t m = patch, cur = md0, next = 1;
-    int used = 0;
+    // Boundary voxels: the id of their neighbour permutation.
+    uint32_t m = bnd;
     while (m) {
         const int k = __clz(m);
         const uint32_t bit = 0x80000000u >> k;
-        uint32_t bid = 0u;
-        if (interior & bit) {
-            bid = 27u;
-        } else if (G & bit) {
-            const uint32_t perm = ((L & bit) ? 1u : 0u) | ((R & bit) ? 2u : 0u) |
-                                  ((Yn & bit) ? 4u : 0u) | ((Yp & bit) ? 8u : 0u) |
-                                  ((Zn & bit) ? 16u : 0u) | ((Zp & bit) ? 32u : 0u);
-            bid = lut[perm];
-        }
-        uint32_t mat = 0u;
-        if (S & bit) {
-            mat = cur & 0xFFu;
-            cur >>= 8;
-            if (++used == 4) {  // the segment's next dword, if materials remain
-                used = 0;
-                cur = (S & (bit - 1u)) ? seg[next++] : 0u;
-            }
-        }
+        const uint32_t perm = ((L & bit) ? 1u : 0u) | ((R & bit) ? 2u : 0u) |
+                              ((Yn & bit) ? 4u : 0u) | ((Yp & bit) ? 8u : 0u) |
+                              ((Zn & bit) ? 16u : 0u) | ((Zp & bit) ? 32u : 0u);
+        const uint32_t bid = lut[perm];
         if (kind == 0)
-            ((uint8_t *)my)[k] = (uint8_t)(((mat << 5) & 0xE0u) | (bid & 31u));
+            ((uint8_t *)my)[k] = (uint8_t)(bid & 31u);
         else
-            ((uint16_t *)my)[k] = (uint16_t)(bid | (mat << 8));
+            ((uint16_t *)my)[k] = (uint16_t)bid;
         m &= ~bit;
     }
+    // Surface voxels: their material, four voxels (one material dword) at a
+    // time under a byte mask of the surface bits.
+    if (S) {
+        const uint32_t md[8] = {ma.x, ma.y, ma.z, ma.w, mb.x, mb.y, mb.z, mb.w};
+        uint32_t *my32 = (uint32_t *)my;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t mm = md[j] & (spread4((S >> (28 - 4 * j)) & 15u) * 0xFFu);
+            if (kind == 0) {
+                my32[j] |= (mm << 5) & 0xE0E0E0E0u;
+            } else {
+                my32[2 * j] |= ((mm & 0xFFu) << 8) | ((mm & 0xFF00u) << 16);
+                my32[2 * j + 1] |= ((mm >> 8) & 0xFF00u) | (mm & 0xFF000000u);
+            }
+        }
+    }
     const int n16 = 2 * nb;
-    for (int j = 0; j < n16; j++) dst[j] = my[j];
+    for (int j = 0; j < n16; j++) {
+        const uint4 v = my[j];
+        dst[j] = u32x4{v.x, v.y, v.z, v.w};
+    }
 }
 
 // Cartesian node write: a block takes kChunk consecutive words of a plane
@@ -1166,18 +1156,17 @@ __global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs
                                                     uint32_t njobs,
                                                     const uint32_t *__restrict__ pre,
                                                     const uint8_t *__restrict__ ystart,
-                                                    const uint8_t *__restrict__ zstart,
-                                                    const uint8_t *__restrict__ mat_list) {
+                                                    const uint8_t *__restrict__ zstart) {
     __shared__ uint8_t lut[64];
     __shared__ uint4 slot[256][4];  // 64 output bytes per thread (LongNode; ShortNode uses 32)
     const uint32_t GL = xcd_logical_block(node_bases[njobs]);  // bases[njobs] = all blocks
     if (GL == UINT32_MAX) return;  // uniform over the block
     const Job &J = jobs[job_of(node_bases, njobs, GL)];
     const GridDims g = J.gd;
-    const uint32_t *__restrict__ solid = J.bits;
-    const MatSource ms{J.surf, J.bmo, J.wrank, mat_list};
+    const VK_G uint32_t *__restrict__ solid = gptr(J.bits);
+    const MatSource ms = mat_source(J);
     const int use_mats = J.use_mats, kind = J.kind;
-    void *__restrict__ nodes = J.nodes;
+    VK_G void *__restrict__ nodes = gptr(J.nodes);
     const BrickDims bd = J.bdn;
     const uint32_t P = g.W * g.Y;  // words per plane
     const uint32_t nch = (P + kChunk - 1u) / kChunk;
@@ -1236,18 +1225,19 @@ __global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs
             zs[k] = hz && zstart[z + 1] != 0;
         }
     }
-    uint32_t mb = UINT32_MAX, S[kNodeZ], wr[kNodeZ];
+    uint32_t vis = 0u, S[kNodeZ];
+    size_t moff[kNodeZ];  // material block of each plane's word
 #pragma unroll
     for (uint32_t k = 0; k < kNodeZ; k++) {
         S[k] = 0u;
-        wr[k] = 0u;
+        moff[k] = 0;
     }
     if (use_mats) {  // uniform; the kNodeZ planes share one brick (kNodeZ divides its depth)
         const uint32_t yy = valid ? y : 0u, ww = valid ? w : 0u;
         const uint32_t b =
             ((zb >> bd.lbz) * bd.nby + (yy >> bd.lby)) * bd.nbx + (ww >> bd.lwx);
-        mb = ms.brick_mat_off[b];
-        const uint32_t bwords = 1u << (bd.lwx + bd.lby + bd.lbz);
+        vis = ms.bhist[b];
+        const size_t bbase = (size_t)b << (bd.lwx + bd.lby + bd.lbz + 5u);  // b * voxels
 #pragma unroll
         for (uint32_t k = 0; k < kNodeZ; k++) {
             const uint32_t z = zb + k;
@@ -1256,18 +1246,24 @@ __global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs
                 ((((z & ((1u << bd.lbz) - 1u)) << bd.lby) + (yy & ((1u << bd.lby) - 1u)))
                  << bd.lwx) + (ww & ((1u << bd.lwx) - 1u));
             S[k] = ms.surf[ok ? (size_t)z * planeW + col0 : 0];
-            wr[k] = ms.wrank[(size_t)b * bwords + (ok ? wb : 0)];
+            moff[k] = bbase + (size_t)wb * 32u;
             if (!ok) S[k] = 0u;
         }
     }
     __syncthreads();  // lut ready
-    // Round trip 2: the first material dword of each surface word.
-    uint32_t md[kNodeZ];
-    const uint32_t *mats32 = (const uint32_t *)ms.mat_list;
+    // Round trip 2: the 32 material bytes of each surface word.
+    uint4 ma[kNodeZ], mb[kNodeZ];
 #pragma unroll
     for (uint32_t k = 0; k < kNodeZ; k++) {
-        if (mb == UINT32_MAX) S[k] = 0u;
-        md[k] = S[k] ? mats32[(mb + wr[k]) >> 2] : 0u;
+        if (!vis) S[k] = 0u;
+        ma[k] = make_uint4(0u, 0u, 0u, 0u);
+        mb[k] = ma[k];
+        if (S[k]) {
+            const VK_G u32x4 *src = (const VK_G u32x4 *)(ms.matgrid + moff[k]);
+            const u32x4 x0 = src[0], x1 = src[1];
+            ma[k] = make_uint4(x0.x, x0.y, x0.z, x0.w);
+            mb[k] = make_uint4(x1.x, x1.y, x1.z, x1.w);
+        }
     }
     uint4 *my = slot[tid];
 #pragma unroll
@@ -1285,9 +1281,8 @@ __global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs
         const uint32_t R = (G << 1) | (gp >> 31);
         const uint32_t Yp = ys ? ypp[k] : yp[k];
         const uint32_t Zp = zs[k] ? zpp[k] : col[k + 2];
-        const uint32_t *seg = S[k] ? mats32 + ((mb + wr[k]) >> 2) : mats32;
         write_word_nodes(kind, nodes, (size_t)z * planeW + col0, G, L, R, yn[k], Yp, col[k], Zp,
-                         S[k], md[k], seg, lut, my);
+                         S[k], ma[k], mb[k], lut, my);
     }
 }
 
@@ -1305,7 +1300,7 @@ __device__ __forceinline__ uint32_t spread16(uint32_t x) {
 }
 
 struct FccGrids {
-    const uint32_t *s[4];  // per-grid solid|surface bits (grid 1..4)
+    const VK_G uint32_t *s[4];  // per-grid solid|surface bits (grid 1..4)
 };
 
 // 32 FCC nodes of row (y, fz), x in [32*fw, 32*fw+32): even x from the
@@ -1317,7 +1312,7 @@ __device__ __forceinline__ uint32_t fcc_word(const FccGrids &G, const GridDims &
         fz >= (int64_t)(2 * g.Z))
         return 0u;
     const int odd = (int)(fz & 1);
-    const uint32_t *ge = G.s[odd ? 2 : 0], *go = G.s[odd ? 3 : 1];
+    const VK_G uint32_t *ge = G.s[odd ? 2 : 0], *go = G.s[odd ? 3 : 1];
     const size_t wi = ((size_t)(fz >> 1) * g.Y + (size_t)y) * g.W + (size_t)(fw >> 1);
     const int sh = (fw & 1) ? 0 : 16;
     const uint32_t e = (ge[wi] >> sh) & 0xFFFFu, o = (go[wi] >> sh) & 0xFFFFu;
@@ -1344,21 +1339,19 @@ __device__ __forceinline__ void lr_shift(const FccGrids &G, const GridDims &g, u
 __global__ __launch_bounds__(256) void k_nodes_fcc(const Job *__restrict__ jobs,
                                                    const uint32_t *__restrict__ fcc_first,
                                                    const uint32_t *__restrict__ fcc_bases,
-                                                   uint32_t nfcc,
-                                                   const uint8_t *__restrict__ mat_list) {
+                                                   uint32_t nfcc) {
     const uint32_t GL = xcd_logical_block(fcc_bases[nfcc]);  // bases[nfcc] = all blocks
     if (GL == UINT32_MAX) return;
     const uint32_t f = job_of(fcc_bases, nfcc, GL);
     const Job *__restrict__ Js = jobs + fcc_first[f];
     const GridDims g = Js[0].gd;
     const int use_mats = Js[0].use_mats, kind = Js[0].kind;
-    void *__restrict__ nodes = Js[0].nodes;
+    VK_G void *__restrict__ nodes = gptr(Js[0].nodes);
     const BrickDims bd = Js[0].bdn;
-    const uint32_t *s1 = Js[0].bits, *s2 = Js[1].bits, *s3 = Js[2].bits, *s4 = Js[3].bits;
-    const MatSource m1{Js[0].surf, Js[0].bmo, Js[0].wrank, mat_list};
-    const MatSource m2{Js[1].surf, Js[1].bmo, Js[1].wrank, mat_list};
-    const MatSource m3{Js[2].surf, Js[2].bmo, Js[2].wrank, mat_list};
-    const MatSource m4{Js[3].surf, Js[3].bmo, Js[3].wrank, mat_list};
+    const VK_G uint32_t *s1 = gptr(Js[0].bits), *s2 = gptr(Js[1].bits), *s3 = gptr(Js[2].bits),
+                        *s4 = gptr(Js[3].bits);
+    const MatSource m1 = mat_source(Js[0]), m2 = mat_source(Js[1]), m3 = mat_source(Js[2]),
+                    m4 = mat_source(Js[3]);
     const uint32_t FW = 2 * g.W, FY = g.Y;
     const uint32_t nxc = (FW + kWave - 1) / kWave, ny4 = (FY + 3) / 4;
     const uint32_t L = GL - fcc_bases[f];  // block within the lattice
@@ -1424,22 +1417,21 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(const Job *__restrict__ jobs,
         const uint32_t wb = (uint32_t)(((((gz & ((1u << bd.lbz) - 1u)) << bd.lby) +
                                          ((uint64_t)y & ((1u << bd.lby) - 1u)))
                                         << bd.lwx) + (gw & ((1u << bd.lwx) - 1u)));
-        const uint32_t bwords = 1u << (bd.lwx + bd.lby + bd.lbz);
+        const size_t bbase = ((size_t)b << (bd.lwx + bd.lby + bd.lbz + 5u)) + (size_t)wb * 32u +
+                             ((fw & 1) ? 16u : 0u);
         const size_t gwi = (gz * g.Y + (uint64_t)y) * g.W + gw;
         const int sh = (fw & 1) ? 0 : 16;
         for (int par = 0; par < 2; par++) {
             const int gi = odd ? 2 + par : par;
             const MatSource &src = gi == 0 ? m1 : gi == 1 ? m2 : gi == 2 ? m3 : m4;
-            const uint32_t mb = src.brick_mat_off[b];
-            if (mb == UINT32_MAX) continue;
+            if (!src.bhist[b]) continue;
             const uint32_t sw = src.surf[gwi];
             uint32_t h = (sw >> sh) & 0xFFFFu;
             if (!h) continue;
-            uint32_t r = mb + src.wrank[(size_t)b * bwords + wb] +
-                         ((fw & 1) ? (uint32_t)__popc(sw >> 16) : 0u);
+            const VK_G uint8_t *mg = src.matgrid + bbase;  // this half-word's 16 voxels
             while (h) {
                 const int i = __clz(h) - 16;  // voxel i of the 16
-                const uint32_t mv = src.mat_list[r++];
+                const uint32_t mv = mg[i];
                 const int k = 2 * i + par;
 #pragma unroll
                 for (int kk = 0; kk < 32; kk++)
