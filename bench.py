@@ -195,8 +195,11 @@ def bench_single(args, va, wl):
         ctx.voxelize_nodes_device(g, kind, out.ptr.value, True, vs, ms)
 
     # Surface brick shape picked by the autotuner (before the timed steps).
-    bdims, bms = ctx.autotune_bricks(grid, kind, out.ptr.value, 2, vs, ms)
-    print(f"# autotuned brick {bdims} ({bms:.3f} ms)", file=sys.stderr, flush=True)
+    if args.autotune == "on":
+        bdims, bms = ctx.autotune_bricks(grid, kind, out.ptr.value, 2, vs, ms)
+        print(f"# autotuned brick {bdims} ({bms:.3f} ms)", file=sys.stderr, flush=True)
+    else:
+        bdims = ctx.brick_dims(kind in (va.NODE_SHORT_FCC, va.NODE_LONG_FCC))
 
     results = {}
     for sched in scheds:
@@ -285,8 +288,10 @@ def bench_batch(args, va, wl):
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     kinds = [va.NODE_SHORT if int(m.materials.max()) <= 7 else va.NODE_LONG for m in meshes]
     ctx = va.Context(0)
-    training = train_cost_model(ctx, va, "room", 1, 0.01, va.NODE_LONG, (1024, 512, 512),
-                                (512, 512, 512))
+    training = None
+    if not (args.only and args.schedule == "plain"):
+        training = train_cost_model(ctx, va, "room", 1, 0.01, va.NODE_LONG, (1024, 512, 512),
+                                    (512, 512, 512))
     items = (va.BatchItem * len(meshes))()
     keep = []
     n_nodes = 0
@@ -315,6 +320,8 @@ def bench_batch(args, va, wl):
     modes = {"plain": (va.SCHEDULE_PLAIN, 1), "fused_plain": (va.SCHEDULE_PLAIN, 0),
              "learned": (va.SCHEDULE_LEARNED, 0),
              f"streams{BATCH_STREAMS}_learned": (va.SCHEDULE_LEARNED, BATCH_STREAMS)}
+    if args.only:  # profiling one mode
+        modes = {args.schedule: modes[args.schedule]}
     results = {}
     for name, (sched, streams) in modes.items():
         for _ in range(args.warmup):
@@ -328,12 +335,15 @@ def bench_batch(args, va, wl):
         print(f"# batch {name}: {results[name] * 1e3 / args.steps:.2f} ms/step", file=sys.stderr,
               flush=True)
     ctx.enable_timing(True)
-    ctx.voxelize_batch(items, *modes["learned"])
+    ctx.voxelize_batch(items, *modes[args.schedule])
     stages = ctx.stage_times()
     ctx.enable_timing(False)
-    elapsed = results["learned" if args.schedule == "learned" else "plain"]
+    elapsed = results[args.schedule]
     gv = {k: n_nodes / (v / args.steps) / 1e9 for k, v in results.items()}
-    plain_v, learned_v = gv["plain"], gv["learned"]
+    plain_v, learned_v = gv.get("plain"), gv.get("learned")
+
+    def ratio(a, b):
+        return round(a / b, 4) if a and b else None
     value = n_nodes / (elapsed / args.steps) / 1e9
     ms_per_step = elapsed * 1e3 / args.steps
     pipe_bytes = out_bytes + sum(12 * m.nv + 13 * m.nt for m in meshes)
@@ -349,11 +359,12 @@ def bench_batch(args, va, wl):
                    "execution": "fused (each stage one launch over all meshes)"
                    if args.schedule == "learned" else "one mesh after another",
                    "parallelism": "single GPU"},
-        "schedules": {"plain_gvox_s": round(plain_v, 4), "learned_gvox_s": round(learned_v, 4),
-                      "learned_over_plain": round(learned_v / plain_v, 4),
+        "schedules": {"plain_gvox_s": plain_v and round(plain_v, 4),
+                      "learned_gvox_s": learned_v and round(learned_v, 4),
+                      "learned_over_plain": ratio(learned_v, plain_v),
                       "modes_gvox_s": {k: round(v, 4) for k, v in gv.items()},
-                      "learned_over_fused_plain": round(learned_v / gv["fused_plain"], 4),
-                      "learned_stages_ms": stages, "cost_model": training},
+                      "learned_over_fused_plain": ratio(learned_v, gv.get("fused_plain")),
+                      "stages_ms": stages, "cost_model": training},
         "roofline": {"bound": "hbm", "achieved": round(pipe_bytes / (ms_per_step * 1e-3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(pipe_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -373,6 +384,8 @@ def main():
     ap.add_argument("--only", action="store_true",
                     help="time only the schedule named by --schedule (plain: no cost-model "
                          "training; for profiling one schedule)")
+    ap.add_argument("--autotune", default="on", choices=["on", "off"],
+                    help="off: keep the default surface brick shape (profiling runs)")
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU work for the oracle baseline sample")
