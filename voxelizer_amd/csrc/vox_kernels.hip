@@ -732,9 +732,9 @@ __device__ __forceinline__ void surf_item(const SurfParams &p, const SplitTable 
 
 // Persistent workgroups over a list of non-empty bricks (`order`: brick ids
 // in index order for the plain schedule, or sorted by predicted cost).  Each
-// workgroup takes the next list entry from a global counter (the grab for
-// the following brick is issued while the current one is processed), so
-// bricks of uneven cost balance over the GPU.  Per brick: its triangles are
+// workgroup takes runs of list entries from a global counter (the grab for
+// the following run is issued while the last brick of the current one is
+// processed), so bricks of uneven cost balance over the GPU.  Per brick: its triangles are
 // taken 64 at a time, each set up once into LDS, and all (triangle, column)
 // items of the chunk are spread over the 256 threads; winners accumulate in
 // LDS (ds_max of bbType<<30 | triangle).  Only rows of the brick that
@@ -756,20 +756,36 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
     __shared__ uint32_t colsum[kSurfChunk + 1];
     __shared__ uint32_t wsurf[kMaxBrickVox / 32];
     __shared__ uint8_t touched[32];
-    __shared__ uint32_t s_q;
+    __shared__ uint32_t s_q, s_qe;
     __shared__ SplitTable spl;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
     for (int32_t i = tid; i < kMaxBrickVox; i += blockDim.x) keys[i] = 0u;
     if (tid < 32u) touched[tid] = 0;
     const uint32_t n_list = *n_order;
-    if (tid == 0) s_q = atomicAdd(next, 1u);
+    // Guided grabs: a workgroup takes a run of list entries sized to the
+    // work left (remaining / (8 * workgroups), 1..32), so the shared counter
+    // sees a few grabs per workgroup rather than one per brick (one address
+    // bumped per brick serialises the pass), while the tail still goes one
+    // brick at a time.
+    auto grab = [&](uint32_t seen, uint32_t &qe) {
+        const uint32_t left = n_list > seen ? n_list - seen : 0u;
+        const uint32_t k = min(max(left / (8u * gridDim.x), 1u), 32u);
+        const uint32_t q0 = atomicAdd(next, k);
+        qe = min(q0 + k, n_list);
+        return q0;
+    };
+    if (tid == 0) {
+        uint32_t qe;
+        s_q = grab(0u, qe);
+        s_qe = qe;
+    }
     __syncthreads();
-    uint32_t q = s_q;
+    uint32_t q = s_q, qend = s_qe;
     uint32_t cur_job = UINT32_MAX;  // job whose split table is in LDS
     uint32_t nsurf = 0u;            // surface voxels found by this wave
-    while (q < n_list) {  // q is uniform over the block
-        uint32_t q_next = 0u;
-        if (tid == 0) q_next = atomicAdd(next, 1u);  // grab the next brick early
+    while (q < n_list) {  // q and qend are uniform over the block
+        uint32_t q_next = q + 1u, qe_next = qend;
+        if (tid == 0 && q + 1u >= qend) q_next = grab(qend, qe_next);  // next run, early
         const uint32_t gb = order[q];
         const uint32_t cnt = brick_hist[gb];
         uint64_t t_start = 0;
@@ -890,9 +906,13 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
         if (cycles && tid == 0) cycles[gb] = __builtin_amdgcn_s_memtime() - t_start;
         __syncthreads();
         if (tid < 32u) touched[tid] = 0;
-        if (tid == 0) s_q = q_next;
+        if (tid == 0) {
+            s_q = q_next;
+            s_qe = qe_next;
+        }
         __syncthreads();
         q = s_q;
+        qend = s_qe;
     }
     // One atomic per wave at exit: a shared counter bumped per brick would
     // serialise every workgroup on one address.
