@@ -530,7 +530,11 @@ vox_status surface_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
     const uint32_t nslots = P.tri_b.back(), nbt = P.brick_b.back();
     uint4 *tri_info, *tri_ext = nullptr;
     uint32_t *cand, *cand_off, *brick_hist, *brick_off, *brick_cursor, *blist, *stats;
-    const bool learned = ctx->schedule == VOX_SCHEDULE_LEARNED && ctx->cost_ready;
+    // The model is consulted only when there are more bricks than resident
+    // brick-pass workgroups: with fewer, every brick starts at once and the
+    // order cannot matter.
+    const bool learned = ctx->schedule == VOX_SCHEDULE_LEARNED && ctx->cost_ready &&
+                         nbt > (uint32_t)(ctx->n_cu * ctx->surf_blocks_per_cu);
     const bool features = learned || ctx->record;
     if (ctx->timing) ctx->times.learned_schedule = learned ? 1u : 0u;
     VOX_TRY(buf(ctx, "f_tri_info", nslots, &tri_info));
@@ -583,10 +587,12 @@ vox_status surface_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
     VOX_HIP(hipGetLastError());
     VOX_TRY(bin(1));
     if (ctx->timing) ctx->times.surface_pairs += total;
-    // Learned schedule: the listed bricks sorted by predicted cost, most
-    // expensive first (features -> MLP -> 1024-bucket counting sort).
+    // Learned schedule: the listed bricks screened by pair count, their cost
+    // predicted by the MLP, the heavy ones on a list the brick pass takes
+    // first (see vox_cost.hip).
     const uint32_t *n_list = blist_order_off + nbt;
-    const uint32_t *brick_list = blist_order;
+    const uint32_t *hlist = nullptr, *n_heavy = nullptr;
+    const uint8_t *skip = nullptr;
     float *feats = nullptr;
     uint64_t *cycles = nullptr;
     if (features) VOX_TRY(buf(ctx, "f_feats", (size_t)nbt * kCostFeatures, &feats));
@@ -596,49 +602,33 @@ vox_status surface_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
             D.brick_b, nj, feats);
         VOX_HIP(hipGetLastError());
     } else if (learned) {
-        // Screen by pair count, predict the candidates' cost, order.
         float *pred;
-        uint32_t *flags2, *soff, *sel, *light, *loff, *chist, *coff, *ccur, *sorted;
+        uint32_t *lcand, *cnt, *heavy;
+        uint8_t *skipf;
         VOX_TRY(buf(ctx, "f_pred", nbt, &pred));
-        VOX_TRY(buf(ctx, "f_sflags", nbt, &flags2));
-        VOX_TRY(buf(ctx, "f_soff", nbt + 1, &soff));
-        VOX_TRY(buf(ctx, "f_sel", nbt, &sel));
-        VOX_TRY(buf(ctx, "f_light", nbt, &light));
-        VOX_TRY(buf(ctx, "f_loff", nbt + 1, &loff));
-        VOX_TRY(buf(ctx, "f_chist", kCostBuckets, &chist));
-        VOX_TRY(buf(ctx, "f_coff", kCostBuckets + 1, &coff));
-        VOX_TRY(buf(ctx, "f_ccur", kCostBuckets, &ccur));
-        VOX_TRY(buf(ctx, "f_sorted", nbt, &sorted));
-        VOX_TRY(fill(ctx, flags2, nbt, 0u));
-        VOX_TRY(fill(ctx, light, nbt, 0u));
-        VOX_TRY(fill(ctx, chist, kCostBuckets, 0u));
-        VOX_TRY(fill(ctx, ccur, kCostBuckets, 0u));
+        VOX_TRY(buf(ctx, "f_lcand", nbt, &lcand));
+        VOX_TRY(buf(ctx, "f_cnt", 2, &cnt));
+        VOX_TRY(buf(ctx, "f_heavy", nbt, &heavy));
+        VOX_TRY(buf(ctx, "f_skip", nbt, &skipf));
+        VOX_TRY(fill(ctx, cnt, 2, 0u));
         const uint32_t g256 = grid_for(nbt, 256, 1024);
         k_cost_screen<<<g256, 256, 0, ctx->stream>>>(blist_order, n_list, brick_hist,
-                                                     cand_off + nslots, ctx->screen_ratio, flags2,
-                                                     light);
-        VOX_HIP(hipGetLastError());
-        VOX_TRY(scan(ctx, flags2, soff, nbt));
-        const uint32_t *n_sel = soff + nbt;
-        k_cost_compact<<<g256, 256, 0, ctx->stream>>>(flags2, soff, n_list, sel);
+                                                     cand_off + nslots, ctx->screen_ratio, lcand,
+                                                     cnt, skipf);
         VOX_HIP(hipGetLastError());
         k_brick_features<<<grid_for(nbt, 4, 65535), 256, 0, ctx->stream>>>(
-            blist_order, sel, n_sel, brick_hist, brick_off, blist, tri_info, tri_ext, D.jobs,
+            blist_order, lcand, cnt, brick_hist, brick_off, blist, tri_info, tri_ext, D.jobs,
             D.brick_b, nj, feats);
         VOX_HIP(hipGetLastError());
         k_mlp_forward_n<<<(nbt + kMlpBlockRows - 1) / kMlpBlockRows, 256, 0, ctx->stream>>>(
-            ctx->cost_params, feats, n_sel, pred);
+            ctx->cost_params, feats, cnt, pred);
         VOX_HIP(hipGetLastError());
-        k_cost_hist<<<g256, 256, 0, ctx->stream>>>(pred, sel, n_sel, ctx->heavy_cost, light,
-                                                   chist);
+        k_cost_heavy<<<g256, 256, 0, ctx->stream>>>(pred, lcand, cnt, ctx->heavy_cost, blist_order,
+                                                    heavy, skipf);
         VOX_HIP(hipGetLastError());
-        VOX_TRY(scan(ctx, chist, coff, kCostBuckets));
-        VOX_TRY(scan(ctx, light, loff, nbt));
-        k_cost_scatter<<<g256, 256, 0, ctx->stream>>>(pred, blist_order, n_list, sel, n_sel,
-                                                      ctx->heavy_cost, coff, light, loff, ccur,
-                                                      sorted);
-        VOX_HIP(hipGetLastError());
-        brick_list = sorted;
+        hlist = heavy;
+        n_heavy = cnt + 1;
+        skip = skipf;
     }
     if (ctx->record) VOX_TRY(buf(ctx, "f_cycles", nbt, &cycles));
     VOX_TRY(fill(ctx, stats, 1, 0u));
@@ -646,7 +636,8 @@ vox_status surface_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
     const uint32_t blocks = std::min<uint32_t>(nbt, (uint32_t)(ctx->n_cu * ctx->surf_blocks_per_cu));
     k_surf_brick<<<blocks, 256, 0, ctx->stream>>>(D.jobs, D.brick_b, nj, tri_rec, brick_off,
                                                   brick_hist, blist, ctx->timing ? stats : nullptr,
-                                                  brick_list, n_list, next, cycles);
+                                                  blist_order, n_list, hlist, n_heavy, skip, next,
+                                                  cycles);
     VOX_HIP(hipGetLastError());
     if (ctx->timing) {
         uint32_t vox = 0;

@@ -2,16 +2,16 @@
 //
 // The surface pass (k_surf_brick) takes its bricks from a list.  The plain
 // schedule lists the non-empty bricks in index order; the learned schedule
-// sorts them by a cost predicted from 16 per-brick features by a small MLP
-// (16 -> 64 -> 64 -> 1, ReLU, FP32), so the expensive bricks start first and
-// the persistent workgroups finish together.  Training targets are the
+// puts the bricks that a small MLP (16 -> 64 -> 64 -> 1, ReLU, FP32, on 16
+// per-brick features) predicts to be expensive on a list taken first, so
+// they start early and the persistent workgroups finish together.  Training targets are the
 // per-brick durations the brick pass records (s_memtime) on seeded meshes.
 // Both schedules produce identical output: the order only decides which
 // workgroup processes a brick and when.
 //
 // Kernels: per-brick features, MLP forward, MLP gradient (per-block partial
-// sums reduced in a fixed order, so training is deterministic), Adam, and a
-// 1024-bucket counting sort of the bricks by predicted cost.
+// sums reduced in a fixed order, so training is deterministic), Adam, and the
+// screening and heavy-list passes of the learned order.
 #include "vox_kernels.h"
 
 namespace vk {
@@ -337,91 +337,47 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ P, const float
 }
 
 // ---------------------------------------------------------------------------
-// Learned brick order.  Candidates are the bricks whose pair count is at
-// least `ratio` times the mean (the plain counts screen the list); the MLP
+// Learned brick order.  Candidates are the listed bricks whose pair count is
+// at least `ratio` times the mean (the plain counts screen the list); the MLP
 // predicts their cost, and those predicted heavy (standardized log cost above
-// `heavy`) go first, most expensive first (1024-bucket counting sort on
-// bucket = (4 - y) * 128).  Every other brick follows in index order, which
-// keeps neighbouring bricks -- which share triangles and solid rows in L2 --
-// together.
+// `heavy`) go on a short list that the brick pass works through first, one
+// brick per grab.  The plain list follows in index order (neighbouring bricks
+// share triangles and solid rows in L2), skipping the bricks already taken.
+// cnt[0] counts candidates, cnt[1] heavy bricks; both start at zero.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t cost_bucket(float y) {
-    const float f = (4.0f - y) * 128.0f;
-    if (!(f > 0.0f)) return 0u;  // also NaN
-    return f >= 1023.0f ? 1023u : (uint32_t)f;
-}
-
-// flags[q] = 1 for candidate list positions q < n; light[q] = 1 for all q < n.
 __global__ __launch_bounds__(256) void k_cost_screen(const uint32_t *__restrict__ order,
                                                      const uint32_t *__restrict__ n_order,
                                                      const uint32_t *__restrict__ brick_hist,
                                                      const uint32_t *__restrict__ total_pairs,
-                                                     float ratio, uint32_t *__restrict__ flags,
-                                                     uint32_t *__restrict__ light) {
+                                                     float ratio, uint32_t *__restrict__ cand,
+                                                     uint32_t *__restrict__ cnt,
+                                                     uint8_t *__restrict__ skip) {
     const uint32_t n = *n_order;
     const float thr = n ? ratio * (float)(*total_pairs) / (float)n : 0.0f;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        flags[q] = (float)brick_hist[order[q]] >= thr ? 1u : 0u;
-        light[q] = 1u;
+        const bool c = (float)brick_hist[order[q]] >= thr;
+        skip[q] = 0u;
+        const uint32_t slot = wave_bump(&cnt[0], c);  // one atomic per wave
+        if (c) cand[slot] = q;
     }
 }
 
-__global__ __launch_bounds__(256) void k_cost_compact(const uint32_t *__restrict__ flags,
-                                                      const uint32_t *__restrict__ off,
-                                                      const uint32_t *__restrict__ n_order,
-                                                      uint32_t *__restrict__ sel) {
-    const uint32_t n = *n_order;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
-        if (flags[q]) sel[off[q]] = q;
-}
-
-// Heavy candidates: light[sel[c]] = 0 and one count in their cost bucket.
-__global__ __launch_bounds__(256) void k_cost_hist(const float *__restrict__ Y,
-                                                   const uint32_t *__restrict__ sel,
-                                                   const uint32_t *__restrict__ n_sel,
-                                                   float heavy, uint32_t *__restrict__ light,
-                                                   uint32_t *__restrict__ hist) {
-    __shared__ uint32_t h[kCostBuckets];
-    for (uint32_t i = threadIdx.x; i < kCostBuckets; i += blockDim.x) h[i] = 0u;
-    __syncthreads();
-    const uint32_t n = *n_sel;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-        const float y = Y[c];
-        if (y > heavy) {
-            light[sel[c]] = 0u;
-            atomicAdd(&h[cost_bucket(y)], 1u);
+__global__ __launch_bounds__(256) void k_cost_heavy(const float *__restrict__ Y,
+                                                    const uint32_t *__restrict__ cand,
+                                                    uint32_t *__restrict__ cnt, float heavy,
+                                                    const uint32_t *__restrict__ order,
+                                                    uint32_t *__restrict__ hlist,
+                                                    uint8_t *__restrict__ skip) {
+    const uint32_t n = cnt[0];
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        const bool h = Y[r] > heavy;  // false for NaN
+        const uint32_t slot = wave_bump(&cnt[1], h);
+        if (h) {
+            const uint32_t q = cand[r];
+            hlist[slot] = order[q];
+            skip[q] = 1u;
         }
     }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < kCostBuckets; i += blockDim.x)
-        if (h[i]) atomicAdd(&hist[i], h[i]);
-}
-
-// off: exclusive scan of hist (off[kCostBuckets] = heavy count); loff:
-// exclusive scan of light.
-__global__ __launch_bounds__(256) void k_cost_scatter(const float *__restrict__ Y,
-                                                      const uint32_t *__restrict__ order,
-                                                      const uint32_t *__restrict__ n_order,
-                                                      const uint32_t *__restrict__ sel,
-                                                      const uint32_t *__restrict__ n_sel,
-                                                      float heavy,
-                                                      const uint32_t *__restrict__ off,
-                                                      const uint32_t *__restrict__ light,
-                                                      const uint32_t *__restrict__ loff,
-                                                      uint32_t *__restrict__ cursor,
-                                                      uint32_t *__restrict__ out) {
-    const uint32_t n = *n_order, nc = *n_sel;
-    const uint32_t nheavy = off[kCostBuckets];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += stride) {
-        const float y = Y[c];
-        if (y > heavy) {
-            const uint32_t k = cost_bucket(y);
-            out[off[k] + atomicAdd(&cursor[k], 1u)] = order[sel[c]];
-        }
-    }
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
-        if (light[q]) out[nheavy + loff[q]] = order[q];
 }
 
 }  // namespace vk

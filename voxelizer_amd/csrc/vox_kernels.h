@@ -90,10 +90,9 @@ constexpr uint32_t kErodeZ = 8;
 // thread's planes share one surface brick.
 constexpr uint32_t kNodeZ = 4;
 static_assert(kCartBZ % kNodeZ == 0, "node planes must not straddle a brick");  // z >= 4
-// Cost model: features per brick, MLP parameters (16-64-64-1), sort buckets.
+// Cost model: features per brick, MLP parameters (16-64-64-1).
 constexpr int kCostFeatures = 16;
 constexpr int kCostParams = 64 * 16 + 64 + 64 * 64 + 64 + 64 + 1;  // 5313
-constexpr uint32_t kCostBuckets = 1024;
 constexpr uint32_t kMlpBlockRows = 32;  // samples per MLP workgroup
 constexpr int kMaxSplits = 64;
 
@@ -346,7 +345,10 @@ __global__ void k_surf_brick(const Job *__restrict__ jobs, const uint32_t *__res
                              const uint32_t *__restrict__ brick_hist,
                              const uint32_t *__restrict__ blist, uint32_t *__restrict__ stats,
                              const uint32_t *__restrict__ order,
-                             const uint32_t *__restrict__ n_order, uint32_t *__restrict__ next,
+                             const uint32_t *__restrict__ n_order,
+                             const uint32_t *__restrict__ hlist,
+                             const uint32_t *__restrict__ n_heavy,
+                             const uint8_t *__restrict__ skip, uint32_t *__restrict__ next,
                              uint64_t *__restrict__ cycles);
 __global__ void k_nonzero_flags(const uint32_t *__restrict__ hist, uint32_t *__restrict__ flags,
                                 uint32_t n);
@@ -428,18 +430,11 @@ __global__ void k_cost_screen(const uint32_t *__restrict__ order,
                               const uint32_t *__restrict__ n_order,
                               const uint32_t *__restrict__ brick_hist,
                               const uint32_t *__restrict__ total_pairs, float ratio,
-                              uint32_t *__restrict__ flags, uint32_t *__restrict__ light);
-__global__ void k_cost_compact(const uint32_t *__restrict__ flags, const uint32_t *__restrict__ off,
-                               const uint32_t *__restrict__ n_order, uint32_t *__restrict__ sel);
-__global__ void k_cost_hist(const float *__restrict__ Y, const uint32_t *__restrict__ sel,
-                            const uint32_t *__restrict__ n_sel, float heavy,
-                            uint32_t *__restrict__ light, uint32_t *__restrict__ hist);
-__global__ void k_cost_scatter(const float *__restrict__ Y, const uint32_t *__restrict__ order,
-                               const uint32_t *__restrict__ n_order,
-                               const uint32_t *__restrict__ sel, const uint32_t *__restrict__ n_sel,
-                               float heavy, const uint32_t *__restrict__ off,
-                               const uint32_t *__restrict__ light,
-                               const uint32_t *__restrict__ loff, uint32_t *__restrict__ cursor,
-                               uint32_t *__restrict__ out);
+                              uint32_t *__restrict__ cand, uint32_t *__restrict__ cnt,
+                              uint8_t *__restrict__ skip);
+__global__ void k_cost_heavy(const float *__restrict__ Y, const uint32_t *__restrict__ cand,
+                             uint32_t *__restrict__ cnt, float heavy,
+                             const uint32_t *__restrict__ order, uint32_t *__restrict__ hlist,
+                             uint8_t *__restrict__ skip);
 
 }  // namespace vk

@@ -749,8 +749,9 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
     const float4 *__restrict__ tri_rec, const uint32_t *__restrict__ brick_off,
     const uint32_t *__restrict__ brick_hist, const uint32_t *__restrict__ blist,
     uint32_t *__restrict__ stats, const uint32_t *__restrict__ order,
-    const uint32_t *__restrict__ n_order, uint32_t *__restrict__ next,
-    uint64_t *__restrict__ cycles) {
+    const uint32_t *__restrict__ n_order, const uint32_t *__restrict__ hlist,
+    const uint32_t *__restrict__ n_heavy, const uint8_t *__restrict__ skip,
+    uint32_t *__restrict__ next, uint64_t *__restrict__ cycles) {
     __shared__ uint32_t keys[kMaxBrickVox];
     __shared__ SurfTri tris[kSurfChunk];
     __shared__ uint32_t colsum[kSurfChunk + 1];
@@ -761,15 +762,17 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
     for (int32_t i = tid; i < kMaxBrickVox; i += blockDim.x) keys[i] = 0u;
     if (tid < 32u) touched[tid] = 0;
-    const uint32_t n_list = *n_order;
+    // Entries: the heavy list first (learned order), then the plain list.
+    const uint32_t nh = hlist ? *n_heavy : 0u;
+    const uint32_t n_list = *n_order + nh;
     // Guided grabs: a workgroup takes a run of list entries sized to the
-    // work left (remaining / (8 * workgroups), 1..32), so the shared counter
-    // sees a few grabs per workgroup rather than one per brick (one address
-    // bumped per brick serialises the pass), while the tail still goes one
-    // brick at a time.
+    // work left (remaining / (8 * workgroups), 1..32; heavy entries one at a
+    // time), so the shared counter sees a few grabs per workgroup rather
+    // than one per brick (one address bumped per brick serialises the pass),
+    // while the tail still goes one brick at a time.
     auto grab = [&](uint32_t seen, uint32_t &qe) {
         const uint32_t left = n_list > seen ? n_list - seen : 0u;
-        const uint32_t k = min(max(left / (8u * gridDim.x), 1u), 32u);
+        const uint32_t k = seen < nh ? 1u : min(max(left / (8u * gridDim.x), 1u), 32u);
         const uint32_t q0 = atomicAdd(next, k);
         qe = min(q0 + k, n_list);
         return q0;
@@ -786,124 +789,128 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
     while (q < n_list) {  // q and qend are uniform over the block
         uint32_t q_next = q + 1u, qe_next = qend;
         if (tid == 0 && q + 1u >= qend) q_next = grab(qend, qe_next);  // next run, early
-        const uint32_t gb = order[q];
-        const uint32_t cnt = brick_hist[gb];
-        uint64_t t_start = 0;
-        if (cycles && tid == 0) t_start = __builtin_amdgcn_s_memtime();
-        const uint32_t j = job_of(brick_bases, njobs, gb);
-        const Job &J = jobs[j];
-        if (j != cur_job) {  // uniform: the job's matSplitRes table into LDS
+        const bool from_heavy = q < nh;
+        const uint32_t gb = from_heavy ? hlist[q] : order[q - nh];
+        // A plain entry already taken from the heavy list is passed over.
+        if (from_heavy || !hlist || !skip[q - nh]) {
+            const uint32_t cnt = brick_hist[gb];
+            uint64_t t_start = 0;
+            if (cycles && tid == 0) t_start = __builtin_amdgcn_s_memtime();
+            const uint32_t j = job_of(brick_bases, njobs, gb);
+            const Job &J = jobs[j];
+            if (j != cur_job) {  // uniform: the job's matSplitRes table into LDS
+                __syncthreads();
+                const VK_G uint32_t *src = (const VK_G uint32_t *)gptr(J.splits);
+                uint32_t *dst = (uint32_t *)&spl;
+                for (uint32_t i = tid; i < sizeof(SplitTable) / 4; i += blockDim.x) dst[i] = src[i];
+                cur_job = j;
+            }
+            const SurfParams p = surf_params(J);
+            const VK_G uint8_t *__restrict__ materials = gptr(J.mats);
+            const bool embed = J.embed != 0;
+            VK_G uint32_t *__restrict__ bits = gptr(J.bits);
+            const uint32_t b = gb - J.brick_base;  // brick within the job
+            const int32_t bd0 = p.bdim[0], bd1 = p.bdim[1], bd2 = p.bdim[2];
+            const int32_t nvox = bd0 * bd1 * bd2;
+            const int32_t wpr = bd0 / 32;     // words per row
+            const int32_t nwords = nvox / 32;
+            const int32_t row_shift = 31 - __clz(bd0);
+            const uint32_t bxi = b % p.nb[0], byi = (b / p.nb[0]) % p.nb[1],
+                           bzi = b / (p.nb[0] * p.nb[1]);
+            // Bricks are laid over alloc coordinates; ext = alloc - 1.
+            const int32_t blx = (int32_t)(bxi * bd0) - 1, bly = (int32_t)(byi * bd1) - 1,
+                          blz = (int32_t)(bzi * bd2) - 1;
+            // This thread's brick word (for the OR into the solid grid), early.
+            size_t gi = 0;
+            bool gword = false;
+            uint32_t bw = 0u;
+            if ((int32_t)tid < nwords) {
+                const int32_t wx = (int32_t)tid % wpr, ly = ((int32_t)tid / wpr) % bd1,
+                              lz = (int32_t)tid / (wpr * bd1);
+                const uint32_t aw = bxi * (uint32_t)wpr + (uint32_t)wx;
+                const uint32_t ay = byi * (uint32_t)bd1 + (uint32_t)ly;
+                const uint32_t az = bzi * (uint32_t)bd2 + (uint32_t)lz;
+                gword = aw < p.alloc[0] / 32u && ay < p.alloc[1] && az < p.alloc[2];
+                if (gword) {
+                    gi = ((size_t)az * p.alloc[1] + ay) * (p.alloc[0] / 32u) + aw;
+                    bw = bits[gi];
+                }
+            }
+            const uint32_t off = brick_off[gb];
+            for (uint32_t c0 = 0; c0 < cnt; c0 += kSurfChunk) {
+                __syncthreads();
+                if (tid < (uint32_t)kSurfChunk) {
+                    uint32_t items = 0u;
+                    if (c0 + tid < cnt) {
+                        const uint32_t t = blist[off + c0 + tid];
+                        items = surf_tri_setup(p, tri_rec, t, J.tri_base, embed, blx, bly, blz,
+                                               tris[tid]);
+                    }
+                    uint32_t incl = items;  // wave-level inclusive scan
+                    for (int d = 1; d < kSurfChunk; d <<= 1) {
+                        const uint32_t o = (uint32_t)__shfl_up((int)incl, d);
+                        if ((int)tid >= d) incl += o;
+                    }
+                    colsum[tid + 1] = incl;
+                    if (tid == 0) colsum[0] = 0u;
+                }
+                __syncthreads();
+                const uint32_t total = colsum[kSurfChunk];
+                for (uint32_t c = tid; c < total; c += blockDim.x) {
+                    uint32_t lo = 0, hi = kSurfChunk;  // colsum[lo] <= c < colsum[lo + 1]
+                    while (lo + 1 < hi) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (colsum[mid] <= c) lo = mid;
+                        else hi = mid;
+                    }
+                    surf_item(p, spl, tris[lo], c - colsum[lo], keys, touched, nvox, row_shift);
+                }
+            }
             __syncthreads();
-            const VK_G uint32_t *src = (const VK_G uint32_t *)gptr(J.splits);
-            uint32_t *dst = (uint32_t *)&spl;
-            for (uint32_t i = tid; i < sizeof(SplitTable) / 4; i += blockDim.x) dst[i] = src[i];
-            cur_job = j;
-        }
-        const SurfParams p = surf_params(J);
-        const VK_G uint8_t *__restrict__ materials = gptr(J.mats);
-        const bool embed = J.embed != 0;
-        VK_G uint32_t *__restrict__ bits = gptr(J.bits);
-        const uint32_t b = gb - J.brick_base;  // brick within the job
-        const int32_t bd0 = p.bdim[0], bd1 = p.bdim[1], bd2 = p.bdim[2];
-        const int32_t nvox = bd0 * bd1 * bd2;
-        const int32_t wpr = bd0 / 32;     // words per row
-        const int32_t nwords = nvox / 32;
-        const int32_t row_shift = 31 - __clz(bd0);
-        const uint32_t bxi = b % p.nb[0], byi = (b / p.nb[0]) % p.nb[1],
-                       bzi = b / (p.nb[0] * p.nb[1]);
-        // Bricks are laid over alloc coordinates; ext = alloc - 1.
-        const int32_t blx = (int32_t)(bxi * bd0) - 1, bly = (int32_t)(byi * bd1) - 1,
-                      blz = (int32_t)(bzi * bd2) - 1;
-        // This thread's brick word (for the OR into the solid grid), early.
-        size_t gi = 0;
-        bool gword = false;
-        uint32_t bw = 0u;
-        if ((int32_t)tid < nwords) {
-            const int32_t wx = (int32_t)tid % wpr, ly = ((int32_t)tid / wpr) % bd1,
-                          lz = (int32_t)tid / (wpr * bd1);
-            const uint32_t aw = bxi * (uint32_t)wpr + (uint32_t)wx;
-            const uint32_t ay = byi * (uint32_t)bd1 + (uint32_t)ly;
-            const uint32_t az = bzi * (uint32_t)bd2 + (uint32_t)lz;
-            gword = aw < p.alloc[0] / 32u && ay < p.alloc[1] && az < p.alloc[2];
-            if (gword) {
-                gi = ((size_t)az * p.alloc[1] + ay) * (p.alloc[0] / 32u) + aw;
-                bw = bits[gi];
-            }
-        }
-        const uint32_t off = brick_off[gb];
-        for (uint32_t c0 = 0; c0 < cnt; c0 += kSurfChunk) {
+            // Surface bit words and winner materials: one lane per voxel of the
+            // touched rows, in 64-voxel groups taken by the waves in turn (a
+            // group lies in one row).  Every lane of a touched group writes its
+            // voxel's byte of the brick's block in the material grid (the
+            // winner's material, or 0), so the grid sees whole 64-byte
+            // segments.  Keys are cleared for the next brick.  The two loops
+            // differ in where the material comes from: with materials in the
+            // keys the loop issues no loads, so its stores never wait (on this
+            // architecture a wait for a load also waits for earlier stores).
+            VK_G uint8_t *__restrict__ mg = gptr(J.matgrid) + (size_t)b * (uint32_t)nvox;
+            auto rows_out = [&](auto embed_c) {
+                constexpr bool kEmbedded = decltype(embed_c)::value;
+                for (int32_t v0 = (int32_t)wave * 64; v0 < nvox; v0 += 256) {
+                    if (!touched[v0 >> row_shift]) {
+                        if (lane < 2u) wsurf[(v0 >> 5) + (int32_t)lane] = 0u;
+                        continue;
+                    }
+                    const uint32_t key = keys[v0 + (int32_t)lane];
+                    const unsigned long long m = __ballot(key != 0u);
+                    if (lane == 0) {
+                        wsurf[v0 >> 5] = __brev((uint32_t)m);
+                        wsurf[(v0 >> 5) + 1] = __brev((uint32_t)(m >> 32));
+                    }
+                    nsurf += (uint32_t)__popcll(m);
+                    uint32_t mat = kEmbedded ? key & 0xFFu : 0u;
+                    if (key) {
+                        keys[v0 + (int32_t)lane] = 0u;
+                        if (!kEmbedded && materials) mat = materials[key & 0x3FFFFFFFu];
+                    }
+                    mg[v0 + (int32_t)lane] = (uint8_t)mat;
+                }
+            };
+            if (embed)
+                rows_out(std::integral_constant<bool, true>{});
+            else
+                rows_out(std::integral_constant<bool, false>{});
             __syncthreads();
-            if (tid < (uint32_t)kSurfChunk) {
-                uint32_t items = 0u;
-                if (c0 + tid < cnt) {
-                    const uint32_t t = blist[off + c0 + tid];
-                    items = surf_tri_setup(p, tri_rec, t, J.tri_base, embed, blx, bly, blz,
-                                           tris[tid]);
-                }
-                uint32_t incl = items;  // wave-level inclusive scan
-                for (int d = 1; d < kSurfChunk; d <<= 1) {
-                    const uint32_t o = (uint32_t)__shfl_up((int)incl, d);
-                    if ((int)tid >= d) incl += o;
-                }
-                colsum[tid + 1] = incl;
-                if (tid == 0) colsum[0] = 0u;
+            if ((int32_t)tid < nwords && gword) {
+                const uint32_t sw = wsurf[tid];
+                gptr(J.surf)[gi] = sw;
+                if (sw) bits[gi] = bw | sw;
             }
-            __syncthreads();
-            const uint32_t total = colsum[kSurfChunk];
-            for (uint32_t c = tid; c < total; c += blockDim.x) {
-                uint32_t lo = 0, hi = kSurfChunk;  // colsum[lo] <= c < colsum[lo + 1]
-                while (lo + 1 < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (colsum[mid] <= c) lo = mid;
-                    else hi = mid;
-                }
-                surf_item(p, spl, tris[lo], c - colsum[lo], keys, touched, nvox, row_shift);
-            }
-        }
-        __syncthreads();
-        // Surface bit words and winner materials: one lane per voxel of the
-        // touched rows, in 64-voxel groups taken by the waves in turn (a
-        // group lies in one row).  Every lane of a touched group writes its
-        // voxel's byte of the brick's block in the material grid (the
-        // winner's material, or 0), so the grid sees whole 64-byte
-        // segments.  Keys are cleared for the next brick.  The two loops
-        // differ in where the material comes from: with materials in the
-        // keys the loop issues no loads, so its stores never wait (on this
-        // architecture a wait for a load also waits for earlier stores).
-        VK_G uint8_t *__restrict__ mg = gptr(J.matgrid) + (size_t)b * (uint32_t)nvox;
-        auto rows_out = [&](auto embed_c) {
-            constexpr bool kEmbedded = decltype(embed_c)::value;
-            for (int32_t v0 = (int32_t)wave * 64; v0 < nvox; v0 += 256) {
-                if (!touched[v0 >> row_shift]) {
-                    if (lane < 2u) wsurf[(v0 >> 5) + (int32_t)lane] = 0u;
-                    continue;
-                }
-                const uint32_t key = keys[v0 + (int32_t)lane];
-                const unsigned long long m = __ballot(key != 0u);
-                if (lane == 0) {
-                    wsurf[v0 >> 5] = __brev((uint32_t)m);
-                    wsurf[(v0 >> 5) + 1] = __brev((uint32_t)(m >> 32));
-                }
-                nsurf += (uint32_t)__popcll(m);
-                uint32_t mat = kEmbedded ? key & 0xFFu : 0u;
-                if (key) {
-                    keys[v0 + (int32_t)lane] = 0u;
-                    if (!kEmbedded && materials) mat = materials[key & 0x3FFFFFFFu];
-                }
-                mg[v0 + (int32_t)lane] = (uint8_t)mat;
-            }
-        };
-        if (embed)
-            rows_out(std::integral_constant<bool, true>{});
-        else
-            rows_out(std::integral_constant<bool, false>{});
-        __syncthreads();
-        if ((int32_t)tid < nwords && gword) {
-            const uint32_t sw = wsurf[tid];
-            gptr(J.surf)[gi] = sw;
-            if (sw) bits[gi] = bw | sw;
-        }
-        if (cycles && tid == 0) cycles[gb] = __builtin_amdgcn_s_memtime() - t_start;
+            if (cycles && tid == 0) cycles[gb] = __builtin_amdgcn_s_memtime() - t_start;
+        }  // brick
         __syncthreads();
         if (tid < 32u) touched[tid] = 0;
         if (tid == 0) {
