@@ -222,7 +222,7 @@ vox_status vox_mlp_grad(vox_ctx *ctx, const float *params_dev, const float *x_de
  * reference) -------------------------------------------------------------
  * The surface pass works on bricks of voxels (default 128x4x8 on Cartesian
  * grids, 64x4x4 on each FCC sub-grid).  Shapes are powers of two with x a
- * multiple of 64, y*z <= 32 rows and at most 4096 voxels; Cartesian shapes
+ * multiple of 64, y*z <= 32 rows and 256 to 4096 voxels; Cartesian shapes
  * need z a multiple of 4.  The shape changes speed only, never output. */
 vox_status vox_set_brick_dims(vox_ctx *ctx, int fcc, const uint32_t dims[3]);
 vox_status vox_get_brick_dims(const vox_ctx *ctx, int fcc, uint32_t dims[3]);

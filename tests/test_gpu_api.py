@@ -82,8 +82,10 @@ def test_stage_times_reported(gpu_ctx):
 
 
 @pytest.mark.parametrize("kind,shapes", [
-    (va.NODE_LONG, [(64, 4, 8), (128, 2, 8), (128, 4, 4), (256, 4, 4), (256, 2, 8), (128, 8, 4)]),
-    (va.NODE_LONG_FCC, [(64, 4, 8), (128, 4, 4), (64, 8, 4), (128, 2, 4), (64, 2, 8)]),
+    (va.NODE_LONG, [(64, 4, 8), (128, 2, 8), (128, 4, 4), (256, 4, 4), (256, 2, 8), (128, 8, 4),
+                    (64, 1, 4)]),
+    (va.NODE_LONG_FCC, [(64, 4, 8), (128, 4, 4), (64, 8, 4), (128, 2, 4), (64, 2, 8),
+                        (64, 2, 2), (256, 1, 1)]),
 ])
 def test_brick_shapes_give_identical_nodes(gpu_ctx, oracle, kind, shapes):
     fcc = kind == va.NODE_LONG_FCC
@@ -109,6 +111,8 @@ def test_brick_shape_validation(gpu_ctx):
             gpu_ctx.set_brick_dims(bad, False)
     with pytest.raises(va.VoxError):
         gpu_ctx.set_brick_dims((128, 8, 8), True)
+    with pytest.raises(va.VoxError):  # fewer than 256 voxels
+        gpu_ctx.set_brick_dims((64, 1, 2), True)
 
 
 def test_autotune_bricks(gpu_ctx, oracle):

@@ -213,13 +213,14 @@ BrickDims brick_dims(const vox_grid &g, const int bdim[3]) {
 }
 
 // A brick shape the surface and node passes support: powers of two, x a
-// multiple of 64, at most 32 rows (y*z), at most kMaxBrickVox voxels, and (for
-// the Cartesian node pass, whose threads own kNodeZ planes) z a multiple of
-// kNodeZ.
+// multiple of 64, at most 32 rows (y*z), 256 to kMaxBrickVox voxels (the
+// brick pass finishes 256 voxels per wave step), and (for the Cartesian node
+// pass, whose threads own kNodeZ planes) z a multiple of kNodeZ.
 bool brick_ok(const uint32_t b[3], bool cart) {
     for (int a = 0; a < 3; a++)
         if (b[a] == 0 || (b[a] & (b[a] - 1)) != 0) return false;
-    return b[0] >= 64 && b[1] * b[2] <= 32 && b[0] * b[1] * b[2] <= (uint32_t)kMaxBrickVox &&
+    const uint32_t nvox = b[0] * b[1] * b[2];
+    return b[0] >= 64 && b[1] * b[2] <= 32 && nvox >= 256 && nvox <= (uint32_t)kMaxBrickVox &&
            (!cart || b[2] % kNodeZ == 0);
 }
 

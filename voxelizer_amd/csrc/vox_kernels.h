@@ -67,7 +67,7 @@ __host__ __device__ __forceinline__ uint32_t plane_chunks(uint32_t W, uint32_t Y
 
 // Surface bricks: default shapes of the Cartesian grid and of each FCC
 // sub-grid; the shape is a run-time choice (the autotuner picks among
-// power-of-two shapes with x a multiple of 64, at most 32 rows and at most
+// power-of-two shapes with x a multiple of 64, at most 32 rows and 256 to
 // kMaxBrickVox voxels).
 constexpr int kCartBX = 128, kCartBY = 4, kCartBZ = 8;
 constexpr int kFccBX = 64, kFccBY = 4, kFccBZ = 4;

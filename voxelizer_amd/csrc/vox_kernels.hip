@@ -752,7 +752,7 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
     const uint32_t *__restrict__ n_order, const uint32_t *__restrict__ hlist,
     const uint32_t *__restrict__ n_heavy, const uint8_t *__restrict__ skip,
     uint32_t *__restrict__ next, uint64_t *__restrict__ cycles) {
-    __shared__ uint32_t keys[kMaxBrickVox];
+    __shared__ __align__(16) uint32_t keys[kMaxBrickVox];
     __shared__ SurfTri tris[kSurfChunk];
     __shared__ uint32_t colsum[kSurfChunk + 1];
     __shared__ uint32_t wsurf[kMaxBrickVox / 32];
@@ -785,15 +785,24 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
     __syncthreads();
     uint32_t q = s_q, qend = s_qe;
     uint32_t cur_job = UINT32_MAX;  // job whose split table is in LDS
-    uint32_t nsurf = 0u;            // surface voxels found by this wave
+    uint32_t nsurf = 0u;            // surface voxels found by this lane
     while (q < n_list) {  // q and qend are uniform over the block
         uint32_t q_next = q + 1u, qe_next = qend;
         if (tid == 0 && q + 1u >= qend) q_next = grab(qend, qe_next);  // next run, early
-        const bool from_heavy = q < nh;
-        const uint32_t gb = from_heavy ? hlist[q] : order[q - nh];
-        // A plain entry already taken from the heavy list is passed over.
-        if (from_heavy || !hlist || !skip[q - nh]) {
-            const uint32_t cnt = brick_hist[gb];
+        // The brick's header comes through uniform (scalar) loads, which do
+        // not wait behind the previous brick's outstanding vector stores.
+        const uint32_t qs = __builtin_amdgcn_readfirstlane(q);
+        const bool from_heavy = qs < nh;
+        const uint32_t gb =
+            __builtin_amdgcn_readfirstlane(from_heavy ? hlist[qs] : order[qs - nh]);
+        // A plain entry already taken from the heavy list is passed over
+        // (its flag byte read as part of an aligned dword, uniformly).
+        const bool skipped =
+            !from_heavy && hlist &&
+            ((__builtin_amdgcn_readfirstlane(
+                  ((const uint32_t *)skip)[(qs - nh) >> 2]) >> (8u * ((qs - nh) & 3u))) & 0xFFu);
+        if (!skipped) {
+            const uint32_t cnt = __builtin_amdgcn_readfirstlane(brick_hist[gb]);
             uint64_t t_start = 0;
             if (cycles && tid == 0) t_start = __builtin_amdgcn_s_memtime();
             const uint32_t j = job_of(brick_bases, njobs, gb);
@@ -836,7 +845,7 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
                     bw = bits[gi];
                 }
             }
-            const uint32_t off = brick_off[gb];
+            const uint32_t off = __builtin_amdgcn_readfirstlane(brick_off[gb]);
             for (uint32_t c0 = 0; c0 < cnt; c0 += kSurfChunk) {
                 __syncthreads();
                 if (tid < (uint32_t)kSurfChunk) {
@@ -867,36 +876,50 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
                 }
             }
             __syncthreads();
-            // Surface bit words and winner materials: one lane per voxel of the
-            // touched rows, in 64-voxel groups taken by the waves in turn (a
-            // group lies in one row).  Every lane of a touched group writes its
-            // voxel's byte of the brick's block in the material grid (the
-            // winner's material, or 0), so the grid sees whole 64-byte
-            // segments.  Keys are cleared for the next brick.  The two loops
-            // differ in where the material comes from: with materials in the
-            // keys the loop issues no loads, so its stores never wait (on this
-            // architecture a wait for a load also waits for earlier stores).
-            VK_G uint8_t *__restrict__ mg = gptr(J.matgrid) + (size_t)b * (uint32_t)nvox;
+            // Surface bit words and winner materials, 256 voxels per wave
+            // step: each lane takes 4 consecutive voxels (one 16-byte LDS
+            // read of their keys), the 8 lanes of a 32-voxel word OR their
+            // nibbles together, and each lane writes its 4 material bytes as
+            // one dword of the brick's block in the material grid (the
+            // winner's material, 0 for the other voxels).  Steps whose rows
+            // received no key only zero their surface words.  Keys are
+            // cleared for the next brick.  With materials in the keys the
+            // loop issues no global loads, so its stores never wait (a wait
+            // for a load also waits for earlier stores here).
+            VK_G uint32_t *__restrict__ mg =
+                (VK_G uint32_t *)(gptr(J.matgrid) + (size_t)b * (uint32_t)nvox);
+            const uint32_t tmask = (uint32_t)__ballot(lane < 32u && touched[lane & 31u] != 0);
+            const uint32_t rows_per_step = 256u >> row_shift;  // bd0 >= 64: 1, 2 or 4 rows
             auto rows_out = [&](auto embed_c) {
                 constexpr bool kEmbedded = decltype(embed_c)::value;
-                for (int32_t v0 = (int32_t)wave * 64; v0 < nvox; v0 += 256) {
-                    if (!touched[v0 >> row_shift]) {
-                        if (lane < 2u) wsurf[(v0 >> 5) + (int32_t)lane] = 0u;
+                for (int32_t v0 = (int32_t)wave * 256; v0 < nvox; v0 += 1024) {
+                    const uint32_t rmask = ((1u << rows_per_step) - 1u) << (v0 >> row_shift);
+                    if (!(tmask & rmask)) {
+                        if (lane < 8u) wsurf[(v0 >> 5) + (int32_t)lane] = 0u;
                         continue;
                     }
-                    const uint32_t key = keys[v0 + (int32_t)lane];
-                    const unsigned long long m = __ballot(key != 0u);
-                    if (lane == 0) {
-                        wsurf[v0 >> 5] = __brev((uint32_t)m);
-                        wsurf[(v0 >> 5) + 1] = __brev((uint32_t)(m >> 32));
+                    uint4 *kp = (uint4 *)&keys[v0 + 4 * (int32_t)lane];
+                    const uint4 kv = *kp;
+                    const uint32_t nib = (kv.x ? 8u : 0u) | (kv.y ? 4u : 0u) | (kv.z ? 2u : 0u) |
+                                         (kv.w ? 1u : 0u);
+                    uint32_t w = nib << (28u - 4u * (lane & 7u));
+                    w |= (uint32_t)__shfl_xor((int)w, 1);
+                    w |= (uint32_t)__shfl_xor((int)w, 2);
+                    w |= (uint32_t)__shfl_xor((int)w, 4);
+                    if ((lane & 7u) == 0u) wsurf[(v0 >> 5) + (int32_t)(lane >> 3)] = w;
+                    nsurf += (uint32_t)__popc(nib);
+                    uint32_t m;
+                    if (kEmbedded) {
+                        m = (kv.x & 0xFFu) | (kv.y & 0xFFu) << 8 | (kv.z & 0xFFu) << 16 |
+                            (kv.w & 0xFFu) << 24;
+                    } else {
+                        auto mat = [&](uint32_t key) -> uint32_t {
+                            return key && materials ? (uint32_t)materials[key & 0x3FFFFFFFu] : 0u;
+                        };
+                        m = mat(kv.x) | mat(kv.y) << 8 | mat(kv.z) << 16 | mat(kv.w) << 24;
                     }
-                    nsurf += (uint32_t)__popcll(m);
-                    uint32_t mat = kEmbedded ? key & 0xFFu : 0u;
-                    if (key) {
-                        keys[v0 + (int32_t)lane] = 0u;
-                        if (!kEmbedded && materials) mat = materials[key & 0x3FFFFFFFu];
-                    }
-                    mg[v0 + (int32_t)lane] = (uint8_t)mat;
+                    if (nib) *kp = make_uint4(0u, 0u, 0u, 0u);
+                    mg[(v0 >> 2) + (int32_t)lane] = m;
                 }
             };
             if (embed)
@@ -923,7 +946,10 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
     }
     // One atomic per wave at exit: a shared counter bumped per brick would
     // serialise every workgroup on one address.
-    if (stats && lane == 0 && nsurf) atomicAdd(stats, nsurf);
+    if (stats) {
+        for (int d = kWave / 2; d >= 1; d >>= 1) nsurf += (uint32_t)__shfl_xor((int)nsurf, d);
+        if (lane == 0 && nsurf) atomicAdd(stats, nsurf);
+    }
 }
 
 // Non-empty bricks in index order: flags, then (after an exclusive scan of
