@@ -269,6 +269,28 @@ typedef struct vox_batch_item {
 vox_status vox_voxelize_batch(vox_ctx *ctx, vox_batch_item *items, uint32_t n,
                               vox_schedule schedule, uint32_t streams);
 
+/* ---- replayed batches (no counterpart in the reference) ----------------
+ * A fused batch captured once as a HIP graph and replayed: every stage's
+ * launches with no host round trip in between.  vox_graph_create runs the
+ * batch once (fused, on private buffers of an internal context) to learn the
+ * sizes of its pair lists and the number of erosion passes, then captures
+ * the stages with those sizes (lists 1/8 longer, one pass more).  Each
+ * vox_graph_launch replays the graph and then checks that the lists were
+ * long enough and that erosion reached its fixed point; if not (the meshes
+ * changed since), it runs the batch again without the graph and captures a
+ * new one, so the nodes always equal vox_voxelize_batch's.  The items
+ * (pointers, grids, kinds, splits) are fixed at creation; the contents of the
+ * vertex, index and material arrays may change between launches.  Every item
+ * must be fusable (see vox_voxelize_batch), else VOX_ERR_UNSUPPORTED.
+ * vox_graph_launch returns once the nodes are written. */
+typedef struct vox_graph vox_graph;
+vox_status vox_graph_create(vox_ctx *ctx, const vox_batch_item *items, uint32_t n,
+                            vox_schedule schedule, vox_graph **out);
+vox_status vox_graph_launch(vox_graph *graph);
+/* Launches that fell back to a run without the graph. */
+uint32_t vox_graph_fallbacks(const vox_graph *graph);
+void vox_graph_destroy(vox_graph *graph);
+
 #ifdef __cplusplus
 }
 #endif

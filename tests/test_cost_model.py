@@ -216,3 +216,56 @@ def test_batch_modes_identical_and_match_oracle(gpu_ctx, oracle):
         for i, (o, nb, _) in enumerate(outs):
             assert items[i].status == 0
             assert np.array_equal(o.download(np.uint8, nb), plain[i]), (schedule, streams, i)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("schedule", [va.SCHEDULE_PLAIN, va.SCHEDULE_LEARNED])
+def test_graph_replay_matches_batch_and_falls_back(gpu_ctx, schedule):
+    """A captured fused batch replays to the same nodes as the batch call;
+    when a mesh changes so that it needs longer pair lists than the graph was
+    sized for, the launch falls back to an uncaptured run (same nodes as the
+    batch call on the changed mesh) and captures a new graph."""
+    meshes = W.batch_meshes(n=6, seed=11, tri_lo=200, tri_hi=8000)
+    for m in meshes:
+        m.meta["cube_length"] *= 2.0
+    kinds = [va.NODE_SHORT if int(m.materials.max()) <= 7 else va.NODE_LONG for m in meshes]
+    kinds[2] = va.NODE_SHORT_FCC
+    if schedule == va.SCHEDULE_LEARNED:
+        gpu_ctx.cost_record(True)
+        record_room(gpu_ctx, 25)
+        f, c = gpu_ctx.cost_samples()
+        gpu_ctx.cost_record(False)
+        gpu_ctx.cost_train(f, c, epochs=10, batch=128, lr=3e-3, seed=3)
+    keep = []
+    items, outs = make_batch(gpu_ctx, meshes, kinds, keep)
+
+    def run_batch():
+        gpu_ctx.voxelize_batch(items, schedule, 0)
+        return [o.download(np.uint8, nb) for o, nb, _ in outs]
+
+    def clear():
+        for o, nb, _ in outs:
+            o.upload(np.zeros(nb, np.uint8))
+
+    ref = run_batch()
+    g = gpu_ctx.graph(items, schedule)
+    try:
+        for _ in range(3):
+            clear()
+            g.launch()
+            for i, (o, nb, _) in enumerate(outs):
+                assert np.array_equal(o.download(np.uint8, nb), ref[i]), i
+        assert g.fallbacks == 0
+        # Mesh 0 rewired into large triangles: far more (triangle, tile) pairs.
+        rng = np.random.default_rng(5)
+        idx = rng.permutation(meshes[0].indices.ravel()).astype(np.uint32)
+        keep[1].upload(idx)
+        ref2 = run_batch()
+        for _ in range(2):
+            clear()
+            g.launch()
+            for i, (o, nb, _) in enumerate(outs):
+                assert np.array_equal(o.download(np.uint8, nb), ref2[i]), i
+        assert g.fallbacks == 1
+    finally:
+        g.close()

@@ -82,6 +82,7 @@ EXPORTED = [
     "vox_set_schedule", "vox_set_heavy_threshold", "vox_cost_set_params", "vox_cost_get_params", "vox_cost_record",
     "vox_cost_samples", "vox_cost_train", "vox_mlp_forward", "vox_mlp_grad",
     "vox_voxelize_batch", "vox_set_brick_dims", "vox_get_brick_dims", "vox_autotune_bricks",
+    "vox_graph_create", "vox_graph_launch", "vox_graph_fallbacks", "vox_graph_destroy",
 ]
 
 _lib = None
@@ -137,6 +138,12 @@ def lib():
     L.vox_mlp_forward.argtypes = [P, P, P, U32, P]
     L.vox_mlp_grad.argtypes = [P, P, P, P, U32, P, C.POINTER(F)]
     L.vox_voxelize_batch.argtypes = [P, C.POINTER(BatchItem), U32, I32, U32]
+    L.vox_graph_create.argtypes = [P, C.POINTER(BatchItem), U32, I32, C.POINTER(P)]
+    L.vox_graph_launch.argtypes = [P]
+    L.vox_graph_fallbacks.argtypes = [P]
+    L.vox_graph_fallbacks.restype = U32
+    L.vox_graph_destroy.argtypes = [P]
+    L.vox_graph_destroy.restype = None
     L.vox_set_brick_dims.argtypes = [P, I32, P]
     L.vox_get_brick_dims.argtypes = [P, I32, P]
     L.vox_autotune_bricks.argtypes = [P, C.POINTER(Grid), I32, P, P, P, U32, P, C.POINTER(F)]
@@ -162,6 +169,32 @@ def plan_grid_host(vertices, mode: int, param: float, vox_split_x: int = 1024) -
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
+
+
+class Graph:
+    """A captured fused batch: launch() replays it (vox_graph_launch)."""
+
+    def __init__(self, ctx: "Context", h, items):
+        self.ctx, self.h, self.items = ctx, h, items
+
+    def launch(self):
+        if lib().vox_graph_launch(self.h) != 0:
+            raise VoxError("vox_graph_launch failed")
+
+    @property
+    def fallbacks(self) -> int:
+        return int(lib().vox_graph_fallbacks(self.h))
+
+    def close(self):
+        if self.h:
+            lib().vox_graph_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceBuffer:
@@ -313,6 +346,13 @@ class Context:
         after another, k > 1 k worker streams."""
         self._check(lib().vox_voxelize_batch(self.h, items, len(items), schedule, streams))
 
+    def graph(self, items, schedule: int = 0) -> "Graph":
+        """A replayable fused batch (vox_graph_create); items as for
+        voxelize_batch, fixed for the graph's life."""
+        h = C.c_void_p()
+        self._check(lib().vox_graph_create(self.h, items, len(items), schedule, C.byref(h)))
+        return Graph(self, h, items)
+
     def set_brick_dims(self, dims, fcc: bool = False):
         d = np.array(dims, np.uint32)
         self._check(lib().vox_set_brick_dims(self.h, 1 if fcc else 0, _ptr(d)))
@@ -395,7 +435,8 @@ class Context:
         return bg.download(np.float32, COST_PARAMS), float(loss.value)
 
 
-__all__ = ["Context", "Grid", "BatchItem", "DeviceBuffer", "VoxError", "lib", "device_count",
+__all__ = ["Context", "Grid", "BatchItem", "DeviceBuffer", "Graph", "VoxError", "lib",
+           "device_count",
            "plan_grid_host",
            "NODE_SHORT", "NODE_LONG", "NODE_PARTIAL", "NODE_SHORT_FCC", "NODE_LONG_FCC",
            "PLAN_MAX_DIMENSION", "PLAN_CUBE_LENGTH", "NODE_SIZE", "EXPORTED",

@@ -104,6 +104,16 @@ struct vox_ctx {
     // winner keys (at most 2^22; VOX_EMBED_TRIS lowers it, for tests of the
     // other path)
     uint32_t embed_limit = 1u << 22;
+    // Fixed sizes (graph capture): the pair-list capacities and the number
+    // of erosion passes come from an earlier run instead of read-backs; the
+    // kernels raise ovf[0] when a list was too short, and last_changed points
+    // at the change flag of the last erosion pass.  The last_* fields hold
+    // the sizes the latest run needed.
+    bool fixed = false;
+    uint32_t cap_solid = 0, cap_surf = 0, n_passes = 0;
+    uint32_t *ovf = nullptr;
+    const uint32_t *last_changed = nullptr;
+    uint32_t last_solid = 0, last_surf = 0, last_passes = 0;
     int fcc_bdim[3] = {kFccBX, kFccBY, kFccBZ};
 };
 
@@ -114,6 +124,16 @@ struct vox_ctx {
             ctx->err = std::string(hipGetErrorString(e_)) + " @ " + #expr;          \
             return e_ == hipErrorOutOfMemory ? VOX_ERR_OOM : VOX_ERR_HIP;           \
         }                                                                           \
+    } while (0)
+
+// For entry points whose context is g->w.
+#define VOX_HIP_G(expr)                                        \
+    do {                                                       \
+        hipError_t e_ = (expr);                                \
+        if (e_ != hipSuccess) {                                \
+            g->w->err = std::string(hipGetErrorString(e_));    \
+            return VOX_ERR_HIP;                                \
+        }                                                      \
     } while (0)
 
 #define VOX_TRY(expr)                 \
@@ -465,7 +485,14 @@ vox_status solid_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
     }
     VOX_TRY(scan(ctx, cand_count, cand_off, nslots));
     uint32_t total = 0;
-    VOX_TRY(read_u32(ctx, cand_off + nslots, 1, &total));
+    if (ctx->fixed) {
+        total = ctx->cap_solid;
+        VOX_TRY(fill(ctx, ctx->ovf, 1, 0u));
+    } else {
+        VOX_TRY(read_u32(ctx, cand_off + nslots, 1, &total));
+        ctx->last_solid = total;
+    }
+    uint32_t *ovf = ctx->fixed ? ctx->ovf : nullptr;
     VOX_TRY(buf(ctx, "s_pairs", total, &pairs));
     VOX_TRY(buf(ctx, "s_list", total, &list));
     VOX_TRY(fill(ctx, tile_hist, nTiles, 0u));
@@ -474,17 +501,17 @@ vox_status solid_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
         if (nj > 1)
             k_solid_candidates<true><<<grid_for(total, 256), 256, 0, ctx->stream>>>(
                 D.jobs, tri_job, nslots, ctx->rec, cand_off, cand_off + nslots, cbox, pairs,
-                tile_hist);
+                tile_hist, total, ovf);
         else
             k_solid_candidates<false><<<grid_for(total, 256), 256, 0, ctx->stream>>>(
                 D.jobs, tri_job, nslots, ctx->rec, cand_off, cand_off + nslots, cbox, pairs,
-                tile_hist);
+                tile_hist, total, ovf);
         VOX_HIP(hipGetLastError());
     }
     VOX_TRY(scan(ctx, tile_hist, tile_off, nTiles));
     if (total > 0) {
         k_solid_scatter<<<grid_for(total, 256), 256, 0, ctx->stream>>>(
-            pairs, cand_off + nslots, tile_off, tile_cursor, list);
+            pairs, cand_off + nslots, tile_off, tile_cursor, list, total);
         VOX_HIP(hipGetLastError());
     }
     if (nTiles > 0) {
@@ -555,7 +582,13 @@ vox_status surface_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
     }
     VOX_TRY(scan(ctx, cand, cand_off, nslots));
     uint32_t total = 0;
-    VOX_TRY(read_u32(ctx, cand_off + nslots, 1, &total));
+    if (ctx->fixed) {
+        total = ctx->cap_surf;
+    } else {
+        VOX_TRY(read_u32(ctx, cand_off + nslots, 1, &total));
+        ctx->last_surf = total;
+    }
+    uint32_t *ovf = ctx->fixed ? ctx->ovf : nullptr;
     VOX_TRY(buf(ctx, "f_blist", total, &blist));
     VOX_TRY(fill(ctx, brick_hist, nbt, 0u));
     VOX_TRY(fill(ctx, brick_cursor, nbt, 0u));
@@ -564,11 +597,11 @@ vox_status surface_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
         if (nj > 1)
             k_surf_bin<true><<<grid_for(total, 256), 256, 0, ctx->stream>>>(
                 D.jobs, ctx->tri_job, nslots, cand_off, cand_off + nslots, tri_info, brick_hist,
-                brick_off, brick_cursor, blist, scatter);
+                brick_off, brick_cursor, blist, scatter, total, ovf);
         else
             k_surf_bin<false><<<grid_for(total, 256), 256, 0, ctx->stream>>>(
                 D.jobs, ctx->tri_job, nslots, cand_off, cand_off + nslots, tri_info, brick_hist,
-                brick_off, brick_cursor, blist, scatter);
+                brick_off, brick_cursor, blist, scatter, total, ovf);
         VOX_HIP(hipGetLastError());
         return VOX_OK;
     };
@@ -659,12 +692,30 @@ vox_status erode_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, uint32_t s
     constexpr uint32_t kBatch = 4;
     const uint32_t nj = (uint32_t)P.jobs.size();
     const uint32_t nblocks = P.erode_b.back();
+    ctx->last_changed = nullptr;
     if (nblocks == 0) return VOX_OK;
     uint32_t *changes, *totals, *stamp;
-    VOX_TRY(buf(ctx, "e_changes", (size_t)kBatch * nj, &changes));
-    VOX_TRY(buf(ctx, "e_totals", kBatch, &totals));
+    const uint32_t nbuf = ctx->fixed ? ctx->n_passes : kBatch;  // passes per launch group
+    VOX_TRY(buf(ctx, "e_changes", (size_t)nbuf * nj, &changes));
+    VOX_TRY(buf(ctx, "e_totals", nbuf, &totals));
     VOX_TRY(buf(ctx, "e_stamp", nblocks, &stamp));
     VOX_TRY(fill(ctx, stamp, nblocks, 0u));
+    if (ctx->fixed) {
+        // A set number of passes, no read-back: a pass after one that
+        // changed nothing in a job exits at once for it.  The last pass's
+        // flag tells the caller whether the fixed point was reached.
+        VOX_TRY(fill(ctx, changes, (size_t)nbuf * nj, 0u));
+        VOX_TRY(fill(ctx, totals, nbuf, 0u));
+        for (uint32_t k = 0; k < nbuf; k++) {
+            k_erode<<<xcd_grid(nblocks), 256, 0, ctx->stream>>>(
+                D.jobs, D.erode_b, nj, 0u, UINT32_MAX, 0u, UINT32_MAX, stamp, changes, totals, k,
+                k, k == 0 ? 1 : 0);
+            VOX_HIP(hipGetLastError());
+        }
+        ctx->last_changed = totals + nbuf - 1;
+        ctx->times.erosion_passes += nbuf;
+        return VOX_OK;
+    }
     std::vector<Span> ys{{0u, UINT32_MAX}}, zs{{0u, UINT32_MAX}};
     if (nj == 1) {
         ys = split_axis_max(P.jobs[0].gd.Y, splitY);
@@ -701,6 +752,7 @@ vox_status erode_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, uint32_t s
             }
         }
     ctx->times.erosion_passes += passes;
+    ctx->last_passes = passes;
     return VOX_OK;
 }
 
@@ -725,10 +777,19 @@ vox_status node_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, const uint3
 
 // The whole pipeline for a plan.  splitY/splitZ: voxSplitRes y/z (erosion
 // subspaces; single-job plans only).
+vox_status run_stages(vox_ctx *ctx, Plan &P, const PlanDev &D, uint32_t splitY,
+                      uint32_t splitZ);
+
 vox_status run_plan(vox_ctx *ctx, Plan &P, uint32_t splitY, uint32_t splitZ) {
     PlanDev D;
     record(ctx, 0);
     VOX_TRY(plan_upload(ctx, P, D));
+    return run_stages(ctx, P, D, splitY, splitZ);
+}
+
+// The stages after the job-table upload (what a graph captures).
+vox_status run_stages(vox_ctx *ctx, Plan &P, const PlanDev &D, uint32_t splitY,
+                      uint32_t splitZ) {
     VOX_TRY(solid_stage(ctx, P, D));
     record(ctx, 1);
     if (P.any_mats) VOX_TRY(surface_stage(ctx, P, D));
@@ -1357,6 +1418,157 @@ vox_status vox_voxelize_batch(vox_ctx *ctx, vox_batch_item *items, uint32_t n,
             break;
         }
     return first;
+}
+
+// ---- replayed batches -------------------------------------------------------
+
+struct vox_graph {
+    vox_ctx *w = nullptr;  // private context: stream, buffers, job table
+    std::vector<vox_batch_item> items;
+    int schedule = VOX_SCHEDULE_PLAIN;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const uint32_t *changed = nullptr;  // last erosion pass's change flag (or null)
+    uint32_t fallbacks = 0;
+};
+
+namespace {
+void drop_graph(vox_graph *g) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    g->exec = nullptr;
+    g->graph = nullptr;
+}
+
+vox_status graph_plan(vox_graph *g, Plan &P) {
+    for (const vox_batch_item &it : g->items)
+        VOX_TRY(add_mesh(g->w, P, it.vertices_dev, it.indices_dev, it.nt, it.materials_dev, it.grid,
+                         it.kind, it.materials_dev != nullptr, it.mat_split, it.nodes_dev));
+    return VOX_OK;
+}
+
+// One run without the graph (it also records the sizes the run needed).
+vox_status graph_eager(vox_graph *g) {
+    vox_ctx *w = g->w;
+    w->fixed = false;
+    w->schedule = g->schedule;
+    Plan P;
+    VOX_TRY(graph_plan(g, P));
+    VOX_TRY(run_plan(w, P, UINT32_MAX, UINT32_MAX));
+    return vox_sync(w);
+}
+
+// Captures the stages with the sizes of the last eager run.
+vox_status graph_capture(vox_graph *g) {
+    vox_ctx *w = g->w;
+    drop_graph(g);
+    Plan P;
+    VOX_TRY(graph_plan(g, P));
+    const uint32_t nj = (uint32_t)P.jobs.size();
+    w->cap_solid = w->last_solid + w->last_solid / 8 + 256;
+    w->cap_surf = w->last_surf + w->last_surf / 8 + 256;
+    w->n_passes = w->last_passes + 1;
+    // Every buffer the capture uses is sized now: nothing may allocate or
+    // synchronise while the stream is captured.
+    uint32_t *u;
+    uint2 *pr;
+    VOX_TRY(buf(w, "g_ovf", 1, &w->ovf));
+    VOX_TRY(buf(w, "s_pairs", w->cap_solid, &pr));
+    VOX_TRY(buf(w, "s_list", w->cap_solid, &u));
+    VOX_TRY(buf(w, "f_blist", w->cap_surf, &u));
+    VOX_TRY(buf(w, "e_changes", (size_t)w->n_passes * nj, &u));
+    VOX_TRY(buf(w, "e_totals", w->n_passes, &u));
+    PlanDev D;
+    VOX_TRY(plan_upload(w, P, D));
+    VOX_TRY(vox_sync(w));
+    w->fixed = true;
+    const bool timing = w->timing;
+    w->timing = false;
+    VOX_HIP_G(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+    const vox_status st = run_stages(w, P, D, UINT32_MAX, UINT32_MAX);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(w->stream, &graph);
+    w->fixed = false;
+    w->timing = timing;
+    if (st != VOX_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    VOX_HIP_G(ec);
+    g->graph = graph;
+    g->changed = w->last_changed;
+    VOX_HIP_G(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    return VOX_OK;
+}
+}  // namespace
+
+vox_status vox_graph_create(vox_ctx *ctx, const vox_batch_item *items, uint32_t n,
+                            vox_schedule schedule, vox_graph **out) {
+    if (!ctx || !out || n == 0 || !items) return VOX_ERR_INVALID;
+    if (schedule != VOX_SCHEDULE_PLAIN && schedule != VOX_SCHEDULE_LEARNED) return VOX_ERR_INVALID;
+    *out = nullptr;
+    for (uint32_t i = 0; i < n; i++) {
+        if (!item_ok(items[i])) return fail(ctx, VOX_ERR_INVALID, "graph item " + std::to_string(i) + " is invalid");
+        if (!fusable(items[i]))
+            return fail(ctx, VOX_ERR_UNSUPPORTED,
+                        "graph item " + std::to_string(i) + " erodes in several subspaces");
+    }
+    vox_graph *g = new vox_graph();
+    g->items.assign(items, items + n);
+    g->schedule = schedule;
+    vox_status st = vox_create(ctx->dev, &g->w);
+    if (st != VOX_OK) {
+        delete g;
+        return fail(ctx, st, "cannot create the graph's context");
+    }
+    vox_ctx *w = g->w;
+    w->heavy_cost = ctx->heavy_cost;
+    w->screen_ratio = ctx->screen_ratio;
+    w->embed_limit = ctx->embed_limit;
+    for (int a = 0; a < 3; a++) {
+        w->cart_bdim[a] = ctx->cart_bdim[a];
+        w->fcc_bdim[a] = ctx->fcc_bdim[a];
+    }
+    if (ctx->cost_ready) {
+        std::vector<float> params(kCostParams);
+        st = vox_cost_get_params(ctx, params.data());
+        if (st == VOX_OK) st = vox_cost_set_params(w, params.data());
+    }
+    if (st == VOX_OK) st = graph_eager(g);
+    if (st == VOX_OK) st = graph_capture(g);
+    if (st != VOX_OK) {
+        ctx->err = "graph: " + w->err;
+        vox_graph_destroy(g);
+        return st;
+    }
+    *out = g;
+    return VOX_OK;
+}
+
+vox_status vox_graph_launch(vox_graph *g) {
+    if (!g || !g->exec) return VOX_ERR_INVALID;
+    vox_ctx *w = g->w;
+    VOX_HIP_G(hipSetDevice(w->dev));
+    VOX_HIP_G(hipGraphLaunch(g->exec, w->stream));
+    uint32_t ovf = 0, changed = 0;
+    if (read_u32(w, w->ovf, 1, &ovf) != VOX_OK) return VOX_ERR_HIP;
+    if (g->changed && read_u32(w, g->changed, 1, &changed) != VOX_OK) return VOX_ERR_HIP;
+    if (!ovf && !changed) return VOX_OK;
+    // The lists were too short or erosion had not converged: the meshes need
+    // more than the graph was sized for.  Run without it and capture anew.
+    g->fallbacks++;
+    vox_status st = graph_eager(g);
+    if (st == VOX_OK) st = graph_capture(g);
+    return st;
+}
+
+uint32_t vox_graph_fallbacks(const vox_graph *g) { return g ? g->fallbacks : 0u; }
+
+void vox_graph_destroy(vox_graph *g) {
+    if (!g) return;
+    drop_graph(g);
+    if (g->w) vox_destroy(g->w);
+    delete g;
 }
 
 // ---- brick schedule and cost model ----------------------------------------

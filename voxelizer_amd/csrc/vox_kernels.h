@@ -287,6 +287,9 @@ __device__ __forceinline__ SolidParams solid_params(const Job &J, const float4 *
     return p;
 }
 
+// The (triangle, tile) and (triangle, brick) pair passes take the first
+// min(total, cap) candidates; a total above cap sets *ovf (a replayed graph
+// sized by an earlier run reports that its lists were too short).
 // Per-triangle records: vertices v0, v1, v2 in 48 contiguous bytes
 // (rec[3t] = v0.xyz v1.x, rec[3t+1] = v1.yz v2.xy, rec[3t+2] = v2.z, class
 // word (surface stage), 0, 0), one per triangle slot of every job; tri_job
@@ -305,11 +308,13 @@ __global__ void k_solid_candidates(const Job *__restrict__ jobs,
                                    const uint32_t *__restrict__ cand_off,
                                    const uint32_t *__restrict__ cand_total,
                                    const uint4 *__restrict__ cbox, uint2 *__restrict__ pairs,
-                                   uint32_t *__restrict__ tile_hist);
+                                   uint32_t *__restrict__ tile_hist, uint32_t cap,
+                                   uint32_t *__restrict__ ovf);
 __global__ void k_solid_scatter(const uint2 *__restrict__ pairs,
                                 const uint32_t *__restrict__ cand_total,
                                 const uint32_t *__restrict__ tile_off,
-                                uint32_t *__restrict__ tile_cursor, uint32_t *__restrict__ list);
+                                uint32_t *__restrict__ tile_cursor, uint32_t *__restrict__ list,
+                                uint32_t cap);
 // One workgroup per tile of every job (grid = all tiles).
 __global__ void k_solid_fill(const Job *__restrict__ jobs, const uint32_t *__restrict__ tile_bases,
                              uint32_t njobs, const float4 *__restrict__ rec,
@@ -338,7 +343,7 @@ __global__ void k_surf_bin(const Job *__restrict__ jobs, const uint32_t *__restr
                            const uint4 *__restrict__ tri_info,
                            uint32_t *__restrict__ brick_hist, const uint32_t *__restrict__ brick_off,
                            uint32_t *__restrict__ brick_cursor, uint32_t *__restrict__ blist,
-                           int scatter);
+                           int scatter, uint32_t cap, uint32_t *__restrict__ ovf);
 __global__ void k_surf_brick(const Job *__restrict__ jobs, const uint32_t *__restrict__ brick_bases,
                              uint32_t njobs, const float4 *__restrict__ tri_rec,
                              const uint32_t *__restrict__ brick_off,

@@ -178,10 +178,12 @@ __global__ __launch_bounds__(256) void k_solid_candidates(
     const Job *__restrict__ jobs, const uint32_t *__restrict__ tri_job, uint32_t nslots,
     const float4 *__restrict__ rec, const uint32_t *__restrict__ cand_off,
     const uint32_t *__restrict__ cand_total, const uint4 *__restrict__ cbox,
-    uint2 *__restrict__ pairs, uint32_t *__restrict__ tile_hist) {
+    uint2 *__restrict__ pairs, uint32_t *__restrict__ tile_hist, uint32_t cap,
+    uint32_t *__restrict__ ovf) {
     __shared__ uint32_t trange[2];
     __shared__ AggTable agg;
-    const uint32_t total = *cand_total;
+    const uint32_t full = *cand_total, total = min(full, cap);
+    if (ovf && full > cap && blockIdx.x == 0 && threadIdx.x == 0) *ovf = 1u;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
         __syncthreads();
@@ -234,19 +236,19 @@ __global__ __launch_bounds__(256) void k_solid_candidates(
 template __global__ void k_solid_candidates<false>(const Job *, const uint32_t *, uint32_t,
                                                    const float4 *, const uint32_t *,
                                                    const uint32_t *, const uint4 *, uint2 *,
-                                                   uint32_t *);
+                                                   uint32_t *, uint32_t, uint32_t *);
 template __global__ void k_solid_candidates<true>(const Job *, const uint32_t *, uint32_t,
                                                   const float4 *, const uint32_t *,
                                                   const uint32_t *, const uint4 *, uint2 *,
-                                                  uint32_t *);
+                                                  uint32_t *, uint32_t, uint32_t *);
 
 __global__ __launch_bounds__(256) void k_solid_scatter(const uint2 *__restrict__ pairs,
                                                        const uint32_t *__restrict__ cand_total,
                                                        const uint32_t *__restrict__ tile_off,
                                                        uint32_t *__restrict__ tile_cursor,
-                                                       uint32_t *__restrict__ list) {
+                                                       uint32_t *__restrict__ list, uint32_t cap) {
     __shared__ AggTable agg;
-    const uint32_t n = *cand_total;
+    const uint32_t n = min(*cand_total, cap);
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         __syncthreads();
         agg_clear(agg);
@@ -452,10 +454,11 @@ __global__ __launch_bounds__(256) void k_surf_bin(
     const uint32_t *__restrict__ cand_off, const uint32_t *__restrict__ cand_total,
     const uint4 *__restrict__ tri_info, uint32_t *__restrict__ brick_hist,
     const uint32_t *__restrict__ brick_off, uint32_t *__restrict__ brick_cursor,
-    uint32_t *__restrict__ blist, int scatter) {
+    uint32_t *__restrict__ blist, int scatter, uint32_t cap, uint32_t *__restrict__ ovf) {
     __shared__ uint32_t trange[2];
     __shared__ AggTable agg;
-    const uint32_t total = *cand_total;
+    const uint32_t full = *cand_total, total = min(full, cap);
+    if (ovf && full > cap && blockIdx.x == 0 && threadIdx.x == 0) *ovf = 1u;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
         __syncthreads();
         agg_clear(agg);
@@ -502,11 +505,11 @@ __global__ __launch_bounds__(256) void k_surf_bin(
 template __global__ void k_surf_bin<false>(const Job *, const uint32_t *, uint32_t,
                                            const uint32_t *, const uint32_t *, const uint4 *,
                                            uint32_t *, const uint32_t *, uint32_t *, uint32_t *,
-                                           int);
+                                           int, uint32_t, uint32_t *);
 template __global__ void k_surf_bin<true>(const Job *, const uint32_t *, uint32_t,
                                           const uint32_t *, const uint32_t *, const uint4 *,
                                           uint32_t *, const uint32_t *, uint32_t *, uint32_t *,
-                                          int);
+                                          int, uint32_t, uint32_t *);
 
 // Winner key of a (triangle, voxel) hit; the largest key of a voxel wins
 // (bbType first, then the mesh-local triangle index, as the oracle's rule).
