@@ -269,3 +269,27 @@ def test_graph_replay_matches_batch_and_falls_back(gpu_ctx, schedule):
         assert g.fallbacks == 1
     finally:
         g.close()
+
+
+@pytest.mark.gpu
+def test_graph_single_mesh_with_erosion_subspaces(gpu_ctx, oracle):
+    """A one-mesh graph keeps the mesh's voxSplitRes y/z: erosion subspaces
+    with a fixed pass count each, and the pre-erosion grid for the ids next to
+    a later subspace."""
+    m = W.shoebox_room(seed=41, n_boxes=10, subdiv=3)
+    m.meta["cube_length"] = 0.02
+    vs = (1024, 96, 64)
+    keep = []
+    items, outs = make_batch(gpu_ctx, [m], [va.NODE_SHORT], keep, vox_split=[vs])
+    out, nb, _ = outs[0]
+    ref = oracle.voxelize_nodes(m, oracle.plan_grid(m.vertices, va.PLAN_CUBE_LENGTH, 0.02, 1024),
+                                va.NODE_SHORT, True, vox_split=vs)
+    g = gpu_ctx.graph(items, va.SCHEDULE_PLAIN)
+    try:
+        for _ in range(2):
+            out.upload(np.zeros(nb, np.uint8))
+            g.launch()
+            assert np.array_equal(out.download(np.uint8, nb), ref)
+        assert g.fallbacks == 0
+    finally:
+        g.close()

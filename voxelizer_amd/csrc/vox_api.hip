@@ -266,6 +266,10 @@ struct Plan {
 struct PlanDev {
     Job *jobs;
     uint32_t *tri_b, *tile_b, *pad_b, *brick_b, *erode_b, *node_b, *fcc_first, *fcc_b;
+    // A single job eroded in several subspaces: the pre-erosion copy of its
+    // grid and the flags of the rows / planes where a later subspace starts.
+    uint32_t *pre = nullptr;
+    uint8_t *ystart = nullptr, *zstart = nullptr;
 };
 
 // matSplitRes subspaces over extPartition (src/host_side.cpp:1107-1111).
@@ -401,8 +405,8 @@ vox_status upload(vox_ctx *ctx, void *dst, const void *src, size_t bytes) {
 }
 
 // Arenas of the call and the device job table (one upload: jobs, split
-// tables, base arrays).
-vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D) {
+// tables, base arrays, erosion subspace flags).
+vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D, uint32_t splitY, uint32_t splitZ) {
     const uint32_t nj = (uint32_t)P.jobs.size(), nf = (uint32_t)P.fcc_first.size();
     uint32_t *bits = nullptr, *surf = nullptr, *bhist = nullptr;
     uint8_t *matgrid = nullptr;
@@ -417,11 +421,28 @@ vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D) {
         bases.insert(bases.end(), v->begin(), v->end());
     bases.insert(bases.end(), P.fcc_first.begin(), P.fcc_first.end());
     bases.insert(bases.end(), P.fcc_b.begin(), P.fcc_b.end());
+    // A single job eroded in several subspaces: the ids of nodes next to a
+    // later subspace read the pre-erosion grid (see k_nodes_cart); flags of
+    // the rows and planes where a later subspace starts.
+    std::vector<uint8_t> starts;
+    D.pre = nullptr;
+    D.ystart = D.zstart = nullptr;
+    if (nj == 1 && P.erode_b.back() > 0) {
+        const Job &J = P.jobs[0];
+        std::vector<Span> eys = split_axis_max(J.gd.Y, splitY);
+        std::vector<Span> ezs = split_axis_max(J.gd.Z, splitZ);
+        if (eys.size() > 1 || ezs.size() > 1) {
+            VOX_TRY(buf(ctx, "n_bits_pre", (size_t)J.gd.W * J.gd.Y * J.gd.Z, &D.pre));
+            starts.assign(J.gd.Y + J.gd.Z, 0);
+            for (size_t k = 1; k < eys.size(); k++) starts[eys[k].min] = 1;
+            for (size_t k = 1; k < ezs.size(); k++) starts[J.gd.Y + ezs[k].min] = 1;
+        }
+    }
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t jb = al(sizeof(Job) * nj), sb = al(sizeof(SplitTable) * nj);
-    const size_t bb = bases.size() * 4;
+    const size_t bb = al(bases.size() * 4);
     uint8_t *dev;
-    VOX_TRY(buf(ctx, "j_table", jb + sb + bb, &dev));
+    VOX_TRY(buf(ctx, "j_table", jb + sb + bb + starts.size(), &dev));
     const SplitTable *dsplit = (const SplitTable *)(dev + jb);
     for (uint32_t k = 0; k < nj; k++) {
         Job &J = P.jobs[k];
@@ -433,10 +454,12 @@ vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D) {
             J.matgrid = matgrid + P.mat_off[k];
         }
     }
-    std::vector<uint8_t> host(jb + sb + bb);
+    // One staging copy: jobs, split tables, base arrays, subspace flags.
+    std::vector<uint8_t> host(jb + sb + bb + starts.size());
     std::memcpy(host.data(), P.jobs.data(), sizeof(Job) * nj);
     std::memcpy(host.data() + jb, P.splits.data(), sizeof(SplitTable) * nj);
-    std::memcpy(host.data() + jb + sb, bases.data(), bb);
+    std::memcpy(host.data() + jb + sb, bases.data(), bases.size() * 4);
+    if (!starts.empty()) std::memcpy(host.data() + jb + sb + bb, starts.data(), starts.size());
     VOX_TRY(upload(ctx, dev, host.data(), host.size()));
     D.jobs = (Job *)dev;
     uint32_t *b = (uint32_t *)(dev + jb + sb);
@@ -448,6 +471,10 @@ vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D) {
     D.node_b = (b += nj + 1);
     D.fcc_first = (b += nj + 1);
     D.fcc_b = b + nf;
+    if (!starts.empty()) {
+        D.ystart = dev + jb + sb + bb;
+        D.zstart = D.ystart + P.jobs[0].gd.Y;
+    }
     return VOX_OK;
 }
 
@@ -686,7 +713,10 @@ vox_status surface_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
 // job may have several alloc y/z subspaces, eroded one after another in the
 // reference's order; jobs of a batch have one each.  Passes are launched
 // kBatch at a time; a job's pass after one that changed nothing in it exits
-// at once, so one host read per batch of passes suffices.
+// at once, so one host read per batch of passes suffices.  With fixed sizes
+// (graph capture) each subspace gets ctx->n_passes passes and no read-back;
+// the last pass of every subspace raises the not-converged flag if it still
+// changed something.
 vox_status erode_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, uint32_t splitY,
                        uint32_t splitZ) {
     constexpr uint32_t kBatch = 4;
@@ -694,48 +724,43 @@ vox_status erode_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, uint32_t s
     const uint32_t nblocks = P.erode_b.back();
     ctx->last_changed = nullptr;
     if (nblocks == 0) return VOX_OK;
-    uint32_t *changes, *totals, *stamp;
+    uint32_t *changes, *totals, *stamp, *notconv = nullptr;
     const uint32_t nbuf = ctx->fixed ? ctx->n_passes : kBatch;  // passes per launch group
     VOX_TRY(buf(ctx, "e_changes", (size_t)nbuf * nj, &changes));
     VOX_TRY(buf(ctx, "e_totals", nbuf, &totals));
     VOX_TRY(buf(ctx, "e_stamp", nblocks, &stamp));
     VOX_TRY(fill(ctx, stamp, nblocks, 0u));
     if (ctx->fixed) {
-        // A set number of passes, no read-back: a pass after one that
-        // changed nothing in a job exits at once for it.  The last pass's
-        // flag tells the caller whether the fixed point was reached.
-        VOX_TRY(fill(ctx, changes, (size_t)nbuf * nj, 0u));
-        VOX_TRY(fill(ctx, totals, nbuf, 0u));
-        for (uint32_t k = 0; k < nbuf; k++) {
-            k_erode<<<xcd_grid(nblocks), 256, 0, ctx->stream>>>(
-                D.jobs, D.erode_b, nj, 0u, UINT32_MAX, 0u, UINT32_MAX, stamp, changes, totals, k,
-                k, k == 0 ? 1 : 0);
-            VOX_HIP(hipGetLastError());
-        }
-        ctx->last_changed = totals + nbuf - 1;
-        ctx->times.erosion_passes += nbuf;
-        return VOX_OK;
+        VOX_TRY(buf(ctx, "e_notconv", 1, &notconv));
+        VOX_TRY(fill(ctx, notconv, 1, 0u));
+        ctx->last_changed = notconv;
     }
     std::vector<Span> ys{{0u, UINT32_MAX}}, zs{{0u, UINT32_MAX}};
     if (nj == 1) {
         ys = split_axis_max(P.jobs[0].gd.Y, splitY);
         zs = split_axis_max(P.jobs[0].gd.Z, splitZ);
     }
-    uint32_t passes = 0, gpass = 0;
+    uint32_t passes = 0, most = 0, gpass = 0;
     for (const Span &zsp : zs)
         for (const Span &ysp : ys) {
             bool first = true;
+            uint32_t sub = 0;  // passes this subspace needed
             for (int guard = 0; guard < 1000000; guard++) {
-                VOX_TRY(fill(ctx, changes, (size_t)kBatch * nj, 0u));
-                VOX_TRY(fill(ctx, totals, kBatch, 0u));
-                for (uint32_t k = 0; k < kBatch; k++) {
+                VOX_TRY(fill(ctx, changes, (size_t)nbuf * nj, 0u));
+                VOX_TRY(fill(ctx, totals, nbuf, 0u));
+                for (uint32_t k = 0; k < nbuf; k++) {
                     k_erode<<<xcd_grid(nblocks), 256, 0, ctx->stream>>>(
                         D.jobs, D.erode_b, nj, ysp.min, ysp.max, zsp.min, zsp.max, stamp,
-                        changes, totals, k, gpass + k, (first && k == 0) ? 1 : 0);
+                        changes, totals, k == nbuf - 1 ? notconv : nullptr, k, gpass + k,
+                        (first && k == 0) ? 1 : 0);
                     VOX_HIP(hipGetLastError());
                 }
-                gpass += kBatch;
+                gpass += nbuf;
                 first = false;
+                if (ctx->fixed) {
+                    sub = nbuf;
+                    break;
+                }
                 uint32_t h[kBatch];
                 VOX_TRY(read_u32(ctx, totals, kBatch, h));
                 uint32_t k0 = kBatch;
@@ -745,14 +770,16 @@ vox_status erode_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, uint32_t s
                         break;
                     }
                 if (k0 < kBatch) {
-                    passes += k0 + 1;
+                    sub += k0 + 1;
                     break;
                 }
-                passes += kBatch;
+                sub += kBatch;
             }
+            passes += sub;
+            most = std::max(most, sub);
         }
     ctx->times.erosion_passes += passes;
-    ctx->last_passes = passes;
+    if (!ctx->fixed) ctx->last_passes = most;
     return VOX_OK;
 }
 
@@ -783,7 +810,7 @@ vox_status run_stages(vox_ctx *ctx, Plan &P, const PlanDev &D, uint32_t splitY,
 vox_status run_plan(vox_ctx *ctx, Plan &P, uint32_t splitY, uint32_t splitZ) {
     PlanDev D;
     record(ctx, 0);
-    VOX_TRY(plan_upload(ctx, P, D));
+    VOX_TRY(plan_upload(ctx, P, D, splitY, splitZ));
     return run_stages(ctx, P, D, splitY, splitZ);
 }
 
@@ -794,29 +821,14 @@ vox_status run_stages(vox_ctx *ctx, Plan &P, const PlanDev &D, uint32_t splitY,
     record(ctx, 1);
     if (P.any_mats) VOX_TRY(surface_stage(ctx, P, D));
     record(ctx, 2);
-    uint32_t *pre = nullptr;
-    uint8_t *ystart = nullptr, *zstart = nullptr;
-    if (P.jobs.size() == 1 && P.erode_b.back() > 0) {
-        // Several erosion subspaces: keep the pre-erosion grid for the ids
-        // of nodes next to a later subspace.
+    if (D.pre) {
         const Job &J = P.jobs[0];
-        std::vector<Span> eys = split_axis_max(J.gd.Y, splitY);
-        std::vector<Span> ezs = split_axis_max(J.gd.Z, splitZ);
-        if (eys.size() > 1 || ezs.size() > 1) {
-            const size_t words = (size_t)J.gd.W * J.gd.Y * J.gd.Z;
-            VOX_TRY(buf(ctx, "n_bits_pre", words, &pre));
-            VOX_HIP(hipMemcpyAsync(pre, J.bits, words * 4, hipMemcpyDeviceToDevice, ctx->stream));
-            std::vector<uint8_t> f(J.gd.Y + J.gd.Z, 0);
-            for (size_t k = 1; k < eys.size(); k++) f[eys[k].min] = 1;
-            for (size_t k = 1; k < ezs.size(); k++) f[J.gd.Y + ezs[k].min] = 1;
-            VOX_TRY(buf(ctx, "n_start", f.size(), &ystart));
-            zstart = ystart + J.gd.Y;
-            VOX_TRY(upload(ctx, ystart, f.data(), f.size()));
-        }
+        VOX_HIP(hipMemcpyAsync(D.pre, J.bits, (size_t)J.gd.W * J.gd.Y * J.gd.Z * 4,
+                               hipMemcpyDeviceToDevice, ctx->stream));
     }
     VOX_TRY(erode_stage(ctx, P, D, splitY, splitZ));
     record(ctx, 3);
-    VOX_TRY(node_stage(ctx, P, D, pre, ystart, zstart));
+    VOX_TRY(node_stage(ctx, P, D, D.pre, D.ystart, D.zstart));
     record(ctx, 4);
     return VOX_OK;
 }
@@ -1189,7 +1201,7 @@ vox_status vox_solid_bits(vox_ctx *ctx, const vox_grid *g, const double shift[3]
                     ctx->cart_bdim, false, nullptr, 0, bits_dev));
     PlanDev D;
     record(ctx, 0);
-    VOX_TRY(plan_upload(ctx, P, D));
+    VOX_TRY(plan_upload(ctx, P, D, UINT32_MAX, UINT32_MAX));
     VOX_TRY(solid_stage(ctx, P, D));
     record(ctx, 1);
     if (ctx->timing) {
@@ -1447,6 +1459,12 @@ vox_status graph_plan(vox_graph *g, Plan &P) {
     return VOX_OK;
 }
 
+// Erosion subspaces: a single mesh keeps its voxSplitRes y/z; a batch's
+// meshes each erode whole (they are fusable).
+uint32_t graph_split(const vox_graph *g, int a) {
+    return g->items.size() == 1 ? g->items[0].vox_split[a] : UINT32_MAX;
+}
+
 // One run without the graph (it also records the sizes the run needed).
 vox_status graph_eager(vox_graph *g) {
     vox_ctx *w = g->w;
@@ -1454,7 +1472,7 @@ vox_status graph_eager(vox_graph *g) {
     w->schedule = g->schedule;
     Plan P;
     VOX_TRY(graph_plan(g, P));
-    VOX_TRY(run_plan(w, P, UINT32_MAX, UINT32_MAX));
+    VOX_TRY(run_plan(w, P, graph_split(g, 1), graph_split(g, 2)));
     return vox_sync(w);
 }
 
@@ -1478,14 +1496,15 @@ vox_status graph_capture(vox_graph *g) {
     VOX_TRY(buf(w, "f_blist", w->cap_surf, &u));
     VOX_TRY(buf(w, "e_changes", (size_t)w->n_passes * nj, &u));
     VOX_TRY(buf(w, "e_totals", w->n_passes, &u));
+    VOX_TRY(buf(w, "e_notconv", 1, &u));
     PlanDev D;
-    VOX_TRY(plan_upload(w, P, D));
+    VOX_TRY(plan_upload(w, P, D, graph_split(g, 1), graph_split(g, 2)));
     VOX_TRY(vox_sync(w));
     w->fixed = true;
     const bool timing = w->timing;
     w->timing = false;
     VOX_HIP_G(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
-    const vox_status st = run_stages(w, P, D, UINT32_MAX, UINT32_MAX);
+    const vox_status st = run_stages(w, P, D, graph_split(g, 1), graph_split(g, 2));
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(w->stream, &graph);
     w->fixed = false;
@@ -1509,7 +1528,7 @@ vox_status vox_graph_create(vox_ctx *ctx, const vox_batch_item *items, uint32_t 
     *out = nullptr;
     for (uint32_t i = 0; i < n; i++) {
         if (!item_ok(items[i])) return fail(ctx, VOX_ERR_INVALID, "graph item " + std::to_string(i) + " is invalid");
-        if (!fusable(items[i]))
+        if (n > 1 && !fusable(items[i]))
             return fail(ctx, VOX_ERR_UNSUPPORTED,
                         "graph item " + std::to_string(i) + " erodes in several subspaces");
     }

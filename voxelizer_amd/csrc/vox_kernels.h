@@ -365,11 +365,13 @@ __global__ void k_nonzero_list(const uint32_t *__restrict__ hist, const uint32_t
 // ---------------------------------------------------------------------------
 // Erosion pass over every Cartesian job's row blocks (grid = all blocks);
 // voxels outside [y0,y1) x [z0,z1) are left alone.  changes[pass * njobs + j]
-// is set to 1 if pass `pass` changed job j, totals[pass] if it changed any.
+// is set to 1 if pass `pass` changed job j, totals[pass] if it changed any,
+// and *notconv too when given (the last pass of a fixed pass count).
 __global__ void k_erode(const Job *__restrict__ jobs, const uint32_t *__restrict__ erode_bases,
                         uint32_t njobs, uint32_t y0, uint32_t y1, uint32_t z0, uint32_t z1,
                         uint32_t *__restrict__ stamp, uint32_t *__restrict__ changes,
-                        uint32_t *__restrict__ totals, uint32_t pass, uint32_t gpass, int full);
+                        uint32_t *__restrict__ totals, uint32_t *__restrict__ notconv,
+                        uint32_t pass, uint32_t gpass, int full);
 inline uint32_t erode_blocks(const GridDims &g) {
     return plane_chunks(g.W, g.Y) * ((g.Z + kErodeZ - 1u) / kErodeZ);
 }

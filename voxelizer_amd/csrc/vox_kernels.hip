@@ -988,7 +988,8 @@ __global__ __launch_bounds__(256) void k_erode(const Job *__restrict__ jobs,
                                                uint32_t z0, uint32_t z1,
                                                uint32_t *__restrict__ stamp,
                                                uint32_t *__restrict__ changes,
-                                               uint32_t *__restrict__ totals, uint32_t pass,
+                                               uint32_t *__restrict__ totals,
+                                               uint32_t *__restrict__ notconv, uint32_t pass,
                                                uint32_t gpass, int full) {
     const uint32_t GL = xcd_logical_block(erode_bases[njobs]);  // bases[njobs] = all blocks
     if (GL == UINT32_MAX) return;
@@ -1082,6 +1083,7 @@ __global__ __launch_bounds__(256) void k_erode(const Job *__restrict__ jobs,
     if (lane == 0 && m) {
         changes[pass * njobs + j] = 1u;
         totals[pass] = 1u;
+        if (notconv) *notconv = 1u;
         stamp[L] = gpass + 1u;
     }
 }
