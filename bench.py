@@ -17,20 +17,24 @@ Workloads:
                  and the 128^3 icosphere.
 
 Schedules: "plain" processes surface bricks in index order (and, for a batch,
-the meshes one after another on one stream); "learned" orders the bricks by
-the MLP cost model, trained first by the library's own GPU kernels on
-per-brick durations recorded on a differently seeded mesh (for a batch: all
-meshes fused into each stage's launches, bricks of all meshes in cost
-order).  Both are timed; `value` is the schedule named by --schedule (default
-learned), the other is reported beside it.  A batch also reports the fused
-run with the plain brick order (the model's own share) and 16 worker
-streams.
+the meshes one after another on one stream); "learned" takes the bricks the
+MLP cost model predicts to be expensive first (the model is trained first by
+the library's own GPU kernels on per-brick durations recorded on a
+differently seeded mesh; for a batch: all meshes fused into each stage's
+launches, bricks of all meshes in one list).  Executions: "graph" replays a
+HIP graph of the pipeline captured once (vox_graph_*; the step still plans
+the grid first), "eager" launches it call by call.  Every (schedule,
+execution) is timed; `value` is the one named by --schedule / --execution
+(default learned, graph).  A batch also reports the fused runs with the plain
+brick order (the model's own share) and 16 worker streams.
 
 Prints ONE JSON line.  `value` = allocated nodes per step / wall time per step
 (Gvoxels/s), timed over --steps steps between two device synchronisations.
 `roofline` is for the node-write kernel (k_nodes_cart / k_nodes_fcc): its
 algorithmic bytes (one node write + one solid-bit read per node) over its
-average HIP-event duration on the library's stream during the timed steps;
+average HIP-event duration on the library's stream, from a second run of the
+same steps with events on (eager execution; events cost a synchronisation per
+step, so the timed steps run without them);
 `traffic` is the HBM bytes per launch that rocprofv3 counted for it
 (profiles/traffic_*.json: FETCH_SIZE and WRITE_SIZE scaled by a plain copy).
 `cpu_baseline` times the CPU oracle (the restatement of the reference
@@ -201,34 +205,61 @@ def bench_single(args, va, wl):
     else:
         bdims = ctx.brick_dims(kind in (va.NODE_SHORT_FCC, va.NODE_LONG_FCC))
 
-    results = {}
-    for sched in scheds:
-        ctx.set_schedule(sched)
+    # Graph replay of the same step (the mesh as a one-item batch on the
+    # grid planned above; the step still plans the grid first).
+    item = (va.BatchItem * 1)()
+    it = item[0]
+    it.vertices_dev, it.nv = vbuf.ptr.value, mesh.nv
+    it.indices_dev, it.nt = ibuf.ptr.value, mesh.nt
+    it.materials_dev, it.n_unique = mbuf.ptr.value, int(mesh.materials.max()) + 1
+    it.kind, it.grid = kind, grid
+    it.vox_split[:] = vs
+    it.mat_split[:] = ms
+    it.nodes_dev = out.ptr.value
+
+    def timed(run):
         for _ in range(args.warmup):
-            step()
+            run()
         ctx.sync()
-        # Timed region: K steps between two synchronisations.
-        ctx.enable_timing(True)
-        node_ms, stage = [], {}
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            step()
-            st = ctx.stage_times()
-            node_ms.append(st["nodes_ms"])
-            for k, v in st.items():
-                stage[k] = stage.get(k, 0.0) + v
+            run()
         ctx.sync()
-        elapsed = time.perf_counter() - t0
-        ctx.enable_timing(False)
-        results[sched] = (elapsed, node_ms, {k: v / args.steps for k, v in stage.items()})
+        return time.perf_counter() - t0
+
+    execs = ["eager", "graph"] if not args.only else [args.execution]
+    results = {}  # (schedule, execution) -> seconds for the timed steps
+    for sched in scheds:
+        ctx.set_schedule(sched)
+        for ex in execs:
+            if ex == "eager":
+                results[(sched, ex)] = timed(step)
+            else:
+                gr = ctx.graph(item, sched)
+                results[(sched, ex)] = timed(lambda: (ctx.plan_grid(mode, param), gr.launch()))
+                gr.close()
+    chosen = va.SCHEDULE_LEARNED if args.schedule == "learned" else va.SCHEDULE_PLAIN
+    # HIP events on the library stream over the same number of steps (eager
+    # execution, chosen schedule): per-stage times and the node kernel's
+    # duration for the roofline.
+    ctx.set_schedule(chosen)
+    ctx.enable_timing(True)
+    node_ms, stage = [], {}
+    for _ in range(args.steps):
+        step()
+        st = ctx.stage_times()
+        node_ms.append(st["nodes_ms"])
+        for k, v in st.items():
+            stage[k] = stage.get(k, 0.0) + v
+    ctx.enable_timing(False)
+    stage = {k: v / args.steps for k, v in stage.items()}
     ctx.set_schedule(va.SCHEDULE_PLAIN)
 
-    chosen = va.SCHEDULE_LEARNED if args.schedule == "learned" else va.SCHEDULE_PLAIN
-    other = va.SCHEDULE_PLAIN if chosen == va.SCHEDULE_LEARNED else va.SCHEDULE_LEARNED
-    elapsed, node_ms, stage = results[chosen]
+    elapsed = results[(chosen, args.execution)]
     ms_per_step = elapsed * 1e3 / args.steps
     value = n_nodes / (elapsed / args.steps) / 1e9
-    gv = {s: n_nodes / (results[s][0] / args.steps) / 1e9 for s in results}
+    gv = {f"{'learned' if s == va.SCHEDULE_LEARNED else 'plain'}_{ex}":
+          n_nodes / (t / args.steps) / 1e9 for (s, ex), t in results.items()}
 
     kname = "k_nodes_fcc" if kind in (3, 4) else "k_nodes_cart"
     kern_ms = float(np.mean(node_ms))
@@ -253,11 +284,12 @@ def bench_single(args, va, wl):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded procedural mesh)",
-        "config": {"workload": label, "schedule": args.schedule, "nodes": n_nodes,
+        "config": {"workload": label, "schedule": args.schedule, "execution": args.execution,
+                   "nodes": n_nodes,
                    "dims": list(ctx.node_dims(grid, kind)), "triangles": mesh.nt,
                    "vox_split": list(vs), "mat_split": list(ms), "parallelism": "single GPU",
                    "surface_brick": list(bdims)},
-        "schedules": schedules_block(va, gv, results, other, training),
+        "schedules": schedules_block(gv, args.execution, training),
         "stages_ms": {k: round(v, 4) for k, v in stage.items()},
         "roofline": roof, "cpu_baseline": cpu,
     }
@@ -266,15 +298,17 @@ def bench_single(args, va, wl):
     return res
 
 
-def schedules_block(va, gv, results, other, training):
-    out = {}
-    if va.SCHEDULE_PLAIN in gv:
-        out["plain_gvox_s"] = round(gv[va.SCHEDULE_PLAIN], 4)
-    if va.SCHEDULE_LEARNED in gv:
-        out["learned_gvox_s"] = round(gv[va.SCHEDULE_LEARNED], 4)
-    if len(gv) == 2:
-        out["learned_over_plain"] = round(gv[va.SCHEDULE_LEARNED] / gv[va.SCHEDULE_PLAIN], 4)
-        out["other_stages_ms"] = {k: round(v, 4) for k, v in results[other][2].items()}
+def schedules_block(gv, execution, training):
+    """Gvoxels/s of every (schedule, execution) timed, and learned / plain
+    for the reported execution."""
+    out = {"gvox_s": {k: round(v, 4) for k, v in gv.items()}}
+    p, l = gv.get(f"plain_{execution}"), gv.get(f"learned_{execution}")
+    if p:
+        out["plain_gvox_s"] = round(p, 4)
+    if l:
+        out["learned_gvox_s"] = round(l, 4)
+    if p and l:
+        out["learned_over_plain"] = round(l / p, 4)
     out["cost_model"] = training
     return out
 
@@ -296,6 +330,7 @@ def bench_batch(args, va, wl):
     keep = []
     n_nodes = 0
     out_bytes = 0
+    node_alg = 0.0
     for i, (m, kind) in enumerate(zip(meshes, kinds)):
         g = va.plan_grid_host(m.vertices, va.PLAN_CUBE_LENGTH, m.meta["cube_length"], vs[0])
         bufs = [va.DeviceBuffer(ctx, a.nbytes) for a in (m.vertices, m.indices, m.materials)]
@@ -315,32 +350,49 @@ def bench_batch(args, va, wl):
         it.nodes_dev = out.ptr.value
         n_nodes += nn
         out_bytes += nn * node_bytes(kind)
-    # Modes: (brick order, streams); streams 0 = fused (each stage one launch
-    # over all meshes), 1 = one mesh after another, k = k worker streams.
-    modes = {"plain": (va.SCHEDULE_PLAIN, 1), "fused_plain": (va.SCHEDULE_PLAIN, 0),
-             "learned": (va.SCHEDULE_LEARNED, 0),
-             f"streams{BATCH_STREAMS}_learned": (va.SCHEDULE_LEARNED, BATCH_STREAMS)}
+        node_alg += nn * (node_bytes(kind) + 0.125)  # node write + solid-bit read
+    # Modes: (brick order, streams, graph); streams 0 = fused (each stage one
+    # launch over all meshes), 1 = one mesh after another, k = k worker
+    # streams; graph = the fused batch captured once and replayed.
+    modes = {"plain": (va.SCHEDULE_PLAIN, 1, False),
+             "fused_plain": (va.SCHEDULE_PLAIN, 0, False),
+             "learned": (va.SCHEDULE_LEARNED, 0, False),
+             "fused_plain_graph": (va.SCHEDULE_PLAIN, 0, True),
+             "learned_graph": (va.SCHEDULE_LEARNED, 0, True),
+             f"streams{BATCH_STREAMS}_learned": (va.SCHEDULE_LEARNED, BATCH_STREAMS, False)}
+    reported = ("learned_graph" if args.execution == "graph" else "learned") \
+        if args.schedule == "learned" else "plain"
     if args.only:  # profiling one mode
-        modes = {args.schedule: modes[args.schedule]}
+        modes = {reported: modes[reported]}
     results = {}
-    for name, (sched, streams) in modes.items():
+    for name, (sched, streams, graph) in modes.items():
+        if graph:
+            gr = ctx.graph(items, sched)
+            run = gr.launch
+        else:
+            def run(sched=sched, streams=streams):
+                ctx.voxelize_batch(items, sched, streams)
         for _ in range(args.warmup):
-            ctx.voxelize_batch(items, sched, streams)
+            run()
         ctx.sync()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            ctx.voxelize_batch(items, sched, streams)
+            run()
         ctx.sync()
         results[name] = time.perf_counter() - t0
+        if graph:
+            gr.close()
         print(f"# batch {name}: {results[name] * 1e3 / args.steps:.2f} ms/step", file=sys.stderr,
               flush=True)
+    # Per-stage HIP-event times of one fused run of the reported schedule.
+    sched_r = va.SCHEDULE_LEARNED if args.schedule == "learned" else va.SCHEDULE_PLAIN
     ctx.enable_timing(True)
-    ctx.voxelize_batch(items, *modes[args.schedule])
+    ctx.voxelize_batch(items, sched_r, 0)
     stages = ctx.stage_times()
     ctx.enable_timing(False)
-    elapsed = results[args.schedule]
+    elapsed = results[reported]
     gv = {k: n_nodes / (v / args.steps) / 1e9 for k, v in results.items()}
-    plain_v, learned_v = gv.get("plain"), gv.get("learned")
+    plain_v, learned_v = gv.get("plain"), gv.get(reported if args.schedule == "learned" else "")
 
     def ratio(a, b):
         return round(a / b, 4) if a and b else None
@@ -356,19 +408,25 @@ def bench_batch(args, va, wl):
         "data": "synthetic (seeded procedural meshes)",
         "config": {"workload": label, "schedule": args.schedule, "meshes": len(meshes),
                    "nodes": n_nodes, "triangles": sum(m.nt for m in meshes),
-                   "execution": "fused (each stage one launch over all meshes)"
-                   if args.schedule == "learned" else "one mesh after another",
+                   "execution": {"learned_graph": "fused batch, replayed HIP graph",
+                                 "learned": "fused batch (each stage one launch over all meshes)",
+                                 "plain": "one mesh after another"}[reported],
                    "parallelism": "single GPU"},
         "schedules": {"plain_gvox_s": plain_v and round(plain_v, 4),
                       "learned_gvox_s": learned_v and round(learned_v, 4),
                       "learned_over_plain": ratio(learned_v, plain_v),
                       "modes_gvox_s": {k: round(v, 4) for k, v in gv.items()},
-                      "learned_over_fused_plain": ratio(learned_v, gv.get("fused_plain")),
+                      "learned_over_fused_plain": ratio(gv.get("learned"), gv.get("fused_plain")),
+                      "graph_learned_over_fused_plain_graph":
+                          ratio(gv.get("learned_graph"), gv.get("fused_plain_graph")),
                       "stages_ms": stages, "cost_model": training},
-        "roofline": {"bound": "hbm", "achieved": round(pipe_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+        "roofline": {"bound": "hbm", "achieved": round(node_alg / (stages["nodes_ms"] * 1e-3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(pipe_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "whole batch (node bytes + mesh reads per step)"},
+                     "frac": round(node_alg / (stages["nodes_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "kernel": "k_nodes_cart + k_nodes_fcc (node stage of the fused batch)",
+                     "kernel_ms": round(stages["nodes_ms"], 4), "algorithmic_bytes": int(node_alg),
+                     "pipeline_frac": round(pipe_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
     }
 
@@ -384,6 +442,9 @@ def main():
     ap.add_argument("--only", action="store_true",
                     help="time only the schedule named by --schedule (plain: no cost-model "
                          "training; for profiling one schedule)")
+    ap.add_argument("--execution", default="graph", choices=["graph", "eager"],
+                    help="which execution `value` reports: a captured HIP graph replayed per "
+                         "step, or the pipeline launched call by call (both are timed)")
     ap.add_argument("--autotune", default="on", choices=["on", "off"],
                     help="off: keep the default surface brick shape (profiling runs)")
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
