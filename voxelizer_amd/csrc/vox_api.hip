@@ -81,6 +81,7 @@ struct vox_ctx {
     // persistent surface launch: compute units and resident bricks per CU
     int n_cu = 0;
     int surf_blocks_per_cu = 0;
+    int erode_blocks_per_cu = 0;  // resident erosion workgroups per CU
     // brick schedule of the surface pass and its cost model
     int schedule = VOX_SCHEDULE_PLAIN;
     float *cost_params = nullptr;  // device, kCostParams floats
@@ -740,6 +741,9 @@ vox_status erode_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, uint32_t s
         ys = split_axis_max(P.jobs[0].gd.Y, splitY);
         zs = split_axis_max(P.jobs[0].gd.Z, splitZ);
     }
+    // Persistent passes: at most the resident workgroups (a multiple of 8).
+    const uint32_t grid = std::min<uint32_t>(
+        xcd_grid(nblocks), xcd_grid((uint32_t)(ctx->n_cu * ctx->erode_blocks_per_cu)));
     uint32_t passes = 0, most = 0, gpass = 0;
     for (const Span &zsp : zs)
         for (const Span &ysp : ys) {
@@ -749,7 +753,7 @@ vox_status erode_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, uint32_t s
                 VOX_TRY(fill(ctx, changes, (size_t)nbuf * nj, 0u));
                 VOX_TRY(fill(ctx, totals, nbuf, 0u));
                 for (uint32_t k = 0; k < nbuf; k++) {
-                    k_erode<<<xcd_grid(nblocks), 256, 0, ctx->stream>>>(
+                    k_erode<<<grid, 256, 0, ctx->stream>>>(
                         D.jobs, D.erode_b, nj, ysp.min, ysp.max, zsp.min, zsp.max, stamp,
                         changes, totals, k == nbuf - 1 ? notconv : nullptr, k, gpass + k,
                         (first && k == 0) ? 1 : 0);
@@ -880,7 +884,9 @@ vox_status vox_create(int device, vox_ctx **out) {
     if (hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) !=
             hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->surf_blocks_per_cu, k_surf_brick, 256,
-                                                     0) != hipSuccess) {
+                                                     0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->erode_blocks_per_cu, k_erode, 256, 0) !=
+            hipSuccess) {
         vox_destroy(ctx);
         return VOX_ERR_HIP;
     }
@@ -888,6 +894,7 @@ vox_status vox_create(int device, vox_ctx **out) {
     if (const char *e = std::getenv("VOX_EMBED_TRIS"))
         ctx->embed_limit = (uint32_t)std::min<unsigned long>(std::strtoul(e, nullptr, 10), 1ul << 22);
     ctx->surf_blocks_per_cu = std::max(ctx->surf_blocks_per_cu, 1);
+    ctx->erode_blocks_per_cu = std::max(ctx->erode_blocks_per_cu, 1);
     uint8_t lut[64] = {0};
     for (auto &pr : kOrientPairs) lut[pr[0]] = pr[1];
     if (set_orient_table(lut) != hipSuccess) {

@@ -982,20 +982,14 @@ __global__ __launch_bounds__(256) void k_nonzero_list(const uint32_t *__restrict
 // [y0,y1) x [z0,z1) are left alone.  Plane-linear launch: a block takes
 // kChunk consecutive words of a plane over kErodeZ planes.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_erode(const Job *__restrict__ jobs,
-                                               const uint32_t *__restrict__ erode_bases,
-                                               uint32_t njobs, uint32_t y0, uint32_t y1,
-                                               uint32_t z0, uint32_t z1,
-                                               uint32_t *__restrict__ stamp,
-                                               uint32_t *__restrict__ changes,
-                                               uint32_t *__restrict__ totals,
-                                               uint32_t *__restrict__ notconv, uint32_t pass,
-                                               uint32_t gpass, int full) {
-    const uint32_t GL = xcd_logical_block(erode_bases[njobs]);  // bases[njobs] = all blocks
-    if (GL == UINT32_MAX) return;
-    const uint32_t j = job_of(erode_bases, njobs, GL);
-    if (pass > 0 && changes[(pass - 1u) * njobs + j] == 0u) return;
-    const Job &J = jobs[j];
+// One logical block GL (of job j) of an erosion pass.
+__device__ __forceinline__ void erode_block(const Job &J, uint32_t j, uint32_t GL, uint32_t njobs,
+                                            uint32_t y0, uint32_t y1, uint32_t z0, uint32_t z1,
+                                            uint32_t *__restrict__ stamp,
+                                            uint32_t *__restrict__ changes,
+                                            uint32_t *__restrict__ totals,
+                                            uint32_t *__restrict__ notconv, uint32_t pass,
+                                            uint32_t gpass, int full) {
     const GridDims g = J.gd;
     VK_G uint32_t *__restrict__ bits = gptr(J.bits);
     const uint32_t P = g.W * g.Y;  // words per plane
@@ -1085,6 +1079,52 @@ __global__ __launch_bounds__(256) void k_erode(const Job *__restrict__ jobs,
         totals[pass] = 1u;
         if (notconv) *notconv = 1u;
         stamp[L] = gpass + 1u;
+    }
+}
+
+// Persistent pass: the grid is at most the resident workgroups; each XCD
+// walks a contiguous run of logical blocks (its L2 then holds the neighbour
+// planes).  After the first pass only the jobs the previous pass changed are
+// visited (a batch may have one slow job among many finished ones).
+__global__ __launch_bounds__(256) void k_erode(const Job *__restrict__ jobs,
+                                               const uint32_t *__restrict__ erode_bases,
+                                               uint32_t njobs, uint32_t y0, uint32_t y1,
+                                               uint32_t z0, uint32_t z1,
+                                               uint32_t *__restrict__ stamp,
+                                               uint32_t *__restrict__ changes,
+                                               uint32_t *__restrict__ totals,
+                                               uint32_t *__restrict__ notconv, uint32_t pass,
+                                               uint32_t gpass, int full) {
+    const uint32_t total = erode_bases[njobs];  // bases[njobs] = all blocks
+    const uint32_t nx = gridDim.x / kXcds;      // workgroups per XCD (grid: a multiple of 8)
+    const uint32_t x = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
+    const uint32_t per = (total + kXcds - 1u) / kXcds;
+    const uint32_t lo = x * per, hi = min(lo + per, total);
+    const uint32_t start = lo + slot;  // this workgroup takes start, start + nx, ...
+    if (pass == 0) {
+        for (uint32_t GL = start; GL < hi; GL += nx) {
+            const uint32_t j = job_of(erode_bases, njobs, GL);
+            erode_block(jobs[j], j, GL, njobs, y0, y1, z0, z1, stamp, changes, totals, notconv,
+                        pass, gpass, full);
+        }
+        return;
+    }
+    // Later passes: only jobs the previous pass changed, found 64 at a time
+    // by a ballot over their flags.
+    const uint32_t lane = threadIdx.x & (kWave - 1u);
+    const uint32_t *prev = changes + (pass - 1u) * njobs;
+    for (uint32_t g0 = 0; g0 < njobs; g0 += kWave) {
+        const bool live = g0 + lane < njobs && prev[g0 + lane] != 0u;
+        unsigned long long m = __ballot(live);
+        while (m) {
+            const uint32_t j = g0 + (uint32_t)(__ffsll((long long)m) - 1);
+            m &= m - 1ull;
+            const uint32_t a = max(erode_bases[j], start), e = min(erode_bases[j + 1u], hi);
+            uint32_t GL = a == start ? start : start + ((a - start + nx - 1u) / nx) * nx;
+            for (; GL < e; GL += nx)
+                erode_block(jobs[j], j, GL, njobs, y0, y1, z0, z1, stamp, changes, totals,
+                            notconv, pass, gpass, full);
+        }
     }
 }
 
