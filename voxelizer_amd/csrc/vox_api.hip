@@ -895,11 +895,18 @@ vox_status vox_create(int device, vox_ctx **out) {
         ctx->embed_limit = (uint32_t)std::min<unsigned long>(std::strtoul(e, nullptr, 10), 1ul << 22);
     ctx->surf_blocks_per_cu = std::max(ctx->surf_blocks_per_cu, 1);
     ctx->erode_blocks_per_cu = std::max(ctx->erode_blocks_per_cu, 1);
+    // The node pass evaluates the boundary-id table as logic; check it.
     uint8_t lut[64] = {0};
     for (auto &pr : kOrientPairs) lut[pr[0]] = pr[1];
-    if (set_orient_table(lut) != hipSuccess) {
-        vox_destroy(ctx);
-        return VOX_ERR_HIP;
+    for (uint32_t p = 0; p < 64; p++) {
+        const IdPlanes o = orient_planes(p & 1u, (p >> 1) & 1u, (p >> 2) & 1u, (p >> 3) & 1u,
+                                         (p >> 4) & 1u, (p >> 5) & 1u);
+        uint32_t id = 0;
+        for (int b = 0; b < 5; b++) id |= (o.b[b] & 1u) << b;
+        if (id != lut[p]) {
+            vox_destroy(ctx);
+            return VOX_ERR_INVALID;
+        }
     }
     *out = ctx;
     return VOX_OK;

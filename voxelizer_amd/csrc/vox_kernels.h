@@ -376,14 +376,39 @@ inline uint32_t erode_blocks(const GridDims &g) {
     return plane_chunks(g.W, g.Y) * ((g.Z + kErodeZ - 1u) / kErodeZ);
 }
 
-// Boundary-id table (permutation -> id) in constant memory.
-hipError_t set_orient_table(const uint8_t lut[64]);
+// Boundary ids, bit-sliced: bit b of the ids of 32 voxels from the words of
+// their neighbour bits (mx/px: the -x/+x neighbour is solid; my/py, mz/pz
+// likewise).  This is the permutation -> id table of
+// src/voxelizer.cu:2142-2236 (ids documented at include/voxelizer.h:47-83;
+// permutation bits -x 1, +x 2, -y 4, +y 8, -z 16, +z 32) as two-level logic
+// over its 64 entries, minimised with the 37 permutations that leave an axis
+// without neighbours as don't-cares and then masked to 0 there, as the
+// table has them.  vox_create checks it against the table entry by entry.
+struct IdPlanes {
+    uint32_t b[5];
+};
+__host__ __device__ inline IdPlanes orient_planes(uint32_t mx, uint32_t px, uint32_t my,
+                                                  uint32_t py, uint32_t mz, uint32_t pz) {
+    const uint32_t valid = (mx | px) & (my | py) & (mz | pz);
+    IdPlanes o;
+    o.b[0] = valid & (~px | (mx & ~py) | (mx & my & mz));
+    o.b[1] = valid & ((mx & ~my) | (~mx & ~py) | (mx & py & ~mz) | (mx & mz & px & pz) |
+                      (py & ~px & ~pz));
+    o.b[2] = valid & ((~mz & ~px) | (~mz & ~py) | (mz & ~mx & ~my) | (mx & px & pz & ~my) |
+                      (mx & px & pz & ~py) | (my & px & py & ~pz));
+    o.b[3] = valid & ((mx & px & ~mz) | (my & py & ~px) | (my & mz & py & pz) |
+                      (my & mz & py & ~mx) | (~mx & ~my & ~mz) | (mx & px & ~my & ~pz) |
+                      (mx & px & ~py & ~pz));
+    o.b[4] = valid & ((mz & pz) | (mx & my & px & py) | (my & py & ~mx & ~mz));
+    return o;
+}
 
 // Surface data of one grid as the node passes read it.  Brick b's voxels
 // occupy bytes [b * nvox, (b + 1) * nvox) of matgrid in the brick's word
-// order (z, y, x-word), 32 bytes per word; bytes of non-surface voxels are
-// never written.  Bricks the surface pass did not visit (bhist == 0) hold
-// stale surface words, so the readers check bhist first.
+// order (z, y, x-word), 32 bytes per word.  A word that holds a surface
+// voxel has all 32 bytes written in the call (0 for its other voxels); the
+// rest may be stale.  Bricks the surface pass did not visit (bhist == 0)
+// hold stale surface words, so the readers check bhist first.
 struct MatSource {
     const VK_G uint32_t *surf;   // surface bit words (same layout as bits)
     const VK_G uint32_t *bhist;  // per brick: nonzero if visited

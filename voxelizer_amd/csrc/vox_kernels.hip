@@ -1131,119 +1131,53 @@ __global__ __launch_bounds__(256) void k_erode(const Job *__restrict__ jobs,
 // ---------------------------------------------------------------------------
 // Node writing
 // ---------------------------------------------------------------------------
-__constant__ uint8_t c_orient[64];
+// Buffer offset past every range used below: loads there return 0 and
+// stores there are dropped (raw buffer range check, offset >= size).
+constexpr uint32_t kOob = 0x80000000u;
 
-hipError_t set_orient_table(const uint8_t lut[64]) {
-    return hipMemcpyToSymbol(HIP_SYMBOL(c_orient), lut, 64);
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ __forceinline__ Rsrc rsrc(const VK_G void *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint32_t ld32(Rsrc r, uint32_t off) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
+}
+__device__ __forceinline__ u32x4 ld128(Rsrc r, uint32_t off) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void st128(Rsrc r, uint32_t off, u32x4 v) {
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)off, 0, 0);
 }
 
-// 4 bits (MSB = first node) -> 4 bytes of 0/1 in little-endian node order.
-__device__ __forceinline__ uint32_t spread4(uint32_t nib) {
-    return ((nib >> 3) & 1u) | (((nib >> 2) & 1u) << 8) | (((nib >> 1) & 1u) << 16) |
-           ((nib & 1u) << 24);
-}
-// 2 bits -> 2 halfwords of 0/1.
-__device__ __forceinline__ uint32_t spread2(uint32_t v) {
-    return ((v >> 1) & 1u) | ((v & 1u) << 16);
-}
-
-// Writes the 32 nodes of one bit word.  Fully surrounded nodes get boundary
-// id 27 as a bitwise pattern; the remaining boundary and surface voxels are
-// patched byte-wise in the thread's LDS slot (the 64-entry boundary-id table
-// is in LDS too), which is then stored with 16-byte writes.  md holds the
-// word's first material dword; seg the address of its material segment.
-__device__ __forceinline__ void write_word_nodes(int kind, VK_G void *__restrict__ nodes, size_t wi,
-                                                 uint32_t G, uint32_t L, uint32_t R, uint32_t Yn,
-                                                 uint32_t Yp, uint32_t Zn, uint32_t Zp,
-                                                 uint32_t S, const uint4 &ma, const uint4 &mb,
-                                                 const uint8_t *lut, uint4 *my) {
-    const uint32_t interior = G & L & R & Yn & Yp & Zn & Zp;
-    const uint32_t bnd = G & ~interior;  // solid, some neighbour missing
-    const bool patch = (bnd | S) != 0u;
-    const size_t node0 = wi * 32u;
-    const int nb = kind == 0 ? 1 : 2;  // bytes per node
-    VK_G u32x4 *dst = (VK_G u32x4 *)((VK_G uint8_t *)nodes + node0 * (size_t)nb);
-    if (!patch && (interior == 0u || interior == 0xFFFFFFFFu)) {
-        // All 32 nodes empty, or all fully surrounded (id 27, no material).
-        const uint32_t v = interior ? (kind == 0 ? 0x1B1B1B1Bu : 0x001B001Bu) : 0u;
-        const u32x4 q = {v, v, v, v};
-        dst[0] = q;
-        dst[1] = q;
-        if (kind != 0) {
-            dst[2] = q;
-            dst[3] = q;
-        }
-        return;
-    }
-    // Interior voxels (27) and empty ones (0).
-    if (kind == 0) {
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            uint32_t o[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                o[q] = spread4((interior >> (28 - 4 * (4 * j + q))) & 15u) * 27u;
-            if (patch) my[j] = make_uint4(o[0], o[1], o[2], o[3]);
-            else dst[j] = u32x4{o[0], o[1], o[2], o[3]};
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            uint32_t o[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                o[q] = spread2((interior >> (30 - 2 * (4 * j + q))) & 3u) * 27u;
-            if (patch) my[j] = make_uint4(o[0], o[1], o[2], o[3]);
-            else dst[j] = u32x4{o[0], o[1], o[2], o[3]};
-        }
-    }
-    if (!patch) return;
-    // Boundary voxels: the id of their neighbour permutation.
-    uint32_t m = bnd;
-    while (m) {
-        const int k = __clz(m);
-        const uint32_t bit = 0x80000000u >> k;
-        const uint32_t perm = ((L & bit) ? 1u : 0u) | ((R & bit) ? 2u : 0u) |
-                              ((Yn & bit) ? 4u : 0u) | ((Yp & bit) ? 8u : 0u) |
-                              ((Zn & bit) ? 16u : 0u) | ((Zp & bit) ? 32u : 0u);
-        const uint32_t bid = lut[perm];
-        if (kind == 0)
-            ((uint8_t *)my)[k] = (uint8_t)(bid & 31u);
-        else
-            ((uint16_t *)my)[k] = (uint16_t)bid;
-        m &= ~bit;
-    }
-    // Surface voxels: their material, four voxels (one material dword) at a
-    // time under a byte mask of the surface bits.
-    if (S) {
-        const uint32_t md[8] = {ma.x, ma.y, ma.z, ma.w, mb.x, mb.y, mb.z, mb.w};
-        uint32_t *my32 = (uint32_t *)my;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const uint32_t mm = md[j] & (spread4((S >> (28 - 4 * j)) & 15u) * 0xFFu);
-            if (kind == 0) {
-                my32[j] |= (mm << 5) & 0xE0E0E0E0u;
-            } else {
-                my32[2 * j] |= ((mm & 0xFFu) << 8) | ((mm & 0xFF00u) << 16);
-                my32[2 * j + 1] |= ((mm >> 8) & 0xFF00u) | (mm & 0xFF000000u);
-            }
-        }
-    }
-    const int n16 = 2 * nb;
-    for (int j = 0; j < n16; j++) {
-        const uint4 v = my[j];
-        dst[j] = u32x4{v.x, v.y, v.z, v.w};
-    }
+// Nibble n (bit i = voxel i) -> 4 bytes holding 27 where the bit is set.
+// The multiply moves bit i to bit 8i (its four partial products occupy
+// disjoint bits, so nothing carries); 27 = 0b11011 is then two shifted ORs.
+// The empty asm keeps them apart: folded together they become one 32-bit
+// multiply by 27, which issues at a quarter of the rate of the two ORs.
+__device__ __forceinline__ uint32_t nibble_27(uint32_t n) {
+    const uint32_t s = __umul24(n, 0x204081u) & 0x01010101u;
+    uint32_t t = s | (s << 1);
+    __asm__("" : "+v"(t));
+    return t | (t << 3);
 }
 
 // Cartesian node write: a block takes kChunk consecutive words of a plane
 // (plane-linear, so no lane idles on narrow rows), each thread a column of
-// kNodeZ planes.  Every global load the column needs (the words of planes
-// zb-1..zb+kNodeZ, the rows above and below, the words across the wave's
-// edges, the surface words, the brick's material offset and word
-// ranks) is issued at once from clamped, always-valid addresses, so a wave
-// waits for memory once; the words' first material dwords come in a second
-// round trip.  x neighbours come from adjacent lanes.
+// kNodeZ planes.  All loads go through buffer resources, one per plane, so
+// every address is a 32-bit lane offset and a neighbour outside the grid
+// (z -1 / Z, rows -1 / Y, lanes past the plane) falls outside its range and
+// reads 0 without a branch.  Every load of the column is issued before the
+// first use, so a wave waits for memory once; the 32 material bytes of each
+// surface word come in a second round trip.  x neighbours come from
+// adjacent lanes.
+//
+// A word whose 32 voxels are all empty or all fully surrounded (the bulk of
+// a grid) is stored as a constant.  Otherwise the fully surrounded voxels
+// get id 27 by a nibble spread, boundary voxels (solid, some neighbour
+// missing) get the id of their neighbour permutation from the bit-sliced
+// table (orient_planes), patched byte-wise into the thread's LDS slot, and
+// surface voxels OR in their material (bytes of the word's other voxels are
+// 0 in the material grid).
 //
 // With several erosion subspaces (voxSplitRes y/z smaller than the grid) the
 // reference computes a subspace's ids before later subspaces erode
@@ -1255,52 +1189,46 @@ __global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs
                                                     const uint32_t *__restrict__ pre,
                                                     const uint8_t *__restrict__ ystart,
                                                     const uint8_t *__restrict__ zstart) {
-    __shared__ uint8_t lut[64];
-    __shared__ uint4 slot[256][4];  // 64 output bytes per thread (LongNode; ShortNode uses 32)
+    __shared__ uint4 slot[256][2];  // id bytes of the thread's word while boundary ids go in
     const uint32_t GL = xcd_logical_block(node_bases[njobs]);  // bases[njobs] = all blocks
     if (GL == UINT32_MAX) return;  // uniform over the block
     const Job &J = jobs[job_of(node_bases, njobs, GL)];
     const GridDims g = J.gd;
-    const VK_G uint32_t *__restrict__ solid = gptr(J.bits);
-    const MatSource ms = mat_source(J);
     const int use_mats = J.use_mats, kind = J.kind;
-    VK_G void *__restrict__ nodes = gptr(J.nodes);
     const BrickDims bd = J.bdn;
     const uint32_t P = g.W * g.Y;  // words per plane
     const uint32_t nch = (P + kChunk - 1u) / kChunk;
     const uint32_t lb = GL - J.node_base;  // block within the job
     const uint32_t tid = threadIdx.x, tx = tid & (kWave - 1u);
-    if (tid < 64u) lut[tid] = c_orient[tid];
     const uint32_t zb = (lb / nch) * kNodeZ;
-    const uint32_t i = (lb % nch) * kChunk + tid;
+    const uint32_t i0 = (lb % nch) * kChunk, i = i0 + tid;
     const bool valid = i < P;
     const uint32_t y = valid ? i / g.W : 0u;
     const uint32_t w = valid ? i - y * g.W : 0u;
-    const size_t rowW = g.W, planeW = P;
-    const size_t col0 = valid ? i : 0;
-    const bool hyn = valid && y > 0, hyp = valid && y + 1 < g.Y;
+    const uint32_t P4 = P * 4u, W4 = g.W * 4u;
+    // Byte offsets within a plane: the word, its y neighbours (row -1 wraps
+    // past the range, row Y lands past it) and, for the wave's edge lanes,
+    // its x neighbours in the adjacent waves.
+    const uint32_t o = valid ? i * 4u : kOob;
+    const uint32_t oyn = valid ? o - W4 : kOob;
+    const uint32_t oyp = o + W4;
+    const uint32_t oem = valid && tx == 0u && w > 0u ? o - 4u : kOob;
+    const uint32_t oep = valid && tx == kWave - 1u && w + 1u < g.W ? o + 4u : kOob;
+    const VK_G uint32_t *bits = gptr(J.bits);
+    auto plane = [&](const VK_G uint32_t *base, uint32_t z) {  // z -1 wraps to UINT32_MAX
+        return z < g.Z ? rsrc(base + (size_t)z * P, P4) : rsrc(base, 0u);
+    };
     // Round trip 1.
     uint32_t col[kNodeZ + 2], yn[kNodeZ], yp[kNodeZ], em[kNodeZ], ep[kNodeZ];
 #pragma unroll
-    for (uint32_t k = 0; k < kNodeZ + 2; k++) {
-        const uint32_t z = zb + k - 1u;  // wraps to UINT32_MAX for zb == 0, k == 0
-        const bool ok = valid && z < g.Z;
-        col[k] = solid[ok ? (size_t)z * planeW + col0 : 0];
-        if (!ok) col[k] = 0u;
-    }
+    for (uint32_t k = 0; k < kNodeZ + 2; k++) col[k] = ld32(plane(bits, zb + k - 1u), o);
 #pragma unroll
     for (uint32_t k = 0; k < kNodeZ; k++) {
-        const uint32_t z = zb + k;
-        const bool ok = valid && z < g.Z;
-        const size_t wi = ok ? (size_t)z * planeW + col0 : 0;
-        yn[k] = solid[ok && hyn ? wi - rowW : 0];
-        yp[k] = solid[ok && hyp ? wi + rowW : 0];
-        if (!(ok && hyn)) yn[k] = 0u;
-        if (!(ok && hyp)) yp[k] = 0u;
-        const bool lm = ok && tx == 0 && w > 0;
-        const bool rp = ok && tx == kWave - 1 && w + 1 < g.W;
-        em[k] = lm ? solid[wi - 1] : 0u;
-        ep[k] = rp ? solid[wi + 1] : 0u;
+        const Rsrc r = plane(bits, zb + k);
+        yn[k] = ld32(r, oyn);
+        yp[k] = ld32(r, oyp);
+        em[k] = ld32(r, oem);
+        ep[k] = ld32(r, oep);
     }
     uint32_t ypp[kNodeZ], zpp[kNodeZ];
     bool ys = false, zs[kNodeZ];
@@ -1311,62 +1239,63 @@ __global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs
         zs[k] = false;
     }
     if (pre) {  // uniform
-        ys = hyp && ystart[y + 1] != 0;
+        const VK_G uint32_t *pg = gptr(pre);
+        ys = valid && y + 1u < g.Y && ystart[y + 1u] != 0;
 #pragma unroll
         for (uint32_t k = 0; k < kNodeZ; k++) {
             const uint32_t z = zb + k;
-            const bool ok = valid && z < g.Z;
-            const size_t wi = ok ? (size_t)z * planeW + col0 : 0;
-            const bool hz = ok && z + 1 < g.Z;
-            ypp[k] = pre[ok && hyp ? wi + rowW : 0];
-            zpp[k] = pre[hz ? wi + planeW : 0];
-            zs[k] = hz && zstart[z + 1] != 0;
+            ypp[k] = ld32(plane(pg, z), oyp);
+            zpp[k] = ld32(plane(pg, z + 1u), o);
+            zs[k] = z + 1u < g.Z && zstart[z + 1u] != 0;
         }
     }
-    uint32_t vis = 0u, S[kNodeZ];
-    size_t moff[kNodeZ];  // material block of each plane's word
+    uint32_t S[kNodeZ], moff[kNodeZ];
 #pragma unroll
     for (uint32_t k = 0; k < kNodeZ; k++) {
         S[k] = 0u;
-        moff[k] = 0;
+        moff[k] = kOob;
     }
-    if (use_mats) {  // uniform; the kNodeZ planes share one brick (kNodeZ divides its depth)
-        const uint32_t yy = valid ? y : 0u, ww = valid ? w : 0u;
-        const uint32_t b =
-            ((zb >> bd.lbz) * bd.nby + (yy >> bd.lby)) * bd.nbx + (ww >> bd.lwx);
-        vis = ms.bhist[b];
-        const size_t bbase = (size_t)b << (bd.lwx + bd.lby + bd.lbz + 5u);  // b * voxels
+    Rsrc mr = rsrc(gptr(J.bits), 0u);
+    if (use_mats) {  // uniform; the kNodeZ planes share one brick layer (kNodeZ divides its depth)
+        const VK_G uint32_t *surf = gptr(J.surf);
+#pragma unroll
+        for (uint32_t k = 0; k < kNodeZ; k++) S[k] = ld32(plane(surf, zb + k), o);
+        // Material blocks relative to the first brick row the block touches.
+        const uint32_t lbytes = bd.lwx + bd.lby + bd.lbz + 5u;  // log2 bytes per brick
+        const uint32_t yb0 = (i0 / g.W) >> bd.lby;
+        const uint32_t yb1 = ((min(i0 + kChunk, P) - 1u) / g.W) >> bd.lby;
+        const uint32_t brow0 = ((zb >> bd.lbz) * bd.nby + yb0) * bd.nbx;
+        mr = rsrc(gptr(J.matgrid) + ((size_t)brow0 << lbytes), ((yb1 - yb0 + 1u) * bd.nbx) << lbytes);
+        const uint32_t b = ((y >> bd.lby) - yb0) * bd.nbx + (w >> bd.lwx);
+        const uint32_t vis = valid ? gptr(J.bhist)[brow0 + b] : 0u;
+        const uint32_t lane_off =
+            (b << lbytes) +
+            ((((y & ((1u << bd.lby) - 1u)) << bd.lwx) + (w & ((1u << bd.lwx) - 1u))) << 5);
 #pragma unroll
         for (uint32_t k = 0; k < kNodeZ; k++) {
-            const uint32_t z = zb + k;
-            const bool ok = valid && z < g.Z;
-            const uint32_t wb =
-                ((((z & ((1u << bd.lbz) - 1u)) << bd.lby) + (yy & ((1u << bd.lby) - 1u)))
-                 << bd.lwx) + (ww & ((1u << bd.lwx) - 1u));
-            S[k] = ms.surf[ok ? (size_t)z * planeW + col0 : 0];
-            moff[k] = bbase + (size_t)wb * 32u;
-            if (!ok) S[k] = 0u;
+            if (!vis) S[k] = 0u;
+            moff[k] = lane_off + ((((zb + k) & ((1u << bd.lbz) - 1u)) << (bd.lby + bd.lwx)) << 5);
         }
     }
-    __syncthreads();  // lut ready
     // Round trip 2: the 32 material bytes of each surface word.
-    uint4 ma[kNodeZ], mb[kNodeZ];
+    u32x4 ma[kNodeZ], mb[kNodeZ];
 #pragma unroll
     for (uint32_t k = 0; k < kNodeZ; k++) {
-        if (!vis) S[k] = 0u;
-        ma[k] = make_uint4(0u, 0u, 0u, 0u);
+        ma[k] = u32x4{0u, 0u, 0u, 0u};
         mb[k] = ma[k];
         if (S[k]) {
-            const VK_G u32x4 *src = (const VK_G u32x4 *)(ms.matgrid + moff[k]);
-            const u32x4 x0 = src[0], x1 = src[1];
-            ma[k] = make_uint4(x0.x, x0.y, x0.z, x0.w);
-            mb[k] = make_uint4(x1.x, x1.y, x1.z, x1.w);
+            ma[k] = ld128(mr, moff[k]);
+            mb[k] = ld128(mr, moff[k] + 16u);
         }
     }
-    uint4 *my = slot[tid];
+    const uint32_t nbw = kind == 0 ? 32u : 64u;  // node bytes per word
+    const uint32_t oo = valid ? i * nbw : kOob;
+    VK_G uint8_t *nodes = (VK_G uint8_t *)gptr(J.nodes);
+    uint8_t *sb = (uint8_t *)slot[tid];
 #pragma unroll
     for (uint32_t k = 0; k < kNodeZ; k++) {
         const uint32_t z = zb + k;
+        if (z >= g.Z) break;  // uniform
         const uint32_t G = col[k + 1];
         uint32_t gm = (uint32_t)__shfl_up((int)G, 1);
         uint32_t gp = (uint32_t)__shfl_down((int)G, 1);
@@ -1374,13 +1303,64 @@ __global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs
         if (tx == kWave - 1) gp = ep[k];
         if (w == 0) gm = 0u;
         if (w + 1 >= g.W) gp = 0u;
-        if (!valid || z >= g.Z) continue;
         const uint32_t L = (G >> 1) | (gm << 31);
         const uint32_t R = (G << 1) | (gp >> 31);
+        const uint32_t Yn = yn[k];
         const uint32_t Yp = ys ? ypp[k] : yp[k];
+        const uint32_t Zn = col[k];
         const uint32_t Zp = zs[k] ? zpp[k] : col[k + 2];
-        write_word_nodes(kind, nodes, (size_t)z * planeW + col0, G, L, R, yn[k], Yp, col[k], Zp,
-                         S[k], ma[k], mb[k], lut, my);
+        const uint32_t interior = G & L & R & Yn & Yp & Zn & Zp;
+        const uint32_t bnd = G & ~interior;  // solid, some neighbour missing
+        uint32_t idw[8];
+        if ((bnd | S[k]) == 0u) {
+            // All 32 voxels empty, or all fully surrounded (id 27).
+            const uint32_t v = interior ? 0x1B1B1B1Bu : 0u;
+#pragma unroll
+            for (int j = 0; j < 8; j++) idw[j] = v;
+        } else {
+            const uint32_t ir = __brev(interior);  // bit i = voxel i
+#pragma unroll
+            for (int j = 0; j < 8; j++) idw[j] = nibble_27((ir >> (4 * j)) & 15u);
+            if (bnd) {
+                const IdPlanes op = orient_planes(L, R, Yn, Yp, Zn, Zp);
+                slot[tid][0] = make_uint4(idw[0], idw[1], idw[2], idw[3]);
+                slot[tid][1] = make_uint4(idw[4], idw[5], idw[6], idw[7]);
+                uint32_t m = bnd;
+                while (m) {
+                    const uint32_t s = 31u - (uint32_t)__clz(m);  // voxel 31 - s
+                    const uint32_t id = ((op.b[0] >> s) & 1u) | (((op.b[1] >> s) & 1u) << 1) |
+                                        (((op.b[2] >> s) & 1u) << 2) |
+                                        (((op.b[3] >> s) & 1u) << 3) |
+                                        (((op.b[4] >> s) & 1u) << 4);
+                    sb[31u - s] = (uint8_t)id;
+                    m ^= 1u << s;
+                }
+                const uint4 q0 = slot[tid][0], q1 = slot[tid][1];
+                idw[0] = q0.x; idw[1] = q0.y; idw[2] = q0.z; idw[3] = q0.w;
+                idw[4] = q1.x; idw[5] = q1.y; idw[6] = q1.z; idw[7] = q1.w;
+            }
+        }
+        const uint32_t md[8] = {ma[k].x, ma[k].y, ma[k].z, ma[k].w,
+                                mb[k].x, mb[k].y, mb[k].z, mb[k].w};
+        const Rsrc nr = rsrc(nodes + (size_t)z * P * nbw, P * nbw);
+        if (kind == 0) {  // ShortNode byte: id | material << 5
+            uint32_t out[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) out[j] = idw[j] | ((md[j] << 5) & 0xE0E0E0E0u);
+            st128(nr, oo, u32x4{out[0], out[1], out[2], out[3]});
+            st128(nr, oo + 16u, u32x4{out[4], out[5], out[6], out[7]});
+        } else {  // LongNode halfword: id | material << 8
+            uint32_t out[16];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                out[2 * j] = __builtin_amdgcn_perm(md[j], idw[j], 0x05010400u);
+                out[2 * j + 1] = __builtin_amdgcn_perm(md[j], idw[j], 0x07030602u);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                st128(nr, oo + 16u * q,
+                      u32x4{out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]});
+        }
     }
 }
 
