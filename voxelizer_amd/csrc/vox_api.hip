@@ -794,8 +794,12 @@ vox_status node_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, const uint3
                       const uint8_t *ystart, const uint8_t *zstart) {
     const uint32_t nj = (uint32_t)P.jobs.size(), nf = (uint32_t)P.fcc_first.size();
     if (P.node_b.back() > 0) {
-        k_nodes_cart<<<xcd_grid(P.node_b.back()), 256, 0, ctx->stream>>>(
-            D.jobs, D.node_b, nj, pre, ystart, zstart);
+        if (pre)
+            k_nodes_cart_pre<<<xcd_grid(P.node_b.back()), 256, 0, ctx->stream>>>(
+                D.jobs, D.node_b, nj, pre, ystart, zstart);
+        else
+            k_nodes_cart<<<xcd_grid(P.node_b.back()), 256, 0, ctx->stream>>>(D.jobs, D.node_b,
+                                                                              nj);
         VOX_HIP(hipGetLastError());
     }
     if (nf > 0) {

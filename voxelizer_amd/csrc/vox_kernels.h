@@ -424,8 +424,13 @@ __device__ __forceinline__ MatSource mat_source(const Job &J) {
 // and kind, 3 ShortFCCNode or 4 LongFCCNode) and its blocks start at
 // fcc_bases[f].
 __global__ void k_nodes_cart(const Job *__restrict__ jobs, const uint32_t *__restrict__ node_bases,
-                             uint32_t njobs, const uint32_t *__restrict__ pre,
-                             const uint8_t *__restrict__ ystart, const uint8_t *__restrict__ zstart);
+                             uint32_t njobs);
+// The same with neighbours across subspace seams read from pre (ystart/zstart).
+__global__ void k_nodes_cart_pre(const Job *__restrict__ jobs,
+                                 const uint32_t *__restrict__ node_bases, uint32_t njobs,
+                                 const uint32_t *__restrict__ pre,
+                                 const uint8_t *__restrict__ ystart,
+                                 const uint8_t *__restrict__ zstart);
 inline uint32_t node_blocks(const GridDims &g) {
     return plane_chunks(g.W, g.Y) * ((g.Z + kNodeZ - 1u) / kNodeZ);
 }

@@ -1182,13 +1182,15 @@ __device__ __forceinline__ uint32_t nibble_27(uint32_t n) {
 // With several erosion subspaces (voxSplitRes y/z smaller than the grid) the
 // reference computes a subspace's ids before later subspaces erode
 // (src/host_side.cpp:1183-1206), so neighbours across a +y/+z seam are read
-// from the pre-erosion snapshot `pre` (seam rows flagged in ystart/zstart).
-__global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs,
-                                                    const uint32_t *__restrict__ node_bases,
-                                                    uint32_t njobs,
-                                                    const uint32_t *__restrict__ pre,
-                                                    const uint8_t *__restrict__ ystart,
-                                                    const uint8_t *__restrict__ zstart) {
+// from the pre-erosion snapshot `pre` (seam rows flagged in ystart/zstart);
+// that path is a kernel of its own (k_nodes_cart_pre), so the common one
+// holds no registers for it.
+template <bool kPre>
+__device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
+                                           const uint32_t *__restrict__ node_bases,
+                                           uint32_t njobs, const uint32_t *__restrict__ pre,
+                                           const uint8_t *__restrict__ ystart,
+                                           const uint8_t *__restrict__ zstart) {
     __shared__ uint4 slot[256][2];  // id bytes of the thread's word while boundary ids go in
     const uint32_t GL = xcd_logical_block(node_bases[njobs]);  // bases[njobs] = all blocks
     if (GL == UINT32_MAX) return;  // uniform over the block
@@ -1207,19 +1209,20 @@ __global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs
     const uint32_t w = valid ? i - y * g.W : 0u;
     const uint32_t P4 = P * 4u, W4 = g.W * 4u;
     // Byte offsets within a plane: the word, its y neighbours (row -1 wraps
-    // past the range, row Y lands past it) and, for the wave's edge lanes,
-    // its x neighbours in the adjacent waves.
+    // past the range, row Y lands past it) and, for the wave's first and
+    // last lane, the x neighbour held by the adjacent wave.
     const uint32_t o = valid ? i * 4u : kOob;
     const uint32_t oyn = valid ? o - W4 : kOob;
     const uint32_t oyp = o + W4;
-    const uint32_t oem = valid && tx == 0u && w > 0u ? o - 4u : kOob;
-    const uint32_t oep = valid && tx == kWave - 1u && w + 1u < g.W ? o + 4u : kOob;
+    const uint32_t oex = valid && tx == 0u && w > 0u                 ? o - 4u
+                         : valid && tx == kWave - 1u && w + 1u < g.W ? o + 4u
+                                                                      : kOob;
     const VK_G uint32_t *bits = gptr(J.bits);
     auto plane = [&](const VK_G uint32_t *base, uint32_t z) {  // z -1 wraps to UINT32_MAX
         return z < g.Z ? rsrc(base + (size_t)z * P, P4) : rsrc(base, 0u);
     };
     // Round trip 1.
-    uint32_t col[kNodeZ + 2], yn[kNodeZ], yp[kNodeZ], em[kNodeZ], ep[kNodeZ];
+    uint32_t col[kNodeZ + 2], yn[kNodeZ], yp[kNodeZ], ex[kNodeZ];
 #pragma unroll
     for (uint32_t k = 0; k < kNodeZ + 2; k++) col[k] = ld32(plane(bits, zb + k - 1u), o);
 #pragma unroll
@@ -1227,8 +1230,7 @@ __global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs
         const Rsrc r = plane(bits, zb + k);
         yn[k] = ld32(r, oyn);
         yp[k] = ld32(r, oyp);
-        em[k] = ld32(r, oem);
-        ep[k] = ld32(r, oep);
+        ex[k] = ld32(r, oex);  // the x neighbour across the wave's edge (edge lanes)
     }
     uint32_t ypp[kNodeZ], zpp[kNodeZ];
     bool ys = false, zs[kNodeZ];
@@ -1238,7 +1240,7 @@ __global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs
         zpp[k] = 0u;
         zs[k] = false;
     }
-    if (pre) {  // uniform
+    if (kPre) {
         const VK_G uint32_t *pg = gptr(pre);
         ys = valid && y + 1u < g.Y && ystart[y + 1u] != 0;
 #pragma unroll
@@ -1299,16 +1301,19 @@ __global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs
         const uint32_t G = col[k + 1];
         uint32_t gm = (uint32_t)__shfl_up((int)G, 1);
         uint32_t gp = (uint32_t)__shfl_down((int)G, 1);
-        if (tx == 0) gm = em[k];
-        if (tx == kWave - 1) gp = ep[k];
+        if (tx == 0) gm = ex[k];
+        if (tx == kWave - 1) gp = ex[k];
         if (w == 0) gm = 0u;
         if (w + 1 >= g.W) gp = 0u;
         const uint32_t L = (G >> 1) | (gm << 31);
         const uint32_t R = (G << 1) | (gp >> 31);
+        // Seam selects as masks: a select between two array elements
+        // becomes a select between the arrays, which then live in scratch.
+        const uint32_t ysm = ys ? ~0u : 0u, zsm = zs[k] ? ~0u : 0u;
         const uint32_t Yn = yn[k];
-        const uint32_t Yp = ys ? ypp[k] : yp[k];
+        const uint32_t Yp = (yp[k] & ~ysm) | (ypp[k] & ysm);
         const uint32_t Zn = col[k];
-        const uint32_t Zp = zs[k] ? zpp[k] : col[k + 2];
+        const uint32_t Zp = (col[k + 2] & ~zsm) | (zpp[k] & zsm);
         const uint32_t interior = G & L & R & Yn & Yp & Zn & Zp;
         const uint32_t bnd = G & ~interior;  // solid, some neighbour missing
         uint32_t idw[8];
@@ -1362,6 +1367,19 @@ __global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs
                       u32x4{out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]});
         }
     }
+}
+__global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs,
+                                                    const uint32_t *__restrict__ node_bases,
+                                                    uint32_t njobs) {
+    nodes_cart<false>(jobs, node_bases, njobs, nullptr, nullptr, nullptr);
+}
+// The seam path holds 8 more registers per thread; it runs at one wave per
+// SIMD less rather than spill.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 5))) void k_nodes_cart_pre(
+    const Job *__restrict__ jobs, const uint32_t *__restrict__ node_bases, uint32_t njobs,
+    const uint32_t *__restrict__ pre, const uint8_t *__restrict__ ystart,
+    const uint8_t *__restrict__ zstart) {
+    nodes_cart<true>(jobs, node_bases, njobs, pre, ystart, zstart);
 }
 
 // ---------------------------------------------------------------------------
