@@ -655,14 +655,17 @@ __device__ __forceinline__ void surf_item(const SurfParams &p, const SplitTable 
         for (int j = 0; j < kSurfCols; j++) {
             const int32_t kk = (int32_t)k * kSurfCols + j;
             if (kk >= len) break;
-            key_write(keys, touched, st.base + kk * st.s[2], nvox, row_shift, st.key);
+            key_write(keys, touched, st.base + (int32_t)__umul24(kk, st.s[2]), nvox, row_shift,
+                      st.key);
         }
         return;
     }
     // item k covers columns u0 .. u0+kSurfCols-1 of row v:
     // k = (v - lo_v) * ngu + (u0 - lo_u) / kSurfCols
+    // Strides and indices here are below 2^24 (bricks hold at most 4096
+    // voxels), so products use the full-rate 24-bit multiply.
     int32_t q = (int32_t)((float)k * st.rngu);
-    int32_t r = (int32_t)k - q * st.ngu;
+    int32_t r = (int32_t)k - (int32_t)__umul24(q, st.ngu);
     if (r < 0) { q--; r += st.ngu; }
     if (r >= st.ngu) { q++; r -= st.ngu; }
     const int32_t v = st.lo[1] + q;
@@ -687,7 +690,8 @@ __device__ __forceinline__ void surf_item(const SurfParams &p, const SplitTable 
 #pragma unroll
         for (int e = 0; e < 3; e++) in &= (st.dom[e][0] * pu + kdom[e]) + st.dom[e][2] >= 0.0f;
         if (!in) continue;
-        const int32_t col = st.base + cu * st.s[0] + q * st.s[1];
+        const int32_t col =
+            st.base + (int32_t)__umul24(cu, st.s[0]) + (int32_t)__umul24(q, st.s[1]);
         if (!threeD) {
             key_write(keys, touched, col, nvox, row_shift, st.key);
             continue;
@@ -698,37 +702,45 @@ __device__ __forceinline__ void surf_item(const SurfParams &p, const SplitTable 
         depth_range(t0, pp, nn, p.m, p.d, a, lo, hi);
         const int32_t ilo = max(lo, st.lo[2]), ihi = min(hi, st.hi[2]);
         if (ilo > ihi) continue;
-        // Constant parts of the two side-plane edge functions along the column.
-        float kc[2][3];
+        // The two side planes' edge functions along the column,
+        // (c * pa + k) + e, with their constant parts in registers.
+        float sc[6], sk[6], se[6];
 #pragma unroll
         for (int q2 = 0; q2 < 2; q2++) {
             const float po = (flags >> (4 + q2)) & 1u ? pv : pu;
 #pragma unroll
-            for (int e = 0; e < 3; e++) kc[q2][e] = st.pl[q2][e][1] * po;
+            for (int e = 0; e < 3; e++) {
+                sc[3 * q2 + e] = st.pl[q2][e][0];
+                sk[3 * q2 + e] = st.pl[q2][e][1] * po;
+                se[3 * q2 + e] = st.pl[q2][e][2];
+            }
         }
-        // matSplitRes interval of the first voxel along a, advanced as i grows.
+        // matSplitRes interval of the voxel along a (advanced as i grows)
+        // and the single-voxel shortcut's range clipped to it.
         uint32_t si = 0;
         while (si + 1 < ns && (uint32_t)ilo >= spl.hi[a][si]) si++;
+        uint32_t shi = spl.hi[a][si];
+        int32_t rl = max(lo, max(st.vmin_a, (int32_t)spl.lo[a][si]));
+        int32_t rh = min(hi, min(st.vmax_a, (int32_t)shi - 1));
+        const int32_t sa = st.s[2], col0 = col - __mul24(st.lo[2], sa);  // lo may be -1
         for (int32_t i = ilo; i <= ihi; i++) {
-            while (si + 1 < ns && (uint32_t)i >= spl.hi[a][si]) si++;
-            const int32_t slo = (int32_t)spl.lo[a][si], shi = (int32_t)spl.hi[a][si];
-            const int32_t bl = max(st.vmin_a, slo), bh = min(st.vmax_a, shi - 1);
-            const int32_t rl = lo < bl ? bl : lo;
-            const int32_t rh = hi > bh ? bh : hi;
+            if (si + 1 < ns && (uint32_t)i >= shi) {
+                do si++;
+                while (si + 1 < ns && (uint32_t)i >= spl.hi[a][si]);
+                shi = spl.hi[a][si];
+                rl = max(lo, max(st.vmin_a, (int32_t)spl.lo[a][si]));
+                rh = min(hi, min(st.vmax_a, (int32_t)shi - 1));
+            }
             bool write = rl == rh;
             if (!write) {
                 const float pa = vox_corner(i, ma, p.d);
-                bool ok = true;
+                float f[6];
 #pragma unroll
-                for (int q2 = 0; q2 < 2; q2++)
-#pragma unroll
-                    for (int e = 0; e < 3; e++)
-                        ok &= (st.pl[q2][e][0] * pa + kc[q2][e]) + st.pl[q2][e][2] >= 0.0f;
-                write = ok;
+                for (int e = 0; e < 6; e++) f[e] = (sc[e] * pa + sk[e]) + se[e];
+                write = f[0] >= 0.0f && f[1] >= 0.0f && f[2] >= 0.0f && f[3] >= 0.0f &&
+                        f[4] >= 0.0f && f[5] >= 0.0f;
             }
-            if (write)
-                key_write(keys, touched, col + (i - st.lo[2]) * st.s[2], nvox, row_shift,
-                          st.key);
+            if (write) key_write(keys, touched, col0 + __mul24(i, sa), nvox, row_shift, st.key);
         }
     }
 }

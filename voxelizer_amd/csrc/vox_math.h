@@ -42,7 +42,14 @@ __device__ __forceinline__ uint32_t sat_u32(double v) {
     if (v <= -1.0) return 0u;
     return (uint32_t)v;
 }
-__device__ __forceinline__ int32_t f2i(float v) { return sat_i32((double)v); }
+// sat_i32((double)v) without the double: NaN -> 0, v >= 2^31 -> INT32_MAX,
+// v < -2^31 -> INT32_MIN (the largest float below -2^31 is -2^31 - 128, so
+// the double bound -2^31 - 1 selects the same floats), else truncation.
+// Written as selects so that it compiles to compares, not branches.
+__device__ __forceinline__ int32_t f2i(float v) {
+    const int32_t t = (v >= -2147483648.0f && v < 2147483648.0f) ? (int32_t)v : 0;
+    return v != v ? 0 : v >= 2147483648.0f ? INT32_MAX : v < -2147483648.0f ? INT32_MIN : t;
+}
 
 __device__ __forceinline__ D3 fetch_d(const float *__restrict__ V, const uint32_t *__restrict__ I,
                                       uint32_t t, int k) {
