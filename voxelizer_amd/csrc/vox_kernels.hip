@@ -1420,7 +1420,12 @@ __device__ __forceinline__ uint32_t fcc_word(const FccGrids &G, const GridDims &
         fz >= (int64_t)(2 * g.Z))
         return 0u;
     const int odd = (int)(fz & 1);
-    const VK_G uint32_t *ge = G.s[odd ? 2 : 0], *go = G.s[odd ? 3 : 1];
+    // Picked by arithmetic: a select between elements of the pointer array
+    // becomes a computed index, which puts the array in scratch.
+    const uintptr_t e0 = (uintptr_t)G.s[0], o0 = (uintptr_t)G.s[1];
+    const uintptr_t k = (uintptr_t)odd;
+    const VK_G uint32_t *ge = (const VK_G uint32_t *)(e0 + k * ((uintptr_t)G.s[2] - e0));
+    const VK_G uint32_t *go = (const VK_G uint32_t *)(o0 + k * ((uintptr_t)G.s[3] - o0));
     const size_t wi = ((size_t)(fz >> 1) * g.Y + (size_t)y) * g.W + (size_t)(fw >> 1);
     const int sh = (fw & 1) ? 0 : 16;
     const uint32_t e = (ge[wi] >> sh) & 0xFFFFu, o = (go[wi] >> sh) & 0xFFFFu;
@@ -1448,6 +1453,7 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(const Job *__restrict__ jobs,
                                                    const uint32_t *__restrict__ fcc_first,
                                                    const uint32_t *__restrict__ fcc_bases,
                                                    uint32_t nfcc) {
+    __shared__ uint4 fslot[256][4];  // 32 node masks (u16) per thread while boundary ids go in
     const uint32_t GL = xcd_logical_block(fcc_bases[nfcc]);  // bases[nfcc] = all blocks
     if (GL == UINT32_MAX) return;
     const uint32_t f = job_of(fcc_bases, nfcc, GL);
@@ -1458,8 +1464,6 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(const Job *__restrict__ jobs,
     const BrickDims bd = Js[0].bdn;
     const VK_G uint32_t *s1 = gptr(Js[0].bits), *s2 = gptr(Js[1].bits), *s3 = gptr(Js[2].bits),
                         *s4 = gptr(Js[3].bits);
-    const MatSource m1 = mat_source(Js[0]), m2 = mat_source(Js[1]), m3 = mat_source(Js[2]),
-                    m4 = mat_source(Js[3]);
     const uint32_t FW = 2 * g.W, FY = g.Y;
     const uint32_t nxc = (FW + kWave - 1) / kWave, ny4 = (FY + 3) / 4;
     const uint32_t L = GL - fcc_bases[f];  // block within the lattice
@@ -1499,24 +1503,44 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(const Job *__restrict__ jobs,
     uint32_t interior = M0;
 #pragma unroll
     for (int j = 0; j < 12; j++) interior &= n[j];
-    uint32_t mask[32];
+    const uint32_t boundary = M0 & ~interior;
+    // Neighbour masks, two nodes per dword (node 2d in the low half): 4095
+    // for fully surrounded nodes, 0 for empty ones; boundary nodes get their
+    // 12 neighbour bits, patched halfword-wise in the thread's LDS slot.
+    const uint32_t ir = __brev(interior);  // bit k = node k
+    uint32_t mk[16];
 #pragma unroll
-    for (int k = 0; k < 32; k++) mask[k] = ((interior >> (31 - k)) & 1u) ? 4095u : 0u;
-    uint32_t boundary = M0 & ~interior;
-    while (boundary) {
-        const int k = __clz(boundary);
-        const uint32_t bit = 0x80000000u >> k;
-        uint32_t m = 0;
-#pragma unroll
-        for (int j = 0; j < 12; j++) m |= ((n[j] & bit) ? 1u : 0u) << j;
-#pragma unroll
-        for (int kk = 0; kk < 32; kk++)
-            if (kk == k) mask[kk] = m;
-        boundary &= ~bit;
+    for (int d = 0; d < 16; d++) {
+        const uint32_t two = (ir >> (2 * d)) & 3u;
+        mk[d] = ((two & 1u) ? 0x0FFFu : 0u) | ((two & 2u) ? 0x0FFF0000u : 0u);
     }
-    uint32_t mat[32];
+    if (boundary) {
+        uint4 *my = fslot[threadIdx.y * kWave + threadIdx.x];
 #pragma unroll
-    for (int k = 0; k < 32; k++) mat[k] = 0u;
+        for (int q = 0; q < 4; q++)
+            my[q] = make_uint4(mk[4 * q], mk[4 * q + 1], mk[4 * q + 2], mk[4 * q + 3]);
+        uint16_t *mh = (uint16_t *)my;
+        uint32_t m = boundary;
+        while (m) {
+            const uint32_t s = 31u - (uint32_t)__clz(m);  // node 31 - s
+            uint32_t v = 0u;
+#pragma unroll
+            for (int j = 0; j < 12; j++) v |= ((n[j] >> s) & 1u) << j;
+            mh[31u - s] = (uint16_t)v;
+            m ^= 1u << s;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 v = my[q];
+            mk[4 * q] = v.x; mk[4 * q + 1] = v.y; mk[4 * q + 2] = v.z; mk[4 * q + 3] = v.w;
+        }
+    }
+    // Materials: node k is voxel k / 2 of the sub-grid with x parity k & 1,
+    // whose surface word's half for this FCC word holds 16 voxels.  A word
+    // with a surface voxel has all 32 material bytes written in the call (0
+    // for its other voxels), so each half is read as 16 bytes, or taken as 0
+    // when it has no surface voxel.
+    uint32_t ma[2][4] = {{0u, 0u, 0u, 0u}, {0u, 0u, 0u, 0u}};
     if (use_mats) {
         const int odd = (int)(fz & 1);
         const uint64_t gw = (uint64_t)(fw >> 1), gz = (uint64_t)(fz >> 1);
@@ -1529,46 +1553,43 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(const Job *__restrict__ jobs,
                              ((fw & 1) ? 16u : 0u);
         const size_t gwi = (gz * g.Y + (uint64_t)y) * g.W + gw;
         const int sh = (fw & 1) ? 0 : 16;
-        for (int par = 0; par < 2; par++) {
-            const int gi = odd ? 2 + par : par;
-            const MatSource &src = gi == 0 ? m1 : gi == 1 ? m2 : gi == 2 ? m3 : m4;
-            if (!src.bhist[b]) continue;
-            const uint32_t sw = src.surf[gwi];
-            uint32_t h = (sw >> sh) & 0xFFFFu;
-            if (!h) continue;
-            const VK_G uint8_t *mg = src.matgrid + bbase;  // this half-word's 16 voxels
-            while (h) {
-                const int i = __clz(h) - 16;  // voxel i of the 16
-                const uint32_t mv = mg[i];
-                const int k = 2 * i + par;
 #pragma unroll
-                for (int kk = 0; kk < 32; kk++)
-                    if (kk == k) mat[kk] = mv;
-                h &= ~(0x8000u >> i);
-            }
+        for (int par = 0; par < 2; par++) {
+            const MatSource src = mat_source(Js[odd ? 2 + par : par]);  // uniform
+            if (!src.bhist[b]) continue;
+            if (((src.surf[gwi] >> sh) & 0xFFFFu) == 0u) continue;
+            const u32x4 v = *(const VK_G u32x4 *)(src.matgrid + bbase);
+            ma[par][0] = v.x; ma[par][1] = v.y; ma[par][2] = v.z; ma[par][3] = v.w;
         }
     }
     const size_t node0 = (((size_t)fz * FY + (size_t)y) * FW + (size_t)fw) * 32u;
     if (kind == 3) {  // ShortFCCNode: mat<<12 | bid&4095
         uint32_t out[16];
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-            uint32_t lo = ((mat[2 * j] << 12) | (mask[2 * j] & 4095u)) & 0xFFFFu;
-            uint32_t hi = ((mat[2 * j + 1] << 12) | (mask[2 * j + 1] & 4095u)) & 0xFFFFu;
-            out[j] = lo | (hi << 16);
+        for (int d = 0; d < 16; d++) {
+            const uint32_t a = (ma[0][d >> 2] >> (8 * (d & 3))) & 0xFu;
+            const uint32_t c = (ma[1][d >> 2] >> (8 * (d & 3))) & 0xFu;
+            out[d] = mk[d] | (a << 12) | (c << 28);
         }
-        uint4 *dst = (uint4 *)((uint8_t *)nodes + node0 * 2u);
+        VK_G u32x4 *dst = (VK_G u32x4 *)((VK_G uint8_t *)gptr(nodes) + node0 * 2u);
 #pragma unroll
-        for (int j = 0; j < 4; j++)
-            dst[j] = make_uint4(out[4 * j], out[4 * j + 1], out[4 * j + 2], out[4 * j + 3]);
+        for (int q = 0; q < 4; q++)
+            dst[q] = u32x4{out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]};
     } else {  // LongFCCNode: {u16 bid, u8 mat, u8 0}
-        uint4 *dst = (uint4 *)((uint8_t *)nodes + node0 * 4u);
+        VK_G u32x4 *dst = (VK_G u32x4 *)((VK_G uint8_t *)gptr(nodes) + node0 * 4u);
 #pragma unroll
-        for (int j = 0; j < 8; j++)
-            dst[j] = make_uint4(mask[4 * j] | (mat[4 * j] & 0xFFu) << 16,
-                                mask[4 * j + 1] | (mat[4 * j + 1] & 0xFFu) << 16,
-                                mask[4 * j + 2] | (mat[4 * j + 2] & 0xFFu) << 16,
-                                mask[4 * j + 3] | (mat[4 * j + 3] & 0xFFu) << 16);
+        for (int q = 0; q < 8; q++) {
+            uint32_t o[4];
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const int d = 2 * q + e;
+                const uint32_t a = (ma[0][d >> 2] >> (8 * (d & 3))) & 0xFFu;
+                const uint32_t c = (ma[1][d >> 2] >> (8 * (d & 3))) & 0xFFu;
+                o[2 * e] = (mk[d] & 0xFFFFu) | (a << 16);
+                o[2 * e + 1] = (mk[d] >> 16) | (c << 16);
+            }
+            dst[q] = u32x4{o[0], o[1], o[2], o[3]};
+        }
     }
 }
 
