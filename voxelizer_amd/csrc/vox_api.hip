@@ -521,19 +521,26 @@ vox_status solid_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
         ctx->last_solid = total;
     }
     uint32_t *ovf = ctx->fixed ? ctx->ovf : nullptr;
+    uint32_t *first;
     VOX_TRY(buf(ctx, "s_pairs", total, &pairs));
     VOX_TRY(buf(ctx, "s_list", total, &list));
+    VOX_TRY(buf(ctx, "s_first", block_starts_size(total), &first));
+    if (total > 0 && nslots > 0) {
+        k_block_starts<<<grid_for(nslots, 256), 256, 0, ctx->stream>>>(cand_off, nslots, total,
+                                                                        first);
+        VOX_HIP(hipGetLastError());
+    }
     VOX_TRY(fill(ctx, tile_hist, nTiles, 0u));
     VOX_TRY(fill(ctx, tile_cursor, nTiles, 0u));
     if (total > 0) {
         if (nj > 1)
             k_solid_candidates<true><<<grid_for(total, 256), 256, 0, ctx->stream>>>(
-                D.jobs, tri_job, nslots, ctx->rec, cand_off, cand_off + nslots, cbox, pairs,
-                tile_hist, total, ovf);
+                D.jobs, tri_job, nslots, ctx->rec, cand_off, cand_off + nslots, first, cbox,
+                pairs, tile_hist, total, ovf);
         else
             k_solid_candidates<false><<<grid_for(total, 256), 256, 0, ctx->stream>>>(
-                D.jobs, tri_job, nslots, ctx->rec, cand_off, cand_off + nslots, cbox, pairs,
-                tile_hist, total, ovf);
+                D.jobs, tri_job, nslots, ctx->rec, cand_off, cand_off + nslots, first, cbox,
+                pairs, tile_hist, total, ovf);
         VOX_HIP(hipGetLastError());
     }
     VOX_TRY(scan(ctx, tile_hist, tile_off, nTiles));
@@ -617,19 +624,26 @@ vox_status surface_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
         ctx->last_surf = total;
     }
     uint32_t *ovf = ctx->fixed ? ctx->ovf : nullptr;
+    uint32_t *first;
     VOX_TRY(buf(ctx, "f_blist", total, &blist));
+    VOX_TRY(buf(ctx, "f_first", block_starts_size(total), &first));
+    if (total > 0 && nslots > 0) {
+        k_block_starts<<<grid_for(nslots, 256), 256, 0, ctx->stream>>>(cand_off, nslots, total,
+                                                                        first);
+        VOX_HIP(hipGetLastError());
+    }
     VOX_TRY(fill(ctx, brick_hist, nbt, 0u));
     VOX_TRY(fill(ctx, brick_cursor, nbt, 0u));
     auto bin = [&](int scatter) -> vox_status {
         if (total == 0) return VOX_OK;
         if (nj > 1)
             k_surf_bin<true><<<grid_for(total, 256), 256, 0, ctx->stream>>>(
-                D.jobs, ctx->tri_job, nslots, cand_off, cand_off + nslots, tri_info, brick_hist,
-                brick_off, brick_cursor, blist, scatter, total, ovf);
+                D.jobs, ctx->tri_job, nslots, cand_off, cand_off + nslots, first, tri_info,
+                brick_hist, brick_off, brick_cursor, blist, scatter, total, ovf);
         else
             k_surf_bin<false><<<grid_for(total, 256), 256, 0, ctx->stream>>>(
-                D.jobs, ctx->tri_job, nslots, cand_off, cand_off + nslots, tri_info, brick_hist,
-                brick_off, brick_cursor, blist, scatter, total, ovf);
+                D.jobs, ctx->tri_job, nslots, cand_off, cand_off + nslots, first, tri_info,
+                brick_hist, brick_off, brick_cursor, blist, scatter, total, ovf);
         VOX_HIP(hipGetLastError());
         return VOX_OK;
     };
@@ -1512,6 +1526,8 @@ vox_status graph_capture(vox_graph *g) {
     VOX_TRY(buf(w, "s_pairs", w->cap_solid, &pr));
     VOX_TRY(buf(w, "s_list", w->cap_solid, &u));
     VOX_TRY(buf(w, "f_blist", w->cap_surf, &u));
+    VOX_TRY(buf(w, "s_first", block_starts_size(w->cap_solid), &u));
+    VOX_TRY(buf(w, "f_first", block_starts_size(w->cap_surf), &u));
     VOX_TRY(buf(w, "e_changes", (size_t)w->n_passes * nj, &u));
     VOX_TRY(buf(w, "e_totals", w->n_passes, &u));
     VOX_TRY(buf(w, "e_notconv", 1, &u));

@@ -1,9 +1,9 @@
 // vox_kernels.h -- HIP kernels of the voxelizer hot path (gfx950, wave64).
 //
-// Solid stage:   k_solid_setup -> scan -> k_solid_candidates -> scan ->
+// Solid stage:   k_solid_setup -> scan -> k_block_starts -> k_solid_candidates -> scan ->
 //                k_solid_scatter -> k_solid_fill (LDS-staged crossings +
 //                per-row prefix XOR, one coalesced write per bit word).
-// Surface stage: k_surf_setup -> scan -> k_surf_bin (count) -> scan ->
+// Surface stage: k_surf_setup -> scan -> k_block_starts -> k_surf_bin (count) -> scan ->
 //                k_surf_bin (scatter) -> k_surf_brick (LDS atomicMax winners).
 // Nodes:         k_erode (bitwise 6-neighbour erosion passes) ->
 //                k_nodes_cart / k_nodes_fcc (bid/mask + material, node write).
@@ -301,12 +301,18 @@ __global__ void k_tri_records(const Job *__restrict__ jobs, const uint32_t *__re
 __global__ void k_solid_setup(const Job *__restrict__ jobs, const uint32_t *__restrict__ tri_bases,
                               uint32_t njobs, uint32_t nslots, const float4 *__restrict__ rec,
                               uint32_t *__restrict__ cand_count, uint4 *__restrict__ cbox);
+// first[b] = the triangle holding candidate 256*b of a candidate pass (cand_off:
+// exclusive offsets, nslots + 1 entries); candidates at or past cap are skipped.
+__global__ void k_block_starts(const uint32_t *__restrict__ cand_off, uint32_t nslots,
+                               uint32_t cap, uint32_t *__restrict__ first);
+inline size_t block_starts_size(uint32_t total) { return (size_t)(total + 255u) / 256u + 1u; }
 template <bool kMulti>
 __global__ void k_solid_candidates(const Job *__restrict__ jobs,
                                    const uint32_t *__restrict__ tri_job, uint32_t nslots,
                                    const float4 *__restrict__ rec,
                                    const uint32_t *__restrict__ cand_off,
                                    const uint32_t *__restrict__ cand_total,
+                                   const uint32_t *__restrict__ first,
                                    const uint4 *__restrict__ cbox, uint2 *__restrict__ pairs,
                                    uint32_t *__restrict__ tile_hist, uint32_t cap,
                                    uint32_t *__restrict__ ovf);
@@ -340,6 +346,7 @@ template <bool kMulti>
 __global__ void k_surf_bin(const Job *__restrict__ jobs, const uint32_t *__restrict__ tri_job,
                            uint32_t nslots, const uint32_t *__restrict__ cand_off,
                            const uint32_t *__restrict__ cand_total,
+                           const uint32_t *__restrict__ first,
                            const uint4 *__restrict__ tri_info,
                            uint32_t *__restrict__ brick_hist, const uint32_t *__restrict__ brick_off,
                            uint32_t *__restrict__ brick_cursor, uint32_t *__restrict__ blist,

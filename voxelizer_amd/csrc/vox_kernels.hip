@@ -173,13 +173,28 @@ __global__ __launch_bounds__(256) void k_solid_setup(const Job *__restrict__ job
 // triangle.  The triangle of a candidate is found by a binary search narrowed
 // to the triangles the block's candidates span.  Queued pairs are counted per
 // tile (LDS-grouped atomics) for the counting sort.
+// Block starts of a candidate pass: first[b] = the triangle that holds
+// candidate 256*b (the passes' blocks take 256 consecutive candidates), so a
+// block bounds its triangles with two loads instead of a binary search over
+// the whole offset array (about 21 dependent loads on the large hall).
+// Candidates at or past `cap` are not processed.
+__global__ __launch_bounds__(256) void k_block_starts(const uint32_t *__restrict__ cand_off,
+                                                      uint32_t nslots, uint32_t cap,
+                                                      uint32_t *__restrict__ first) {
+    const uint32_t total = min(cand_off[nslots], cap);
+    for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < nslots; t += gridDim.x * 256u) {
+        const uint32_t lo = min(cand_off[t], total), hi = min(cand_off[t + 1u], total);
+        for (uint32_t b = (lo + 255u) / 256u; b * 256u < hi; b++) first[b] = t;
+    }
+}
+
 template <bool kMulti>
 __global__ __launch_bounds__(256) void k_solid_candidates(
     const Job *__restrict__ jobs, const uint32_t *__restrict__ tri_job, uint32_t nslots,
     const float4 *__restrict__ rec, const uint32_t *__restrict__ cand_off,
-    const uint32_t *__restrict__ cand_total, const uint4 *__restrict__ cbox,
-    uint2 *__restrict__ pairs, uint32_t *__restrict__ tile_hist, uint32_t cap,
-    uint32_t *__restrict__ ovf) {
+    const uint32_t *__restrict__ cand_total, const uint32_t *__restrict__ first,
+    const uint4 *__restrict__ cbox, uint2 *__restrict__ pairs, uint32_t *__restrict__ tile_hist,
+    uint32_t cap, uint32_t *__restrict__ ovf) {
     __shared__ uint32_t trange[2];
     __shared__ AggTable agg;
     const uint32_t full = *cand_total, total = min(full, cap);
@@ -188,9 +203,9 @@ __global__ __launch_bounds__(256) void k_solid_candidates(
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
         __syncthreads();
         agg_clear(agg);
-        if (threadIdx.x < 2) {
-            const uint32_t key = threadIdx.x == 0 ? base : min(base + blockDim.x, total) - 1u;
-            trange[threadIdx.x] = upper_bound_u32(cand_off, nslots, key) - 1u;
+        if (threadIdx.x < 2) {  // the triangles of candidates base and base + 256 bound the block's
+            const uint32_t b = base / 256u + threadIdx.x;
+            trange[threadIdx.x] = threadIdx.x == 0 || base + 256u < total ? first[b] : nslots - 1u;
         }
         __syncthreads();
         const uint32_t c = base + threadIdx.x;
@@ -235,12 +250,14 @@ __global__ __launch_bounds__(256) void k_solid_candidates(
 }
 template __global__ void k_solid_candidates<false>(const Job *, const uint32_t *, uint32_t,
                                                    const float4 *, const uint32_t *,
-                                                   const uint32_t *, const uint4 *, uint2 *,
-                                                   uint32_t *, uint32_t, uint32_t *);
+                                                   const uint32_t *, const uint32_t *,
+                                                   const uint4 *, uint2 *, uint32_t *, uint32_t,
+                                                   uint32_t *);
 template __global__ void k_solid_candidates<true>(const Job *, const uint32_t *, uint32_t,
                                                   const float4 *, const uint32_t *,
-                                                  const uint32_t *, const uint4 *, uint2 *,
-                                                  uint32_t *, uint32_t, uint32_t *);
+                                                  const uint32_t *, const uint32_t *,
+                                                  const uint4 *, uint2 *, uint32_t *, uint32_t,
+                                                  uint32_t *);
 
 __global__ __launch_bounds__(256) void k_solid_scatter(const uint2 *__restrict__ pairs,
                                                        const uint32_t *__restrict__ cand_total,
@@ -452,7 +469,8 @@ template <bool kMulti>
 __global__ __launch_bounds__(256) void k_surf_bin(
     const Job *__restrict__ jobs, const uint32_t *__restrict__ tri_job, uint32_t nslots,
     const uint32_t *__restrict__ cand_off, const uint32_t *__restrict__ cand_total,
-    const uint4 *__restrict__ tri_info, uint32_t *__restrict__ brick_hist,
+    const uint32_t *__restrict__ first, const uint4 *__restrict__ tri_info,
+    uint32_t *__restrict__ brick_hist,
     const uint32_t *__restrict__ brick_off, uint32_t *__restrict__ brick_cursor,
     uint32_t *__restrict__ blist, int scatter, uint32_t cap, uint32_t *__restrict__ ovf) {
     __shared__ uint32_t trange[2];
@@ -462,9 +480,9 @@ __global__ __launch_bounds__(256) void k_surf_bin(
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
         __syncthreads();
         agg_clear(agg);
-        if (threadIdx.x < 2) {
-            const uint32_t key = threadIdx.x == 0 ? base : min(base + blockDim.x, total) - 1u;
-            trange[threadIdx.x] = upper_bound_u32(cand_off, nslots, key) - 1u;
+        if (threadIdx.x < 2) {  // the triangles of candidates base and base + 256 bound the block's
+            const uint32_t b = base / 256u + threadIdx.x;
+            trange[threadIdx.x] = threadIdx.x == 0 || base + 256u < total ? first[b] : nslots - 1u;
         }
         __syncthreads();
         const uint32_t c = base + threadIdx.x;
@@ -503,13 +521,13 @@ __global__ __launch_bounds__(256) void k_surf_bin(
     }
 }
 template __global__ void k_surf_bin<false>(const Job *, const uint32_t *, uint32_t,
-                                           const uint32_t *, const uint32_t *, const uint4 *,
-                                           uint32_t *, const uint32_t *, uint32_t *, uint32_t *,
-                                           int, uint32_t, uint32_t *);
+                                           const uint32_t *, const uint32_t *, const uint32_t *,
+                                           const uint4 *, uint32_t *, const uint32_t *,
+                                           uint32_t *, uint32_t *, int, uint32_t, uint32_t *);
 template __global__ void k_surf_bin<true>(const Job *, const uint32_t *, uint32_t,
-                                          const uint32_t *, const uint32_t *, const uint4 *,
-                                          uint32_t *, const uint32_t *, uint32_t *, uint32_t *,
-                                          int, uint32_t, uint32_t *);
+                                          const uint32_t *, const uint32_t *, const uint32_t *,
+                                          const uint4 *, uint32_t *, const uint32_t *,
+                                          uint32_t *, uint32_t *, int, uint32_t, uint32_t *);
 
 // Winner key of a (triangle, voxel) hit; the largest key of a voxel wins
 // (bbType first, then the mesh-local triangle index, as the oracle's rule).
