@@ -1012,16 +1012,16 @@ __global__ __launch_bounds__(256) void k_nonzero_list(const uint32_t *__restrict
 // [y0,y1) x [z0,z1) are left alone.  Plane-linear launch: a block takes
 // kChunk consecutive words of a plane over kErodeZ planes.
 // ---------------------------------------------------------------------------
-// One logical block GL (of job j) of an erosion pass.
-__device__ __forceinline__ void erode_block(const Job &J, uint32_t j, uint32_t GL, uint32_t njobs,
-                                            uint32_t y0, uint32_t y1, uint32_t z0, uint32_t z1,
-                                            uint32_t *__restrict__ stamp,
-                                            uint32_t *__restrict__ changes,
-                                            uint32_t *__restrict__ totals,
-                                            uint32_t *__restrict__ notconv, uint32_t pass,
-                                            uint32_t gpass, int full) {
+// Whether logical block GL of job J lies in the subspace and, after the
+// subspace's first pass, is near a change of the previous pass: a word
+// depends on words at most W positions away in its plane and on the same
+// word in the planes next to it, so the block is visited when any chunk
+// within W words, or this chunk in the plane groups next to it, changed.
+__device__ __forceinline__ bool erode_visit(const Job &J, uint32_t GL, uint32_t y0, uint32_t y1,
+                                            uint32_t z0, uint32_t z1,
+                                            const uint32_t *__restrict__ stamp, uint32_t gpass,
+                                            int full) {
     const GridDims g = J.gd;
-    VK_G uint32_t *__restrict__ bits = gptr(J.bits);
     const uint32_t P = g.W * g.Y;  // words per plane
     const uint32_t nch = (P + kChunk - 1u) / kChunk;
     const uint32_t nzc = (g.Z + kErodeZ - 1) / kErodeZ;
@@ -1031,20 +1031,34 @@ __device__ __forceinline__ void erode_block(const Job &J, uint32_t j, uint32_t G
     const uint32_t zb = zg * kErodeZ;
     const uint32_t i0 = c * kChunk;
     const uint32_t ylo = i0 / g.W, yhi = min(i0 + kChunk - 1u, P - 1u) / g.W;
-    if (zb >= z1 || zb + kErodeZ <= z0 || yhi < y0 || ylo >= y1) return;
-    if (!full) {
-        // A word depends on words at most W positions away in its plane and
-        // on the same word in the planes next to it: visit when any chunk
-        // within W words, or this chunk in the plane groups next to it,
-        // changed in the previous pass.
-        const uint32_t r = g.W / kChunk + 1u;
-        const uint32_t clo = c >= r ? c - r : 0u, chi = min(c + r, nch - 1u);
-        bool visit = false;
-        for (uint32_t cc = clo; cc <= chi; cc++) visit |= stamp[zg * nch + cc] >= gpass;
-        if (zg > 0) visit |= stamp[L - nch] >= gpass;
-        if (zg + 1 < nzc) visit |= stamp[L + nch] >= gpass;
-        if (!visit) return;
-    }
+    if (zb >= z1 || zb + kErodeZ <= z0 || yhi < y0 || ylo >= y1) return false;
+    if (full) return true;
+    const uint32_t r = g.W / kChunk + 1u;
+    const uint32_t clo = c >= r ? c - r : 0u, chi = min(c + r, nch - 1u);
+    bool visit = false;
+    for (uint32_t cc = clo; cc <= chi; cc++) visit |= stamp[zg * nch + cc] >= gpass;
+    if (zg > 0) visit |= stamp[L - nch] >= gpass;
+    if (zg + 1 < nzc) visit |= stamp[L + nch] >= gpass;
+    return visit;
+}
+
+// One logical block GL (of job j) of an erosion pass.
+__device__ __forceinline__ void erode_block(const Job &J, uint32_t j, uint32_t GL, uint32_t njobs,
+                                            uint32_t y0, uint32_t y1, uint32_t z0, uint32_t z1,
+                                            uint32_t *__restrict__ stamp,
+                                            uint32_t *__restrict__ changes,
+                                            uint32_t *__restrict__ totals,
+                                            uint32_t *__restrict__ notconv, uint32_t pass,
+                                            uint32_t gpass) {
+    const GridDims g = J.gd;
+    VK_G uint32_t *__restrict__ bits = gptr(J.bits);
+    const uint32_t P = g.W * g.Y;  // words per plane
+    const uint32_t nch = (P + kChunk - 1u) / kChunk;
+    const uint32_t L = GL - J.erode_base;  // block within the job
+    stamp += J.erode_base;
+    const uint32_t c = L % nch, zg = L / nch;
+    const uint32_t zb = zg * kErodeZ;
+    const uint32_t i0 = c * kChunk;
     const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1u);
     const uint32_t i = i0 + tid;
     const bool valid = i < P;
@@ -1134,27 +1148,39 @@ __global__ __launch_bounds__(256) void k_erode(const Job *__restrict__ jobs,
     if (pass == 0) {
         for (uint32_t GL = start; GL < hi; GL += nx) {
             const uint32_t j = job_of(erode_bases, njobs, GL);
-            erode_block(jobs[j], j, GL, njobs, y0, y1, z0, z1, stamp, changes, totals, notconv,
-                        pass, gpass, full);
+            if (erode_visit(jobs[j], GL, y0, y1, z0, z1, stamp, gpass, full))
+                erode_block(jobs[j], j, GL, njobs, y0, y1, z0, z1, stamp, changes, totals,
+                            notconv, pass, gpass);
         }
         return;
     }
-    // Later passes: only jobs the previous pass changed, found 64 at a time
-    // by a ballot over their flags.
-    const uint32_t lane = threadIdx.x & (kWave - 1u);
+    // Later passes: the workgroup's blocks are checked 256 at a time, one
+    // per thread (its job changed in the previous pass, and the stamps near
+    // it), and only the blocks that pass are visited, from a list in LDS.
+    // (Checked one after another, each block would cost the workgroup a
+    // round trip to memory, most of them to skip it.)
+    __shared__ uint32_t s_list[256];
+    __shared__ uint32_t s_n;
+    const uint32_t tid = threadIdx.x;
     const uint32_t *prev = changes + (pass - 1u) * njobs;
-    for (uint32_t g0 = 0; g0 < njobs; g0 += kWave) {
-        const bool live = g0 + lane < njobs && prev[g0 + lane] != 0u;
-        unsigned long long m = __ballot(live);
-        while (m) {
-            const uint32_t j = g0 + (uint32_t)(__ffsll((long long)m) - 1);
-            m &= m - 1ull;
-            const uint32_t a = max(erode_bases[j], start), e = min(erode_bases[j + 1u], hi);
-            uint32_t GL = a == start ? start : start + ((a - start + nx - 1u) / nx) * nx;
-            for (; GL < e; GL += nx)
-                erode_block(jobs[j], j, GL, njobs, y0, y1, z0, z1, stamp, changes, totals,
-                            notconv, pass, gpass, full);
+    for (uint32_t k0 = 0; start + k0 * nx < hi; k0 += blockDim.x) {  // uniform
+        if (tid == 0) s_n = 0u;
+        __syncthreads();
+        const uint32_t GLt = start + (k0 + tid) * nx;
+        if (GLt < hi) {
+            const uint32_t jt = job_of(erode_bases, njobs, GLt);
+            if (prev[jt] != 0u && erode_visit(jobs[jt], GLt, y0, y1, z0, z1, stamp, gpass, full))
+                s_list[atomicAdd(&s_n, 1u)] = GLt;
         }
+        __syncthreads();
+        const uint32_t n = s_n;
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t GL = s_list[i];
+            const uint32_t j = job_of(erode_bases, njobs, GL);
+            erode_block(jobs[j], j, GL, njobs, y0, y1, z0, z1, stamp, changes, totals, notconv,
+                        pass, gpass);
+        }
+        __syncthreads();
     }
 }
 
