@@ -200,6 +200,15 @@ vox_status scan(vox_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t n, int
     std::snprintf(name, sizeof(name), "scan_part%d", depth);
     uint32_t *part;
     VOX_TRY(buf(ctx, name, nblocks + 1, &part));
+    if (nblocks <= kScanDirectTiles) {
+        if (nblocks > 1) {
+            k_scan_reduce<<<nblocks, kScanThreads, 0, ctx->stream>>>(in, n, part);
+            VOX_HIP(hipGetLastError());
+        }
+        k_scan_down<<<nblocks, kScanThreads, 0, ctx->stream>>>(in, out, n, part);
+        VOX_HIP(hipGetLastError());
+        return VOX_OK;
+    }
     k_scan_blocks<<<nblocks, kScanThreads, 0, ctx->stream>>>(in, out, n, part);
     VOX_HIP(hipGetLastError());
     if (nblocks > 1) {

@@ -243,11 +243,20 @@ __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *__restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Scan (exclusive, uint32) -- three passes, 256 threads x 16 items per block.
+// Scan (exclusive, uint32), 256 threads x 16 items per tile.  Up to
+// kScanDirectTiles tiles: k_scan_reduce (tile sums) then k_scan_down (each
+// tile adds the sums before it and scans itself); more: tiles scanned with
+// k_scan_blocks, their sums scanned recursively, then k_scan_add.
 // ---------------------------------------------------------------------------
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr uint32_t kScanDirectTiles = 4096;
+
+__global__ void k_scan_reduce(const uint32_t *__restrict__ in, uint32_t n,
+                              uint32_t *__restrict__ partials);
+__global__ void k_scan_down(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                            uint32_t n, const uint32_t *__restrict__ partials);
 
 __global__ void k_scan_blocks(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
                               uint32_t n, uint32_t *__restrict__ partials);

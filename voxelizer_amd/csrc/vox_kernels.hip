@@ -46,6 +46,78 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(const uint32_t *__
     if (tid == kScanThreads - 1) partials[blockIdx.x] = sh[tid];
 }
 
+// Block-wide sum and exclusive prefix of one value per thread: a wave scan
+// by shuffles, the four wave totals through LDS (one barrier).
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t s, uint32_t &total) {
+    __shared__ uint32_t wsum[kScanThreads / kWave];
+    const uint32_t lane = threadIdx.x & (kWave - 1u), wave = threadIdx.x / kWave;
+    uint32_t incl = s;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_up((int)incl, d);
+        if ((int)lane >= d) incl += o;
+    }
+    if (lane == kWave - 1u) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0u;
+    total = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < kScanThreads / kWave; w++) {
+        before += w < wave ? wsum[w] : 0u;
+        total += wsum[w];
+    }
+    return before + incl - s;
+}
+
+// Tile sums of in[0, n) (direct path).
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__restrict__ in,
+                                                              uint32_t n,
+                                                              uint32_t *__restrict__ partials) {
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t s = 0u;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const uint64_t idx = base + k;
+        s += idx < n ? in[idx] : 0u;
+    }
+    uint32_t total;
+    block_excl_scan(s, total);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// Exclusive scan of in[0, n) into out[0, n] (out[n] = total), direct path:
+// the tile first sums the tile sums before it (at most kScanDirectTiles).
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *__restrict__ in,
+                                                            uint32_t *__restrict__ out,
+                                                            uint32_t n,
+                                                            const uint32_t *__restrict__ partials) {
+    __shared__ uint32_t s_off;
+    uint32_t p = 0u;
+    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kScanThreads) p += partials[i];
+    uint32_t ptotal;
+    block_excl_scan(p, ptotal);
+    if (threadIdx.x == 0) s_off = ptotal;
+    __syncthreads();  // also orders the two uses of block_excl_scan's LDS
+    const uint32_t off = s_off;
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0u;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const uint64_t idx = base + k;
+        v[k] = idx < n ? in[idx] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t excl = off + block_excl_scan(s, total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const uint64_t idx = base + k;
+        if (idx <= n) out[idx] = excl;
+        excl += v[k];
+    }
+}
+
 __global__ __launch_bounds__(kScanThreads) void k_scan_add(uint32_t *__restrict__ out,
                                                            uint32_t n,
                                                            const uint32_t *__restrict__ partials) {
