@@ -99,6 +99,10 @@ struct vox_ctx {
     void *up = nullptr;
     size_t up_cap = 0;
     hipEvent_t up_ev = nullptr;
+    // The surface stage's preparation (setup, binning, brick order) runs on
+    // `side` beside the solid stage; fork and join through these events.
+    hipStream_t side = nullptr, main_stream = nullptr;  // main_stream: `stream` as created
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // surface brick shapes (voxels along x, y, z)
     int cart_bdim[3] = {kCartBX, kCartBY, kCartBZ};
     // meshes with fewer triangles carry materials in the surface pass's
@@ -158,8 +162,10 @@ vox_status buf(vox_ctx *ctx, const char *name, size_t count, T **out) {
     Buf &b = ctx->bufs[name];
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
     if (bytes > b.cap) {
-        if (b.p) {
+        if (b.p) {  // either stream may still use the old buffer
             VOX_HIP(hipStreamSynchronize(ctx->stream));
+            if (ctx->side && ctx->side != ctx->stream) VOX_HIP(hipStreamSynchronize(ctx->side));
+            if (ctx->side && ctx->stream == ctx->side) VOX_HIP(hipStreamSynchronize(ctx->main_stream));
             VOX_HIP(hipFree(b.p));
             b.p = nullptr;
             b.cap = 0;
@@ -197,7 +203,8 @@ vox_status fill(vox_ctx *ctx, uint32_t *p, size_t n, uint32_t v) {
 vox_status scan(vox_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t n, int depth = 0) {
     const uint32_t nblocks = n / kScanTile + 1;
     char name[32];
-    std::snprintf(name, sizeof(name), "scan_part%d", depth);
+    const char *tag = ctx->stream == ctx->side ? "b" : "";  // one set per stream
+    std::snprintf(name, sizeof(name), "scan%s_part%d", tag, depth);
     uint32_t *part;
     VOX_TRY(buf(ctx, name, nblocks + 1, &part));
     if (nblocks <= kScanDirectTiles) {
@@ -212,7 +219,7 @@ vox_status scan(vox_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t n, int
     k_scan_blocks<<<nblocks, kScanThreads, 0, ctx->stream>>>(in, out, n, part);
     VOX_HIP(hipGetLastError());
     if (nblocks > 1) {
-        std::snprintf(name, sizeof(name), "scan_psum%d", depth);
+        std::snprintf(name, sizeof(name), "scan%s_psum%d", tag, depth);
         uint32_t *psum;
         VOX_TRY(buf(ctx, name, nblocks + 1, &psum));
         VOX_TRY(scan(ctx, part, psum, nblocks, depth + 1));
@@ -497,25 +504,37 @@ void record(vox_ctx *ctx, int i) {
 // sort by tile, fill.  The solid result does not depend on voxSplitRes y/z
 // (the subspaces partition the centres), so every grid is filled whole.
 // ---------------------------------------------------------------------------
+// Triangle records (vertices, class word, material) and slot jobs, read by
+// both the solid and the surface stage; with fixed sizes the overflow flag
+// is cleared here, before either stage can raise it.
+vox_status records_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
+    const uint32_t nj = (uint32_t)P.jobs.size();
+    const uint32_t nslots = P.tri_b.back();
+    VOX_TRY(buf(ctx, "tri_rec", (size_t)std::max<uint32_t>(nslots, 1) * 3, &ctx->rec));
+    VOX_TRY(buf(ctx, "tri_job", nslots, &ctx->tri_job));
+    if (ctx->fixed) VOX_TRY(fill(ctx, ctx->ovf, 1, 0u));
+    if (nslots > 0) {
+        k_tri_records<<<grid_for(nslots, 256), 256, 0, ctx->stream>>>(D.jobs, D.tri_b, nj, nslots,
+                                                                       ctx->rec, ctx->tri_job);
+        VOX_HIP(hipGetLastError());
+    }
+    return VOX_OK;
+}
+
 vox_status solid_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
     const uint32_t nj = (uint32_t)P.jobs.size();
     const uint32_t nslots = P.tri_b.back(), nTiles = P.tile_b.back();
-    uint32_t *tri_job, *cand_count, *cand_off, *tile_hist, *tile_off, *tile_cursor, *list;
+    uint32_t *tri_job = ctx->tri_job, *cand_count, *cand_off, *tile_hist, *tile_off, *tile_cursor,
+             *list;
     uint2 *pairs;
     uint4 *cbox;
-    VOX_TRY(buf(ctx, "tri_rec", (size_t)std::max<uint32_t>(nslots, 1) * 3, &ctx->rec));
-    VOX_TRY(buf(ctx, "tri_job", nslots, &tri_job));
     VOX_TRY(buf(ctx, "s_cand_count", nslots, &cand_count));
     VOX_TRY(buf(ctx, "s_cand_off", nslots + 1, &cand_off));
     VOX_TRY(buf(ctx, "s_cbox", nslots, &cbox));
     VOX_TRY(buf(ctx, "s_tile_hist", nTiles, &tile_hist));
     VOX_TRY(buf(ctx, "s_tile_off", nTiles + 1, &tile_off));
     VOX_TRY(buf(ctx, "s_tile_cursor", nTiles, &tile_cursor));
-    ctx->tri_job = tri_job;
     if (nslots > 0) {
-        k_tri_records<<<grid_for(nslots, 256), 256, 0, ctx->stream>>>(D.jobs, D.tri_b, nj, nslots,
-                                                                       ctx->rec, tri_job);
-        VOX_HIP(hipGetLastError());
         k_solid_setup<<<grid_for(nslots, 256), 256, 0, ctx->stream>>>(D.jobs, D.tri_b, nj, nslots,
                                                                        ctx->rec, cand_count, cbox);
         VOX_HIP(hipGetLastError());
@@ -524,7 +543,6 @@ vox_status solid_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
     uint32_t total = 0;
     if (ctx->fixed) {
         total = ctx->cap_solid;
-        VOX_TRY(fill(ctx, ctx->ovf, 1, 0u));
     } else {
         VOX_TRY(read_u32(ctx, cand_off + nslots, 1, &total));
         ctx->last_solid = total;
@@ -597,7 +615,23 @@ vox_status record_samples(vox_ctx *ctx, const uint32_t *list, const uint32_t *n_
 // order (plain or learned), the persistent brick pass, which leaves the
 // surface words and the material grid the node passes read.
 // ---------------------------------------------------------------------------
-vox_status surface_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
+// Triangle slots from which the surface preparation runs on its own stream.
+constexpr uint32_t kForkMinSlots = 16384;
+
+// What the brick pass needs from the surface stage's preparation.
+struct SurfPass {
+    uint32_t *brick_off = nullptr, *brick_hist = nullptr, *blist = nullptr, *stats = nullptr;
+    uint32_t *blist_order = nullptr, *next = nullptr;
+    const uint32_t *n_list = nullptr, *hlist = nullptr, *n_heavy = nullptr;
+    const uint8_t *skip = nullptr;
+    float *feats = nullptr;
+};
+
+// Surface stage up to the brick pass: triangle setup, (triangle, brick)
+// pairs binned by brick, the brick list and, for the learned schedule, its
+// heavy bricks.  It only reads the triangle records, so it can run beside
+// the solid stage.
+vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass &S) {
     const uint32_t nj = (uint32_t)P.jobs.size();
     const uint32_t nslots = P.tri_b.back(), nbt = P.brick_b.back();
     uint4 *tri_info, *tri_ext = nullptr;
@@ -715,21 +749,43 @@ vox_status surface_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
         n_heavy = cnt + 1;
         skip = skipf;
     }
-    if (ctx->record) VOX_TRY(buf(ctx, "f_cycles", nbt, &cycles));
     VOX_TRY(fill(ctx, stats, 1, 0u));
     VOX_TRY(fill(ctx, next, 1, 0u));
+    (void)cycles;
+    S.brick_off = brick_off;
+    S.brick_hist = brick_hist;
+    S.blist = blist;
+    S.stats = stats;
+    S.blist_order = blist_order;
+    S.next = next;
+    S.n_list = n_list;
+    S.hlist = hlist;
+    S.n_heavy = n_heavy;
+    S.skip = skip;
+    S.feats = feats;
+    return VOX_OK;
+}
+
+// The brick pass (after the solid stage: it ORs surface bits into the solid
+// grid).
+vox_status surface_brick(vox_ctx *ctx, const Plan &P, const PlanDev &D, const SurfPass &S) {
+    const uint32_t nj = (uint32_t)P.jobs.size(), nbt = P.brick_b.back();
+    uint64_t *cycles = nullptr;
+    if (ctx->record) VOX_TRY(buf(ctx, "f_cycles", nbt, &cycles));
     const uint32_t blocks = std::min<uint32_t>(nbt, (uint32_t)(ctx->n_cu * ctx->surf_blocks_per_cu));
-    k_surf_brick<<<blocks, 256, 0, ctx->stream>>>(D.jobs, D.brick_b, nj, tri_rec, brick_off,
-                                                  brick_hist, blist, ctx->timing ? stats : nullptr,
-                                                  blist_order, n_list, hlist, n_heavy, skip, next,
+    k_surf_brick<<<blocks, 256, 0, ctx->stream>>>(D.jobs, D.brick_b, nj, ctx->rec, S.brick_off,
+                                                  S.brick_hist, S.blist,
+                                                  ctx->timing ? S.stats : nullptr, S.blist_order,
+                                                  S.n_list, S.hlist, S.n_heavy, S.skip, S.next,
                                                   cycles);
     VOX_HIP(hipGetLastError());
     if (ctx->timing) {
         uint32_t vox = 0;
-        VOX_TRY(read_u32(ctx, stats, 1, &vox));
+        VOX_TRY(read_u32(ctx, S.stats, 1, &vox));
         ctx->times.surface_voxels += vox;
     }
-    if (ctx->record) VOX_TRY(record_samples(ctx, blist_order, n_list, feats, cycles, nbt));
+    if (ctx->record)
+        VOX_TRY(record_samples(ctx, S.blist_order, S.n_list, S.feats, cycles, nbt));
     return VOX_OK;
 }
 
@@ -848,9 +904,33 @@ vox_status run_plan(vox_ctx *ctx, Plan &P, uint32_t splitY, uint32_t splitZ) {
 // The stages after the job-table upload (what a graph captures).
 vox_status run_stages(vox_ctx *ctx, Plan &P, const PlanDev &D, uint32_t splitY,
                       uint32_t splitZ) {
+    VOX_TRY(records_stage(ctx, P, D));
+    // Fork: the surface preparation on the side stream, the solid stage on
+    // the main one; they join before the brick pass.  (Inside a graph
+    // capture the side stream joins the capture through the fork event.)
+    SurfPass S;
+    // Small meshes (a few thousand triangles) have too little of either
+    // stage to overlap; the fork's extra dependencies would cost more.
+    const bool fork = P.any_mats && P.tri_b.back() >= kForkMinSlots;
+    if (fork) {
+        VOX_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+        VOX_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+        hipStream_t main = ctx->stream;
+        ctx->stream = ctx->side;
+        const vox_status st = surface_prep(ctx, P, D, S);
+        ctx->stream = main;
+        VOX_TRY(st);
+        VOX_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+    }
     VOX_TRY(solid_stage(ctx, P, D));
     record(ctx, 1);
-    if (P.any_mats) VOX_TRY(surface_stage(ctx, P, D));
+    if (fork) {
+        VOX_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+        VOX_TRY(surface_brick(ctx, P, D, S));
+    } else if (P.any_mats) {
+        VOX_TRY(surface_prep(ctx, P, D, S));
+        VOX_TRY(surface_brick(ctx, P, D, S));
+    }
     record(ctx, 2);
     if (D.pre) {
         const Job &J = P.jobs[0];
@@ -899,12 +979,16 @@ vox_status vox_create(int device, vox_ctx **out) {
     vox_ctx *ctx = new vox_ctx();
     ctx->dev = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
+        (ctx->main_stream = ctx->stream, false) ||
         hipHostMalloc((void **)&ctx->pinned, kPinnedBytes, hipHostMallocDefault) != hipSuccess) {
         delete ctx;
         return VOX_ERR_HIP;
     }
     for (auto &e : ctx->ev) (void)hipEventCreate(&e);
-    if (hipEventCreateWithFlags(&ctx->up_ev, hipEventDisableTiming) != hipSuccess) {
+    if (hipEventCreateWithFlags(&ctx->up_ev, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreate(&ctx->side) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
         vox_destroy(ctx);
         return VOX_ERR_HIP;
     }
@@ -943,6 +1027,7 @@ void vox_destroy(vox_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->dev);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     for (auto &kv : ctx->bufs)
         if (kv.second.p) (void)hipFree(kv.second.p);
     if (ctx->own_mesh) {
@@ -957,7 +1042,10 @@ void vox_destroy(vox_ctx *ctx) {
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->up) (void)hipHostFree(ctx->up);
     if (ctx->up_ev) (void)hipEventDestroy(ctx->up_ev);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->cost_params) (void)hipFree(ctx->cost_params);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1243,6 +1331,7 @@ vox_status vox_solid_bits(vox_ctx *ctx, const vox_grid *g, const double shift[3]
     PlanDev D;
     record(ctx, 0);
     VOX_TRY(plan_upload(ctx, P, D, UINT32_MAX, UINT32_MAX));
+    VOX_TRY(records_stage(ctx, P, D));
     VOX_TRY(solid_stage(ctx, P, D));
     record(ctx, 1);
     if (ctx->timing) {
