@@ -87,7 +87,11 @@ struct vox_ctx {
     float *cost_params = nullptr;  // device, kCostParams floats
     bool cost_ready = false;
     float heavy_cost = 1.5f;    // standardized log cost above which a brick goes first
-    float screen_ratio = 1.5f;  // pair count / mean above which the model is consulted
+    // Pair count / mean from which a brick's cost is predicted.  Measured on
+    // the large hall and the 256-mesh batch: 1.5 -> 543 / 329 Gvoxels/s,
+    // 6 -> 548 / 335, 12 -> 552 / 337, 24 -> 551 / 337 (the features of a
+    // few thousand light bricks cost more than their order gains).
+    float screen_ratio = 12.0f;
     bool record = false;
     std::vector<float> rec_feats, rec_cycles;
     // worker contexts of batched runs (same device, own streams and buffers)
