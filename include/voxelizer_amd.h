@@ -12,7 +12,9 @@
  * src/host_side.cpp); the replaced interface is named in each comment.
  *
  * Device pointers ("_dev") are HIP device addresses on the context's device.
- * All work is enqueued on the context's stream; vox_sync() waits for it.
+ * All work is ordered on the context's stream; vox_sync() waits for it.  (A
+ * call may also run work on a second stream of the context, forked from and
+ * joined back into the context's stream inside the call.)
  * Calls on one context are not re-entrant (as in the reference, where one
  * Voxelizer instance mutates its options per call, src/host_side.cpp:1742).
  */
