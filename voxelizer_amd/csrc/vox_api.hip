@@ -718,17 +718,15 @@ vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass 
     const uint8_t *skip = nullptr;
     float *feats = nullptr;
     uint64_t *cycles = nullptr;
-    if (features) VOX_TRY(buf(ctx, "f_feats", (size_t)nbt * kCostFeatures, &feats));
+    if (ctx->record) VOX_TRY(buf(ctx, "f_feats", (size_t)nbt * kCostFeatures, &feats));
     if (ctx->record) {  // training samples: features of every listed brick
         k_brick_features<<<grid_for(nbt, 4, 65535), 256, 0, ctx->stream>>>(
             blist_order, nullptr, n_list, brick_hist, brick_off, blist, tri_info, tri_ext, D.jobs,
             D.brick_b, nj, feats);
         VOX_HIP(hipGetLastError());
     } else if (learned) {
-        float *pred;
         uint32_t *lcand, *cnt, *heavy;
         uint8_t *skipf;
-        VOX_TRY(buf(ctx, "f_pred", nbt, &pred));
         VOX_TRY(buf(ctx, "f_lcand", nbt, &lcand));
         VOX_TRY(buf(ctx, "f_cnt", 2, &cnt));
         VOX_TRY(buf(ctx, "f_heavy", nbt, &heavy));
@@ -739,15 +737,9 @@ vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass 
                                                      cand_off + nslots, ctx->screen_ratio, lcand,
                                                      cnt, skipf);
         VOX_HIP(hipGetLastError());
-        k_brick_features<<<grid_for(nbt, 4, 65535), 256, 0, ctx->stream>>>(
-            blist_order, lcand, cnt, brick_hist, brick_off, blist, tri_info, tri_ext, D.jobs,
-            D.brick_b, nj, feats);
-        VOX_HIP(hipGetLastError());
-        k_mlp_forward_n<<<(nbt + kMlpBlockRows - 1) / kMlpBlockRows, 256, 0, ctx->stream>>>(
-            ctx->cost_params, feats, cnt, pred);
-        VOX_HIP(hipGetLastError());
-        k_cost_heavy<<<g256, 256, 0, ctx->stream>>>(pred, lcand, cnt, ctx->heavy_cost, blist_order,
-                                                    heavy, skipf);
+        k_cost_predict<<<grid_for(nbt, 4, (uint32_t)ctx->n_cu * 8u), 256, 0, ctx->stream>>>(
+            ctx->cost_params, blist_order, lcand, cnt, ctx->heavy_cost, brick_hist, brick_off,
+            blist, tri_info, tri_ext, D.jobs, D.brick_b, nj, heavy, skipf);
         VOX_HIP(hipGetLastError());
         hlist = heavy;
         n_heavy = cnt + 1;

@@ -37,6 +37,76 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// The 16 features of brick b, computed by one wave (every lane ends with
+// all of them: the sums are butterflies).
+__device__ __forceinline__ void brick_feats(uint32_t b, const uint32_t *__restrict__ brick_hist,
+                                            const uint32_t *__restrict__ brick_off,
+                                            const uint32_t *__restrict__ blist,
+                                            const uint4 *__restrict__ tri_info,
+                                            const uint4 *__restrict__ tri_ext,
+                                            const Job *__restrict__ jobs,
+                                            const uint32_t *__restrict__ brick_bases,
+                                            uint32_t njobs, float f[kCostFeatures]) {
+    const uint32_t lane = lane_id();
+    const Job &J = jobs[job_of(brick_bases, njobs, b)];
+    const int32_t bd0 = J.bdim[0], bd1 = J.bdim[1], bd2 = J.bdim[2];
+    const float bvox = (float)(bd0 * bd1 * bd2);
+    const uint32_t cnt = brick_hist[b], off = brick_off[b];
+    float c1 = 0, c2 = 0, c3 = 0, i1 = 0, i2 = 0, i3 = 0, v3 = 0, mx = 0, span = 0, small = 0,
+          axx = 0, axy = 0;
+    for (uint32_t i = lane; i < cnt; i += kWave) {
+        const uint32_t t = blist[off + i];
+        const uint4 ti = tri_info[t];
+        const uint4 te = tri_ext[t];
+        const int bb = (int)(ti.x >> 30), a = (int)((ti.x >> 28) & 3u);
+        const uint32_t wbx = ti.z >> 16, wby = ti.w & 0xFFFFu, wbz = ti.w >> 16;
+        const uint32_t wa = a == AX_X ? wbx : a == AX_Y ? wby : wbz;
+        const uint32_t wuv = (wbx * wby * wbz) / max(wa, 1u);
+        const int32_t bda = a == AX_X ? bd0 : a == AX_Y ? bd1 : bd2;
+        float est;
+        if (bb == BB_1D) {
+            est = (float)te.w / (float)max(wa, 1u);
+            c1 += 1.0f;
+            i1 += est;
+        } else {
+            est = (float)te.w / (float)max(wuv, 1u);
+            if (bb == BB_2D) {
+                c2 += 1.0f;
+                i2 += est;
+            } else {
+                c3 += 1.0f;
+                i3 += est;
+                v3 += est * (float)min((int32_t)te.z, bda);
+                axx += a == AX_X ? 1.0f : 0.0f;
+                axy += a == AX_Y ? 1.0f : 0.0f;
+            }
+        }
+        mx = fmaxf(mx, est);
+        span += (float)(wbx * wby * wbz);
+        small += est <= 4.0f ? 1.0f : 0.0f;
+    }
+    c1 = wave_sum(c1); c2 = wave_sum(c2); c3 = wave_sum(c3);
+    i1 = wave_sum(i1); i2 = wave_sum(i2); i3 = wave_sum(i3);
+    v3 = wave_sum(v3); mx = wave_max(mx); span = wave_sum(span);
+    small = wave_sum(small); axx = wave_sum(axx); axy = wave_sum(axy);
+    f[0] = log1pf((float)cnt);
+    f[1] = log1pf(c1);
+    f[2] = log1pf(c2);
+    f[3] = log1pf(c3);
+    f[4] = log1pf(i1);
+    f[5] = log1pf(i2);
+    f[6] = log1pf(i3);
+    f[7] = log1pf(v3);
+    f[8] = log1pf(mx);
+    f[9] = log1pf((float)((cnt + 63u) / 64u));
+    f[10] = c3 > 0.0f ? axx / c3 : 0.0f;
+    f[11] = c3 > 0.0f ? axy / c3 : 0.0f;
+    f[12] = log1pf(span);
+    f[13] = (i1 + i2 + i3) / bvox;
+    f[14] = log1pf(small);
+    f[15] = 1.0f;
+}
+
 // Row r describes brick order[sel ? sel[r] : r], for r < *n_rows.
 __global__ __launch_bounds__(256) void k_brick_features(
     const uint32_t *__restrict__ order, const uint32_t *__restrict__ sel,
@@ -46,69 +116,15 @@ __global__ __launch_bounds__(256) void k_brick_features(
     const Job *__restrict__ jobs, const uint32_t *__restrict__ brick_bases, uint32_t njobs,
     float *__restrict__ feats) {
     const uint32_t n = *n_rows;
-    const uint32_t lane = lane_id();
     const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
     for (uint32_t q = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < n; q += nwaves) {
-        const uint32_t b = order[sel ? sel[q] : q];
-        const Job &J = jobs[job_of(brick_bases, njobs, b)];
-        const int32_t bd0 = J.bdim[0], bd1 = J.bdim[1], bd2 = J.bdim[2];
-        const float bvox = (float)(bd0 * bd1 * bd2);
-        const uint32_t cnt = brick_hist[b], off = brick_off[b];
-        float c1 = 0, c2 = 0, c3 = 0, i1 = 0, i2 = 0, i3 = 0, v3 = 0, mx = 0, span = 0,
-              small = 0, axx = 0, axy = 0;
-        for (uint32_t i = lane; i < cnt; i += kWave) {
-            const uint32_t t = blist[off + i];
-            const uint4 ti = tri_info[t];
-            const uint4 te = tri_ext[t];
-            const int bb = (int)(ti.x >> 30), a = (int)((ti.x >> 28) & 3u);
-            const uint32_t wbx = ti.z >> 16, wby = ti.w & 0xFFFFu, wbz = ti.w >> 16;
-            const uint32_t wa = a == AX_X ? wbx : a == AX_Y ? wby : wbz;
-            const uint32_t wuv = (wbx * wby * wbz) / max(wa, 1u);
-            const int32_t bda = a == AX_X ? bd0 : a == AX_Y ? bd1 : bd2;
-            float est;
-            if (bb == BB_1D) {
-                est = (float)te.w / (float)max(wa, 1u);
-                c1 += 1.0f;
-                i1 += est;
-            } else {
-                est = (float)te.w / (float)max(wuv, 1u);
-                if (bb == BB_2D) {
-                    c2 += 1.0f;
-                    i2 += est;
-                } else {
-                    c3 += 1.0f;
-                    i3 += est;
-                    v3 += est * (float)min((int32_t)te.z, bda);
-                    axx += a == AX_X ? 1.0f : 0.0f;
-                    axy += a == AX_Y ? 1.0f : 0.0f;
-                }
-            }
-            mx = fmaxf(mx, est);
-            span += (float)(wbx * wby * wbz);
-            small += est <= 4.0f ? 1.0f : 0.0f;
-        }
-        c1 = wave_sum(c1); c2 = wave_sum(c2); c3 = wave_sum(c3);
-        i1 = wave_sum(i1); i2 = wave_sum(i2); i3 = wave_sum(i3);
-        v3 = wave_sum(v3); mx = wave_max(mx); span = wave_sum(span);
-        small = wave_sum(small); axx = wave_sum(axx); axy = wave_sum(axy);
-        if (lane == 0) {
-            float *f = feats + (size_t)q * kCostFeatures;
-            f[0] = log1pf((float)cnt);
-            f[1] = log1pf(c1);
-            f[2] = log1pf(c2);
-            f[3] = log1pf(c3);
-            f[4] = log1pf(i1);
-            f[5] = log1pf(i2);
-            f[6] = log1pf(i3);
-            f[7] = log1pf(v3);
-            f[8] = log1pf(mx);
-            f[9] = log1pf((float)((cnt + 63u) / 64u));
-            f[10] = c3 > 0.0f ? axx / c3 : 0.0f;
-            f[11] = c3 > 0.0f ? axy / c3 : 0.0f;
-            f[12] = log1pf(span);
-            f[13] = (i1 + i2 + i3) / bvox;
-            f[14] = log1pf(small);
-            f[15] = 1.0f;
+        float f[kCostFeatures];
+        brick_feats(order[sel ? sel[q] : q], brick_hist, brick_off, blist, tri_info, tri_ext,
+                    jobs, brick_bases, njobs, f);
+        if (lane_id() == 0) {
+            float *o = feats + (size_t)q * kCostFeatures;
+#pragma unroll
+            for (int i = 0; i < kCostFeatures; i++) o[i] = f[i];
         }
     }
 }
@@ -375,6 +391,45 @@ __global__ __launch_bounds__(256) void k_cost_heavy(const float *__restrict__ Y,
         if (h) {
             const uint32_t q = cand[r];
             hlist[slot] = order[q];
+            skip[q] = 1u;
+        }
+    }
+}
+
+// The learned path in one launch: for each candidate (one wave per brick)
+// its features, the MLP forward pass with lane j as hidden neuron j (weights
+// read from global memory, where they stay cached; the sums run in the order
+// of k_mlp_forward, so the prediction is the same), and the heavy test that
+// puts the brick on the heavy list and flags its plain entry.
+__global__ __launch_bounds__(256) void k_cost_predict(
+    const float *__restrict__ P, const uint32_t *__restrict__ order,
+    const uint32_t *__restrict__ cand, uint32_t *__restrict__ cnt, float heavy,
+    const uint32_t *__restrict__ brick_hist, const uint32_t *__restrict__ brick_off,
+    const uint32_t *__restrict__ blist, const uint4 *__restrict__ tri_info,
+    const uint4 *__restrict__ tri_ext, const Job *__restrict__ jobs,
+    const uint32_t *__restrict__ brick_bases, uint32_t njobs, uint32_t *__restrict__ hlist,
+    uint8_t *__restrict__ skip) {
+    const uint32_t n = cnt[0];
+    const uint32_t j = lane_id();
+    const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
+    for (uint32_t r = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; r < n; r += nwaves) {
+        const uint32_t q = cand[r], b = order[q];
+        float f[kCostFeatures];
+        brick_feats(b, brick_hist, brick_off, blist, tri_info, tri_ext, jobs, brick_bases, njobs,
+                    f);
+        float z = P[kOffB1 + j];
+#pragma unroll
+        for (int i = 0; i < kIn; i++) z += P[j * kIn + i] * f[i];
+        const float h1 = fmaxf(z, 0.0f);
+        z = P[kOffB2 + j];
+#pragma unroll
+        for (int k = 0; k < kHid; k++)
+            z += P[kOffW2 + j * kHid + k] *
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h1), k));
+        const float h2 = fmaxf(z, 0.0f);
+        const float y = wave_sum(P[kOffW3 + j] * h2) + P[kOffB3];
+        if (j == 0 && y > heavy) {  // false for NaN
+            hlist[atomicAdd(&cnt[1], 1u)] = b;
             skip[q] = 1u;
         }
     }

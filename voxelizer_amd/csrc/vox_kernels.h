@@ -489,5 +489,16 @@ __global__ void k_cost_heavy(const float *__restrict__ Y, const uint32_t *__rest
                              uint32_t *__restrict__ cnt, float heavy,
                              const uint32_t *__restrict__ order, uint32_t *__restrict__ hlist,
                              uint8_t *__restrict__ skip);
+// Features, MLP forward and heavy test of the screened candidates in one
+// launch (one wave per brick); the learned schedule's path.
+__global__ void k_cost_predict(const float *__restrict__ P, const uint32_t *__restrict__ order,
+                               const uint32_t *__restrict__ cand, uint32_t *__restrict__ cnt,
+                               float heavy, const uint32_t *__restrict__ brick_hist,
+                               const uint32_t *__restrict__ brick_off,
+                               const uint32_t *__restrict__ blist,
+                               const uint4 *__restrict__ tri_info, const uint4 *__restrict__ tri_ext,
+                               const Job *__restrict__ jobs,
+                               const uint32_t *__restrict__ brick_bases, uint32_t njobs,
+                               uint32_t *__restrict__ hlist, uint8_t *__restrict__ skip);
 
 }  // namespace vk
