@@ -382,13 +382,21 @@ __global__ __launch_bounds__(kWave) void k_solid_fill(const Job *__restrict__ jo
     const uint32_t tile = blockIdx.x;
     const uint32_t cnt = tile_hist[tile];
     const uint32_t off = tile_off[tile];
-    for (uint32_t i = tid; i < cnt; i += kWave) {
-        const uint32_t t = list[off + i];
+    // The tile's 16 rows are split over 2^lg lanes per triangle (the largest
+    // lg <= 4 with cnt << lg <= 64), so a tile with few triangles (the
+    // median one on large meshes has 8) still keeps the wave's lanes busy;
+    // crossings are XORed, so the split does not change the result.
+    uint32_t lg = 0;
+    while (lg < 4u && (cnt << (lg + 1u)) <= kWave) lg++;
+    const uint32_t items = cnt << lg, rows = 16u >> lg;
+    for (uint32_t i = tid; i < items; i += kWave) {
+        const uint32_t t = list[off + (i >> lg)];
+        const uint32_t r0 = (i & ((1u << lg) - 1u)) * rows;
         D3 v[3];
         load_tri_d(p, t, v);
         FillSetup fs;
         fill_setup(v, fs);
-        for (uint32_t r = 0; r < 16u; r++) {
+        for (uint32_t r = r0; r < r0 + rows; r++) {
             const uint32_t cy = ty * 4u + (r & 3u), cz = tz * 4u + (r >> 2);
             if (cy < p.smin[0] || cy >= p.smax[0] || cz < p.smin[1] || cz >= p.smax[1]) continue;
             uint32_t vx;
