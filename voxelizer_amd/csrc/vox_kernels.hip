@@ -387,7 +387,7 @@ __global__ __launch_bounds__(kWave) void k_solid_fill(const Job *__restrict__ jo
     // median one on large meshes has 8) still keeps the wave's lanes busy;
     // crossings are XORed, so the split does not change the result.
     uint32_t lg = 0;
-    while (lg < 4u && (cnt << (lg + 1u)) <= kWave) lg++;
+    while (lg < 4u && cnt <= (kWave >> (lg + 1u))) lg++;
     const uint32_t items = cnt << lg, rows = 16u >> lg;
     for (uint32_t i = tid; i < items; i += kWave) {
         const uint32_t t = list[off + (i >> lg)];
