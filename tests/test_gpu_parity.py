@@ -2,8 +2,9 @@
 C ABI.  Covers the workload configurations at their full sizes (icosphere
 128^3, shoebox room 512x403x303, FCC room 640x255x384, large hall
 1536x1024x768), every node type, materials on/off, y/z and material
-subspaces, FCC shifts of the solid grid, open and degenerate meshes, and
-size-independent properties at full size."""
+subspaces, FCC shifts of the solid grid, open and degenerate meshes, tiles
+and bricks holding from 1 to 130 triangles, and size-independent properties
+at full size."""
 import sys
 
 import numpy as np
@@ -138,6 +139,35 @@ def test_material_lookup_path(oracle, monkeypatch, kind):
         check_nodes(ctx, oracle, m, va.PLAN_CUBE_LENGTH, 0.012, kind)
     finally:
         ctx.close()
+
+
+def stacked_tiles_mesh(d):
+    """A unit box plus, per count n, a stack of n triangles in planes x = const
+    over one 4x4 tile of voxel-centre rows (stack k covers centre rows
+    8k+1..8k+3 in y and 5..7 in z), so tiles hold from 1 to 130 triangles:
+    every split of a tile's rows over lanes, and several passes per wave."""
+    counts = [1, 2, 3, 4, 5, 8, 9, 16, 17, 32, 33, 63, 64, 65, 130]
+    rng = np.random.default_rng(11)
+    b = W._Builder()
+    box = W.box((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), subdiv=3)
+    b.add(box.vertices, box.indices, np.full(box.nt, 1, np.uint8))
+    for k, n in enumerate(counts):
+        y0, z0 = (8 * k + 0.3) * d, 4.3 * d
+        xs = 0.05 + 0.9 * (np.arange(n) + rng.random(n) * 0.5) / n
+        v = np.zeros((3 * n, 3))
+        v[:, 0] = np.repeat(xs, 3)
+        v[0::3, 1:] = (y0, z0)
+        v[1::3, 1:] = (y0 + 6.8 * d, z0)
+        v[2::3, 1:] = (y0, z0 + 6.8 * d)
+        b.add(v, np.arange(3 * n).reshape(-1, 3), 1 + rng.integers(0, 7, n))
+    return b.mesh("stacked_tiles")
+
+
+@pytest.mark.parametrize("kind", [va.NODE_SHORT, va.NODE_LONG_FCC])
+def test_stacked_tiles(gpu_ctx, oracle, kind):
+    d = 0.0075
+    m = stacked_tiles_mesh(d)
+    check_nodes(gpu_ctx, oracle, m, va.PLAN_CUBE_LENGTH, d, kind)
 
 
 def test_repeatable(gpu_ctx):
