@@ -54,36 +54,55 @@ __device__ __forceinline__ void brick_feats(uint32_t b, const uint32_t *__restri
     const uint32_t cnt = brick_hist[b], off = brick_off[b];
     float c1 = 0, c2 = 0, c3 = 0, i1 = 0, i2 = 0, i3 = 0, v3 = 0, mx = 0, span = 0, small = 0,
           axx = 0, axy = 0;
-    for (uint32_t i = lane; i < cnt; i += kWave) {
-        const uint32_t t = blist[off + i];
-        const uint4 ti = tri_info[t];
-        const uint4 te = tri_ext[t];
-        const int bb = (int)(ti.x >> 30), a = (int)((ti.x >> 28) & 3u);
-        const uint32_t wbx = ti.z >> 16, wby = ti.w & 0xFFFFu, wbz = ti.w >> 16;
-        const uint32_t wa = a == AX_X ? wbx : a == AX_Y ? wby : wbz;
-        const uint32_t wuv = (wbx * wby * wbz) / max(wa, 1u);
-        const int32_t bda = a == AX_X ? bd0 : a == AX_Y ? bd1 : bd2;
-        float est;
-        if (bb == BB_1D) {
-            est = (float)te.w / (float)max(wa, 1u);
-            c1 += 1.0f;
-            i1 += est;
-        } else {
-            est = (float)te.w / (float)max(wuv, 1u);
-            if (bb == BB_2D) {
-                c2 += 1.0f;
-                i2 += est;
-            } else {
-                c3 += 1.0f;
-                i3 += est;
-                v3 += est * (float)min((int32_t)te.z, bda);
-                axx += a == AX_X ? 1.0f : 0.0f;
-                axy += a == AX_Y ? 1.0f : 0.0f;
-            }
+    // Four triangles per lane per step, their loads issued together (a heavy
+    // brick's hundreds of triangles would otherwise cost one round trip per
+    // 64); each lane still accumulates its triangles in index order.
+    constexpr uint32_t kU = 4;
+    for (uint32_t i0 = lane; i0 < cnt; i0 += kU * kWave) {
+        uint32_t tt[kU];
+        uint4 tis[kU], tes[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; u++) {
+            const uint32_t i = i0 + u * kWave;
+            tt[u] = i < cnt ? blist[off + i] : 0u;
         }
-        mx = fmaxf(mx, est);
-        span += (float)(wbx * wby * wbz);
-        small += est <= 4.0f ? 1.0f : 0.0f;
+#pragma unroll
+        for (uint32_t u = 0; u < kU; u++) {
+            const bool ok = i0 + u * kWave < cnt;
+            tis[u] = ok ? tri_info[tt[u]] : make_uint4(0u, 0u, 0u, 0u);
+            tes[u] = ok ? tri_ext[tt[u]] : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; u++) {
+            if (i0 + u * kWave >= cnt) break;
+            const uint4 ti = tis[u], te = tes[u];
+            const int bb = (int)(ti.x >> 30), a = (int)((ti.x >> 28) & 3u);
+            const uint32_t wbx = ti.z >> 16, wby = ti.w & 0xFFFFu, wbz = ti.w >> 16;
+            const uint32_t wa = a == AX_X ? wbx : a == AX_Y ? wby : wbz;
+            const uint32_t wuv = (wbx * wby * wbz) / max(wa, 1u);
+            const int32_t bda = a == AX_X ? bd0 : a == AX_Y ? bd1 : bd2;
+            float est;
+            if (bb == BB_1D) {
+                est = (float)te.w / (float)max(wa, 1u);
+                c1 += 1.0f;
+                i1 += est;
+            } else {
+                est = (float)te.w / (float)max(wuv, 1u);
+                if (bb == BB_2D) {
+                    c2 += 1.0f;
+                    i2 += est;
+                } else {
+                    c3 += 1.0f;
+                    i3 += est;
+                    v3 += est * (float)min((int32_t)te.z, bda);
+                    axx += a == AX_X ? 1.0f : 0.0f;
+                    axy += a == AX_Y ? 1.0f : 0.0f;
+                }
+            }
+            mx = fmaxf(mx, est);
+            span += (float)(wbx * wby * wbz);
+            small += est <= 4.0f ? 1.0f : 0.0f;
+        }
     }
     c1 = wave_sum(c1); c2 = wave_sum(c2); c3 = wave_sum(c3);
     i1 = wave_sum(i1); i2 = wave_sum(i2); i3 = wave_sum(i3);
