@@ -54,10 +54,10 @@ __device__ __forceinline__ void brick_feats(uint32_t b, const uint32_t *__restri
     const uint32_t cnt = brick_hist[b], off = brick_off[b];
     float c1 = 0, c2 = 0, c3 = 0, i1 = 0, i2 = 0, i3 = 0, v3 = 0, mx = 0, span = 0, small = 0,
           axx = 0, axy = 0;
-    // Four triangles per lane per step, their loads issued together (a heavy
+    // Eight triangles per lane per step, their loads issued together (a heavy
     // brick's hundreds of triangles would otherwise cost one round trip per
     // 64); each lane still accumulates its triangles in index order.
-    constexpr uint32_t kU = 4;
+    constexpr uint32_t kU = 8;
     for (uint32_t i0 = lane; i0 < cnt; i0 += kU * kWave) {
         uint32_t tt[kU];
         uint4 tis[kU], tes[kU];
