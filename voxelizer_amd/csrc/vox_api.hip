@@ -88,10 +88,12 @@ struct vox_ctx {
     bool cost_ready = false;
     float heavy_cost = 1.5f;    // standardized log cost above which a brick goes first
     // Pair count / mean from which a brick's cost is predicted.  Measured on
-    // the large hall and the 256-mesh batch: 1.5 -> 543 / 329 Gvoxels/s,
-    // 6 -> 548 / 335, 12 -> 552 / 337, 24 -> 551 / 337 (the features of a
-    // few thousand light bricks cost more than their order gains).
-    float screen_ratio = 12.0f;
+    // the large hall and the 256-mesh batch (Gvoxels/s): 6 -> 571 / 365,
+    // 12 -> 576 / 366, 24 -> 579 / 368 in one run, 24 -> 577 / 362,
+    // 48 -> 567 / 359, 96 -> 560 / 342 in another.  Below 24 the features of
+    // light bricks cost more than their order gains; above it heavy bricks
+    // are missed.
+    float screen_ratio = 24.0f;
     bool record = false;
     std::vector<float> rec_feats, rec_cycles;
     // worker contexts of batched runs (same device, own streams and buffers)
