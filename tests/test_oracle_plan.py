@@ -1,8 +1,8 @@
 """Grid planning of the oracle (src/host_side.cpp:2462-2634, 717-738).
 
 Pinned by hand-computed integer cases (the dims of the benchmark workloads
-worked out in the survey of the reference, and small boxes whose d divides
-their sides) and by the rounding rule of the x axis."""
+worked out by hand from the reference's dimension rules, and small boxes
+whose d divides their sides) and by the rounding rule of the x axis."""
 import numpy as np
 import pytest
 
