@@ -918,7 +918,13 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
             const uint32_t cnt = __builtin_amdgcn_readfirstlane(brick_hist[gb]);
             uint64_t t_start = 0;
             if (cycles && tid == 0) t_start = __builtin_amdgcn_s_memtime();
-            const uint32_t j = job_of(brick_bases, njobs, gb);
+            // Consecutive entries mostly share a job: its range is tested
+            // before a search.
+            const uint32_t j =
+                njobs > 1u && cur_job != UINT32_MAX && gb >= brick_bases[cur_job] &&
+                        gb < brick_bases[cur_job + 1u]
+                    ? cur_job
+                    : job_of(brick_bases, njobs, gb);
             const Job &J = jobs[j];
             if (j != cur_job) {  // uniform: the job's matSplitRes table into LDS
                 __syncthreads();
