@@ -16,29 +16,33 @@ Workloads:
   room, fcc, icosphere  the shoebox room (LongNode), the FCC room (LongFCCNode)
                  and the 128^3 icosphere.
 
-Schedules: "plain" processes surface bricks in index order (and, for a batch,
-the meshes one after another on one stream); "learned" takes the bricks the
-MLP cost model predicts to be expensive first (the model is trained first by
-the library's own GPU kernels on per-brick durations recorded on a
-differently seeded mesh; for a batch: all meshes fused into each stage's
-launches, bricks of all meshes in one list).  Executions: "graph" replays a
-HIP graph of the pipeline captured once (vox_graph_*; the step still plans
-the grid first), "eager" launches it call by call.  Every (schedule,
-execution) is timed; `value` is the one named by --schedule / --execution
-(default learned, graph).  A batch also reports the fused runs with the plain
-brick order (the model's own share) and 16 worker streams.
+Schedules: "plain" processes surface bricks in index order; "learned" takes
+the bricks the MLP cost model predicts to be expensive first, split into
+sub-bricks (the model is trained first by the library's own GPU kernels on
+per-brick durations recorded on a differently seeded mesh).  A batch runs
+fused under both (every stage one launch over all meshes, bricks of all
+meshes in one list), so learned / plain is the model's own share; the batch
+also reports the meshes one after another and 16 worker streams.
+Executions: "graph" replays a HIP graph of the pipeline captured once
+(vox_graph_*; the step still plans the grid first), "eager" launches it call
+by call.  Every (schedule, execution) is timed; `value` is the one named by
+--schedule / --execution (default learned, graph).
 
 Prints ONE JSON line.  `value` = allocated nodes per step / wall time per step
 (Gvoxels/s), timed over --steps steps between two device synchronisations.
 `roofline` is for the node-write kernel (k_nodes_cart / k_nodes_fcc): its
-algorithmic bytes (one node write + one solid-bit read per node) over its
+algorithmic bytes (one node written per node, N * sizeof(Node)) over its
 average HIP-event duration on the library's stream, from a second run of the
 same steps with events on (eager execution; events cost a synchronisation per
-step, so the timed steps run without them);
+step, so the timed steps run without them); `pipeline_frac` puts the whole
+step's algorithmic bytes (N * sizeof(Node) + 12 V + 12 T + T) over the step
+time;
 `traffic` is the HBM bytes per launch that rocprofv3 counted for it
 (profiles/traffic_*.json: FETCH_SIZE and WRITE_SIZE scaled by a plain copy).
 `cpu_baseline` times the CPU oracle (the restatement of the reference
-algorithm) on the host cores on a bounded sample of the same workload.
+algorithm) on the host cores: all cores on the full workload (`value`) and
+one thread on a bounded sample of the same mesh.  `check_vs_plain` compares
+the reported configuration's nodes with a plain eager run's.
 """
 from __future__ import annotations
 
@@ -87,6 +91,8 @@ def node_bytes(kind):
 
 
 def host_cores():
+    """Threads the CPU baseline uses: the host cores this process may run on,
+    at most 16 (the GPU box's CPU share)."""
     cores = os.cpu_count() or 1
     try:
         cores = len(os.sched_getaffinity(0))
@@ -95,28 +101,51 @@ def host_cores():
     return min(cores, 16)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def run_cpu_baseline(mesh, wl, budget_s):
-    """The CPU oracle on the host cores; if the full grid would exceed the
-    budget, on a coarser grid of the same mesh (named in the output)."""
+    """The CPU oracle (the restatement of the reference algorithm) on the
+    host: all cores (up to 16 threads) on the full workload, and one thread on
+    the same mesh at a coarser cube length sized to the budget (the full
+    1.2 G-node hall takes about a minute on one core).  value = all cores."""
     from oracle import oracle_py as O
     _, mode, param, kind, vs, ms, _ = wl
     cores = host_cores()
-    g = O.plan_grid(mesh.vertices, mode, param)
     fcc = 4 if kind in (3, 4) else 1
-    n_full = g.alloc[0] * g.alloc[1] * g.alloc[2] * fcc
-    # About 60 ns per node per core on this oracle; keep inside the budget.
-    est = n_full * 60e-9 * 8 / cores
-    sample = "full workload"
-    if est > budget_s and mode == 1:
-        scale = (budget_s / est) ** (1.0 / 3.0)
-        g = O.plan_grid(mesh.vertices, mode, param / scale)
-        sample = f"same mesh at cube length x{1 / scale:.2f} ({g.alloc[0]}x{g.alloc[1]}x{g.alloc[2]})"
-    n = g.alloc[0] * g.alloc[1] * g.alloc[2] * fcc
-    t0 = time.perf_counter()
-    O.voxelize_nodes(mesh, g, kind, True, vox_split=vs, mat_split=ms, n_threads=cores)
-    dt = time.perf_counter() - t0
-    return {"value": round(n / dt / 1e9, 4), "unit": "Gvoxels/s", "cores": cores,
-            "kind": "port", "sample": f"{sample}; {n} nodes in {dt:.2f} s"}
+
+    def timed(g, threads):
+        n = g.alloc[0] * g.alloc[1] * g.alloc[2] * fcc
+        t0 = time.perf_counter()
+        O.voxelize_nodes(mesh, g, kind, True, vox_split=vs, mat_split=ms, n_threads=threads)
+        return n, time.perf_counter() - t0
+
+    g = O.plan_grid(mesh.vertices, mode, param)
+    n, dt = timed(g, cores)
+    out = {"value": round(n / dt / 1e9, 4), "unit": "Gvoxels/s", "cores": cores, "kind": "port",
+           "sample": f"full workload, {n} nodes in {dt:.2f} s on {cores} threads",
+           "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+    # One thread: about 8x the all-core time per node on this box's share;
+    # a coarser grid of the same mesh keeps it inside the budget.
+    est1 = dt * max(cores, 1) * 0.7
+    g1, sample1 = g, "full workload"
+    if est1 > budget_s and mode == 1:
+        scale = (budget_s / est1) ** (1.0 / 3.0)
+        g1 = O.plan_grid(mesh.vertices, mode, param / scale)
+        sample1 = (f"same mesh at cube length x{1 / scale:.2f} "
+                   f"({g1.alloc[0]}x{g1.alloc[1]}x{g1.alloc[2]})")
+    n1, dt1 = timed(g1, 1)
+    out["single_thread"] = {"value": round(n1 / dt1 / 1e9, 4), "unit": "Gvoxels/s",
+                            "sample": f"{sample1}, {n1} nodes in {dt1:.2f} s"}
+    return out
 
 
 def run_cpu_baseline_batch(meshes, kinds, budget_s):
@@ -255,6 +284,26 @@ def bench_single(args, va, wl):
     stage = {k: v / args.steps for k, v in stage.items()}
     ctx.set_schedule(va.SCHEDULE_PLAIN)
 
+    # The nodes the reported configuration left in `out` against a plain,
+    # eager run of the same step into a second buffer.
+    check_plain = None
+    if (chosen, args.execution) != (va.SCHEDULE_PLAIN, "eager"):
+        ctx.set_schedule(chosen)
+        if args.execution == "graph":
+            gr = ctx.graph(item, chosen)
+            gr.launch()
+            gr.close()
+        else:
+            step()
+        ctx.sync()
+        got = out.download(np.uint8, n_nodes * nb)
+        ctx.set_schedule(va.SCHEDULE_PLAIN)
+        ref_buf = va.DeviceBuffer(ctx, n_nodes * nb)
+        g0 = ctx.plan_grid(mode, param)
+        ctx.voxelize_nodes_device(g0, kind, ref_buf.ptr.value, True, vs, ms)
+        check_plain = bool(np.array_equal(got, ref_buf.download(np.uint8, n_nodes * nb)))
+        ref_buf.free()
+        del got
     elapsed = results[(chosen, args.execution)]
     ms_per_step = elapsed * 1e3 / args.steps
     value = n_nodes / (elapsed / args.steps) / 1e9
@@ -263,7 +312,9 @@ def bench_single(args, va, wl):
 
     kname = "k_nodes_fcc" if kind in (3, 4) else "k_nodes_cart"
     kern_ms = float(np.mean(node_ms))
-    alg_bytes = n_nodes * (nb + 0.125)
+    # Algorithmic bytes of the node kernel: one node written per node (N s);
+    # the solid-bit read (1/8 B per node) is reported beside it.
+    alg_bytes = n_nodes * nb
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
     pipe_bytes = n_nodes * nb + 12 * mesh.nv + 12 * mesh.nt + mesh.nt
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -271,6 +322,9 @@ def bench_single(args, va, wl):
             "traffic": load_traffic(kname, args.workload),
             "kernel": kname, "kernel_ms": round(kern_ms, 4),
             "algorithmic_bytes": int(alg_bytes),
+            "frac_with_bit_read": round(n_nodes * (nb + 0.125) / (kern_ms * 1e-3) / 1e9 /
+                                        HBM_PEAK_GBS, 4),
+            "pipeline_bytes": int(pipe_bytes),
             "pipeline_frac": round(pipe_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     cpu = run_cpu_baseline(mesh, wl, args.cpu_budget) if args.cpu_baseline == "on" else None
     check = None
@@ -293,6 +347,8 @@ def bench_single(args, va, wl):
         "stages_ms": {k: round(v, 4) for k, v in stage.items()},
         "roofline": roof, "cpu_baseline": cpu,
     }
+    if check_plain is not None:
+        res["check_vs_plain"] = check_plain
     if check is not None:
         res["check_vs_oracle"] = check
     return res
@@ -350,18 +406,23 @@ def bench_batch(args, va, wl):
         it.nodes_dev = out.ptr.value
         n_nodes += nn
         out_bytes += nn * node_bytes(kind)
-        node_alg += nn * (node_bytes(kind) + 0.125)  # node write + solid-bit read
+        node_alg += nn * node_bytes(kind)  # node write
     # Modes: (brick order, streams, graph); streams 0 = fused (each stage one
     # launch over all meshes), 1 = one mesh after another, k = k worker
-    # streams; graph = the fused batch captured once and replayed.
-    modes = {"plain": (va.SCHEDULE_PLAIN, 1, False),
+    # streams; graph = the fused batch captured once and replayed.  The
+    # plain schedule the learned one is compared with is the same fused
+    # execution with bricks in index order.
+    modes = {"sequential_plain": (va.SCHEDULE_PLAIN, 1, False),
              "fused_plain": (va.SCHEDULE_PLAIN, 0, False),
              "learned": (va.SCHEDULE_LEARNED, 0, False),
              "fused_plain_graph": (va.SCHEDULE_PLAIN, 0, True),
              "learned_graph": (va.SCHEDULE_LEARNED, 0, True),
              f"streams{BATCH_STREAMS}_learned": (va.SCHEDULE_LEARNED, BATCH_STREAMS, False)}
-    reported = ("learned_graph" if args.execution == "graph" else "learned") \
-        if args.schedule == "learned" else "plain"
+    graph_x = args.execution == "graph"
+    reported = ("learned_graph" if graph_x else "learned") if args.schedule == "learned" \
+        else ("fused_plain_graph" if graph_x else "fused_plain")
+    plain_mode = "fused_plain_graph" if graph_x else "fused_plain"
+    learned_mode = "learned_graph" if graph_x else "learned"
     if args.only:  # profiling one mode
         modes = {reported: modes[reported]}
     results = {}
@@ -392,7 +453,7 @@ def bench_batch(args, va, wl):
     ctx.enable_timing(False)
     elapsed = results[reported]
     gv = {k: n_nodes / (v / args.steps) / 1e9 for k, v in results.items()}
-    plain_v, learned_v = gv.get("plain"), gv.get(reported if args.schedule == "learned" else "")
+    plain_v, learned_v = gv.get(plain_mode), gv.get(learned_mode)
 
     def ratio(a, b):
         return round(a / b, 4) if a and b else None
@@ -408,17 +469,16 @@ def bench_batch(args, va, wl):
         "data": "synthetic (seeded procedural meshes)",
         "config": {"workload": label, "schedule": args.schedule, "meshes": len(meshes),
                    "nodes": n_nodes, "triangles": sum(m.nt for m in meshes),
-                   "execution": {"learned_graph": "fused batch, replayed HIP graph",
-                                 "learned": "fused batch (each stage one launch over all meshes)",
-                                 "plain": "one mesh after another"}[reported],
+                   "execution": "fused batch, replayed HIP graph" if graph_x else
+                                "fused batch (each stage one launch over all meshes)",
                    "parallelism": "single GPU"},
         "schedules": {"plain_gvox_s": plain_v and round(plain_v, 4),
                       "learned_gvox_s": learned_v and round(learned_v, 4),
                       "learned_over_plain": ratio(learned_v, plain_v),
+                      "plain_is": f"{plain_mode}: the same fused execution, bricks in index order",
                       "modes_gvox_s": {k: round(v, 4) for k, v in gv.items()},
-                      "learned_over_fused_plain": ratio(gv.get("learned"), gv.get("fused_plain")),
-                      "graph_learned_over_fused_plain_graph":
-                          ratio(gv.get("learned_graph"), gv.get("fused_plain_graph")),
+                      "fused_over_sequential": ratio(gv.get("fused_plain"),
+                                                     gv.get("sequential_plain")),
                       "stages_ms": stages, "cost_model": training},
         "roofline": {"bound": "hbm", "achieved": round(node_alg / (stages["nodes_ms"] * 1e-3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",

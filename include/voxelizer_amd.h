@@ -177,21 +177,26 @@ vox_status vox_get_stage_times(const vox_ctx *ctx, vox_stage_times *out);
 /* ---- brick schedule and its learned cost model (no counterpart in the
  * reference) --------------------------------------------------------------
  * The surface pass processes its non-empty bricks from a list with
- * persistent workgroups.  PLAIN lists them in index order; LEARNED sorts them
- * by the cost an MLP (16 features -> 64 -> 64 -> 1, ReLU, FP32) predicts, most
- * expensive first.  Outputs are identical under both.  LEARNED falls back to
- * PLAIN while no model is set (vox_stage_times.learned_schedule says which
- * ran). */
+ * persistent workgroups.  PLAIN lists them in index order; LEARNED moves the
+ * bricks an MLP (16 features -> 64 -> 64 -> 1, ReLU, FP32) predicts to be
+ * expensive to the front of the list and cuts the most expensive into
+ * sub-bricks (see vox_set_heavy_threshold).  Outputs are identical under
+ * both.  LEARNED falls back to PLAIN while no model is set
+ * (vox_stage_times.learned_schedule says which ran). */
 typedef enum vox_schedule { VOX_SCHEDULE_PLAIN = 0, VOX_SCHEDULE_LEARNED = 1 } vox_schedule;
 #define VOX_COST_FEATURES 16
 #define VOX_COST_PARAMS 5313 /* W1[64][16], b1[64], W2[64][64], b2[64], W3[64], b3 */
 vox_status vox_set_schedule(vox_ctx *ctx, vox_schedule schedule);
-/* LEARNED order: bricks with at least 1.5 times the mean number of
- * (triangle, brick) pairs are scored by the model; those whose predicted cost
- * (standardized log cycles) exceeds `threshold` run first, most expensive
- * first; all others keep index order (neighbouring bricks share triangles in
- * cache).  Default threshold 1.5. */
+/* LEARNED order: bricks with at least `screen ratio` times the mean number
+ * of (triangle, brick) pairs (default 24) are scored by the model; those
+ * whose predicted cost (standardized log cycles) exceeds `threshold` run
+ * first, as they were predicted, one per grab; a brick predicted more than 1
+ * above the threshold is cut into 2 sub-bricks along z (more than 2 above:
+ * 4, at most its depth), so several workgroups share it.  All other bricks
+ * keep index order (neighbouring bricks share triangles in cache).  Default
+ * threshold 1.5. */
 vox_status vox_set_heavy_threshold(vox_ctx *ctx, float threshold);
+vox_status vox_set_screen_ratio(vox_ctx *ctx, float ratio);
 /* Model parameters (host arrays of VOX_COST_PARAMS floats). */
 vox_status vox_cost_set_params(vox_ctx *ctx, const float *params);
 vox_status vox_cost_get_params(vox_ctx *ctx, float *params);

@@ -961,6 +961,349 @@ int voxelize_nodes(const float *V, u32 nv, const u32 *I, u32 nt, const uint8_t *
     return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// Fragile decisions: where the original CUDA build could legitimately end
+// with different nodes than this restatement.
+//  * nvcc contracts a*b + c into one FMA by default, so a test value within a
+//    few ulps of its threshold can fall on the other side there.  A decision
+//    is listed when its value lies within eps times the magnitude of its
+//    terms (eps64 for the FP64 solid tests, eps32 for the FP32 surface
+//    tests), and so are the voxels it decides.
+//  * processVoxel's plain store races (src/voxelizer.cu:2988): a voxel
+//    written by several triangles of the highest class (kernels run 1D, 2D,
+//    3D in order) with different materials may keep any of them.
+// Per-voxel flags: 1 solid state rests on a fragile solid decision, 2 surface
+// membership rests on a fragile surface decision, 4 written by several
+// triangles, 8 several materials among the highest-class writers.
+// ---------------------------------------------------------------------------
+enum : uint8_t { FR_SOLID = 1, FR_SURF = 2, FR_MULTI = 4, FR_RACE = 8 };
+
+// Values that are exactly on their threshold are not listed: they come from
+// structured geometry (planes on voxel boundaries, centres on edges, the
+// cases the DBL_MIN tie break settles) whose products are exact, and a
+// contracted evaluation computes the same exact value.
+inline bool near_zero(double v, double scale, double eps) {
+    return v != 0.0 && std::fabs(v) <= eps * scale;
+}
+// x within eps of an integer (relative to |x| + 1), but not on it.
+inline bool near_int(double x, double eps) {
+    const double r = std::fabs(x - std::nearbyint(x));
+    return r != 0.0 && r <= eps * (std::fabs(x) + 1.0);
+}
+
+struct FragileMap {
+    uint8_t *flags;
+    size_t n;
+    // node index of padded voxel (x, y, z) of a Cartesian grid or FCC sub-grid
+    u32 X, Y, Z;
+    int gridType;
+    size_t index(u32 x, u32 y, u32 z) const {
+        if (gridType == 0) return ((size_t)z * Y + y) * X + x;
+        size_t l = 2 * (size_t)X, a = l * Y;
+        size_t ox = gridType == 2 || gridType == 4, oz = gridType == 3 || gridType == 4;
+        return a * (2 * (size_t)z + oz) + l * y + 2 * (size_t)x + ox;
+    }
+    void mark(u32 x, u32 y, u32 z, uint8_t f) const {
+        size_t i = index(x, y, z);
+        if (i < n) __atomic_fetch_or(&flags[i], f, __ATOMIC_RELAXED);
+    }
+};
+
+// Solid decisions of one subspace (perform_voxelization's queue and fill).
+// Counts: [0] fill tests evaluated, [1] of them fragile, [2] fragile crossing
+// positions.
+void fragile_solid_subspace(const SolidCtx &c, Span ys, Span zs, double eps,
+                            const FragileMap &fm, uint64_t counts[3]) {
+    const u32 smin[2] = {ys.min, zs.min}, smax[2] = {ys.max, zs.max};
+    uint64_t n_tests = 0, n_frag = 0, n_cross = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : n_tests, n_frag, n_cross)
+    for (long tl = 0; tl < (long)c.nt; tl++) {
+        const u32 t = (u32)tl;
+        D3 v[3] = {fetch_d(c.V, c.I, t, 0), fetch_d(c.V, c.I, t, 1), fetch_d(c.V, c.I, t, 2)};
+        TileSetup ts;
+        if (!tile_setup(v, c.extMin, c.d, smin, smax, ts)) continue;
+        const double tN = (v[0].y * v[1].z + v[2].y * v[0].z + v[2].z * v[1].y) -
+                          (v[2].y * v[1].z + v[0].z * v[1].y + v[2].z * v[0].y);
+        const double tNs = std::fabs(v[0].y * v[1].z) + std::fabs(v[2].y * v[0].z) +
+                           std::fabs(v[2].z * v[1].y) + std::fabs(v[2].y * v[1].z) +
+                           std::fabs(v[0].z * v[1].y) + std::fabs(v[2].z * v[0].y);
+        const bool tri_fragile = near_zero(tN, tNs, eps);
+        const FillSetup fs = fill_setup(v);
+        for (u32 ty = ts.tmin[0]; ty <= ts.tmax[0]; ty++)
+            for (u32 tz = ts.tmin[1]; tz <= ts.tmax[1]; tz++) {
+                // Filter: queued, and queued only through fragile centres?
+                bool hit = false, solid_hit = false;
+                for (u32 i = ty * 4; i < ty * 4 + 4 && i < ts.cmax[0]; i++)
+                    for (u32 j = tz * 4; j < tz * 4 + 4 && j < ts.cmax[1]; j++) {
+                        if (i < ts.cmin[0] || j < ts.cmin[1]) continue;
+                        D2 p = {double(i) * c.d + c.extMin[1], double(j) * c.d + c.extMin[2]};
+                        bool r = true, sure = true;
+                        for (int k = 0; k < 3; k++) {
+                            double a = ts.ne[k].x * p.x, b = ts.ne[k].y * p.y;
+                            double val = a + b + ts.de[k];
+                            r &= val >= 0.0;
+                            sure &= !near_zero(val, std::fabs(a) + std::fabs(b) + std::fabs(ts.de[k]), eps);
+                        }
+                        if (r) {
+                            hit = true;
+                            solid_hit |= sure;
+                        }
+                    }
+                if (!hit) continue;
+                const bool queue_fragile = tri_fragile || !solid_hit;
+                for (int lane = 0; lane < 16; lane++) {
+                    const u32 cy = u32(lane % 4) + ty * 4, cz = u32(lane / 4) + tz * 4;
+                    if (cy < smin[0] || cy >= smax[0] || cz < smin[1] || cz >= smax[1]) continue;
+                    D2 p = {double(cy) * c.d + c.extMin[1], double(cz) * c.d + c.extMin[2]};
+                    u32 ok = 0;
+                    bool frag = queue_fragile;
+                    for (int i = 0; i < 3; i++) {
+                        double a = fs.ne[i].x * p.x, b = fs.ne[i].y * p.y;
+                        double r = a + b + fs.de[i];
+                        frag |= near_zero(r, std::fabs(a) + std::fabs(b) + std::fabs(fs.de[i]), eps);
+                        if (fs.ne[i].x > 0.0 || (fs.ne[i].x == 0.0 && fs.ne[i].y < 0.0)) r += DBL_MIN;
+                        if (r > 0.0) ok++;
+                    }
+                    n_tests++;
+                    if (ok != 3 && !frag) continue;
+                    // Where the crossing is (or would be).
+                    D3 A = {fs.v0.x - c.extMin[0], fs.v0.y - p.x, fs.v0.z - p.y};
+                    double B = A.x * fs.n.x + A.y * fs.n.y + A.z * fs.n.z;
+                    double px = B / fs.n.x;
+                    if (!(px == px)) continue;
+                    px = px < 0.0 ? 0.0 : px;
+                    const double q = px / c.d;
+                    const u32 X0 = sat_u32(std::ceil(q)) + 1u;
+                    if (frag) {
+                        n_frag++;
+                        for (u32 x = X0; x < c.X; x++) fm.mark(x, cy + 1, cz + 1, FR_SOLID);
+                    } else if (near_int(q, eps)) {
+                        // ceil(q) may land one voxel lower or higher: the voxel
+                        // between the two crossing positions changes state.
+                        n_cross++;
+                        const double k = std::nearbyint(q);
+                        const u32 xv = sat_u32(k) + 1u;
+                        if (xv < c.X) fm.mark(xv, cy + 1, cz + 1, FR_SOLID);
+                    }
+                }
+            }
+    }
+    counts[0] += n_tests;
+    counts[1] += n_frag;
+    counts[2] += n_cross;
+}
+
+// Surface decisions of one triangle in one subspace, mirroring
+// surface_triangle.  Counts: [0] tests evaluated, [1] fragile.  Every write
+// also goes to the writer record of the voxel (rec: class << 10 | count << 8
+// | material, count saturating at 3; a 0x80000000 bit marks two materials
+// among the highest-class writers).
+struct WriterRec {
+    u32 *rec;
+    size_t n;
+};
+inline void record_writer(const WriterRec &w, size_t i, u32 cls, u32 mat) {
+    if (i >= w.n) return;
+    u32 cur = __atomic_load_n(&w.rec[i], __ATOMIC_RELAXED);
+    while (true) {
+        u32 nxt;
+        const u32 cnt = (cur >> 8) & 3u;
+        if (cnt == 0) {
+            nxt = (cls << 10) | (1u << 8) | mat;
+        } else {
+            const u32 ccls = (cur >> 10) & 3u, cmat = cur & 0xFFu;
+            const u32 ncnt = cnt < 3 ? cnt + 1 : 3;
+            if (cls > ccls) nxt = (cls << 10) | (ncnt << 8) | mat;
+            else if (cls == ccls)
+                nxt = (cur & ~(3u << 8)) | (ncnt << 8) | (mat != cmat ? 0x80000000u : 0u);
+            else nxt = (cur & ~(3u << 8)) | (ncnt << 8);
+        }
+        if (__atomic_compare_exchange_n(&w.rec[i], &cur, nxt, true, __ATOMIC_RELAXED,
+                                        __ATOMIC_RELAXED))
+            return;
+    }
+}
+
+// A point's projection on plane P: 0 = (y, z), 1 = (z, x), 2 = (x, y).
+inline F2 proj(const float p[3], int P) {
+    return P == 0 ? F2{p[1], p[2]} : P == 1 ? F2{p[2], p[0]} : F2{p[0], p[1]};
+}
+// Whether one of the three edge values of a projected test lies within eps
+// of 0 relative to its terms.
+inline bool frag_test(const OverlapData &o, F2 p2, double eps) {
+    bool f = false;
+    for (int i = 0; i < 3; i++) {
+        const float a = o.ne[i].x * p2.x, b = o.ne[i].y * p2.y;
+        const float v = (a + b) + o.de[i];
+        f |= near_zero((double)v, std::fabs((double)a) + std::fabs((double)b) +
+                                      std::fabs((double)o.de[i]), eps);
+    }
+    return f;
+}
+
+void fragile_surface_triangle(const F3 t[3], u32 type, u32 mat, F3 m, float d, const u32 smin[3],
+                              const u32 smax[3], const SurfTarget &st, const FragileMap &fm,
+                              const WriterRec &wr, double eps, uint64_t &n_tests,
+                              uint64_t &n_frag) {
+    const int bb = (int)(type >> 30);
+    const int axis = (int)((type >> 28) & 3u);
+    F3 tmn, tmx;
+    tri_bounds(t, tmn, tmx);
+    VoxBB vb = voxel_bounds_half(tmn, tmx, m, d);
+    const float mm[3] = {m.x, m.y, m.z};
+    const float lo3[3] = {(tmn.x - m.x) / d - 0.5f, (tmn.y - m.y) / d - 0.5f, (tmn.z - m.z) / d - 0.5f};
+    const float hi3[3] = {(tmx.x - m.x) / d + 0.5f, (tmx.y - m.y) / d + 0.5f, (tmx.z - m.z) / d + 0.5f};
+    // The class, axis and voxel range rest on six roundings and, for 3D
+    // triangles, on the dominant-axis comparisons.
+    bool tri_frag = false;
+    for (int a = 0; a < 3; a++) tri_frag |= near_int(lo3[a], eps) || near_int(hi3[a], eps);
+    const F3 n = cross(sub(t[0], t[2]), sub(t[1], t[0]));
+    const float nn[3] = {n.x, n.y, n.z};
+    if (bb == BB_3D) {
+        const float ax = std::fabs(n.x), ay = std::fabs(n.y), az = std::fabs(n.z);
+        const double big = std::max(ax, std::max(ay, az));
+        tri_frag |= near_zero((double)ax - ay, big, eps) || near_zero((double)ax - az, big, eps) ||
+                    near_zero((double)ay - az, big, eps);
+    }
+    if (!adjust_range(vb, smin, smax)) return;
+    const int32_t vlo[3] = {vb.min.x, vb.min.y, vb.min.z}, vhi[3] = {vb.max.x, vb.max.y, vb.max.z};
+    auto visit = [&](const int32_t idx[3], bool written, bool frag) {
+        const int32_t x = idx[0] + 1, y = idx[1] + 1, z = idx[2] + 1;
+        if (written) {
+            const size_t i = st.index(x, y, z);
+            if (i < st.nkeys) record_writer(wr, i, (u32)bb, mat);
+        }
+        if ((written || frag) && (frag || tri_frag)) fm.mark((u32)x, (u32)y, (u32)z, FR_SURF);
+    };
+    if (bb == BB_1D) {
+        int32_t idx[3] = {vlo[0], vlo[1], vlo[2]};
+        for (idx[axis] = vlo[axis]; idx[axis] <= vhi[axis]; idx[axis]++) visit(idx, true, false);
+        return;
+    }
+    const OverlapData ov[3] = {setup_yz(t, n, d), setup_zx(t, n, d), setup_xy(t, n, d)};
+    const int pd = axis == AX_X ? 0 : axis == AX_Y ? 1 : 2;  // plane normal to the axis
+    const int u = axis == AX_X ? AX_Y : AX_X, w = axis == AX_Z ? AX_Y : AX_Z;
+    int32_t idx[3];
+    float p[3];
+    for (idx[u] = vlo[u]; idx[u] <= vhi[u]; idx[u]++)
+        for (idx[w] = vlo[w]; idx[w] <= vhi[w]; idx[w]++) {
+            p[u] = vox_corner(idx[u], mm[u], d);
+            p[w] = vox_corner(idx[w], mm[w], d);
+            p[axis] = 0.0f;
+            n_tests++;
+            const bool fcol = frag_test(ov[pd], proj(p, pd), eps);
+            n_frag += fcol;
+            const bool in = test2(ov[pd], proj(p, pd));
+            if (bb == BB_2D) {
+                idx[axis] = vlo[axis];
+                visit(idx, in, fcol);
+                continue;
+            }
+            if (!in && !fcol) continue;
+            // Depth range of the column (determineDepthRange*), with the
+            // margins of its two end roundings.
+            float pmin[3] = {p[0], p[1], p[2]}, pmax[3] = {p[0], p[1], p[2]};
+            for (int c = 0; c < 3; c++) {
+                if (c == axis) continue;
+                const bool pos = nn[c] > 0.0f;
+                pmin[c] = p[c] + float(pos) * d;
+                pmax[c] = p[c] + float(!pos) * d;
+            }
+            auto depth = [&](const float q[3], double &scale) {
+                const F3 e = {t[0].x - q[0], t[0].y - q[1], t[0].z - q[2]};
+                scale = (std::fabs((double)e.x * n.x) + std::fabs((double)e.y * n.y) +
+                         std::fabs((double)e.z * n.z)) / std::fabs((double)nn[axis]);
+                return dot(e, n) / nn[axis];
+            };
+            double sa, sb;
+            float a = depth(pmin, sa), b = depth(pmax, sb);
+            if (a > b) {
+                std::swap(a, b);
+                std::swap(sa, sb);
+            }
+            const float qa = (a - mm[axis]) / d - 0.5f, qb = (b - mm[axis]) / d + 0.5f;
+            const int32_t lo0 = vox_cbound_min(a, mm[axis], d), hi0 = vox_cbound_max(b, mm[axis], d);
+            const double ra = std::fabs(qa - std::nearbyint(qa)), rb = std::fabs(qb - std::nearbyint(qb));
+            const bool fa = ra != 0.0 && ra <= eps * (std::fabs(qa) + 1.0 + sa / d);
+            const bool fb = rb != 0.0 && rb <= eps * (std::fabs(qb) + 1.0 + sb / d);
+            n_tests++;
+            n_frag += fa || fb;
+            const int32_t lo = std::max(lo0 - (fa ? 1 : 0), vlo[axis]);
+            const int32_t hi = std::min(hi0 + (fb ? 1 : 0), vhi[axis]);
+            const int32_t rlo = std::max(lo0, vlo[axis]), rhi = std::min(hi0, vhi[axis]);
+            for (idx[axis] = lo; idx[axis] <= hi; idx[axis]++) {
+                const bool inrange = idx[axis] >= rlo && idx[axis] <= rhi;
+                const bool endf = (fa && (idx[axis] == lo0 || idx[axis] == lo0 - 1)) ||
+                                  (fb && (idx[axis] == hi0 || idx[axis] == hi0 + 1));
+                p[axis] = vox_corner(idx[axis], mm[axis], d);
+                bool write = inrange && rlo == rhi;
+                bool f = fcol || endf;
+                if (inrange && !write) {
+                    n_tests++;
+                    const int p1 = (pd + 1) % 3, p2 = (pd + 2) % 3;
+                    const bool fi = frag_test(ov[p1], proj(p, p1), eps) ||
+                                    frag_test(ov[p2], proj(p, p2), eps);
+                    n_frag += fi;
+                    f |= fi;
+                    write = test2(ov[p1], proj(p, p1)) && test2(ov[p2], proj(p, p2));
+                }
+                visit(idx, in && write, f);
+            }
+        }
+}
+
+// Whole analysis for one grid (Cartesian, or FCC sub-grid gridType 1..4).
+void fragile_grid(const float *V, const u32 *I, u32 nt, const uint8_t *mats, const vox_grid &g,
+                  const double extMin[3], int gridType, const u32 voxSplit[3],
+                  const u32 matSplit[3], double eps64, double eps32, uint8_t *flags, size_t N,
+                  u32 *writers, uint64_t counts[5]) {
+    SolidCtx c;
+    c.V = V;
+    c.I = I;
+    c.nt = nt;
+    for (int a = 0; a < 3; a++) c.extMin[a] = extMin[a];
+    c.d = g.voxel_length;
+    c.X = g.res[0];
+    c.Yext = g.res[1];
+    c.Zext = g.res[2];
+    c.xSplits = g.x_splits;
+    c.bits = nullptr;
+    FragileMap fm{flags, N, g.alloc[0], g.alloc[1], g.alloc[2], gridType};
+    std::vector<Span> ys = split_axis_max(c.Yext, voxSplit[1]), zs = split_axis_max(c.Zext, voxSplit[2]);
+    for (const Span &z : zs)
+        for (const Span &y : ys) fragile_solid_subspace(c, y, z, eps64, fm, counts);
+    if (!mats) return;
+    const F3 m = {(float)extMin[0], (float)extMin[1], (float)extMin[2]};
+    const float d = (float)g.voxel_length;
+    std::vector<u32> types(nt);
+    for (u32 t = 0; t < nt; t++) {
+        F3 tri[3] = {fetch_f(V, I, t, 0), fetch_f(V, I, t, 1), fetch_f(V, I, t, 2)};
+        types[t] = encode_type(classify(tri, m, d));
+    }
+    SurfTarget st{g.alloc[0], g.alloc[1], g.alloc[2], gridType, nullptr, N};
+    WriterRec wr{writers, N};
+    std::vector<Span> xs = split_axis_max(g.res[0], matSplit[0]);
+    std::vector<Span> sys = split_axis_max(g.res[1], matSplit[1]);
+    std::vector<Span> szs = split_axis_max(g.res[2], matSplit[2]);
+    uint64_t n_tests = 0, n_frag = 0;
+    for (const Span &z : szs)
+        for (const Span &y : sys)
+            for (const Span &x : xs) {
+                const u32 smin[3] = {x.min, y.min, z.min}, smax[3] = {x.max, y.max, z.max};
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : n_tests, n_frag)
+                for (long t = 0; t < (long)nt; t++) {
+                    if ((types[t] >> 30) == BB_DEGEN) continue;
+                    F3 tri[3] = {fetch_f(V, I, (u32)t, 0), fetch_f(V, I, (u32)t, 1),
+                                 fetch_f(V, I, (u32)t, 2)};
+                    fragile_surface_triangle(tri, types[t], mats[t], m, d, smin, smax, st, fm, wr,
+                                             eps32, n_tests, n_frag);
+                }
+            }
+    counts[3] += n_tests;
+    counts[4] += n_frag;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1120,6 +1463,58 @@ int orc_fcc_boundaries(const uint8_t *solid, uint32_t FX, uint32_t FY, uint32_t 
     for (size_t i = 0; i < N; i++) nodes[i] = {(uint16_t)(solid[i] ? 1 : 0), 0};
     fcc_boundaries(nodes, FX, FY, FZ, {0, FY}, {0, FZ});
     for (size_t i = 0; i < N; i++) bid_out[i] = nodes[i].bid;
+    return 0;
+}
+
+int orc_fragile(const float *vertices, uint32_t nv, const uint32_t *indices, uint32_t nt,
+                const uint8_t *materials, const vox_grid *grid, int node_kind,
+                const uint32_t vox_split[3], const uint32_t mat_split[3], double eps64,
+                double eps32, uint8_t *flags, uint64_t counts[8]) {
+    (void)nv;
+    const bool fcc = node_kind == VOX_NODE_SHORT_FCC || node_kind == VOX_NODE_LONG_FCC;
+    if (!fcc && node_kind != VOX_NODE_SHORT && node_kind != VOX_NODE_LONG) return 1;
+    const vox_grid &g = *grid;
+    const size_t N = (size_t)g.alloc[0] * g.alloc[1] * g.alloc[2] * (fcc ? 4 : 1);
+    std::memset(flags, 0, N);
+    for (int k = 0; k < 8; k++) counts[k] = 0;
+    std::vector<u32> writers(materials ? N : 0, 0u);
+    const double base[3] = {g.min_vertex[0], g.min_vertex[1], g.min_vertex[2]};
+    if (!fcc) {
+        fragile_grid(vertices, indices, nt, materials, g, base, 0, vox_split, mat_split, eps64,
+                     eps32, flags, N, writers.data(), counts);
+    } else {
+        const double h = g.voxel_length / 2.0;
+        const double shifts[4][3] = {{0.0, 0.0, 0.0}, {h, h, 0.0}, {0.0, h, h}, {h, 0.0, h}};
+        for (int gt = 1; gt <= 4; gt++) {
+            const double em[3] = {base[0] + shifts[gt - 1][0], base[1] + shifts[gt - 1][1],
+                                  base[2] + shifts[gt - 1][2]};
+            fragile_grid(vertices, indices, nt, materials, g, em, gt, vox_split, mat_split, eps64,
+                         eps32, flags, N, writers.data(), counts);
+        }
+    }
+    uint64_t multi = 0, race = 0;
+    if (materials) {
+#pragma omp parallel for reduction(+ : multi, race) schedule(static)
+        for (long i = 0; i < (long)N; i++) {
+            const u32 r = writers[i];
+            uint8_t f = 0;
+            if (((r >> 8) & 3u) >= 2u) {
+                f |= FR_MULTI;
+                multi++;
+            }
+            if (r & 0x80000000u) {
+                f |= FR_RACE;
+                race++;
+            }
+            flags[i] |= f;
+        }
+    }
+    counts[5] = multi;
+    counts[6] = race;
+    uint64_t any = 0;
+#pragma omp parallel for reduction(+ : any) schedule(static)
+    for (long i = 0; i < (long)N; i++) any += flags[i] != 0;
+    counts[7] = any;
     return 0;
 }
 

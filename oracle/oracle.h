@@ -74,6 +74,24 @@ int orc_center_crossing(const float tri[9], const double ext_min[3], double d,
 uint32_t orc_tile_overlaps(const float tri[9], const double ext_min[3], double d,
                            uint32_t ext_y, uint32_t ext_z);
 
+/* Decisions where the original CUDA build could end with different nodes:
+ * FP tests whose value lies within eps (relative to the magnitude of their
+ * terms; eps64 for the FP64 solid tests, eps32 for the FP32 surface tests)
+ * of the threshold, which FMA contraction in nvcc's default build can move,
+ * and voxels several triangles write (the racy store of processVoxel,
+ * src/voxelizer.cu:2988).  flags (one byte per node, node_count bytes):
+ * 1 solid state rests on a fragile solid decision (a fragile crossing test
+ * marks the rest of its row), 2 surface membership rests on a fragile
+ * surface decision, 4 written by two or more triangles, 8 the writers of the
+ * highest class carry two or more materials (the material may differ).
+ * counts: [0] solid fill tests, [1] fragile ones, [2] fragile crossing
+ * positions, [3] surface tests, [4] fragile ones, [5] voxels flagged 4,
+ * [6] flagged 8, [7] flagged at all.  Returns 0 on success. */
+int orc_fragile(const float *vertices, uint32_t nv, const uint32_t *indices, uint32_t nt,
+                const uint8_t *materials, const vox_grid *grid, int node_kind,
+                const uint32_t vox_split[3], const uint32_t mat_split[3], double eps64,
+                double eps32, uint8_t *flags, uint64_t counts[8]);
+
 /* Erosion + boundary ids on a raw Cartesian solid grid (0/1 per node),
  * per y/z subspace like the reference's host loop; literal = in-place
  * sequential passes, else snapshot passes (same fixed point). */

@@ -63,6 +63,8 @@ def lib():
         L.orc_fcc_boundaries.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, P]
         L.orc_tile_overlaps.restype = C.c_uint32
         L.orc_tile_overlaps.argtypes = [P, P, C.c_double, C.c_uint32, C.c_uint32]
+        L.orc_fragile.argtypes = [P, C.c_uint32, P, C.c_uint32, P, C.POINTER(Grid), C.c_int, P, P,
+                                  C.c_double, C.c_double, P, P]
         _lib = L
     return _lib
 
@@ -185,3 +187,32 @@ def bits_to_grid(bits: np.ndarray, alloc) -> np.ndarray:
     # MSB first: bit 31 is x%32 == 0.
     be = b.astype(">u4").view(np.uint8).reshape(Z, Y, X // 32, 4)
     return np.unpackbits(be, axis=3, bitorder="big").reshape(Z, Y, X).astype(bool)
+
+
+FRAGILE_SOLID, FRAGILE_SURF, FRAGILE_MULTI, FRAGILE_RACE = 1, 2, 4, 8
+FRAGILE_COUNTS = ["solid_tests", "solid_fragile_tests", "solid_fragile_crossings",
+                  "surface_tests", "surface_fragile_tests", "multi_writer_voxels",
+                  "material_race_voxels", "flagged_voxels"]
+
+
+def fragile(mesh, grid: Grid, kind: int, use_materials: bool = True,
+            vox_split=(1024, 512, 512), mat_split=(512, 512, 512), eps64=1e-12, eps32=1e-5):
+    """(flags per node, counts) of orc_fragile: nodes whose value rests on a
+    test within eps of its threshold (where the original build's FMA
+    contraction could decide otherwise) or on the racy material store."""
+    v = np.ascontiguousarray(mesh.vertices, np.float32)
+    t = np.ascontiguousarray(mesh.indices, np.uint32)
+    dims = node_dims(grid, kind)
+    flags = np.zeros(dims[0] * dims[1] * dims[2], np.uint8)
+    mats = None
+    if use_materials and mesh.materials is not None:
+        mats = np.ascontiguousarray(mesh.materials, np.uint8)
+    vs = np.array(vox_split, np.uint32)
+    ms = np.array(mat_split, np.uint32)
+    counts = np.zeros(8, np.uint64)
+    r = lib().orc_fragile(_p(v), v.shape[0], _p(t), t.shape[0],
+                          _p(mats) if mats is not None else None, C.byref(grid), kind, _p(vs),
+                          _p(ms), float(eps64), float(eps32), _p(flags), _p(counts))
+    if r:
+        raise ValueError("orc_fragile failed")
+    return flags, dict(zip(FRAGILE_COUNTS, (int(c) for c in counts)))

@@ -122,15 +122,21 @@ def test_training_fits_recorded_costs(gpu_ctx):
     assert f.shape[0] == c.shape[0] > 200
     assert np.all(np.isfinite(f)) and np.all(c > 0)
     assert np.all(f[:, 15] == 1.0)
-    mse = gpu_ctx.cost_train(f, c, epochs=40, batch=128, lr=3e-3, seed=1)
-    # Standardized targets have variance 1; the model must explain a large
-    # share of it.  The targets are measured per-brick durations of two small
-    # rooms (a few hundred bricks of a few microseconds each), so timing
-    # noise alone is a sizeable part of the variance: fits land around 0.3-0.5.
-    assert mse < 0.6, mse
+    # The fit to measured durations is reported, not asserted: timing noise
+    # is a sizeable share of their variance (fits land around 0.3-0.5).
+    mse_measured = gpu_ctx.cost_train(f, c, epochs=40, batch=128, lr=3e-3, seed=1)
+    print(f"fit to measured brick durations: mse {mse_measured:.3f} (standardized)")
+    assert np.isfinite(mse_measured)
+    # Fitting ability on deterministic targets: durations that are a fixed
+    # smooth function of the same features.  Standardized targets have
+    # variance 1; the model must explain nearly all of it.
+    x = f.astype(np.float64)
+    t = np.exp(0.8 * x[:, 0] + 0.5 * np.tanh(x[:, 2]) + 0.3 * np.sqrt(np.abs(x[:, 5])) + 5.0)
+    mse = gpu_ctx.cost_train(f, t.astype(np.float32), epochs=40, batch=128, lr=3e-3, seed=1)
+    assert mse < 0.1, mse
     # deterministic for a given seed
     p1 = gpu_ctx.cost_params()
-    mse2 = gpu_ctx.cost_train(f, c, epochs=40, batch=128, lr=3e-3, seed=1)
+    mse2 = gpu_ctx.cost_train(f, t.astype(np.float32), epochs=40, batch=128, lr=3e-3, seed=1)
     assert mse2 == mse and np.array_equal(p1, gpu_ctx.cost_params())
 
 
@@ -154,6 +160,46 @@ def test_learned_schedule_output_identical(gpu_ctx, oracle, kind):
     used = gpu_ctx.stage_times()["learned_schedule"]
     gpu_ctx.enable_timing(False)
     gpu_ctx.set_schedule(va.SCHEDULE_PLAIN)
+    assert used == 1
+    assert np.array_equal(plain, learned)
+    og = oracle.plan_grid(m.vertices, va.PLAN_CUBE_LENGTH, 0.015)
+    assert np.array_equal(learned, oracle.voxelize_nodes(m, og, kind, True))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,brick", [(va.NODE_LONG, (128, 4, 8)), (va.NODE_SHORT, (128, 8, 4)),
+                                        (va.NODE_SHORT_FCC, None)])
+def test_learned_sub_bricks_output_identical(gpu_ctx, oracle, kind, brick):
+    """Every listed brick scored (screen ratio 0) and predicted far above
+    the threshold, so each is cut into min(4, depth) sub-bricks along z, each
+    taken by its own workgroup: the nodes equal the plain schedule's and the
+    oracle's."""
+    gpu_ctx.cost_record(True)
+    record_room(gpu_ctx, 23)
+    f, c = gpu_ctx.cost_samples()
+    gpu_ctx.cost_record(False)
+    gpu_ctx.cost_train(f, c, epochs=10, batch=128, lr=3e-3, seed=2)
+    m = W.shoebox_room(seed=25, n_boxes=16, subdiv=3)
+    gpu_ctx.set_mesh(m.vertices, m.indices)
+    gpu_ctx.set_materials(m.materials, 9)
+    g = gpu_ctx.plan_grid(va.PLAN_CUBE_LENGTH, 0.015)
+    if brick:
+        gpu_ctx.set_brick_dims(brick)
+    try:
+        gpu_ctx.set_schedule(va.SCHEDULE_PLAIN)
+        plain = gpu_ctx.voxelize_nodes(g, kind)
+        gpu_ctx.set_schedule(va.SCHEDULE_LEARNED)
+        gpu_ctx.set_heavy_threshold(-1e6)
+        gpu_ctx.set_screen_ratio(0.0)
+        gpu_ctx.enable_timing(True)
+        learned = gpu_ctx.voxelize_nodes(g, kind)
+        used = gpu_ctx.stage_times()["learned_schedule"]
+    finally:
+        gpu_ctx.enable_timing(False)
+        gpu_ctx.set_schedule(va.SCHEDULE_PLAIN)
+        gpu_ctx.set_heavy_threshold(1.5)
+        gpu_ctx.set_screen_ratio(24.0)
+        gpu_ctx.set_brick_dims((128, 4, 8))
     assert used == 1
     assert np.array_equal(plain, learned)
     og = oracle.plan_grid(m.vertices, va.PLAN_CUBE_LENGTH, 0.015)

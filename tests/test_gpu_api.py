@@ -132,3 +132,30 @@ def test_autotune_bricks(gpu_ctx, oracle):
     finally:
         gpu_ctx.set_brick_dims(default, False)
         buf.free()
+
+
+def test_candidate_count_past_2_pow_32_is_rejected(gpu_ctx):
+    """600 triangles that each span the whole y/z extent of a 32 x 8192 x
+    16382 bit grid give 600 * 2048 * 4095 (about 5.0e9) (triangle, tile)
+    candidates, more than the 32-bit offsets hold: the scan flags the wrap and
+    the call fails before any pair kernel reads the offsets."""
+    d = 1.0
+    ny, nz = 8190, 16380  # ext resolution in y and z
+    y1, z1 = (ny - 1) * d, (nz - 1) * d
+    n = 600
+    xs = np.linspace(1.0, 20.0, n)
+    v = np.zeros((3 * n, 3), np.float32)
+    v[:, 0] = np.repeat(xs, 3)
+    v[0::3, 1:] = (0.0, 0.0)
+    v[1::3, 1:] = (y1, 0.0)
+    v[2::3, 1:] = (y1, z1)
+    t = np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+    gpu_ctx.set_mesh(v, t)
+    g = gpu_ctx.plan_grid(va.PLAN_CUBE_LENGTH, d)
+    assert tuple(g.alloc) == (32, ny + 2, nz + 2)
+    tiles = ((ny + 3) // 4) * ((nz + 3) // 4)
+    assert n * tiles >= 2 ** 32
+    with pytest.raises(va.VoxError) as e:
+        gpu_ctx.solid_bits(g)
+    assert e.value.status == va.ERR_UNSUPPORTED
+    assert "2^32" in str(e.value)

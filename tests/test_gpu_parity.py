@@ -190,3 +190,17 @@ def test_hall_full_size(gpu_ctx, oracle):
     assert not n[0].any() and not n[-1].any() and not n[:, 0].any() and not n[:, -1].any()
     assert not n[:, :, 0].any()
     assert (n & 31).max() <= 27
+
+
+@pytest.mark.parametrize("d,vs", [(0.5, (1024, 5, 7)), (0.25, (1024, 3, 2)),
+                                  (0.25, (1024, 512, 512)), (0.25, (1024, 9, 1))])
+def test_disagreeing_slivers_across_seams(gpu_ctx, oracle, d, vs):
+    """Triangles whose solid-filter orientation (tN) and fill orientation
+    (n.x) have different signs, lying across many y/z subspace seams: the
+    per-subspace queue of the reference decides which tile rows count."""
+    m = W.disagreeing_slivers()
+    check_nodes(gpu_ctx, oracle, m, va.PLAN_CUBE_LENGTH, d, va.NODE_SHORT, True, vs)
+    g = gpu_ctx.plan_grid(va.PLAN_CUBE_LENGTH, d)
+    og = oracle.plan_grid(m.vertices, va.PLAN_CUBE_LENGTH, d)
+    got = gpu_ctx.solid_bits(g, split_yz=vs[1:])
+    assert np.array_equal(got, oracle.solid_bits(m, og, split_yz=vs[1:]))

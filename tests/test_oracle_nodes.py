@@ -175,3 +175,79 @@ def test_short_node_packing(oracle):
     m.materials = np.full(m.nt, 13, np.uint8)
     short = oracle.voxelize_nodes(m, g, oracle.NODE_SHORT, True)
     assert set(np.unique(short >> 5)) <= {0, 13 & 7}
+
+
+def ray_parity_at(mesh, pts, eps=1e-9):
+    """(solid, certain) of arbitrary points by +x ray parity in float64: a
+    point is inside when an odd number of triangles cross the ray from -inf
+    to it; points whose y/z lies within eps of a triangle edge, or whose x
+    lies within eps of a crossing, are uncertain."""
+    V = mesh.vertices.astype(np.float64)
+    P = np.asarray(pts, np.float64)
+    count = np.zeros(len(P), np.int64)
+    unsure = np.zeros(len(P), bool)
+    scale = eps * (1.0 + np.abs(V).max()) ** 2
+    for tri in mesh.indices:
+        a, b, c = V[tri[0]], V[tri[1]], V[tri[2]]
+        n = np.cross(b - a, c - a)
+        lo = np.minimum(np.minimum(a, b), c) - 1e-6
+        hi = np.maximum(np.maximum(a, b), c) + 1e-6
+        near = (P[:, 1] >= lo[1]) & (P[:, 1] <= hi[1]) & (P[:, 2] >= lo[2]) & (P[:, 2] <= hi[2])
+        if not near.any() or abs(n[0]) < 1e-300:
+            continue
+        Q = P[near]
+        e = np.stack([(q[1] - p[1]) * (Q[:, 2] - p[2]) - (q[2] - p[2]) * (Q[:, 1] - p[1])
+                      for p, q in ((a, b), (b, c), (c, a))])
+        inside = np.all(e > scale, 0) | np.all(e < -scale, 0)
+        edge = np.any(np.abs(e) <= scale, 0) & ~(np.any(e > scale, 0) & np.any(e < -scale, 0))
+        xi = a[0] - (n[1] * (Q[:, 1] - a[1]) + n[2] * (Q[:, 2] - a[2])) / n[0]
+        idx = np.nonzero(near)[0]
+        count[idx] += inside & (xi < Q[:, 0] - eps)
+        unsure[idx] |= edge | (inside & (np.abs(xi - Q[:, 0]) <= eps))
+    return count % 2 == 1, ~unsure
+
+
+@pytest.mark.parametrize("kind_name,a", [("LONG_FCC", 0.043), ("SHORT_FCC", 0.037)])
+def test_fcc_nodes_sit_on_the_documented_lattice(oracle, kind_name, a):
+    """Node (fx, fy, fz) of the FCC array lies at the documented lattice point
+    (include/node_types.h:428-441) m * a/2 with y doubled and shifted by
+    ((m_x + m_z) mod 2) * a/2, taken from the grid's min corner less one
+    padding cell: p = min + (fx a/2 - a, (fy - 1) a + ((fx + fz) mod 2) a/2,
+    fz a/2 - a).  Its solidity is checked against float64 ray parity at p and
+    its mask's population against the solid lattice neighbours at
+    p + (two of +-a/2): this pins the placement of the four shifted
+    sub-grids, so e.g. swapping sub-grids 3 and 4 fails."""
+    kind = getattr(oracle, "NODE_" + kind_name)
+    m = W.icosphere(seed=1, subdiv=3)
+    g = oracle.plan_grid(m.vertices, 1, a)
+    FX, FY, FZ = oracle.node_dims(g, kind)
+    raw = oracle.voxelize_nodes(m, g, kind, False)
+    if kind == oracle.NODE_LONG_FCC:
+        bid = raw.reshape(-1, 4)[:, :2].copy().view(np.uint16).ravel()
+    else:
+        bid = raw.view(np.uint16) & 4095
+    fz, fy, fx = np.meshgrid(np.arange(FZ), np.arange(FY), np.arange(FX), indexing="ij")
+    fx, fy, fz = fx.ravel(), fy.ravel(), fz.ravel()
+    mn = np.array(g.min_vertex)
+    d = g.voxel_length
+    pos = np.stack([mn[0] + fx * (d / 2) - d,
+                    mn[1] + (fy - 1) * d + ((fx + fz) % 2) * (d / 2),
+                    mn[2] + fz * (d / 2) - d], 1)
+    solid, cert = ray_parity_at(m, pos)
+    h = d / 2
+    offs = [(sx * h, sy * h, 0) for sx in (-1, 1) for sy in (-1, 1)] + \
+           [(sx * h, 0, sz * h) for sx in (-1, 1) for sz in (-1, 1)] + \
+           [(0, sy * h, sz * h) for sy in (-1, 1) for sz in (-1, 1)]
+    nsolid = np.zeros(len(pos), np.int64)
+    ncert = np.ones(len(pos), bool)
+    for o in offs:
+        s2, c2 = ray_parity_at(m, pos + np.array(o))
+        nsolid += s2
+        ncert &= c2
+    pop = np.array([bin(int(b)).count("1") for b in bid])
+    sure = cert & ncert
+    assert sure.mean() > 0.98
+    assert np.all(bid[sure & ~solid] == 0)
+    assert np.array_equal(pop[sure & solid], nsolid[sure & solid])
+    assert (sure & solid & (nsolid == 12)).sum() > 1000  # interior nodes: mask 4095
+    assert (sure & solid & (nsolid < 12)).sum() > 200    # surface nodes

@@ -191,3 +191,83 @@ def test_sphere_volume_within_shell(oracle):
     r = 0.4 / 0.01
     shell = 4 * np.pi * r * r * 1.8  # surface voxels: a generous shell
     assert abs(n - 4 / 3 * np.pi * r ** 3) < shell
+
+
+def test_sliver_mesh_orientations_disagree():
+    """The sliver mesh of the seam tests holds 64 triangles whose tN and n.x
+    (evaluated in float64 in the reference's two expressions, no FMA) have
+    different signs, both ways round."""
+    m = W.disagreeing_slivers()
+    tN, nx = W.normal_x_signs(m.vertices.astype(np.float64)[m.indices])
+    dis = (tN != 0) & ((tN > 0) != (nx > 0))
+    assert dis.sum() == 64
+    assert ((tN > 0) & dis).any() and ((tN < 0) & dis).any()
+    # tN and n.x are the same exact polynomial of the coordinates:
+    # sums of exact products evaluated in rationals are zero for these.
+    from fractions import Fraction
+    for tri in m.vertices[m.indices][dis][:8]:
+        y = [Fraction(float(c)) for c in tri[:, 1]]
+        z = [Fraction(float(c)) for c in tri[:, 2]]
+        exact = (y[0] * z[1] + y[2] * z[0] + z[2] * y[1]) - (y[2] * z[1] + z[0] * y[1] + z[2] * y[0])
+        assert exact == 0
+
+
+def test_sliver_mesh_depends_on_subspaces(oracle):
+    """On the sliver mesh the reference's per-subspace queues give a solid
+    grid different from one queue over the whole y/z space: the filter
+    (oriented by tN) queues a tile for a subspace only through that
+    subspace's centres, and the fill (oriented by n.x) then writes crossings
+    the whole-space queue would also have allowed elsewhere in the tile."""
+    m = W.disagreeing_slivers()
+    g = oracle.plan_grid(m.vertices, 1, 0.25)
+    whole = oracle.solid_bits(m, g, split_yz=(4096, 4096))
+    for sp in [(3, 2), (5, 7)]:
+        part = oracle.solid_bits(m, g, split_yz=sp)
+        assert np.count_nonzero(whole != part) > 0, sp
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_tile_filter_counts_tiles_with_a_centre_in_the_triangle(oracle, seed):
+    """The count pass's filter (src/voxelizer.cu:840-1033, orc_tile_overlaps)
+    queues a 4x4 tile of voxel centres when a centre of the tile inside the
+    triangle's y/z bounding-box range lies in or on the triangle's y/z
+    projection.  Checked here by geometry in float64: tiles with a centre
+    inside by a margin must be counted, tiles whose centres are all outside
+    by a margin must not; the two bounds meet for all but tie cases."""
+    rng = np.random.default_rng(50 + seed)
+    d = 0.05
+    ext_y, ext_z = 40, 36
+    em = np.array([0.013, -0.021, 0.034])
+    exact = 0
+    for _ in range(150):
+        c = em + np.array([0.5, ext_y * d / 2, ext_z * d / 2])
+        tri = (c + rng.normal(0, 0.4, (3, 3)) * np.array([1, ext_y * d / 3, ext_z * d / 3])
+               ).astype(np.float32)
+        got = oracle.tile_overlaps(tri.ravel(), em, d, ext_y, ext_z)
+        t = tri.astype(np.float64)
+        y, z = t[:, 1], t[:, 2]
+        area2 = (y[1] - y[0]) * (z[2] - z[0]) - (z[1] - z[0]) * (y[2] - y[0])
+        if area2 == 0:
+            assert got == 0
+            continue
+        ylo = max(int(np.ceil((y.min() - em[1]) / d)), 0)
+        yhi = min(int(np.floor((y.max() - em[1]) / d)) + 1, ext_y)
+        zlo = max(int(np.ceil((z.min() - em[2]) / d)), 0)
+        zhi = min(int(np.floor((z.max() - em[2]) / d)) + 1, ext_z)
+        sure, maybe = set(), set()
+        for i in range(ylo, yhi):
+            for j in range(zlo, zhi):
+                py, pz = i * d + em[1], j * d + em[2]
+                e = []
+                for a, b in ((0, 1), (1, 2), (2, 0)):
+                    e.append(((y[b] - y[a]) * (pz - z[a]) - (z[b] - z[a]) * (py - y[a])) *
+                             np.sign(area2))
+                e = np.array(e)
+                tol = 1e-9
+                if np.all(e > tol):
+                    sure.add((i // 4, j // 4))
+                if np.all(e > -tol):
+                    maybe.add((i // 4, j // 4))
+        assert len(sure) <= got <= len(maybe)
+        exact += len(sure) == len(maybe)
+    assert exact > 130

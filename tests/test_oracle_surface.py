@@ -145,3 +145,50 @@ def test_mat_split_changes_only_3d_shortcut(oracle):
     # skips the remaining plane tests), never drop one
     assert not np.any((k0 != 0) & (k1 == 0))
     assert diff.sum() < 0.05 * (k0 != 0).sum()
+
+
+def test_mat_split_clip_writes_one_voxel_without_plane_tests(oracle):
+    """A hand-checked 3D triangle (dominant axis x) on a 1 m grid from the
+    origin (ext resolution 32 x 15 x 15).  Worked out in float32 below,
+    following the reference's 3D kernel (src/voxelizer.cu:3604-3760): the
+    triangle's voxel range in x is [5, 6], column (y, z) = (7, 6) passes the
+    YZ test and its depth range covers x = 5 and 6, and voxel x = 6 fails the
+    ZX/XY tests, so one subspace writes only x = 5.  With matSplitRes.x = 6
+    the x subspaces of the 32 ext columns are [0,6) [6,12) [12,17) [17,22)
+    [22,27) [27,32) (splitResolutionByNrOfParts, src/host_side.cpp:3003-3064):
+    in [6,12) the range is clipped to the single voxel 6, which the reference
+    writes without the remaining tests (3722-3740), so x = 6 appears."""
+    tri = np.array([[6.122635364532471, 7.888801097869873, 6.838565349578857],
+                    [4.590999603271484, 6.417872428894043, 6.335206985473633],
+                    [5.817395210266113, 7.076710224151611, 8.09256649017334]], np.float32)
+    frame = np.array([[0, 0, 0], [29, 14, 14]], np.float32)
+    v = np.concatenate([tri, frame])
+    m = W.Mesh(v, np.array([[0, 1, 2]], np.uint32), np.array([1], np.uint8), "clip")
+    g = oracle.plan_grid(v, 1, 1.0)
+    assert list(g.res) == [32, 15, 15]
+    f = np.float32
+    half, one = f(0.5), f(1.0)
+    # hand computation in float32 (m = 0, d = 1)
+    t0, t1, t2 = tri
+    n = np.cross(t0 - t2, t1 - t0).astype(np.float32)
+    assert abs(n[0]) > abs(n[1]) and abs(n[0]) > abs(n[2])  # dominant axis x
+    lo_x = int(np.ceil(tri[:, 0].min() - half))
+    hi_x = int(np.floor(tri[:, 0].max() + half))
+    assert (lo_x, hi_x) == (5, 6)
+    j, k = 7, 6
+    p = np.array([0, f(j) - half, f(k) - half], np.float32)
+    pmin = p + np.array([0, one if n[1] > 0 else 0, one if n[2] > 0 else 0], np.float32)
+    pmax = p + np.array([0, 0 if n[1] > 0 else one, 0 if n[2] > 0 else one], np.float32)
+    dot = lambda a, b: f(f(f(a[0] * b[0]) + f(a[1] * b[1])) + f(a[2] * b[2]))
+    a = f(dot(t0 - pmin, n) / n[0])
+    b = f(dot(t0 - pmax, n) / n[0])
+    a, b = min(a, b), max(a, b)
+    lo, hi = int(np.ceil(a - half)), int(np.floor(b + half))
+    assert lo <= 5 and hi >= 6
+    k0 = oracle.surface_keys(m, g, mat_split=(4096, 4096, 4096)).reshape(17, 17, 32)
+    k1 = oracle.surface_keys(m, g, mat_split=(6, 4096, 4096)).reshape(17, 17, 32)
+    # alloc index = ext + 1
+    assert k0[k + 1, j + 1, 6] != 0 and k0[k + 1, j + 1, 7] == 0
+    assert k1[k + 1, j + 1, 7] == (3 << 30)  # the 3D triangle 0, by the shortcut
+    diff = np.argwhere(k0 != k1)
+    assert diff.tolist() == [[k + 1, j + 1, 7]]

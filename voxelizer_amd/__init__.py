@@ -79,7 +79,8 @@ EXPORTED = [
     "vox_set_mesh", "vox_set_mesh_device", "vox_set_materials", "vox_set_materials_device",
     "vox_plan_grid", "vox_plan_grid_host", "vox_solid_bits", "vox_voxelize_nodes", "vox_node_dims",
     "vox_node_count", "vox_node_size", "vox_enable_timing", "vox_get_stage_times",
-    "vox_set_schedule", "vox_set_heavy_threshold", "vox_cost_set_params", "vox_cost_get_params", "vox_cost_record",
+    "vox_set_schedule", "vox_set_heavy_threshold", "vox_set_screen_ratio", "vox_cost_set_params",
+    "vox_cost_get_params", "vox_cost_record",
     "vox_cost_samples", "vox_cost_train", "vox_mlp_forward", "vox_mlp_grad",
     "vox_voxelize_batch", "vox_set_brick_dims", "vox_get_brick_dims", "vox_autotune_bricks",
     "vox_graph_create", "vox_graph_launch", "vox_graph_fallbacks", "vox_graph_destroy",
@@ -130,6 +131,7 @@ def lib():
     F = C.c_float
     L.vox_set_schedule.argtypes = [P, I32]
     L.vox_set_heavy_threshold.argtypes = [P, F]
+    L.vox_set_screen_ratio.argtypes = [P, F]
     L.vox_cost_set_params.argtypes = [P, P]
     L.vox_cost_get_params.argtypes = [P, P]
     L.vox_cost_record.argtypes = [P, I32]
@@ -178,8 +180,9 @@ class Graph:
         self.ctx, self.h, self.items = ctx, h, items
 
     def launch(self):
-        if lib().vox_graph_launch(self.h) != 0:
-            raise VoxError("vox_graph_launch failed")
+        st = lib().vox_graph_launch(self.h)
+        if st != OK:
+            raise VoxError(st, "vox_graph_launch failed")
 
     @property
     def fallbacks(self) -> int:
@@ -377,6 +380,11 @@ class Context:
 
     def set_heavy_threshold(self, threshold: float):
         self._check(lib().vox_set_heavy_threshold(self.h, float(threshold)))
+
+    def set_screen_ratio(self, ratio: float):
+        """Bricks with at least `ratio` times the mean pair count are scored
+        by the cost model (learned schedule)."""
+        self._check(lib().vox_set_screen_ratio(self.h, float(ratio)))
 
     def cost_record(self, on: bool = True):
         self._check(lib().vox_cost_record(self.h, 1 if on else 0))

@@ -205,14 +205,15 @@ vox_status fill(vox_ctx *ctx, uint32_t *p, size_t n, uint32_t v) {
     return VOX_OK;
 }
 
-// Exclusive scan of in[0,n) into out[0,n]; out[n] is the total.
+// Exclusive scan of in[0,n) into out[0,n+1]: out[n] is the total mod 2^32,
+// out[n+1] is 1 if the true total reached 2^32 (see k_scan_down).
 vox_status scan(vox_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t n, int depth = 0) {
     const uint32_t nblocks = n / kScanTile + 1;
     char name[32];
     const char *tag = ctx->stream == ctx->side ? "b" : "";  // one set per stream
     std::snprintf(name, sizeof(name), "scan%s_part%d", tag, depth);
     uint32_t *part;
-    VOX_TRY(buf(ctx, name, nblocks + 1, &part));
+    VOX_TRY(buf(ctx, name, 2 * (size_t)nblocks + 1, &part));
     if (nblocks <= kScanDirectTiles) {
         if (nblocks > 1) {
             k_scan_reduce<<<nblocks, kScanThreads, 0, ctx->stream>>>(in, n, part);
@@ -227,9 +228,9 @@ vox_status scan(vox_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t n, int
     if (nblocks > 1) {
         std::snprintf(name, sizeof(name), "scan%s_psum%d", tag, depth);
         uint32_t *psum;
-        VOX_TRY(buf(ctx, name, nblocks + 1, &psum));
+        VOX_TRY(buf(ctx, name, nblocks + 2, &psum));
         VOX_TRY(scan(ctx, part, psum, nblocks, depth + 1));
-        k_scan_add<<<nblocks, kScanThreads, 0, ctx->stream>>>(out, n, psum);
+        k_scan_add<<<nblocks, kScanThreads, 0, ctx->stream>>>(out, n, part, psum);
         VOX_HIP(hipGetLastError());
     }
     return VOX_OK;
@@ -256,15 +257,16 @@ BrickDims brick_dims(const vox_grid &g, const int bdim[3]) {
 }
 
 // A brick shape the surface and node passes support: powers of two, x a
-// multiple of 64, at most 32 rows (y*z), 256 to kMaxBrickVox voxels (the
-// brick pass finishes 256 voxels per wave step), and (for the Cartesian node
-// pass, whose threads own kNodeZ planes) z a multiple of kNodeZ.
+// multiple of 64, at most 32 rows (y*z), at most 32 words per z plane (one
+// mask word per brick layer), 256 to kMaxBrickVox voxels (the brick pass
+// finishes 256 voxels per wave step), and (for the Cartesian node pass, whose
+// threads own kNodeZ planes) z a multiple of kNodeZ.
 bool brick_ok(const uint32_t b[3], bool cart) {
     for (int a = 0; a < 3; a++)
         if (b[a] == 0 || (b[a] & (b[a] - 1)) != 0) return false;
     const uint32_t nvox = b[0] * b[1] * b[2];
-    return b[0] >= 64 && b[1] * b[2] <= 32 && nvox >= 256 && nvox <= (uint32_t)kMaxBrickVox &&
-           (!cart || b[2] % kNodeZ == 0);
+    return b[0] >= 64 && b[1] * b[2] <= 32 && (b[0] / 32) * b[1] <= 32 && nvox >= 256 &&
+           nvox <= (uint32_t)kMaxBrickVox && (!cart || b[2] % kNodeZ == 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -278,9 +280,9 @@ struct Plan {
     std::vector<SplitTable> splits;  // one per job
     std::vector<uint32_t> tri_b{0}, tile_b{0}, pad_b{0}, brick_b{0}, erode_b{0}, node_b{0};
     std::vector<uint32_t> fcc_first, fcc_b{0};
-    std::vector<size_t> bits_off, surf_off, mat_off;  // into the call's arenas
-    std::vector<uint32_t *> ext_bits;                 // caller's bit array or null
-    size_t bits_words = 0, surf_words = 0, mat_bytes = 0;
+    std::vector<size_t> bits_off, smask_off, mat_off;  // into the call's arenas
+    std::vector<uint32_t *> ext_bits;                  // caller's bit array or null
+    size_t bits_words = 0, smask_words = 0, mat_bytes = 0;
     uint32_t maxW = 0;
     bool any_mats = false;
 };
@@ -293,6 +295,8 @@ struct PlanDev {
     // grid and the flags of the rows / planes where a later subspace starts.
     uint32_t *pre = nullptr;
     uint8_t *ystart = nullptr, *zstart = nullptr;
+    // The solid stage runs per voxSplitRes y/z subspace (Job::sstart set).
+    bool solid_split = false;
 };
 
 // matSplitRes subspaces over extPartition (src/host_side.cpp:1107-1111).
@@ -372,11 +376,14 @@ vox_status add_job(vox_ctx *ctx, Plan &P, const float *V, const uint32_t *I, uin
     P.ext_bits.push_back(ext_bits);
     P.bits_off.push_back(P.bits_words);
     if (!ext_bits) P.bits_words += words;
-    P.surf_off.push_back(P.surf_words);
+    P.smask_off.push_back(P.smask_words);
     P.mat_off.push_back(P.mat_bytes);
     if (use_mats) {
-        P.surf_words += words;
-        P.mat_bytes += (size_t)nbt * (size_t)(bdim[0] * bdim[1] * bdim[2]);  // 1 B per voxel
+        // One mask word per brick layer (offsets kept 16-byte aligned for the
+        // node pass's 4-layer loads); the material bytes of a brick's
+        // surface words are packed into a block of one byte per brick voxel.
+        P.smask_words += ((size_t)nbt * (size_t)bdim[2] + 3u) & ~(size_t)3u;
+        P.mat_bytes += (size_t)nbt * (size_t)(bdim[0] * bdim[1] * bdim[2]);
         P.any_mats = true;
     }
     P.maxW = std::max(P.maxW, W);
@@ -431,11 +438,11 @@ vox_status upload(vox_ctx *ctx, void *dst, const void *src, size_t bytes) {
 // tables, base arrays, erosion subspace flags).
 vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D, uint32_t splitY, uint32_t splitZ) {
     const uint32_t nj = (uint32_t)P.jobs.size(), nf = (uint32_t)P.fcc_first.size();
-    uint32_t *bits = nullptr, *surf = nullptr, *bhist = nullptr;
+    uint32_t *bits = nullptr, *smask = nullptr, *bhist = nullptr;
     uint8_t *matgrid = nullptr;
     if (P.bits_words) VOX_TRY(buf(ctx, "a_bits", P.bits_words, &bits));
     if (P.any_mats) {
-        VOX_TRY(buf(ctx, "a_surf", P.surf_words, &surf));
+        VOX_TRY(buf(ctx, "a_smask", P.smask_words, &smask));
         VOX_TRY(buf(ctx, "a_matgrid", P.mat_bytes, &matgrid));
         VOX_TRY(buf(ctx, "f_brick_hist", P.brick_b.back(), &bhist));
     }
@@ -461,6 +468,24 @@ vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D, uint32_t splitY, uint3
             for (size_t k = 1; k < ezs.size(); k++) starts[J.gd.Y + ezs[k].min] = 1;
         }
     }
+    // A single mesh (one Cartesian job, or the four sub-grids of one FCC
+    // lattice, which share the ext resolution) whose voxSplitRes y/z gives
+    // several solid subspaces: flags of the ext rows and planes where a later
+    // subspace starts (Job::sstart).
+    const size_t erode_flags = starts.size();
+    const bool single_mesh = nj == 1 || (nj == 4 && nf == 1);
+    if (single_mesh && nj > 0) {
+        const Job &J = P.jobs[0];
+        std::vector<Span> sys = split_axis_max((uint32_t)J.res[1], splitY);
+        std::vector<Span> szs = split_axis_max((uint32_t)J.res[2], splitZ);
+        if (sys.size() > 1 || szs.size() > 1) {
+            const size_t o = starts.size();
+            starts.resize(o + (size_t)J.res[1] + (size_t)J.res[2], 0);
+            for (size_t k = 1; k < sys.size(); k++) starts[o + sys[k].min] = 1;
+            for (size_t k = 1; k < szs.size(); k++) starts[o + (size_t)J.res[1] + szs[k].min] = 1;
+        }
+    }
+    const bool solid_split = starts.size() > erode_flags;
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t jb = al(sizeof(Job) * nj), sb = al(sizeof(SplitTable) * nj);
     const size_t bb = al(bases.size() * 4);
@@ -471,8 +496,9 @@ vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D, uint32_t splitY, uint3
         Job &J = P.jobs[k];
         J.bits = P.ext_bits[k] ? P.ext_bits[k] : bits + P.bits_off[k];
         J.splits = dsplit + k;
+        J.sstart = solid_split ? dev + jb + sb + bb + erode_flags : nullptr;
         if (J.use_mats) {
-            J.surf = surf + P.surf_off[k];
+            J.smask = smask + P.smask_off[k];
             J.bhist = bhist + J.brick_base;
             J.matgrid = matgrid + P.mat_off[k];
         }
@@ -494,10 +520,11 @@ vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D, uint32_t splitY, uint3
     D.node_b = (b += nj + 1);
     D.fcc_first = (b += nj + 1);
     D.fcc_b = b + nf;
-    if (!starts.empty()) {
+    if (erode_flags) {
         D.ystart = dev + jb + sb + bb;
         D.zstart = D.ystart + P.jobs[0].gd.Y;
     }
+    D.solid_split = solid_split;
     return VOX_OK;
 }
 
@@ -507,8 +534,9 @@ void record(vox_ctx *ctx, int i) {
 
 // ---------------------------------------------------------------------------
 // Solid stage over all jobs: per-triangle records, candidate tiles, counting
-// sort by tile, fill.  The solid result does not depend on voxSplitRes y/z
-// (the subspaces partition the centres), so every grid is filled whole.
+// sort by tile, fill.  Every grid is filled in one pass; with several
+// voxSplitRes y/z subspaces each (triangle, tile) pair carries the tile rows
+// whose subspace queues it, as the reference's per-subspace queues decide.
 // ---------------------------------------------------------------------------
 // Triangle records (vertices, class word, material) and slot jobs, read by
 // both the solid and the surface stage; with fixed sizes the overflow flag
@@ -535,10 +563,10 @@ vox_status solid_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
     uint2 *pairs;
     uint4 *cbox;
     VOX_TRY(buf(ctx, "s_cand_count", nslots, &cand_count));
-    VOX_TRY(buf(ctx, "s_cand_off", nslots + 1, &cand_off));
+    VOX_TRY(buf(ctx, "s_cand_off", nslots + 2, &cand_off));
     VOX_TRY(buf(ctx, "s_cbox", nslots, &cbox));
     VOX_TRY(buf(ctx, "s_tile_hist", nTiles, &tile_hist));
-    VOX_TRY(buf(ctx, "s_tile_off", nTiles + 1, &tile_off));
+    VOX_TRY(buf(ctx, "s_tile_off", nTiles + 2, &tile_off));
     VOX_TRY(buf(ctx, "s_tile_cursor", nTiles, &tile_cursor));
     if (nslots > 0) {
         k_solid_setup<<<grid_for(nslots, 256), 256, 0, ctx->stream>>>(D.jobs, D.tri_b, nj, nslots,
@@ -550,13 +578,23 @@ vox_status solid_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
     if (ctx->fixed) {
         total = ctx->cap_solid;
     } else {
-        VOX_TRY(read_u32(ctx, cand_off + nslots, 1, &total));
+        uint32_t h[2];
+        VOX_TRY(read_u32(ctx, cand_off + nslots, 2, h));
+        if (h[1])
+            return fail(ctx, VOX_ERR_UNSUPPORTED,
+                        "2^32 or more (triangle, tile) candidates in one call");
+        total = h[0];
         ctx->last_solid = total;
     }
     uint32_t *ovf = ctx->fixed ? ctx->ovf : nullptr;
     uint32_t *first;
+    uint16_t *pmask = nullptr, *list_mask = nullptr;
     VOX_TRY(buf(ctx, "s_pairs", total, &pairs));
     VOX_TRY(buf(ctx, "s_list", total, &list));
+    if (D.solid_split) {
+        VOX_TRY(buf(ctx, "s_pmask", total, &pmask));
+        VOX_TRY(buf(ctx, "s_list_mask", total, &list_mask));
+    }
     VOX_TRY(buf(ctx, "s_first", block_starts_size(total), &first));
     if (total > 0 && nslots > 0) {
         k_block_starts<<<grid_for(nslots, 256), 256, 0, ctx->stream>>>(cand_off, nslots, total,
@@ -569,22 +607,22 @@ vox_status solid_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
         if (nj > 1)
             k_solid_candidates<true><<<grid_for(total, 256), 256, 0, ctx->stream>>>(
                 D.jobs, tri_job, nslots, ctx->rec, cand_off, cand_off + nslots, first, cbox,
-                pairs, tile_hist, total, ovf);
+                pairs, pmask, tile_hist, total, ovf);
         else
             k_solid_candidates<false><<<grid_for(total, 256), 256, 0, ctx->stream>>>(
                 D.jobs, tri_job, nslots, ctx->rec, cand_off, cand_off + nslots, first, cbox,
-                pairs, tile_hist, total, ovf);
+                pairs, pmask, tile_hist, total, ovf);
         VOX_HIP(hipGetLastError());
     }
     VOX_TRY(scan(ctx, tile_hist, tile_off, nTiles));
     if (total > 0) {
         k_solid_scatter<<<grid_for(total, 256), 256, 0, ctx->stream>>>(
-            pairs, cand_off + nslots, tile_off, tile_cursor, list, total);
+            pairs, pmask, cand_off + nslots, tile_off, tile_cursor, list, list_mask, total);
         VOX_HIP(hipGetLastError());
     }
     if (nTiles > 0) {
         k_solid_fill<<<nTiles, kWave, 16 * P.maxW * 4, ctx->stream>>>(
-            D.jobs, D.tile_b, nj, ctx->rec, tile_off, tile_hist, list);
+            D.jobs, D.tile_b, nj, ctx->rec, tile_off, tile_hist, list, list_mask);
         VOX_HIP(hipGetLastError());
     }
     if (P.pad_b.back() > 0) {
@@ -628,7 +666,7 @@ constexpr uint32_t kForkMinSlots = 16384;
 struct SurfPass {
     uint32_t *brick_off = nullptr, *brick_hist = nullptr, *blist = nullptr, *stats = nullptr;
     uint32_t *blist_order = nullptr, *next = nullptr;
-    const uint32_t *n_list = nullptr, *hlist = nullptr, *n_heavy = nullptr;
+    const uint32_t *n_list = nullptr, *hlist = nullptr, *hsub = nullptr, *n_heavy = nullptr;
     const uint8_t *skip = nullptr;
     float *feats = nullptr;
 };
@@ -653,9 +691,9 @@ vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass 
     if (features) VOX_TRY(buf(ctx, "f_tri_ext", nslots, &tri_ext));
     float4 *tri_rec = ctx->rec;
     VOX_TRY(buf(ctx, "f_cand", nslots, &cand));
-    VOX_TRY(buf(ctx, "f_cand_off", nslots + 1, &cand_off));
+    VOX_TRY(buf(ctx, "f_cand_off", nslots + 2, &cand_off));
     VOX_TRY(buf(ctx, "f_brick_hist", nbt, &brick_hist));
-    VOX_TRY(buf(ctx, "f_brick_off", nbt + 1, &brick_off));
+    VOX_TRY(buf(ctx, "f_brick_off", nbt + 2, &brick_off));
     VOX_TRY(buf(ctx, "f_brick_cursor", nbt, &brick_cursor));
     VOX_TRY(buf(ctx, "f_stats", 1, &stats));
     if (nslots > 0) {
@@ -669,7 +707,12 @@ vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass 
     if (ctx->fixed) {
         total = ctx->cap_surf;
     } else {
-        VOX_TRY(read_u32(ctx, cand_off + nslots, 1, &total));
+        uint32_t h[2];
+        VOX_TRY(read_u32(ctx, cand_off + nslots, 2, h));
+        if (h[1])
+            return fail(ctx, VOX_ERR_UNSUPPORTED,
+                        "2^32 or more (triangle, brick) candidates in one call");
+        total = h[0];
         ctx->last_surf = total;
     }
     uint32_t *ovf = ctx->fixed ? ctx->ovf : nullptr;
@@ -702,7 +745,7 @@ vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass 
     uint32_t *flags, *blist_order, *blist_order_off, *next;
     VOX_TRY(buf(ctx, "f_flags", nbt, &flags));
     VOX_TRY(buf(ctx, "f_order", nbt, &blist_order));
-    VOX_TRY(buf(ctx, "f_order_off", nbt + 1, &blist_order_off));
+    VOX_TRY(buf(ctx, "f_order_off", nbt + 2, &blist_order_off));
     VOX_TRY(buf(ctx, "f_next", 1, &next));
     k_nonzero_flags<<<grid_for(nbt, 256), 256, 0, ctx->stream>>>(brick_hist, flags, nbt);
     VOX_HIP(hipGetLastError());
@@ -716,7 +759,7 @@ vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass 
     // predicted by the MLP, the heavy ones on a list the brick pass takes
     // first (see vox_cost.hip).
     const uint32_t *n_list = blist_order_off + nbt;
-    const uint32_t *hlist = nullptr, *n_heavy = nullptr;
+    const uint32_t *hlist = nullptr, *hsubs = nullptr, *n_heavy = nullptr;
     const uint8_t *skip = nullptr;
     float *feats = nullptr;
     uint64_t *cycles = nullptr;
@@ -727,11 +770,12 @@ vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass 
             D.brick_b, nj, feats);
         VOX_HIP(hipGetLastError());
     } else if (learned) {
-        uint32_t *lcand, *cnt, *heavy;
+        uint32_t *lcand, *cnt, *heavy, *hsub;
         uint8_t *skipf;
         VOX_TRY(buf(ctx, "f_lcand", nbt, &lcand));
         VOX_TRY(buf(ctx, "f_cnt", 2, &cnt));
-        VOX_TRY(buf(ctx, "f_heavy", nbt, &heavy));
+        VOX_TRY(buf(ctx, "f_heavy", (size_t)kMaxSplitParts * nbt, &heavy));
+        VOX_TRY(buf(ctx, "f_hsub", (size_t)kMaxSplitParts * nbt, &hsub));
         VOX_TRY(buf(ctx, "f_skip", nbt, &skipf));
         VOX_TRY(fill(ctx, cnt, 2, 0u));
         const uint32_t g256 = grid_for(nbt, 256, 1024);
@@ -741,9 +785,10 @@ vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass 
         VOX_HIP(hipGetLastError());
         k_cost_predict<<<grid_for(nbt, 4, (uint32_t)ctx->n_cu * 8u), 256, 0, ctx->stream>>>(
             ctx->cost_params, blist_order, lcand, cnt, ctx->heavy_cost, brick_hist, brick_off,
-            blist, tri_info, tri_ext, D.jobs, D.brick_b, nj, heavy, skipf);
+            blist, tri_info, tri_ext, D.jobs, D.brick_b, nj, heavy, hsub, skipf);
         VOX_HIP(hipGetLastError());
         hlist = heavy;
+        hsubs = hsub;
         n_heavy = cnt + 1;
         skip = skipf;
     }
@@ -758,6 +803,7 @@ vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass 
     S.next = next;
     S.n_list = n_list;
     S.hlist = hlist;
+    S.hsub = hsubs;
     S.n_heavy = n_heavy;
     S.skip = skip;
     S.feats = feats;
@@ -774,8 +820,8 @@ vox_status surface_brick(vox_ctx *ctx, const Plan &P, const PlanDev &D, const Su
     k_surf_brick<<<blocks, 256, 0, ctx->stream>>>(D.jobs, D.brick_b, nj, ctx->rec, S.brick_off,
                                                   S.brick_hist, S.blist,
                                                   ctx->timing ? S.stats : nullptr, S.blist_order,
-                                                  S.n_list, S.hlist, S.n_heavy, S.skip, S.next,
-                                                  cycles);
+                                                  S.n_list, S.hlist, S.hsub, S.n_heavy, S.skip,
+                                                  S.next, cycles);
     VOX_HIP(hipGetLastError());
     if (ctx->timing) {
         uint32_t vox = 0;
@@ -1320,15 +1366,15 @@ vox_status vox_solid_bits(vox_ctx *ctx, const vox_grid *g, const double shift[3]
     const double zero[3] = {0.0, 0.0, 0.0};
     const uint32_t sy = split_yz ? split_yz[0] : 512u, sz = split_yz ? split_yz[1] : 512u;
     if (sy == 0 || sz == 0) return VOX_ERR_INVALID;
-    // The solid grid does not depend on the y/z subspaces (they partition
-    // the voxel centres), so one pass covers the grid.
+    // Solid subspaces come from split_yz (over the ext resolution, as
+    // performVoxelization's extYzSplits); the material split is unused here.
     const uint32_t ms[3] = {512u, 512u, 512u};
     Plan P;
     VOX_TRY(add_job(ctx, P, ctx->V, ctx->I, ctx->nt, nullptr, *g, shift ? shift : zero, false, ms,
                     ctx->cart_bdim, false, nullptr, 0, bits_dev));
     PlanDev D;
     record(ctx, 0);
-    VOX_TRY(plan_upload(ctx, P, D, UINT32_MAX, UINT32_MAX));
+    VOX_TRY(plan_upload(ctx, P, D, sy, sz));
     VOX_TRY(records_stage(ctx, P, D));
     VOX_TRY(solid_stage(ctx, P, D));
     record(ctx, 1);
@@ -1621,6 +1667,9 @@ vox_status graph_capture(vox_graph *g) {
     VOX_TRY(buf(w, "g_ovf", 1, &w->ovf));
     VOX_TRY(buf(w, "s_pairs", w->cap_solid, &pr));
     VOX_TRY(buf(w, "s_list", w->cap_solid, &u));
+    uint16_t *u16;
+    VOX_TRY(buf(w, "s_pmask", w->cap_solid, &u16));
+    VOX_TRY(buf(w, "s_list_mask", w->cap_solid, &u16));
     VOX_TRY(buf(w, "f_blist", w->cap_surf, &u));
     VOX_TRY(buf(w, "s_first", block_starts_size(w->cap_solid), &u));
     VOX_TRY(buf(w, "f_first", block_starts_size(w->cap_surf), &u));
@@ -1732,6 +1781,12 @@ vox_status vox_set_schedule(vox_ctx *ctx, vox_schedule schedule) {
 vox_status vox_set_heavy_threshold(vox_ctx *ctx, float threshold) {
     if (!ctx || threshold != threshold) return VOX_ERR_INVALID;
     ctx->heavy_cost = threshold;
+    return VOX_OK;
+}
+
+vox_status vox_set_screen_ratio(vox_ctx *ctx, float ratio) {
+    if (!ctx || !(ratio >= 0.0f)) return VOX_ERR_INVALID;
+    ctx->screen_ratio = ratio;
     return VOX_OK;
 }
 

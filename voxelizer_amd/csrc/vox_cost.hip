@@ -376,9 +376,14 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ P, const float
 // at least `ratio` times the mean (the plain counts screen the list); the MLP
 // predicts their cost, and those predicted heavy (standardized log cost above
 // `heavy`) go on a short list that the brick pass works through first, one
-// brick per grab.  The plain list follows in index order (neighbouring bricks
-// share triangles and solid rows in L2), skipping the bricks already taken.
-// cnt[0] counts candidates, cnt[1] heavy bricks; both start at zero.
+// entry per grab.  A brick predicted more than kSplitStep above `heavy` is cut
+// into 2 sub-bricks along z, more than 2 kSplitStep into 4 (at most its
+// depth): each entry carries its layer range (hsub: first | end << 16), so
+// the workgroups that take them share the brick's work (voxels of different
+// layers are disjoint, so the split does not change the winners).  The
+// plain list follows in index order (neighbouring bricks share triangles and
+// solid rows in L2), skipping the bricks already taken.  cnt[0] counts
+// candidates, cnt[1] heavy entries; both start at zero.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_cost_screen(const uint32_t *__restrict__ order,
                                                      const uint32_t *__restrict__ n_order,
@@ -427,7 +432,7 @@ __global__ __launch_bounds__(256) void k_cost_predict(
     const uint32_t *__restrict__ blist, const uint4 *__restrict__ tri_info,
     const uint4 *__restrict__ tri_ext, const Job *__restrict__ jobs,
     const uint32_t *__restrict__ brick_bases, uint32_t njobs, uint32_t *__restrict__ hlist,
-    uint8_t *__restrict__ skip) {
+    uint32_t *__restrict__ hsub, uint8_t *__restrict__ skip) {
     const uint32_t n = cnt[0];
     const uint32_t j = lane_id();
     const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
@@ -448,7 +453,14 @@ __global__ __launch_bounds__(256) void k_cost_predict(
         const float h2 = fmaxf(z, 0.0f);
         const float y = wave_sum(P[kOffW3 + j] * h2) + P[kOffB3];
         if (j == 0 && y > heavy) {  // false for NaN
-            hlist[atomicAdd(&cnt[1], 1u)] = b;
+            const uint32_t depth = (uint32_t)jobs[job_of(brick_bases, njobs, b)].bdim[2];
+            uint32_t parts = y > heavy + 2.0f * kSplitStep ? 4u : y > heavy + kSplitStep ? 2u : 1u;
+            parts = min(parts, depth);  // depths are powers of two
+            const uint32_t base = atomicAdd(&cnt[1], parts);
+            for (uint32_t k = 0; k < parts; k++) {
+                hlist[base + k] = b;
+                hsub[base + k] = (k * depth / parts) | (((k + 1u) * depth / parts) << 16);
+            }
             skip[q] = 1u;
         }
     }

@@ -133,6 +133,10 @@ struct Job {
     uint32_t tilesY, tilesZ;  // 4x4 tiles of voxel-centre rows
     uint32_t tile_base;    // first tile (one fill workgroup per tile)
     uint32_t pad_base;     // first block of the pad-row clear
+    // voxSplitRes y/z subspaces of the solid stage when there are several:
+    // sstart[y] (y < res[1]) and sstart[res[1] + z] (z < res[2]) are 1 where
+    // a subspace other than the first starts; null for one subspace.
+    const uint8_t *sstart;
     // surface stage
     F3 m;                  // float(origin)
     float fd;              // float(voxel length)
@@ -141,9 +145,9 @@ struct Job {
     uint32_t brick_base;   // first brick
     BrickDims bdn;         // brick geometry for the node passes
     const SplitTable *splits;  // matSplitRes intervals
-    uint32_t *surf;        // surface bit words
+    uint32_t *smask;       // per brick layer: mask of the words holding a surface voxel
     const uint32_t *bhist; // per brick: (triangle, brick) pairs; 0 = not visited
-    uint8_t *matgrid;      // per voxel, brick-major: winner material (surface voxels)
+    uint8_t *matgrid;      // per brick: material bytes of its surface words, packed
     int32_t use_mats;      // surface stage and materials wanted
     int32_t embed;         // winner keys carry the material (mesh-local index < 2^22)
     // erosion and nodes (Cartesian jobs; FCC sub-grids have no such blocks)
@@ -246,7 +250,10 @@ __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *__restrict__
 // Scan (exclusive, uint32), 256 threads x 16 items per tile.  Up to
 // kScanDirectTiles tiles: k_scan_reduce (tile sums) then k_scan_down (each
 // tile adds the sums before it and scans itself); more: tiles scanned with
-// k_scan_blocks, their sums scanned recursively, then k_scan_add.
+// k_scan_blocks, their sums scanned recursively, then k_scan_add.  The
+// output has n + 2 entries: out[n] is the total mod 2^32 and out[n + 1] is 1
+// when the true total reached 2^32 (the offsets wrapped), else 0.  partials
+// holds 2 * tiles + 1 words (tile sums' low words, then their high words).
 // ---------------------------------------------------------------------------
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
@@ -261,7 +268,8 @@ __global__ void k_scan_down(const uint32_t *__restrict__ in, uint32_t *__restric
 __global__ void k_scan_blocks(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
                               uint32_t n, uint32_t *__restrict__ partials);
 __global__ void k_scan_add(uint32_t *__restrict__ out, uint32_t n,
-                           const uint32_t *__restrict__ partials);
+                           const uint32_t *__restrict__ partials,
+                           const uint32_t *__restrict__ psum);
 
 // ---------------------------------------------------------------------------
 // Grid planning
@@ -315,6 +323,10 @@ __global__ void k_solid_setup(const Job *__restrict__ jobs, const uint32_t *__re
 __global__ void k_block_starts(const uint32_t *__restrict__ cand_off, uint32_t nslots,
                                uint32_t cap, uint32_t *__restrict__ first);
 inline size_t block_starts_size(uint32_t total) { return (size_t)(total + 255u) / 256u + 1u; }
+// With pmask (jobs with several y/z subspaces), pmask[c] receives the rows of
+// the candidate's tile whose subspace queues the triangle (bit r = centre row
+// r of the tile, y fastest); the scatter pass carries it into list_mask and
+// the fill applies a triangle's crossings only on those rows.
 template <bool kMulti>
 __global__ void k_solid_candidates(const Job *__restrict__ jobs,
                                    const uint32_t *__restrict__ tri_job, uint32_t nslots,
@@ -323,19 +335,22 @@ __global__ void k_solid_candidates(const Job *__restrict__ jobs,
                                    const uint32_t *__restrict__ cand_total,
                                    const uint32_t *__restrict__ first,
                                    const uint4 *__restrict__ cbox, uint2 *__restrict__ pairs,
+                                   uint16_t *__restrict__ pmask,
                                    uint32_t *__restrict__ tile_hist, uint32_t cap,
                                    uint32_t *__restrict__ ovf);
 __global__ void k_solid_scatter(const uint2 *__restrict__ pairs,
+                                const uint16_t *__restrict__ pmask,
                                 const uint32_t *__restrict__ cand_total,
                                 const uint32_t *__restrict__ tile_off,
                                 uint32_t *__restrict__ tile_cursor, uint32_t *__restrict__ list,
-                                uint32_t cap);
+                                uint16_t *__restrict__ list_mask, uint32_t cap);
 // One workgroup per tile of every job (grid = all tiles).
 __global__ void k_solid_fill(const Job *__restrict__ jobs, const uint32_t *__restrict__ tile_bases,
                              uint32_t njobs, const float4 *__restrict__ rec,
                              const uint32_t *__restrict__ tile_off,
                              const uint32_t *__restrict__ tile_hist,
-                             const uint32_t *__restrict__ list);
+                             const uint32_t *__restrict__ list,
+                             const uint16_t *__restrict__ list_mask);
 // Zeroes the pad rows (y or z at the alloc border); grid = all jobs' blocks.
 __global__ void k_zero_pad_rows(const Job *__restrict__ jobs,
                                 const uint32_t *__restrict__ pad_bases, uint32_t njobs);
@@ -368,6 +383,7 @@ __global__ void k_surf_brick(const Job *__restrict__ jobs, const uint32_t *__res
                              const uint32_t *__restrict__ order,
                              const uint32_t *__restrict__ n_order,
                              const uint32_t *__restrict__ hlist,
+                             const uint32_t *__restrict__ hsub,
                              const uint32_t *__restrict__ n_heavy,
                              const uint8_t *__restrict__ skip, uint32_t *__restrict__ next,
                              uint64_t *__restrict__ cycles);
@@ -419,19 +435,28 @@ __host__ __device__ inline IdPlanes orient_planes(uint32_t mx, uint32_t px, uint
     return o;
 }
 
-// Surface data of one grid as the node passes read it.  Brick b's voxels
-// occupy bytes [b * nvox, (b + 1) * nvox) of matgrid in the brick's word
-// order (z, y, x-word), 32 bytes per word.  A word that holds a surface
-// voxel has all 32 bytes written in the call (0 for its other voxels); the
-// rest may be stale.  Bricks the surface pass did not visit (bhist == 0)
-// hold stale surface words, so the readers check bhist first.
+// Surface data of one grid as the node passes read it.  A brick layer is one
+// z plane of a brick: wpl = (bx / 32) * by words (at most 32) in (y, x-word)
+// order.  smask[b * bz + l] has bit j set when word j of layer l of brick b
+// holds a surface voxel.  Brick b owns bytes [b * nvox, (b + 1) * nvox) of
+// matgrid; layer l's part starts at byte l * wpl * 32 and holds, for each of
+// its surface words in word order, the word's 32 material bytes (the
+// winner's material for surface voxels, 0 for the others).  Bricks the
+// surface pass did not visit (bhist == 0) hold stale masks, so the readers
+// check bhist first.
 struct MatSource {
-    const VK_G uint32_t *surf;   // surface bit words (same layout as bits)
+    const VK_G uint32_t *smask;  // per brick layer: surface-word mask
     const VK_G uint32_t *bhist;  // per brick: nonzero if visited
     const VK_G uint8_t *matgrid;
 };
 __device__ __forceinline__ MatSource mat_source(const Job &J) {
-    return MatSource{gptr(J.surf), gptr(J.bhist), gptr(J.matgrid)};
+    return MatSource{gptr(J.smask), gptr(J.bhist), gptr(J.matgrid)};
+}
+// Byte offset, within brick b's block, of the material bytes of word wl of
+// layer l whose layer mask is m (the word must hold a surface voxel).
+__device__ __forceinline__ uint32_t packed_word_offset(uint32_t l, uint32_t lwl, uint32_t wl,
+                                                       uint32_t m) {
+    return ((l << lwl) + (uint32_t)__popc(m & ((1u << wl) - 1u))) << 5;
 }
 
 // Node passes.  Cartesian: every Cartesian job's row blocks (grid = all);
@@ -499,6 +524,12 @@ __global__ void k_cost_predict(const float *__restrict__ P, const uint32_t *__re
                                const uint4 *__restrict__ tri_info, const uint4 *__restrict__ tri_ext,
                                const Job *__restrict__ jobs,
                                const uint32_t *__restrict__ brick_bases, uint32_t njobs,
-                               uint32_t *__restrict__ hlist, uint8_t *__restrict__ skip);
+                               uint32_t *__restrict__ hlist, uint32_t *__restrict__ hsub,
+                               uint8_t *__restrict__ skip);
+// Standardized log cost, above the heavy threshold, per doubling of the
+// number of sub-bricks a heavy brick is cut into (k_cost_predict).
+constexpr float kSplitStep = 1.0f;
+// Heavy-list entries per listed brick at most (sub-bricks).
+constexpr uint32_t kMaxSplitParts = 4;
 
 }  // namespace vk

@@ -11,8 +11,33 @@ namespace vk {
 // ---------------------------------------------------------------------------
 // Scan
 // ---------------------------------------------------------------------------
+// Every scan writes out[0, n] (exclusive prefix, out[n] = total mod 2^32) and
+// out[n + 1] = 1 if the true total reached 2^32 (the 32-bit offsets wrapped),
+// else 0.  Tile sums are kept in 64 bits for that: partials[b] holds tile b's
+// low word and partials[ntiles + b] its high word.
+
+// Sum of one value per thread over a 256-thread block, in 64 bits.
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t s) {
+    __shared__ uint64_t wsum64[kScanThreads / kWave];
+#pragma unroll
+    for (int d = kWave / 2; d >= 1; d >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)s, d);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(s >> 32), d);
+        s += ((uint64_t)hi << 32) | lo;
+    }
+    if ((threadIdx.x & (kWave - 1u)) == 0u) wsum64[threadIdx.x / kWave] = s;
+    __syncthreads();
+    uint64_t t = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kScanThreads / kWave; w++) t += wsum64[w];
+    __syncthreads();  // wsum64 may be reused by the caller's next call
+    return t;
+}
+
 // Exclusive scan of in[0, n) into out[0, n] (out[n] = total); one block scans
-// kScanTile elements and leaves its sum in partials[block].
+// kScanTile elements and leaves its sum in partials[block] (low word) and
+// partials[nblocks + block] (high word).  The last block clears the wrap flag
+// out[n + 1]; k_scan_add sets it.
 __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(const uint32_t *__restrict__ in,
                                                               uint32_t *__restrict__ out,
                                                               uint32_t n,
@@ -22,12 +47,15 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(const uint32_t *__
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)tid * kScanItems;
     uint32_t v[kScanItems];
     uint32_t s = 0;
+    uint64_t s64 = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) {
         uint64_t idx = base + k;
         v[k] = idx < n ? in[idx] : 0u;
         s += v[k];
+        s64 += v[k];
     }
+    const uint64_t tile64 = block_sum_u64(s64);
     sh[tid] = s;
     __syncthreads();
     for (uint32_t off = 1; off < kScanThreads; off <<= 1) {
@@ -43,7 +71,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(const uint32_t *__
         if (idx <= n) out[idx] = excl;
         excl += v[k];
     }
-    if (tid == kScanThreads - 1) partials[blockIdx.x] = sh[tid];
+    if (tid == kScanThreads - 1) {
+        partials[blockIdx.x] = sh[tid];
+        partials[gridDim.x + blockIdx.x] = (uint32_t)(tile64 >> 32);
+    }
+    if (tid == 0 && blockIdx.x == gridDim.x - 1u) out[(uint64_t)n + 1u] = 0u;
 }
 
 // Block-wide sum and exclusive prefix of one value per thread: a wave scan
@@ -69,31 +101,40 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t s, uint32_t &total)
     return before + incl - s;
 }
 
-// Tile sums of in[0, n) (direct path).
+// Tile sums of in[0, n) (direct path): low words in partials[0, nblocks),
+// high words in partials[nblocks, 2 * nblocks).
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__restrict__ in,
                                                               uint32_t n,
                                                               uint32_t *__restrict__ partials) {
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
-    uint32_t s = 0u;
+    uint64_t s = 0u;
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) {
         const uint64_t idx = base + k;
         s += idx < n ? in[idx] : 0u;
     }
-    uint32_t total;
-    block_excl_scan(s, total);
-    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+    const uint64_t total = block_sum_u64(s);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = (uint32_t)total;
+        partials[gridDim.x + blockIdx.x] = (uint32_t)(total >> 32);
+    }
 }
 
 // Exclusive scan of in[0, n) into out[0, n] (out[n] = total), direct path:
 // the tile first sums the tile sums before it (at most kScanDirectTiles).
+// The last tile also adds them in 64 bits and writes the wrap flag.
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *__restrict__ in,
                                                             uint32_t *__restrict__ out,
                                                             uint32_t n,
                                                             const uint32_t *__restrict__ partials) {
     __shared__ uint32_t s_off;
+    const bool last = blockIdx.x == gridDim.x - 1u;
     uint32_t p = 0u;
-    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kScanThreads) p += partials[i];
+    uint64_t p64 = 0u;
+    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kScanThreads) {
+        p += partials[i];
+        if (last) p64 += ((uint64_t)partials[gridDim.x + i] << 32) | partials[i];
+    }
     uint32_t ptotal;
     block_excl_scan(p, ptotal);
     if (threadIdx.x == 0) s_off = ptotal;
@@ -107,6 +148,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *__re
         const uint64_t idx = base + k;
         v[k] = idx < n ? in[idx] : 0u;
         s += v[k];
+        if (last) p64 += v[k];
     }
     uint32_t total;
     uint32_t excl = off + block_excl_scan(s, total);
@@ -116,17 +158,28 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *__re
         if (idx <= n) out[idx] = excl;
         excl += v[k];
     }
+    if (last) {  // uniform
+        const uint64_t all = block_sum_u64(p64);
+        if (threadIdx.x == 0) out[(uint64_t)n + 1u] = (all >> 32) != 0u ? 1u : 0u;
+    }
 }
 
+// Adds the scanned tile sums (recursive path) and sets the wrap flag when a
+// tile's sum or the sum of the tile sums reached 2^32 (psum[nblocks + 1] is
+// the recursive scan's own flag).
 __global__ __launch_bounds__(kScanThreads) void k_scan_add(uint32_t *__restrict__ out,
                                                            uint32_t n,
-                                                           const uint32_t *__restrict__ partials) {
+                                                           const uint32_t *__restrict__ partials,
+                                                           const uint32_t *__restrict__ psum) {
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-    const uint32_t add = partials[blockIdx.x];
+    const uint32_t add = psum[blockIdx.x];
     for (uint32_t k = threadIdx.x; k < (uint32_t)kScanTile; k += kScanThreads) {
         uint64_t idx = base + k;
         if (idx <= n) out[idx] += add;
     }
+    if (threadIdx.x == 0 &&
+        (partials[gridDim.x + blockIdx.x] != 0u || (blockIdx.x == 0 && psum[gridDim.x + 1u] != 0u)))
+        out[(uint64_t)n + 1u] = 1u;
 }
 
 // ---------------------------------------------------------------------------
@@ -253,11 +306,71 @@ __global__ __launch_bounds__(256) void k_solid_setup(const Job *__restrict__ job
 __global__ __launch_bounds__(256) void k_block_starts(const uint32_t *__restrict__ cand_off,
                                                       uint32_t nslots, uint32_t cap,
                                                       uint32_t *__restrict__ first) {
-    const uint32_t total = min(cand_off[nslots], cap);
+    const uint32_t total = cand_off[nslots + 1u] ? 0u : min(cand_off[nslots], cap);
     for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < nslots; t += gridDim.x * 256u) {
         const uint32_t lo = min(cand_off[t], total), hi = min(cand_off[t + 1u], total);
         for (uint32_t b = (lo + 255u) / 256u; b * 256u < hi; b++) first[b] = t;
     }
+}
+
+// Rows of tile (ty, tz) whose voxSplitRes subspace queues the triangle (bit
+// r = centre (4 ty + (r & 3), 4 tz + (r >> 2))).  The reference builds its
+// queue per y/z subspace (src/host_side.cpp:275-397): a tile is queued for a
+// subspace when one of its centres inside that subspace passes the (>= 0)
+// filter (src/voxelizer.cu:880-925, 998-1031), and the fill then tests only
+// that subspace's centres.  So a centre's crossing counts only when some
+// centre of the same subspace part of the tile passes.  sy/sz: the subspace
+// start flags of the ext rows (Job::sstart).
+__device__ __forceinline__ uint32_t tile_split_mask(const TileSetup &ts, uint32_t ty, uint32_t tz,
+                                                    double emy, double emz, double d,
+                                                    const uint8_t *__restrict__ sy, uint32_t ry,
+                                                    const uint8_t *__restrict__ sz, uint32_t rz) {
+    uint32_t pass = 0u;
+#pragma unroll
+    for (uint32_t r = 0; r < 16u; r++) {
+        const uint32_t i = ty * 4u + (r & 3u), j = tz * 4u + (r >> 2);
+        if (i >= ts.cmin[0] && i < ts.cmax[0] && j >= ts.cmin[1] && j < ts.cmax[1]) {
+            const double px = double(i) * d + emy;
+            const double py = double(j) * d + emz;
+            bool ok = true;
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                ok &= (ts.ne[k].x * px + ts.ne[k].y * py + ts.de[k] >= 0.0);
+            pass |= (ok ? 1u : 0u) << r;
+        }
+    }
+    if (!pass) return 0u;
+    // Subspace parts of the tile's 4 y rows and 4 z rows: part ids grow by
+    // one where a subspace starts.
+    uint32_t gy[4], gz[4];
+    gy[0] = gz[0] = 0u;
+#pragma unroll
+    for (uint32_t k = 1; k < 4u; k++) {
+        const uint32_t y = ty * 4u + k, z = tz * 4u + k;
+        gy[k] = gy[k - 1] + (y < ry && sy[y] ? 1u : 0u);
+        gz[k] = gz[k - 1] + (z < rz && sz[z] ? 1u : 0u);
+    }
+    uint32_t ysame[4], zsame[4];  // rows in the same part as row k (4-bit sets)
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; k++) {
+        ysame[k] = zsame[k] = 0u;
+#pragma unroll
+        for (uint32_t q = 0; q < 4u; q++) {
+            ysame[k] |= (gy[q] == gy[k] ? 1u : 0u) << q;
+            zsame[k] |= (gz[q] == gz[k] ? 1u : 0u) << q;
+        }
+    }
+    uint32_t mask = 0u;
+#pragma unroll
+    for (uint32_t r = 0; r < 16u; r++) {
+        if (!((pass >> r) & 1u)) continue;
+        const uint32_t ys = ysame[r & 3u], zs = zsame[r >> 2];
+        const uint32_t yrows = ys | (ys << 4) | (ys << 8) | (ys << 12);
+        const uint32_t zrows = ((zs & 1u) ? 0x000Fu : 0u) | ((zs & 2u) ? 0x00F0u : 0u) |
+                               ((zs & 4u) ? 0x0F00u : 0u) | ((zs & 8u) ? 0xF000u : 0u);
+        mask |= yrows & zrows;
+    }
+    return mask;
 }
 
 template <bool kMulti>
@@ -265,12 +378,13 @@ __global__ __launch_bounds__(256) void k_solid_candidates(
     const Job *__restrict__ jobs, const uint32_t *__restrict__ tri_job, uint32_t nslots,
     const float4 *__restrict__ rec, const uint32_t *__restrict__ cand_off,
     const uint32_t *__restrict__ cand_total, const uint32_t *__restrict__ first,
-    const uint4 *__restrict__ cbox, uint2 *__restrict__ pairs, uint32_t *__restrict__ tile_hist,
-    uint32_t cap, uint32_t *__restrict__ ovf) {
+    const uint4 *__restrict__ cbox, uint2 *__restrict__ pairs, uint16_t *__restrict__ pmask,
+    uint32_t *__restrict__ tile_hist, uint32_t cap, uint32_t *__restrict__ ovf) {
     __shared__ uint32_t trange[2];
     __shared__ AggTable agg;
-    const uint32_t full = *cand_total, total = min(full, cap);
-    if (ovf && full > cap && blockIdx.x == 0 && threadIdx.x == 0) *ovf = 1u;
+    // cand_total[1]: the candidate count reached 2^32 (the offsets wrapped).
+    const uint32_t full = *cand_total, total = cand_total[1] ? 0u : min(full, cap);
+    if (ovf && (full > cap || cand_total[1]) && blockIdx.x == 0 && threadIdx.x == 0) *ovf = 1u;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
         __syncthreads();
@@ -306,7 +420,15 @@ __global__ __launch_bounds__(256) void k_solid_candidates(
             const uint32_t w = tmax0 - tmin0 + 1u;
             const uint32_t local = c - cand_off[t];
             const uint32_t ty = tmin0 + local % w, tz = tmin1 + local / w;
-            hit = tile_hit(ts, ty, tz, p.emy, p.emz, p.d);
+            if (J.sstart) {  // several y/z subspaces
+                const uint32_t m = tile_split_mask(ts, ty, tz, p.emy, p.emz, p.d, J.sstart,
+                                                   (uint32_t)J.res[1], J.sstart + J.res[1],
+                                                   (uint32_t)J.res[2]);
+                hit = m != 0u;
+                pmask[c] = (uint16_t)m;
+            } else {
+                hit = tile_hit(ts, ty, tz, p.emy, p.emz, p.d);
+            }
             tile = J.tile_base + tz * p.tilesY + ty;
         }
         // One slot per candidate (no global counter); misses are skipped
@@ -323,21 +445,24 @@ __global__ __launch_bounds__(256) void k_solid_candidates(
 template __global__ void k_solid_candidates<false>(const Job *, const uint32_t *, uint32_t,
                                                    const float4 *, const uint32_t *,
                                                    const uint32_t *, const uint32_t *,
-                                                   const uint4 *, uint2 *, uint32_t *, uint32_t,
-                                                   uint32_t *);
+                                                   const uint4 *, uint2 *, uint16_t *,
+                                                   uint32_t *, uint32_t, uint32_t *);
 template __global__ void k_solid_candidates<true>(const Job *, const uint32_t *, uint32_t,
                                                   const float4 *, const uint32_t *,
                                                   const uint32_t *, const uint32_t *,
-                                                  const uint4 *, uint2 *, uint32_t *, uint32_t,
-                                                  uint32_t *);
+                                                  const uint4 *, uint2 *, uint16_t *,
+                                                  uint32_t *, uint32_t, uint32_t *);
 
 __global__ __launch_bounds__(256) void k_solid_scatter(const uint2 *__restrict__ pairs,
+                                                       const uint16_t *__restrict__ pmask,
                                                        const uint32_t *__restrict__ cand_total,
                                                        const uint32_t *__restrict__ tile_off,
                                                        uint32_t *__restrict__ tile_cursor,
-                                                       uint32_t *__restrict__ list, uint32_t cap) {
+                                                       uint32_t *__restrict__ list,
+                                                       uint16_t *__restrict__ list_mask,
+                                                       uint32_t cap) {
     __shared__ AggTable agg;
-    const uint32_t n = min(*cand_total, cap);
+    const uint32_t n = cand_total[1] ? 0u : min(*cand_total, cap);
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         __syncthreads();
         agg_clear(agg);
@@ -352,7 +477,11 @@ __global__ __launch_bounds__(256) void k_solid_scatter(const uint2 *__restrict__
         for (uint32_t k = threadIdx.x; k < kAggSlots; k += blockDim.x)
             if (agg.key[k] != kAggEmpty) agg.base[k] = atomicAdd(&tile_cursor[agg.key[k]], agg.cnt[k]);
         __syncthreads();
-        if (live) list[tile_off[pr.y] + agg.base[slot] + rank] = pr.x;
+        if (live) {
+            const uint32_t pos = tile_off[pr.y] + agg.base[slot] + rank;
+            list[pos] = pr.x;
+            if (list_mask) list_mask[pos] = pmask[i];
+        }
     }
 }
 
@@ -367,7 +496,8 @@ __global__ __launch_bounds__(kWave) void k_solid_fill(const Job *__restrict__ jo
                                                       const float4 *__restrict__ rec,
                                                       const uint32_t *__restrict__ tile_off,
                                                       const uint32_t *__restrict__ tile_hist,
-                                                      const uint32_t *__restrict__ list) {
+                                                      const uint32_t *__restrict__ list,
+                                                      const uint16_t *__restrict__ list_mask) {
     extern __shared__ uint32_t cross[];  // [16][W]
     const Job &J = jobs[job_of(tile_bases, njobs, blockIdx.x)];
     const SolidParams p = solid_params(J, rec);
@@ -391,6 +521,8 @@ __global__ __launch_bounds__(kWave) void k_solid_fill(const Job *__restrict__ jo
     const uint32_t items = cnt << lg, rows = 16u >> lg;
     for (uint32_t i = tid; i < items; i += kWave) {
         const uint32_t t = list[off + (i >> lg)];
+        // Rows whose subspace queued the triangle (all rows with one subspace).
+        const uint32_t rmask = list_mask ? (uint32_t)list_mask[off + (i >> lg)] : 0xFFFFu;
         const uint32_t r0 = (i & ((1u << lg) - 1u)) * rows;
         D3 v[3];
         load_tri_d(p, t, v);
@@ -399,6 +531,7 @@ __global__ __launch_bounds__(kWave) void k_solid_fill(const Job *__restrict__ jo
         for (uint32_t r = r0; r < r0 + rows; r++) {
             const uint32_t cy = ty * 4u + (r & 3u), cz = tz * 4u + (r >> 2);
             if (cy < p.smin[0] || cy >= p.smax[0] || cz < p.smin[1] || cz >= p.smax[1]) continue;
+            if (!((rmask >> r) & 1u)) continue;
             uint32_t vx;
             if (fill_center(fs, p.emx, p.emy, p.emz, p.d, cy, cz, vx) && vx < p.X)
                 atomicXor(&cross[r * W + (vx >> 5)], 1u << (31u - (vx & 31u)));
@@ -555,8 +688,8 @@ __global__ __launch_bounds__(256) void k_surf_bin(
     uint32_t *__restrict__ blist, int scatter, uint32_t cap, uint32_t *__restrict__ ovf) {
     __shared__ uint32_t trange[2];
     __shared__ AggTable agg;
-    const uint32_t full = *cand_total, total = min(full, cap);
-    if (ovf && full > cap && blockIdx.x == 0 && threadIdx.x == 0) *ovf = 1u;
+    const uint32_t full = *cand_total, total = cand_total[1] ? 0u : min(full, cap);
+    if (ovf && (full > cap || cand_total[1]) && blockIdx.x == 0 && threadIdx.x == 0) *ovf = 1u;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
         __syncthreads();
         agg_clear(agg);
@@ -649,13 +782,14 @@ constexpr int kSurfCols = 1;
 
 
 // Sets up triangle t for the brick whose alloc-coordinate box starts at
-// (ext coordinates) (blx, bly, blz); returns its work items in the brick: voxels along
-// the axis for 1D, dominant-plane columns for 2D/3D.
+// (ext coordinates) (blx, bly, blz), restricted to its z layers [z0, z1)
+// (a sub-brick); returns its work items there: voxels along the axis for 1D,
+// dominant-plane columns for 2D/3D.
 __device__ __forceinline__ uint32_t surf_tri_setup(const SurfParams &p,
                                                    const float4 *__restrict__ tri_rec, uint32_t t,
                                                    uint32_t tri_base, bool embed,
                                                    int32_t blx, int32_t bly, int32_t blz,
-                                                   SurfTri &st) {
+                                                   int32_t z0, int32_t z1, SurfTri &st) {
     const float4 r0 = tri_rec[3 * (size_t)t], r1 = tri_rec[3 * (size_t)t + 1],
                  r2 = tri_rec[3 * (size_t)t + 2];
     const F3 tri[3] = {{r0.x, r0.y, r0.z}, {r0.w, r1.x, r1.y}, {r1.z, r1.w, r2.x}};
@@ -671,8 +805,8 @@ __device__ __forceinline__ uint32_t surf_tri_setup(const SurfParams &p,
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const int ax = ax3[k];
-        const int32_t bl = ax == AX_X ? blx : ax == AX_Y ? bly : blz;
-        const int32_t bs = ax == AX_X ? bd[0] : ax == AX_Y ? bd[1] : bd[2];
+        const int32_t bl = ax == AX_X ? blx : ax == AX_Y ? bly : blz + z0;
+        const int32_t bs = ax == AX_X ? bd[0] : ax == AX_Y ? bd[1] : z1 - z0;
         lo[k] = max(pick3i(vb.min, ax), bl);
         hi[k] = min(pick3i(vb.max, ax), bl + bs - 1);
         if (lo[k] > hi[k]) return 0u;
@@ -863,12 +997,14 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
     const uint32_t *__restrict__ brick_hist, const uint32_t *__restrict__ blist,
     uint32_t *__restrict__ stats, const uint32_t *__restrict__ order,
     const uint32_t *__restrict__ n_order, const uint32_t *__restrict__ hlist,
-    const uint32_t *__restrict__ n_heavy, const uint8_t *__restrict__ skip,
-    uint32_t *__restrict__ next, uint64_t *__restrict__ cycles) {
+    const uint32_t *__restrict__ hsub, const uint32_t *__restrict__ n_heavy,
+    const uint8_t *__restrict__ skip, uint32_t *__restrict__ next,
+    uint64_t *__restrict__ cycles) {
     __shared__ __align__(16) uint32_t keys[kMaxBrickVox];
     __shared__ SurfTri tris[kSurfChunk];
     __shared__ uint32_t colsum[kSurfChunk + 1];
     __shared__ uint32_t wsurf[kMaxBrickVox / 32];
+    __shared__ uint32_t lmask[32];  // per brick layer: words holding a surface voxel
     __shared__ uint8_t touched[32];
     __shared__ uint32_t s_q, s_qe;
     __shared__ SplitTable spl;
@@ -914,6 +1050,9 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
             !from_heavy && hlist &&
             ((__builtin_amdgcn_readfirstlane(
                   ((const uint32_t *)skip)[(qs - nh) >> 2]) >> (8u * ((qs - nh) & 3u))) & 0xFFu);
+        // Layer range of the entry: a heavy entry may be a sub-brick.
+        const uint32_t sub =
+            from_heavy ? __builtin_amdgcn_readfirstlane(hsub[qs]) : 0xFFFF0000u;
         if (!skipped) {
             const uint32_t cnt = __builtin_amdgcn_readfirstlane(brick_hist[gb]);
             uint64_t t_start = 0;
@@ -940,6 +1079,7 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
             const uint32_t b = gb - J.brick_base;  // brick within the job
             const int32_t bd0 = p.bdim[0], bd1 = p.bdim[1], bd2 = p.bdim[2];
             const int32_t nvox = bd0 * bd1 * bd2;
+            const int32_t z0 = (int32_t)(sub & 0xFFFFu), z1 = min((int32_t)(sub >> 16), bd2);
             const int32_t wpr = bd0 / 32;     // words per row
             const int32_t nwords = nvox / 32;
             const int32_t row_shift = 31 - __clz(bd0);
@@ -972,7 +1112,7 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
                     if (c0 + tid < cnt) {
                         const uint32_t t = blist[off + c0 + tid];
                         items = surf_tri_setup(p, tri_rec, t, J.tri_base, embed, blx, bly, blz,
-                                               tris[tid]);
+                                               z0, z1, tris[tid]);
                     }
                     uint32_t incl = items;  // wave-level inclusive scan
                     for (int d = 1; d < kSurfChunk; d <<= 1) {
@@ -998,58 +1138,83 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
             // Surface bit words and winner materials, 256 voxels per wave
             // step: each lane takes 4 consecutive voxels (one 16-byte LDS
             // read of their keys), the 8 lanes of a 32-voxel word OR their
-            // nibbles together, and each lane writes its 4 material bytes as
-            // one dword of the brick's block in the material grid (the
-            // winner's material, 0 for the other voxels).  Steps whose rows
-            // received no key only zero their surface words.  Keys are
-            // cleared for the next brick.  With materials in the keys the
-            // loop issues no global loads, so its stores never wait (a wait
-            // for a load also waits for earlier stores here).
-            VK_G uint32_t *__restrict__ mg =
-                (VK_G uint32_t *)(gptr(J.matgrid) + (size_t)b * (uint32_t)nvox);
+            // nibbles together, and each lane keeps its 4 material bytes as
+            // one dword (the winner's material, 0 for the other voxels).
+            // Steps whose rows received no key only zero their surface
+            // words.  Keys are cleared for the next brick.  With materials in
+            // the keys the loop issues no global loads.
             const uint32_t tmask = (uint32_t)__ballot(lane < 32u && touched[lane & 31u] != 0);
             const uint32_t rows_per_step = 256u >> row_shift;  // bd0 >= 64: 1, 2 or 4 rows
-            auto rows_out = [&](auto embed_c) {
-                constexpr bool kEmbedded = decltype(embed_c)::value;
-                for (int32_t v0 = (int32_t)wave * 256; v0 < nvox; v0 += 1024) {
-                    const uint32_t rmask = ((1u << rows_per_step) - 1u) << (v0 >> row_shift);
-                    if (!(tmask & rmask)) {
-                        if (lane < 8u) wsurf[(v0 >> 5) + (int32_t)lane] = 0u;
-                        continue;
-                    }
-                    uint4 *kp = (uint4 *)&keys[v0 + 4 * (int32_t)lane];
-                    const uint4 kv = *kp;
-                    const uint32_t nib = (kv.x ? 8u : 0u) | (kv.y ? 4u : 0u) | (kv.z ? 2u : 0u) |
-                                         (kv.w ? 1u : 0u);
-                    uint32_t w = nib << (28u - 4u * (lane & 7u));
-                    w |= (uint32_t)__shfl_xor((int)w, 1);
-                    w |= (uint32_t)__shfl_xor((int)w, 2);
-                    w |= (uint32_t)__shfl_xor((int)w, 4);
-                    if ((lane & 7u) == 0u) wsurf[(v0 >> 5) + (int32_t)(lane >> 3)] = w;
-                    nsurf += (uint32_t)__popc(nib);
-                    uint32_t m;
-                    if (kEmbedded) {
-                        m = (kv.x & 0xFFu) | (kv.y & 0xFFu) << 8 | (kv.z & 0xFFu) << 16 |
-                            (kv.w & 0xFFu) << 24;
-                    } else {
-                        auto mat = [&](uint32_t key) -> uint32_t {
-                            return key && materials ? (uint32_t)materials[key & 0x3FFFFFFFu] : 0u;
-                        };
-                        m = mat(kv.x) | mat(kv.y) << 8 | mat(kv.z) << 16 | mat(kv.w) << 24;
-                    }
-                    if (nib) *kp = make_uint4(0u, 0u, 0u, 0u);
-                    mg[(v0 >> 2) + (int32_t)lane] = m;
+            constexpr int kSteps = kMaxBrickVox / 1024;  // wave steps per brick, at most
+            uint32_t mdw[kSteps];
+#pragma unroll
+            for (int st = 0; st < kSteps; st++) {
+                const int32_t v0 = (int32_t)wave * 256 + st * 1024;
+                mdw[st] = 0u;
+                if (v0 >= nvox) continue;  // uniform
+                const uint32_t rmask = ((1u << rows_per_step) - 1u) << (v0 >> row_shift);
+                if (!(tmask & rmask)) {
+                    if (lane < 8u) wsurf[(v0 >> 5) + (int32_t)lane] = 0u;
+                    continue;
                 }
-            };
-            if (embed)
-                rows_out(std::integral_constant<bool, true>{});
-            else
-                rows_out(std::integral_constant<bool, false>{});
+                uint4 *kp = (uint4 *)&keys[v0 + 4 * (int32_t)lane];
+                const uint4 kv = *kp;
+                const uint32_t nib = (kv.x ? 8u : 0u) | (kv.y ? 4u : 0u) | (kv.z ? 2u : 0u) |
+                                     (kv.w ? 1u : 0u);
+                uint32_t w = nib << (28u - 4u * (lane & 7u));
+                w |= (uint32_t)__shfl_xor((int)w, 1);
+                w |= (uint32_t)__shfl_xor((int)w, 2);
+                w |= (uint32_t)__shfl_xor((int)w, 4);
+                if ((lane & 7u) == 0u) wsurf[(v0 >> 5) + (int32_t)(lane >> 3)] = w;
+                nsurf += (uint32_t)__popc(nib);
+                if (embed) {  // uniform
+                    mdw[st] = (kv.x & 0xFFu) | (kv.y & 0xFFu) << 8 | (kv.z & 0xFFu) << 16 |
+                              (kv.w & 0xFFu) << 24;
+                } else {
+                    auto mat = [&](uint32_t key) -> uint32_t {
+                        return key && materials ? (uint32_t)materials[key & 0x3FFFFFFFu] : 0u;
+                    };
+                    mdw[st] = mat(kv.x) | mat(kv.y) << 8 | mat(kv.z) << 16 | mat(kv.w) << 24;
+                }
+                if (nib) *kp = make_uint4(0u, 0u, 0u, 0u);
+            }
             __syncthreads();
-            if ((int32_t)tid < nwords && gword) {
-                const uint32_t sw = wsurf[tid];
-                gptr(J.surf)[gi] = sw;
-                if (sw) bits[gi] = bw | sw;
+            // Per brick layer (one z plane of the brick, wpl = words per
+            // layer <= 32, a power of two), the mask of its words that hold a
+            // surface voxel: one ballot per 64 words, a layer never straddles
+            // two waves.  Surface bits are ORed into the solid grid.
+            const int32_t wpl = wpr * bd1;
+            if ((int32_t)(wave * kWave) < nwords) {  // the waves that hold words (uniform)
+                const uint32_t sw = (int32_t)tid < nwords ? wsurf[tid] : 0u;
+                const unsigned long long bal = __ballot(sw != 0u);
+                if ((int32_t)tid < nwords && ((int32_t)tid & (wpl - 1)) == 0) {
+                    const uint32_t lm =
+                        (uint32_t)(bal >> lane) & (wpl == 32 ? 0xFFFFFFFFu : (1u << wpl) - 1u);
+                    const int32_t layer = (int32_t)tid / wpl;
+                    lmask[layer] = lm;
+                    if (layer >= z0 && layer < z1)  // the entry's layers only (sub-bricks)
+                        gptr(J.smask)[(size_t)b * (uint32_t)bd2 + (uint32_t)layer] = lm;
+                }
+                if ((int32_t)tid < nwords && gword && sw) bits[gi] = bw | sw;
+            }
+            __syncthreads();
+            // Material dwords of the words that hold a surface voxel, packed
+            // per layer in word order: layer l's words start at dword
+            // l * wpl * 8 of the brick's block, a word's 8 dwords at its rank
+            // among the layer's surface words.
+            VK_G uint32_t *__restrict__ mg =
+                (VK_G uint32_t *)(gptr(J.matgrid) + (size_t)b * (uint32_t)nvox);
+#pragma unroll
+            for (int st = 0; st < kSteps; st++) {
+                const int32_t v0 = (int32_t)wave * 256 + st * 1024;
+                if (v0 >= nvox) continue;  // uniform
+                const int32_t wi = (v0 >> 5) + (int32_t)(lane >> 3);  // word in the brick
+                const int32_t layer = wi / wpl, wl = wi & (wpl - 1);
+                const uint32_t lm = lmask[layer];
+                if ((lm >> wl) & 1u) {
+                    const uint32_t rank = (uint32_t)__popc(lm & ((1u << wl) - 1u));
+                    mg[((uint32_t)(layer * wpl) + rank) * 8u + (lane & 7u)] = mdw[st];
+                }
             }
             if (cycles && tid == 0) cycles[gb] = __builtin_amdgcn_s_memtime() - t_start;
         }  // brick
@@ -1400,10 +1565,7 @@ __device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
         moff[k] = kOob;
     }
     Rsrc mr = rsrc(gptr(J.bits), 0u);
-    if (use_mats) {  // uniform; the kNodeZ planes share one brick layer (kNodeZ divides its depth)
-        const VK_G uint32_t *surf = gptr(J.surf);
-#pragma unroll
-        for (uint32_t k = 0; k < kNodeZ; k++) S[k] = ld32(plane(surf, zb + k), o);
+    if (use_mats) {  // uniform; the kNodeZ planes lie in one brick (kNodeZ divides its depth)
         // Material blocks relative to the first brick row the block touches.
         const uint32_t lbytes = bd.lwx + bd.lby + bd.lbz + 5u;  // log2 bytes per brick
         const uint32_t yb0 = (i0 / g.W) >> bd.lby;
@@ -1411,14 +1573,23 @@ __device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
         const uint32_t brow0 = ((zb >> bd.lbz) * bd.nby + yb0) * bd.nbx;
         mr = rsrc(gptr(J.matgrid) + ((size_t)brow0 << lbytes), ((yb1 - yb0 + 1u) * bd.nbx) << lbytes);
         const uint32_t b = ((y >> bd.lby) - yb0) * bd.nbx + (w >> bd.lwx);
+        // The layer masks of the kNodeZ planes are consecutive (one 16-byte
+        // load, issued with the bit loads; stale unless the brick was
+        // visited, so they are masked by bhist).
+        static_assert(kNodeZ == 4, "one 16-byte load of the layer masks");
+        const uint32_t layer0 = zb & ((1u << bd.lbz) - 1u);
         const uint32_t vis = valid ? gptr(J.bhist)[brow0 + b] : 0u;
-        const uint32_t lane_off =
-            (b << lbytes) +
-            ((((y & ((1u << bd.lby) - 1u)) << bd.lwx) + (w & ((1u << bd.lwx) - 1u))) << 5);
+        const u32x4 lm = valid ? *(const VK_G u32x4 *)(gptr(J.smask) +
+                                                       ((size_t)(brow0 + b) << bd.lbz) + layer0)
+                               : u32x4{0u, 0u, 0u, 0u};
+        const uint32_t lwl = bd.lby + bd.lwx;  // log2 words per layer
+        const uint32_t wl = ((y & ((1u << bd.lby) - 1u)) << bd.lwx) + (w & ((1u << bd.lwx) - 1u));
+        const uint32_t mk[kNodeZ] = {lm.x, lm.y, lm.z, lm.w};
 #pragma unroll
         for (uint32_t k = 0; k < kNodeZ; k++) {
-            if (!vis) S[k] = 0u;
-            moff[k] = lane_off + ((((zb + k) & ((1u << bd.lbz) - 1u)) << (bd.lby + bd.lwx)) << 5);
+            const uint32_t m = vis ? mk[k] : 0u;
+            S[k] = (m >> wl) & 1u;
+            moff[k] = (b << lbytes) + packed_word_offset(layer0 + k, lwl, wl, m);
         }
     }
     // Round trip 2: the 32 material bytes of each surface word.
@@ -1676,19 +1847,19 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(const Job *__restrict__ jobs,
         const uint64_t gw = (uint64_t)(fw >> 1), gz = (uint64_t)(fz >> 1);
         const uint32_t b = (uint32_t)((((gz >> bd.lbz) * bd.nby + ((uint64_t)y >> bd.lby)) *
                                        bd.nbx) + (gw >> bd.lwx));
-        const uint32_t wb = (uint32_t)(((((gz & ((1u << bd.lbz) - 1u)) << bd.lby) +
-                                         ((uint64_t)y & ((1u << bd.lby) - 1u)))
-                                        << bd.lwx) + (gw & ((1u << bd.lwx) - 1u)));
-        const size_t bbase = ((size_t)b << (bd.lwx + bd.lby + bd.lbz + 5u)) + (size_t)wb * 32u +
-                             ((fw & 1) ? 16u : 0u);
-        const size_t gwi = (gz * g.Y + (uint64_t)y) * g.W + gw;
-        const int sh = (fw & 1) ? 0 : 16;
+        const uint32_t layer = (uint32_t)(gz & ((1u << bd.lbz) - 1u));
+        const uint32_t lwl = bd.lby + bd.lwx;  // log2 words per layer
+        const uint32_t wl = (uint32_t)((((uint64_t)y & ((1u << bd.lby) - 1u)) << bd.lwx) +
+                                       (gw & ((1u << bd.lwx) - 1u)));
+        const size_t bbase = ((size_t)b << (bd.lwx + bd.lby + bd.lbz + 5u)) + ((fw & 1) ? 16u : 0u);
 #pragma unroll
         for (int par = 0; par < 2; par++) {
             const MatSource src = mat_source(Js[odd ? 2 + par : par]);  // uniform
             if (!src.bhist[b]) continue;
-            if (((src.surf[gwi] >> sh) & 0xFFFFu) == 0u) continue;
-            const u32x4 v = *(const VK_G u32x4 *)(src.matgrid + bbase);
+            const uint32_t m = src.smask[((size_t)b << bd.lbz) + layer];
+            if (!((m >> wl) & 1u)) continue;
+            const u32x4 v = *(const VK_G u32x4 *)(src.matgrid + bbase +
+                                                  packed_word_offset(layer, lwl, wl, m));
             ma[par][0] = v.x; ma[par][1] = v.y; ma[par][2] = v.z; ma[par][3] = v.w;
         }
     }

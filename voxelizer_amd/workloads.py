@@ -459,3 +459,54 @@ def random_soup(seed: int, n_tris: int, lo=0.0, hi=1.0) -> Mesh:
     m = b.mesh("soup", seed=seed)
     m.materials = (1 + rng.integers(0, 8, m.nt)).astype(np.uint8)
     return m
+
+
+def normal_x_signs(v: np.ndarray):
+    """(tN, n.x) of triangles v (T, 3, 3) in float64, in the two expressions
+    the reference uses: the solid filter's tN (src/voxelizer.cu:882-884) and
+    the x component of cross(v0 - v2, v1 - v0) of the fill
+    (src/voxelizer.cu:1375-1378).  Both equal the same exact value; rounding
+    can give them different signs."""
+    v = np.asarray(v, np.float64)
+    y, z = v[:, :, 1], v[:, :, 2]
+    tN = (y[:, 0] * z[:, 1] + y[:, 2] * z[:, 0] + z[:, 2] * y[:, 1]) - \
+         (y[:, 2] * z[:, 1] + z[:, 0] * y[:, 1] + z[:, 2] * y[:, 0])
+    nx = (y[:, 0] - y[:, 2]) * (z[:, 1] - z[:, 0]) - (z[:, 0] - z[:, 2]) * (y[:, 1] - y[:, 0])
+    return tN, nx
+
+
+def disagreeing_slivers(seed: int = 3, n: int = 64, box_hi=(8.0, 34.0, 34.0)) -> Mesh:
+    """A closed box [0, box_hi] plus n triangles whose y/z projection is a
+    segment through the origin (vertices (x, a s, b s) with small odd a, b and
+    s of at most 20 significant bits, so every coordinate is exact in FP32)
+    and whose tN and n.x have different signs: the sums in tN round where the
+    exact value is 0, n.x is computed exactly.  Such triangles orient the
+    solid filter's edges one way and the fill's the other."""
+    rng = _rng(seed)
+    dirs = [(3, 5), (5, 3), (3, 7), (7, 3), (1, 3), (3, 1)]
+    tris = []
+    while len(tris) < n:
+        a, b = dirs[int(rng.integers(0, len(dirs)))]
+        lim = min(box_hi[1] / a, box_hi[2] / b) * 0.95
+
+        def few_bits(lo, hi):
+            x = rng.uniform(lo, hi)
+            e = math.floor(math.log2(x))
+            return round(x / 2.0 ** (e - 19)) * 2.0 ** (e - 19)
+        s = [few_bits(2.0 ** -30, 2.0 ** -29), few_bits(0.55 * lim, lim),
+             few_bits(0.05 * lim, 0.5 * lim)]
+        if rng.random() < 0.5:
+            s[1], s[2] = s[2], s[1]
+        xs = rng.uniform(0.5, box_hi[0] - 0.5, 3).astype(np.float32).astype(np.float64)
+        t = np.array([[xs[k], a * s[k], b * s[k]] for k in range(3)], np.float64)
+        assert np.array_equal(t.astype(np.float32).astype(np.float64), t)
+        tN, nx = normal_x_signs(t[None])
+        if tN[0] != 0.0 and (tN[0] > 0.0) != (nx[0] > 0.0):
+            tris.append(t)
+    bx = box((0.0, 0.0, 0.0), box_hi, subdiv=2)
+    b = _Builder()
+    b.add(bx.vertices.astype(np.float64), bx.indices.astype(np.int64), bx.materials)
+    v = np.concatenate(tris)
+    b.add(v, np.arange(v.shape[0], dtype=np.int64).reshape(-1, 3),
+          (1 + np.arange(n) % 7).astype(np.uint8))
+    return b.mesh("slivers", seed=seed)
