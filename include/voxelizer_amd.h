@@ -153,6 +153,47 @@ vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *grid, vox_node_kind 
                               int use_materials, const uint32_t vox_split[3],
                               const uint32_t mat_split[3], void *nodes_dev);
 
+/* One partition of a Cartesian grid, as one device of the reference's
+ * multi-device mode voxelizes it (voxelizeWorker, src/host_side.cpp:901-1244,
+ * the subspaces of voxelizeEntry's devConfig split, 782-813): part_min /
+ * part_max bound the partition in the plan's ext (y, z) coordinates, x is
+ * whole.  Where the partition stops short of the grid's edge it has a
+ * neighbour and reaches one row or plane into it, so ids at the seam see the
+ * neighbour's voxels (994-1011; src/voxelizer.cu:1487-1502, 3374-3383), and
+ * the finished array's border is cleared (zeroPadding,
+ * src/voxelizer.cu:4811-4844).  nodes_dev holds res[0] x (py + 2) x (pz + 2)
+ * nodes.  ShortNode and LongNode (VOX_ERR_UNSUPPORTED for FCC). */
+vox_status vox_voxelize_partition(vox_ctx *ctx, const vox_grid *grid, vox_node_kind kind,
+                                  int use_materials, const uint32_t vox_split[3],
+                                  const uint32_t mat_split[3], const uint32_t part_min[2],
+                                  const uint32_t part_max[2], void *nodes_dev);
+/* Plain (bit) voxelization of one partition, as voxelize() with a devConfig
+ * leaves it on each device (voxelizeWorker up to its nodeOutput return,
+ * src/host_side.cpp:901-1055): the solid bits of the ext partition (the
+ * partition plus one row or plane into each neighbour) in an array of
+ * alloc[0]/32 x (py + 2) x (pz + 2) words, ext row / plane e stored at
+ * e + 1 less one where the partition has a neighbour below; rows and planes
+ * with no ext row behind them are zero.  split_yz as in vox_solid_bits. */
+vox_status vox_solid_bits_partition(vox_ctx *ctx, const vox_grid *grid, const uint32_t split_yz[2],
+                                    const uint32_t part_min[2], const uint32_t part_max[2],
+                                    uint32_t *bits_dev);
+/* One slice of the current mesh (voxelizeSlice / voxelizeSliceToRAM,
+ * src/host_side.cpp:1928-2262): the grid planned by mode/param, the slice
+ * `slice` (1 .. resolution, 0 and past the grid give all zeros) along
+ * direction 0 x, 1 y or 2 z, voxelized as a one-row partition with its two
+ * neighbour rows.  Along x the model is rotated a quarter turn about z first
+ * (rotateVertices, 3261-3290) and the nodes rotated back (unRotateNodes,
+ * src/voxelizer.cu:2061-2124); that needs vox_split[1] at least the rotated
+ * grid's x size (VOX_ERR_UNSUPPORTED otherwise).  vox_split / mat_split are
+ * the reference's (x, y) pairs, y serving z too.  dims_out receives the node
+ * array's dimensions: (X, 3, Z + 2) along y, (X, Y + 2, 3) along z,
+ * (3, X', Z' + 2) along x; with nodes_dev NULL only the dimensions are
+ * computed.  ShortNode and LongNode (VOX_ERR_UNSUPPORTED for FCC). */
+vox_status vox_voxelize_slice(vox_ctx *ctx, vox_plan_mode mode, double param, vox_node_kind kind,
+                              int use_materials, int direction, uint32_t slice,
+                              const uint32_t vox_split[2], const uint32_t mat_split[2],
+                              void *nodes_dev, uint32_t dims_out[3]);
+
 /* Node array dimensions / count / node size for a plan and node kind
  * (FCC: 2*alloc[0] x alloc[1] x 2*alloc[2], src/host_side.cpp:1494-1499). */
 void vox_node_dims(const vox_grid *grid, vox_node_kind kind, uint32_t dims_out[3]);
@@ -197,6 +238,11 @@ vox_status vox_set_schedule(vox_ctx *ctx, vox_schedule schedule);
  * threshold 1.5. */
 vox_status vox_set_heavy_threshold(vox_ctx *ctx, float threshold);
 vox_status vox_set_screen_ratio(vox_ctx *ctx, float ratio);
+/* At most `parts` (1, 2 or 4) sub-bricks per heavy brick; 0 (default) picks
+ * it from the number of bricks per resident brick-pass workgroup (16 or more:
+ * 1, 4 or more: 2, else 4): a part repeats the brick's triangle setup, which
+ * pays only where one long brick could end the pass last. */
+vox_status vox_set_split_parts(vox_ctx *ctx, uint32_t parts);
 /* Model parameters (host arrays of VOX_COST_PARAMS floats). */
 vox_status vox_cost_set_params(vox_ctx *ctx, const float *params);
 vox_status vox_cost_get_params(vox_ctx *ctx, float *params);
@@ -297,6 +343,36 @@ vox_status vox_graph_launch(vox_graph *graph);
 /* Launches that fell back to a run without the graph. */
 uint32_t vox_graph_fallbacks(const vox_graph *graph);
 void vox_graph_destroy(vox_graph *graph);
+
+/* ---- mesh files and output dumps (host only, no context) -----------------
+ * vox_mesh_load replaces the reference tester's VTK readers and its loadModel
+ * copy loop (tests/src/main.cpp:359-404, 911-955): OBJ, PLY (ASCII and
+ * binary), STL (ASCII and binary) and legacy VTK POLYDATA (ASCII and binary,
+ * DataFile 2.0-5.1) by file extension, vertices as float in file order,
+ * triangles in file order.  Polygons with more than three corners become a
+ * fan (0, k, k+1); cells with fewer are skipped.  STL corners that coincide
+ * exactly are merged into one vertex (first appearance order) and facets that
+ * collapse under the merge are dropped, as VTK's STL reader does by default.
+ * The arrays are malloc'ed; vox_mesh_free releases them.  On failure err
+ * (may be NULL) receives a message. */
+typedef struct vox_mesh {
+    float *vertices;   /* 3 * nv */
+    uint32_t nv;
+    uint32_t *indices; /* 3 * nt */
+    uint32_t nt;
+} vox_mesh;
+vox_status vox_mesh_load(const char *path, vox_mesh *out, char *err, size_t err_len);
+void vox_mesh_free(vox_mesh *mesh);
+/* A node array (dims x, y, z; x fastest) as a legacy VTK STRUCTURED_POINTS
+ * file with the point scalars "bid" and "material" (and "ratio" for
+ * PartialNode), ASCII or big-endian binary; origin may be NULL (zero). */
+vox_status vox_write_nodes_vtk(const char *path, const void *nodes, vox_node_kind kind,
+                               const uint32_t dims[3], const double origin[3], double spacing,
+                               int binary);
+/* A solid voxel bit array (dims x (a multiple of 32), y, z; voxel x of a
+ * word at bit 31 - x % 32) as STRUCTURED_POINTS with the scalar "solid". */
+vox_status vox_write_bits_vtk(const char *path, const uint32_t *bits, const uint32_t dims[3],
+                              const double origin[3], double spacing, int binary);
 
 #ifdef __cplusplus
 }

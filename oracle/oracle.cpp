@@ -249,6 +249,10 @@ struct SolidCtx {
     u32 X, Yext, Zext;  // ext resolution (x already padded/rounded)
     u32 xSplits;
     u32 *bits;
+    // Allocated rows and planes, and the alloc index of ext row / plane 0:
+    // 1 (padding before it), or 0 where another subspace lies before the
+    // partition and ext row 0 is its overlap row (src/voxelizer.cu:1487-1502).
+    u32 Yalloc, Zalloc, oy = 1, oz = 1;
 };
 
 // generateVoxelization for one tile and one x split, per voxel-centre lane
@@ -263,8 +267,8 @@ void fill_tile_lane(const SolidCtx &c, const std::vector<FillSetup> &tris, u32 t
     if (!(vY0 >= sub_min[1] && vY0 < sub_max[1] && vZ0 >= sub_min[2] && vZ0 < sub_max[2]))
         return;
     u32 active = 0, current = UINT32_MAX, bitmask = 0;
-    const u32 resY = c.Yext + 2, resXw = c.X >> 5;
-    const u32 vY = vY0 + 1, vZ = vZ0 + 1;  // single device: no neighbours
+    const u32 resY = c.Yalloc, resXw = c.X >> 5;
+    const u32 vY = vY0 + c.oy, vZ = vZ0 + c.oz;
     for (const FillSetup &fs : tris) {
         u32 voxelX;
         if (!fill_center(fs, c.extMin, c.d, vY0, vZ0, voxelX)) continue;
@@ -349,26 +353,57 @@ void perform_voxelization(const SolidCtx &c, Span ys, Span zs) {
     }
 }
 
-void solid_bits(const float *V, const u32 *I, u32 nt, const vox_grid &g,
-                const double extMin[3], u32 splitY, u32 splitZ, u32 *bits) {
+// Ext space and alloc layout of one partition (one device's subspace of
+// voxelizeWorker, src/host_side.cpp:901-1011); the whole grid is the
+// partition with no neighbours.
+struct PartLayout {
+    u32 Yext, Zext;      // ext rows / planes (partition + overlap rows)
+    u32 Yalloc, Zalloc;  // allocated rows / planes (partition + 2)
+    u32 oy, oz;          // alloc index of ext row / plane 0
+    double extMin[3];    // minVertex + extPartition.min * d
+    bool any_neighbour;
+};
+PartLayout whole_layout(const vox_grid &g, const double extMin[3]) {
+    PartLayout L;
+    L.Yext = g.res[1];
+    L.Zext = g.res[2];
+    L.Yalloc = g.alloc[1];
+    L.Zalloc = g.alloc[2];
+    L.oy = L.oz = 1;
+    for (int a = 0; a < 3; a++) L.extMin[a] = extMin[a];
+    L.any_neighbour = false;
+    return L;
+}
+
+void solid_bits_layout(const float *V, const u32 *I, u32 nt, const vox_grid &g,
+                       const PartLayout &L, u32 splitY, u32 splitZ, u32 *bits) {
     SolidCtx c;
     c.V = V;
     c.I = I;
     c.nt = nt;
-    for (int a = 0; a < 3; a++) c.extMin[a] = extMin[a];
+    for (int a = 0; a < 3; a++) c.extMin[a] = L.extMin[a];
     c.d = g.voxel_length;
     c.X = g.res[0];
-    c.Yext = g.res[1];
-    c.Zext = g.res[2];
+    c.Yext = L.Yext;
+    c.Zext = L.Zext;
     c.xSplits = g.x_splits;
     c.bits = bits;
-    size_t words = (size_t)g.alloc[0] / 32 * g.alloc[1] * g.alloc[2];
+    c.Yalloc = L.Yalloc;
+    c.Zalloc = L.Zalloc;
+    c.oy = L.oy;
+    c.oz = L.oz;
+    size_t words = (size_t)g.alloc[0] / 32 * L.Yalloc * L.Zalloc;
     std::memset(bits, 0, words * 4);
     // extYzSplits over extPartition; subspaces ordered z outer, y inner
     // (src/host_side.cpp:3100-3112).
     std::vector<Span> ys = split_axis_max(c.Yext, splitY), zs = split_axis_max(c.Zext, splitZ);
     for (const Span &z : zs)
         for (const Span &y : ys) perform_voxelization(c, y, z);
+}
+
+void solid_bits(const float *V, const u32 *I, u32 nt, const vox_grid &g,
+                const double extMin[3], u32 splitY, u32 splitZ, u32 *bits) {
+    solid_bits_layout(V, I, nt, g, whole_layout(g, extMin), splitY, splitZ, bits);
 }
 
 // ---------------------------------------------------------------------------
@@ -573,8 +608,13 @@ struct SurfTarget {
     int gridType;          // 0: Cartesian, 1..4: FCC sub-grid
     u32 *keys;
     size_t nkeys;
+    // Cartesian partitions: the writes' y / z adjustment is oy / oz, not 1
+    // (src/voxelizer.cu:3374-3383).
+    int32_t dy = 0, dz = 0;
     size_t index(int32_t x, int32_t y, int32_t z) const {
         if (gridType == 0) {
+            y += dy;
+            z += dz;
             return (size_t)resX * resY * (u32)z + (size_t)resX * (u32)y + (u32)x;
         }
         size_t l = 2 * (size_t)resX, a = l * resY;
@@ -700,8 +740,18 @@ void surface_triangle(const F3 t[3], u32 triIdx, u32 type, F3 m, float d, const 
 
 // calcTriangleClassification + calcOptSurfaceVoxelization for one grid
 // (src/voxelizer.cu:505-724; splits as in src/host_side.cpp:1107-1134).
+void surface_grid_res(const float *V, const u32 *I, u32 nt, const vox_grid &g,
+                      const double extMin[3], const u32 extRes[3], const u32 matSplit[3],
+                      const SurfTarget &st);
 void surface_grid(const float *V, const u32 *I, u32 nt, const vox_grid &g,
                   const double extMin[3], const u32 matSplit[3], const SurfTarget &st) {
+    surface_grid_res(V, I, nt, g, extMin, g.res, matSplit, st);
+}
+// The surface stage over an ext space of extRes voxels (matSplitRes splits
+// the ext partition, src/host_side.cpp:1107-1111).
+void surface_grid_res(const float *V, const u32 *I, u32 nt, const vox_grid &g,
+                      const double extMin[3], const u32 extRes[3], const u32 matSplit[3],
+                      const SurfTarget &st) {
     const F3 m = {(float)extMin[0], (float)extMin[1], (float)extMin[2]};
     const float d = (float)g.voxel_length;
     std::vector<u32> types(nt);
@@ -711,9 +761,9 @@ void surface_grid(const float *V, const u32 *I, u32 nt, const vox_grid &g,
     }
     // The reference sorts by type and runs the 1D, 2D and 3D kernels on the
     // class ranges; with the max-key winner rule only membership matters.
-    std::vector<Span> xs = split_axis_max(g.res[0], matSplit[0]);
-    std::vector<Span> ys = split_axis_max(g.res[1], matSplit[1]);
-    std::vector<Span> zs = split_axis_max(g.res[2], matSplit[2]);
+    std::vector<Span> xs = split_axis_max(extRes[0], matSplit[0]);
+    std::vector<Span> ys = split_axis_max(extRes[1], matSplit[1]);
+    std::vector<Span> zs = split_axis_max(extRes[2], matSplit[2]);
     for (const Span &z : zs)
         for (const Span &y : ys)
             for (const Span &x : xs) {
@@ -730,9 +780,163 @@ void surface_grid(const float *V, const u32 *I, u32 nt, const vox_grid &g,
 }
 
 // ---------------------------------------------------------------------------
+// PartialNode volume ratio: processVoxel<PartialNode> (src/voxelizer.cu:
+// 2997-3033) with calculateVoxelPlaneIntersectionVolume (4220-4307),
+// boxParallelCutVolume (4313-4326), constructPolyhedronFaces (4333-4537),
+// volumeOfPolyhedron (4548-4706), rotX / rotY (4719-4757) and
+// volumeOfPyramid (4764-4785).  FP32 in the reference's operation order.
+// length(float2 v) is sqrtf(v.x*v.x + v.y*v.y) and normalize(v) is
+// v * rsqrtf(dot(v, v)); rsqrtf is taken as 1.0f / sqrtf(x), two correctly
+// rounded operations (CUDA's approximate rsqrtf is not reproduced).
+// ---------------------------------------------------------------------------
+inline F3 add(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline F3 scale(float s, F3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline F3 rot_x(F3 v, float s, float c) { return {v.x, v.y * c - v.z * s, v.y * s + v.z * c}; }
+inline F3 rot_y(F3 v, float s, float c) { return {v.x * c + v.z * s, v.y, v.z * c - v.x * s}; }
+inline float len2(float a, float b) { return std::sqrt(a * a + b * b); }
+inline F3 normalize3(F3 v) {
+    const float inv = 1.0f / std::sqrt(dot(v, v));
+    return {v.x * inv, v.y * inv, v.z * inv};
+}
+
+// Surveyor's formula over the base, times a third of the height.
+float pyramid_volume(const F3 *base, int n, F3 h) {
+    float area = 0.0f;
+    for (int i = 0; i < n; i++) {
+        const F3 a = base[i], b = base[(i + 1) % n];
+        area += 0.5f * (a.x * b.y - b.x * a.y);
+    }
+    return std::fabs((1.0f / 3.0f) * area * (h.z - base[0].z));
+}
+
+// Turns the polygon (and the apex) so that its normal n points along z, then
+// adds its pyramid (the three face polygons pass their own normal).
+float turned_pyramid(F3 *poly, int n, F3 rotN, F3 apex) {
+    F3 h = apex;
+    if (rotN.y != 0.0f) {
+        const float pl = len2(rotN.y, rotN.z), s = rotN.y / pl, c = rotN.z / pl;
+        h = rot_x(h, s, c);
+        for (int i = 0; i < n; i++) poly[i] = rot_x(poly[i], s, c);
+        rotN = rot_x(rotN, s, c);
+    }
+    if (rotN.x != 0.0f) {
+        const float pl = len2(rotN.z, rotN.x), s = -rotN.x / pl, c = rotN.z / pl;
+        h = rot_y(h, s, c);
+        for (int i = 0; i < n; i++) poly[i] = rot_y(poly[i], s, c);
+    }
+    return pyramid_volume(poly, n, h);
+}
+
+// Volume of the part of the voxel at `voxel` (centre m + d * voxel) behind
+// the plane of the triangle with normal n: the reference's exact arithmetic,
+// including its results for normals that do not cut the box.
+float cut_volume(const F3 tri[3], const int32_t voxel[3], F3 n, F3 m, float d) {
+    const F3 vc = add(m, scale(d, F3{float(voxel[0]), float(voxel[1]), float(voxel[2])}));
+    if (std::fabs(n.x) == 1.0f) return (d * d) * ((d / 2.0f) + tri[0].x - vc.x);
+    if (std::fabs(n.y) == 1.0f) return (d * d) * ((d / 2.0f) + tri[0].y - vc.y);
+    if (std::fabs(n.z) == 1.0f) return (d * d) * ((d / 2.0f) + tri[0].z - vc.z);
+    const bool bx = n.x > 0.0f, by = n.y > 0.0f, bz = n.z > 0.0f;
+    auto step = [d](bool b) { return float(!b) * d + float(b) * -d; };
+    F3 dx = {step(bx), 0.0f, 0.0f}, dy, dz;
+    if ((int(bx) + int(by) + int(bz)) % 2 == 0) {
+        dy = {0.0f, step(by), 0.0f};
+        dz = {0.0f, 0.0f, step(bz)};
+    } else {
+        dz = {0.0f, step(by), 0.0f};
+        dy = {0.0f, 0.0f, step(bz)};
+    }
+    const float dh = d / 2.0f;
+    auto half = [dh](bool b) { return dh * float(b) - dh * float(!b); };
+    F3 v[8];
+    v[0] = add(vc, F3{half(bx), half(by), half(bz)});
+    v[1] = add(v[0], dx);
+    v[2] = add(v[0], dy);
+    v[3] = add(v[0], dz);
+    v[4] = add(v[1], dz);
+    v[5] = add(v[2], dx);
+    v[6] = add(v[3], dy);
+    v[7] = add(v[4], dy);
+    // constructPolyhedronFaces: three paths from v0, each edge tested for the
+    // plane; the arrays start zeroed (an empty cut polygon still reads its
+    // first entry's z in volumeOfPyramid).
+    F3 ip[6] = {}, face[3][6] = {};
+    int ni = 0, nf[3] = {0, 0, 0};
+    auto T = [&](int p, int q) { return dot(n, sub(tri[0], v[p])) / dot(n, sub(v[q], v[p])); };
+    auto at = [&](int p, int q, float t) { return add(v[p], scale(t, sub(v[q], v[p]))); };
+    // path k: edges (0,a), (a,b), (b,7) and (a,c); face k, partner face k-1
+    const int pa[3] = {1, 2, 3}, pb[3] = {4, 5, 6}, pc[3] = {5, 6, 4}, partner[3] = {2, 0, 1};
+    for (int k = 0; k < 3; k++) {
+        const int a = pa[k], b = pb[k], c = pc[k], f = k, g = partner[k];
+        float t = T(0, a);
+        if (t < 0.0f || t > 1.0f) {
+            face[f][nf[f]++] = v[a];
+            t = T(a, b);
+            if (t < 0.0f || t > 1.0f) {
+                t = T(b, 7);
+                if (t >= 0.0f && t <= 1.0f) {
+                    ip[ni] = at(b, 7, t);
+                    face[f][nf[f]++] = v[b];
+                    face[f][nf[f]++] = ip[ni];
+                    face[g][nf[g]++] = ip[ni];
+                    face[g][nf[g]++] = v[b];
+                    ni++;
+                } else {
+                    face[f][nf[f]++] = v[b];
+                    face[g][nf[g]++] = v[b];
+                }
+            } else {
+                ip[ni] = at(a, b, t);
+                face[f][nf[f]++] = ip[ni];
+                ni++;
+            }
+        } else {
+            ip[ni] = at(0, a, t);
+            ni++;
+        }
+        t = T(a, c);
+        if (t >= 0.0f && t <= 1.0f) {
+            ip[ni] = at(a, c, t);
+            face[f][nf[f]++] = ip[ni];
+            ni++;
+        }
+    }
+    // volumeOfPolyhedron: pyramids from v0 over the cut polygon and the three
+    // faces (the other faces hold v0 and add nothing).
+    float vol = 0.0f;
+    vol += turned_pyramid(ip, ni, n, v[0]);
+    for (int f = 0; f < 3; f++)
+        if (nf[f] > 2) {
+            const F3 rn = normalize3(cross(sub(face[f][0], face[f][2]), sub(face[f][1], face[f][0])));
+            vol += turned_pyramid(face[f], nf[f], rn, v[0]);
+        }
+    return vol;
+}
+
+// The ratio processVoxel<PartialNode> stores for triangle t (class bb) at the
+// voxel it is handed: the padded node coordinates (x + 1, y + adj.y,
+// z + adj.z) of process{1D,2D,3D}Triangles (3374-3383, 3489-3495), so the
+// box is centred at m + d * (node coordinates).  1D triangles pass a zero
+// normal (3389-3407).  NaN results are stored as the quiet NaN 0x7FC00000.
+float partial_ratio(const float *V, const u32 *I, u32 t, u32 bb, const int32_t node[3], F3 m,
+                    float d) {
+    const F3 tri[3] = {fetch_f(V, I, t, 0), fetch_f(V, I, t, 1), fetch_f(V, I, t, 2)};
+    const F3 n = bb == BB_1D ? F3{0.0f, 0.0f, 0.0f}
+                             : cross(sub(tri[0], tri[2]), sub(tri[1], tri[0]));
+    const float vol = cut_volume(tri, node, n, m, d);
+    float r = std::fabs(1.0f - vol / (d * d * d));
+    if (r != r) {
+        const u32 q = 0x7FC00000u;
+        std::memcpy(&r, &q, 4);
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
 // Nodes.
 // ---------------------------------------------------------------------------
-struct NodeRec { uint16_t bid; uint8_t mat; };
+// r is PartialNode's ratio: float(solid) from constructNodeList2
+// (src/voxelizer.cu:1675), the cut ratio where a triangle wrote the node.
+struct NodeRec { uint16_t bid; uint8_t mat; float r; };
 
 // fillNodeList2 repeated until the error flag stays false, per alloc subspace
 // (src/voxelizer.cu:1791-1913, src/host_side.cpp:1183-1206).  In place, in
@@ -862,6 +1066,15 @@ void pack_nodes(const std::vector<NodeRec> &nodes, int kind, void *out) {
             o[2 * i] = (uint8_t)nodes[i].bid;
             o[2 * i + 1] = nodes[i].mat;
         }
+    } else if (kind == VOX_NODE_PARTIAL) {
+        // {uchar bid, uchar mat, 2 padding bytes (written 0), float ratio}
+        uint8_t *o = (uint8_t *)out;
+        for (size_t i = 0; i < n; i++) {
+            o[8 * i] = (uint8_t)nodes[i].bid;
+            o[8 * i + 1] = nodes[i].mat;
+            o[8 * i + 2] = o[8 * i + 3] = 0;
+            std::memcpy(o + 8 * i + 4, &nodes[i].r, 4);
+        }
     } else if (kind == VOX_NODE_SHORT_FCC) {
         uint16_t *o = (uint16_t *)out;
         for (size_t i = 0; i < n; i++)
@@ -877,42 +1090,155 @@ void pack_nodes(const std::vector<NodeRec> &nodes, int kind, void *out) {
     }
 }
 
+// One device's partition of the grid (voxelizeWorker, src/host_side.cpp:
+// 901-1244): the partition [pmin, pmax) in (y, z) ext coordinates, its
+// neighbour flags (908-950), the ext partition that reaches one row into each
+// neighbour (994-1011) and the alloc layout (partition + 2 rows / planes).
+PartLayout part_layout(const vox_grid &g, const u32 pmin[2], const u32 pmax[2]) {
+    const u32 res[2] = {g.res[1], g.res[2]};
+    bool lo[2] = {false, false}, hi[2] = {false, false};  // (left, down), (right, up)
+    for (int a = 0; a < 2; a++) {
+        if (pmin[a] == 0 && pmax[a] != res[a]) hi[a] = true;
+        else if (pmax[a] == res[a] && pmin[a] != 0) lo[a] = true;
+        else if (pmin[a] != 0 && pmax[a] != res[a]) lo[a] = hi[a] = true;
+    }
+    PartLayout L;
+    const u32 emin[2] = {pmin[0] - (lo[0] ? 1u : 0u), pmin[1] - (lo[1] ? 1u : 0u)};
+    const u32 emax[2] = {pmax[0] + (hi[0] ? 1u : 0u), pmax[1] + (hi[1] ? 1u : 0u)};
+    L.Yext = emax[0] - emin[0];
+    L.Zext = emax[1] - emin[1];
+    L.Yalloc = pmax[0] - pmin[0] + 2;
+    L.Zalloc = pmax[1] - pmin[1] + 2;
+    L.oy = lo[0] ? 0u : 1u;
+    L.oz = lo[1] ? 0u : 1u;
+    L.extMin[0] = g.min_vertex[0] + double(0u) * g.voxel_length;
+    L.extMin[1] = g.min_vertex[1] + double(emin[0]) * g.voxel_length;
+    L.extMin[2] = g.min_vertex[2] + double(emin[1]) * g.voxel_length;
+    L.any_neighbour = lo[0] || hi[0] || lo[1] || hi[1];
+    return L;
+}
+
+// zeroPadding (src/voxelizer.cu:4811-4844), Cartesian nodes.
+void zero_border(std::vector<NodeRec> &nodes, u32 X, u32 Y, u32 Z) {
+#pragma omp parallel for schedule(static)
+    for (long zz = 0; zz < (long)Z; zz++) {
+        const u32 z = (u32)zz;
+        for (u32 y = 0; y < Y; y++)
+            for (u32 x = 0; x < X; x++)
+                if (x == 0 || x == X - 1 || y == 0 || y == Y - 1 || z == 0 || z == Z - 1)
+                    nodes[((size_t)z * Y + y) * X + x] = {0, 0, 0.0f};
+    }
+}
+
+// Nodes of one Cartesian partition before classification: the solid bits and
+// the surface winners of its ext space, written with its alloc offsets.
+// PartialNode (partial): r = float(solid) (constructNodeList2,
+// src/voxelizer.cu:1675), and where a triangle wrote the node, its cut ratio
+// at the node's alloc coordinates.
+std::vector<NodeRec> part_nodes(const float *V, const u32 *I, u32 nt, const uint8_t *mats,
+                                const vox_grid &g, const PartLayout &L, const u32 voxSplit[3],
+                                const u32 matSplit[3], bool partial) {
+    const u32 X = g.alloc[0], Y = L.Yalloc, Z = L.Zalloc;
+    const size_t words = (size_t)X / 32 * Y * Z, N = (size_t)X * Y * Z;
+    std::vector<u32> bits(words);
+    solid_bits_layout(V, I, nt, g, L, voxSplit[1], voxSplit[2], bits.data());
+    std::vector<NodeRec> nodes(N);
+#pragma omp parallel for schedule(static)
+    for (long w = 0; w < (long)words; w++)
+        for (u32 b = 0; b < 32; b++) {
+            const uint16_t bit = (uint16_t)((bits[w] >> (31 - b)) & 1u);
+            nodes[(size_t)w * 32 + b] = {bit, 0, float(bit)};
+        }
+    if (mats) {
+        std::vector<u32> keys(N, 0u);
+        SurfTarget st{X, Y, Z, 0, keys.data(), N};
+        st.dy = (int32_t)L.oy - 1;
+        st.dz = (int32_t)L.oz - 1;
+        const u32 extRes[3] = {g.res[0], L.Yext, L.Zext};
+        surface_grid_res(V, I, nt, g, L.extMin, extRes, matSplit, st);
+        const F3 m = {(float)L.extMin[0], (float)L.extMin[1], (float)L.extMin[2]};
+        const float d = (float)g.voxel_length;
+        // processVoxel stores Node(1, materials[tri]) (src/voxelizer.cu:2988),
+        // PartialNode(1, materials[tri], ratio) (3032).
+#pragma omp parallel for schedule(dynamic, 4096)
+        for (long i = 0; i < (long)N; i++) {
+            if (!keys[i]) continue;
+            const u32 t = keys[i] & 0x3FFFFFFFu;
+            float r = nodes[i].r;
+            if (partial) {
+                const int32_t node[3] = {(int32_t)(i % X), (int32_t)((i / X) % Y),
+                                         (int32_t)(i / ((size_t)X * Y))};
+                r = partial_ratio(V, I, t, keys[i] >> 30, node, m, d);
+            }
+            nodes[i] = {1, mats[t], r};
+        }
+    }
+    return nodes;
+}
+
+// Erosion per alloc subspace, then the padding cleared when the partition
+// has neighbours (src/host_side.cpp:1183-1220).
+void classify_part(std::vector<NodeRec> &nodes, u32 X, u32 Y, u32 Z, const u32 voxSplit[3],
+                   bool any_neighbour, bool literal) {
+    std::vector<Span> ys = split_axis_max(Y, voxSplit[1]), zs = split_axis_max(Z, voxSplit[2]);
+    for (const Span &z : zs)
+        for (const Span &y : ys) {
+            if (literal) erode_inplace(nodes, X, Y, Z, y, z);
+            else erode_parallel(nodes, X, Y, Z, y, z);
+        }
+    if (any_neighbour) zero_border(nodes, X, Y, Z);
+}
+
+// rotateVertices (src/host_side.cpp:3261-3290): (x, y) -> (a - y, b + x) with
+// a = c.y + c.x, b = c.y - c.x and c = (max - min) / 2 of the bounding box
+// (calculateBoundingBox's strict comparisons, 2462-2488).
+std::vector<float> rotate_vertices(const float *V, u32 nv) {
+    std::vector<float> r(V, V + 3 * (size_t)nv);
+    if (nv == 0) return r;
+    double mn[3] = {V[0], V[1], V[2]}, mx[3] = {V[0], V[1], V[2]};
+    for (u32 i = 1; i < nv; i++)
+        for (int c = 0; c < 3; c++) {
+            if (V[3 * i + c] > mx[c]) mx[c] = V[3 * i + c];
+            if (V[3 * i + c] < mn[c]) mn[c] = V[3 * i + c];
+        }
+    const double cx = (mx[0] - mn[0]) / 2.0, cy = (mx[1] - mn[1]) / 2.0;
+    const double a = cy + cx, b = cy - cx;
+    for (u32 i = 0; i < nv; i++) {
+        const double tx = r[3 * i];
+        r[3 * i] = float(a - double(r[3 * i + 1]));
+        r[3 * i + 1] = float(b + tx);
+    }
+    return r;
+}
+
+// unRotateNodes (src/voxelizer.cu:2061-2124), Cartesian: node (x, y, z) of a
+// (X, Y, Z) array goes to (y, X - 1 - x, z) of a (Y, X, Z) array.
+std::vector<NodeRec> unrotate_nodes(const std::vector<NodeRec> &in, u32 X, u32 Y, u32 Z) {
+    std::vector<NodeRec> out(in.size());
+    for (u32 z = 0; z < Z; z++)
+        for (u32 y = 0; y < Y; y++)
+            for (u32 x = 0; x < X; x++)
+                out[(size_t)X * Y * z + (size_t)Y * (X - 1 - x) + y] = in[((size_t)z * Y + y) * X + x];
+    return out;
+}
+
 int voxelize_nodes(const float *V, u32 nv, const u32 *I, u32 nt, const uint8_t *mats,
                    const vox_grid &g, int kind, const u32 voxSplit[3], const u32 matSplit[3],
                    void *out, bool literal) {
     (void)nv;
     const bool fcc = kind == VOX_NODE_SHORT_FCC || kind == VOX_NODE_LONG_FCC;
-    if (kind != VOX_NODE_SHORT && kind != VOX_NODE_LONG && !fcc) return 1;
+    if (kind != VOX_NODE_SHORT && kind != VOX_NODE_LONG && kind != VOX_NODE_PARTIAL && !fcc)
+        return 1;
     const u32 X = g.alloc[0], Y = g.alloc[1], Z = g.alloc[2];
     const size_t words = (size_t)X / 32 * Y * Z;
     std::vector<u32> bits(words);
     const double base[3] = {g.min_vertex[0], g.min_vertex[1], g.min_vertex[2]};
     if (!fcc) {
-        // voxelizeWorker (src/host_side.cpp:901-1244).
-        const size_t N = (size_t)X * Y * Z;
-        solid_bits(V, I, nt, g, base, voxSplit[1], voxSplit[2], bits.data());
-        std::vector<NodeRec> nodes(N);
-        // constructNodeList2 (src/voxelizer.cu:1627-1681).
-#pragma omp parallel for schedule(static)
-        for (long w = 0; w < (long)words; w++)
-            for (u32 b = 0; b < 32; b++)
-                nodes[(size_t)w * 32 + b] = {(uint16_t)((bits[w] >> (31 - b)) & 1u), 0};
-        if (mats) {
-            std::vector<u32> keys(N, 0u);
-            SurfTarget st{X, Y, Z, 0, keys.data(), N};
-            surface_grid(V, I, nt, g, base, matSplit, st);
-            // processVoxel stores Node(1, materials[tri]).
-#pragma omp parallel for schedule(static)
-            for (long i = 0; i < (long)N; i++)
-                if (keys[i]) nodes[i] = {1, mats[keys[i] & 0x3FFFFFFFu]};
-        }
-        // allocYzSplits over allocPartition (src/host_side.cpp:1062-1067).
-        std::vector<Span> ys = split_axis_max(Y, voxSplit[1]), zs = split_axis_max(Z, voxSplit[2]);
-        for (const Span &z : zs)
-            for (const Span &y : ys) {
-                if (literal) erode_inplace(nodes, X, Y, Z, y, z);
-                else erode_parallel(nodes, X, Y, Z, y, z);
-            }
+        // voxelizeWorker (src/host_side.cpp:901-1244) on the whole grid.
+        const PartLayout L = whole_layout(g, base);
+        std::vector<NodeRec> nodes =
+            part_nodes(V, I, nt, mats, g, L, voxSplit, matSplit, kind == VOX_NODE_PARTIAL);
+        classify_part(nodes, X, Y, Z, voxSplit, false, literal);
         pack_nodes(nodes, kind, out);
         return 0;
     }
@@ -936,7 +1262,7 @@ int voxelize_nodes(const float *V, u32 nv, const u32 *I, u32 nt, const uint8_t *
                     size_t w = ((size_t)z * Y + y) * (X / 32) + x / 32;
                     uint16_t bit = (uint16_t)((bits[w] >> (31 - x % 32)) & 1u);
                     size_t idx = (size_t)FX * FY * (2 * z + oz) + (size_t)FX * y + 2 * x + ox;
-                    nodes[idx] = {bit, 0};
+                    nodes[idx] = {bit, 0, 0.0f};
                 }
         }
     }
@@ -950,7 +1276,7 @@ int voxelize_nodes(const float *V, u32 nv, const u32 *I, u32 nt, const uint8_t *
         }
 #pragma omp parallel for schedule(static)
         for (long i = 0; i < (long)N; i++)
-            if (keys[i]) nodes[i] = {1, mats[keys[i] & 0x3FFFFFFFu]};
+            if (keys[i]) nodes[i] = {1, mats[keys[i] & 0x3FFFFFFFu], 0.0f};
     }
     // allocYzSplits computed on the per-grid alloc partition, then z doubled
     // (src/host_side.cpp:1415-1419, 1500-1511).
@@ -1269,6 +1595,8 @@ void fragile_grid(const float *V, const u32 *I, u32 nt, const uint8_t *mats, con
     c.Zext = g.res[2];
     c.xSplits = g.x_splits;
     c.bits = nullptr;
+    c.Yalloc = g.alloc[1];
+    c.Zalloc = g.alloc[2];
     FragileMap fm{flags, N, g.alloc[0], g.alloc[1], g.alloc[2], gridType};
     std::vector<Span> ys = split_axis_max(c.Yext, voxSplit[1]), zs = split_axis_max(c.Zext, voxSplit[2]);
     for (const Span &z : zs)
@@ -1445,7 +1773,7 @@ int orc_erode_nodes(const uint8_t *solid, uint32_t X, uint32_t Y, uint32_t Z, ui
     if (!split_y || !split_z) return 1;
     const size_t N = (size_t)X * Y * Z;
     std::vector<NodeRec> nodes(N);
-    for (size_t i = 0; i < N; i++) nodes[i] = {(uint16_t)(solid[i] ? 1 : 0), 0};
+    for (size_t i = 0; i < N; i++) nodes[i] = {(uint16_t)(solid[i] ? 1 : 0), 0, 0.0f};
     std::vector<Span> ys = split_axis_max(Y, split_y), zs = split_axis_max(Z, split_z);
     for (const Span &z : zs)
         for (const Span &y : ys) {
@@ -1460,7 +1788,7 @@ int orc_fcc_boundaries(const uint8_t *solid, uint32_t FX, uint32_t FY, uint32_t 
                        uint16_t *bid_out) {
     const size_t N = (size_t)FX * FY * FZ;
     std::vector<NodeRec> nodes(N);
-    for (size_t i = 0; i < N; i++) nodes[i] = {(uint16_t)(solid[i] ? 1 : 0), 0};
+    for (size_t i = 0; i < N; i++) nodes[i] = {(uint16_t)(solid[i] ? 1 : 0), 0, 0.0f};
     fcc_boundaries(nodes, FX, FY, FZ, {0, FY}, {0, FZ});
     for (size_t i = 0; i < N; i++) bid_out[i] = nodes[i].bid;
     return 0;
@@ -1516,6 +1844,110 @@ int orc_fragile(const float *vertices, uint32_t nv, const uint32_t *indices, uin
     for (long i = 0; i < (long)N; i++) any += flags[i] != 0;
     counts[7] = any;
     return 0;
+}
+
+// Plain output of one partition: voxelizeWorker returns after the solid
+// stage when no nodes are asked for (src/host_side.cpp:1040-1055).
+int orc_solid_bits_part(const float *vertices, const uint32_t *indices, uint32_t nt,
+                        const vox_grid *grid, uint32_t split_y, uint32_t split_z,
+                        const uint32_t part_min[2], const uint32_t part_max[2], uint32_t *bits) {
+    if (!grid || grid->x_splits == 0 || split_y == 0 || split_z == 0) return 1;
+    const vox_grid &g = *grid;
+    for (int a = 0; a < 2; a++)
+        if (part_min[a] >= part_max[a] || part_max[a] > g.res[1 + a]) return 2;
+    solid_bits_layout(vertices, indices, nt, g, part_layout(g, part_min, part_max), split_y,
+                      split_z, bits);
+    return 0;
+}
+
+int orc_voxelize_part(const float *vertices, uint32_t nv, const uint32_t *indices, uint32_t nt,
+                      const uint8_t *materials, const vox_grid *grid, int node_kind,
+                      const uint32_t vox_split[3], const uint32_t mat_split[3],
+                      const uint32_t part_min[2], const uint32_t part_max[2], void *nodes_out,
+                      int n_threads) {
+    (void)nv;
+    if (node_kind != VOX_NODE_SHORT && node_kind != VOX_NODE_LONG && node_kind != VOX_NODE_PARTIAL)
+        return 1;
+    const vox_grid &g = *grid;
+    for (int a = 0; a < 2; a++)
+        if (part_min[a] >= part_max[a] || part_max[a] > g.res[1 + a]) return 2;
+#ifdef _OPENMP
+    int prev = omp_get_max_threads();
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#endif
+    const PartLayout L = part_layout(g, part_min, part_max);
+    std::vector<NodeRec> nodes =
+        part_nodes(vertices, indices, nt, materials, g, L, vox_split, mat_split,
+                   node_kind == VOX_NODE_PARTIAL);
+    classify_part(nodes, g.alloc[0], L.Yalloc, L.Zalloc, vox_split, L.any_neighbour,
+                  n_threads == 1);
+    pack_nodes(nodes, node_kind, nodes_out);
+#ifdef _OPENMP
+    omp_set_num_threads(prev);
+#endif
+    return 0;
+}
+
+int orc_voxelize_slice(const float *vertices, uint32_t nv, const uint32_t *indices, uint32_t nt,
+                       const uint8_t *materials, int mode, double param, int node_kind,
+                       int direction, uint32_t slice, const uint32_t vox_split[2],
+                       const uint32_t mat_split[2], void *nodes, uint32_t dims[3]) {
+    if (node_kind != VOX_NODE_SHORT && node_kind != VOX_NODE_LONG && node_kind != VOX_NODE_PARTIAL)
+        return 1;
+    if (direction < 0 || direction > 2) return 2;
+    // voxelizeSlice (src/host_side.cpp:1928-2120): voxSplitRes and
+    // matSplitRes are (x, y) and y serves z too.
+    const u32 vs[3] = {vox_split[0], vox_split[1], vox_split[1]};
+    const u32 ms[3] = {mat_split[0], mat_split[1], mat_split[1]};
+    std::vector<float> rot;
+    const float *V = vertices;
+    if (direction == 0) {  // voxelizeEntry rotates the model (703-716)
+        rot = rotate_vertices(vertices, nv);
+        V = rot.data();
+    }
+    vox_grid g;
+    if (orc_plan_grid(V, nv, mode, param, vs[0], &g)) return 3;
+    // The slice (voxelizeEntry, 743-780): y for directions 0 and 1, z for 2;
+    // slice 0 and slices past the grid give zeros (sliceOOB).
+    const int a = direction == 2 ? 1 : 0;
+    bool oob = false;
+    u32 s = slice;
+    if (s == 0 || s > g.res[1 + a]) {
+        s = 0;
+        oob = true;
+    } else {
+        s -= 1;
+    }
+    u32 pmin[2] = {0, 0}, pmax[2] = {g.res[1], g.res[2]};
+    pmin[a] = s;
+    pmax[a] = s + 1;
+    const PartLayout L = part_layout(g, pmin, pmax);
+    const u32 X = g.alloc[0], Y = L.Yalloc, Z = L.Zalloc;
+    std::vector<NodeRec> n =
+        part_nodes(V, indices, nt, materials, g, L, vs, ms, node_kind == VOX_NODE_PARTIAL);
+    if (direction == 0) {
+        // restoreRotatedNodes, then erosion on the un-rotated array of
+        // (Y, X, Z) nodes (1144-1220)
+        n = unrotate_nodes(n, X, Y, Z);
+        classify_part(n, Y, X, Z, vs, L.any_neighbour, false);
+        dims[0] = Y;
+        dims[1] = X;
+    } else {
+        classify_part(n, X, Y, Z, vs, L.any_neighbour, false);
+        dims[0] = X;
+        dims[1] = Y;
+    }
+    dims[2] = Z;
+    if (oob)
+        for (auto &r : n) r = {0, 0, 0.0f};
+    if (nodes) pack_nodes(n, node_kind, nodes);
+    return 0;
+}
+
+float orc_cut_volume(const float tri[9], const int32_t voxel[3], const float normal[3],
+                     const float m[3], float d) {
+    const F3 t[3] = {{tri[0], tri[1], tri[2]}, {tri[3], tri[4], tri[5]}, {tri[6], tri[7], tri[8]}};
+    return cut_volume(t, voxel, F3{normal[0], normal[1], normal[2]}, F3{m[0], m[1], m[2]}, d);
 }
 
 uint32_t orc_tile_overlaps(const float tri[9], const double ext_min[3], double d, uint32_t ext_y,

@@ -74,6 +74,42 @@ int orc_center_crossing(const float tri[9], const double ext_min[3], double d,
 uint32_t orc_tile_overlaps(const float tri[9], const double ext_min[3], double d,
                            uint32_t ext_y, uint32_t ext_z);
 
+/* One device's partition of a Cartesian grid (voxelizeWorker,
+ * src/host_side.cpp:901-1244): part_min/part_max bound the partition in
+ * (y, z) ext coordinates of the plan; its neighbour flags, ext partition (one
+ * row into each neighbour) and alloc layout follow the reference; nodes holds
+ * res[0] x (py + 2) x (pz + 2) nodes, the border cleared when the partition
+ * has a neighbour.  Returns 0 on success. */
+int orc_voxelize_part(const float *vertices, uint32_t nv, const uint32_t *indices, uint32_t nt,
+                      const uint8_t *materials, const vox_grid *grid, int node_kind,
+                      const uint32_t vox_split[3], const uint32_t mat_split[3],
+                      const uint32_t part_min[2], const uint32_t part_max[2], void *nodes,
+                      int n_threads);
+/* Solid bits of one partition (voxelize() with a devConfig: voxelizeWorker
+ * up to its nodeOutput return, src/host_side.cpp:901-1055): alloc[0]/32 x
+ * (py + 2) x (pz + 2) words, ext rows at their alloc positions, the rest zero.
+ * Returns 0 on success. */
+int orc_solid_bits_part(const float *vertices, const uint32_t *indices, uint32_t nt,
+                        const vox_grid *grid, uint32_t split_y, uint32_t split_z,
+                        const uint32_t part_min[2], const uint32_t part_max[2], uint32_t *bits);
+/* One slice (voxelizeSlice / voxelizeSliceToRAM, src/host_side.cpp:
+ * 1928-2262): plan by mode/param (0 maxDimension, 1 cubeLength), direction
+ * 0 x (the model rotated, rotateVertices 3261-3290, and the nodes rotated
+ * back, unRotateNodes src/voxelizer.cu:2061-2124), 1 y, 2 z; slice 1..res
+ * along it (0 or past the grid: all zeros).  vox_split/mat_split are the
+ * reference's (x, y) pairs.  dims receives the node array dimensions;
+ * nodes may be NULL to query them.  Returns 0 on success. */
+int orc_voxelize_slice(const float *vertices, uint32_t nv, const uint32_t *indices, uint32_t nt,
+                       const uint8_t *materials, int mode, double param, int node_kind,
+                       int direction, uint32_t slice, const uint32_t vox_split[2],
+                       const uint32_t mat_split[2], void *nodes, uint32_t dims[3]);
+
+/* calculateVoxelPlaneIntersectionVolume (src/voxelizer.cu:4220-4307 and its
+ * helpers to 4785): the volume the reference computes for the box centred at
+ * m + d * voxel cut by the plane through tri[0] with normal `normal`. */
+float orc_cut_volume(const float tri[9], const int32_t voxel[3], const float normal[3],
+                     const float m[3], float d);
+
 /* Decisions where the original CUDA build could end with different nodes:
  * FP tests whose value lies within eps (relative to the magnitude of their
  * terms; eps64 for the FP64 solid tests, eps32 for the FP32 surface tests)

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 NODE_SHORT, NODE_LONG, NODE_PARTIAL, NODE_SHORT_FCC, NODE_LONG_FCC = range(5)
-NODE_SIZE = {NODE_SHORT: 1, NODE_LONG: 2, NODE_SHORT_FCC: 2, NODE_LONG_FCC: 4}
+NODE_SIZE = {NODE_SHORT: 1, NODE_LONG: 2, NODE_PARTIAL: 8, NODE_SHORT_FCC: 2, NODE_LONG_FCC: 4}
 
 
 class Grid(C.Structure):
@@ -65,6 +65,14 @@ def lib():
         L.orc_tile_overlaps.argtypes = [P, P, C.c_double, C.c_uint32, C.c_uint32]
         L.orc_fragile.argtypes = [P, C.c_uint32, P, C.c_uint32, P, C.POINTER(Grid), C.c_int, P, P,
                                   C.c_double, C.c_double, P, P]
+        L.orc_voxelize_part.argtypes = [P, C.c_uint32, P, C.c_uint32, P, C.POINTER(Grid), C.c_int,
+                                        P, P, P, P, P, C.c_int]
+        L.orc_cut_volume.restype = C.c_float
+        L.orc_cut_volume.argtypes = [P, P, P, P, C.c_float]
+        L.orc_solid_bits_part.argtypes = [P, P, C.c_uint32, C.POINTER(Grid), C.c_uint32,
+                                          C.c_uint32, P, P, P]
+        L.orc_voxelize_slice.argtypes = [P, C.c_uint32, P, C.c_uint32, P, C.c_int, C.c_double,
+                                         C.c_int, C.c_int, C.c_uint32, P, P, P, P]
         _lib = L
     return _lib
 
@@ -216,3 +224,70 @@ def fragile(mesh, grid: Grid, kind: int, use_materials: bool = True,
     if r:
         raise ValueError("orc_fragile failed")
     return flags, dict(zip(FRAGILE_COUNTS, (int(c) for c in counts)))
+
+
+def voxelize_part(mesh, grid: Grid, kind: int, part_min, part_max, use_materials: bool = True,
+                  vox_split=(1024, 512, 512), mat_split=(512, 512, 512), n_threads: int = 0):
+    """Nodes of one partition [part_min, part_max) (y, z ext coordinates):
+    res[0] x (py + 2) x (pz + 2) nodes (orc_voxelize_part)."""
+    v = np.ascontiguousarray(mesh.vertices, np.float32)
+    t = np.ascontiguousarray(mesh.indices, np.uint32)
+    py, pz = part_max[0] - part_min[0], part_max[1] - part_min[1]
+    out = np.zeros(grid.alloc[0] * (py + 2) * (pz + 2) * NODE_SIZE[kind], np.uint8)
+    mats = None
+    if use_materials and mesh.materials is not None:
+        mats = np.ascontiguousarray(mesh.materials, np.uint8)
+    r = lib().orc_voxelize_part(_p(v), v.shape[0], _p(t), t.shape[0],
+                                _p(mats) if mats is not None else None, C.byref(grid), kind,
+                                _p(np.array(vox_split, np.uint32)),
+                                _p(np.array(mat_split, np.uint32)),
+                                _p(np.array(part_min, np.uint32)),
+                                _p(np.array(part_max, np.uint32)), _p(out), n_threads)
+    if r:
+        raise ValueError("orc_voxelize_part failed (%d)" % r)
+    return out
+
+
+def cut_volume(tri, voxel, normal, m, d: float) -> float:
+    """calculateVoxelPlaneIntersectionVolume as the oracle restates it
+    (orc_cut_volume): tri (3, 3) float32, voxel int (3,), normal and m (3,)."""
+    t = np.ascontiguousarray(tri, np.float32).reshape(9)
+    return float(lib().orc_cut_volume(_p(t), _p(np.ascontiguousarray(voxel, np.int32)),
+                                      _p(np.ascontiguousarray(normal, np.float32)),
+                                      _p(np.ascontiguousarray(m, np.float32)), float(d)))
+
+
+def solid_bits_part(mesh, grid: Grid, part_min, part_max, split_yz=(512, 512)) -> np.ndarray:
+    """Solid bits of one partition [part_min, part_max) (y, z ext
+    coordinates): alloc[0]/32 x (py + 2) x (pz + 2) words (orc_solid_bits_part)."""
+    v = np.ascontiguousarray(mesh.vertices, np.float32)
+    t = np.ascontiguousarray(mesh.indices, np.uint32)
+    py, pz = part_max[0] - part_min[0], part_max[1] - part_min[1]
+    bits = np.zeros(grid.alloc[0] // 32 * (py + 2) * (pz + 2), np.uint32)
+    r = lib().orc_solid_bits_part(_p(v), _p(t), t.shape[0], C.byref(grid), split_yz[0],
+                                  split_yz[1], _p(np.array(part_min, np.uint32)),
+                                  _p(np.array(part_max, np.uint32)), _p(bits))
+    if r:
+        raise ValueError("orc_solid_bits_part failed (%d)" % r)
+    return bits
+
+
+def voxelize_slice(mesh, mode: int, param: float, kind: int, direction: int, slice_: int,
+                   use_materials: bool = True, vox_split=(1024, 512), mat_split=(512, 512)):
+    """(nodes, dims) of one slice (orc_voxelize_slice); dims (x, y, z)."""
+    v = np.ascontiguousarray(mesh.vertices, np.float32)
+    t = np.ascontiguousarray(mesh.indices, np.uint32)
+    mats = None
+    if use_materials and mesh.materials is not None:
+        mats = np.ascontiguousarray(mesh.materials, np.uint8)
+    dims = np.zeros(3, np.uint32)
+    vs = np.array(vox_split, np.uint32)
+    ms = np.array(mat_split, np.uint32)
+    args = (_p(v), v.shape[0], _p(t), t.shape[0], _p(mats) if mats is not None else None,
+            mode, float(param), kind, direction, slice_, _p(vs), _p(ms))
+    r = lib().orc_voxelize_slice(*args, None, _p(dims))
+    if r:
+        raise ValueError("orc_voxelize_slice failed (%d)" % r)
+    out = np.zeros(int(dims[0]) * int(dims[1]) * int(dims[2]) * NODE_SIZE[kind], np.uint8)
+    lib().orc_voxelize_slice(*args, _p(out), _p(dims))
+    return out, tuple(int(x) for x in dims)

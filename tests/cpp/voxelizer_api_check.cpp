@@ -2,10 +2,23 @@
 // way a program written against the reference class would.
 //
 //   voxelizer_api_check host
-//       Checks that need no device: getters, getArrayDimensions, the
-//       setMaterials checks, the printed-and-empty results for FCC nodes in
-//       the maxDimension / plain calls, the slicing exceptions, and (when no
-//       device exists) the "no device" exception.  Prints "host ok".
+//       Checks that need no device: getters, getArrayDimensions (also along
+//       x), the setMaterials checks, the printed-and-empty results for FCC
+//       nodes in the maxDimension / plain / slice calls, the devConfig
+//       exceptions, and (when no device exists) the "no device" exception.
+//       Prints "host ok".
+//   voxelizer_api_check dims <mesh.bin> <kind> <mode> <param> <maxX> <alongX>
+//       Prints getArrayDimensions as "dims X Y Z".
+//   voxelizer_api_check parts <mesh.bin> <kind> <plain> <mode> <param>
+//                             <materials> <dcx> <dcy> <vsx> <vsy> <vsz>
+//                             <msx> <msy> <msz> <out.bin>
+//       voxelize / voxelizeToNodes with devConfig (dcx, dcy) inside
+//       simulateMultidevice; writes the number of parts, then per part
+//       dim[3], loc[3] (uint32) and the raw array.
+//   voxelizer_api_check slice <mesh.bin> <kind> <mode> <param> <materials>
+//                             <direction> <slice> <vsx> <vsy> <msx> <msy>
+//                             <to_ram 0|1> <out.bin>
+//       voxelizeSlice / voxelizeSliceToRAM; writes dim[3] and the array.
 //   voxelizer_api_check run <mesh.bin> <node kind 0..4> <plain 0|1>
 //                           <mode 0=maxDimension|1=cubeLength> <param>
 //                           <materials 0|1> <vsx> <vsy> <vsz> <msx> <msy> <msz>
@@ -77,26 +90,37 @@ static int host_checks() {
     // Two x splits (maxInternalXSize 32) round x to a multiple of 64.
     a = vx.getArrayDimensions(0.05, 32u, false);
     EXPECT(a.x == 64);
-    EXPECT(throws([&] { vx.getArrayDimensions(0.05, 1024u, true); }, "not built"));
+    // Along x the box is turned a quarter about z: (x, y) -> (1.5 - y,
+    // x - 0.5), extents 1 x 2 x 0.5, rotated grid 32 x 43 x 13 (x 21 + 2 ->
+    // 32; y 41 + 2; z 11 + 2), reported un-rotated.
+    a = vx.getArrayDimensions(0.05, 1024u, true);
+    EXPECT(a.x == 43 && a.y == 32 && a.z == 13);
 
     Voxelizer<LongFCCNode> fx(v.data(), t.data(), 8, 12);
     a = fx.getArrayDimensions(0.05, 1024u, false);
     EXPECT(a.x == 128 && a.y == 23 && a.z == 26);
+    a = fx.getArrayDimensions(0.05, 1024u, true);
+    EXPECT(a.x == 86 && a.y == 32 && a.z == 26);
     // Requests the reference answers with a message and an empty result.
     EXPECT(fx.voxelizeToNodes(128u).empty());
     EXPECT(fx.voxelize(128u).empty());
     EXPECT(fx.voxelizeToRAM(0.05).ptr == nullptr);
     EXPECT(fx.voxelizeToNodesToRAM(128u).ptr == nullptr);
+    EXPECT(fx.voxelizeSlice(128u, 0, 3u).empty());
+    EXPECT(fx.voxelizeSliceToRAM(128u, 2, 3u).ptr == nullptr);
 
     std::vector<uchar> m(12, 3);
     EXPECT(throws([&] { vx.setMaterials(nullptr, 2); }, "null pointer"));
     EXPECT(throws([&] { vx.setMaterials(m.data(), 9); }, "exceeds"));
     vx.setMaterials(m.data(), 8);  // ShortNode holds 8 materials
-    EXPECT(throws([&] { vx.voxelizeSlice(64u, 1, 3u); }, "not built"));
-    EXPECT(throws([&] { vx.voxelizeToNodes(64u, make_uint2(1, 2)); }, "devConfig"));
+    EXPECT(throws([&] { vx.voxelizeSlice(64u, 1, 3u, 2u); }, "devConfig must be 1"));
 
-    if (vox_device_count() == 0)
+    if (vox_device_count() == 0) {
         EXPECT(throws([&] { vx.voxelizeToNodes(64u); }, "Couldn't find any HIP"));
+        EXPECT(throws([&] { vx.voxelizeToNodes(64u, make_uint2(1, 2)); }, "Couldn't find any HIP"));
+    } else if (vox_device_count() == 1) {
+        EXPECT(throws([&] { vx.voxelizeToNodes(64u, make_uint2(1, 2)); }, "Not enough devices"));
+    }
     std::printf("\n%s\n", failures ? "host FAILED" : "host ok");
     return failures ? 1 : 0;
 }
@@ -153,8 +177,174 @@ static int run(const std::vector<float> &v, const std::vector<uint> &t,
     return 0;
 }
 
+struct Mesh {
+    std::vector<float> v;
+    std::vector<uint> t;
+    std::vector<uchar> m;
+    uint nUnique = 0;
+};
+
+static bool read_mesh(const char *path, Mesh &M) {
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return false;
+    uint32_t hdr[4];
+    if (std::fread(hdr, 4, 4, f) != 4) {
+        std::fclose(f);
+        return false;
+    }
+    M.v.resize(3 * (size_t)hdr[0]);
+    M.t.resize(3 * (size_t)hdr[1]);
+    M.m.resize(hdr[2] ? hdr[1] : 0);
+    M.nUnique = hdr[3];
+    bool ok = std::fread(M.v.data(), 4, M.v.size(), f) == M.v.size() &&
+              std::fread(M.t.data(), 4, M.t.size(), f) == M.t.size() &&
+              std::fread(M.m.data(), 1, M.m.size(), f) == M.m.size();
+    std::fclose(f);
+    return ok;
+}
+
+template <class Node>
+static int dims(const Mesh &M, int mode, double param, uint maxX, bool alongX) {
+    Voxelizer<Node> vx(M.v.data(), M.t.data(), uint(M.v.size() / 3), uint(M.t.size() / 3));
+    const uint3 a = mode == 0 ? vx.getArrayDimensions(uint(param), maxX, alongX)
+                              : vx.getArrayDimensions(param, maxX, alongX);
+    std::printf("dims %u %u %u\n", a.x, a.y, a.z);
+    return 0;
+}
+
+template <class Node>
+static int parts(const Mesh &M, bool plainOnly, int mode, double param, bool mats, uint2 dc,
+                 uint3 vsr, uint3 msr, const char *out) {
+    Voxelizer<Node> vx(M.v.data(), M.t.data(), uint(M.v.size() / 3), uint(M.t.size() / 3));
+    if (!M.m.empty()) vx.setMaterials(M.m.data(), M.nUnique);
+    vx.setMaterialOutput(mats);
+    std::vector<NodePointer<Node> > ps = vx.simulateMultidevice([&] {
+        if (plainOnly)
+            return mode == 0 ? vx.voxelize(uint(param), dc, vsr) : vx.voxelize(param, dc, vsr);
+        return mode == 0 ? vx.voxelizeToNodes(uint(param), dc, vsr, msr)
+                         : vx.voxelizeToNodes(param, dc, vsr, msr);
+    });
+    FILE *f = std::fopen(out, "wb");
+    if (!f) return 1;
+    const uint32_t n = uint32_t(ps.size());
+    std::fwrite(&n, 4, 1, f);
+    int rc = 0;
+    for (NodePointer<Node> &p : ps) {
+        const size_t bytes = plainOnly ? (size_t)p.dim.x / 32 * p.dim.y * p.dim.z * sizeof(VoxInt)
+                                       : (size_t)p.dim.x * p.dim.y * p.dim.z * sizeof(Node);
+        std::vector<unsigned char> host(bytes);
+        if (hipMemcpy(host.data(), p.ptr, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipFree(p.ptr) != hipSuccess)
+            rc = 1;
+        const uint32_t hdr[6] = {p.dim.x, p.dim.y, p.dim.z, p.loc.x, p.loc.y, p.loc.z};
+        std::fwrite(hdr, 4, 6, f);
+        std::fwrite(host.data(), 1, bytes, f);
+    }
+    std::fclose(f);
+    std::printf("parts ok: %u parts\n", n);
+    return rc;
+}
+
+template <class Node>
+static int slice(const Mesh &M, int mode, double param, bool mats, int direction, uint sl,
+                 uint2 vsr, uint2 msr, bool toRam, const char *out) {
+    Voxelizer<Node> vx(M.v.data(), M.t.data(), uint(M.v.size() / 3), uint(M.t.size() / 3));
+    if (!M.m.empty()) vx.setMaterials(M.m.data(), M.nUnique);
+    vx.setMaterialOutput(mats);
+    NodePointer<Node> p;
+    if (toRam)
+        p = mode == 0 ? vx.voxelizeSliceToRAM(uint(param), direction, sl, vsr, msr)
+                      : vx.voxelizeSliceToRAM(param, direction, sl, vsr, msr);
+    else
+        p = (mode == 0 ? vx.voxelizeSlice(uint(param), direction, sl, 1u, vsr, msr)
+                       : vx.voxelizeSlice(param, direction, sl, 1u, vsr, msr))
+                .at(0);
+    const size_t bytes = (size_t)p.dim.x * p.dim.y * p.dim.z * sizeof(Node);
+    std::vector<unsigned char> host(bytes);
+    if (toRam) {
+        std::memcpy(host.data(), p.ptr, bytes);
+        delete[] p.ptr;
+    } else if (hipMemcpy(host.data(), p.ptr, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+               hipFree(p.ptr) != hipSuccess) {
+        std::fprintf(stderr, "hipMemcpy/hipFree failed\n");
+        return 1;
+    }
+    FILE *f = std::fopen(out, "wb");
+    if (!f) return 1;
+    const uint32_t dim[3] = {p.dim.x, p.dim.y, p.dim.z};
+    std::fwrite(dim, 4, 3, f);
+    std::fwrite(host.data(), 1, bytes, f);
+    std::fclose(f);
+    std::printf("slice ok: dim %u %u %u\n", dim[0], dim[1], dim[2]);
+    return 0;
+}
+
+static int sub_command(int argc, char **argv) {
+    const std::string cmd = argv[1];
+    Mesh M;
+    if (argc < 3 || !read_mesh(argv[2], M)) return 2;
+    try {
+        if (cmd == "dims" && argc == 8) {
+            const int kind = std::atoi(argv[3]);
+            const int mode = std::atoi(argv[4]);
+            const double param = std::strtod(argv[5], nullptr);
+            const uint maxX = uint(std::atoi(argv[6]));
+            const bool alongX = std::atoi(argv[7]) != 0;
+            switch (kind) {
+            case 0: return dims<ShortNode>(M, mode, param, maxX, alongX);
+            case 1: return dims<LongNode>(M, mode, param, maxX, alongX);
+            case 3: return dims<ShortFCCNode>(M, mode, param, maxX, alongX);
+            case 4: return dims<LongFCCNode>(M, mode, param, maxX, alongX);
+            }
+            return 2;
+        }
+        if (cmd == "parts" && argc == 17) {
+            const int kind = std::atoi(argv[3]);
+            const bool plainOnly = std::atoi(argv[4]) != 0;
+            const int mode = std::atoi(argv[5]);
+            const double param = std::strtod(argv[6], nullptr);
+            const bool mats = std::atoi(argv[7]) != 0;
+            const uint2 dc = make_uint2(std::atoi(argv[8]), std::atoi(argv[9]));
+            const uint3 vsr = make_uint3(std::atoi(argv[10]), std::atoi(argv[11]), std::atoi(argv[12]));
+            const uint3 msr = make_uint3(std::atoi(argv[13]), std::atoi(argv[14]), std::atoi(argv[15]));
+            switch (kind) {
+            case 0: return parts<ShortNode>(M, plainOnly, mode, param, mats, dc, vsr, msr, argv[16]);
+            case 1: return parts<LongNode>(M, plainOnly, mode, param, mats, dc, vsr, msr, argv[16]);
+            case 2: return parts<PartialNode>(M, plainOnly, mode, param, mats, dc, vsr, msr, argv[16]);
+            case 3: return parts<ShortFCCNode>(M, plainOnly, mode, param, mats, dc, vsr, msr, argv[16]);
+            }
+            return 2;
+        }
+        if (cmd == "slice" && argc == 15) {
+            const int kind = std::atoi(argv[3]);
+            const int mode = std::atoi(argv[4]);
+            const double param = std::strtod(argv[5], nullptr);
+            const bool mats = std::atoi(argv[6]) != 0;
+            const int direction = std::atoi(argv[7]);
+            const uint sl = uint(std::atoi(argv[8]));
+            const uint2 vsr = make_uint2(std::atoi(argv[9]), std::atoi(argv[10]));
+            const uint2 msr = make_uint2(std::atoi(argv[11]), std::atoi(argv[12]));
+            const bool toRam = std::atoi(argv[13]) != 0;
+            switch (kind) {
+            case 0: return slice<ShortNode>(M, mode, param, mats, direction, sl, vsr, msr, toRam, argv[14]);
+            case 1: return slice<LongNode>(M, mode, param, mats, direction, sl, vsr, msr, toRam, argv[14]);
+            case 2: return slice<PartialNode>(M, mode, param, mats, direction, sl, vsr, msr, toRam, argv[14]);
+            case 4: return slice<LongFCCNode>(M, mode, param, mats, direction, sl, vsr, msr, toRam, argv[14]);
+            }
+            return 2;
+        }
+    } catch (const Exception &e) {
+        std::fprintf(stderr, "vox::Exception: %s\n", e.what());
+        return 3;
+    }
+    return 2;
+}
+
 int main(int argc, char **argv) {
     if (argc >= 2 && std::string(argv[1]) == "host") return host_checks();
+    if (argc >= 2 && (std::string(argv[1]) == "dims" || std::string(argv[1]) == "parts" ||
+                      std::string(argv[1]) == "slice"))
+        return sub_command(argc, argv);
     if (argc != 16 || std::string(argv[1]) != "run") {
         std::fprintf(stderr, "usage: %s host | run <mesh.bin> <kind> <plain> <mode> <param> "
                              "<mats> <vsx> <vsy> <vsz> <msx> <msy> <msz> <to_ram> <out.bin>\n",

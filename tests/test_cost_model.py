@@ -191,6 +191,7 @@ def test_learned_sub_bricks_output_identical(gpu_ctx, oracle, kind, brick):
         gpu_ctx.set_schedule(va.SCHEDULE_LEARNED)
         gpu_ctx.set_heavy_threshold(-1e6)
         gpu_ctx.set_screen_ratio(0.0)
+        gpu_ctx.set_split_parts(4)
         gpu_ctx.enable_timing(True)
         learned = gpu_ctx.voxelize_nodes(g, kind)
         used = gpu_ctx.stage_times()["learned_schedule"]
@@ -199,6 +200,7 @@ def test_learned_sub_bricks_output_identical(gpu_ctx, oracle, kind, brick):
         gpu_ctx.set_schedule(va.SCHEDULE_PLAIN)
         gpu_ctx.set_heavy_threshold(1.5)
         gpu_ctx.set_screen_ratio(24.0)
+        gpu_ctx.set_split_parts(0)
         gpu_ctx.set_brick_dims((128, 4, 8))
     assert used == 1
     assert np.array_equal(plain, learned)

@@ -103,13 +103,123 @@ def test_plain_bits_match_oracle(tmp_path, oracle, to_ram):
 
 
 @pytest.mark.gpu
-def test_materials_off_and_partial_node_error(tmp_path, oracle):
+def test_materials_off_and_partial_nodes(tmp_path, oracle):
     m = W.icosphere(seed=5)
     r, op = run(tmp_path, m, 2, va.NODE_LONG, 0, va.PLAN_CUBE_LENGTH, 0.01, False)
     assert r.returncode == 0, r.stdout + r.stderr
     _, got = read_out(op)
     og = oracle.plan_grid(m.vertices, va.PLAN_CUBE_LENGTH, 0.01, 1024)
     assert np.array_equal(got, oracle.voxelize_nodes(m, og, va.NODE_LONG, False))
-    # PartialNode is not built: the class throws vox::Exception.
-    r, _ = run(tmp_path, m, 2, va.NODE_PARTIAL, 0, va.PLAN_CUBE_LENGTH, 0.01, True)
-    assert r.returncode == 3 and "PartialNode" in r.stderr, r.stdout + r.stderr
+    # Voxelizer<PartialNode>: 8-byte nodes with the volume ratio
+    for to_ram in (1, 0):
+        r, op = run(tmp_path, m, 2, va.NODE_PARTIAL, 0, va.PLAN_CUBE_LENGTH, 0.01, True,
+                    to_ram=to_ram)
+        assert r.returncode == 0, r.stdout + r.stderr
+        dim, got = read_out(op)
+        assert dim == tuple(og.alloc)
+        assert np.array_equal(got, oracle.voxelize_nodes(m, og, va.NODE_PARTIAL, True))
+
+
+def rotated_about_z(m):
+    """The model turned a quarter about z as rotateVertices does
+    (src/host_side.cpp:3261-3290), in numpy."""
+    v = m.vertices.astype(np.float64)
+    mn, mx = v.min(0), v.max(0)
+    cx, cy = (mx[0] - mn[0]) / 2.0, (mx[1] - mn[1]) / 2.0
+    r = m.vertices.copy()
+    r[:, 0] = ((cy + cx) - v[:, 1]).astype(np.float32)
+    r[:, 1] = ((cy - cx) + v[:, 0]).astype(np.float32)
+    return r
+
+
+@pytest.mark.parametrize("kind,mode,param,along_x", [
+    (va.NODE_SHORT, va.PLAN_MAX_DIMENSION, 150, 0),
+    (va.NODE_SHORT, va.PLAN_MAX_DIMENSION, 150, 1),
+    (va.NODE_LONG, va.PLAN_CUBE_LENGTH, 0.013, 1),
+    (va.NODE_LONG_FCC, va.PLAN_CUBE_LENGTH, 0.02, 1),
+])
+def test_array_dimensions_match_oracle_plan(tmp_path, oracle, kind, mode, param, along_x):
+    m = W.shoebox_room(seed=2, n_boxes=5, subdiv=2)
+    mp = str(tmp_path / "mesh.bin")
+    write_mesh(mp, m, 1)
+    r = subprocess.run([exe(), "dims", mp, str(kind), str(mode), repr(float(param)), "1024",
+                        str(along_x)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = tuple(int(x) for x in r.stdout.split()[1:4])
+    g = oracle.plan_grid(rotated_about_z(m) if along_x else m.vertices, mode, param, 1024)
+    X, Y, Z = g.alloc
+    exp = (Y, X, Z) if along_x else (X, Y, Z)
+    if kind in (va.NODE_SHORT_FCC, va.NODE_LONG_FCC):
+        exp = (2 * exp[0], exp[1], 2 * exp[2])
+    assert got == exp
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("plain,kind,dc", [(0, va.NODE_LONG, (2, 2)), (0, va.NODE_SHORT, (3, 1)),
+                                           (0, va.NODE_PARTIAL, (1, 2)),
+                                           (1, va.NODE_SHORT, (1, 3))])
+def test_simulated_devconfig_partitions_match_oracle(tmp_path, oracle, plain, kind, dc):
+    """voxelize / voxelizeToNodes with a devConfig inside simulateMultidevice:
+    one array per partition (z outer, y inner), each placed at (0, i % x,
+    i / x), bit-equal to the oracle's partition."""
+    r = W.shoebox_room(seed=6, n_boxes=12, subdiv=3)
+    m = W.Mesh(r.vertices, r.indices, (r.materials % 8).astype(np.uint8), r.name)  # ShortNode
+    mp, op = str(tmp_path / "mesh.bin"), str(tmp_path / "out.bin")
+    write_mesh(mp, m, int(m.materials.max()) + 1)
+    vs, ms = (1024, 40, 30), (512, 512, 512)
+    args = [exe(), "parts", mp, str(kind), str(plain), str(va.PLAN_CUBE_LENGTH), "0.02", "1",
+            str(dc[0]), str(dc[1]), *map(str, vs), *map(str, ms), op]
+    print(" ".join(args[1:]), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    og = oracle.plan_grid(m.vertices, va.PLAN_CUBE_LENGTH, 0.02, vs[0])
+    X = og.alloc[0]
+
+    def bounds(n, k):  # splitResolutionByNrOfParts along one axis
+        b, rest = [None] * k, n
+        for i in range(k, 0, -1):
+            b[i - 1] = (rest - rest // i, rest)
+            rest = b[i - 1][0]
+        return b
+
+    ys, zs = bounds(og.res[1], dc[0]), bounds(og.res[2], dc[1])
+    raw = np.fromfile(op, np.uint8)
+    n = int(raw[:4].view(np.uint32)[0])
+    assert n == dc[0] * dc[1]
+    off = 4
+    for i in range(n):
+        hdr = tuple(int(x) for x in raw[off:off + 24].view(np.uint32))
+        off += 24
+        y, z = i % dc[0], i // dc[0]
+        pmin, pmax = (ys[y][0], zs[z][0]), (ys[y][1], zs[z][1])
+        assert hdr[3:] == (0, y, z)
+        assert hdr[:3] == (X, pmax[0] - pmin[0] + 2, pmax[1] - pmin[1] + 2)
+        if plain:
+            nb = X // 32 * hdr[1] * hdr[2] * 4
+            ref = oracle.solid_bits_part(m, og, pmin, pmax, vs[1:]).view(np.uint8)
+        else:
+            nb = X * hdr[1] * hdr[2] * va.NODE_SIZE[kind]
+            ref = oracle.voxelize_part(m, og, kind, pmin, pmax, True, vs, ms)
+        got = raw[off:off + nb]
+        off += nb
+        assert np.array_equal(got, ref), (i, pmin, pmax)
+    assert off == raw.size
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("direction,slice_,to_ram", [(0, 7, 1), (1, 33, 0), (2, 20, 1), (2, 0, 0)])
+def test_slices_match_oracle(tmp_path, oracle, direction, slice_, to_ram):
+    m = W.shoebox_room(seed=8, n_boxes=10, subdiv=3)
+    mp, op = str(tmp_path / "mesh.bin"), str(tmp_path / "out.bin")
+    write_mesh(mp, m, int(m.materials.max()) + 1)
+    vs, ms = (1024, 1024), (512, 512)
+    args = [exe(), "slice", mp, str(va.NODE_LONG), str(va.PLAN_CUBE_LENGTH), "0.03", "1",
+            str(direction), str(slice_), *map(str, vs), *map(str, ms), str(to_ram), op]
+    print(" ".join(args[1:]), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dim, got = read_out(op)
+    ref, rd = oracle.voxelize_slice(m, va.PLAN_CUBE_LENGTH, 0.03, va.NODE_LONG, direction, slice_,
+                                    True, vs, ms)
+    assert dim == tuple(rd)
+    assert np.array_equal(got, ref)

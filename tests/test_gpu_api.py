@@ -11,16 +11,6 @@ from voxelizer_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def test_partial_node_is_reported_unsupported(gpu_ctx):
-    m = W.icosphere()
-    gpu_ctx.set_mesh(m.vertices, m.indices)
-    gpu_ctx.set_materials(m.materials, 2)
-    g = gpu_ctx.plan_grid(va.PLAN_MAX_DIMENSION, 32)
-    with pytest.raises(va.VoxError) as e:
-        gpu_ctx.voxelize_nodes(g, va.NODE_PARTIAL)
-    assert e.value.status == va.ERR_UNSUPPORTED
-
-
 def test_materials_required_when_requested(gpu_ctx):
     m = W.icosphere()
     gpu_ctx.set_mesh(m.vertices, m.indices)

@@ -61,6 +61,12 @@ class BatchItem(C.Structure):
                 ("nodes_dev", C.c_void_p), ("status", C.c_int)]
 
 
+class MeshC(C.Structure):
+    """vox_mesh."""
+    _fields_ = [("vertices", C.POINTER(C.c_float)), ("nv", C.c_uint32),
+                ("indices", C.POINTER(C.c_uint32)), ("nt", C.c_uint32)]
+
+
 class StageTimes(C.Structure):
     _fields_ = [("plan_ms", C.c_float), ("solid_bin_ms", C.c_float), ("solid_fill_ms", C.c_float),
                 ("surface_ms", C.c_float), ("erode_ms", C.c_float), ("nodes_ms", C.c_float),
@@ -77,13 +83,16 @@ EXPORTED = [
     "vox_create", "vox_destroy", "vox_last_error", "vox_device_count", "vox_sync",
     "vox_build_info", "vox_alloc", "vox_free", "vox_memcpy_h2d", "vox_memcpy_d2h",
     "vox_set_mesh", "vox_set_mesh_device", "vox_set_materials", "vox_set_materials_device",
-    "vox_plan_grid", "vox_plan_grid_host", "vox_solid_bits", "vox_voxelize_nodes", "vox_node_dims",
+    "vox_plan_grid", "vox_plan_grid_host", "vox_solid_bits", "vox_voxelize_nodes",
+    "vox_voxelize_partition", "vox_solid_bits_partition",
+    "vox_voxelize_slice", "vox_node_dims",
     "vox_node_count", "vox_node_size", "vox_enable_timing", "vox_get_stage_times",
-    "vox_set_schedule", "vox_set_heavy_threshold", "vox_set_screen_ratio", "vox_cost_set_params",
+    "vox_set_schedule", "vox_set_heavy_threshold", "vox_set_screen_ratio", "vox_set_split_parts", "vox_cost_set_params",
     "vox_cost_get_params", "vox_cost_record",
     "vox_cost_samples", "vox_cost_train", "vox_mlp_forward", "vox_mlp_grad",
     "vox_voxelize_batch", "vox_set_brick_dims", "vox_get_brick_dims", "vox_autotune_bricks",
     "vox_graph_create", "vox_graph_launch", "vox_graph_fallbacks", "vox_graph_destroy",
+    "vox_mesh_load", "vox_mesh_free", "vox_write_nodes_vtk", "vox_write_bits_vtk",
 ]
 
 _lib = None
@@ -120,6 +129,9 @@ def lib():
     L.vox_plan_grid_host.argtypes = [P, U32, I32, D, U32, C.POINTER(Grid)]
     L.vox_solid_bits.argtypes = [P, C.POINTER(Grid), P, P, P]
     L.vox_voxelize_nodes.argtypes = [P, C.POINTER(Grid), I32, I32, P, P, P]
+    L.vox_voxelize_partition.argtypes = [P, C.POINTER(Grid), I32, I32, P, P, P, P, P]
+    L.vox_solid_bits_partition.argtypes = [P, C.POINTER(Grid), P, P, P, P]
+    L.vox_voxelize_slice.argtypes = [P, I32, D, I32, I32, I32, U32, P, P, P, P]
     L.vox_node_dims.argtypes = [C.POINTER(Grid), I32, P]
     L.vox_node_dims.restype = None
     L.vox_node_count.argtypes = [C.POINTER(Grid), I32]
@@ -132,6 +144,7 @@ def lib():
     L.vox_set_schedule.argtypes = [P, I32]
     L.vox_set_heavy_threshold.argtypes = [P, F]
     L.vox_set_screen_ratio.argtypes = [P, F]
+    L.vox_set_split_parts.argtypes = [P, U32]
     L.vox_cost_set_params.argtypes = [P, P]
     L.vox_cost_get_params.argtypes = [P, P]
     L.vox_cost_record.argtypes = [P, I32]
@@ -146,6 +159,11 @@ def lib():
     L.vox_graph_fallbacks.restype = U32
     L.vox_graph_destroy.argtypes = [P]
     L.vox_graph_destroy.restype = None
+    L.vox_mesh_load.argtypes = [C.c_char_p, C.POINTER(MeshC), C.c_char_p, C.c_size_t]
+    L.vox_mesh_free.argtypes = [C.POINTER(MeshC)]
+    L.vox_mesh_free.restype = None
+    L.vox_write_nodes_vtk.argtypes = [C.c_char_p, P, I32, P, P, D, I32]
+    L.vox_write_bits_vtk.argtypes = [C.c_char_p, P, P, P, D, I32]
     L.vox_set_brick_dims.argtypes = [P, I32, P]
     L.vox_get_brick_dims.argtypes = [P, I32, P]
     L.vox_autotune_bricks.argtypes = [P, C.POINTER(Grid), I32, P, P, P, U32, P, C.POINTER(F)]
@@ -167,6 +185,52 @@ def plan_grid_host(vertices, mode: int, param: float, vox_split_x: int = 1024) -
     if st != 0:
         raise VoxError(st, "vox_plan_grid_host rejected its arguments")
     return g
+
+
+def load_mesh(path: str):
+    """(vertices float32 [nv, 3], indices uint32 [nt, 3]) of an OBJ, PLY, STL
+    or legacy VTK file (vox_mesh_load)."""
+    m = MeshC()
+    err = C.create_string_buffer(512)
+    st = lib().vox_mesh_load(os.fsencode(path), C.byref(m), err, len(err))
+    if st != 0:
+        raise VoxError(st, err.value.decode(errors="replace"))
+    try:
+        v = np.ctypeslib.as_array(m.vertices, shape=(3 * m.nv,)).copy().reshape(-1, 3) \
+            if m.nv else np.zeros((0, 3), np.float32)
+        t = np.ctypeslib.as_array(m.indices, shape=(3 * m.nt,)).copy().reshape(-1, 3) \
+            if m.nt else np.zeros((0, 3), np.uint32)
+    finally:
+        lib().vox_mesh_free(C.byref(m))
+    return v.astype(np.float32, copy=False), t.astype(np.uint32, copy=False)
+
+
+def write_nodes_vtk(path: str, nodes: np.ndarray, kind: int, dims, origin=(0.0, 0.0, 0.0),
+                    spacing: float = 1.0, binary: bool = True):
+    """A node array as a legacy VTK STRUCTURED_POINTS file (vox_write_nodes_vtk)."""
+    nodes = np.ascontiguousarray(nodes)
+    n = int(dims[0]) * int(dims[1]) * int(dims[2])
+    if nodes.nbytes != n * NODE_SIZE[kind]:
+        raise ValueError("node array size does not match dims")
+    st = lib().vox_write_nodes_vtk(os.fsencode(path), _ptr(nodes), kind,
+                                   _ptr(np.array(dims, np.uint32)),
+                                   _ptr(np.array(origin, np.float64)), float(spacing),
+                                   1 if binary else 0)
+    if st != 0:
+        raise VoxError(st, f"cannot write {path}")
+
+
+def write_bits_vtk(path: str, bits: np.ndarray, dims, origin=(0.0, 0.0, 0.0),
+                   spacing: float = 1.0, binary: bool = True):
+    """A solid bit array as a legacy VTK STRUCTURED_POINTS file (vox_write_bits_vtk)."""
+    bits = np.ascontiguousarray(bits, np.uint32)
+    if dims[0] % 32 or bits.size != dims[0] // 32 * dims[1] * dims[2]:
+        raise ValueError("bit array size does not match dims")
+    st = lib().vox_write_bits_vtk(os.fsencode(path), _ptr(bits), _ptr(np.array(dims, np.uint32)),
+                                  _ptr(np.array(origin, np.float64)), float(spacing),
+                                  1 if binary else 0)
+    if st != 0:
+        raise VoxError(st, f"cannot write {path}")
 
 
 def _ptr(a: np.ndarray):
@@ -318,6 +382,60 @@ class Context:
         self._check(lib().vox_voxelize_nodes(self.h, C.byref(grid), kind, 1 if use_materials else 0,
                                              _ptr(vs), _ptr(ms), C.c_void_p(int(out_ptr))))
 
+    def voxelize_partition(self, grid: Grid, kind: int, part_min, part_max, use_materials=True,
+                           vox_split=DEFAULT_VOX_SPLIT, mat_split=DEFAULT_MAT_SPLIT) -> np.ndarray:
+        """Nodes of the partition [part_min, part_max) of the grid's ext (y, z)
+        space: res[0] x (py + 2) x (pz + 2) nodes (vox_voxelize_partition)."""
+        py, pz = part_max[0] - part_min[0], part_max[1] - part_min[1]
+        nbytes = grid.alloc[0] * (py + 2) * (pz + 2) * NODE_SIZE[kind]
+        buf = DeviceBuffer(self, nbytes)
+        self._check(lib().vox_voxelize_partition(
+            self.h, C.byref(grid), kind, 1 if use_materials else 0,
+            _ptr(np.array(vox_split, np.uint32)), _ptr(np.array(mat_split, np.uint32)),
+            _ptr(np.array(part_min, np.uint32)), _ptr(np.array(part_max, np.uint32)), buf.ptr))
+        out = buf.download(np.uint8, nbytes)
+        buf.free()
+        return out
+
+    def solid_bits_partition(self, grid: Grid, part_min, part_max,
+                             split_yz=(512, 512)) -> np.ndarray:
+        """Solid bits of the partition [part_min, part_max) of the grid's ext
+        (y, z) space: alloc[0]/32 x (py + 2) x (pz + 2) words
+        (vox_solid_bits_partition)."""
+        py, pz = part_max[0] - part_min[0], part_max[1] - part_min[1]
+        words = grid.alloc[0] // 32 * (py + 2) * (pz + 2)
+        buf = DeviceBuffer(self, words * 4)
+        self._check(lib().vox_solid_bits_partition(
+            self.h, C.byref(grid), _ptr(np.array(split_yz, np.uint32)),
+            _ptr(np.array(part_min, np.uint32)), _ptr(np.array(part_max, np.uint32)), buf.ptr))
+        out = buf.download(np.uint32, words)
+        buf.free()
+        return out
+
+    def slice_dims(self, mode: int, param: float, kind: int, direction: int, slice_: int = 1,
+                   vox_split=(1024, 512), mat_split=(512, 512)):
+        d = np.zeros(3, np.uint32)
+        self._check(lib().vox_voxelize_slice(self.h, mode, float(param), kind, 0, direction,
+                                             slice_, _ptr(np.array(vox_split, np.uint32)),
+                                             _ptr(np.array(mat_split, np.uint32)), None, _ptr(d)))
+        return tuple(int(x) for x in d)
+
+    def voxelize_slice(self, mode: int, param: float, kind: int, direction: int, slice_: int,
+                       use_materials=True, vox_split=(1024, 512), mat_split=(512, 512)):
+        """(nodes, dims) of one slice (vox_voxelize_slice); dims (x, y, z)."""
+        dims = self.slice_dims(mode, param, kind, direction, slice_, vox_split, mat_split)
+        nbytes = dims[0] * dims[1] * dims[2] * NODE_SIZE[kind]
+        buf = DeviceBuffer(self, nbytes)
+        d = np.zeros(3, np.uint32)
+        self._check(lib().vox_voxelize_slice(self.h, mode, float(param), kind,
+                                             1 if use_materials else 0, direction, slice_,
+                                             _ptr(np.array(vox_split, np.uint32)),
+                                             _ptr(np.array(mat_split, np.uint32)), buf.ptr,
+                                             _ptr(d)))
+        out = buf.download(np.uint8, nbytes)
+        buf.free()
+        return out, dims
+
     def voxelize_nodes(self, grid: Grid, kind: int, use_materials=True,
                        vox_split=DEFAULT_VOX_SPLIT, mat_split=DEFAULT_MAT_SPLIT) -> np.ndarray:
         n = int(lib().vox_node_count(C.byref(grid), kind))
@@ -380,6 +498,10 @@ class Context:
 
     def set_heavy_threshold(self, threshold: float):
         self._check(lib().vox_set_heavy_threshold(self.h, float(threshold)))
+
+    def set_split_parts(self, parts: int):
+        """Sub-bricks per heavy brick at most (1, 2, 4; 0 = automatic)."""
+        self._check(lib().vox_set_split_parts(self.h, int(parts)))
 
     def set_screen_ratio(self, ratio: float):
         """Bricks with at least `ratio` times the mean pair count are scored
@@ -448,4 +570,5 @@ __all__ = ["Context", "Grid", "BatchItem", "DeviceBuffer", "Graph", "VoxError", 
            "plan_grid_host",
            "NODE_SHORT", "NODE_LONG", "NODE_PARTIAL", "NODE_SHORT_FCC", "NODE_LONG_FCC",
            "PLAN_MAX_DIMENSION", "PLAN_CUBE_LENGTH", "NODE_SIZE", "EXPORTED",
-           "SCHEDULE_PLAIN", "SCHEDULE_LEARNED", "COST_FEATURES", "COST_PARAMS"]
+           "SCHEDULE_PLAIN", "SCHEDULE_LEARNED", "COST_FEATURES", "COST_PARAMS",
+           "load_mesh", "write_nodes_vtk", "write_bits_vtk"]

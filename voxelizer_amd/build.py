@@ -15,7 +15,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvoxelizer_amd.so")
-SOURCES = ["vox_api.hip", "vox_kernels.hip", "vox_cost.hip"]
+SOURCES = ["vox_api.hip", "vox_kernels.hip", "vox_cost.hip", "vox_mesh_io.cpp"]
 ARCH = os.environ.get("VOX_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -94,6 +94,9 @@ struct vox_ctx {
     // light bricks cost more than their order gains; above it heavy bricks
     // are missed.
     float screen_ratio = 24.0f;
+    // Sub-bricks per heavy brick at most (1, 2 or 4); 0: from the number of
+    // bricks per resident brick-pass workgroup.
+    uint32_t split_parts = 0;
     bool record = false;
     std::vector<float> rec_feats, rec_cycles;
     // worker contexts of batched runs (same device, own streams and buffers)
@@ -165,8 +168,8 @@ vox_status fail(vox_ctx *ctx, vox_status s, const std::string &msg) {
 
 template <class T>
 vox_status buf(vox_ctx *ctx, const char *name, size_t count, T **out) {
-    Buf &b = ctx->bufs[name];
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    Buf &b = ctx->bufs[name];
     if (bytes > b.cap) {
         if (b.p) {  // either stream may still use the old buffer
             VOX_HIP(hipStreamSynchronize(ctx->stream));
@@ -258,15 +261,20 @@ BrickDims brick_dims(const vox_grid &g, const int bdim[3]) {
 
 // A brick shape the surface and node passes support: powers of two, x a
 // multiple of 64, at most 32 rows (y*z), at most 32 words per z plane (one
-// mask word per brick layer), 256 to kMaxBrickVox voxels (the brick pass
-// finishes 256 voxels per wave step), and (for the Cartesian node pass, whose
-// threads own kNodeZ planes) z a multiple of kNodeZ.
+// mask word per brick layer), at least 4 planes (a layer then never spans
+// two waves of the brick pass's epilogue), 256 to kMaxBrickVox voxels (the
+// brick pass finishes 256 voxels per wave step), and (for the Cartesian node
+// pass, whose threads own kNodeZ planes) z a multiple of kNodeZ.
 bool brick_ok(const uint32_t b[3], bool cart) {
     for (int a = 0; a < 3; a++)
         if (b[a] == 0 || (b[a] & (b[a] - 1)) != 0) return false;
     const uint32_t nvox = b[0] * b[1] * b[2];
-    return b[0] >= 64 && b[1] * b[2] <= 32 && (b[0] / 32) * b[1] <= 32 && nvox >= 256 &&
-           nvox <= (uint32_t)kMaxBrickVox && (!cart || b[2] % kNodeZ == 0);
+    // The brick pass's epilogue gives each wave whole layers: a layer of at
+    // most 256 voxels tiles its 256-voxel steps; a larger one needs bricks at
+    // least 4 deep (each wave then takes a multiple of a layer's steps).
+    return b[0] >= 64 && b[1] * b[2] <= 32 && (b[0] / 32) * b[1] <= 32 &&
+           (b[0] * b[1] <= 256 || b[2] >= 4) && nvox >= 256 && nvox <= (uint32_t)kMaxBrickVox &&
+           (!cart || b[2] % kNodeZ == 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -280,11 +288,14 @@ struct Plan {
     std::vector<SplitTable> splits;  // one per job
     std::vector<uint32_t> tri_b{0}, tile_b{0}, pad_b{0}, brick_b{0}, erode_b{0}, node_b{0};
     std::vector<uint32_t> fcc_first, fcc_b{0};
-    std::vector<size_t> bits_off, smask_off, mat_off;  // into the call's arenas
+    std::vector<size_t> bits_off, surf_off, mat_off;  // into the call's arenas
     std::vector<uint32_t *> ext_bits;                  // caller's bit array or null
-    size_t bits_words = 0, smask_words = 0, mat_bytes = 0;
+    size_t bits_words = 0, surf_words = 0, mat_bytes = 0;
+    // PartialNode jobs: their LongNode arrays and winner-key grids
+    std::vector<size_t> plong_off, key_off, pbits_off;
+    size_t plong_bytes = 0, key_bytes = 0, pbits_words = 0;
     uint32_t maxW = 0;
-    bool any_mats = false;
+    bool any_mats = false, any_partial = false;
 };
 
 // Device copies of a plan's tables.
@@ -299,11 +310,68 @@ struct PlanDev {
     bool solid_split = false;
 };
 
+// Ext space and alloc layout of one grid: the whole plan, or one partition
+// of it (a devConfig subspace or a slice; voxelizeWorker,
+// src/host_side.cpp:901-1011).
+struct Layout {
+    uint32_t res[3];    // ext resolution (x, y, z)
+    uint32_t alloc[3];  // allocated dims
+    uint32_t oy, oz;    // alloc row / plane of ext row / plane 0
+    double em[3];       // ext min vertex (before an FCC sub-grid shift)
+    uint32_t padmask;   // Job::padmask
+    bool zero_border, rot_ids;
+};
+Layout whole_layout(const vox_grid &g) {
+    Layout L;
+    for (int a = 0; a < 3; a++) {
+        L.res[a] = g.res[a];
+        L.alloc[a] = g.alloc[a];
+        L.em[a] = g.min_vertex[a];
+    }
+    L.oy = L.oz = 1;
+    L.padmask = 15u;
+    L.zero_border = L.rot_ids = false;
+    return L;
+}
+// The partition [pmin, pmax) of the plan's ext y/z space: its neighbour flags
+// (a neighbour wherever the partition stops short of the grid's edge,
+// 908-950), the ext partition reaching one row into each neighbour
+// (994-1011), alloc dims partition + 2 and the ext min corner
+// minVertex + extPartition.min * d.
+Layout part_layout(const vox_grid &g, const uint32_t pmin[2], const uint32_t pmax[2]) {
+    bool lo[2] = {false, false}, hi[2] = {false, false};
+    for (int a = 0; a < 2; a++) {
+        const uint32_t r = g.res[1 + a];
+        if (pmin[a] == 0 && pmax[a] != r) hi[a] = true;
+        else if (pmax[a] == r && pmin[a] != 0) lo[a] = true;
+        else if (pmin[a] != 0 && pmax[a] != r) lo[a] = hi[a] = true;
+    }
+    Layout L;
+    const uint32_t emin[2] = {pmin[0] - (lo[0] ? 1u : 0u), pmin[1] - (lo[1] ? 1u : 0u)};
+    const uint32_t emax[2] = {pmax[0] + (hi[0] ? 1u : 0u), pmax[1] + (hi[1] ? 1u : 0u)};
+    L.res[0] = g.res[0];
+    L.res[1] = emax[0] - emin[0];
+    L.res[2] = emax[1] - emin[1];
+    L.alloc[0] = g.alloc[0];
+    L.alloc[1] = pmax[0] - pmin[0] + 2;
+    L.alloc[2] = pmax[1] - pmin[1] + 2;
+    L.oy = lo[0] ? 0u : 1u;
+    L.oz = lo[1] ? 0u : 1u;
+    L.em[0] = g.min_vertex[0] + double(0u) * g.voxel_length;
+    L.em[1] = g.min_vertex[1] + double(emin[0]) * g.voxel_length;
+    L.em[2] = g.min_vertex[2] + double(emin[1]) * g.voxel_length;
+    L.padmask = (lo[0] ? 0u : 1u) | (hi[0] ? 0u : 2u) | (lo[1] ? 0u : 4u) | (hi[1] ? 0u : 8u);
+    L.zero_border = lo[0] || hi[0] || lo[1] || hi[1];
+    L.rot_ids = false;
+    return L;
+}
+
 // matSplitRes subspaces over extPartition (src/host_side.cpp:1107-1111).
-vox_status split_table(vox_ctx *ctx, const vox_grid &g, const uint32_t matSplit[3], SplitTable &st) {
+vox_status split_table(vox_ctx *ctx, const uint32_t res[3], const uint32_t matSplit[3],
+                       SplitTable &st) {
     std::memset(&st, 0, sizeof(st));
     for (int a = 0; a < 3; a++) {
-        std::vector<Span> sp = split_axis_max(g.res[a], matSplit[a]);
+        std::vector<Span> sp = split_axis_max(res[a], matSplit[a]);
         if (sp.empty() || sp.size() > (size_t)kMaxSplits)
             return fail(ctx, VOX_ERR_UNSUPPORTED, "matSplitRes gives too many subspaces");
         st.n[a] = (uint32_t)sp.size();
@@ -315,12 +383,12 @@ vox_status split_table(vox_ctx *ctx, const vox_grid &g, const uint32_t matSplit[
     return VOX_OK;
 }
 
-// Adds one grid; shift is the FCC sub-grid offset (zero for Cartesian).
-// cart: the job gets erosion and Cartesian node blocks.
+// Adds one grid with layout L; shift is the FCC sub-grid offset (zero for
+// Cartesian).  cart: the job gets erosion and Cartesian node blocks.
 vox_status add_job(vox_ctx *ctx, Plan &P, const float *V, const uint32_t *I, uint32_t nt,
-                   const uint8_t *mats, const vox_grid &g, const double shift[3], bool use_mats,
-                   const uint32_t ms[3], const int bdim[3], bool cart, void *nodes, int kind,
-                   uint32_t *ext_bits) {
+                   const uint8_t *mats, const vox_grid &g, const Layout &L, const double shift[3],
+                   bool use_mats, const uint32_t ms[3], const int bdim[3], bool cart, void *nodes,
+                   int kind, uint32_t *ext_bits) {
     const uint32_t X = g.alloc[0], W = X / 32;
     if (W == 0 || (size_t)16 * W * 4 > 65536)
         return fail(ctx, VOX_ERR_UNSUPPORTED, "x resolution beyond the fill kernel's LDS row buffer");
@@ -338,51 +406,72 @@ vox_status add_job(vox_ctx *ctx, Plan &P, const float *V, const uint32_t *I, uin
     J.nt = nt;
     J.tri_base = P.tri_b.back();
     P.tri_b.push_back(J.tri_base + (nt + 255u) / 256u * 256u);
-    J.gd = GridDims{g.alloc[0], g.alloc[1], g.alloc[2], W};
+    J.gd = GridDims{X, L.alloc[1], L.alloc[2], W};
     for (int a = 0; a < 3; a++) {
-        J.res[a] = (int32_t)g.res[a];
-        J.em[a] = g.min_vertex[a] + shift[a];
+        J.res[a] = (int32_t)L.res[a];
+        J.em[a] = L.em[a] + shift[a];
     }
+    J.oy = L.oy;
+    J.oz = L.oz;
+    J.padmask = L.padmask;
+    J.zero_border = L.zero_border ? 1 : 0;
+    J.rot_ids = L.rot_ids ? 1 : 0;
     J.d = g.voxel_length;
-    J.tilesY = (g.res[1] + 3) / 4;
-    J.tilesZ = (g.res[2] + 3) / 4;
+    J.tilesY = (L.res[1] + 3) / 4;
+    J.tilesZ = (L.res[2] + 3) / 4;
     J.tile_base = P.tile_b.back();
     P.tile_b.push_back(J.tile_base + J.tilesY * J.tilesZ);
     J.pad_base = P.pad_b.back();
-    P.pad_b.push_back(J.pad_base + (pad_words(W, g.alloc[1], g.alloc[2]) + 255u) / 256u);
+    P.pad_b.push_back(J.pad_base + (pad_words(W, L.alloc[1], L.alloc[2]) + 255u) / 256u);
     J.m = F3{(float)J.em[0], (float)J.em[1], (float)J.em[2]};
     J.fd = (float)g.voxel_length;
     uint32_t nbt = 1;
     for (int a = 0; a < 3; a++) {
         J.bdim[a] = bdim[a];
-        J.nb[a] = (g.alloc[a] + bdim[a] - 1) / bdim[a];
+        J.nb[a] = (L.alloc[a] + bdim[a] - 1) / bdim[a];
         nbt *= J.nb[a];
     }
     J.use_mats = use_mats ? 1 : 0;
     J.embed = nt < ctx->embed_limit ? 1 : 0;
     J.brick_base = P.brick_b.back();
     P.brick_b.push_back(J.brick_base + (use_mats ? nbt : 0u));
-    J.bdn = brick_dims(g, bdim);
+    vox_grid ga = g;
+    for (int a = 0; a < 3; a++) ga.alloc[a] = L.alloc[a];
+    J.bdn = brick_dims(ga, bdim);
     SplitTable st;
-    VOX_TRY(split_table(ctx, g, ms, st));
+    VOX_TRY(split_table(ctx, L.res, ms, st));
     P.splits.push_back(st);
     J.erode_base = P.erode_b.back();
     J.node_base = P.node_b.back();
     P.erode_b.push_back(J.erode_base + (cart ? erode_blocks(J.gd) : 0u));
     P.node_b.push_back(J.node_base + (cart ? node_blocks(J.gd) : 0u));
-    J.nodes = nodes;
-    J.kind = kind;
-    const size_t words = (size_t)W * g.alloc[1] * g.alloc[2];
+    // PartialNode (kind 2) runs as LongNode into a LongNode array of the
+    // call and is expanded into the caller's array after the node pass.
+    const bool partial = kind == 2;
+    J.nodes = partial ? nullptr : nodes;
+    J.kind = partial ? 1 : kind;
+    J.pnodes = partial ? nodes : nullptr;
+    J.keygrid = nullptr;
+    J.pbits = nullptr;
+    P.plong_off.push_back(P.plong_bytes);
+    P.key_off.push_back(P.key_bytes);
+    P.pbits_off.push_back(P.pbits_words);
+    if (partial) {
+        P.plong_bytes += ((size_t)X * L.alloc[1] * L.alloc[2] * 2u + 255u) & ~(size_t)255u;
+        P.pbits_words += ((size_t)W * L.alloc[1] * L.alloc[2] + 63u) & ~(size_t)63u;
+        if (use_mats) P.key_bytes += (size_t)nbt * (size_t)(bdim[0] * bdim[1] * bdim[2]) * 4u;
+        P.any_partial = true;
+    }
+    const size_t words = (size_t)W * L.alloc[1] * L.alloc[2];
     P.ext_bits.push_back(ext_bits);
     P.bits_off.push_back(P.bits_words);
     if (!ext_bits) P.bits_words += words;
-    P.smask_off.push_back(P.smask_words);
+    P.surf_off.push_back(P.surf_words);
     P.mat_off.push_back(P.mat_bytes);
     if (use_mats) {
-        // One mask word per brick layer (offsets kept 16-byte aligned for the
-        // node pass's 4-layer loads); the material bytes of a brick's
-        // surface words are packed into a block of one byte per brick voxel.
-        P.smask_words += ((size_t)nbt * (size_t)bdim[2] + 3u) & ~(size_t)3u;
+        // Surface bit words in the layout of the solid grid; one material
+        // byte per brick voxel, brick-major.
+        P.surf_words += words;
         P.mat_bytes += (size_t)nbt * (size_t)(bdim[0] * bdim[1] * bdim[2]);
         P.any_mats = true;
     }
@@ -396,16 +485,17 @@ vox_status add_mesh(vox_ctx *ctx, Plan &P, const float *V, const uint32_t *I, ui
                     const uint8_t *mats, const vox_grid &g, vox_node_kind kind, bool use_mats,
                     const uint32_t ms[3], void *nodes) {
     const double zero[3] = {0.0, 0.0, 0.0};
-    if (kind == VOX_NODE_SHORT || kind == VOX_NODE_LONG)
-        return add_job(ctx, P, V, I, nt, mats, g, zero, use_mats, ms, ctx->cart_bdim, true, nodes,
-                       kind == VOX_NODE_SHORT ? 0 : 1, nullptr);
+    if (kind == VOX_NODE_SHORT || kind == VOX_NODE_LONG || kind == VOX_NODE_PARTIAL)
+        return add_job(ctx, P, V, I, nt, mats, g, whole_layout(g), zero, use_mats, ms,
+                       ctx->cart_bdim, true, nodes, (int)kind, nullptr);
     // fccWorker (src/host_side.cpp:1262-1698): four sub-grids shifted by
     // (0,0,0), (h,h,0), (0,h,h), (h,0,h), h = d/2 (1436-1463).
     const double h = g.voxel_length / 2.0;
     const double shifts[4][3] = {{0.0, 0.0, 0.0}, {h, h, 0.0}, {0.0, h, h}, {h, 0.0, h}};
     const uint32_t first = (uint32_t)P.jobs.size();
     for (int k = 0; k < 4; k++)
-        VOX_TRY(add_job(ctx, P, V, I, nt, mats, g, shifts[k], use_mats, ms, ctx->fcc_bdim, false,
+        VOX_TRY(add_job(ctx, P, V, I, nt, mats, g, whole_layout(g), shifts[k], use_mats, ms,
+                        ctx->fcc_bdim, false,
                         k == 0 ? nodes : nullptr, kind == VOX_NODE_SHORT_FCC ? 3 : 4, nullptr));
     P.fcc_first.push_back(first);
     P.fcc_b.push_back(P.fcc_b.back() + fcc_node_blocks(P.jobs[first].gd));
@@ -438,11 +528,15 @@ vox_status upload(vox_ctx *ctx, void *dst, const void *src, size_t bytes) {
 // tables, base arrays, erosion subspace flags).
 vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D, uint32_t splitY, uint32_t splitZ) {
     const uint32_t nj = (uint32_t)P.jobs.size(), nf = (uint32_t)P.fcc_first.size();
-    uint32_t *bits = nullptr, *smask = nullptr, *bhist = nullptr;
-    uint8_t *matgrid = nullptr;
+    uint32_t *bits = nullptr, *surf = nullptr, *bhist = nullptr, *keygrid = nullptr,
+             *pbits = nullptr;
+    uint8_t *matgrid = nullptr, *plong = nullptr;
     if (P.bits_words) VOX_TRY(buf(ctx, "a_bits", P.bits_words, &bits));
+    if (P.plong_bytes) VOX_TRY(buf(ctx, "a_plong", P.plong_bytes, &plong));
+    if (P.key_bytes) VOX_TRY(buf(ctx, "a_keygrid", P.key_bytes / 4u, &keygrid));
+    if (P.pbits_words) VOX_TRY(buf(ctx, "a_pbits", P.pbits_words, &pbits));
     if (P.any_mats) {
-        VOX_TRY(buf(ctx, "a_smask", P.smask_words, &smask));
+        VOX_TRY(buf(ctx, "a_surf", P.surf_words, &surf));
         VOX_TRY(buf(ctx, "a_matgrid", P.mat_bytes, &matgrid));
         VOX_TRY(buf(ctx, "f_brick_hist", P.brick_b.back(), &bhist));
     }
@@ -498,9 +592,14 @@ vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D, uint32_t splitY, uint3
         J.splits = dsplit + k;
         J.sstart = solid_split ? dev + jb + sb + bb + erode_flags : nullptr;
         if (J.use_mats) {
-            J.smask = smask + P.smask_off[k];
+            J.surf = surf + P.surf_off[k];
             J.bhist = bhist + J.brick_base;
             J.matgrid = matgrid + P.mat_off[k];
+        }
+        if (J.pnodes) {
+            J.nodes = plong + P.plong_off[k];
+            J.pbits = pbits + P.pbits_off[k];
+            if (J.use_mats) J.keygrid = keygrid + P.key_off[k] / 4u;
         }
     }
     // One staging copy: jobs, split tables, base arrays, subspace flags.
@@ -783,9 +882,16 @@ vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass 
                                                      cand_off + nslots, ctx->screen_ratio, lcand,
                                                      cnt, skipf);
         VOX_HIP(hipGetLastError());
+        // Cutting a brick repeats its triangle setup in every part; it pays
+        // only where one long brick can end the pass last, i.e. when there
+        // are few bricks per resident workgroup.
+        const uint32_t resident = (uint32_t)(ctx->n_cu * ctx->surf_blocks_per_cu);
+        const uint32_t per_wg = nbt / std::max(resident, 1u);
+        const uint32_t max_parts =
+            ctx->split_parts ? ctx->split_parts : per_wg >= 16u ? 1u : per_wg >= 4u ? 2u : 4u;
         k_cost_predict<<<grid_for(nbt, 4, (uint32_t)ctx->n_cu * 8u), 256, 0, ctx->stream>>>(
-            ctx->cost_params, blist_order, lcand, cnt, ctx->heavy_cost, brick_hist, brick_off,
-            blist, tri_info, tri_ext, D.jobs, D.brick_b, nj, heavy, hsub, skipf);
+            ctx->cost_params, blist_order, lcand, cnt, ctx->heavy_cost, max_parts, brick_hist,
+            brick_off, blist, tri_info, tri_ext, D.jobs, D.brick_b, nj, heavy, hsub, skipf);
         VOX_HIP(hipGetLastError());
         hlist = heavy;
         hsubs = hsub;
@@ -817,11 +923,16 @@ vox_status surface_brick(vox_ctx *ctx, const Plan &P, const PlanDev &D, const Su
     uint64_t *cycles = nullptr;
     if (ctx->record) VOX_TRY(buf(ctx, "f_cycles", nbt, &cycles));
     const uint32_t blocks = std::min<uint32_t>(nbt, (uint32_t)(ctx->n_cu * ctx->surf_blocks_per_cu));
-    k_surf_brick<<<blocks, 256, 0, ctx->stream>>>(D.jobs, D.brick_b, nj, ctx->rec, S.brick_off,
-                                                  S.brick_hist, S.blist,
-                                                  ctx->timing ? S.stats : nullptr, S.blist_order,
-                                                  S.n_list, S.hlist, S.hsub, S.n_heavy, S.skip,
-                                                  S.next, cycles);
+    if (P.any_partial)
+        k_surf_brick<true><<<blocks, 256, 0, ctx->stream>>>(
+            D.jobs, D.brick_b, nj, ctx->rec, S.brick_off, S.brick_hist, S.blist,
+            ctx->timing ? S.stats : nullptr, S.blist_order, S.n_list, S.hlist, S.hsub, S.n_heavy,
+            S.skip, S.next, cycles);
+    else
+        k_surf_brick<false><<<blocks, 256, 0, ctx->stream>>>(
+            D.jobs, D.brick_b, nj, ctx->rec, S.brick_off, S.brick_hist, S.blist,
+            ctx->timing ? S.stats : nullptr, S.blist_order, S.n_list, S.hlist, S.hsub, S.n_heavy,
+            S.skip, S.next, cycles);
     VOX_HIP(hipGetLastError());
     if (ctx->timing) {
         uint32_t vox = 0;
@@ -930,6 +1041,13 @@ vox_status node_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, const uint3
             D.jobs, D.fcc_first, D.fcc_b, nf);
         VOX_HIP(hipGetLastError());
     }
+    for (uint32_t j = 0; j < nj; j++) {
+        const Job &J = P.jobs[j];
+        if (!J.pnodes) continue;
+        const size_t n = (size_t)J.gd.X * J.gd.Y * J.gd.Z;
+        k_nodes_partial<<<grid_for(n, 256, 16384), 256, 0, ctx->stream>>>(D.jobs, j);
+        VOX_HIP(hipGetLastError());
+    }
     return VOX_OK;
 }
 
@@ -981,6 +1099,12 @@ vox_status run_stages(vox_ctx *ctx, Plan &P, const PlanDev &D, uint32_t splitY,
         VOX_HIP(hipMemcpyAsync(D.pre, J.bits, (size_t)J.gd.W * J.gd.Y * J.gd.Z * 4,
                                hipMemcpyDeviceToDevice, ctx->stream));
     }
+    // PartialNode keeps r = float(solid) of the nodes erosion removes: the
+    // bits as they were before erosion.
+    for (const Job &J : P.jobs)
+        if (J.pbits)
+            VOX_HIP(hipMemcpyAsync(J.pbits, J.bits, (size_t)J.gd.W * J.gd.Y * J.gd.Z * 4,
+                                   hipMemcpyDeviceToDevice, ctx->stream));
     VOX_TRY(erode_stage(ctx, P, D, splitY, splitZ));
     record(ctx, 3);
     VOX_TRY(node_stage(ctx, P, D, D.pre, D.ystart, D.zstart));
@@ -1038,7 +1162,7 @@ vox_status vox_create(int device, vox_ctx **out) {
     }
     if (hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) !=
             hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->surf_blocks_per_cu, k_surf_brick, 256,
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->surf_blocks_per_cu, k_surf_brick<false>, 256,
                                                      0) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->erode_blocks_per_cu, k_erode, 256, 0) !=
             hipSuccess) {
@@ -1370,7 +1494,8 @@ vox_status vox_solid_bits(vox_ctx *ctx, const vox_grid *g, const double shift[3]
     // performVoxelization's extYzSplits); the material split is unused here.
     const uint32_t ms[3] = {512u, 512u, 512u};
     Plan P;
-    VOX_TRY(add_job(ctx, P, ctx->V, ctx->I, ctx->nt, nullptr, *g, shift ? shift : zero, false, ms,
+    VOX_TRY(add_job(ctx, P, ctx->V, ctx->I, ctx->nt, nullptr, *g, whole_layout(*g),
+                    shift ? shift : zero, false, ms,
                     ctx->cart_bdim, false, nullptr, 0, bits_dev));
     PlanDev D;
     record(ctx, 0);
@@ -1387,16 +1512,24 @@ vox_status vox_solid_bits(vox_ctx *ctx, const vox_grid *g, const double shift[3]
 }
 
 static bool node_kind_ok(vox_node_kind kind) {
-    return kind == VOX_NODE_SHORT || kind == VOX_NODE_LONG || kind == VOX_NODE_SHORT_FCC ||
-           kind == VOX_NODE_LONG_FCC;
+    return kind == VOX_NODE_SHORT || kind == VOX_NODE_LONG || kind == VOX_NODE_PARTIAL ||
+           kind == VOX_NODE_SHORT_FCC || kind == VOX_NODE_LONG_FCC;
 }
+
+static vox_status voxelize_nodes_impl(vox_ctx *ctx, const vox_grid *g, vox_node_kind kind,
+                                      int use_materials, const uint32_t vox_split[3],
+                                      const uint32_t mat_split[3], void *nodes_dev);
 
 vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *g, vox_node_kind kind,
                               int use_materials, const uint32_t vox_split[3],
                               const uint32_t mat_split[3], void *nodes_dev) {
     if (!ctx || !grid_ok(g) || !nodes_dev) return VOX_ERR_INVALID;
-    if (kind == VOX_NODE_PARTIAL)
-        return fail(ctx, VOX_ERR_UNSUPPORTED, "PartialNode (volume ratio) is not built");
+    return voxelize_nodes_impl(ctx, g, kind, use_materials, vox_split, mat_split, nodes_dev);
+}
+
+static vox_status voxelize_nodes_impl(vox_ctx *ctx, const vox_grid *g, vox_node_kind kind,
+                                      int use_materials, const uint32_t vox_split[3],
+                                      const uint32_t mat_split[3], void *nodes_dev) {
     if (!node_kind_ok(kind)) return VOX_ERR_INVALID;
     if (use_materials && !ctx->mats)
         return fail(ctx, VOX_ERR_INVALID, "materials requested but none set");
@@ -1411,6 +1544,206 @@ vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *g, vox_node_kind kin
     VOX_TRY(add_mesh(ctx, P, ctx->V, ctx->I, ctx->nt, ctx->mats, *g, kind, use_materials != 0, ms,
                      nodes_dev));
     VOX_TRY(run_plan(ctx, P, vs[1], vs[2]));
+    return finish_times(ctx);
+}
+
+// ---- partitions and slices ----------------------------------------------------
+
+namespace {
+// rotateVertices (src/host_side.cpp:3261-3290): (x, y, z) ->
+// (float(a - y), float(b + x), z), evaluated in double.
+__global__ void k_rotate_vertices(const float *__restrict__ V, uint32_t nv, double a, double b,
+                                  float *__restrict__ out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x) {
+        const float x = V[3u * i], y = V[3u * i + 1u], z = V[3u * i + 2u];
+        out[3u * i] = float(a - double(y));
+        out[3u * i + 1u] = float(b + double(x));
+        out[3u * i + 2u] = z;
+    }
+}
+
+// unRotateNodes (src/voxelizer.cu:2061-2124) for Cartesian nodes of `size`
+// bytes: node (x, y, z) of an X x Y x Z array goes to (y, X - 1 - x, z) of a
+// Y x X x Z array.
+__global__ void k_unrotate_nodes(const uint8_t *__restrict__ in, uint8_t *__restrict__ out,
+                                 uint32_t X, uint32_t Y, uint32_t Z, uint32_t size) {
+    const size_t n = (size_t)X * Y * Z;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t x = (uint32_t)(i % X), y = (uint32_t)((i / X) % Y);
+        const size_t z = i / ((size_t)X * Y);
+        const size_t o = (size_t)X * Y * z + (size_t)Y * (X - 1u - x) + y;
+        for (uint32_t k = 0; k < size; k++) out[o * size + k] = in[i * size + k];
+    }
+}
+
+// Bounding box of n device vertices (calculateBoundingBox).
+vox_status device_bbox(vox_ctx *ctx, const float *V, uint32_t nv, double mn[3], double mx[3]) {
+    uint32_t *bb;
+    VOX_TRY(buf(ctx, "p_bbox", 6, &bb));
+    VOX_TRY(fill(ctx, bb, 3, UINT32_MAX));
+    VOX_TRY(fill(ctx, bb + 3, 3, 0u));
+    k_bbox<<<grid_for(nv, 256, 256), 256, 0, ctx->stream>>>(V, nv, bb);
+    VOX_HIP(hipGetLastError());
+    uint32_t h[6];
+    VOX_TRY(read_u32(ctx, bb, 6, h));
+    for (int c = 0; c < 3; c++) {
+        uint32_t a = h[c], b = h[3 + c];
+        uint32_t ua = (a & 0x80000000u) ? (a & 0x7FFFFFFFu) : ~a;
+        uint32_t ub = (b & 0x80000000u) ? (b & 0x7FFFFFFFu) : ~b;
+        float fa, fb;
+        std::memcpy(&fa, &ua, 4);
+        std::memcpy(&fb, &ub, 4);
+        mn[c] = fa;
+        mx[c] = fb;
+    }
+    return VOX_OK;
+}
+}  // namespace
+
+vox_status vox_voxelize_partition(vox_ctx *ctx, const vox_grid *g, vox_node_kind kind,
+                                  int use_materials, const uint32_t vox_split[3],
+                                  const uint32_t mat_split[3], const uint32_t part_min[2],
+                                  const uint32_t part_max[2], void *nodes_dev) {
+    if (!ctx || !grid_ok(g) || !nodes_dev || !part_min || !part_max) return VOX_ERR_INVALID;
+    if (kind != VOX_NODE_SHORT && kind != VOX_NODE_LONG && kind != VOX_NODE_PARTIAL)
+        return fail(ctx, VOX_ERR_UNSUPPORTED, "partitions are built for Cartesian nodes");
+    for (int a = 0; a < 2; a++)
+        if (part_min[a] >= part_max[a] || part_max[a] > g->res[1 + a])
+            return fail(ctx, VOX_ERR_INVALID, "partition outside the grid");
+    if (use_materials && !ctx->mats)
+        return fail(ctx, VOX_ERR_INVALID, "materials requested but none set");
+    const uint32_t vs[3] = {vox_split ? vox_split[0] : 1024u, vox_split ? vox_split[1] : 512u,
+                            vox_split ? vox_split[2] : 512u};
+    const uint32_t ms[3] = {mat_split ? mat_split[0] : 512u, mat_split ? mat_split[1] : 512u,
+                            mat_split ? mat_split[2] : 512u};
+    if (!vs[1] || !vs[2] || !ms[0] || !ms[1] || !ms[2]) return VOX_ERR_INVALID;
+    VOX_HIP(hipSetDevice(ctx->dev));
+    ctx->times = vox_stage_times{};
+    const double zero[3] = {0.0, 0.0, 0.0};
+    Plan P;
+    VOX_TRY(add_job(ctx, P, ctx->V, ctx->I, ctx->nt, ctx->mats, *g, part_layout(*g, part_min, part_max),
+                    zero, use_materials != 0, ms, ctx->cart_bdim, true, nodes_dev,
+                    (int)kind, nullptr));
+    VOX_TRY(run_plan(ctx, P, vs[1], vs[2]));
+    return finish_times(ctx);
+}
+
+vox_status vox_solid_bits_partition(vox_ctx *ctx, const vox_grid *g, const uint32_t split_yz[2],
+                                    const uint32_t part_min[2], const uint32_t part_max[2],
+                                    uint32_t *bits_dev) {
+    if (!ctx || !grid_ok(g) || !bits_dev || !part_min || !part_max) return VOX_ERR_INVALID;
+    if (!ctx->V && ctx->nv) return fail(ctx, VOX_ERR_INVALID, "no mesh");
+    for (int a = 0; a < 2; a++)
+        if (part_min[a] >= part_max[a] || part_max[a] > g->res[1 + a])
+            return fail(ctx, VOX_ERR_INVALID, "partition outside the grid");
+    const uint32_t sy = split_yz ? split_yz[0] : 512u, sz = split_yz ? split_yz[1] : 512u;
+    if (sy == 0 || sz == 0) return VOX_ERR_INVALID;
+    VOX_HIP(hipSetDevice(ctx->dev));
+    ctx->times = vox_stage_times{};
+    const double zero[3] = {0.0, 0.0, 0.0};
+    const uint32_t ms[3] = {512u, 512u, 512u};
+    Plan P;
+    VOX_TRY(add_job(ctx, P, ctx->V, ctx->I, ctx->nt, nullptr, *g,
+                    part_layout(*g, part_min, part_max), zero, false, ms, ctx->cart_bdim, false,
+                    nullptr, 0, bits_dev));
+    PlanDev D;
+    record(ctx, 0);
+    VOX_TRY(plan_upload(ctx, P, D, sy, sz));
+    VOX_TRY(records_stage(ctx, P, D));
+    VOX_TRY(solid_stage(ctx, P, D));
+    record(ctx, 1);
+    if (ctx->timing) {
+        VOX_HIP(hipEventSynchronize(ctx->ev[1]));
+        (void)hipEventElapsedTime(&ctx->times.solid_fill_ms, ctx->ev[0], ctx->ev[1]);
+        ctx->times.total_ms = ctx->times.solid_fill_ms;
+    }
+    return VOX_OK;
+}
+
+vox_status vox_voxelize_slice(vox_ctx *ctx, vox_plan_mode mode, double param, vox_node_kind kind,
+                              int use_materials, int direction, uint32_t slice,
+                              const uint32_t vox_split[2], const uint32_t mat_split[2],
+                              void *nodes_dev, uint32_t dims_out[3]) {
+    if (!ctx || !dims_out || direction < 0 || direction > 2) return VOX_ERR_INVALID;
+    if (kind != VOX_NODE_SHORT && kind != VOX_NODE_LONG && kind != VOX_NODE_PARTIAL)
+        return fail(ctx, VOX_ERR_UNSUPPORTED, "slices are built for Cartesian nodes");
+    if (ctx->nv == 0) return fail(ctx, VOX_ERR_INVALID, "no vertices");
+    // voxelizeSlice takes voxSplitRes and matSplitRes as (x, y); y serves z
+    // too (src/host_side.cpp:1963-1970).
+    const uint32_t vx = vox_split ? vox_split[0] : 1024u, vy = vox_split ? vox_split[1] : 512u;
+    const uint32_t mx_ = mat_split ? mat_split[0] : 512u, my = mat_split ? mat_split[1] : 512u;
+    const uint32_t ms[3] = {mx_, my, my};
+    if (!vx || !vy || !mx_ || !my || !plan_args_ok(mode, param, vx)) return VOX_ERR_INVALID;
+    if (use_materials && !ctx->mats)
+        return fail(ctx, VOX_ERR_INVALID, "materials requested but none set");
+    VOX_HIP(hipSetDevice(ctx->dev));
+    ctx->times = vox_stage_times{};
+    // The model: for slices along x, rotated a quarter turn about z first
+    // (voxelizeEntry, src/host_side.cpp:703-716), around the centre
+    // (max - min) / 2 of its bounding box.
+    const float *V = ctx->V;
+    double mn[3], mx[3];
+    VOX_TRY(device_bbox(ctx, V, ctx->nv, mn, mx));
+    if (direction == 0) {
+        const double cx = (mx[0] - mn[0]) / 2.0, cy = (mx[1] - mn[1]) / 2.0;
+        float *rv;
+        VOX_TRY(buf(ctx, "slice_rot_v", (size_t)3 * ctx->nv, &rv));
+        k_rotate_vertices<<<grid_for(ctx->nv, 256, 4096), 256, 0, ctx->stream>>>(
+            V, ctx->nv, cy + cx, cy - cx, rv);
+        VOX_HIP(hipGetLastError());
+        V = rv;
+        VOX_TRY(device_bbox(ctx, V, ctx->nv, mn, mx));
+    }
+    vox_grid g;
+    plan_from_bbox(mn, mx, mode, param, vx, &g);
+    if (!grid_ok(&g)) return fail(ctx, VOX_ERR_INVALID, "grid too large");
+    // The slice (voxelizeEntry, 743-780): a one-row partition along y
+    // (directions 0 and 1) or z; slice 0 and slices past the grid give zeros.
+    const int a = direction == 2 ? 1 : 0;
+    const bool oob = slice == 0 || slice > g.res[1 + a];
+    const uint32_t srow = oob ? 0u : slice - 1u;
+    uint32_t pmin[2] = {0u, 0u}, pmax[2] = {g.res[1], g.res[2]};
+    pmin[a] = srow;
+    pmax[a] = srow + 1u;
+    Layout L = part_layout(g, pmin, pmax);
+    const uint32_t X = g.alloc[0];
+    if (direction == 0) {
+        dims_out[0] = L.alloc[1];
+        dims_out[1] = X;
+    } else {
+        dims_out[0] = X;
+        dims_out[1] = L.alloc[1];
+    }
+    dims_out[2] = L.alloc[2];
+    if (!nodes_dev) return VOX_OK;
+    const size_t size = vox_node_size(kind);
+    const size_t bytes = (size_t)dims_out[0] * dims_out[1] * dims_out[2] * size;
+    if (oob) return fill(ctx, (uint32_t *)nodes_dev, bytes / 4, 0u);  // bytes: a multiple of 32
+    const double zero[3] = {0.0, 0.0, 0.0};
+    Plan P;
+    if (direction == 0) {
+        // The erosion runs on the un-rotated array in the reference, with
+        // its y (the rotated x) split by voxSplitRes.y; computed here in the
+        // rotated frame, which needs that axis whole.
+        if (vy < X)
+            return fail(ctx, VOX_ERR_UNSUPPORTED,
+                        "x slices need voxSplitRes.y at least the rotated grid's x size");
+        L.rot_ids = true;
+        uint8_t *rot;
+        VOX_TRY(buf(ctx, "slice_rot_nodes", bytes, &rot));
+        VOX_TRY(add_job(ctx, P, V, ctx->I, ctx->nt, ctx->mats, g, L, zero, use_materials != 0, ms,
+                        ctx->cart_bdim, true, rot, (int)kind, nullptr));
+        // one erosion subspace across the rotated rows: vy >= X > the 3 rows
+        VOX_TRY(run_plan(ctx, P, vy, vy));
+        k_unrotate_nodes<<<grid_for(bytes / size, 256, 4096), 256, 0, ctx->stream>>>(
+            rot, (uint8_t *)nodes_dev, X, L.alloc[1], L.alloc[2], (uint32_t)size);
+        VOX_HIP(hipGetLastError());
+    } else {
+        VOX_TRY(add_job(ctx, P, V, ctx->I, ctx->nt, ctx->mats, g, L, zero, use_materials != 0, ms,
+                        ctx->cart_bdim, true, nodes_dev, (int)kind, nullptr));
+        VOX_TRY(run_plan(ctx, P, vy, vy));
+    }
     return finish_times(ctx);
 }
 
@@ -1569,6 +1902,8 @@ vox_status vox_voxelize_batch(vox_ctx *ctx, vox_batch_item *items, uint32_t n,
         vox_ctx *w = ctx->workers[k];
         w->schedule = schedule;
         w->heavy_cost = ctx->heavy_cost;
+        w->screen_ratio = ctx->screen_ratio;
+        w->split_parts = ctx->split_parts;
         w->timing = false;
         for (int a = 0; a < 3; a++) {
             w->cart_bdim[a] = ctx->cart_bdim[a];
@@ -1722,6 +2057,7 @@ vox_status vox_graph_create(vox_ctx *ctx, const vox_batch_item *items, uint32_t 
     vox_ctx *w = g->w;
     w->heavy_cost = ctx->heavy_cost;
     w->screen_ratio = ctx->screen_ratio;
+    w->split_parts = ctx->split_parts;
     w->embed_limit = ctx->embed_limit;
     for (int a = 0; a < 3; a++) {
         w->cart_bdim[a] = ctx->cart_bdim[a];
@@ -1787,6 +2123,13 @@ vox_status vox_set_heavy_threshold(vox_ctx *ctx, float threshold) {
 vox_status vox_set_screen_ratio(vox_ctx *ctx, float ratio) {
     if (!ctx || !(ratio >= 0.0f)) return VOX_ERR_INVALID;
     ctx->screen_ratio = ratio;
+    return VOX_OK;
+}
+
+vox_status vox_set_split_parts(vox_ctx *ctx, uint32_t parts) {
+    if (!ctx || (parts != 0u && parts != 1u && parts != 2u && parts != 4u))
+        return VOX_ERR_INVALID;
+    ctx->split_parts = parts;
     return VOX_OK;
 }
 

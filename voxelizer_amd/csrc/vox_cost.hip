@@ -378,7 +378,7 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ P, const float
 // `heavy`) go on a short list that the brick pass works through first, one
 // entry per grab.  A brick predicted more than kSplitStep above `heavy` is cut
 // into 2 sub-bricks along z, more than 2 kSplitStep into 4 (at most its
-// depth): each entry carries its layer range (hsub: first | end << 16), so
+// depth and max_parts): each entry carries its layer range (hsub: first | end << 16), so
 // the workgroups that take them share the brick's work (voxels of different
 // layers are disjoint, so the split does not change the winners).  The
 // plain list follows in index order (neighbouring bricks share triangles and
@@ -428,7 +428,8 @@ __global__ __launch_bounds__(256) void k_cost_heavy(const float *__restrict__ Y,
 __global__ __launch_bounds__(256) void k_cost_predict(
     const float *__restrict__ P, const uint32_t *__restrict__ order,
     const uint32_t *__restrict__ cand, uint32_t *__restrict__ cnt, float heavy,
-    const uint32_t *__restrict__ brick_hist, const uint32_t *__restrict__ brick_off,
+    uint32_t max_parts, const uint32_t *__restrict__ brick_hist,
+    const uint32_t *__restrict__ brick_off,
     const uint32_t *__restrict__ blist, const uint4 *__restrict__ tri_info,
     const uint4 *__restrict__ tri_ext, const Job *__restrict__ jobs,
     const uint32_t *__restrict__ brick_bases, uint32_t njobs, uint32_t *__restrict__ hlist,
@@ -455,7 +456,7 @@ __global__ __launch_bounds__(256) void k_cost_predict(
         if (j == 0 && y > heavy) {  // false for NaN
             const uint32_t depth = (uint32_t)jobs[job_of(brick_bases, njobs, b)].bdim[2];
             uint32_t parts = y > heavy + 2.0f * kSplitStep ? 4u : y > heavy + kSplitStep ? 2u : 1u;
-            parts = min(parts, depth);  // depths are powers of two
+            parts = min(min(parts, max_parts), depth);  // all powers of two
             const uint32_t base = atomicAdd(&cnt[1], parts);
             for (uint32_t k = 0; k < parts; k++) {
                 hlist[base + k] = b;

@@ -30,6 +30,7 @@ __device__ __forceinline__ VK_G T *gptr(T *p) {
 }
 // Native 4 x u32 vector for 16-byte global accesses through VK_G pointers.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 // Workgroups are dispatched round-robin over the XCDs.  Row kernels launch
 // kXcds * ceil(n / kXcds) workgroups and map workgroup i to logical block
@@ -110,6 +111,7 @@ struct SurfParams {
     uint32_t alloc[3];
     int32_t bdim[3];  // brick dims in voxels
     uint32_t nb[3];   // bricks per axis
+    int32_t off[3];   // alloc coordinate of ext coordinate 0 (1, Job::oy, Job::oz)
 };
 
 // One grid to voxelize: a mesh on a Cartesian grid, or one of the four
@@ -133,6 +135,14 @@ struct Job {
     uint32_t tilesY, tilesZ;  // 4x4 tiles of voxel-centre rows
     uint32_t tile_base;    // first tile (one fill workgroup per tile)
     uint32_t pad_base;     // first block of the pad-row clear
+    // Alloc layout of the ext space (a partition of a grid, as the
+    // reference's devConfig / slice subspaces, src/host_side.cpp:901-1011):
+    // alloc row / plane of ext row / plane 0 (1 behind a padding row, 0 when
+    // the first ext row is the overlap row of a neighbouring partition).
+    uint32_t oy, oz;
+    uint32_t padmask;      // alloc rows / planes that are padding: 1 y = 0, 2 y = Y-1, 4 z = 0, 8 z = Z-1
+    int32_t zero_border;   // clear the node array's border after classification
+    int32_t rot_ids;       // ids of a model rotated about z (x slices): -x/+x <- -y/+y, -y/+y <- +x/-x
     // voxSplitRes y/z subspaces of the solid stage when there are several:
     // sstart[y] (y < res[1]) and sstart[res[1] + z] (z < res[2]) are 1 where
     // a subspace other than the first starts; null for one subspace.
@@ -145,15 +155,22 @@ struct Job {
     uint32_t brick_base;   // first brick
     BrickDims bdn;         // brick geometry for the node passes
     const SplitTable *splits;  // matSplitRes intervals
-    uint32_t *smask;       // per brick layer: mask of the words holding a surface voxel
+    uint32_t *surf;        // surface bit words (the layout of bits)
     const uint32_t *bhist; // per brick: (triangle, brick) pairs; 0 = not visited
-    uint8_t *matgrid;      // per brick: material bytes of its surface words, packed
+    uint8_t *matgrid;      // per voxel, brick-major: winner material (surface words)
     int32_t use_mats;      // surface stage and materials wanted
     int32_t embed;         // winner keys carry the material (mesh-local index < 2^22)
     // erosion and nodes (Cartesian jobs; FCC sub-grids have no such blocks)
     uint32_t erode_base, node_base;
     void *nodes;
     int32_t kind;          // 0 ShortNode, 1 LongNode, 3 ShortFCCNode, 4 LongFCCNode
+    // PartialNode grids run as LongNode (nodes: a LongNode array) and are
+    // expanded into pnodes afterwards; keygrid holds, laid out as matgrid
+    // with 4 bytes per voxel, the winner key of every voxel of the surface
+    // words.
+    uint32_t *keygrid;
+    void *pnodes;
+    uint32_t *pbits;       // PartialNode: the solid|surface bits before erosion
 };
 
 // Job owning item x of a pass whose jobs start at bases[0..n) (ascending;
@@ -184,6 +201,9 @@ __device__ __forceinline__ SurfParams surf_params(const Job &J) {
     p.alloc[0] = J.gd.X;
     p.alloc[1] = J.gd.Y;
     p.alloc[2] = J.gd.Z;
+    p.off[0] = 1;
+    p.off[1] = (int32_t)J.oy;
+    p.off[2] = (int32_t)J.oz;
     return p;
 }
 
@@ -285,7 +305,8 @@ struct SolidParams {
     uint32_t smin[2], smax[2];  // y/z extent of the centres, ext coords
     uint32_t tilesY;            // (ext y + 3) / 4
     uint32_t X;                 // alloc x
-    uint32_t rowsY;             // alloc y (= ext y + 2)
+    uint32_t rowsY;             // alloc y
+    uint32_t oy, oz;            // alloc row / plane of ext row / plane 0
 };
 __device__ __forceinline__ SolidParams solid_params(const Job &J, const float4 *rec) {
     SolidParams p;
@@ -301,6 +322,8 @@ __device__ __forceinline__ SolidParams solid_params(const Job &J, const float4 *
     p.tilesY = J.tilesY;
     p.X = J.gd.X;
     p.rowsY = J.gd.Y;
+    p.oy = J.oy;
+    p.oz = J.oz;
     return p;
 }
 
@@ -375,6 +398,8 @@ __global__ void k_surf_bin(const Job *__restrict__ jobs, const uint32_t *__restr
                            uint32_t *__restrict__ brick_hist, const uint32_t *__restrict__ brick_off,
                            uint32_t *__restrict__ brick_cursor, uint32_t *__restrict__ blist,
                            int scatter, uint32_t cap, uint32_t *__restrict__ ovf);
+// kKeys: jobs with a keygrid also get their winner keys there (PartialNode).
+template <bool kKeys>
 __global__ void k_surf_brick(const Job *__restrict__ jobs, const uint32_t *__restrict__ brick_bases,
                              uint32_t njobs, const float4 *__restrict__ tri_rec,
                              const uint32_t *__restrict__ brick_off,
@@ -435,28 +460,30 @@ __host__ __device__ inline IdPlanes orient_planes(uint32_t mx, uint32_t px, uint
     return o;
 }
 
-// Surface data of one grid as the node passes read it.  A brick layer is one
-// z plane of a brick: wpl = (bx / 32) * by words (at most 32) in (y, x-word)
-// order.  smask[b * bz + l] has bit j set when word j of layer l of brick b
-// holds a surface voxel.  Brick b owns bytes [b * nvox, (b + 1) * nvox) of
-// matgrid; layer l's part starts at byte l * wpl * 32 and holds, for each of
-// its surface words in word order, the word's 32 material bytes (the
-// winner's material for surface voxels, 0 for the others).  Bricks the
-// surface pass did not visit (bhist == 0) hold stale masks, so the readers
-// check bhist first.
+// Surface data of one grid as the node passes read it.  Brick b's voxels
+// occupy bytes [b * nvox, (b + 1) * nvox) of matgrid in the brick's word
+// order (z, y, x-word), 32 bytes per word.  A word that holds a surface
+// voxel has all 32 bytes written in the call (0 for its other voxels); the
+// rest may be stale.  Bricks the surface pass did not visit (bhist == 0)
+// hold stale surface words, so the readers check bhist first.
 struct MatSource {
-    const VK_G uint32_t *smask;  // per brick layer: surface-word mask
+    const VK_G uint32_t *surf;   // surface bit words (same layout as bits)
     const VK_G uint32_t *bhist;  // per brick: nonzero if visited
     const VK_G uint8_t *matgrid;
 };
 __device__ __forceinline__ MatSource mat_source(const Job &J) {
-    return MatSource{gptr(J.smask), gptr(J.bhist), gptr(J.matgrid)};
+    return MatSource{gptr(J.surf), gptr(J.bhist), gptr(J.matgrid)};
 }
-// Byte offset, within brick b's block, of the material bytes of word wl of
-// layer l whose layer mask is m (the word must hold a surface voxel).
-__device__ __forceinline__ uint32_t packed_word_offset(uint32_t l, uint32_t lwl, uint32_t wl,
-                                                       uint32_t m) {
-    return ((l << lwl) + (uint32_t)__popc(m & ((1u << wl) - 1u))) << 5;
+// Index of word (w, y, z) of a grid within its brick's block of words, and
+// the brick: brick-major, words in (z, y, x-word) order inside a brick.
+__device__ __forceinline__ uint32_t brick_of_word(const BrickDims &bd, uint32_t w, uint32_t y,
+                                                  uint32_t z) {
+    return ((z >> bd.lbz) * bd.nby + (y >> bd.lby)) * bd.nbx + (w >> bd.lwx);
+}
+__device__ __forceinline__ uint32_t word_in_brick(const BrickDims &bd, uint32_t w, uint32_t y,
+                                                  uint32_t z) {
+    return (((z & ((1u << bd.lbz) - 1u)) << (bd.lby + bd.lwx)) +
+            ((y & ((1u << bd.lby) - 1u)) << bd.lwx)) + (w & ((1u << bd.lwx) - 1u));
 }
 
 // Node passes.  Cartesian: every Cartesian job's row blocks (grid = all);
@@ -472,6 +499,11 @@ __global__ void k_nodes_cart_pre(const Job *__restrict__ jobs,
                                  const uint32_t *__restrict__ pre,
                                  const uint8_t *__restrict__ ystart,
                                  const uint8_t *__restrict__ zstart);
+// PartialNode expansion of job j (after the node pass): each LongNode of
+// its nodes array becomes {bid, mat, 0, 0, ratio} in pnodes, the ratio 1 or 0
+// for a solid or empty node and the winner triangle's cut ratio for a
+// surface node (0 on a cleared partition border).
+__global__ void k_nodes_partial(const Job *__restrict__ jobs, uint32_t j);
 inline uint32_t node_blocks(const GridDims &g) {
     return plane_chunks(g.W, g.Y) * ((g.Z + kNodeZ - 1u) / kNodeZ);
 }
@@ -518,7 +550,8 @@ __global__ void k_cost_heavy(const float *__restrict__ Y, const uint32_t *__rest
 // launch (one wave per brick); the learned schedule's path.
 __global__ void k_cost_predict(const float *__restrict__ P, const uint32_t *__restrict__ order,
                                const uint32_t *__restrict__ cand, uint32_t *__restrict__ cnt,
-                               float heavy, const uint32_t *__restrict__ brick_hist,
+                               float heavy, uint32_t max_parts,
+                               const uint32_t *__restrict__ brick_hist,
                                const uint32_t *__restrict__ brick_off,
                                const uint32_t *__restrict__ blist,
                                const uint4 *__restrict__ tri_info, const uint4 *__restrict__ tri_ext,

@@ -545,7 +545,7 @@ __global__ __launch_bounds__(kWave) void k_solid_fill(const Job *__restrict__ jo
     for (uint32_t r = 0; r < 16u; r++) {
         const uint32_t cy = ty * 4u + (r & 3u), cz = tz * 4u + (r >> 2);
         if (cy < p.smin[0] || cy >= p.smax[0] || cz < p.smin[1] || cz >= p.smax[1]) continue;
-        const size_t rowIdx = (size_t)(cz + 1u) * p.rowsY + (cy + 1u);
+        const size_t rowIdx = (size_t)(cz + p.oz) * p.rowsY + (cy + p.oy);
         uint32_t carry = 0u;
         for (uint32_t w0 = 0; w0 < W; w0 += kWave) {
             const uint32_t w = w0 + lane;
@@ -573,16 +573,20 @@ __global__ __launch_bounds__(256) void k_zero_pad_rows(const Job *__restrict__ j
     const uint32_t i = (blockIdx.x - J.pad_base) * 256u + threadIdx.x;
     if (i >= pad_words(W, rowsY, rowsZ)) return;
     const uint32_t r = i / W, w = i % W;
-    uint32_t y, z;
+    uint32_t y, z, bit;
     if (r < 2u * rowsZ) {
         z = r >> 1;
         y = (r & 1u) ? rowsY - 1u : 0u;
+        bit = (r & 1u) ? 2u : 1u;
     } else {
         const uint32_t q = r - 2u * rowsZ;
         y = q >> 1;
         z = (q & 1u) ? rowsZ - 1u : 0u;
+        bit = (q & 1u) ? 8u : 4u;
     }
-    gptr(J.bits)[((size_t)z * rowsY + y) * W + w] = 0u;
+    // Rows and planes that are a neighbouring partition's overlap hold
+    // voxelized data (written by the fill) instead of padding.
+    if (J.padmask & bit) gptr(J.bits)[((size_t)z * rowsY + y) * W + w] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -614,15 +618,15 @@ __device__ __forceinline__ int32_t pick3i(const I3 &p, int a) {
     return a == AX_X ? p.x : a == AX_Y ? p.y : p.z;
 }
 
-// Bricks are laid over alloc coordinates (ext + 1).
+// Bricks are laid over alloc coordinates (ext + off).
 __device__ __forceinline__ void brick_range(const SurfParams &p, const VoxBB &vb, uint32_t lo[3],
                                             uint32_t hi[3]) {
-    lo[0] = (uint32_t)(vb.min.x + 1) / (uint32_t)p.bdim[0];
-    lo[1] = (uint32_t)(vb.min.y + 1) / (uint32_t)p.bdim[1];
-    lo[2] = (uint32_t)(vb.min.z + 1) / (uint32_t)p.bdim[2];
-    hi[0] = (uint32_t)(vb.max.x + 1) / (uint32_t)p.bdim[0];
-    hi[1] = (uint32_t)(vb.max.y + 1) / (uint32_t)p.bdim[1];
-    hi[2] = (uint32_t)(vb.max.z + 1) / (uint32_t)p.bdim[2];
+    lo[0] = (uint32_t)(vb.min.x + p.off[0]) / (uint32_t)p.bdim[0];
+    lo[1] = (uint32_t)(vb.min.y + p.off[1]) / (uint32_t)p.bdim[1];
+    lo[2] = (uint32_t)(vb.min.z + p.off[2]) / (uint32_t)p.bdim[2];
+    hi[0] = (uint32_t)(vb.max.x + p.off[0]) / (uint32_t)p.bdim[0];
+    hi[1] = (uint32_t)(vb.max.y + p.off[1]) / (uint32_t)p.bdim[1];
+    hi[2] = (uint32_t)(vb.max.z + p.off[2]) / (uint32_t)p.bdim[2];
 }
 
 // Per triangle: class and dominant axis (classifyTriangles), and the box of
@@ -990,7 +994,9 @@ __device__ __forceinline__ void surf_item(const SurfParams &p, const SplitTable 
 // into the brick's block of the material grid (one byte per voxel, brick
 // order), and the rows' keys are cleared for the next brick.  Surface bits
 // are ORed into the solid grid.  With `stats`, surface voxels are counted;
-// with `cycles`, each brick's duration (s_memtime) is recorded.
+// with `cycles`, each brick's duration (s_memtime) is recorded.  The kKeys
+// variant also stores the winner keys of jobs with a keygrid (PartialNode).
+template <bool kKeys>
 __global__ __launch_bounds__(256, 5) void k_surf_brick(
     const Job *__restrict__ jobs, const uint32_t *__restrict__ brick_bases, uint32_t njobs,
     const float4 *__restrict__ tri_rec, const uint32_t *__restrict__ brick_off,
@@ -1004,7 +1010,6 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
     __shared__ SurfTri tris[kSurfChunk];
     __shared__ uint32_t colsum[kSurfChunk + 1];
     __shared__ uint32_t wsurf[kMaxBrickVox / 32];
-    __shared__ uint32_t lmask[32];  // per brick layer: words holding a surface voxel
     __shared__ uint8_t touched[32];
     __shared__ uint32_t s_q, s_qe;
     __shared__ SplitTable spl;
@@ -1075,20 +1080,21 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
             const SurfParams p = surf_params(J);
             const VK_G uint8_t *__restrict__ materials = gptr(J.mats);
             const bool embed = J.embed != 0;
-            VK_G uint32_t *__restrict__ bits = gptr(J.bits);
             const uint32_t b = gb - J.brick_base;  // brick within the job
             const int32_t bd0 = p.bdim[0], bd1 = p.bdim[1], bd2 = p.bdim[2];
             const int32_t nvox = bd0 * bd1 * bd2;
             const int32_t z0 = (int32_t)(sub & 0xFFFFu), z1 = min((int32_t)(sub >> 16), bd2);
             const int32_t wpr = bd0 / 32;     // words per row
-            const int32_t nwords = nvox / 32;
             const int32_t row_shift = 31 - __clz(bd0);
             const uint32_t bxi = b % p.nb[0], byi = (b / p.nb[0]) % p.nb[1],
                            bzi = b / (p.nb[0] * p.nb[1]);
-            // Bricks are laid over alloc coordinates; ext = alloc - 1.
-            const int32_t blx = (int32_t)(bxi * bd0) - 1, bly = (int32_t)(byi * bd1) - 1,
-                          blz = (int32_t)(bzi * bd2) - 1;
-            // This thread's brick word (for the OR into the solid grid), early.
+            // Bricks are laid over alloc coordinates; ext = alloc - off.
+            const int32_t blx = (int32_t)(bxi * bd0) - p.off[0],
+                          bly = (int32_t)(byi * bd1) - p.off[1],
+                          blz = (int32_t)(bzi * bd2) - p.off[2];
+            const int32_t nwords = nvox / 32;
+            // This thread's brick word (for the OR into the solid grid), early;
+            // a sub-brick owns the words of its layers only.
             size_t gi = 0;
             bool gword = false;
             uint32_t bw = 0u;
@@ -1098,10 +1104,11 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
                 const uint32_t aw = bxi * (uint32_t)wpr + (uint32_t)wx;
                 const uint32_t ay = byi * (uint32_t)bd1 + (uint32_t)ly;
                 const uint32_t az = bzi * (uint32_t)bd2 + (uint32_t)lz;
-                gword = aw < p.alloc[0] / 32u && ay < p.alloc[1] && az < p.alloc[2];
+                gword = lz >= z0 && lz < z1 && aw < p.alloc[0] / 32u && ay < p.alloc[1] &&
+                        az < p.alloc[2];
                 if (gword) {
                     gi = ((size_t)az * p.alloc[1] + ay) * (p.alloc[0] / 32u) + aw;
-                    bw = bits[gi];
+                    bw = gptr(J.bits)[gi];
                 }
             }
             const uint32_t off = __builtin_amdgcn_readfirstlane(brick_off[gb]);
@@ -1138,83 +1145,64 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
             // Surface bit words and winner materials, 256 voxels per wave
             // step: each lane takes 4 consecutive voxels (one 16-byte LDS
             // read of their keys), the 8 lanes of a 32-voxel word OR their
-            // nibbles together, and each lane keeps its 4 material bytes as
-            // one dword (the winner's material, 0 for the other voxels).
-            // Steps whose rows received no key only zero their surface
-            // words.  Keys are cleared for the next brick.  With materials in
-            // the keys the loop issues no global loads.
-            const uint32_t tmask = (uint32_t)__ballot(lane < 32u && touched[lane & 31u] != 0);
-            const uint32_t rows_per_step = 256u >> row_shift;  // bd0 >= 64: 1, 2 or 4 rows
-            constexpr int kSteps = kMaxBrickVox / 1024;  // wave steps per brick, at most
-            uint32_t mdw[kSteps];
-#pragma unroll
-            for (int st = 0; st < kSteps; st++) {
-                const int32_t v0 = (int32_t)wave * 256 + st * 1024;
-                mdw[st] = 0u;
-                if (v0 >= nvox) continue;  // uniform
-                const uint32_t rmask = ((1u << rows_per_step) - 1u) << (v0 >> row_shift);
-                if (!(tmask & rmask)) {
-                    if (lane < 8u) wsurf[(v0 >> 5) + (int32_t)lane] = 0u;
-                    continue;
-                }
-                uint4 *kp = (uint4 *)&keys[v0 + 4 * (int32_t)lane];
-                const uint4 kv = *kp;
-                const uint32_t nib = (kv.x ? 8u : 0u) | (kv.y ? 4u : 0u) | (kv.z ? 2u : 0u) |
-                                     (kv.w ? 1u : 0u);
-                uint32_t w = nib << (28u - 4u * (lane & 7u));
-                w |= (uint32_t)__shfl_xor((int)w, 1);
-                w |= (uint32_t)__shfl_xor((int)w, 2);
-                w |= (uint32_t)__shfl_xor((int)w, 4);
-                if ((lane & 7u) == 0u) wsurf[(v0 >> 5) + (int32_t)(lane >> 3)] = w;
-                nsurf += (uint32_t)__popc(nib);
-                if (embed) {  // uniform
-                    mdw[st] = (kv.x & 0xFFu) | (kv.y & 0xFFu) << 8 | (kv.z & 0xFFu) << 16 |
-                              (kv.w & 0xFFu) << 24;
-                } else {
-                    auto mat = [&](uint32_t key) -> uint32_t {
-                        return key && materials ? (uint32_t)materials[key & 0x3FFFFFFFu] : 0u;
-                    };
-                    mdw[st] = mat(kv.x) | mat(kv.y) << 8 | mat(kv.z) << 16 | mat(kv.w) << 24;
-                }
-                if (nib) *kp = make_uint4(0u, 0u, 0u, 0u);
-            }
-            __syncthreads();
-            // Per brick layer (one z plane of the brick, wpl = words per
-            // layer <= 32, a power of two), the mask of its words that hold a
-            // surface voxel: one ballot per 64 words, a layer never straddles
-            // two waves.  Surface bits are ORed into the solid grid.
-            const int32_t wpl = wpr * bd1;
-            if ((int32_t)(wave * kWave) < nwords) {  // the waves that hold words (uniform)
-                const uint32_t sw = (int32_t)tid < nwords ? wsurf[tid] : 0u;
-                const unsigned long long bal = __ballot(sw != 0u);
-                if ((int32_t)tid < nwords && ((int32_t)tid & (wpl - 1)) == 0) {
-                    const uint32_t lm =
-                        (uint32_t)(bal >> lane) & (wpl == 32 ? 0xFFFFFFFFu : (1u << wpl) - 1u);
-                    const int32_t layer = (int32_t)tid / wpl;
-                    lmask[layer] = lm;
-                    if (layer >= z0 && layer < z1)  // the entry's layers only (sub-bricks)
-                        gptr(J.smask)[(size_t)b * (uint32_t)bd2 + (uint32_t)layer] = lm;
-                }
-                if ((int32_t)tid < nwords && gword && sw) bits[gi] = bw | sw;
-            }
-            __syncthreads();
-            // Material dwords of the words that hold a surface voxel, packed
-            // per layer in word order: layer l's words start at dword
-            // l * wpl * 8 of the brick's block, a word's 8 dwords at its rank
-            // among the layer's surface words.
+            // nibbles together, and each lane of a word holding a surface
+            // voxel writes its 4 material bytes as one dword of the brick's
+            // block in the material grid (the winner's material, 0 for the
+            // word's other voxels).  Steps whose rows received no key only
+            // zero their surface words.  Keys are cleared for the next brick.
+            // With materials in the keys the loop issues no global loads, so
+            // its stores never wait (a wait for a load also waits for earlier
+            // stores here).
             VK_G uint32_t *__restrict__ mg =
                 (VK_G uint32_t *)(gptr(J.matgrid) + (size_t)b * (uint32_t)nvox);
-#pragma unroll
-            for (int st = 0; st < kSteps; st++) {
-                const int32_t v0 = (int32_t)wave * 256 + st * 1024;
-                if (v0 >= nvox) continue;  // uniform
-                const int32_t wi = (v0 >> 5) + (int32_t)(lane >> 3);  // word in the brick
-                const int32_t layer = wi / wpl, wl = wi & (wpl - 1);
-                const uint32_t lm = lmask[layer];
-                if ((lm >> wl) & 1u) {
-                    const uint32_t rank = (uint32_t)__popc(lm & ((1u << wl) - 1u));
-                    mg[((uint32_t)(layer * wpl) + rank) * 8u + (lane & 7u)] = mdw[st];
+            VK_G u32x4 *__restrict__ kg =
+                kKeys && J.keygrid ? (VK_G u32x4 *)(gptr(J.keygrid) + (size_t)b * (uint32_t)nvox)
+                                   : nullptr;
+            const uint32_t tmask = (uint32_t)__ballot(lane < 32u && touched[lane & 31u] != 0);
+            const uint32_t rows_per_step = 256u >> row_shift;  // bd0 >= 64: 1, 2 or 4 rows
+            auto rows_out = [&](auto embed_c) {
+                constexpr bool kEmbedded = decltype(embed_c)::value;
+                for (int32_t v0 = (int32_t)wave * 256; v0 < nvox; v0 += 1024) {
+                    const uint32_t rmask = ((1u << rows_per_step) - 1u) << (v0 >> row_shift);
+                    if (!(tmask & rmask)) {
+                        if (lane < 8u) wsurf[(v0 >> 5) + (int32_t)lane] = 0u;
+                        continue;
+                    }
+                    uint4 *kp = (uint4 *)&keys[v0 + 4 * (int32_t)lane];
+                    const uint4 kv = *kp;
+                    const uint32_t nib = (kv.x ? 8u : 0u) | (kv.y ? 4u : 0u) | (kv.z ? 2u : 0u) |
+                                         (kv.w ? 1u : 0u);
+                    uint32_t w = nib << (28u - 4u * (lane & 7u));
+                    w |= (uint32_t)__shfl_xor((int)w, 1);
+                    w |= (uint32_t)__shfl_xor((int)w, 2);
+                    w |= (uint32_t)__shfl_xor((int)w, 4);
+                    if ((lane & 7u) == 0u) wsurf[(v0 >> 5) + (int32_t)(lane >> 3)] = w;
+                    if (!w) continue;
+                    nsurf += (uint32_t)__popc(nib);
+                    uint32_t m;
+                    if (kEmbedded) {
+                        m = (kv.x & 0xFFu) | (kv.y & 0xFFu) << 8 | (kv.z & 0xFFu) << 16 |
+                            (kv.w & 0xFFu) << 24;
+                    } else {
+                        auto mat = [&](uint32_t key) -> uint32_t {
+                            return key && materials ? (uint32_t)materials[key & 0x3FFFFFFFu] : 0u;
+                        };
+                        m = mat(kv.x) | mat(kv.y) << 8 | mat(kv.z) << 16 | mat(kv.w) << 24;
+                    }
+                    if (nib) *kp = make_uint4(0u, 0u, 0u, 0u);
+                    mg[(v0 >> 2) + (int32_t)lane] = m;
+                    if (kKeys && kg) kg[(v0 >> 2) + (int32_t)lane] = u32x4{kv.x, kv.y, kv.z, kv.w};
                 }
+            };
+            if (embed)
+                rows_out(std::integral_constant<bool, true>{});
+            else
+                rows_out(std::integral_constant<bool, false>{});
+            __syncthreads();
+            if (gword) {
+                const uint32_t sw = wsurf[tid];
+                gptr(J.surf)[gi] = sw;
+                if (sw) gptr(J.bits)[gi] = bw | sw;
             }
             if (cycles && tid == 0) cycles[gb] = __builtin_amdgcn_s_memtime() - t_start;
         }  // brick
@@ -1235,6 +1223,18 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
         if (lane == 0 && nsurf) atomicAdd(stats, nsurf);
     }
 }
+template __global__ void k_surf_brick<false>(const Job *, const uint32_t *, uint32_t,
+                                             const float4 *, const uint32_t *, const uint32_t *,
+                                             const uint32_t *, uint32_t *, const uint32_t *,
+                                             const uint32_t *, const uint32_t *, const uint32_t *,
+                                             const uint32_t *, const uint8_t *, uint32_t *,
+                                             uint64_t *);
+template __global__ void k_surf_brick<true>(const Job *, const uint32_t *, uint32_t,
+                                            const float4 *, const uint32_t *, const uint32_t *,
+                                            const uint32_t *, uint32_t *, const uint32_t *,
+                                            const uint32_t *, const uint32_t *, const uint32_t *,
+                                            const uint32_t *, const uint8_t *, uint32_t *,
+                                            uint64_t *);
 
 // Non-empty bricks in index order: flags, then (after an exclusive scan of
 // the flags) the compacted list.
@@ -1504,6 +1504,7 @@ __device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
     const Job &J = jobs[job_of(node_bases, njobs, GL)];
     const GridDims g = J.gd;
     const int use_mats = J.use_mats, kind = J.kind;
+    const bool zbrd = J.zero_border != 0, rot = J.rot_ids != 0;
     const BrickDims bd = J.bdn;
     const uint32_t P = g.W * g.Y;  // words per plane
     const uint32_t nch = (P + kChunk - 1u) / kChunk;
@@ -1566,6 +1567,9 @@ __device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
     }
     Rsrc mr = rsrc(gptr(J.bits), 0u);
     if (use_mats) {  // uniform; the kNodeZ planes lie in one brick (kNodeZ divides its depth)
+        const VK_G uint32_t *surf = gptr(J.surf);
+#pragma unroll
+        for (uint32_t k = 0; k < kNodeZ; k++) S[k] = ld32(plane(surf, zb + k), o);
         // Material blocks relative to the first brick row the block touches.
         const uint32_t lbytes = bd.lwx + bd.lby + bd.lbz + 5u;  // log2 bytes per brick
         const uint32_t yb0 = (i0 / g.W) >> bd.lby;
@@ -1573,23 +1577,13 @@ __device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
         const uint32_t brow0 = ((zb >> bd.lbz) * bd.nby + yb0) * bd.nbx;
         mr = rsrc(gptr(J.matgrid) + ((size_t)brow0 << lbytes), ((yb1 - yb0 + 1u) * bd.nbx) << lbytes);
         const uint32_t b = ((y >> bd.lby) - yb0) * bd.nbx + (w >> bd.lwx);
-        // The layer masks of the kNodeZ planes are consecutive (one 16-byte
-        // load, issued with the bit loads; stale unless the brick was
-        // visited, so they are masked by bhist).
-        static_assert(kNodeZ == 4, "one 16-byte load of the layer masks");
-        const uint32_t layer0 = zb & ((1u << bd.lbz) - 1u);
+        // Surface words of bricks the surface pass did not visit are stale.
         const uint32_t vis = valid ? gptr(J.bhist)[brow0 + b] : 0u;
-        const u32x4 lm = valid ? *(const VK_G u32x4 *)(gptr(J.smask) +
-                                                       ((size_t)(brow0 + b) << bd.lbz) + layer0)
-                               : u32x4{0u, 0u, 0u, 0u};
-        const uint32_t lwl = bd.lby + bd.lwx;  // log2 words per layer
-        const uint32_t wl = ((y & ((1u << bd.lby) - 1u)) << bd.lwx) + (w & ((1u << bd.lwx) - 1u));
-        const uint32_t mk[kNodeZ] = {lm.x, lm.y, lm.z, lm.w};
+        const uint32_t lane_off = (b << lbytes) + (word_in_brick(bd, w, y, zb) << 5);
 #pragma unroll
         for (uint32_t k = 0; k < kNodeZ; k++) {
-            const uint32_t m = vis ? mk[k] : 0u;
-            S[k] = (m >> wl) & 1u;
-            moff[k] = (b << lbytes) + packed_word_offset(layer0 + k, lwl, wl, m);
+            if (!vis) S[k] = 0u;
+            moff[k] = lane_off + ((k << (bd.lby + bd.lwx)) << 5);
         }
     }
     // Round trip 2: the 32 material bytes of each surface word.
@@ -1640,7 +1634,11 @@ __device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
 #pragma unroll
             for (int j = 0; j < 8; j++) idw[j] = nibble_27((ir >> (4 * j)) & 15u);
             if (bnd) {
-                const IdPlanes op = orient_planes(L, R, Yn, Yp, Zn, Zp);
+                // A model rotated about z (x slices) takes its ids in the
+                // un-rotated frame: its -x/+x are the rotated -y/+y, its -y/+y
+                // the rotated +x/-x.
+                const IdPlanes op = rot ? orient_planes(Yn, Yp, R, L, Zn, Zp)
+                                        : orient_planes(L, R, Yn, Yp, Zn, Zp);
                 slot[tid][0] = make_uint4(idw[0], idw[1], idw[2], idw[3]);
                 slot[tid][1] = make_uint4(idw[4], idw[5], idw[6], idw[7]);
                 uint32_t m = bnd;
@@ -1661,19 +1659,29 @@ __device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
         const uint32_t md[8] = {ma[k].x, ma[k].y, ma[k].z, ma[k].w,
                                 mb[k].x, mb[k].y, mb[k].z, mb[k].w};
         const Rsrc nr = rsrc(nodes + (size_t)z * P * nbw, P * nbw);
+        // A partition with neighbours clears its border (zeroPadding,
+        // src/voxelizer.cu:4811-4844): whole rows and planes, and the first
+        // and last node of each row.
+        const uint32_t all =
+            zbrd && (y == 0u || y + 1u == g.Y || z == 0u || z + 1u == g.Z) ? 0u : ~0u;
+        const bool x0 = zbrd && w == 0u, x1 = zbrd && w + 1u == g.W;
         if (kind == 0) {  // ShortNode byte: id | material << 5
             uint32_t out[8];
 #pragma unroll
-            for (int j = 0; j < 8; j++) out[j] = idw[j] | ((md[j] << 5) & 0xE0E0E0E0u);
+            for (int j = 0; j < 8; j++) out[j] = (idw[j] | ((md[j] << 5) & 0xE0E0E0E0u)) & all;
+            if (x0) out[0] &= 0xFFFFFF00u;
+            if (x1) out[7] &= 0x00FFFFFFu;
             st128(nr, oo, u32x4{out[0], out[1], out[2], out[3]});
             st128(nr, oo + 16u, u32x4{out[4], out[5], out[6], out[7]});
         } else {  // LongNode halfword: id | material << 8
             uint32_t out[16];
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                out[2 * j] = __builtin_amdgcn_perm(md[j], idw[j], 0x05010400u);
-                out[2 * j + 1] = __builtin_amdgcn_perm(md[j], idw[j], 0x07030602u);
+                out[2 * j] = __builtin_amdgcn_perm(md[j], idw[j], 0x05010400u) & all;
+                out[2 * j + 1] = __builtin_amdgcn_perm(md[j], idw[j], 0x07030602u) & all;
             }
+            if (x0) out[0] &= 0xFFFF0000u;
+            if (x1) out[15] &= 0x0000FFFFu;
 #pragma unroll
             for (int q = 0; q < 4; q++)
                 st128(nr, oo + 16u * q,
@@ -1693,6 +1701,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 5))) voi
     const uint32_t *__restrict__ pre, const uint8_t *__restrict__ ystart,
     const uint8_t *__restrict__ zstart) {
     nodes_cart<true>(jobs, node_bases, njobs, pre, ystart, zstart);
+}
+
+// PartialNode expansion (processVoxel<PartialNode>, src/voxelizer.cu:
+// 2997-3033; r = float(solid) from constructNodeList2, 1675, which erosion
+// leaves alone, so the bits are read from the pre-erosion copy): one thread
+// per node.  A surface node's winner key comes from the keygrid at the node's
+// position in its brick (as its material in matgrid); the ratio is computed for
+// the winner at the node's alloc coordinates.  A cleared partition border
+// keeps r = 0 (zeroPadding stores Node()).
+__global__ __launch_bounds__(256) void k_nodes_partial(const Job *__restrict__ jobs, uint32_t j) {
+    const Job &J = jobs[j];
+    const GridDims g = J.gd;
+    const BrickDims bd = J.bdn;
+    const size_t n = (size_t)g.X * g.Y * g.Z;
+    const VK_G uint16_t *ln = (const VK_G uint16_t *)gptr(J.nodes);
+    VK_G u32x2 *out = (VK_G u32x2 *)J.pnodes;
+    const VK_G uint32_t *bits = gptr(J.pbits);  // before erosion
+    const bool zbrd = J.zero_border != 0, mats = J.use_mats != 0, embed = J.embed != 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t x = (uint32_t)(i % g.X), y = (uint32_t)((i / g.X) % g.Y);
+        const uint32_t z = (uint32_t)(i / ((size_t)g.X * g.Y));
+        const uint32_t w = x >> 5;
+        const uint32_t bit =
+            (bits[((size_t)z * g.Y + y) * g.W + w] >> (31u - (x & 31u))) & 1u;
+        float r = bit ? 1.0f : 0.0f;
+        if (mats && bit) {
+            const uint32_t b = brick_of_word(bd, w, y, z);
+            const size_t wi = ((size_t)z * g.Y + y) * g.W + w;
+            if (gptr(J.bhist)[b] && ((gptr(J.surf)[wi] >> (31u - (x & 31u))) & 1u)) {
+                const uint32_t lvox = bd.lwx + bd.lby + bd.lbz + 5u;  // log2 voxels per brick
+                const uint32_t key = gptr(J.keygrid)[((size_t)b << lvox) +
+                                                     (word_in_brick(bd, w, y, z) << 5) + (x & 31u)];
+                const uint32_t t = embed ? (key >> 8) & 0x3FFFFFu : key & 0x3FFFFFFFu;
+                const int32_t node[3] = {(int32_t)x, (int32_t)y, (int32_t)z};
+                r = partial_ratio(J.V, J.I, t, key >> 30, node, J.m, J.fd);
+            }
+        }
+        if (zbrd && (x == 0u || x + 1u == g.X || y == 0u || y + 1u == g.Y || z == 0u ||
+                     z + 1u == g.Z))
+            r = 0.0f;
+        out[i] = u32x2{(uint32_t)ln[i], __float_as_uint(r)};
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1847,19 +1898,17 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(const Job *__restrict__ jobs,
         const uint64_t gw = (uint64_t)(fw >> 1), gz = (uint64_t)(fz >> 1);
         const uint32_t b = (uint32_t)((((gz >> bd.lbz) * bd.nby + ((uint64_t)y >> bd.lby)) *
                                        bd.nbx) + (gw >> bd.lwx));
-        const uint32_t layer = (uint32_t)(gz & ((1u << bd.lbz) - 1u));
-        const uint32_t lwl = bd.lby + bd.lwx;  // log2 words per layer
-        const uint32_t wl = (uint32_t)((((uint64_t)y & ((1u << bd.lby) - 1u)) << bd.lwx) +
-                                       (gw & ((1u << bd.lwx) - 1u)));
-        const size_t bbase = ((size_t)b << (bd.lwx + bd.lby + bd.lbz + 5u)) + ((fw & 1) ? 16u : 0u);
+        const size_t bbase = ((size_t)b << (bd.lwx + bd.lby + bd.lbz + 5u)) +
+                             (size_t)word_in_brick(bd, (uint32_t)gw, y, (uint32_t)gz) * 32u +
+                             ((fw & 1) ? 16u : 0u);
+        const size_t gwi = (gz * g.Y + (uint64_t)y) * g.W + gw;
+        const int sh = (fw & 1) ? 0 : 16;
 #pragma unroll
         for (int par = 0; par < 2; par++) {
             const MatSource src = mat_source(Js[odd ? 2 + par : par]);  // uniform
             if (!src.bhist[b]) continue;
-            const uint32_t m = src.smask[((size_t)b << bd.lbz) + layer];
-            if (!((m >> wl) & 1u)) continue;
-            const u32x4 v = *(const VK_G u32x4 *)(src.matgrid + bbase +
-                                                  packed_word_offset(layer, lwl, wl, m));
+            if (((src.surf[gwi] >> sh) & 0xFFFFu) == 0u) continue;
+            const u32x4 v = *(const VK_G u32x4 *)(src.matgrid + bbase);
             ma[par][0] = v.x; ma[par][1] = v.y; ma[par][2] = v.z; ma[par][3] = v.w;
         }
     }

@@ -337,4 +337,141 @@ __device__ __forceinline__ void depth_range(const F3 t0, F3 p, F3 n, F3 m, float
     hi = f2i(floorf((b - mc) / d + 0.5f));
 }
 
+// ---------------------------------------------------------------------------
+// PartialNode volume ratio: processVoxel<PartialNode> (src/voxelizer.cu:
+// 2997-3033) with calculateVoxelPlaneIntersectionVolume (4220-4307),
+// boxParallelCutVolume (4313-4326), constructPolyhedronFaces (4333-4537),
+// volumeOfPolyhedron (4548-4706), rotX / rotY (4719-4757) and
+// volumeOfPyramid (4764-4785), FP32 in the reference's operation order.
+// length(float2) is sqrt(x*x + y*y) and normalize(v) is v * rsqrt(dot(v, v))
+// with rsqrt = 1 / sqrt: correctly rounded divide and square root.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ F3 fadd3(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 fscale3(float s, F3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ F3 rot_x(F3 v, float s, float c) {
+    return {v.x, v.y * c - v.z * s, v.y * s + v.z * c};
+}
+__device__ __forceinline__ F3 rot_y(F3 v, float s, float c) {
+    return {v.x * c + v.z * s, v.y, v.z * c - v.x * s};
+}
+__device__ __forceinline__ float len2f(float a, float b) { return sqrtf(a * a + b * b); }
+
+// Turns the polygon and the apex so that rotN points along z, then returns
+// the pyramid's volume (surveyor's formula times a third of the height).
+__device__ __noinline__ float turned_pyramid(F3 *poly, int n, F3 rotN, F3 apex) {
+    F3 h = apex;
+    if (rotN.y != 0.0f) {
+        const float pl = len2f(rotN.y, rotN.z), s = rotN.y / pl, c = rotN.z / pl;
+        h = rot_x(h, s, c);
+        for (int i = 0; i < n; i++) poly[i] = rot_x(poly[i], s, c);
+        rotN = rot_x(rotN, s, c);
+    }
+    if (rotN.x != 0.0f) {
+        const float pl = len2f(rotN.z, rotN.x), s = -rotN.x / pl, c = rotN.z / pl;
+        h = rot_y(h, s, c);
+        for (int i = 0; i < n; i++) poly[i] = rot_y(poly[i], s, c);
+    }
+    float area = 0.0f;
+    for (int i = 0; i < n; i++) {
+        const F3 a = poly[i], b = poly[(i + 1) % n];
+        area += 0.5f * (a.x * b.y - b.x * a.y);
+    }
+    return fabsf((1.0f / 3.0f) * area * (h.z - poly[0].z));
+}
+
+// Volume the reference computes for the box centred at m + d * node, cut by
+// the plane through t0 with normal n (the part the normal points into for
+// a plane through the box).
+__device__ __noinline__ float cut_volume(F3 t0, const int32_t node[3], F3 n, F3 m, float d) {
+    const F3 vc = fadd3(m, fscale3(d, F3{float(node[0]), float(node[1]), float(node[2])}));
+    if (fabsf(n.x) == 1.0f) return (d * d) * ((d / 2.0f) + t0.x - vc.x);
+    if (fabsf(n.y) == 1.0f) return (d * d) * ((d / 2.0f) + t0.y - vc.y);
+    if (fabsf(n.z) == 1.0f) return (d * d) * ((d / 2.0f) + t0.z - vc.z);
+    const bool bx = n.x > 0.0f, by = n.y > 0.0f, bz = n.z > 0.0f;
+    const float sx = float(!bx) * d + float(bx) * -d, sy = float(!by) * d + float(by) * -d,
+                sz = float(!bz) * d + float(bz) * -d;
+    const bool even = (int(bx) + int(by) + int(bz)) % 2 == 0;
+    const F3 dx = {sx, 0.0f, 0.0f};
+    const F3 dy = even ? F3{0.0f, sy, 0.0f} : F3{0.0f, 0.0f, sz};
+    const F3 dz = even ? F3{0.0f, 0.0f, sz} : F3{0.0f, sy, 0.0f};
+    const float dh = d / 2.0f;
+    F3 v[8];
+    v[0] = fadd3(vc, F3{dh * float(bx) - dh * float(!bx), dh * float(by) - dh * float(!by),
+                        dh * float(bz) - dh * float(!bz)});
+    v[1] = fadd3(v[0], dx);
+    v[2] = fadd3(v[0], dy);
+    v[3] = fadd3(v[0], dz);
+    v[4] = fadd3(v[1], dz);
+    v[5] = fadd3(v[2], dx);
+    v[6] = fadd3(v[3], dy);
+    v[7] = fadd3(v[4], dy);
+    // Three paths from v0 (edges (0,a), (a,b), (b,7) and (a,c)) build the cut
+    // polygon and faces 1-3; the arrays start zeroed (an empty cut polygon
+    // still reads its first entry's z).
+    F3 ip[6], face[3][6];
+    for (int i = 0; i < 6; i++) {
+        ip[i] = F3{0.0f, 0.0f, 0.0f};
+        for (int f = 0; f < 3; f++) face[f][i] = F3{0.0f, 0.0f, 0.0f};
+    }
+    int ni = 0, nf[3] = {0, 0, 0};
+    const int pa[3] = {1, 2, 3}, pb[3] = {4, 5, 6}, pc[3] = {5, 6, 4}, partner[3] = {2, 0, 1};
+    for (int k = 0; k < 3; k++) {
+        const int a = pa[k], b = pb[k], c = pc[k], f = k, g = partner[k];
+        float t = fdot3(n, fsub(t0, v[0])) / fdot3(n, fsub(v[a], v[0]));
+        if (t < 0.0f || t > 1.0f) {
+            face[f][nf[f]++] = v[a];
+            t = fdot3(n, fsub(t0, v[a])) / fdot3(n, fsub(v[b], v[a]));
+            if (t < 0.0f || t > 1.0f) {
+                t = fdot3(n, fsub(t0, v[b])) / fdot3(n, fsub(v[7], v[b]));
+                if (t >= 0.0f && t <= 1.0f) {
+                    ip[ni] = fadd3(v[b], fscale3(t, fsub(v[7], v[b])));
+                    face[f][nf[f]++] = v[b];
+                    face[f][nf[f]++] = ip[ni];
+                    face[g][nf[g]++] = ip[ni];
+                    face[g][nf[g]++] = v[b];
+                    ni++;
+                } else {
+                    face[f][nf[f]++] = v[b];
+                    face[g][nf[g]++] = v[b];
+                }
+            } else {
+                ip[ni] = fadd3(v[a], fscale3(t, fsub(v[b], v[a])));
+                face[f][nf[f]++] = ip[ni];
+                ni++;
+            }
+        } else {
+            ip[ni] = fadd3(v[0], fscale3(t, fsub(v[a], v[0])));
+            ni++;
+        }
+        t = fdot3(n, fsub(t0, v[a])) / fdot3(n, fsub(v[c], v[a]));
+        if (t >= 0.0f && t <= 1.0f) {
+            ip[ni] = fadd3(v[a], fscale3(t, fsub(v[c], v[a])));
+            face[f][nf[f]++] = ip[ni];
+            ni++;
+        }
+    }
+    float vol = 0.0f;
+    vol += turned_pyramid(ip, ni, n, v[0]);
+    for (int f = 0; f < 3; f++)
+        if (nf[f] > 2) {
+            const F3 e = fcross(fsub(face[f][0], face[f][2]), fsub(face[f][1], face[f][0]));
+            const float inv = 1.0f / sqrtf(fdot3(e, e));
+            vol += turned_pyramid(face[f], nf[f], F3{e.x * inv, e.y * inv, e.z * inv}, v[0]);
+        }
+    return vol;
+}
+
+// The ratio stored for the winner triangle t (class bb, 1 = 1D) at node
+// coordinates `node` (the padded coordinates processVoxel receives): 1D
+// triangles pass a zero normal; NaN is stored as 0x7FC00000.
+__device__ __forceinline__ float partial_ratio(const float *__restrict__ V,
+                                               const uint32_t *__restrict__ I, uint32_t t,
+                                               uint32_t bb, const int32_t node[3], F3 m, float d) {
+    const F3 t0 = fetch_f(V, I, t, 0), t1 = fetch_f(V, I, t, 1), t2 = fetch_f(V, I, t, 2);
+    const F3 n = bb == 1u ? F3{0.0f, 0.0f, 0.0f} : fcross(fsub(t0, t2), fsub(t1, t0));
+    const float vol = cut_volume(t0, node, n, m, d);
+    const float r = fabsf(1.0f - vol / (d * d * d));
+    return r != r ? __uint_as_float(0x7FC00000u) : r;
+}
+
 }  // namespace vk
