@@ -194,6 +194,24 @@ vox_status vox_voxelize_slice(vox_ctx *ctx, vox_plan_mode mode, double param, vo
                               const uint32_t vox_split[2], const uint32_t mat_split[2],
                               void *nodes_dev, uint32_t dims_out[3]);
 
+/* vox_voxelize_nodes without allocating: no hipMalloc, hipFree or other
+ * device or pinned allocation happens in the call.  No counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid it).  Every scratch buffer is
+ * carved from a device array of 128 MiB that the library holds on each GPU it
+ * runs on until the process ends (its address is looked up by vox_create);
+ * grids that need more (about 1/8 B per voxel of solid bits, 1 B per voxel
+ * with materials, ~100 B per triangle and 12 B per (triangle, tile) pair --
+ * e.g. the 512x403x303 room fits, the 1.2 G-voxel hall does not) fail with
+ * VOX_ERR_UNSUPPORTED before any work is queued.  All calls on one GPU share
+ * that array and each leaves its contents behind (what must start at zero is
+ * cleared on the stream); calls that overlap in time on one GPU (several
+ * contexts, threads or streams) race on it and must be serialized by the
+ * caller.  Cost recording (vox_cost_record) is not available here. */
+vox_status vox_voxelize_nodes_noalloc(vox_ctx *ctx, const vox_grid *grid, vox_node_kind kind,
+                                      int use_materials, const uint32_t vox_split[3],
+                                      const uint32_t mat_split[3], void *nodes_dev);
+
 /* Node array dimensions / count / node size for a plan and node kind
  * (FCC: 2*alloc[0] x alloc[1] x 2*alloc[2], src/host_side.cpp:1494-1499). */
 void vox_node_dims(const vox_grid *grid, vox_node_kind kind, uint32_t dims_out[3]);

@@ -84,7 +84,7 @@ EXPORTED = [
     "vox_build_info", "vox_alloc", "vox_free", "vox_memcpy_h2d", "vox_memcpy_d2h",
     "vox_set_mesh", "vox_set_mesh_device", "vox_set_materials", "vox_set_materials_device",
     "vox_plan_grid", "vox_plan_grid_host", "vox_solid_bits", "vox_voxelize_nodes",
-    "vox_voxelize_partition", "vox_solid_bits_partition",
+    "vox_voxelize_nodes_noalloc", "vox_voxelize_partition", "vox_solid_bits_partition",
     "vox_voxelize_slice", "vox_node_dims",
     "vox_node_count", "vox_node_size", "vox_enable_timing", "vox_get_stage_times",
     "vox_set_schedule", "vox_set_heavy_threshold", "vox_set_screen_ratio", "vox_set_split_parts", "vox_cost_set_params",
@@ -129,6 +129,7 @@ def lib():
     L.vox_plan_grid_host.argtypes = [P, U32, I32, D, U32, C.POINTER(Grid)]
     L.vox_solid_bits.argtypes = [P, C.POINTER(Grid), P, P, P]
     L.vox_voxelize_nodes.argtypes = [P, C.POINTER(Grid), I32, I32, P, P, P]
+    L.vox_voxelize_nodes_noalloc.argtypes = [P, C.POINTER(Grid), I32, I32, P, P, P]
     L.vox_voxelize_partition.argtypes = [P, C.POINTER(Grid), I32, I32, P, P, P, P, P]
     L.vox_solid_bits_partition.argtypes = [P, C.POINTER(Grid), P, P, P, P]
     L.vox_voxelize_slice.argtypes = [P, I32, D, I32, I32, I32, U32, P, P, P, P]
@@ -381,6 +382,17 @@ class Context:
         ms = np.array(mat_split, np.uint32)
         self._check(lib().vox_voxelize_nodes(self.h, C.byref(grid), kind, 1 if use_materials else 0,
                                              _ptr(vs), _ptr(ms), C.c_void_p(int(out_ptr))))
+
+    def voxelize_nodes_noalloc_device(self, grid: Grid, kind: int, out_ptr: int,
+                                      use_materials=True, vox_split=DEFAULT_VOX_SPLIT,
+                                      mat_split=DEFAULT_MAT_SPLIT):
+        """vox_voxelize_nodes_noalloc: the node pipeline with every scratch
+        buffer in the library's per-device scratch array (no allocation)."""
+        vs = np.array(vox_split, np.uint32)
+        ms = np.array(mat_split, np.uint32)
+        self._check(lib().vox_voxelize_nodes_noalloc(self.h, C.byref(grid), kind,
+                                                     1 if use_materials else 0, _ptr(vs),
+                                                     _ptr(ms), C.c_void_p(int(out_ptr))))
 
     def voxelize_partition(self, grid: Grid, kind: int, part_min, part_max, use_materials=True,
                            vox_split=DEFAULT_VOX_SPLIT, mat_split=DEFAULT_MAT_SPLIT) -> np.ndarray:

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -56,6 +57,44 @@ const uint8_t kOrientPairs[27][2] = {
     {42, 8},  {23, 9},  {27, 10}, {29, 11}, {30, 12}, {39, 13}, {43, 14},
     {45, 15}, {46, 16}, {53, 17}, {54, 18}, {57, 19}, {58, 20}, {31, 21},
     {59, 22}, {55, 23}, {62, 24}, {61, 25}, {47, 26}, {63, 27}};
+
+// Scratch of vox_voxelize_nodes_noalloc: a file-scope device array, one per
+// GPU the library runs on, held until the process ends.  Host code never
+// uses the array's name as an address: each device's copy is looked up once
+// with hipGetSymbolAddress (scratch_on) and that pointer is used instead.
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+constexpr int kMaxScratchDevices = 64;
+__device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices];
+bool g_scratch_done[kMaxScratchDevices];
+
+// Device address of `scratch` on `device` (current), looked up once per
+// device; null (and the HIP error in *err) if the lookup fails.
+void *scratch_on(int device, hipError_t *err) {
+    *err = hipSuccess;
+    if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            *err = e;
+            return nullptr;
+        }
+        g_scratch_ptr[device] = ptr;
+        g_scratch_done[device] = true;
+    }
+    return g_scratch_ptr[device];
+}
+
+// Named regions of the scratch array during one no-allocation call (a fixed
+// table: that call allocates nothing, not even host memory for names).
+constexpr int kArenaSlots = 96;
+struct ArenaSlot {
+    char name[32];
+    size_t off, cap;
+};
 
 }  // namespace
 
@@ -129,6 +168,14 @@ struct vox_ctx {
     const uint32_t *last_changed = nullptr;
     uint32_t last_solid = 0, last_surf = 0, last_passes = 0;
     int fcc_bdim[3] = {kFccBX, kFccBY, kFccBZ};
+    // No-allocation mode (vox_voxelize_nodes_noalloc): buffers are carved
+    // from this device's scratch array (scratch_dev, set up by vox_create)
+    // instead of hipMalloc.
+    uint8_t *scratch_dev = nullptr;
+    bool arena = false;
+    size_t arena_used = 0;
+    int arena_n = 0;
+    ArenaSlot arena_slot[kArenaSlots];
 };
 
 #define VOX_HIP(expr)                                                               \
@@ -166,9 +213,49 @@ vox_status fail(vox_ctx *ctx, vox_status s, const std::string &msg) {
     return s;
 }
 
+// A named region of the scratch array (no-allocation mode): reused when the
+// name was given out before in this call and is large enough, else carved
+// after the regions given out so far.
+vox_status arena_buf(vox_ctx *ctx, const char *name, size_t bytes, void **out) {
+    ArenaSlot *slot = nullptr;
+    for (int i = 0; i < ctx->arena_n; i++)
+        if (std::strncmp(ctx->arena_slot[i].name, name, sizeof(slot->name)) == 0) {
+            slot = &ctx->arena_slot[i];
+            break;
+        }
+    if (slot && slot->cap >= bytes) {
+        *out = ctx->scratch_dev + slot->off;
+        return VOX_OK;
+    }
+    if (!slot) {
+        if (ctx->arena_n == kArenaSlots)
+            return fail(ctx, VOX_ERR_UNSUPPORTED, "no-allocation call: too many scratch regions");
+        slot = &ctx->arena_slot[ctx->arena_n++];
+        std::strncpy(slot->name, name, sizeof(slot->name) - 1);
+        slot->name[sizeof(slot->name) - 1] = '\0';
+    }
+    const size_t off = (ctx->arena_used + 255) & ~(size_t)255;
+    if (off > kNoAllocScratchBytes || bytes > kNoAllocScratchBytes - off)
+        return fail(ctx, VOX_ERR_UNSUPPORTED,
+                    "no-allocation call: this grid needs more scratch than the " +
+                        std::to_string(kNoAllocScratchBytes >> 20) +
+                        " MiB the library holds per device (use vox_voxelize_nodes)");
+    slot->off = off;
+    slot->cap = bytes;
+    ctx->arena_used = off + bytes;
+    *out = ctx->scratch_dev + off;
+    return VOX_OK;
+}
+
 template <class T>
 vox_status buf(vox_ctx *ctx, const char *name, size_t count, T **out) {
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (ctx->arena) {
+        void *p = nullptr;
+        VOX_TRY(arena_buf(ctx, name, bytes, &p));
+        *out = (T *)p;
+        return VOX_OK;
+    }
     Buf &b = ctx->bufs[name];
     if (bytes > b.cap) {
         if (b.p) {  // either stream may still use the old buffer
@@ -506,6 +593,9 @@ vox_status add_mesh(vox_ctx *ctx, Plan &P, const float *V, const uint32_t *I, ui
 // for the previous upload to leave the area first.
 vox_status upload(vox_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return VOX_OK;
+    if (bytes > ctx->up_cap && ctx->arena)
+        return fail(ctx, VOX_ERR_UNSUPPORTED,
+                    "no-allocation call: job table larger than the staging area");
     if (bytes > ctx->up_cap) {
         if (ctx->up) {
             VOX_HIP(hipEventSynchronize(ctx->up_ev));
@@ -1170,6 +1260,22 @@ vox_status vox_create(int device, vox_ctx **out) {
         return VOX_ERR_HIP;
     }
     ctx->n_cu = std::max(ctx->n_cu, 1);
+    // Set-up of the no-allocation entry point: this device's copy of the
+    // scratch array, and a pinned staging area for job tables.
+    {
+        hipError_t e = hipSuccess;
+        ctx->scratch_dev = (uint8_t *)scratch_on(device, &e);
+        if (e != hipSuccess) {
+            vox_destroy(ctx);
+            return VOX_ERR_HIP;
+        }
+        constexpr size_t kStaging = 64 * 1024;
+        if (hipHostMalloc(&ctx->up, kStaging, hipHostMallocDefault) != hipSuccess) {
+            vox_destroy(ctx);
+            return VOX_ERR_HIP;
+        }
+        ctx->up_cap = kStaging;
+    }
     if (const char *e = std::getenv("VOX_EMBED_TRIS"))
         ctx->embed_limit = (uint32_t)std::min<unsigned long>(std::strtoul(e, nullptr, 10), 1ul << 22);
     ctx->surf_blocks_per_cu = std::max(ctx->surf_blocks_per_cu, 1);
@@ -1525,6 +1631,26 @@ vox_status vox_voxelize_nodes(vox_ctx *ctx, const vox_grid *g, vox_node_kind kin
                               const uint32_t mat_split[3], void *nodes_dev) {
     if (!ctx || !grid_ok(g) || !nodes_dev) return VOX_ERR_INVALID;
     return voxelize_nodes_impl(ctx, g, kind, use_materials, vox_split, mat_split, nodes_dev);
+}
+
+// The same pipeline with every buffer carved from this device's scratch
+// array; the job table goes through the staging area vox_create allocated.
+vox_status vox_voxelize_nodes_noalloc(vox_ctx *ctx, const vox_grid *g, vox_node_kind kind,
+                                      int use_materials, const uint32_t vox_split[3],
+                                      const uint32_t mat_split[3], void *nodes_dev) {
+    if (!ctx || !grid_ok(g) || !nodes_dev) return VOX_ERR_INVALID;
+    if (!ctx->scratch_dev)
+        return fail(ctx, VOX_ERR_UNSUPPORTED, "no scratch array on this device");
+    if (ctx->record)
+        return fail(ctx, VOX_ERR_UNSUPPORTED,
+                    "cost recording is not available in the no-allocation call");
+    ctx->arena = true;
+    ctx->arena_used = 0;
+    ctx->arena_n = 0;
+    const vox_status st =
+        voxelize_nodes_impl(ctx, g, kind, use_materials, vox_split, mat_split, nodes_dev);
+    ctx->arena = false;
+    return st;
 }
 
 static vox_status voxelize_nodes_impl(vox_ctx *ctx, const vox_grid *g, vox_node_kind kind,
