@@ -17,9 +17,10 @@ Workloads:
                  and the 128^3 icosphere.
 
 Schedules: "plain" processes surface bricks in index order; "learned" takes
-the bricks the MLP cost model predicts to be expensive first, split into
-sub-bricks (the model is trained first by the library's own GPU kernels on
-per-brick durations recorded on a differently seeded mesh).  A batch runs
+the bricks the MLP cost model predicts to be expensive first (cut into
+sub-bricks where there are few bricks per resident workgroup; the model is
+trained first by the library's own GPU kernels on per-brick durations
+recorded on a differently seeded mesh).  A batch runs
 fused under both (every stage one launch over all meshes, bricks of all
 meshes in one list), so learned / plain is the model's own share; the batch
 also reports the meshes one after another and 16 worker streams.
