@@ -1008,7 +1008,8 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
     uint64_t *__restrict__ cycles) {
     __shared__ __align__(16) uint32_t keys[kMaxBrickVox];
     __shared__ SurfTri tris[kSurfChunk];
-    __shared__ uint32_t colsum[kSurfChunk + 1];
+    __shared__ uint32_t cstart[kSurfChunk], ctotal[2], smask[256 / kWave][2];
+    __shared__ uint8_t cslot[kSurfChunk];
     __shared__ uint32_t wsurf[kMaxBrickVox / 32];
     __shared__ uint8_t touched[32];
     __shared__ uint32_t s_q, s_qe;
@@ -1016,6 +1017,7 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
     for (int32_t i = tid; i < kMaxBrickVox; i += blockDim.x) keys[i] = 0u;
     if (tid < 32u) touched[tid] = 0;
+    if (tid < 2u * (256u / kWave)) (&smask[0][0])[tid] = 0u;
     // Entries: the heavy list first (learned order), then the plain list.
     const uint32_t nh = hlist ? *n_heavy : 0u;
     const uint32_t n_list = *n_order + nh;
@@ -1126,19 +1128,47 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
                         const uint32_t o = (uint32_t)__shfl_up((int)incl, d);
                         if ((int)tid >= d) incl += o;
                     }
-                    colsum[tid + 1] = incl;
-                    if (tid == 0) colsum[0] = 0u;
+                    // The triangles with items, in order: first item and LDS slot.
+                    const unsigned long long nz = __ballot(items != 0u);
+                    if (items) {
+                        const uint32_t r = __builtin_amdgcn_mbcnt_hi(
+                            (uint32_t)(nz >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)nz, 0u));
+                        cstart[r] = incl - items;
+                        cslot[r] = (uint8_t)tid;
+                    }
+                    if (tid == (uint32_t)kSurfChunk - 1u) {
+                        ctotal[0] = incl;
+                        ctotal[1] = (uint32_t)__popcll(nz);
+                    }
                 }
                 __syncthreads();
-                const uint32_t total = colsum[kSurfChunk];
-                for (uint32_t c = tid; c < total; c += blockDim.x) {
-                    uint32_t lo = 0, hi = kSurfChunk;  // colsum[lo] <= c < colsum[lo + 1]
-                    while (lo + 1 < hi) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (colsum[mid] <= c) lo = mid;
-                        else hi = mid;
+                const uint32_t total = ctotal[0], ntri = ctotal[1];
+                // Lane j holds the first item of the j-th triangle with items
+                // (strictly increasing in j) and its slot.  A wave takes 64
+                // consecutive items at a time: the triangle of its first item
+                // is a ballot over the held starts, and each lane adds the
+                // triangles starting between the first item and its own, read
+                // from a 64-bit mask of start offsets that the starting lanes
+                // OR together in LDS.
+                const uint32_t cs = lane < ntri ? cstart[lane] : UINT32_MAX;
+                const uint32_t sl = lane < ntri ? (uint32_t)cslot[lane] : 0u;
+                uint32_t *sm = &smask[wave][0];
+                for (uint32_t w0 = wave * kWave; w0 < total; w0 += blockDim.x) {  // uniform
+                    const uint32_t t0 = (uint32_t)__popcll(__ballot(cs <= w0)) - 1u;
+                    const uint32_t ps = cs - w0 - 1u;  // start offset - 1, in 0..62
+                    if (cs > w0 && ps < kWave - 1u) atomicOr(&sm[ps >> 5], 1u << (ps & 31u));
+                    const uint32_t m_lo = sm[0], m_hi = sm[1];
+                    if (lane == 0u) {
+                        sm[0] = 0u;
+                        sm[1] = 0u;
                     }
-                    surf_item(p, spl, tris[lo], c - colsum[lo], keys, touched, nvox, row_shift);
+                    const uint32_t ti =
+                        t0 + __builtin_amdgcn_mbcnt_hi(m_hi, __builtin_amdgcn_mbcnt_lo(m_lo, 0u));
+                    const uint32_t slot = (uint32_t)__shfl((int)sl, (int)ti);
+                    const uint32_t first = (uint32_t)__shfl((int)cs, (int)ti);
+                    const uint32_t c = w0 + lane;
+                    if (c < total)
+                        surf_item(p, spl, tris[slot], c - first, keys, touched, nvox, row_shift);
                 }
             }
             __syncthreads();
