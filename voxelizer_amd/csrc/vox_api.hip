@@ -470,6 +470,10 @@ vox_status split_table(vox_ctx *ctx, const uint32_t res[3], const uint32_t matSp
     return VOX_OK;
 }
 
+// The FCC interleave and the PartialNode expansion read surface bit words;
+// the Cartesian node pass reads only the per-brick word masks.
+inline bool job_needs_surf(const Job &J) { return J.pnodes != nullptr || J.kind >= 3; }
+
 // Adds one grid with layout L; shift is the FCC sub-grid offset (zero for
 // Cartesian).  cart: the job gets erosion and Cartesian node blocks.
 vox_status add_job(vox_ctx *ctx, Plan &P, const float *V, const uint32_t *I, uint32_t nt,
@@ -556,9 +560,10 @@ vox_status add_job(vox_ctx *ctx, Plan &P, const float *V, const uint32_t *I, uin
     P.surf_off.push_back(P.surf_words);
     P.mat_off.push_back(P.mat_bytes);
     if (use_mats) {
-        // Surface bit words in the layout of the solid grid; one material
-        // byte per brick voxel, brick-major.
-        P.surf_words += words;
+        // Surface bit words in the layout of the solid grid (for the FCC and
+        // PartialNode passes); one material byte per brick voxel,
+        // brick-major.
+        if (job_needs_surf(J)) P.surf_words += words;
         P.mat_bytes += (size_t)nbt * (size_t)(bdim[0] * bdim[1] * bdim[2]);
         P.any_mats = true;
     }
@@ -619,16 +624,17 @@ vox_status upload(vox_ctx *ctx, void *dst, const void *src, size_t bytes) {
 vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D, uint32_t splitY, uint32_t splitZ) {
     const uint32_t nj = (uint32_t)P.jobs.size(), nf = (uint32_t)P.fcc_first.size();
     uint32_t *bits = nullptr, *surf = nullptr, *bhist = nullptr, *keygrid = nullptr,
-             *pbits = nullptr;
+             *pbits = nullptr, *wmask = nullptr;
     uint8_t *matgrid = nullptr, *plong = nullptr;
     if (P.bits_words) VOX_TRY(buf(ctx, "a_bits", P.bits_words, &bits));
     if (P.plong_bytes) VOX_TRY(buf(ctx, "a_plong", P.plong_bytes, &plong));
     if (P.key_bytes) VOX_TRY(buf(ctx, "a_keygrid", P.key_bytes / 4u, &keygrid));
     if (P.pbits_words) VOX_TRY(buf(ctx, "a_pbits", P.pbits_words, &pbits));
     if (P.any_mats) {
-        VOX_TRY(buf(ctx, "a_surf", P.surf_words, &surf));
+        if (P.surf_words) VOX_TRY(buf(ctx, "a_surf", P.surf_words, &surf));
         VOX_TRY(buf(ctx, "a_matgrid", P.mat_bytes, &matgrid));
         VOX_TRY(buf(ctx, "f_brick_hist", P.brick_b.back(), &bhist));
+        VOX_TRY(buf(ctx, "f_wmask", (size_t)P.brick_b.back() * 4u, &wmask));
     }
     std::vector<uint32_t> bases;
     for (auto *v : {&P.tri_b, &P.tile_b, &P.pad_b, &P.brick_b, &P.erode_b, &P.node_b})
@@ -682,8 +688,9 @@ vox_status plan_upload(vox_ctx *ctx, Plan &P, PlanDev &D, uint32_t splitY, uint3
         J.splits = dsplit + k;
         J.sstart = solid_split ? dev + jb + sb + bb + erode_flags : nullptr;
         if (J.use_mats) {
-            J.surf = surf + P.surf_off[k];
+            J.surf = job_needs_surf(J) ? surf + P.surf_off[k] : nullptr;
             J.bhist = bhist + J.brick_base;
+            J.wmask = wmask + (size_t)J.brick_base * 4u;
             J.matgrid = matgrid + P.mat_off[k];
         }
         if (J.pnodes) {
@@ -915,6 +922,9 @@ vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass 
     }
     VOX_TRY(fill(ctx, brick_hist, nbt, 0u));
     VOX_TRY(fill(ctx, brick_cursor, nbt, 0u));
+    uint32_t *wmask;
+    VOX_TRY(buf(ctx, "f_wmask", (size_t)nbt * 4u, &wmask));
+    VOX_TRY(fill(ctx, wmask, (size_t)nbt * 4u, 0u));
     auto bin = [&](int scatter) -> vox_status {
         if (total == 0) return VOX_OK;
         if (nj > 1)

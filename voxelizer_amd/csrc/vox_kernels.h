@@ -155,8 +155,9 @@ struct Job {
     uint32_t brick_base;   // first brick
     BrickDims bdn;         // brick geometry for the node passes
     const SplitTable *splits;  // matSplitRes intervals
-    uint32_t *surf;        // surface bit words (the layout of bits)
+    uint32_t *surf;        // surface bit words (the layout of bits); FCC and PartialNode only
     const uint32_t *bhist; // per brick: (triangle, brick) pairs; 0 = not visited
+    uint32_t *wmask;       // per brick, 4 words: bit i set where brick word i holds a surface voxel
     uint8_t *matgrid;      // per voxel, brick-major: winner material (surface words)
     int32_t use_mats;      // surface stage and materials wanted
     int32_t embed;         // winner keys carry the material (mesh-local index < 2^22)

@@ -1229,10 +1229,18 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
             else
                 rows_out(std::integral_constant<bool, false>{});
             __syncthreads();
+            const uint32_t sw = gword ? wsurf[tid] : 0u;
             if (gword) {
-                const uint32_t sw = wsurf[tid];
-                gptr(J.surf)[gi] = sw;
+                if (J.surf) gptr(J.surf)[gi] = sw;
                 if (sw) gptr(J.bits)[gi] = bw | sw;
+            }
+            // The brick's word mask (words of other layers belong to other
+            // parts of a cut brick, hence the OR into a cleared mask).
+            const unsigned long long hm = __ballot(sw != 0u);
+            if (wave < 2u && lane == 0u && hm) {
+                uint32_t *wm = J.wmask + (size_t)b * 4u + 2u * wave;
+                if ((uint32_t)hm) atomicOr(wm, (uint32_t)hm);
+                if ((uint32_t)(hm >> 32)) atomicOr(wm + 1, (uint32_t)(hm >> 32));
             }
             if (cycles && tid == 0) cycles[gb] = __builtin_amdgcn_s_memtime() - t_start;
         }  // brick
@@ -1589,30 +1597,35 @@ __device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
             zs[k] = z + 1u < g.Z && zstart[z + 1u] != 0;
         }
     }
-    uint32_t S[kNodeZ], moff[kNodeZ];
+    // Surface words (a word holding a surface voxel) from the brick's word
+    // mask, and the offsets of their material bytes.
+    bool S[kNodeZ];
+    uint32_t moff[kNodeZ];
 #pragma unroll
     for (uint32_t k = 0; k < kNodeZ; k++) {
-        S[k] = 0u;
+        S[k] = false;
         moff[k] = kOob;
     }
     Rsrc mr = rsrc(gptr(J.bits), 0u);
     if (use_mats) {  // uniform; the kNodeZ planes lie in one brick (kNodeZ divides its depth)
-        const VK_G uint32_t *surf = gptr(J.surf);
-#pragma unroll
-        for (uint32_t k = 0; k < kNodeZ; k++) S[k] = ld32(plane(surf, zb + k), o);
         // Material blocks relative to the first brick row the block touches.
         const uint32_t lbytes = bd.lwx + bd.lby + bd.lbz + 5u;  // log2 bytes per brick
         const uint32_t yb0 = (i0 / g.W) >> bd.lby;
         const uint32_t yb1 = ((min(i0 + kChunk, P) - 1u) / g.W) >> bd.lby;
         const uint32_t brow0 = ((zb >> bd.lbz) * bd.nby + yb0) * bd.nbx;
-        mr = rsrc(gptr(J.matgrid) + ((size_t)brow0 << lbytes), ((yb1 - yb0 + 1u) * bd.nbx) << lbytes);
+        const uint32_t nbrow = (yb1 - yb0 + 1u) * bd.nbx;  // bricks the block touches
+        mr = rsrc(gptr(J.matgrid) + ((size_t)brow0 << lbytes), nbrow << lbytes);
         const uint32_t b = ((y >> bd.lby) - yb0) * bd.nbx + (w >> bd.lwx);
-        // Surface words of bricks the surface pass did not visit are stale.
-        const uint32_t vis = valid ? gptr(J.bhist)[brow0 + b] : 0u;
-        const uint32_t lane_off = (b << lbytes) + (word_in_brick(bd, w, y, zb) << 5);
+        const u32x4 hm = ld128(rsrc(gptr(J.wmask) + (size_t)brow0 * 4u, nbrow * 16u),
+                               valid ? b * 16u : kOob);
+        const uint32_t wib = word_in_brick(bd, w, y, zb);
+        const uint32_t lane_off = (b << lbytes) + (wib << 5);
 #pragma unroll
         for (uint32_t k = 0; k < kNodeZ; k++) {
-            if (!vis) S[k] = 0u;
+            const uint32_t bit = wib + (k << (bd.lby + bd.lwx));
+            const uint32_t q = bit >> 5;
+            const uint32_t mw = q == 0u ? hm.x : q == 1u ? hm.y : q == 2u ? hm.z : hm.w;
+            S[k] = ((mw >> (bit & 31u)) & 1u) != 0u;
             moff[k] = lane_off + ((k << (bd.lby + bd.lwx)) << 5);
         }
     }
@@ -1654,7 +1667,7 @@ __device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
         const uint32_t interior = G & L & R & Yn & Yp & Zn & Zp;
         const uint32_t bnd = G & ~interior;  // solid, some neighbour missing
         uint32_t idw[8];
-        if ((bnd | S[k]) == 0u) {
+        if (bnd == 0u && !S[k]) {
             // All 32 voxels empty, or all fully surrounded (id 27).
             const uint32_t v = interior ? 0x1B1B1B1Bu : 0u;
 #pragma unroll
