@@ -37,44 +37,48 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-// The 16 features of brick b, computed by one wave (every lane ends with
-// all of them: the sums are butterflies).
-__device__ __forceinline__ void brick_feats(uint32_t b, const uint32_t *__restrict__ brick_hist,
-                                            const uint32_t *__restrict__ brick_off,
-                                            const uint32_t *__restrict__ blist,
-                                            const uint4 *__restrict__ tri_info,
-                                            const uint4 *__restrict__ tri_ext,
-                                            const Job *__restrict__ jobs,
-                                            const uint32_t *__restrict__ brick_bases,
-                                            uint32_t njobs, float f[kCostFeatures]) {
-    const uint32_t lane = lane_id();
+// Per-lane sums of the features (feature order: see above).
+struct FeatAcc {
+    float c1, c2, c3, i1, i2, i3, v3, mx, span, small, axx, axy;
+};
+
+// Accumulates triangles first, first + stride, ... of brick b's list (each
+// thread in index order).  Returns the brick's triangle count and voxels.
+__device__ __forceinline__ void feat_accumulate(uint32_t b, const uint32_t *__restrict__ brick_hist,
+                                                const uint32_t *__restrict__ brick_off,
+                                                const uint32_t *__restrict__ blist,
+                                                const uint4 *__restrict__ tri_info,
+                                                const uint4 *__restrict__ tri_ext,
+                                                const Job *__restrict__ jobs,
+                                                const uint32_t *__restrict__ brick_bases,
+                                                uint32_t njobs, uint32_t first, uint32_t stride,
+                                                FeatAcc &A, uint32_t &cnt, float &bvox) {
     const Job &J = jobs[job_of(brick_bases, njobs, b)];
     const int32_t bd0 = J.bdim[0], bd1 = J.bdim[1], bd2 = J.bdim[2];
-    const float bvox = (float)(bd0 * bd1 * bd2);
-    const uint32_t cnt = brick_hist[b], off = brick_off[b];
-    float c1 = 0, c2 = 0, c3 = 0, i1 = 0, i2 = 0, i3 = 0, v3 = 0, mx = 0, span = 0, small = 0,
-          axx = 0, axy = 0;
-    // Eight triangles per lane per step, their loads issued together (a heavy
-    // brick's hundreds of triangles would otherwise cost one round trip per
-    // 64); each lane still accumulates its triangles in index order.
+    bvox = (float)(bd0 * bd1 * bd2);
+    cnt = brick_hist[b];
+    const uint32_t off = brick_off[b];
+    // Eight triangles per thread per step, their loads issued together (a
+    // heavy brick's hundreds of triangles would otherwise cost one round
+    // trip per step).
     constexpr uint32_t kU = 8;
-    for (uint32_t i0 = lane; i0 < cnt; i0 += kU * kWave) {
+    for (uint32_t i0 = first; i0 < cnt; i0 += kU * stride) {
         uint32_t tt[kU];
         uint4 tis[kU], tes[kU];
 #pragma unroll
         for (uint32_t u = 0; u < kU; u++) {
-            const uint32_t i = i0 + u * kWave;
+            const uint32_t i = i0 + u * stride;
             tt[u] = i < cnt ? blist[off + i] : 0u;
         }
 #pragma unroll
         for (uint32_t u = 0; u < kU; u++) {
-            const bool ok = i0 + u * kWave < cnt;
+            const bool ok = i0 + u * stride < cnt;
             tis[u] = ok ? tri_info[tt[u]] : make_uint4(0u, 0u, 0u, 0u);
             tes[u] = ok ? tri_ext[tt[u]] : make_uint4(0u, 0u, 0u, 0u);
         }
 #pragma unroll
         for (uint32_t u = 0; u < kU; u++) {
-            if (i0 + u * kWave >= cnt) break;
+            if (i0 + u * stride >= cnt) break;
             const uint4 ti = tis[u], te = tes[u];
             const int bb = (int)(ti.x >> 30), a = (int)((ti.x >> 28) & 3u);
             const uint32_t wbx = ti.z >> 16, wby = ti.w & 0xFFFFu, wbz = ti.w >> 16;
@@ -84,46 +88,73 @@ __device__ __forceinline__ void brick_feats(uint32_t b, const uint32_t *__restri
             float est;
             if (bb == BB_1D) {
                 est = (float)te.w / (float)max(wa, 1u);
-                c1 += 1.0f;
-                i1 += est;
+                A.c1 += 1.0f;
+                A.i1 += est;
             } else {
                 est = (float)te.w / (float)max(wuv, 1u);
                 if (bb == BB_2D) {
-                    c2 += 1.0f;
-                    i2 += est;
+                    A.c2 += 1.0f;
+                    A.i2 += est;
                 } else {
-                    c3 += 1.0f;
-                    i3 += est;
-                    v3 += est * (float)min((int32_t)te.z, bda);
-                    axx += a == AX_X ? 1.0f : 0.0f;
-                    axy += a == AX_Y ? 1.0f : 0.0f;
+                    A.c3 += 1.0f;
+                    A.i3 += est;
+                    A.v3 += est * (float)min((int32_t)te.z, bda);
+                    A.axx += a == AX_X ? 1.0f : 0.0f;
+                    A.axy += a == AX_Y ? 1.0f : 0.0f;
                 }
             }
-            mx = fmaxf(mx, est);
-            span += (float)(wbx * wby * wbz);
-            small += est <= 4.0f ? 1.0f : 0.0f;
+            A.mx = fmaxf(A.mx, est);
+            A.span += (float)(wbx * wby * wbz);
+            A.small += est <= 4.0f ? 1.0f : 0.0f;
         }
     }
-    c1 = wave_sum(c1); c2 = wave_sum(c2); c3 = wave_sum(c3);
-    i1 = wave_sum(i1); i2 = wave_sum(i2); i3 = wave_sum(i3);
-    v3 = wave_sum(v3); mx = wave_max(mx); span = wave_sum(span);
-    small = wave_sum(small); axx = wave_sum(axx); axy = wave_sum(axy);
+}
+
+// Sums over the wave (butterflies: every lane ends with the totals).
+__device__ __forceinline__ void feat_wave_reduce(FeatAcc &A) {
+    A.c1 = wave_sum(A.c1); A.c2 = wave_sum(A.c2); A.c3 = wave_sum(A.c3);
+    A.i1 = wave_sum(A.i1); A.i2 = wave_sum(A.i2); A.i3 = wave_sum(A.i3);
+    A.v3 = wave_sum(A.v3); A.mx = wave_max(A.mx); A.span = wave_sum(A.span);
+    A.small = wave_sum(A.small); A.axx = wave_sum(A.axx); A.axy = wave_sum(A.axy);
+}
+
+__device__ __forceinline__ void feat_finish(uint32_t cnt, float bvox, const FeatAcc &A,
+                                            float f[kCostFeatures]) {
     f[0] = log1pf((float)cnt);
-    f[1] = log1pf(c1);
-    f[2] = log1pf(c2);
-    f[3] = log1pf(c3);
-    f[4] = log1pf(i1);
-    f[5] = log1pf(i2);
-    f[6] = log1pf(i3);
-    f[7] = log1pf(v3);
-    f[8] = log1pf(mx);
+    f[1] = log1pf(A.c1);
+    f[2] = log1pf(A.c2);
+    f[3] = log1pf(A.c3);
+    f[4] = log1pf(A.i1);
+    f[5] = log1pf(A.i2);
+    f[6] = log1pf(A.i3);
+    f[7] = log1pf(A.v3);
+    f[8] = log1pf(A.mx);
     f[9] = log1pf((float)((cnt + 63u) / 64u));
-    f[10] = c3 > 0.0f ? axx / c3 : 0.0f;
-    f[11] = c3 > 0.0f ? axy / c3 : 0.0f;
-    f[12] = log1pf(span);
-    f[13] = (i1 + i2 + i3) / bvox;
-    f[14] = log1pf(small);
+    f[10] = A.c3 > 0.0f ? A.axx / A.c3 : 0.0f;
+    f[11] = A.c3 > 0.0f ? A.axy / A.c3 : 0.0f;
+    f[12] = log1pf(A.span);
+    f[13] = (A.i1 + A.i2 + A.i3) / bvox;
+    f[14] = log1pf(A.small);
     f[15] = 1.0f;
+}
+
+// The 16 features of brick b, computed by one wave (every lane ends with
+// all of them).
+__device__ __forceinline__ void brick_feats(uint32_t b, const uint32_t *__restrict__ brick_hist,
+                                            const uint32_t *__restrict__ brick_off,
+                                            const uint32_t *__restrict__ blist,
+                                            const uint4 *__restrict__ tri_info,
+                                            const uint4 *__restrict__ tri_ext,
+                                            const Job *__restrict__ jobs,
+                                            const uint32_t *__restrict__ brick_bases,
+                                            uint32_t njobs, float f[kCostFeatures]) {
+    FeatAcc A = {};
+    uint32_t cnt;
+    float bvox;
+    feat_accumulate(b, brick_hist, brick_off, blist, tri_info, tri_ext, jobs, brick_bases, njobs,
+                    lane_id(), kWave, A, cnt, bvox);
+    feat_wave_reduce(A);
+    feat_finish(cnt, bvox, A, f);
 }
 
 // Row r describes brick order[sel ? sel[r] : r], for r < *n_rows.
@@ -420,11 +451,12 @@ __global__ __launch_bounds__(256) void k_cost_heavy(const float *__restrict__ Y,
     }
 }
 
-// The learned path in one launch: for each candidate (one wave per brick)
-// its features, the MLP forward pass with lane j as hidden neuron j (weights
-// read from global memory, where they stay cached; the sums run in the order
-// of k_mlp_forward, so the prediction is the same), and the heavy test that
-// puts the brick on the heavy list and flags its plain entry.
+// The learned path in one launch: for each candidate (one workgroup per
+// brick, its four waves taking every fourth run of 64 triangles, their sums
+// combined in a fixed order) its features, the MLP forward pass on wave 0
+// with lane j as hidden neuron j (weights read from global memory, where they
+// stay cached; the sums run in the order of k_mlp_forward), and the heavy
+// test that puts the brick on the heavy list and flags its plain entry.
 __global__ __launch_bounds__(256) void k_cost_predict(
     const float *__restrict__ P, const uint32_t *__restrict__ order,
     const uint32_t *__restrict__ cand, uint32_t *__restrict__ cnt, float heavy,
@@ -434,36 +466,62 @@ __global__ __launch_bounds__(256) void k_cost_predict(
     const uint4 *__restrict__ tri_ext, const Job *__restrict__ jobs,
     const uint32_t *__restrict__ brick_bases, uint32_t njobs, uint32_t *__restrict__ hlist,
     uint32_t *__restrict__ hsub, uint8_t *__restrict__ skip) {
+    constexpr int kWaves = 256 / kWave;
+    __shared__ FeatAcc part[kWaves];
     const uint32_t n = cnt[0];
-    const uint32_t j = lane_id();
-    const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
-    for (uint32_t r = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; r < n; r += nwaves) {
+    const uint32_t j = lane_id(), w = threadIdx.x / kWave;
+    for (uint32_t r = blockIdx.x; r < n; r += gridDim.x) {  // uniform over the block
         const uint32_t q = cand[r], b = order[q];
-        float f[kCostFeatures];
-        brick_feats(b, brick_hist, brick_off, blist, tri_info, tri_ext, jobs, brick_bases, njobs,
-                    f);
-        float z = P[kOffB1 + j];
+        FeatAcc A = {};
+        uint32_t tris;
+        float bvox;
+        feat_accumulate(b, brick_hist, brick_off, blist, tri_info, tri_ext, jobs, brick_bases,
+                        njobs, threadIdx.x, blockDim.x, A, tris, bvox);
+        feat_wave_reduce(A);
+        if (j == 0) part[w] = A;
+        __syncthreads();
+        if (w == 0) {
+            const FeatAcc p0 = part[0], p1 = part[1], p2 = part[2], p3 = part[3];
+            auto add = [](float a, float b, float c, float d) { return (a + b) + (c + d); };
+            A.c1 = add(p0.c1, p1.c1, p2.c1, p3.c1);
+            A.c2 = add(p0.c2, p1.c2, p2.c2, p3.c2);
+            A.c3 = add(p0.c3, p1.c3, p2.c3, p3.c3);
+            A.i1 = add(p0.i1, p1.i1, p2.i1, p3.i1);
+            A.i2 = add(p0.i2, p1.i2, p2.i2, p3.i2);
+            A.i3 = add(p0.i3, p1.i3, p2.i3, p3.i3);
+            A.v3 = add(p0.v3, p1.v3, p2.v3, p3.v3);
+            A.mx = fmaxf(fmaxf(p0.mx, p1.mx), fmaxf(p2.mx, p3.mx));
+            A.span = add(p0.span, p1.span, p2.span, p3.span);
+            A.small = add(p0.small, p1.small, p2.small, p3.small);
+            A.axx = add(p0.axx, p1.axx, p2.axx, p3.axx);
+            A.axy = add(p0.axy, p1.axy, p2.axy, p3.axy);
+            float f[kCostFeatures];
+            feat_finish(tris, bvox, A, f);
+            float z = P[kOffB1 + j];
 #pragma unroll
-        for (int i = 0; i < kIn; i++) z += P[j * kIn + i] * f[i];
-        const float h1 = fmaxf(z, 0.0f);
-        z = P[kOffB2 + j];
+            for (int i = 0; i < kIn; i++) z += P[j * kIn + i] * f[i];
+            const float h1 = fmaxf(z, 0.0f);
+            z = P[kOffB2 + j];
 #pragma unroll
-        for (int k = 0; k < kHid; k++)
-            z += P[kOffW2 + j * kHid + k] *
-                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h1), k));
-        const float h2 = fmaxf(z, 0.0f);
-        const float y = wave_sum(P[kOffW3 + j] * h2) + P[kOffB3];
-        if (j == 0 && y > heavy) {  // false for NaN
-            const uint32_t depth = (uint32_t)jobs[job_of(brick_bases, njobs, b)].bdim[2];
-            uint32_t parts = y > heavy + 2.0f * kSplitStep ? 4u : y > heavy + kSplitStep ? 2u : 1u;
-            parts = min(min(parts, max_parts), depth);  // all powers of two
-            const uint32_t base = atomicAdd(&cnt[1], parts);
-            for (uint32_t k = 0; k < parts; k++) {
-                hlist[base + k] = b;
-                hsub[base + k] = (k * depth / parts) | (((k + 1u) * depth / parts) << 16);
+            for (int k = 0; k < kHid; k++)
+                z += P[kOffW2 + j * kHid + k] *
+                     __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h1), k));
+            const float h2 = fmaxf(z, 0.0f);
+            const float y = wave_sum(P[kOffW3 + j] * h2) + P[kOffB3];
+            if (j == 0 && y > heavy) {  // false for NaN
+                const uint32_t depth = (uint32_t)jobs[job_of(brick_bases, njobs, b)].bdim[2];
+                uint32_t parts =
+                    y > heavy + 2.0f * kSplitStep ? 4u : y > heavy + kSplitStep ? 2u : 1u;
+                parts = min(min(parts, max_parts), depth);  // all powers of two
+                const uint32_t base = atomicAdd(&cnt[1], parts);
+                for (uint32_t k = 0; k < parts; k++) {
+                    hlist[base + k] = b;
+                    hsub[base + k] = (k * depth / parts) | (((k + 1u) * depth / parts) << 16);
+                }
+                skip[q] = 1u;
             }
-            skip[q] = 1u;
         }
+        __syncthreads();  // part[] is read before the next candidate writes it
     }
 }
 
