@@ -481,8 +481,7 @@ vox_status add_job(vox_ctx *ctx, Plan &P, const float *V, const uint32_t *I, uin
                    bool use_mats, const uint32_t ms[3], const int bdim[3], bool cart, void *nodes,
                    int kind, uint32_t *ext_bits) {
     const uint32_t X = g.alloc[0], W = X / 32;
-    if (W == 0 || (size_t)16 * W * 4 > 65536)
-        return fail(ctx, VOX_ERR_UNSUPPORTED, "x resolution beyond the fill kernel's LDS row buffer");
+    if (W == 0) return fail(ctx, VOX_ERR_UNSUPPORTED, "empty x resolution");
     if (g.x_splits == 0 || X / g.x_splits > 1024)
         return fail(ctx, VOX_ERR_UNSUPPORTED,
                     "voxSplitRes.x gives x splits wider than 1024 voxels, where the reference's "
@@ -530,7 +529,8 @@ vox_status add_job(vox_ctx *ctx, Plan &P, const float *V, const uint32_t *I, uin
     for (int a = 0; a < 3; a++) ga.alloc[a] = L.alloc[a];
     J.bdn = brick_dims(ga, bdim);
     SplitTable st;
-    VOX_TRY(split_table(ctx, L.res, ms, st));
+    std::memset(&st, 0, sizeof(st));
+    if (use_mats) VOX_TRY(split_table(ctx, L.res, ms, st));  // only the surface pass reads it
     P.splits.push_back(st);
     J.erode_base = P.erode_b.back();
     J.node_base = P.node_b.back();
@@ -817,7 +817,7 @@ vox_status solid_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
         VOX_HIP(hipGetLastError());
     }
     if (nTiles > 0) {
-        k_solid_fill<<<nTiles, kWave, 16 * P.maxW * 4, ctx->stream>>>(
+        k_solid_fill<<<nTiles, kWave, 16 * std::min(P.maxW, kFillSegWords) * 4, ctx->stream>>>(
             D.jobs, D.tile_b, nj, ctx->rec, tile_off, tile_hist, list, list_mask);
         VOX_HIP(hipGetLastError());
     }

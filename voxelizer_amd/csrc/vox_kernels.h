@@ -62,6 +62,9 @@ __device__ __forceinline__ RowBlock row_block(uint32_t L, uint32_t nxc, uint32_t
 // consecutive words; a 256-thread block takes one chunk over a group of
 // planes, so every lane has a word whatever the row width W.
 constexpr uint32_t kChunk = 256;
+// x words of a solid-fill pass: a tile's 16 rows of them fill 64 KB of LDS;
+// wider rows take several passes.
+constexpr uint32_t kFillSegWords = 1024;
 __host__ __device__ __forceinline__ uint32_t plane_chunks(uint32_t W, uint32_t Y) {
     return (W * Y + kChunk - 1u) / kChunk;
 }

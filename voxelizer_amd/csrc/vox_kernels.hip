@@ -498,17 +498,19 @@ __global__ __launch_bounds__(kWave) void k_solid_fill(const Job *__restrict__ jo
                                                       const uint32_t *__restrict__ tile_hist,
                                                       const uint32_t *__restrict__ list,
                                                       const uint16_t *__restrict__ list_mask) {
-    extern __shared__ uint32_t cross[];  // [16][W]
+    extern __shared__ uint32_t cross[];  // [16][SW]
     const Job &J = jobs[job_of(tile_bases, njobs, blockIdx.x)];
     const SolidParams p = solid_params(J, rec);
     VK_G uint32_t *__restrict__ bits = gptr(J.bits);
     const uint32_t W = p.X >> 5;
+    // Rows wider than kFillSegWords words are filled a segment at a time:
+    // the tile's triangles are walked once per segment, and each row's
+    // crossing parity carries from one segment into the next.
+    const uint32_t SW = min(W, kFillSegWords);
     const uint32_t tid = threadIdx.x;
     const uint32_t local = blockIdx.x - J.tile_base;
     const uint32_t ty = local % p.tilesY;
     const uint32_t tz = local / p.tilesY;
-    for (uint32_t i = tid; i < 16u * W; i += kWave) cross[i] = 0u;
-    __syncthreads();
     const uint32_t tile = blockIdx.x;
     const uint32_t cnt = tile_hist[tile];
     const uint32_t off = tile_off[tile];
@@ -519,49 +521,60 @@ __global__ __launch_bounds__(kWave) void k_solid_fill(const Job *__restrict__ jo
     uint32_t lg = 0;
     while (lg < 4u && cnt <= (kWave >> (lg + 1u))) lg++;
     const uint32_t items = cnt << lg, rows = 16u >> lg;
-    for (uint32_t i = tid; i < items; i += kWave) {
-        const uint32_t t = list[off + (i >> lg)];
-        // Rows whose subspace queued the triangle (all rows with one subspace).
-        const uint32_t rmask = list_mask ? (uint32_t)list_mask[off + (i >> lg)] : 0xFFFFu;
-        const uint32_t r0 = (i & ((1u << lg) - 1u)) * rows;
-        D3 v[3];
-        load_tri_d(p, t, v);
-        FillSetup fs;
-        fill_setup(v, fs);
-        for (uint32_t r = r0; r < r0 + rows; r++) {
+    const uint32_t lane = tid;
+    uint32_t carries = 0u;  // bit r: crossing parity of row r before the segment
+    for (uint32_t s0 = 0; s0 < W; s0 += SW) {
+        const uint32_t sw = min(SW, W - s0);
+        for (uint32_t i = tid; i < 16u * SW; i += kWave) cross[i] = 0u;
+        __syncthreads();
+        for (uint32_t i = tid; i < items; i += kWave) {
+            const uint32_t t = list[off + (i >> lg)];
+            // Rows whose subspace queued the triangle (all rows with one subspace).
+            const uint32_t rmask = list_mask ? (uint32_t)list_mask[off + (i >> lg)] : 0xFFFFu;
+            const uint32_t r0 = (i & ((1u << lg) - 1u)) * rows;
+            D3 v[3];
+            load_tri_d(p, t, v);
+            FillSetup fs;
+            fill_setup(v, fs);
+            for (uint32_t r = r0; r < r0 + rows; r++) {
+                const uint32_t cy = ty * 4u + (r & 3u), cz = tz * 4u + (r >> 2);
+                if (cy < p.smin[0] || cy >= p.smax[0] || cz < p.smin[1] || cz >= p.smax[1]) continue;
+                if (!((rmask >> r) & 1u)) continue;
+                uint32_t vx;
+                if (fill_center(fs, p.emx, p.emy, p.emz, p.d, cy, cz, vx) && vx < p.X) {
+                    const uint32_t wx = (vx >> 5) - s0;  // wraps for words before s0
+                    if (wx < sw) atomicXor(&cross[r * SW + wx], 1u << (31u - (vx & 31u)));
+                }
+            }
+        }
+        __syncthreads();
+        // Crossing parity -> solid bits (MSB-first bits: x%32 == 0 is bit
+        // 31): a prefix XOR inside each word, and across words the parity of
+        // the crossings before it (one ballot per 64 words), written
+        // straight out.
+        for (uint32_t r = 0; r < 16u; r++) {
             const uint32_t cy = ty * 4u + (r & 3u), cz = tz * 4u + (r >> 2);
             if (cy < p.smin[0] || cy >= p.smax[0] || cz < p.smin[1] || cz >= p.smax[1]) continue;
-            if (!((rmask >> r) & 1u)) continue;
-            uint32_t vx;
-            if (fill_center(fs, p.emx, p.emy, p.emz, p.d, cy, cz, vx) && vx < p.X)
-                atomicXor(&cross[r * W + (vx >> 5)], 1u << (31u - (vx & 31u)));
+            const size_t rowIdx = (size_t)(cz + p.oz) * p.rowsY + (cy + p.oy);
+            uint32_t carry = (carries >> r) & 1u;
+            for (uint32_t w0 = 0; w0 < sw; w0 += kWave) {
+                const uint32_t w = w0 + lane;
+                const uint32_t c = w < sw ? cross[r * SW + w] : 0u;
+                uint32_t sx = c;
+                sx ^= sx >> 1;
+                sx ^= sx >> 2;
+                sx ^= sx >> 4;
+                sx ^= sx >> 8;
+                sx ^= sx >> 16;
+                const unsigned long long odd = __ballot((__popc(c) & 1) != 0);
+                const unsigned long long below = lane ? (odd & (~0ull >> (64u - lane))) : 0ull;
+                const uint32_t before = carry ^ ((uint32_t)__popcll(below) & 1u);
+                if (w < sw) bits[rowIdx * W + s0 + w] = sx ^ (before ? 0xFFFFFFFFu : 0u);
+                carry ^= (uint32_t)__popcll(odd) & 1u;
+            }
+            carries = (carries & ~(1u << r)) | (carry << r);
         }
-    }
-    __syncthreads();
-    // Crossing parity -> solid bits (MSB-first bits: x%32 == 0 is bit 31):
-    // a prefix XOR inside each word, and across words the parity of the
-    // crossings before it (one ballot per 64 words), written straight out.
-    const uint32_t lane = tid;
-    for (uint32_t r = 0; r < 16u; r++) {
-        const uint32_t cy = ty * 4u + (r & 3u), cz = tz * 4u + (r >> 2);
-        if (cy < p.smin[0] || cy >= p.smax[0] || cz < p.smin[1] || cz >= p.smax[1]) continue;
-        const size_t rowIdx = (size_t)(cz + p.oz) * p.rowsY + (cy + p.oy);
-        uint32_t carry = 0u;
-        for (uint32_t w0 = 0; w0 < W; w0 += kWave) {
-            const uint32_t w = w0 + lane;
-            const uint32_t c = w < W ? cross[r * W + w] : 0u;
-            uint32_t sx = c;
-            sx ^= sx >> 1;
-            sx ^= sx >> 2;
-            sx ^= sx >> 4;
-            sx ^= sx >> 8;
-            sx ^= sx >> 16;
-            const unsigned long long odd = __ballot((__popc(c) & 1) != 0);
-            const unsigned long long below = lane ? (odd & (~0ull >> (64u - lane))) : 0ull;
-            const uint32_t before = carry ^ ((uint32_t)__popcll(below) & 1u);
-            if (w < W) bits[rowIdx * W + w] = sx ^ (before ? 0xFFFFFFFFu : 0u);
-            carry ^= (uint32_t)__popcll(odd) & 1u;
-        }
+        __syncthreads();  // the next segment clears cross[]
     }
 }
 
