@@ -295,6 +295,45 @@ vox_status fill(vox_ctx *ctx, uint32_t *p, size_t n, uint32_t v) {
     return VOX_OK;
 }
 
+// Several fills in one launch (each launch on the stream costs a few
+// microseconds whatever its size).
+struct FillSeg {
+    uint32_t *p;
+    size_t n;
+    uint32_t v;
+};
+constexpr int kMaxFillSegs = 6;
+struct FillSegs {
+    uint32_t *p[kMaxFillSegs];
+    size_t n[kMaxFillSegs];
+    uint32_t v[kMaxFillSegs];
+    int count;
+};
+__global__ void k_fill_many(FillSegs s) {
+    for (int k = 0; k < s.count; k++)
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < s.n[k];
+             i += (size_t)gridDim.x * blockDim.x)
+            s.p[k][i] = s.v[k];
+}
+vox_status fill_many(vox_ctx *ctx, std::initializer_list<FillSeg> segs) {
+    FillSegs s;
+    s.count = 0;
+    size_t most = 0;
+    for (const FillSeg &f : segs) {
+        if (f.n == 0 || !f.p) continue;
+        if (s.count == kMaxFillSegs) return fail(ctx, VOX_ERR_INVALID, "too many fill segments");
+        s.p[s.count] = f.p;
+        s.n[s.count] = f.n;
+        s.v[s.count] = f.v;
+        s.count++;
+        most = std::max(most, f.n);
+    }
+    if (s.count == 0) return VOX_OK;
+    k_fill_many<<<grid_for(most, 256, 4096), 256, 0, ctx->stream>>>(s);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
 // Exclusive scan of in[0,n) into out[0,n+1]: out[n] is the total mod 2^32,
 // out[n+1] is 1 if the true total reached 2^32 (see k_scan_down).
 vox_status scan(vox_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t n, int depth = 0) {
@@ -797,8 +836,7 @@ vox_status solid_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D) {
                                                                         first);
         VOX_HIP(hipGetLastError());
     }
-    VOX_TRY(fill(ctx, tile_hist, nTiles, 0u));
-    VOX_TRY(fill(ctx, tile_cursor, nTiles, 0u));
+    VOX_TRY(fill_many(ctx, {{tile_hist, nTiles, 0u}, {tile_cursor, nTiles, 0u}}));
     if (total > 0) {
         if (nj > 1)
             k_solid_candidates<true><<<grid_for(total, 256), 256, 0, ctx->stream>>>(
@@ -920,11 +958,17 @@ vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass 
                                                                         first);
         VOX_HIP(hipGetLastError());
     }
-    VOX_TRY(fill(ctx, brick_hist, nbt, 0u));
-    VOX_TRY(fill(ctx, brick_cursor, nbt, 0u));
-    uint32_t *wmask;
+    // Everything this stage and the brick pass count into starts at zero.
+    uint32_t *wmask, *next, *cnt = nullptr;
     VOX_TRY(buf(ctx, "f_wmask", (size_t)nbt * 4u, &wmask));
-    VOX_TRY(fill(ctx, wmask, (size_t)nbt * 4u, 0u));
+    VOX_TRY(buf(ctx, "f_next", 1, &next));
+    if (learned && !ctx->record) VOX_TRY(buf(ctx, "f_cnt", 2, &cnt));
+    VOX_TRY(fill_many(ctx, {{brick_hist, nbt, 0u},
+                            {brick_cursor, nbt, 0u},
+                            {wmask, (size_t)nbt * 4u, 0u},
+                            {stats, 1, 0u},
+                            {next, 1, 0u},
+                            {cnt, cnt ? 2u : 0u, 0u}}));
     auto bin = [&](int scatter) -> vox_status {
         if (total == 0) return VOX_OK;
         if (nj > 1)
@@ -941,11 +985,10 @@ vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass 
     VOX_TRY(bin(0));
     VOX_TRY(scan(ctx, brick_hist, brick_off, nbt));
     // Non-empty bricks in index order (the plain schedule of the brick pass).
-    uint32_t *flags, *blist_order, *blist_order_off, *next;
+    uint32_t *flags, *blist_order, *blist_order_off;
     VOX_TRY(buf(ctx, "f_flags", nbt, &flags));
     VOX_TRY(buf(ctx, "f_order", nbt, &blist_order));
     VOX_TRY(buf(ctx, "f_order_off", nbt + 2, &blist_order_off));
-    VOX_TRY(buf(ctx, "f_next", 1, &next));
     k_nonzero_flags<<<grid_for(nbt, 256), 256, 0, ctx->stream>>>(brick_hist, flags, nbt);
     VOX_HIP(hipGetLastError());
     VOX_TRY(scan(ctx, flags, blist_order_off, nbt));
@@ -969,14 +1012,12 @@ vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass 
             D.brick_b, nj, feats);
         VOX_HIP(hipGetLastError());
     } else if (learned) {
-        uint32_t *lcand, *cnt, *heavy, *hsub;
+        uint32_t *lcand, *heavy, *hsub;
         uint8_t *skipf;
         VOX_TRY(buf(ctx, "f_lcand", nbt, &lcand));
-        VOX_TRY(buf(ctx, "f_cnt", 2, &cnt));
         VOX_TRY(buf(ctx, "f_heavy", (size_t)kMaxSplitParts * nbt, &heavy));
         VOX_TRY(buf(ctx, "f_hsub", (size_t)kMaxSplitParts * nbt, &hsub));
         VOX_TRY(buf(ctx, "f_skip", nbt, &skipf));
-        VOX_TRY(fill(ctx, cnt, 2, 0u));
         const uint32_t g256 = grid_for(nbt, 256, 1024);
         k_cost_screen<<<g256, 256, 0, ctx->stream>>>(blist_order, n_list, brick_hist,
                                                      cand_off + nslots, ctx->screen_ratio, lcand,
@@ -998,8 +1039,6 @@ vox_status surface_prep(vox_ctx *ctx, const Plan &P, const PlanDev &D, SurfPass 
         n_heavy = cnt + 1;
         skip = skipf;
     }
-    VOX_TRY(fill(ctx, stats, 1, 0u));
-    VOX_TRY(fill(ctx, next, 1, 0u));
     (void)cycles;
     S.brick_off = brick_off;
     S.brick_hist = brick_hist;
@@ -1064,12 +1103,11 @@ vox_status erode_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, uint32_t s
     VOX_TRY(buf(ctx, "e_changes", (size_t)nbuf * nj, &changes));
     VOX_TRY(buf(ctx, "e_totals", nbuf, &totals));
     VOX_TRY(buf(ctx, "e_stamp", nblocks, &stamp));
-    VOX_TRY(fill(ctx, stamp, nblocks, 0u));
     if (ctx->fixed) {
         VOX_TRY(buf(ctx, "e_notconv", 1, &notconv));
-        VOX_TRY(fill(ctx, notconv, 1, 0u));
         ctx->last_changed = notconv;
     }
+    bool cleared = false;  // stamp and notconv go with the first group's flags
     std::vector<Span> ys{{0u, UINT32_MAX}}, zs{{0u, UINT32_MAX}};
     if (nj == 1) {
         ys = split_axis_max(P.jobs[0].gd.Y, splitY);
@@ -1084,8 +1122,11 @@ vox_status erode_stage(vox_ctx *ctx, const Plan &P, const PlanDev &D, uint32_t s
             bool first = true;
             uint32_t sub = 0;  // passes this subspace needed
             for (int guard = 0; guard < 1000000; guard++) {
-                VOX_TRY(fill(ctx, changes, (size_t)nbuf * nj, 0u));
-                VOX_TRY(fill(ctx, totals, nbuf, 0u));
+                VOX_TRY(fill_many(ctx, {{changes, (size_t)nbuf * nj, 0u},
+                                        {totals, nbuf, 0u},
+                                        {cleared ? nullptr : stamp, nblocks, 0u},
+                                        {cleared ? nullptr : notconv, 1, 0u}}));
+                cleared = true;
                 for (uint32_t k = 0; k < nbuf; k++) {
                     k_erode<<<grid, 256, 0, ctx->stream>>>(
                         D.jobs, D.erode_b, nj, ysp.min, ysp.max, zsp.min, zsp.max, stamp,
