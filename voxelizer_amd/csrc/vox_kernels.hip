@@ -606,18 +606,21 @@ __global__ __launch_bounds__(256) void k_zero_pad_rows(const Job *__restrict__ j
 // Surface stage
 // ---------------------------------------------------------------------------
 
-// Voxel range of a triangle clipped to the whole ext space (the union of all
-// matSplitRes subspaces); false if it lies outside (adjustVoxelRange,
+// Voxel range vb of a triangle clipped to the whole ext space (the union of
+// all matSplitRes subspaces); false if it lies outside (adjustVoxelRange,
 // src/voxelizer.cu:3963-3993).
-__device__ __forceinline__ bool surf_range(const SurfParams &p, const F3 tri[3], VoxBB &vb) {
-    F3 mn, mx;
-    tri_bounds(tri, mn, mx);
-    vb = voxel_bounds_half(mn, mx, p.m, p.d);
+__device__ __forceinline__ bool surf_clip(const SurfParams &p, VoxBB &vb) {
     if (0 > vb.max.x || 0 > vb.max.y || 0 > vb.max.z) return false;
     if (vb.min.x > p.res[0] - 1 || vb.min.y > p.res[1] - 1 || vb.min.z > p.res[2] - 1) return false;
     vb.min = {max(vb.min.x, 0), max(vb.min.y, 0), max(vb.min.z, 0)};
     vb.max = {min(vb.max.x, p.res[0] - 1), min(vb.max.y, p.res[1] - 1), min(vb.max.z, p.res[2] - 1)};
     return true;
+}
+__device__ __forceinline__ bool surf_range(const SurfParams &p, const F3 tri[3], VoxBB &vb) {
+    F3 mn, mx;
+    tri_bounds(tri, mn, mx);
+    vb = voxel_bounds_half(mn, mx, p.m, p.d);
+    return surf_clip(p, vb);
 }
 
 // Axis helpers: for dominant axis a, u and v are the other two axes in
@@ -665,13 +668,16 @@ __global__ __launch_bounds__(256) void k_surf_setup(const Job *__restrict__ jobs
         const float4 r0 = tri_rec[3 * (size_t)t], r1 = tri_rec[3 * (size_t)t + 1],
                      r2 = tri_rec[3 * (size_t)t + 2];
         const F3 tri[3] = {{r0.x, r0.y, r0.z}, {r0.w, r1.x, r1.y}, {r1.z, r1.w, r2.x}};
+        // One voxel range for the class and for the brick box.
+        F3 mn, mx;
+        tri_bounds(tri, mn, mx);
+        VoxBB vb = voxel_bounds_half(mn, mx, p.m, p.d);
         int bb, axis;
-        classify(tri, p.m, p.d, bb, axis);
+        classify_vb(tri, vb, bb, axis);
         uint32_t c = 0;
-        VoxBB vb;
         uint32_t lo[3] = {0, 0, 0}, w[3] = {0, 0, 0};
         uint4 ext = make_uint4(0u, 0u, 0u, 0u);
-        if (bb != BB_DEGEN && surf_range(p, tri, vb)) {
+        if (bb != BB_DEGEN && surf_clip(p, vb)) {
             uint32_t hi[3];
             brick_range(p, vb, lo, hi);
             for (int a = 0; a < 3; a++) w[a] = hi[a] - lo[a] + 1u;
@@ -1319,6 +1325,10 @@ __global__ __launch_bounds__(256) void k_nonzero_list(const uint32_t *__restrict
 // depends on words at most W positions away in its plane and on the same
 // word in the planes next to it, so the block is visited when any chunk
 // within W words, or this chunk in the plane groups next to it, changed.
+// DPP controls: every lane takes the value of the lane one below (wave_shr:1)
+// or one above (wave_shl:1); the wave's end lanes get 0.
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
+
 __device__ __forceinline__ bool erode_visit(const Job &J, uint32_t GL, uint32_t y0, uint32_t y1,
                                             uint32_t z0, uint32_t z1,
                                             const uint32_t *__restrict__ stamp, uint32_t gpass,
@@ -1399,10 +1409,11 @@ __device__ __forceinline__ void erode_block(const Job &J, uint32_t j, uint32_t G
     for (uint32_t k = 0; k < kErodeZ; k++) {
         const uint32_t z = zb + k;
         const uint32_t G = col[k + 1];
-        // x neighbours: adjacent lanes hold the adjacent words of the plane;
-        // across a row's ends there is nothing.
-        uint32_t gm = (uint32_t)__shfl_up((int)G, 1);
-        uint32_t gp = (uint32_t)__shfl_down((int)G, 1);
+        // x neighbours: adjacent lanes hold the adjacent words of the plane
+        // (whole-wave DPP shifts, no LDS round trip); across a row's ends
+        // there is nothing.
+        uint32_t gm = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)G, kDppWaveShr1, 0xF, 0xF, false);
+        uint32_t gp = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)G, kDppWaveShl1, 0xF, 0xF, false);
         if (lane == 0) gm = em[k];
         if (lane == kWave - 1) gp = ep[k];
         if (w == 0) gm = 0u;

@@ -229,17 +229,16 @@ __device__ __forceinline__ float vox_corner(int32_t c, float m, float d) {
     return m + (float(c) - 0.5f) * d;
 }
 
-// classifyTriangles + analyzeBoundingBox: class (0..3) and dominant axis.
-__device__ __forceinline__ void classify(const F3 t[3], F3 m, float d, int &bb, int &axis) {
+// classifyTriangles + analyzeBoundingBox: class (0..3) and dominant axis,
+// from the triangle's unclipped voxel range vb (voxel_bounds_half of its
+// bounds).
+__device__ __forceinline__ void classify_vb(const F3 t[3], const VoxBB &vb, int &bb, int &axis) {
     F3 n = fcross(fsub(t[0], t[2]), fsub(t[1], t[0]));
     if (!(n.x != 0.0f || n.y != 0.0f || n.z != 0.0f)) {
         bb = BB_DEGEN;
         axis = AX_X;
         return;
     }
-    F3 mn, mx;
-    tri_bounds(t, mn, mx);
-    VoxBB vb = voxel_bounds_half(mn, mx, m, d);
     int32_t dx = vb.max.x - vb.min.x, dy = vb.max.y - vb.min.y, dz = vb.max.z - vb.min.z;
     if (dx == 0) {
         if (dy == 0) {
