@@ -23,7 +23,7 @@ trained first by the library's own GPU kernels on per-brick durations
 recorded on a differently seeded mesh).  A batch runs
 fused under both (every stage one launch over all meshes, bricks of all
 meshes in one list), so learned / plain is the model's own share; the batch
-also reports the meshes one after another and 16 worker streams.
+also reports the meshes one after another.
 Executions: "graph" replays a HIP graph of the pipeline captured once
 (vox_graph_*; the step still plans the grid first), "eager" launches it call
 by call.  Every (schedule, execution) is timed; `value` is the one named by
@@ -75,7 +75,6 @@ WORKLOADS = {
               "256 meshes (icospheres, rooms, boxes), 500..200k triangles, 64..512 voxels "
               "on the longest side, ShortNode (ids <= 7) or LongNode"),
 }
-BATCH_STREAMS = 16
 
 
 def build_mesh(name, seed=None):
@@ -417,8 +416,7 @@ def bench_batch(args, va, wl):
              "fused_plain": (va.SCHEDULE_PLAIN, 0, False),
              "learned": (va.SCHEDULE_LEARNED, 0, False),
              "fused_plain_graph": (va.SCHEDULE_PLAIN, 0, True),
-             "learned_graph": (va.SCHEDULE_LEARNED, 0, True),
-             f"streams{BATCH_STREAMS}_learned": (va.SCHEDULE_LEARNED, BATCH_STREAMS, False)}
+             "learned_graph": (va.SCHEDULE_LEARNED, 0, True)}
     graph_x = args.execution == "graph"
     reported = ("learned_graph" if graph_x else "learned") if args.schedule == "learned" \
         else ("fused_plain_graph" if graph_x else "fused_plain")
