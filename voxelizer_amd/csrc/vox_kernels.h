@@ -216,6 +216,16 @@ __device__ __forceinline__ SurfParams surf_params(const Job &J) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Where lanes read LDS that other lanes of their wave wrote: the hardware
+// keeps a wave's LDS operations in order, but the compiler may move a lane's
+// access past another to an address that lane itself never shares.  The
+// wavefront-scope fences around the barrier pin the order.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Wave-aggregated atomic bump: one global atomic per wave.
 __device__ __forceinline__ uint32_t wave_bump(uint32_t *counter, bool pred) {
     unsigned long long mask = __ballot(pred);
@@ -530,8 +540,6 @@ __global__ void k_brick_features(const uint32_t *__restrict__ order,
                                  float *__restrict__ feats);
 __global__ void k_mlp_forward(const float *__restrict__ P, const float *__restrict__ X, uint32_t n,
                               float *__restrict__ Y);
-__global__ void k_mlp_forward_n(const float *__restrict__ P, const float *__restrict__ X,
-                                const uint32_t *__restrict__ n_dev, float *__restrict__ Y);
 __global__ void k_mlp_grad(const float *__restrict__ P, const float *__restrict__ X,
                            const float *__restrict__ T, uint32_t n, float inv_scale,
                            float *__restrict__ partial, float *__restrict__ ploss);
@@ -546,10 +554,6 @@ __global__ void k_cost_screen(const uint32_t *__restrict__ order,
                               const uint32_t *__restrict__ total_pairs, float ratio,
                               uint32_t *__restrict__ cand, uint32_t *__restrict__ cnt,
                               uint8_t *__restrict__ skip);
-__global__ void k_cost_heavy(const float *__restrict__ Y, const uint32_t *__restrict__ cand,
-                             uint32_t *__restrict__ cnt, float heavy,
-                             const uint32_t *__restrict__ order, uint32_t *__restrict__ hlist,
-                             uint8_t *__restrict__ skip);
 // Features, MLP forward and heavy test of the screened candidates in one
 // launch (one wave per brick); the learned schedule's path.
 __global__ void k_cost_predict(const float *__restrict__ P, const uint32_t *__restrict__ order,

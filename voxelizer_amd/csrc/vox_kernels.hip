@@ -1560,7 +1560,9 @@ __device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
                                            uint32_t njobs, const uint32_t *__restrict__ pre,
                                            const uint8_t *__restrict__ ystart,
                                            const uint8_t *__restrict__ zstart) {
-    __shared__ uint4 slot[256][2];  // id bytes of the thread's word while boundary ids go in
+    // Per wave 4 KB: the id bytes of each thread's word while boundary ids go
+    // in (32 B per thread), then the wave's node run on its way out.
+    __shared__ uint4 stage[256 * 4];
     const uint32_t GL = xcd_logical_block(node_bases[njobs]);  // bases[njobs] = all blocks
     if (GL == UINT32_MAX) return;  // uniform over the block
     const Job &J = jobs[job_of(node_bases, njobs, GL)];
@@ -1665,9 +1667,15 @@ __device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
         }
     }
     const uint32_t nbw = kind == 0 ? 32u : 64u;  // node bytes per word
-    const uint32_t oo = valid ? i * nbw : kOob;
     VK_G uint8_t *nodes = (VK_G uint8_t *)gptr(J.nodes);
-    uint8_t *sb = (uint8_t *)slot[tid];
+    const uint32_t lane = tx;
+    uint4 *wst = &stage[(tid - lane) * 4u];  // this wave's staging run
+    uint8_t *sb = (uint8_t *)&wst[2u * lane];
+    // The wave's 64 words are consecutive, so its nodes of a plane are one
+    // run (2 KB ShortNode, 4 KB LongNode) starting at byte wrun: staged in
+    // LDS in node order and stored lane by lane, each store writes 1 KB of
+    // whole lines instead of every other 16 bytes.
+    const uint32_t wrun = (i0 + (tid - lane)) * nbw;
 #pragma unroll
     for (uint32_t k = 0; k < kNodeZ; k++) {
         const uint32_t z = zb + k;
@@ -1706,8 +1714,8 @@ __device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
                 // the rotated +x/-x.
                 const IdPlanes op = rot ? orient_planes(Yn, Yp, R, L, Zn, Zp)
                                         : orient_planes(L, R, Yn, Yp, Zn, Zp);
-                slot[tid][0] = make_uint4(idw[0], idw[1], idw[2], idw[3]);
-                slot[tid][1] = make_uint4(idw[4], idw[5], idw[6], idw[7]);
+                wst[2u * lane] = make_uint4(idw[0], idw[1], idw[2], idw[3]);
+                wst[2u * lane + 1u] = make_uint4(idw[4], idw[5], idw[6], idw[7]);
                 uint32_t m = bnd;
                 while (m) {
                     const uint32_t s = 31u - (uint32_t)__clz(m);  // voxel 31 - s
@@ -1718,7 +1726,7 @@ __device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
                     sb[31u - s] = (uint8_t)id;
                     m ^= 1u << s;
                 }
-                const uint4 q0 = slot[tid][0], q1 = slot[tid][1];
+                const uint4 q0 = wst[2u * lane], q1 = wst[2u * lane + 1u];
                 idw[0] = q0.x; idw[1] = q0.y; idw[2] = q0.z; idw[3] = q0.w;
                 idw[4] = q1.x; idw[5] = q1.y; idw[6] = q1.z; idw[7] = q1.w;
             }
@@ -1738,8 +1746,14 @@ __device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
             for (int j = 0; j < 8; j++) out[j] = (idw[j] | ((md[j] << 5) & 0xE0E0E0E0u)) & all;
             if (x0) out[0] &= 0xFFFFFF00u;
             if (x1) out[7] &= 0x00FFFFFFu;
-            st128(nr, oo, u32x4{out[0], out[1], out[2], out[3]});
-            st128(nr, oo + 16u, u32x4{out[4], out[5], out[6], out[7]});
+            wst[2u * lane] = make_uint4(out[0], out[1], out[2], out[3]);
+            wst[2u * lane + 1u] = make_uint4(out[4], out[5], out[6], out[7]);
+            wave_lds_sync();
+#pragma unroll
+            for (uint32_t q = 0; q < 2u; q++) {
+                const uint4 v = wst[q * kWave + lane];
+                st128(nr, wrun + (q * kWave + lane) * 16u, u32x4{v.x, v.y, v.z, v.w});
+            }
         } else {  // LongNode halfword: id | material << 8
             uint32_t out[16];
 #pragma unroll
@@ -1750,10 +1764,16 @@ __device__ __forceinline__ void nodes_cart(const Job *__restrict__ jobs,
             if (x0) out[0] &= 0xFFFF0000u;
             if (x1) out[15] &= 0x0000FFFFu;
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                st128(nr, oo + 16u * q,
-                      u32x4{out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]});
+            for (uint32_t q = 0; q < 4u; q++)
+                wst[4u * lane + q] = make_uint4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+            wave_lds_sync();
+#pragma unroll
+            for (uint32_t q = 0; q < 4u; q++) {
+                const uint4 v = wst[q * kWave + lane];
+                st128(nr, wrun + (q * kWave + lane) * 16u, u32x4{v.x, v.y, v.z, v.w});
+            }
         }
+        wave_lds_sync();  // the run is read out before the next plane's writes
     }
 }
 __global__ __launch_bounds__(256) void k_nodes_cart(const Job *__restrict__ jobs,
