@@ -281,27 +281,6 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ P
         }
 }
 
-// The same with the row count read from device memory (rows of a brick list
-// whose length only the device knows).
-__global__ __launch_bounds__(256) void k_mlp_forward_n(const float *__restrict__ P,
-                                                       const float *__restrict__ X,
-                                                       const uint32_t *__restrict__ n_dev,
-                                                       float *__restrict__ Y) {
-    __shared__ MlpShared S;
-    const uint32_t n = *n_dev;
-    const uint32_t row0 = blockIdx.x * kMlpRows;
-    if (row0 >= n) return;  // uniform over the block
-    mlp_load(S, P, X, row0, n);
-    float y[kMlpRows / 4];
-    mlp_hidden(S, y);
-    const uint32_t w = threadIdx.x / kWave;
-    if (lane_id() == 0)
-        for (int k = 0; k < kMlpRows / 4; k++) {
-            const uint32_t r = row0 + w + 4 * k;
-            if (r < n) Y[r] = y[k];
-        }
-}
-
 // Gradient of the mean squared error over rows [0, n) (scaled by 1/n_scale)
 // for the block's rows: the block writes its partial sums to
 // partial[block][kCostParams] and its squared-error sum to ploss[block].
@@ -430,24 +409,6 @@ __global__ __launch_bounds__(256) void k_cost_screen(const uint32_t *__restrict_
         skip[q] = 0u;
         const uint32_t slot = wave_bump(&cnt[0], c);  // one atomic per wave
         if (c) cand[slot] = q;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_cost_heavy(const float *__restrict__ Y,
-                                                    const uint32_t *__restrict__ cand,
-                                                    uint32_t *__restrict__ cnt, float heavy,
-                                                    const uint32_t *__restrict__ order,
-                                                    uint32_t *__restrict__ hlist,
-                                                    uint8_t *__restrict__ skip) {
-    const uint32_t n = cnt[0];
-    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-        const bool h = Y[r] > heavy;  // false for NaN
-        const uint32_t slot = wave_bump(&cnt[1], h);
-        if (h) {
-            const uint32_t q = cand[r];
-            hlist[slot] = order[q];
-            skip[q] = 1u;
-        }
     }
 }
 
