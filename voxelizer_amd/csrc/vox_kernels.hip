@@ -1892,9 +1892,7 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(const Job *__restrict__ jobs,
                                                    const uint32_t *__restrict__ fcc_first,
                                                    const uint32_t *__restrict__ fcc_bases,
                                                    uint32_t nfcc) {
-    // Per wave 8 KB: 32 node masks (u16) per lane while boundary ids go in,
-    // then the wave's node run on its way out.
-    __shared__ uint4 fstage[256 * 8];
+    __shared__ uint4 fslot[256][4];  // 32 node masks (u16) per thread while boundary ids go in
     const uint32_t GL = xcd_logical_block(fcc_bases[nfcc]);  // bases[nfcc] = all blocks
     if (GL == UINT32_MAX) return;
     const uint32_t f = job_of(fcc_bases, nfcc, GL);
@@ -1925,8 +1923,7 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(const Job *__restrict__ jobs,
     lr_shift(G, g, M0, fw, y, fz, LM0, RM0);
     lr_shift(G, g, Mp, fw, y + 1, fz, LMp, RMp);
     lr_shift(G, g, U0, fw, y, fz + 1, LU0, RU0);
-    // Lanes past the row end stay (with zero words, so no boundary) for the
-    // wave's store of its run; their nodes are not stored.
+    if (!valid) return;
     const uint32_t E = (fz & 1) ? 0x55555555u : 0xAAAAAAAAu;  // (x+z) even
     const uint32_t NE = ~E;
     uint32_t n[12];
@@ -1956,10 +1953,8 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(const Job *__restrict__ jobs,
         const uint32_t two = (ir >> (2 * d)) & 3u;
         mk[d] = ((two & 1u) ? 0x0FFFu : 0u) | ((two & 2u) ? 0x0FFF0000u : 0u);
     }
-    const uint32_t lane = threadIdx.x;
-    uint4 *wst = &fstage[threadIdx.y * kWave * 8u];  // this wave's staging run
     if (boundary) {
-        uint4 *my = &wst[4u * lane];
+        uint4 *my = fslot[threadIdx.y * kWave + threadIdx.x];
 #pragma unroll
         for (int q = 0; q < 4; q++)
             my[q] = make_uint4(mk[4 * q], mk[4 * q + 1], mk[4 * q + 2], mk[4 * q + 3]);
@@ -1979,14 +1974,13 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(const Job *__restrict__ jobs,
             mk[4 * q] = v.x; mk[4 * q + 1] = v.y; mk[4 * q + 2] = v.z; mk[4 * q + 3] = v.w;
         }
     }
-    wave_lds_sync();  // boundary slots are read back before the run overwrites them
     // Materials: node k is voxel k / 2 of the sub-grid with x parity k & 1,
     // whose surface word's half for this FCC word holds 16 voxels.  A word
     // with a surface voxel has all 32 material bytes written in the call (0
     // for its other voxels), so each half is read as 16 bytes, or taken as 0
     // when it has no surface voxel.
     uint32_t ma[2][4] = {{0u, 0u, 0u, 0u}, {0u, 0u, 0u, 0u}};
-    if (use_mats && valid) {
+    if (use_mats) {
         const int odd = (int)(fz & 1);
         const uint64_t gw = (uint64_t)(fw >> 1), gz = (uint64_t)(fz >> 1);
         const uint32_t b = (uint32_t)((((gz >> bd.lbz) * bd.nby + ((uint64_t)y >> bd.lby)) *
@@ -2005,15 +1999,7 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(const Job *__restrict__ jobs,
             ma[par][0] = v.x; ma[par][1] = v.y; ma[par][2] = v.z; ma[par][3] = v.w;
         }
     }
-    // The wave's 64 lanes hold consecutive FCC words of one row, so their
-    // nodes are one run (4 KB ShortFCCNode, 8 KB LongFCCNode): staged in
-    // LDS in node order and stored lane by lane, each store writes 1 KB of
-    // whole lines.  Slot s of the run belongs to lane s / (slots per lane).
-    const uint32_t nq = kind == 3 ? 4u : 8u;  // 16-byte slots per lane
-    const int64_t fw0 = fw - (int64_t)lane;
-    VK_G u32x4 *run = (VK_G u32x4 *)((VK_G uint8_t *)gptr(nodes) +
-                                     (((size_t)fz * FY + (size_t)y) * FW + (size_t)fw0) * 32u *
-                                         (kind == 3 ? 2u : 4u));
+    const size_t node0 = (((size_t)fz * FY + (size_t)y) * FW + (size_t)fw) * 32u;
     if (kind == 3) {  // ShortFCCNode: mat<<12 | bid&4095
         uint32_t out[16];
 #pragma unroll
@@ -2022,31 +2008,24 @@ __global__ __launch_bounds__(256) void k_nodes_fcc(const Job *__restrict__ jobs,
             const uint32_t c = (ma[1][d >> 2] >> (8 * (d & 3))) & 0xFu;
             out[d] = mk[d] | (a << 12) | (c << 28);
         }
+        VK_G u32x4 *dst = (VK_G u32x4 *)((VK_G uint8_t *)gptr(nodes) + node0 * 2u);
 #pragma unroll
-        for (uint32_t q = 0; q < 4u; q++)
-            wst[4u * lane + q] = make_uint4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+        for (int q = 0; q < 4; q++)
+            dst[q] = u32x4{out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]};
     } else {  // LongFCCNode: {u16 bid, u8 mat, u8 0}
+        VK_G u32x4 *dst = (VK_G u32x4 *)((VK_G uint8_t *)gptr(nodes) + node0 * 4u);
 #pragma unroll
-        for (uint32_t q = 0; q < 8u; q++) {
+        for (int q = 0; q < 8; q++) {
             uint32_t o[4];
 #pragma unroll
             for (int e = 0; e < 2; e++) {
-                const int d = 2 * (int)q + e;
+                const int d = 2 * q + e;
                 const uint32_t a = (ma[0][d >> 2] >> (8 * (d & 3))) & 0xFFu;
                 const uint32_t c = (ma[1][d >> 2] >> (8 * (d & 3))) & 0xFFu;
                 o[2 * e] = (mk[d] & 0xFFFFu) | (a << 16);
                 o[2 * e + 1] = (mk[d] >> 16) | (c << 16);
             }
-            wst[8u * lane + q] = make_uint4(o[0], o[1], o[2], o[3]);
-        }
-    }
-    wave_lds_sync();
-    if (y >= (int64_t)FY) return;  // uniform over the wave
-    for (uint32_t q = 0; q < nq; q++) {
-        const uint32_t sl = q * kWave + lane;
-        if (fw0 + (int64_t)(sl / nq) < (int64_t)FW) {
-            const uint4 v = wst[sl];
-            run[sl] = u32x4{v.x, v.y, v.z, v.w};
+            dst[q] = u32x4{o[0], o[1], o[2], o[3]};
         }
     }
 }
