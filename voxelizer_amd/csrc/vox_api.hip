@@ -1572,8 +1572,7 @@ vox_status vox_plan_grid(vox_ctx *ctx, vox_plan_mode mode, double param, uint32_
     VOX_HIP(hipSetDevice(ctx->dev));
     uint32_t *bb;
     VOX_TRY(buf(ctx, "p_bbox", 6, &bb));
-    VOX_TRY(fill(ctx, bb, 3, UINT32_MAX));
-    VOX_TRY(fill(ctx, bb + 3, 3, 0u));
+    VOX_TRY(fill_many(ctx, {{bb, 3, UINT32_MAX}, {bb + 3, 3, 0u}}));
     k_bbox<<<grid_for(ctx->nv, 256, 256), 256, 0, ctx->stream>>>(ctx->V, ctx->nv, bb);
     VOX_HIP(hipGetLastError());
     uint32_t h[6];
@@ -1758,8 +1757,7 @@ __global__ void k_unrotate_nodes(const uint8_t *__restrict__ in, uint8_t *__rest
 vox_status device_bbox(vox_ctx *ctx, const float *V, uint32_t nv, double mn[3], double mx[3]) {
     uint32_t *bb;
     VOX_TRY(buf(ctx, "p_bbox", 6, &bb));
-    VOX_TRY(fill(ctx, bb, 3, UINT32_MAX));
-    VOX_TRY(fill(ctx, bb + 3, 3, 0u));
+    VOX_TRY(fill_many(ctx, {{bb, 3, UINT32_MAX}, {bb + 3, 3, 0u}}));
     k_bbox<<<grid_for(nv, 256, 256), 256, 0, ctx->stream>>>(V, nv, bb);
     VOX_HIP(hipGetLastError());
     uint32_t h[6];

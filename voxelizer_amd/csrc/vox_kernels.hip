@@ -567,8 +567,10 @@ __global__ __launch_bounds__(kWave) void k_solid_fill(const Job *__restrict__ jo
                 sx ^= sx >> 8;
                 sx ^= sx >> 16;
                 const unsigned long long odd = __ballot((__popc(c) & 1) != 0);
-                const unsigned long long below = lane ? (odd & (~0ull >> (64u - lane))) : 0ull;
-                const uint32_t before = carry ^ ((uint32_t)__popcll(below) & 1u);
+                // Odd-crossing words below this lane, counted by mbcnt.
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(odd >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)odd, 0u));
+                const uint32_t before = carry ^ (below & 1u);
                 if (w < sw) bits[rowIdx * W + s0 + w] = sx ^ (before ? 0xFFFFFFFFu : 0u);
                 carry ^= (uint32_t)__popcll(odd) & 1u;
             }
