@@ -498,7 +498,7 @@ __global__ __launch_bounds__(kWave) void k_solid_fill(const Job *__restrict__ jo
                                                       const uint32_t *__restrict__ tile_hist,
                                                       const uint32_t *__restrict__ list,
                                                       const uint16_t *__restrict__ list_mask) {
-    extern __shared__ uint32_t cross[];  // [16][SW]
+    extern __shared__ __align__(16) uint32_t cross[];  // [16][SW]
     const Job &J = jobs[job_of(tile_bases, njobs, blockIdx.x)];
     const SolidParams p = solid_params(J, rec);
     VK_G uint32_t *__restrict__ bits = gptr(J.bits);
@@ -525,7 +525,8 @@ __global__ __launch_bounds__(kWave) void k_solid_fill(const Job *__restrict__ jo
     uint32_t carries = 0u;  // bit r: crossing parity of row r before the segment
     for (uint32_t s0 = 0; s0 < W; s0 += SW) {
         const uint32_t sw = min(SW, W - s0);
-        for (uint32_t i = tid; i < 16u * SW; i += kWave) cross[i] = 0u;
+        uint4 *cross4 = reinterpret_cast<uint4 *>(cross);  // 16 * SW words: 4 * SW uint4
+        for (uint32_t i = tid; i < 4u * SW; i += kWave) cross4[i] = make_uint4(0u, 0u, 0u, 0u);
         __syncthreads();
         for (uint32_t i = tid; i < items; i += kWave) {
             const uint32_t t = list[off + (i >> lg)];
