@@ -1018,6 +1018,9 @@ __device__ __forceinline__ void surf_item(const SurfParams &p, const SplitTable 
 // are ORed into the solid grid.  With `stats`, surface voxels are counted;
 // with `cycles`, each brick's duration (s_memtime) is recorded.  The kKeys
 // variant also stores the winner keys of jobs with a keygrid (PartialNode).
+// DPP controls of the epilogue's 8-lane OR (all lanes active).
+constexpr int kDppQuadSwap1 = 0xB1, kDppQuadSwap2 = 0x4E, kDppHalfMirror = 0x141;
+
 template <bool kKeys>
 __global__ __launch_bounds__(256, 5) void k_surf_brick(
     const Job *__restrict__ jobs, const uint32_t *__restrict__ brick_bases, uint32_t njobs,
@@ -1224,10 +1227,13 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
                     const uint4 kv = *kp;
                     const uint32_t nib = (kv.x ? 8u : 0u) | (kv.y ? 4u : 0u) | (kv.z ? 2u : 0u) |
                                          (kv.w ? 1u : 0u);
+                    // OR over the word's 8 lanes in registers (DPP): the
+                    // quad swaps [1,0,3,2] and [2,3,0,1], then the half-row
+                    // mirror (lane i <-> 7 - i) joins the two quads.
                     uint32_t w = nib << (28u - 4u * (lane & 7u));
-                    w |= (uint32_t)__shfl_xor((int)w, 1);
-                    w |= (uint32_t)__shfl_xor((int)w, 2);
-                    w |= (uint32_t)__shfl_xor((int)w, 4);
+                    w |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)w, kDppQuadSwap1, 0xF, 0xF, false);
+                    w |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)w, kDppQuadSwap2, 0xF, 0xF, false);
+                    w |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)w, kDppHalfMirror, 0xF, 0xF, false);
                     if ((lane & 7u) == 0u) wsurf[(v0 >> 5) + (int32_t)(lane >> 3)] = w;
                     if (!w) continue;
                     nsurf += (uint32_t)__popc(nib);
