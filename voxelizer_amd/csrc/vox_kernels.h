@@ -226,6 +226,22 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Inclusive sum over the wave's lanes in registers (DPP; every lane active):
+// shifts by 1, 2, 4 and 8 within each 16-lane row, then each row's total
+// broadcast into the rows above it.  A lane whose source lies outside its
+// row, or whose row is masked off, adds 0.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118,
+                  kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr1, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr2, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr4, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr8, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowBcast15, 0xA, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowBcast31, 0xC, 0xF, false);
+    return v;
+}
+
 // Wave-aggregated atomic bump: one global atomic per wave.
 __device__ __forceinline__ uint32_t wave_bump(uint32_t *counter, bool pred) {
     unsigned long long mask = __ballot(pred);

@@ -79,16 +79,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(const uint32_t *__
 }
 
 // Block-wide sum and exclusive prefix of one value per thread: a wave scan
-// by shuffles, the four wave totals through LDS (one barrier).
+// in registers, the four wave totals through LDS (one barrier).
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t s, uint32_t &total) {
     __shared__ uint32_t wsum[kScanThreads / kWave];
     const uint32_t lane = threadIdx.x & (kWave - 1u), wave = threadIdx.x / kWave;
-    uint32_t incl = s;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const uint32_t o = (uint32_t)__shfl_up((int)incl, d);
-        if ((int)lane >= d) incl += o;
-    }
+    const uint32_t incl = wave_incl_scan(s);
     if (lane == kWave - 1u) wsum[wave] = incl;
     __syncthreads();
     uint32_t before = 0u;
@@ -1020,9 +1015,6 @@ __device__ __forceinline__ void surf_item(const SurfParams &p, const SplitTable 
 // variant also stores the winner keys of jobs with a keygrid (PartialNode).
 // DPP controls of the epilogue's 8-lane OR (all lanes active).
 constexpr int kDppQuadSwap1 = 0xB1, kDppQuadSwap2 = 0x4E, kDppHalfMirror = 0x141;
-// DPP controls of the setup wave's scan.
-constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118,
-              kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
 
 template <bool kKeys>
 __global__ __launch_bounds__(256, 5) void k_surf_brick(
@@ -1151,19 +1143,8 @@ __global__ __launch_bounds__(256, 5) void k_surf_brick(
                         items = surf_tri_setup(p, tri_rec, t, J.tri_base, embed, blx, bly, blz,
                                                z0, z1, tris[tid]);
                     }
-                    // Wave-level inclusive scan in registers (DPP): shifts
-                    // by 1, 2, 4 and 8 within each 16-lane row, then each
-                    // row's total broadcast into the rows above it.  A lane
-                    // whose source lies outside its row, or whose row is
-                    // masked off, adds 0.
                     static_assert(kSurfChunk == kWave, "one wave sets up a chunk");
-                    uint32_t incl = items;
-                    incl += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)incl, kDppRowShr1, 0xF, 0xF, false);
-                    incl += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)incl, kDppRowShr2, 0xF, 0xF, false);
-                    incl += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)incl, kDppRowShr4, 0xF, 0xF, false);
-                    incl += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)incl, kDppRowShr8, 0xF, 0xF, false);
-                    incl += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)incl, kDppRowBcast15, 0xA, 0xF, false);
-                    incl += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)incl, kDppRowBcast31, 0xC, 0xF, false);
+                    const uint32_t incl = wave_incl_scan(items);
                     // The triangles with items, in order: first item and LDS slot.
                     const unsigned long long nz = __ballot(items != 0u);
                     if (items) {
