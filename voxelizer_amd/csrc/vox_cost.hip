@@ -28,13 +28,31 @@ namespace vk {
 // A triangle's items in the brick are estimated as its items spread evenly
 // over the bricks its box spans across its dominant plane.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float wave_sum(float v) {
-    for (int d = kWave / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d);
-    return v;
+// Wave reductions in registers (DPP; every lane active): within each 16-lane
+// row by the quad swaps and the half-row and row mirrors, then the rows into
+// lane 63 by the row broadcasts (rows 1 and 3 take rows 0 and 2, rows 2 and 3
+// take lane 31), read back as a uniform value.  Masked-off rows keep `old`.
+template <int kCtrl, int kRows>
+__device__ __forceinline__ float dpp_f32(float old, float x) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(x), kCtrl, kRows, 0xF, false));
 }
+template <class Op>
+__device__ __forceinline__ float wave_reduce(float v, float identity, Op op) {
+    v = op(v, dpp_f32<0xB1, 0xF>(identity, v));   // quad_perm [1,0,3,2]
+    v = op(v, dpp_f32<0x4E, 0xF>(identity, v));   // quad_perm [2,3,0,1]
+    v = op(v, dpp_f32<0x141, 0xF>(identity, v));  // row_half_mirror
+    v = op(v, dpp_f32<0x140, 0xF>(identity, v));  // row_mirror
+    v = op(v, dpp_f32<0x142, 0xA>(identity, v));  // row_bcast:15
+    v = op(v, dpp_f32<0x143, 0xC>(identity, v));  // row_bcast:31
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), kWave - 1));
+}
+__device__ __forceinline__ float wave_sum(float v) {
+    return wave_reduce(v, 0.0f, [](float a, float b) { return a + b; });
+}
+// Maxima of non-negative values: the identity 0 never decides.
 __device__ __forceinline__ float wave_max(float v) {
-    for (int d = kWave / 2; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d));
-    return v;
+    return wave_reduce(v, 0.0f, [](float a, float b) { return fmaxf(a, b); });
 }
 
 // Per-lane sums of the features (feature order: see above).
